@@ -1,0 +1,90 @@
+"""ctypes mirror of include/pdk.h (types and enums only)."""
+import ctypes as C
+
+PDK_MODEL_PMB = 0
+PDK_MODEL_LPS = 1
+PDK_MODEL_HERTZ = 2
+PDK_MODEL_HERTZ_JKR = 3
+
+PDK_FP64 = 0
+PDK_MIXED = 1
+
+FIELD = {
+    "x": 0, "u": 1, "v": 2, "f": 3, "vol": 4, "rho": 5,
+    "W": 6, "phi": 7, "m": 8, "theta": 9,
+}
+VECTOR_FIELDS = {"x", "u", "v", "f"}
+
+PDK_BC_FORCE_VALUE = 0
+PDK_BC_FORCE_UPDATE = 1
+
+# Kernel classes of pdk_kernel_time.
+KCLS_FORCE, KCLS_LPS, KCLS_ENERGY, KCLS_BUILD, KCLS_OTHER = range(5)
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("fracture", C.c_int32),
+        ("influence", C.c_int32),
+        ("precision", C.c_int32),
+        ("horizon", C.c_double),
+        ("K", C.c_double),
+        ("G", C.c_double),
+        ("G0", C.c_double),
+        ("dt", C.c_double),
+        ("neighbor_tol", C.c_double),
+        ("radius", C.c_double),
+        ("radius_extend", C.c_double),
+        ("nu", C.c_double),
+        ("E", C.c_double),
+        ("restitution", C.c_double),
+        ("gamma", C.c_double),
+        ("mu_friction", C.c_double),
+        ("rebuild_every", C.c_int32),
+        ("polydisperse", C.c_int32),
+    ]
+
+
+class ModelConstants(C.Structure):
+    _fields_ = [(name, C.c_double) for name in (
+        "c", "s0", "bond_break_coeff", "theta_coeff", "s_coeff",
+        "Rs", "Es", "Gs", "beta", "coeff_h_n", "coeff_h_d", "Gamma", "a0",
+        "delta_tear", "fc")]
+
+
+def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
+              G=0.0, G0=0.0, dt=1e-7, influence=None, neighbor_tol=1e-14,
+              radius=0.0, radius_extend=0.0, nu=0.25, E=0.0, restitution=1.0,
+              gamma=0.0, mu_friction=0.0, rebuild_every=0, polydisperse=False):
+    """Model description with the reference's constructor defaults.
+
+    influence defaults: PMB fracture s0 uses influence_type 1
+    (CabanaPD_ForceModels.hpp:93-101), LPS uses 0
+    (force_models/CabanaPD_LPS.hpp:46-47)."""
+    kinds = {"PMB": PDK_MODEL_PMB, "LPS": PDK_MODEL_LPS,
+             "HERTZ": PDK_MODEL_HERTZ, "HERTZ_JKR": PDK_MODEL_HERTZ_JKR}
+    m = kinds[model.upper()]
+    if influence is None:
+        influence = 0 if m == PDK_MODEL_LPS else 1
+    d = ModelDesc()
+    d.model = m
+    d.fracture = 1 if fracture else 0
+    d.influence = influence
+    d.precision = {"fp64": PDK_FP64, "mixed": PDK_MIXED}[precision]
+    d.horizon = horizon
+    d.K = K
+    d.G = G
+    d.G0 = G0
+    d.dt = dt
+    d.neighbor_tol = neighbor_tol
+    d.radius = radius
+    d.radius_extend = radius_extend
+    d.nu = nu
+    d.E = E
+    d.restitution = restitution
+    d.gamma = gamma
+    d.mu_friction = mu_friction
+    d.rebuild_every = rebuild_every
+    d.polydisperse = 1 if polydisperse else 0
+    return d

@@ -1,0 +1,197 @@
+"""Workload generators: particles, model and loading of the benchmark
+specimens and of the reference's regression inputs.
+
+Every generator returns a dict with keys
+  desc (ModelDesc), x (n,3), vol (n,), rho (n,), u (n,3), v (n,3),
+  nofail (n,) or None, radius (n,) or None,
+  notches: (p0, v1, v2) arrays or None,
+  bc: (kind, idx, value) or None, body (n,3) or None,
+  steps, output_every, name, dx.
+Positions are in the caller order the reference's grid creation produces:
+lexicographic with x fastest.
+"""
+import math
+
+import numpy as np
+
+from ._abi import PDK_BC_FORCE_VALUE, make_desc
+
+
+def grid_cell_centred(low, high, num_cells):
+    """Particles::create on a uniform mesh, one particle per cell at the cell
+    centre, vol = cell volume (CabanaPD_Particles.hpp:242-307)."""
+    low = np.asarray(low, dtype=np.float64)
+    high = np.asarray(high, dtype=np.float64)
+    nc = np.asarray(num_cells, dtype=np.int64)
+    d = (high - low) / nc
+    ax = [low[k] + np.arange(nc[k]) * d[k] + 0.5 * d[k] for k in range(3)]
+    zz, yy, xx = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    x = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1)
+    vol = np.full(x.shape[0], d[0] * d[1] * d[2])
+    return x, vol, d
+
+
+def grid_node_centred(n_side, spacing):
+    """Node-centred cube: x = (i, j, k) * spacing, i, j, k in [0, n_side)
+    (custom Particles::create(x, vol), CabanaPD_Particles.hpp:310-368)."""
+    a = np.arange(n_side, dtype=np.float64) * spacing
+    zz, yy, xx = np.meshgrid(a, a, a, indexing="ij")
+    x = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1)
+    vol = np.full(x.shape[0], spacing ** 3)
+    idx = np.stack(np.meshgrid(np.arange(n_side), np.arange(n_side), np.arange(n_side), indexing="ij"),
+                   axis=-1).reshape(-1, 3)[:, ::-1]
+    return x, vol, idx
+
+
+def _base(desc, x, vol, rho, name, dx, steps, output_every=0):
+    n = x.shape[0]
+    return dict(desc=desc, x=x, vol=vol, rho=np.full(n, rho) if np.isscalar(rho) else rho,
+                u=np.zeros((n, 3)), v=np.zeros((n, 3)), nofail=None, radius=None, notches=None,
+                bc=None, body=None, steps=steps, output_every=output_every, name=name, dx=dx)
+
+
+def elastic_cube(n_side=41, steps=500, precision="fp64", dt=1e-7):
+    """Plain FP64 PMB elastic cube: 41^3 node-centred particles, dx = 1/40,
+    delta = 3.01 dx, E = 72 GPa, nu = 0.25, rho = 2440, Gaussian radial
+    displacement A exp(-r^2/sigma^2) r_hat (A = 1e-4, sigma = 0.1) about the
+    centre (style of examples/mechanics/elastic_wave.cpp:94-95)."""
+    dx = 1.0 / (n_side - 1)
+    x, vol, _ = grid_node_centred(n_side, dx)
+    E, nu = 72e9, 0.25
+    K = E / (3.0 * (1.0 - 2.0 * nu))
+    desc = make_desc("PMB", fracture=False, precision=precision, horizon=3.01 * dx, K=K, dt=dt)
+    w = _base(desc, x, vol, 2440.0, "pmb_elastic_cube_%d" % n_side, dx, steps)
+    c = x - 0.5
+    r = np.sqrt((c * c).sum(axis=1))
+    amp = 1e-4 * np.exp(-r * r / 0.01)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rhat = np.where(r[:, None] > 0, c / r[:, None], 0.0)
+    w["u"] = amp[:, None] * rhat
+    return w
+
+
+def kw_plate(num_cells=(200, 400, 18), size=(0.1, 0.2, 0.009), E=191e9, nu=0.25, rho=8000.0,
+             G0=42408.0, impact_v=16.5, dt=7.5e-8, steps=1000, precision="fp64",
+             horizon=None, horizon_extra=0.0, output_every=0, name=None):
+    """Pre-notched plate under edge impact (Kalthoff-Winkler geometry of
+    examples/mechanics/kalthoff_winkler.cpp:58-127): two notches of length
+    height/2 at y = +-width/8 through the thickness, x-velocity between the
+    notches on the impact edge, zero-force strip at the impact edge."""
+    size = np.asarray(size, dtype=np.float64)
+    low, high = -0.5 * size, 0.5 * size
+    x, vol, d = grid_cell_centred(low, high, num_cells)
+    dx = d[0]
+    K = E / (3.0 * (1.0 - 2.0 * nu))
+    if horizon is None:
+        horizon = 3.01 * dx
+    horizon = horizon + horizon_extra
+    desc = make_desc("PMB", fracture=True, precision=precision, horizon=horizon, K=K, G0=G0, dt=dt)
+    w = _base(desc, x, vol, rho, name or "pmb_kw_plate_%dx%dx%d" % tuple(num_cells), dx, steps, output_every)
+    height, width, thickness = size
+    y1, y2 = -width / 8.0, width / 8.0
+    low_x, low_z = low[0], low[2]
+    p0 = np.array([[low_x, y1, low_z], [low_x, y2, low_z]])
+    v1 = np.array([[height / 2.0, 0.0, 0.0]] * 2)
+    v2 = np.array([[0.0, 0.0, thickness]] * 2)
+    w["notches"] = (p0, v1, v2)
+    sel = (x[:, 1] > y1) & (x[:, 1] < y2) & (x[:, 0] < -0.5 * height + dx)
+    w["v"][sel, 0] = impact_v
+    x_bc = -0.5 * height
+    lo = np.array([x_bc - dx, y1 - 0.25 * dx, -thickness])
+    hi = np.array([x_bc + dx, y2 + 0.25 * dx, thickness])
+    inside = np.all((x >= lo) & (x <= hi), axis=1)
+    w["bc"] = (PDK_BC_FORCE_VALUE, np.nonzero(inside)[0].astype(np.int64), 0.0)
+    return w
+
+
+def kw_regression(precision="fp64"):
+    """regression_test/inputs/kalthoff_winkler.json run through
+    examples/mechanics/kalthoff_winkler.cpp: 75x151x7 cells, horizon
+    0.002 (+1e-10), nu = 1/3, v0 = 16, dt = 1e-7, tf = 70 us, output every 50."""
+    tf, dt = 70e-6, 1e-7
+    steps = int(tf / dt)  # Inputs truncation (CabanaPD_Input.hpp:109)
+    E, nu = 191e9, 1.0 / 3.0
+    return kw_plate(num_cells=(75, 151, 7), size=(0.1, 0.2, 0.009), E=E, nu=nu, rho=8000.0, G0=42408.0,
+                    impact_v=16.0, dt=dt, steps=steps, precision=precision, horizon=0.002,
+                    horizon_extra=1e-10, output_every=50, name="kw_regression")
+
+
+def kw_crack_y(x, phi):
+    """Output time series value of kalthoff_winkler.cpp:134-145: max over
+    particles of (phi > 0.3 ? y : 0)."""
+    vals = np.where(phi > 0.3, x[:, 1], 0.0)
+    return float(vals.max()) if vals.size else 0.0
+
+
+def lps_cube(n_side=128, steps=500, precision="fp64", seed=44758454, dt=None, sigma=1e7, G0=3.8,
+             fracture=True):
+    """LPS elastic-brittle cube: n^3 node-centred particles, dx = 1/(n-1),
+    delta = 3.01 dx, E = 72 GPa, nu = 0.22, rho = 2440, influence 0, seeded
+    uniform perturbation |u_d| <= 1e-3 dx / sqrt(3), opposite-face traction
+    +-sigma/dx on the y faces (style of examples/mechanics/crack_branching.cpp:120-132)."""
+    dx = 1.0 / (n_side - 1)
+    x, vol, idx = grid_node_centred(n_side, dx)
+    E, nu = 72e9, 0.22
+    K = E / (3.0 * (1.0 - 2.0 * nu))
+    G = E / (2.0 * (1.0 + nu))
+    horizon = 3.01 * dx
+    if dt is None:
+        from .engine import critical_timestep
+        try:
+            dt = critical_timestep(dx, dx, dx, K, 2440.0, horizon, 0.85)
+        except Exception:
+            dt = 1.13e-6 * (127.0 * dx)
+    desc = make_desc("LPS", fracture=fracture, precision=precision, horizon=horizon, K=K, G=G, G0=G0, dt=dt,
+                     influence=0)
+    w = _base(desc, x, vol, 2440.0, "lps_cube_%d" % n_side, dx, steps)
+    rng = np.random.default_rng(seed)
+    a = 1e-3 * dx / math.sqrt(3.0)
+    w["u"] = rng.uniform(-a, a, size=x.shape)
+    body = np.zeros_like(x)
+    jy = idx[:, 1]
+    body[jy == 0, 1] = -sigma / dx
+    body[jy == n_side - 1, 1] = sigma / dx
+    w["body"] = body
+    return w
+
+
+def lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.1, K=1.0, G=0.5, influence=None,
+                     fracture=False, G0=1000.0):
+    """Known-answer lattice of unit_test/tstForce.hpp:593-661: cell-centred
+    particles on [-1, 1]^3 with nc cells per side, horizon = m dx, u = s0 x
+    (linear) or u_x = u11 x^2 (quadratic)."""
+    x, vol, d = grid_cell_centred([-1.0, -1.0, -1.0], [1.0, 1.0, 1.0], (nc, nc, nc))
+    dx = 2.0 / nc
+    horizon = dx * m
+    kw = dict(horizon=horizon, K=K, dt=1e-7, fracture=fracture)
+    if fracture:
+        kw["G0"] = G0
+    if model == "LPS":
+        kw["G"] = G
+    desc = make_desc(model, influence=influence, **kw)
+    w = _base(desc, x, vol, 1.0, "lattice_%s_%s" % (model, field), dx, 0)
+    u = np.zeros_like(x)
+    if field == "linear":
+        u = coeff * x
+    else:
+        u[:, 0] = coeff * x[:, 0] * x[:, 0]
+    w["u"] = u
+    return w
+
+
+def upload(spec, w):
+    """Send a workload to an engine spec (or an oracle spec)."""
+    spec.upload(w["x"], w["vol"], u=w["u"], v=w["v"], rho=w["rho"], nofail=w["nofail"], radius=w["radius"])
+
+
+def setup(spec, w, init=True):
+    """Build bonds, prenotch, BC/body term and Solver::init for a workload."""
+    spec.build_bonds()
+    if w["notches"] is not None:
+        spec.prenotch(*w["notches"])
+    if w["bc"] is not None:
+        spec.set_bc(*w["bc"])
+    if w["body"] is not None:
+        spec.set_body_force(w["body"])
+    if init:
+        spec.init()

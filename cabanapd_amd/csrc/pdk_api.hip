@@ -1,0 +1,1057 @@
+// pdk_api.hip -- the C ABI (include/pdk.h): specimen state on one GPU and the
+// host orchestration of the reference's Solver step order.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pdk_internal.hpp"
+#include "pdk_serving.hpp"
+
+using namespace pdk;
+
+namespace
+{
+thread_local std::string g_last_error;
+
+// Kokkos::numbers::pi_v<double> (CabanaPD_Constants.hpp:20)
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+template <class F>
+int guarded( F&& f )
+{
+    try
+    {
+        f();
+        return 0;
+    }
+    catch ( const std::exception& e )
+    {
+        g_last_error = e.what();
+        return 1;
+    }
+    catch ( ... )
+    {
+        g_last_error = "unknown error";
+        return 1;
+    }
+}
+
+void require( bool cond, const char* msg )
+{
+    if ( !cond )
+        throw Error( msg );
+}
+
+bool isContact( const pdk_model_desc& d )
+{
+    return d.model == PDK_MODEL_HERTZ || d.model == PDK_MODEL_HERTZ_JKR;
+}
+
+// Derived constants with the reference's expressions.
+void modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
+{
+    std::memset( &k, 0, sizeof( k ) );
+    const double delta = d.horizon;
+    const double K = d.K;
+    if ( d.model == PDK_MODEL_PMB || d.model == PDK_MODEL_LPS )
+    {
+        require( delta > 0 && K > 0, "horizon and K must be positive" );
+        // force_models/CabanaPD_PMB.hpp:44-46
+        k.c = 18.0 * K / ( kPi * delta * delta * delta * delta );
+        if ( d.fracture )
+        {
+            require( d.G0 > 0, "fracture models need G0 > 0" );
+            // CabanaPD_ForceModels.hpp:97-101; LPS override force_models/CabanaPD_LPS.hpp:214-222
+            k.s0 = std::sqrt( 5.0 * d.G0 / 9.0 / K / delta );
+            if ( d.influence == 0 )
+                k.s0 = std::sqrt( 8.0 * d.G0 / 15.0 / K / delta );
+            k.bond_break_coeff = ( 1.0 + k.s0 ) * ( 1.0 + k.s0 );
+        }
+        // force_models/CabanaPD_LPS.hpp:75-82
+        k.theta_coeff = 3.0 * K - 5.0 * d.G;
+        k.s_coeff = 15.0 * d.G;
+        if ( d.model == PDK_MODEL_LPS )
+            require( d.influence == 0 || d.influence == 1, "Influence function type must be 0 or 1." );
+    }
+    else if ( d.model == PDK_MODEL_HERTZ )
+    {
+        require( d.radius > 0 && d.E > 0 && d.restitution > 0, "Hertz needs radius, E, restitution > 0" );
+        // force_models/CabanaPD_Hertzian.hpp:46-55
+        k.Rs = 0.5 * d.radius;
+        k.Es = d.E / ( 2.0 * ( 1.0 - std::pow( d.nu, 2.0 ) ) );
+        double ln_e = std::log( d.restitution );
+        k.beta = -ln_e / std::sqrt( std::pow( ln_e, 2.0 ) + std::pow( kPi, 2.0 ) );
+        k.coeff_h_n = 4.0 / 3.0 * k.Es * std::sqrt( k.Rs );
+        k.coeff_h_d = -2.0 * std::sqrt( 5.0 / 6.0 ) * k.beta;
+    }
+    else if ( d.model == PDK_MODEL_HERTZ_JKR )
+    {
+        require( d.radius > 0 && d.E > 0 && d.restitution > 0, "JKR needs radius, E, restitution > 0" );
+        // force_models/CabanaPD_HertzianJKR.hpp:59-79
+        k.Rs = 0.5 * d.radius;
+        k.Es = d.E / ( 2.0 * ( 1.0 - std::pow( d.nu, 2.0 ) ) );
+        k.Gs = d.E / ( 4.0 * ( 2 - d.nu ) * ( 1 + d.nu ) );
+        double ln_e = std::log( d.restitution );
+        k.beta = -ln_e / std::sqrt( std::pow( ln_e, 2.0 ) + std::pow( kPi, 2.0 ) );
+        k.coeff_h_n = 4.0 / 3.0 * k.Es / k.Rs;
+        k.coeff_h_d = -2.0 * std::sqrt( 5.0 / 6.0 ) * k.beta;
+        k.Gamma = 2.0 * d.gamma;
+        k.a0 = std::pow( 9.0 / 2.0 * k.Gamma * kPi / k.Es * std::pow( k.Rs, 2.0 ), 1.0 / 3.0 );
+        k.delta_tear = -1.0 / 2.0 * std::pow( 6.0, -1.0 / 3.0 ) * std::pow( k.a0, 2.0 ) / k.Rs;
+        k.fc = 3.0 / 2.0 * kPi * k.Rs * k.Gamma;
+    }
+    else
+        throw Error( "unknown model kind" );
+}
+
+enum KernelClass
+{
+    kClsForce = 0,
+    kClsLps = 1,
+    kClsEnergy = 2,
+    kClsBuild = 3,
+    kClsOther = 4,
+    kNumCls = 5
+};
+
+} // namespace
+
+struct pdk_ctx_s
+{
+    int device = 0;
+    MemAccount acct;
+    std::unique_ptr<BatchEngine> batch;
+};
+
+struct pdk_spec_s
+{
+    pdk_ctx ctx = nullptr;
+    pdk_model_desc d{};
+    pdk_model_constants k{};
+    int64_t n = 0, nslices = 0;
+    hipStream_t st = nullptr;
+    DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
+        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging;
+    std::vector<int64_t> h_perm, h_inv;
+    PDArgs h{};
+    int32_t cur = 0;
+    bool uploaded = false, built = false, inited = false;
+    int64_t col_rows_cap = 0, total_rows = 0, total_bonds = 0;
+    int32_t max_cnt = 0;
+    int64_t steps_done = 0;
+    // contact bookkeeping
+    int64_t steps_since_build = 0;
+    double max_disp = 0.0;
+    // timing
+    bool timing = false;
+    struct Ev
+    {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> evs;
+    double tsum[kNumCls] = {};
+    int64_t tcnt[kNumCls] = {};
+    BuildScratch scratch;
+    // serving
+    bool in_batch = false;
+
+    MemAccount* acct() { return &ctx->acct; }
+
+    void pushArgs()
+    {
+        h.n = n;
+        h.nslices = nslices;
+        h.x = x.as<double>();
+        h.U[0] = U0.as<double>();
+        h.U[1] = U1.as<double>();
+        h.v = v.as<double>();
+        h.f = f.as<double>();
+        h.vol = vol.as<double>();
+        h.rho = rho.as<double>();
+        h.nofail = nofail.as<uint8_t>();
+        h.bcflag = bcflag.as<uint8_t>();
+        h.body = body ? body.as<double>() : nullptr;
+        h.slice_ptr = slice_ptr.as<int64_t>();
+        h.cnt = cnt.as<int32_t>();
+        h.col = col.as<int32_t>();
+        h.intact = intact.as<uint64_t>();
+        h.m = m.as<double>();
+        h.theta = theta.as<double>();
+        h.lpsA = lpsA.as<double>();
+        h.lpsB = lpsB.as<double>();
+        h.W = W.as<double>();
+        h.phi = phi.as<double>();
+        h.radius = radius ? radius.as<double>() : nullptr;
+        h.u_neigh = u_neigh ? u_neigh.as<double>() : nullptr;
+        h.c = k.c;
+        h.bcc = k.bond_break_coeff;
+        h.s0 = k.s0;
+        h.theta_coeff = k.theta_coeff;
+        h.s_coeff = k.s_coeff;
+        h.dt = d.dt;
+        h.half_dt = 0.5 * d.dt;
+        h.s0f = static_cast<float>( k.s0 );
+        // FP32 stretch error is a few ulp of (|eta|/|xi| + |s|); 2^-13 leaves
+        // a margin of ~2^8 over that bound.
+        h.bandf = 1.220703125e-4f;
+        h.Rs = k.Rs;
+        h.Es = k.Es;
+        h.coeff_h_n = k.coeff_h_n;
+        h.coeff_h_d = k.coeff_h_d;
+        h.Gamma = k.Gamma;
+        h.a0 = k.a0;
+        h.delta_tear = k.delta_tear;
+        h.two_r = 2.0 * d.radius;
+        h.mu_f = d.mu_friction;
+        h.nu = d.nu;
+        h.E = d.E;
+        h.model = d.model;
+        h.fracture = d.fracture;
+        h.influence = d.influence;
+        h.precision = d.precision;
+        h.polydisperse = d.polydisperse;
+        PDK_HIP( hipMemcpyAsync( args.as<PDArgs>(), &h, sizeof( PDArgs ), hipMemcpyHostToDevice, st ) );
+    }
+
+    const PDArgs* dargs() const { return args.as<PDArgs>(); }
+
+    template <class F>
+    void timed( int cls, F&& fn )
+    {
+        if ( !timing )
+        {
+            fn();
+            return;
+        }
+        Ev e;
+        e.cls = cls;
+        PDK_HIP( hipEventCreate( &e.a ) );
+        PDK_HIP( hipEventCreate( &e.b ) );
+        PDK_HIP( hipEventRecord( e.a, st ) );
+        fn();
+        PDK_HIP( hipEventRecord( e.b, st ) );
+        evs.push_back( e );
+    }
+
+    void resolveEvents()
+    {
+        if ( evs.empty() )
+            return;
+        PDK_HIP( hipStreamSynchronize( st ) );
+        for ( auto& e : evs )
+        {
+            float ms = 0.f;
+            PDK_HIP( hipEventElapsedTime( &ms, e.a, e.b ) );
+            tsum[e.cls] += ms;
+            tcnt[e.cls] += 1;
+            (void)hipEventDestroy( e.a );
+            (void)hipEventDestroy( e.b );
+        }
+        evs.clear();
+    }
+
+    void stageUpload( const void* src, size_t bytes )
+    {
+        if ( staging.bytes() < bytes )
+            staging.alloc( bytes, acct() );
+        PDK_HIP( hipMemcpyAsync( staging.as<void>(), src, bytes, hipMemcpyHostToDevice, st ) );
+    }
+
+    void uploadAos3( const double* src, double* dst_soa )
+    {
+        stageUpload( src, 3 * n * sizeof( double ) );
+        launchGatherAos3( staging.as<double>(), perm.as<int64_t>(), dst_soa, n, st );
+    }
+    void uploadScalar( const double* src, double* dst )
+    {
+        stageUpload( src, n * sizeof( double ) );
+        launchGatherScalar( staging.as<double>(), perm.as<int64_t>(), dst, n, st );
+    }
+    void fillScalar( double* dst, double value )
+    {
+        std::vector<double> hv( n, value );
+        PDK_HIP( hipMemcpyAsync( dst, hv.data(), n * sizeof( double ), hipMemcpyHostToDevice, st ) );
+        PDK_HIP( hipStreamSynchronize( st ) );
+    }
+
+    double* Ucur() { return cur ? U1.as<double>() : U0.as<double>(); }
+
+    // ---- PD orchestration ---------------------------------------------------
+    bool lps() const { return d.model == PDK_MODEL_LPS; }
+
+    void lpsState( bool wv, bool dil )
+    {
+        if ( !lps() )
+            return;
+        timed( kClsLps, [&] { launchLpsState( dargs(), h, wv, dil, cur, st ); } );
+    }
+
+    // Solver::updateForce (CabanaPD_Solver.hpp:462-477) without the epilogue.
+    void updateForceStore()
+    {
+        if ( lps() )
+            lpsState( d.fracture != 0, true );
+        timed( kClsForce, [&] { launchForce( dargs(), h, kEpiStoreOnly, cur, st ); } );
+    }
+};
+
+// ---- contact orchestration lives in pdk_contact_api.hip ------------------------
+namespace pdk
+{
+void contactBuild( pdk_spec s, bool initial );
+void contactStep( pdk_spec s, int32_t nsteps );
+void contactForceStore( pdk_spec s );
+} // namespace pdk
+
+extern "C" {
+
+const char* pdk_last_error( void )
+{
+    return g_last_error.c_str();
+}
+
+int pdk_model_constants_get( const pdk_model_desc* desc, pdk_model_constants* out )
+{
+    return guarded( [&] {
+        require( desc && out, "null argument" );
+        modelConstants( *desc, *out );
+    } );
+}
+
+int pdk_critical_timestep( double dx, double dy, double dz, double K, double rho, double horizon,
+                           double safety_factor, double* dt_out )
+{
+    return guarded( [&] {
+        require( dt_out && dx > 0 && dy > 0 && dz > 0 && horizon >= dx, "invalid critical time step input" );
+        // Inputs::computeCriticalTimeStep (CabanaPD_Input.hpp:217-336); m from
+        // the Inputs ctor (CabanaPD_Input.hpp:71-83).
+        double v_p = dx * dy * dz;
+        double sum = 0;
+        int mm = static_cast<int>( std::floor( horizon / dx ) );
+        double c = 18.0 * K / ( kPi * horizon * horizon * horizon * horizon );
+        for ( int i = -( mm + 1 ); i < mm + 2; i++ )
+        {
+            double xi_1 = i * dx;
+            for ( int j = -( mm + 1 ); j < mm + 2; j++ )
+            {
+                double xi_2 = j * dy;
+                for ( int kk = -( mm + 1 ); kk < mm + 2; kk++ )
+                {
+                    double xi_3 = kk * dz;
+                    double r2 = xi_1 * xi_1 + xi_2 * xi_2 + xi_3 * xi_3;
+                    if ( r2 < horizon * horizon + 1e-10 && r2 > 0 )
+                        sum += v_p * c / std::sqrt( r2 );
+                }
+            }
+        }
+        *dt_out = safety_factor * std::sqrt( 2.0 * rho / sum );
+    } );
+}
+
+int pdk_ctx_create( int device, uint64_t hbm_budget, pdk_ctx* out )
+{
+    return guarded( [&] {
+        require( out != nullptr, "null ctx out" );
+        int count = 0;
+        PDK_HIP( hipGetDeviceCount( &count ) );
+        require( device >= 0 && device < count, "no such HIP device" );
+        PDK_HIP( hipSetDevice( device ) );
+        auto c = std::make_unique<pdk_ctx_s>();
+        c->device = device;
+        size_t freeb = 0, total = 0;
+        PDK_HIP( hipMemGetInfo( &freeb, &total ) );
+        c->acct.budget = hbm_budget ? hbm_budget : static_cast<uint64_t>( 0.9 * freeb );
+        c->batch = std::make_unique<BatchEngine>( c.get() );
+        *out = c.release();
+    } );
+}
+
+int pdk_ctx_destroy( pdk_ctx ctx )
+{
+    return guarded( [&] {
+        if ( ctx )
+        {
+            ctx->batch.reset();
+            delete ctx;
+        }
+    } );
+}
+
+int pdk_ctx_bytes_in_use( pdk_ctx ctx, uint64_t* bytes )
+{
+    return guarded( [&] {
+        require( ctx && bytes, "null argument" );
+        *bytes = ctx->acct.in_use;
+    } );
+}
+
+int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spec* out )
+{
+    return guarded( [&] {
+        require( ctx && desc && out, "null argument" );
+        require( n > 0, "a specimen needs at least one particle" );
+        require( n < ( int64_t( 1 ) << 31 ), "at most 2^31-1 particles per specimen" );
+        require( desc->precision == PDK_FP64 || desc->precision == PDK_MIXED, "unknown precision" );
+        require( !( isContact( *desc ) && desc->precision != PDK_FP64 ), "contact models run in FP64" );
+        require( desc->dt > 0, "dt must be positive" );
+        PDK_HIP( hipSetDevice( ctx->device ) );
+        auto s = std::make_unique<pdk_spec_s>();
+        s->ctx = ctx;
+        s->d = *desc;
+        modelConstants( s->d, s->k );
+        s->n = n;
+        s->nslices = ( n + kSlice - 1 ) / kSlice;
+        PDK_HIP( hipStreamCreateWithFlags( &s->st, hipStreamNonBlocking ) );
+        auto* a = s->acct();
+        const size_t d3 = 3 * n * sizeof( double ), d1 = n * sizeof( double );
+        s->x.alloc( d3, a );
+        s->U0.alloc( d3, a );
+        s->U1.alloc( d3, a );
+        s->v.alloc( d3, a );
+        s->f.alloc( d3, a );
+        s->vol.alloc( d1, a );
+        s->rho.alloc( d1, a );
+        s->nofail.alloc( n, a );
+        s->bcflag.alloc( n, a );
+        s->W.alloc( d1, a );
+        s->phi.alloc( d1, a );
+        if ( desc->model == PDK_MODEL_LPS )
+        {
+            s->m.alloc( d1, a );
+            s->theta.alloc( d1, a );
+            s->lpsA.alloc( d1, a );
+            s->lpsB.alloc( d1, a );
+        }
+        s->perm.alloc( n * sizeof( int64_t ), a );
+        s->inv.alloc( n * sizeof( int64_t ), a );
+        s->cnt.alloc( n * sizeof( int32_t ), a );
+        s->slice_ptr.alloc( ( s->nslices + 1 ) * sizeof( int64_t ), a );
+        s->partial.alloc( std::max<int64_t>( 2 * ( s->nslices / kSlicesPerBlock + 2 ), 64 ) * sizeof( double ), a );
+        s->args.alloc( sizeof( PDArgs ), a );
+        if ( isContact( *desc ) )
+        {
+            s->u_neigh.alloc( d3, a );
+            if ( desc->polydisperse )
+                s->radius.alloc( d1, a );
+        }
+        s->h.bc_kind = -1;
+        *out = s.release();
+    } );
+}
+
+int pdk_spec_destroy( pdk_spec s )
+{
+    return guarded( [&] {
+        if ( !s )
+            return;
+        if ( s->in_batch && s->ctx && s->ctx->batch )
+            s->ctx->batch->forget( s );
+        s->resolveEvents();
+        if ( s->st )
+        {
+            (void)hipStreamSynchronize( s->st );
+            (void)hipStreamDestroy( s->st );
+        }
+        delete s;
+    } );
+}
+
+int pdk_spec_upload( pdk_spec s, const double* x, const double* u, const double* v, const double* vol,
+                     const double* rho, const int32_t* nofail, const double* radius )
+{
+    return guarded( [&] {
+        require( s && x && vol, "x and vol are required" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        const int64_t n = s->n;
+        // Morton order of the reference positions on a grid of the mean
+        // particle spacing (cube root of the mean volume).
+        double lo[3] = { x[0], x[1], x[2] };
+        double vsum = 0.0;
+        for ( int64_t i = 0; i < n; i++ )
+        {
+            for ( int dd = 0; dd < 3; dd++ )
+                lo[dd] = std::min( lo[dd], x[3 * i + dd] );
+            vsum += vol[i];
+        }
+        double h = std::cbrt( vsum / n );
+        if ( !( h > 0 ) || !std::isfinite( h ) )
+            h = 1.0;
+        s->stageUpload( x, 3 * n * sizeof( double ) );
+        launchMortonOrder( s->staging.as<double>(), n, lo, 1.0 / h, s->perm.as<int64_t>(), s->inv.as<int64_t>(),
+                           s->scratch, s->acct(), s->st );
+        s->h_perm.resize( n );
+        s->h_inv.resize( n );
+        PDK_HIP( hipMemcpyAsync( s->h_perm.data(), s->perm.as<int64_t>(), n * sizeof( int64_t ), hipMemcpyDeviceToHost,
+                                 s->st ) );
+        PDK_HIP( hipMemcpyAsync( s->h_inv.data(), s->inv.as<int64_t>(), n * sizeof( int64_t ), hipMemcpyDeviceToHost,
+                                 s->st ) );
+        launchGatherAos3( s->staging.as<double>(), s->perm.as<int64_t>(), s->x.as<double>(), n, s->st );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+        if ( u )
+            s->uploadAos3( u, s->U0.as<double>() );
+        else
+            PDK_HIP( hipMemsetAsync( s->U0.as<double>(), 0, 3 * n * sizeof( double ), s->st ) );
+        PDK_HIP( hipMemsetAsync( s->U1.as<double>(), 0, 3 * n * sizeof( double ), s->st ) );
+        if ( v )
+            s->uploadAos3( v, s->v.as<double>() );
+        else
+            PDK_HIP( hipMemsetAsync( s->v.as<double>(), 0, 3 * n * sizeof( double ), s->st ) );
+        PDK_HIP( hipMemsetAsync( s->f.as<double>(), 0, 3 * n * sizeof( double ), s->st ) );
+        s->uploadScalar( vol, s->vol.as<double>() );
+        if ( rho )
+            s->uploadScalar( rho, s->rho.as<double>() );
+        else
+            s->fillScalar( s->rho.as<double>(), 1.0 ); // CabanaPD_Particles.hpp:363
+        if ( nofail )
+        {
+            s->stageUpload( nofail, n * sizeof( int32_t ) );
+            launchGatherFlags( s->staging.as<int32_t>(), s->perm.as<int64_t>(), s->nofail.as<uint8_t>(), n, s->st );
+        }
+        else
+            PDK_HIP( hipMemsetAsync( s->nofail.as<uint8_t>(), 0, n, s->st ) );
+        PDK_HIP( hipMemsetAsync( s->bcflag.as<uint8_t>(), 0, n, s->st ) );
+        if ( s->radius )
+        {
+            require( radius != nullptr, "polydisperse contact needs per-particle radii" );
+            s->uploadScalar( radius, s->radius.as<double>() );
+        }
+        PDK_HIP( hipMemsetAsync( s->W.as<double>(), 0, n * sizeof( double ), s->st ) );
+        PDK_HIP( hipMemsetAsync( s->phi.as<double>(), 0, n * sizeof( double ), s->st ) );
+        if ( s->lps() )
+            for ( DevBuf* b : { &s->m, &s->theta, &s->lpsA, &s->lpsB } )
+                PDK_HIP( hipMemsetAsync( b->as<double>(), 0, n * sizeof( double ), s->st ) );
+        if ( s->u_neigh )
+            PDK_HIP( hipMemsetAsync( s->u_neigh.as<double>(), 0, 3 * n * sizeof( double ), s->st ) );
+        s->cur = 0;
+        s->uploaded = true;
+        s->built = false;
+        s->inited = false;
+        s->steps_done = 0;
+        s->pushArgs();
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_spec_set_state( pdk_spec s, const double* u, const double* v )
+{
+    return guarded( [&] {
+        require( s && s->uploaded, "spec not uploaded" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        if ( u )
+            s->uploadAos3( u, s->Ucur() );
+        if ( v )
+            s->uploadAos3( v, s->v.as<double>() );
+        PDK_HIP( hipMemsetAsync( s->f.as<double>(), 0, 3 * s->n * sizeof( double ), s->st ) );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_spec_bytes( pdk_spec s, uint64_t* bytes )
+{
+    return guarded( [&] {
+        require( s && bytes, "null argument" );
+        uint64_t t = 0;
+        for ( const DevBuf* b : { &s->x, &s->U0, &s->U1, &s->v, &s->f, &s->vol, &s->rho, &s->nofail, &s->bcflag,
+                                  &s->body, &s->m, &s->theta, &s->lpsA, &s->lpsB, &s->W, &s->phi, &s->perm, &s->inv,
+                                  &s->cnt, &s->slice_ptr, &s->col, &s->intact, &s->radius, &s->u_neigh,
+                                  &s->partial, &s->args } )
+            t += b->bytes();
+        *bytes = t;
+    } );
+}
+
+int pdk_build_bonds( pdk_spec s )
+{
+    return guarded( [&] {
+        require( s && s->uploaded, "spec not uploaded" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        if ( isContact( s->d ) )
+        {
+            s->timed( kClsBuild, [&] { contactBuild( s, true ); } );
+        }
+        else
+        {
+            NeighborQuery q{};
+            q.x = s->x.as<double>();
+            q.u = nullptr;
+            q.radius = nullptr;
+            q.rc = s->d.horizon + s->d.neighbor_tol;
+            q.extend = 0.0;
+            q.n = s->n;
+            s->timed( kClsBuild, [&] {
+                s->total_rows = buildSlicedCsr( q, s->perm.as<int64_t>(), s->inv.as<int64_t>(),
+                                                s->slice_ptr.as<int64_t>(), s->cnt.as<int32_t>(), s->col, s->intact,
+                                                s->col_rows_cap, s->scratch, s->acct(), s->st, &s->max_cnt );
+            } );
+        }
+        std::vector<int32_t> c( s->n );
+        PDK_HIP( hipMemcpyAsync( c.data(), s->cnt.as<int32_t>(), s->n * sizeof( int32_t ), hipMemcpyDeviceToHost,
+                                 s->st ) );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+        int64_t t = 0;
+        int32_t mx = 0;
+        for ( int32_t v : c )
+        {
+            t += v;
+            mx = std::max( mx, v );
+        }
+        s->total_bonds = t;
+        s->max_cnt = mx;
+        s->built = true;
+        s->inited = false;
+        s->pushArgs();
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_bond_stats( pdk_spec s, int64_t* total_bonds, int32_t* max_neighbors )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        if ( total_bonds )
+            *total_bonds = s->total_bonds;
+        if ( max_neighbors )
+            *max_neighbors = s->max_cnt;
+    } );
+}
+
+int pdk_prenotch( pdk_spec s, int32_t nn, const double* p0, const double* v1, const double* v2 )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        require( s->d.fracture != 0, "Cannot create prenotch in system without fracture." );
+        require( nn >= 0 && ( nn == 0 || ( p0 && v1 && v2 ) ), "invalid prenotch arguments" );
+        if ( nn == 0 )
+            return;
+        std::vector<double> nt( 9 * nn );
+        for ( int t = 0; t < nn; t++ )
+        {
+            for ( int dd = 0; dd < 3; dd++ )
+            {
+                nt[9 * t + dd] = p0[3 * t + dd];
+                nt[9 * t + 3 + dd] = v1[3 * t + dd];
+                nt[9 * t + 6 + dd] = v2[3 * t + dd];
+            }
+            // Prenotch.hpp:108 asserts non-parallel orientation vectors.
+            const double cx = v1[3 * t + 1] * v2[3 * t + 2] - v1[3 * t + 2] * v2[3 * t + 1];
+            const double cy = v1[3 * t + 0] * v2[3 * t + 2] - v1[3 * t + 2] * v2[3 * t + 0];
+            const double cz = v1[3 * t + 0] * v2[3 * t + 1] - v1[3 * t + 1] * v2[3 * t + 0];
+            require( std::sqrt( cx * cx + cy * cy + cz * cz ) > 1e-10, "prenotch vectors are parallel" );
+        }
+        s->notch.alloc( nt.size() * sizeof( double ), s->acct() );
+        PDK_HIP( hipMemcpyAsync( s->notch.as<double>(), nt.data(), nt.size() * sizeof( double ), hipMemcpyHostToDevice,
+                                 s->st ) );
+        s->timed( kClsOther, [&] { launchPrenotch( s->h, nn, s->notch.as<double>(), s->st ); } );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_download_bonds( pdk_spec s, int64_t* offsets, int64_t* nbr, uint8_t* intact )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        const int64_t n = s->n, ns = s->nslices;
+        std::vector<int64_t> sp( ns + 1 );
+        std::vector<int32_t> c( n );
+        std::vector<int32_t> col( static_cast<size_t>( s->total_rows ) * kSlice );
+        std::vector<uint64_t> bits( s->total_rows );
+        PDK_HIP( hipMemcpyAsync( sp.data(), s->slice_ptr.as<int64_t>(), ( ns + 1 ) * sizeof( int64_t ),
+                                 hipMemcpyDeviceToHost, s->st ) );
+        PDK_HIP( hipMemcpyAsync( c.data(), s->cnt.as<int32_t>(), n * sizeof( int32_t ), hipMemcpyDeviceToHost, s->st ) );
+        if ( s->total_rows )
+        {
+            PDK_HIP( hipMemcpyAsync( col.data(), s->col.as<int32_t>(), col.size() * sizeof( int32_t ),
+                                     hipMemcpyDeviceToHost, s->st ) );
+            PDK_HIP( hipMemcpyAsync( bits.data(), s->intact.as<uint64_t>(), bits.size() * sizeof( uint64_t ),
+                                     hipMemcpyDeviceToHost, s->st ) );
+        }
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+        int64_t off = 0;
+        for ( int64_t cid = 0; cid < n; cid++ )
+        {
+            const int64_t p = s->h_inv[cid];
+            const int64_t slice = p / kSlice, lane = p % kSlice;
+            if ( offsets )
+                offsets[cid] = off;
+            for ( int32_t q = 0; q < c[p]; q++ )
+            {
+                const int64_t row = sp[slice] + q;
+                if ( nbr )
+                    nbr[off + q] = s->h_perm[col[row * kSlice + lane]];
+                if ( intact )
+                    intact[off + q] = static_cast<uint8_t>( ( bits[row] >> lane ) & 1ull );
+            }
+            off += c[p];
+        }
+        if ( offsets )
+            offsets[n] = off;
+    } );
+}
+
+int pdk_set_bc( pdk_spec s, int32_t kind, const int64_t* idx, int64_t nidx, double value )
+{
+    return guarded( [&] {
+        require( s && s->uploaded, "spec not uploaded" );
+        require( kind == PDK_BC_FORCE_VALUE || kind == PDK_BC_FORCE_UPDATE, "unknown BC kind" );
+        require( nidx >= 0 && ( nidx == 0 || idx ), "invalid BC index list" );
+        std::vector<uint8_t> flag( s->n, 0 );
+        for ( int64_t b = 0; b < nidx; b++ )
+        {
+            require( idx[b] >= 0 && idx[b] < s->n, "BC particle id out of range" );
+            flag[s->h_inv[idx[b]]] = 1;
+        }
+        PDK_HIP( hipMemcpyAsync( s->bcflag.as<uint8_t>(), flag.data(), s->n, hipMemcpyHostToDevice, s->st ) );
+        s->h.bc_kind = kind;
+        s->h.bc_value = value;
+        s->pushArgs();
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_set_body_force( pdk_spec s, const double* b )
+{
+    return guarded( [&] {
+        require( s && s->uploaded, "spec not uploaded" );
+        if ( b )
+        {
+            if ( !s->body )
+                s->body.alloc( 3 * s->n * sizeof( double ), s->acct() );
+            s->uploadAos3( b, s->body.as<double>() );
+        }
+        else
+            s->body.release();
+        s->pushArgs();
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_compute_weighted_volume( pdk_spec s )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        s->lpsState( true, false );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_compute_dilatation( pdk_spec s )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        s->lpsState( false, true );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_compute_force( pdk_spec s, int32_t reset )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        if ( isContact( s->d ) )
+        {
+            require( reset != 0, "contact force accumulation without reset is not supported" );
+            contactForceStore( s );
+        }
+        else
+            s->timed( kClsForce, [&] { launchForce( s->dargs(), s->h, reset ? kEpiStoreOnly : -1, s->cur, s->st ); } );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_compute_energy( pdk_spec s, double* total_strain_energy, double* total_damage )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        double Phi = 0.0, D = 0.0;
+        if ( !isContact( s->d ) )
+        {
+            const int64_t nb = energyBlocks( s->h );
+            if ( s->partial.bytes() < static_cast<size_t>( 2 * nb ) * sizeof( double ) )
+                s->partial.alloc( 2 * nb * sizeof( double ), s->acct() );
+            s->timed( kClsEnergy, [&] { launchEnergy( s->dargs(), s->h, s->cur, s->partial.as<double>(), nb, s->st ); } );
+            std::vector<double> part( 2 * nb );
+            PDK_HIP( hipMemcpyAsync( part.data(), s->partial.as<double>(), 2 * nb * sizeof( double ),
+                                     hipMemcpyDeviceToHost, s->st ) );
+            PDK_HIP( hipStreamSynchronize( s->st ) );
+            for ( int64_t b = 0; b < nb; b++ )
+            {
+                Phi += part[2 * b];
+                D += part[2 * b + 1];
+            }
+        }
+        if ( total_strain_energy )
+            *total_strain_energy = Phi;
+        if ( total_damage )
+            *total_damage = s->d.fracture ? D : 0.0;
+    } );
+}
+
+int pdk_init( pdk_spec s )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        // Solver::init(bc) (CabanaPD_Solver.hpp:229-279)
+        if ( isContact( s->d ) )
+        {
+            contactForceStore( s );
+        }
+        else
+        {
+            if ( s->lps() && !s->d.fracture )
+                s->lpsState( true, false );
+            s->updateForceStore();
+            const int64_t nb = energyBlocks( s->h );
+            if ( s->partial.bytes() < static_cast<size_t>( 2 * nb ) * sizeof( double ) )
+                s->partial.alloc( 2 * nb * sizeof( double ), s->acct() );
+            s->timed( kClsEnergy, [&] { launchEnergy( s->dargs(), s->h, s->cur, s->partial.as<double>(), nb, s->st ); } );
+        }
+        s->timed( kClsOther, [&] { launchApplyForceTerms( s->dargs(), s->h, s->st ); } );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+        s->inited = true;
+    } );
+}
+
+int pdk_step( pdk_spec s, int32_t nsteps )
+{
+    return guarded( [&] {
+        require( s && s->inited, "call pdk_init before pdk_step" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( nsteps >= 0, "nsteps must be >= 0" );
+        if ( nsteps == 0 )
+            return;
+        if ( isContact( s->d ) )
+        {
+            contactStep( s, nsteps );
+        }
+        else
+        {
+            // Solver::runStep (CabanaPD_Solver.hpp:332-375): the initial half
+            // step of step 1 runs alone; every later initial half step runs in
+            // the previous step's force epilogue.
+            s->timed( kClsOther, [&] { launchInitialHalfStep( s->dargs(), s->h, s->cur, s->st ); } );
+            s->cur ^= 1;
+            for ( int32_t k = 1; k <= nsteps; k++ )
+            {
+                if ( s->lps() )
+                    s->lpsState( s->d.fracture != 0, true );
+                const int32_t epi = k < nsteps ? kEpiFinalInitial : kEpiFinal;
+                s->timed( kClsForce, [&] { launchForce( s->dargs(), s->h, epi, s->cur, s->st ); } );
+                if ( epi == kEpiFinalInitial )
+                    s->cur ^= 1;
+            }
+        }
+        s->steps_done += nsteps;
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_steps_done( pdk_spec s, int64_t* steps )
+{
+    return guarded( [&] {
+        require( s && steps, "null argument" );
+        *steps = s->steps_done;
+    } );
+}
+
+int pdk_download( pdk_spec s, int32_t field, double* out )
+{
+    return guarded( [&] {
+        require( s && s->uploaded && out, "spec not uploaded" );
+        const int64_t n = s->n;
+        const double* src = nullptr;
+        bool vec = false;
+        switch ( field )
+        {
+        case PDK_FIELD_X: src = s->x.as<double>(); vec = true; break;
+        case PDK_FIELD_U: src = s->Ucur(); vec = true; break;
+        case PDK_FIELD_V: src = s->v.as<double>(); vec = true; break;
+        case PDK_FIELD_F: src = s->f.as<double>(); vec = true; break;
+        case PDK_FIELD_VOL: src = s->vol.as<double>(); break;
+        case PDK_FIELD_RHO: src = s->rho.as<double>(); break;
+        case PDK_FIELD_W: src = s->W.as<double>(); break;
+        case PDK_FIELD_PHI: src = s->phi.as<double>(); break;
+        case PDK_FIELD_M: src = s->m.as<double>(); break;
+        case PDK_FIELD_THETA: src = s->theta.as<double>(); break;
+        default: throw Error( "unknown field" );
+        }
+        if ( !src )
+        {
+            std::fill( out, out + n, 0.0 );
+            return;
+        }
+        const size_t bytes = ( vec ? 3 : 1 ) * n * sizeof( double );
+        if ( s->staging.bytes() < bytes )
+            s->staging.alloc( bytes, s->acct() );
+        if ( vec )
+            launchScatterAos3( src, s->perm.as<int64_t>(), s->staging.as<double>(), n, s->st );
+        else
+            launchScatterScalar( src, s->perm.as<int64_t>(), s->staging.as<double>(), n, s->st );
+        PDK_HIP( hipMemcpyAsync( out, s->staging.as<double>(), bytes, hipMemcpyDeviceToHost, s->st ) );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_sync( pdk_spec s )
+{
+    return guarded( [&] {
+        require( s != nullptr, "null spec" );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_kernel_timing( pdk_spec s, int32_t enable )
+{
+    return guarded( [&] {
+        require( s != nullptr, "null spec" );
+        s->resolveEvents();
+        s->timing = enable != 0;
+        for ( int c = 0; c < kNumCls; c++ )
+        {
+            s->tsum[c] = 0.0;
+            s->tcnt[c] = 0;
+        }
+    } );
+}
+
+int pdk_kernel_time( pdk_spec s, int32_t cls, double* total_ms, int64_t* launches )
+{
+    return guarded( [&] {
+        require( s && cls >= 0 && cls < kNumCls, "invalid kernel class" );
+        s->resolveEvents();
+        if ( total_ms )
+            *total_ms = s->tsum[cls];
+        if ( launches )
+            *launches = s->tcnt[cls];
+    } );
+}
+
+int pdk_kernel_name( pdk_spec s, int32_t cls, char* buf, int32_t buflen )
+{
+    return guarded( [&] {
+        require( s && buf && buflen > 0, "invalid argument" );
+        const char* name = "";
+        switch ( cls )
+        {
+        case kClsForce: name = isContact( s->d ) ? contactKernelName( s->h ) : forceKernelName( s->h ); break;
+        case kClsLps: name = lpsKernelName( s->h ); break;
+        case kClsEnergy: name = energyKernelName( s->h ); break;
+        case kClsBuild: name = "kFillNeighbors"; break;
+        default: name = "kInitialHalfStep"; break;
+        }
+        std::snprintf( buf, buflen, "%s", name );
+    } );
+}
+
+} // extern "C"
+
+// ---- accessors for the contact and serving translation units -----------------
+namespace pdk
+{
+SpecView view( pdk_spec s )
+{
+    SpecView v;
+    v.h = &s->h;
+    v.dargs = s->dargs();
+    v.st = s->st;
+    v.n = s->n;
+    v.nslices = s->nslices;
+    v.cur = &s->cur;
+    v.steps_done = &s->steps_done;
+    v.in_batch = &s->in_batch;
+    v.inited = s->inited;
+    v.model = s->d.model;
+    v.fracture = s->d.fracture;
+    v.precision = s->d.precision;
+    v.bytes = 0;
+    pdk_spec_bytes( s, &v.bytes );
+    return v;
+}
+
+void specPushArgs( pdk_spec s )
+{
+    s->pushArgs();
+}
+
+void contactBuildImpl( pdk_spec s, bool initial, double* max_disp, int64_t* since_build )
+{
+    NeighborQuery q{};
+    q.x = s->x.as<double>();
+    q.u = s->Ucur();
+    q.radius = s->d.polydisperse ? s->radius.as<double>() : nullptr;
+    const double tol = initial ? s->d.neighbor_tol : 0.0;
+    q.rc = 2.0 * s->d.radius + s->d.radius_extend + tol;
+    q.extend = s->d.radius_extend + tol;
+    q.n = s->n;
+    s->total_rows = buildSlicedCsr( q, s->perm.as<int64_t>(), s->inv.as<int64_t>(), s->slice_ptr.as<int64_t>(),
+                                    s->cnt.as<int32_t>(), s->col, s->intact, s->col_rows_cap, s->scratch, s->acct(),
+                                    s->st, &s->max_cnt );
+    PDK_HIP( hipMemcpyAsync( s->u_neigh.as<double>(), s->Ucur(), 3 * s->n * sizeof( double ), hipMemcpyDeviceToDevice,
+                             s->st ) );
+    s->pushArgs();
+    *max_disp = 0.0;
+    *since_build = 0;
+    s->max_disp = 0.0;
+    s->steps_since_build = 0;
+}
+
+ContactState contactState( pdk_spec s )
+{
+    ContactState c;
+    c.max_disp = &s->max_disp;
+    c.since_build = &s->steps_since_build;
+    c.extend = s->d.radius_extend;
+    c.rebuild_every = s->d.rebuild_every;
+    c.partial = &s->partial;
+    c.acct = s->acct();
+    return c;
+}
+
+void specTimed( pdk_spec s, int cls, const std::function<void()>& fn )
+{
+    s->timed( cls, fn );
+}
+
+MemAccount& ctxAccount( pdk_ctx_s* c )
+{
+    return c->acct;
+}
+} // namespace pdk
+
+extern "C" {
+
+int pdk_submit( pdk_ctx ctx, pdk_spec s, int32_t nsteps, uint64_t* ticket )
+{
+    return guarded( [&] {
+        require( ctx && s && ticket, "null argument" );
+        require( s->ctx == ctx, "spec belongs to another context" );
+        *ticket = ctx->batch->submit( s, nsteps );
+    } );
+}
+
+int pdk_poll( pdk_ctx ctx, uint64_t ticket, int32_t* done )
+{
+    return guarded( [&] {
+        require( ctx && done, "null argument" );
+        *done = ctx->batch->done( ticket ) ? 1 : 0;
+    } );
+}
+
+int pdk_batch_progress( pdk_ctx ctx, int32_t max_steps, int32_t* launched )
+{
+    return guarded( [&] {
+        require( ctx != nullptr, "null ctx" );
+        const int32_t l = ctx->batch->progress( max_steps );
+        if ( launched )
+            *launched = l;
+    } );
+}
+
+} // extern "C"

@@ -1,0 +1,121 @@
+// pdk_common.hpp -- device-side data layout shared by the engine's kernels.
+//
+// Layout in HBM (one specimen):
+//   particles in Morton order of their reference positions, SoA:
+//     x[3][n] (FP64), U0[3][n] / U1[3][n] (FP64 displacement, double
+//     buffered: the force pass reads U[cur] and its fused velocity-Verlet
+//     epilogue writes U[cur^1]), v[3][n], f[3][n], vol[n], rho[n],
+//     nofail[n] (u8), bcflag[n] (u8), optional body[3][n],
+//     W[n], phi[n], m[n], theta[n].
+//   bond family as a sliced CSR with 64 rows per slice (one wavefront per
+//   slice, one lane per particle):
+//     slice_ptr[nslices+1] (i64, in "slot rows": 64 bonds each),
+//     cnt[n] (i32 row lengths),
+//     col[total_slots][64] (i32 sorted-order neighbor index; padding = self),
+//     intact[total_slots] (u64, bit lane = bond (row, slot) intact),
+//     sid[total_slots][64] (u8 lattice stencil id, optional).
+//   Row order within a slot row is ascending caller id, so a lane's serial
+//   sum visits bonds in the oracle's order.
+#ifndef PDK_COMMON_HPP
+#define PDK_COMMON_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/pdk.h"
+
+namespace pdk
+{
+constexpr int kSlice = 64;           // rows per slice = wavefront size
+constexpr int kSlicesPerBlock = 4;   // 256-thread workgroups
+constexpr int kBlock = kSlice * kSlicesPerBlock;
+constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, round-robin block dispatch
+
+enum EpilogueMode : int32_t
+{
+    kEpiStoreOnly = 0,   // computeForce: store f (after BC/body terms)
+    kEpiFinal = 1,       // final half step (v += dt/2 f/rho), store f
+    kEpiFinalInitial = 2 // final half step of this step + initial half step of the next
+};
+
+// Everything a PD/contact kernel needs for one specimen.  Lives in device
+// memory (one per spec, or an array of them for a packed batch).
+struct PDArgs
+{
+    int64_t n;
+    int64_t nslices;
+    const double* x;        // [3][n]
+    double* U[2];           // [3][n] each
+    double* v;              // [3][n]
+    double* f;              // [3][n]
+    const double* vol;
+    const double* rho;
+    const uint8_t* nofail;
+    const uint8_t* bcflag;
+    const double* body;     // [3][n] or null
+    const int64_t* slice_ptr;
+    const int32_t* cnt;
+    const int32_t* col;
+    uint64_t* intact;
+    const uint8_t* sid;     // stencil ids or null
+    const float4* stencil;  // stencil table (xi_x, xi_y, xi_z, |xi|) or null
+    double* m;
+    double* theta;
+    double* lpsA;           // LPS: theta_coeff * theta / m per particle
+    double* lpsB;           // LPS: s_coeff / m per particle
+    double* W;
+    double* phi;
+    const double* radius;   // contact: per-particle radius or null
+    double* u_neigh;        // contact: u at last list build [3][n]
+    double* block_partial;  // energy / max-displacement partials
+    // model constants
+    double c, bcc, s0, theta_coeff, s_coeff;
+    double dt, half_dt;
+    double bc_value;
+    // mixed-precision guard band: a bond whose FP32 stretch lies within
+    // [s0 - band, s0 + band] is re-tested in FP64
+    float s0f, bandf;
+    // contact constants
+    double Rs, Es, coeff_h_n, coeff_h_d, Gamma, a0, delta_tear, two_r, mu_f, nu, E;
+    int32_t model;
+    int32_t fracture;
+    int32_t influence;
+    int32_t precision;
+    int32_t bc_kind;        // -1 none
+    int32_t polydisperse;
+    int32_t parity_offset;  // batch: current U index = launch parity ^ parity_offset
+    int32_t pad0;
+    int64_t end_launch;     // batch: launch index of this spec's last step
+};
+
+// Block -> logical slice group, keeping consecutive slice groups on one XCD
+// so that their shared halo stays in that XCD's L2.
+__device__ __forceinline__ int64_t xcdRemap( int64_t b, int64_t nb )
+{
+    const int64_t xcd = b % kNumXcd;
+    const int64_t idx = b / kNumXcd;
+    const int64_t q = nb / kNumXcd;
+    const int64_t r = nb % kNumXcd;
+    return ( xcd < r ) ? xcd * ( q + 1 ) + idx : r * ( q + 1 ) + ( xcd - r ) * q + idx;
+}
+
+__host__ __device__ __forceinline__ uint64_t spreadBits3( uint64_t v )
+{
+    v &= 0x1fffff;
+    v = ( v | ( v << 32 ) ) & 0x1f00000000ffffULL;
+    v = ( v | ( v << 16 ) ) & 0x1f0000ff0000ffULL;
+    v = ( v | ( v << 8 ) ) & 0x100f00f00f00f00fULL;
+    v = ( v | ( v << 4 ) ) & 0x10c30c30c30c30c3ULL;
+    v = ( v | ( v << 2 ) ) & 0x1249249249249249ULL;
+    return v;
+}
+
+__host__ __device__ __forceinline__ uint64_t morton3( uint64_t a, uint64_t b, uint64_t c )
+{
+    return spreadBits3( a ) | ( spreadBits3( b ) << 1 ) | ( spreadBits3( c ) << 2 );
+}
+
+} // namespace pdk
+
+#endif

@@ -1,0 +1,731 @@
+// pdk_force.hip -- PD bond passes with the velocity-Verlet update fused into
+// the force epilogue.
+//
+// Mapping: one wavefront per 64-row slice of the sliced CSR, one lane per
+// particle, a serial loop over the lane's bonds, no atomics -- the reference's
+// SerialOpTag traversal (CabanaPD_Neighbor.hpp:31,134-151) laid out for
+// 64-wide wavefronts: each slot row of 64 neighbor ids is one coalesced 256 B
+// load and the broken-bond word of a slot row is one uniform 8 B load.
+//
+// Precision forms:
+//   FP64  : the reference's expressions, operand order and rounding
+//           (contraction off, correctly rounded sqrt/div).  Rows are summed
+//           in the oracle's order, so f/u/v match the oracle bit for bit.
+//   MIXED : bond geometry in FP32 from FP64 differences (no large-coordinate
+//           cancellation: xi = x_j - x_i and eta = u_j - u_i are formed in
+//           FP64 and rounded once), stretch s = (2 xi.eta + eta.eta) /
+//           (|xi| (|xi+eta| + |xi|)), bond forces accumulated in FP64.  A
+//           bond whose FP32 stretch lies inside a guard band around s0 is
+//           re-tested with the FP64 expression, so break decisions equal the
+//           FP64 decisions for the same state.
+#include <cmath>
+
+#include "pdk_internal.hpp"
+
+namespace pdk
+{
+namespace
+{
+enum ModelT : int
+{
+    kPMB = 0,
+    kLPS = 1
+};
+enum PrecT : int
+{
+    kF64 = 0,
+    kMixed = 1
+};
+
+struct Bond64
+{
+    double xi, r, s, rx, ry, rz;
+};
+
+// getDistance (CabanaPD_Force.hpp:74-93), operation for operation.
+__device__ __forceinline__ void getDistance64( double xi0, double xi1, double xi2, double ui0,
+                                               double ui1, double ui2, double xj0, double xj1,
+                                               double xj2, double uj0, double uj1, double uj2,
+                                               Bond64& b )
+{
+#pragma clang fp contract( off )
+    const double xi_x = xj0 - xi0;
+    const double eta_u = uj0 - ui0;
+    const double xi_y = xj1 - xi1;
+    const double eta_v = uj1 - ui1;
+    const double xi_z = xj2 - xi2;
+    const double eta_w = uj2 - ui2;
+    b.rx = xi_x + eta_u;
+    b.ry = xi_y + eta_v;
+    b.rz = xi_z + eta_w;
+    b.r = sqrt( b.rx * b.rx + b.ry * b.ry + b.rz * b.rz );
+    b.xi = sqrt( xi_x * xi_x + xi_y * xi_y + xi_z * xi_z );
+    b.s = ( b.r - b.xi ) / b.xi;
+}
+
+// BaseFractureModel CriticalStretchTag (CabanaPD_ForceModels.hpp:123-128).
+__device__ __forceinline__ bool criticalStretch64( double r, double xi, double bcc )
+{
+#pragma clang fp contract( off )
+    return r * r >= bcc * xi * xi;
+}
+
+struct Bond32
+{
+    float s, rinv, rx, ry, rz, xi;
+    float etan; // |eta| / |xi|, for the guard band
+};
+
+__device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi2, double ui0,
+                                               double ui1, double ui2, double xj0, double xj1,
+                                               double xj2, double uj0, double uj1, double uj2,
+                                               Bond32& b )
+{
+    const float gx = static_cast<float>( xj0 - xi0 );
+    const float gy = static_cast<float>( xj1 - xi1 );
+    const float gz = static_cast<float>( xj2 - xi2 );
+    const float ex = static_cast<float>( uj0 - ui0 );
+    const float ey = static_cast<float>( uj1 - ui1 );
+    const float ez = static_cast<float>( uj2 - ui2 );
+    b.rx = gx + ex;
+    b.ry = gy + ey;
+    b.rz = gz + ez;
+    const float r2 = b.rx * b.rx + b.ry * b.ry + b.rz * b.rz;
+    const float g2 = gx * gx + gy * gy + gz * gz;
+    b.rinv = __builtin_amdgcn_rsqf( r2 );
+    const float r = r2 * b.rinv;
+    const float xinv = __builtin_amdgcn_rsqf( g2 );
+    b.xi = g2 * xinv;
+    const float xe = gx * ex + gy * ey + gz * ez;
+    const float ee = ex * ex + ey * ey + ez * ez;
+    const float num = 2.0f * xe + ee;
+    b.s = num * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
+    b.etan = __builtin_sqrtf( ee ) * xinv;
+}
+
+// Uniform per-wave access to the spec of this wave's slice.
+struct WaveCtx
+{
+    const PDArgs* A;
+    int64_t slice;
+    int32_t epi;
+    int32_t cur;
+};
+
+__device__ __forceinline__ bool waveSetup( const PDArgs* __restrict__ specs,
+                                           const int32_t* __restrict__ tile_spec,
+                                           const int64_t* __restrict__ tile_slice, int32_t epi,
+                                           int32_t parity, int64_t launch, WaveCtx& w )
+{
+    const int wave = threadIdx.x / kSlice;
+    const int64_t t = xcdRemap( blockIdx.x, gridDim.x );
+    if ( tile_spec == nullptr )
+    {
+        w.A = specs;
+        w.slice = t * kSlicesPerBlock + wave;
+        w.epi = epi;
+        w.cur = parity;
+    }
+    else
+    {
+        w.A = specs + tile_spec[t];
+        w.slice = tile_slice[t] + wave;
+        w.cur = parity ^ w.A->parity_offset;
+        w.epi = ( launch + 1 < w.A->end_launch ) ? kEpiFinalInitial : kEpiFinal;
+    }
+    return w.slice < w.A->nslices;
+}
+
+// Force BC and body term (CabanaPD_Boundary.hpp:94-157, CabanaPD_BodyTerm.hpp:46-63),
+// then the velocity-Verlet halves (CabanaPD_Integrate.hpp:100-109, 128-134).
+__device__ __forceinline__ void epilogue( const PDArgs& A, int64_t p, int32_t epi, int32_t cur,
+                                          double fx, double fy, double fz )
+{
+#pragma clang fp contract( off )
+    const int64_t n = A.n;
+    if ( epi != kEpiStoreOnly )
+    {
+        if ( A.body )
+        {
+            fx = fx + A.body[p];
+            fy = fy + A.body[n + p];
+            fz = fz + A.body[2 * n + p];
+        }
+        if ( A.bc_kind >= 0 && A.bcflag[p] )
+        {
+            if ( A.bc_kind == PDK_BC_FORCE_VALUE )
+            {
+                fx = A.bc_value;
+                fy = A.bc_value;
+                fz = A.bc_value;
+            }
+            else
+            {
+                fx = fx + A.bc_value;
+                fy = fy + A.bc_value;
+                fz = fz + A.bc_value;
+            }
+        }
+    }
+    A.f[p] = fx;
+    A.f[n + p] = fy;
+    A.f[2 * n + p] = fz;
+    if ( epi == kEpiStoreOnly )
+        return;
+    const double half_dt_m = A.half_dt / A.rho[p];
+    double vx = A.v[p], vy = A.v[n + p], vz = A.v[2 * n + p];
+    vx = vx + half_dt_m * fx;
+    vy = vy + half_dt_m * fy;
+    vz = vz + half_dt_m * fz;
+    if ( epi == kEpiFinalInitial )
+    {
+        vx = vx + half_dt_m * fx;
+        vy = vy + half_dt_m * fy;
+        vz = vz + half_dt_m * fz;
+        const double* u = A.U[cur];
+        double* un = A.U[cur ^ 1];
+        un[p] = u[p] + A.dt * vx;
+        un[n + p] = u[n + p] + A.dt * vy;
+        un[2 * n + p] = u[2 * n + p] + A.dt * vz;
+    }
+    A.v[p] = vx;
+    A.v[n + p] = vy;
+    A.v[2 * n + p] = vz;
+}
+
+// ---------------------------------------------------------------------------
+// Force pass: Force<PMB|LPS, NoFracture|Fracture>::computeForceFull
+// (force/CabanaPD_PMB.hpp:89-125,222-287; force/CabanaPD_LPS.hpp:143-181,380-447)
+// For LPS the state pass has stored A = theta_coeff*theta/m and B = s_coeff/m
+// per particle (bitwise the per-bond subexpressions of the reference's
+// ForceCoeffTag, force_models/CabanaPD_LPS.hpp:131-133).
+// ---------------------------------------------------------------------------
+template <int MODEL, bool FRAC, int PREC>
+__global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ specs,
+                                                    const int32_t* __restrict__ tile_spec,
+                                                    const int64_t* __restrict__ tile_slice,
+                                                    int32_t epi, int32_t parity, int64_t launch,
+                                                    int32_t accumulate )
+{
+#pragma clang fp contract( off )
+    WaveCtx w;
+    if ( !waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w ) )
+        return;
+    const PDArgs& A = *w.A;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t n = A.n;
+    const int64_t p = w.slice * kSlice + lane;
+    const bool valid = p < n;
+    const int64_t pc = valid ? p : w.slice * kSlice;
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ u = A.U[w.cur];
+    const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
+    const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
+    const int32_t mycnt = valid ? A.cnt[p] : 0;
+    const int64_t base = A.slice_ptr[w.slice];
+    const int32_t len = static_cast<int32_t>( A.slice_ptr[w.slice + 1] - base );
+    const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
+    uint64_t* __restrict__ bits = A.intact + base;
+    const double* __restrict__ vol = A.vol;
+    const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
+    const double c = A.c;
+    const double bcc = A.bcc;
+    double Ai = 0.0, Bi = 0.0;
+    if constexpr ( MODEL == kLPS )
+    {
+        Ai = A.lpsA[pc]; // theta_coeff * theta_i / m_i
+        Bi = A.lpsB[pc]; // s_coeff / m_i
+    }
+    const double* __restrict__ Aarr = A.lpsA;
+    const double* __restrict__ Barr = A.lpsB;
+    const int infl1 = A.influence;
+
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if ( accumulate && valid )
+    {
+        fx = A.f[p];
+        fy = A.f[n + p];
+        fz = A.f[2 * n + p];
+    }
+    for ( int32_t q = 0; q < len; q++ )
+    {
+        const int64_t j = colp[q * kSlice];
+        uint64_t word = 0;
+        if constexpr ( FRAC )
+            word = bits[q];
+        bool brk = false;
+        if ( q < mycnt )
+        {
+            const double xj0 = x[j], xj1 = x[n + j], xj2 = x[2 * n + j];
+            const double uj0 = u[j], uj1 = u[n + j], uj2 = u[2 * n + j];
+            bool intact = true;
+            if constexpr ( FRAC )
+                intact = ( word >> lane ) & 1ull;
+            if constexpr ( PREC == kF64 )
+            {
+                Bond64 b;
+                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
+                if constexpr ( FRAC )
+                {
+                    const bool nf_j = ( MODEL == kLPS ) ? nf_i : ( A.nofail[j] != 0 );
+                    if ( criticalStretch64( b.r, b.xi, bcc ) && !nf_i && !nf_j )
+                        brk = true;
+                }
+                if ( !brk && intact )
+                {
+                    double coeff;
+                    if constexpr ( MODEL == kPMB )
+                        coeff = c * b.s * vol[j];
+                    else
+                    {
+                        const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                        coeff = ( Ai + Aarr[j] + b.s * ( Bi + Barr[j] ) ) * infl * b.xi * vol[j];
+                    }
+                    fx += coeff * b.rx / b.r;
+                    fy += coeff * b.ry / b.r;
+                    fz += coeff * b.rz / b.r;
+                }
+            }
+            else
+            {
+                Bond32 b;
+                getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
+                if constexpr ( FRAC )
+                {
+                    const bool nf_j = ( MODEL == kLPS ) ? nf_i : ( A.nofail[j] != 0 );
+                    if ( !nf_i && !nf_j )
+                    {
+                        const float band = A.bandf * ( b.etan + fabsf( b.s ) + A.s0f );
+                        if ( fabsf( b.s - A.s0f ) <= band )
+                        {
+                            Bond64 e;
+                            getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, e );
+                            brk = criticalStretch64( e.r, e.xi, bcc );
+                        }
+                        else
+                            brk = b.s > A.s0f;
+                    }
+                }
+                if ( !brk && intact )
+                {
+                    float coeff;
+                    if constexpr ( MODEL == kPMB )
+                        coeff = static_cast<float>( c ) * b.s * static_cast<float>( vol[j] );
+                    else
+                    {
+                        const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
+                        coeff = static_cast<float>( Ai + Aarr[j] ) +
+                                b.s * static_cast<float>( Bi + Barr[j] );
+                        coeff = coeff * infl * b.xi * static_cast<float>( vol[j] );
+                    }
+                    const float fr = coeff * b.rinv;
+                    fx += static_cast<double>( fr * b.rx );
+                    fy += static_cast<double>( fr * b.ry );
+                    fz += static_cast<double>( fr * b.rz );
+                }
+            }
+        }
+        if constexpr ( FRAC )
+        {
+            const uint64_t m = __ballot( brk );
+            if ( ( m & word ) && lane == 0 )
+                bits[q] = word & ~m;
+        }
+    }
+    if ( !valid )
+        return;
+    epilogue( A, p, w.epi, w.cur, fx, fy, fz );
+}
+
+// ---------------------------------------------------------------------------
+// LPS state pass: Force<LPS,*>::computeWeightedVolume + computeDilatation
+// (force/CabanaPD_LPS.hpp:89-141, 287-378), fused: m_i and theta_i need only
+// particle i's own bonds.  Also stores the per-particle force-coefficient
+// terms lpsA = theta_coeff*theta/m and lpsB = s_coeff/m used by the force pass.
+// ---------------------------------------------------------------------------
+template <bool FRAC, int PREC>
+__global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict__ specs,
+                                                       const int32_t* __restrict__ tile_spec,
+                                                       const int64_t* __restrict__ tile_slice,
+                                                       int32_t parity, int64_t launch,
+                                                       int32_t do_m, int32_t do_theta )
+{
+#pragma clang fp contract( off )
+    WaveCtx w;
+    if ( !waveSetup( specs, tile_spec, tile_slice, 0, parity, launch, w ) )
+        return;
+    const PDArgs& A = *w.A;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t n = A.n;
+    const int64_t p = w.slice * kSlice + lane;
+    const bool valid = p < n;
+    if ( !valid )
+        return; // no cross-lane operations below
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ u = A.U[w.cur];
+    const double xi0 = x[p], xi1 = x[n + p], xi2 = x[2 * n + p];
+    const double ui0 = u[p], ui1 = u[n + p], ui2 = u[2 * n + p];
+    const int32_t mycnt = A.cnt[p];
+    const int64_t base = A.slice_ptr[w.slice];
+    const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
+    const uint64_t* __restrict__ bits = A.intact + base;
+    const double* __restrict__ vol = A.vol;
+    const int infl1 = A.influence;
+
+    double m_i;
+    if ( do_m )
+    {
+        m_i = 0.0;
+        for ( int32_t q = 0; q < mycnt; q++ )
+        {
+            if constexpr ( FRAC )
+                if ( !( ( bits[q] >> lane ) & 1ull ) )
+                    continue;
+            const int64_t j = colp[q * kSlice];
+            if constexpr ( PREC == kF64 )
+            {
+                Bond64 b;
+                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
+                               u[2 * n + j], b );
+                const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                m_i += infl * b.xi * b.xi * vol[j];
+            }
+            else
+            {
+                const float gx = static_cast<float>( x[j] - xi0 );
+                const float gy = static_cast<float>( x[n + j] - xi1 );
+                const float gz = static_cast<float>( x[2 * n + j] - xi2 );
+                const float xi2f = gx * gx + gy * gy + gz * gz;
+                const float wv = infl1 ? ( xi2f * __builtin_amdgcn_rsqf( xi2f ) ) : xi2f;
+                m_i += static_cast<double>( wv * static_cast<float>( vol[j] ) );
+            }
+        }
+        A.m[p] = m_i;
+    }
+    else
+        m_i = A.m[p];
+    if ( !do_theta )
+        return;
+    double th = 0.0;
+    const bool mpos = FRAC ? ( m_i > 0 ) : true;
+    if ( mpos )
+    {
+        for ( int32_t q = 0; q < mycnt; q++ )
+        {
+            if constexpr ( FRAC )
+                if ( !( ( bits[q] >> lane ) & 1ull ) )
+                    continue;
+            const int64_t j = colp[q * kSlice];
+            if constexpr ( PREC == kF64 )
+            {
+                Bond64 b;
+                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
+                               u[2 * n + j], b );
+                const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                const double theta_b = infl * b.s * b.xi * b.xi * vol[j];
+                th += 3.0 * theta_b / m_i;
+            }
+            else
+            {
+                Bond32 b;
+                getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
+                               u[2 * n + j], b );
+                const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
+                th += static_cast<double>( infl * b.s * b.xi * b.xi * static_cast<float>( vol[j] ) );
+            }
+        }
+        if constexpr ( PREC == kMixed )
+            if ( mycnt > 0 )
+                th = 3.0 * th / m_i;
+    }
+    A.theta[p] = th;
+    // ForceCoeffTag per-particle subexpressions (force_models/CabanaPD_LPS.hpp:131-132)
+    A.lpsA[p] = A.theta_coeff * th / m_i;
+    A.lpsB[p] = A.s_coeff / m_i;
+}
+
+// ---------------------------------------------------------------------------
+// Energy / damage: Force<*>::computeEnergyFull (force/CabanaPD_PMB.hpp:127-155,
+// 289-339; force/CabanaPD_LPS.hpp:183-219, 449-504).  FP64 for every form.
+// Per-particle W and phi; per-block partial sums (Phi, D) reduced on the host
+// in block order, so totals are reproducible run to run.
+// ---------------------------------------------------------------------------
+template <int MODEL, bool FRAC>
+__global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ specs, int32_t parity,
+                                                     const double* __restrict__ mvals,
+                                                     const double* __restrict__ thetavals,
+                                                     double* __restrict__ partial )
+{
+#pragma clang fp contract( off )
+    __shared__ double red[2][kBlock / kSlice];
+    const PDArgs& A = *specs;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int wave = threadIdx.x / kSlice;
+    const int64_t slice = blockIdx.x * (int64_t)kSlicesPerBlock + wave;
+    const int64_t n = A.n;
+    const int64_t p = slice * kSlice + lane;
+    double Phi = 0.0, D = 0.0;
+    if ( slice < A.nslices && p < n )
+    {
+        const double* __restrict__ x = A.x;
+        const double* __restrict__ u = A.U[parity];
+        const double xi0 = x[p], xi1 = x[n + p], xi2 = x[2 * n + p];
+        const double ui0 = u[p], ui1 = u[n + p], ui2 = u[2 * n + p];
+        const int32_t mycnt = A.cnt[p];
+        const int64_t base = A.slice_ptr[slice];
+        const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
+        const uint64_t* __restrict__ bits = A.intact + base;
+        const double* __restrict__ vol = A.vol;
+        const double c = A.c;
+        double num_bonds = 0.0;
+        double m_i = 0.0, theta_i = 0.0;
+        if constexpr ( MODEL == kLPS )
+        {
+            m_i = mvals[p];
+            theta_i = thetavals[p];
+            if constexpr ( FRAC )
+            {
+                for ( int32_t q = 0; q < mycnt; q++ )
+                    num_bonds += static_cast<double>( ( bits[q] >> lane ) & 1ull );
+            }
+            else
+                num_bonds = static_cast<double>( mycnt );
+        }
+        double W = 0.0, phi_i = 0.0, vol_H_i = 0.0;
+        const double vol_i = vol[p];
+        for ( int32_t q = 0; q < mycnt; q++ )
+        {
+            const int64_t j = colp[q * kSlice];
+            Bond64 b;
+            getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
+                           u[2 * n + j], b );
+            double wb;
+            if constexpr ( MODEL == kLPS )
+            {
+                const double infl = A.influence ? 1.0 / b.xi : 1.0;
+                wb = 1.0 / num_bonds * 0.5 * A.theta_coeff / 3.0 * ( theta_i * theta_i ) +
+                     0.5 * ( A.s_coeff / m_i ) * infl * b.s * b.s * b.xi * b.xi * vol[j];
+            }
+            else
+                wb = 0.25 * c * b.s * b.s * b.xi * vol[j];
+            if constexpr ( FRAC )
+            {
+                const double mu = static_cast<double>( ( bits[q] >> lane ) & 1ull );
+                wb = mu * wb;
+                W += wb;
+                phi_i += mu * vol[j];
+                vol_H_i += vol[j];
+            }
+            else
+            {
+                W += wb;
+                Phi += wb * vol_i;
+            }
+        }
+        A.W[p] = W;
+        if constexpr ( FRAC )
+        {
+            Phi = W * vol_i;
+            const double ph = 1 - phi_i / vol_H_i;
+            A.phi[p] = ph;
+            D = ph;
+        }
+    }
+    for ( int off = 32; off > 0; off >>= 1 )
+    {
+        Phi += __shfl_xor( Phi, off );
+        D += __shfl_xor( D, off );
+    }
+    if ( lane == 0 )
+    {
+        red[0][wave] = Phi;
+        red[1][wave] = D;
+    }
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        double a = 0.0, d = 0.0;
+        for ( int k = 0; k < kSlicesPerBlock; k++ )
+        {
+            a += red[0][k];
+            d += red[1][k];
+        }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = d;
+    }
+}
+
+// Velocity-Verlet initial half step on its own (CabanaPD_Integrate.hpp:89-116):
+// the first step of a pdk_step call; later steps run it in the force epilogue.
+__global__ __launch_bounds__( 256 ) void kInitialHalfStep( const PDArgs* __restrict__ specs,
+                                                           int32_t parity )
+{
+#pragma clang fp contract( off )
+    const PDArgs& A = *specs;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t n = A.n;
+    if ( p >= n )
+        return;
+    const double half_dt_m = A.half_dt / A.rho[p];
+    const double* u = A.U[parity];
+    double* un = A.U[parity ^ 1];
+    for ( int d = 0; d < 3; d++ )
+    {
+        double vv = A.v[d * n + p];
+        vv = vv + half_dt_m * A.f[d * n + p];
+        A.v[d * n + p] = vv;
+        un[d * n + p] = u[d * n + p] + A.dt * vv;
+    }
+}
+
+// Force BC / body term on the stored f (init at t = 0, CabanaPD_Solver.hpp:272-274).
+__global__ __launch_bounds__( 256 ) void kApplyForceTerms( const PDArgs* __restrict__ specs )
+{
+#pragma clang fp contract( off )
+    const PDArgs& A = *specs;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t n = A.n;
+    if ( p >= n )
+        return;
+    for ( int d = 0; d < 3; d++ )
+    {
+        double fv = A.f[d * n + p];
+        if ( A.body )
+            fv = fv + A.body[d * n + p];
+        if ( A.bc_kind >= 0 && A.bcflag[p] )
+            fv = ( A.bc_kind == PDK_BC_FORCE_VALUE ) ? A.bc_value : fv + A.bc_value;
+        A.f[d * n + p] = fv;
+    }
+}
+
+using ForceFn = void ( * )( const PDArgs*, const int32_t*, const int64_t*, int32_t, int32_t, int64_t, int32_t );
+
+ForceFn pickForce( const PDArgs& a )
+{
+    const int idx = a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 );
+    switch ( idx )
+    {
+    case 0: return kForce<kPMB, false, kF64>;
+    case 1: return kForce<kPMB, false, kMixed>;
+    case 2: return kForce<kPMB, true, kF64>;
+    case 3: return kForce<kPMB, true, kMixed>;
+    case 4: return kForce<kLPS, false, kF64>;
+    case 5: return kForce<kLPS, false, kMixed>;
+    case 6: return kForce<kLPS, true, kF64>;
+    default: return kForce<kLPS, true, kMixed>;
+    }
+}
+
+const char* kForceNames[8] = {
+    "kForce<0, false, 0>", "kForce<0, false, 1>", "kForce<0, true, 0>", "kForce<0, true, 1>",
+    "kForce<1, false, 0>", "kForce<1, false, 1>", "kForce<1, true, 0>", "kForce<1, true, 1>" };
+
+} // namespace
+
+const char* forceKernelName( const PDArgs& a )
+{
+    return kForceNames[a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 )];
+}
+const char* lpsKernelName( const PDArgs& a )
+{
+    return a.fracture ? ( a.precision == PDK_MIXED ? "kLpsState<true, 1>" : "kLpsState<true, 0>" )
+                      : ( a.precision == PDK_MIXED ? "kLpsState<false, 1>" : "kLpsState<false, 0>" );
+}
+const char* energyKernelName( const PDArgs& a )
+{
+    return a.model == kLPS ? ( a.fracture ? "kEnergy<1, true>" : "kEnergy<1, false>" )
+                           : ( a.fracture ? "kEnergy<0, true>" : "kEnergy<0, false>" );
+}
+
+static unsigned sliceBlocks( const PDArgs& a )
+{
+    return static_cast<unsigned>( ( a.nslices + kSlicesPerBlock - 1 ) / kSlicesPerBlock );
+}
+
+void launchForce( const PDArgs* args_dev, const PDArgs& a, int32_t epilogue_mode, int32_t parity,
+                  hipStream_t st )
+{
+    if ( a.nslices == 0 )
+        return;
+    const int32_t accumulate = epilogue_mode < 0 ? 1 : 0;
+    const int32_t epi = epilogue_mode < 0 ? kEpiStoreOnly : epilogue_mode;
+    ForceFn fn = pickForce( a );
+    hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, nullptr, nullptr, epi,
+                        parity, int64_t( 0 ), accumulate );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchLpsState( const PDArgs* args_dev, const PDArgs& a, bool weighted_volume, bool dilatation,
+                     int32_t parity, hipStream_t st )
+{
+    if ( a.nslices == 0 )
+        return;
+    auto fn = a.fracture ? ( a.precision == PDK_MIXED ? kLpsState<true, kMixed> : kLpsState<true, kF64> )
+                         : ( a.precision == PDK_MIXED ? kLpsState<false, kMixed> : kLpsState<false, kF64> );
+    hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, nullptr, nullptr, parity,
+                        int64_t( 0 ), int32_t( weighted_volume ), int32_t( dilatation ) );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchForceBatched( const PDArgs* specs_dev, const PDArgs& variant, const int32_t* tile_spec,
+                         const int64_t* tile_slice, int64_t ntiles, int64_t launch_index, hipStream_t st )
+{
+    if ( ntiles == 0 )
+        return;
+    ForceFn fn = pickForce( variant );
+    hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( ntiles ) ), dim3( kBlock ), 0, st, specs_dev, tile_spec,
+                        tile_slice, int32_t( kEpiFinalInitial ), static_cast<int32_t>( launch_index & 1 ),
+                        launch_index, int32_t( 0 ) );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchLpsStateBatched( const PDArgs* specs_dev, const PDArgs& variant, const int32_t* tile_spec,
+                            const int64_t* tile_slice, int64_t ntiles, int64_t launch_index, hipStream_t st )
+{
+    if ( ntiles == 0 )
+        return;
+    auto fn = variant.fracture
+                  ? ( variant.precision == PDK_MIXED ? kLpsState<true, kMixed> : kLpsState<true, kF64> )
+                  : ( variant.precision == PDK_MIXED ? kLpsState<false, kMixed> : kLpsState<false, kF64> );
+    hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( ntiles ) ), dim3( kBlock ), 0, st, specs_dev, tile_spec,
+                        tile_slice, static_cast<int32_t>( launch_index & 1 ), launch_index,
+                        int32_t( variant.fracture ? 1 : 0 ), int32_t( 1 ) );
+    PDK_HIP( hipGetLastError() );
+}
+
+int64_t energyBlocks( const PDArgs& a )
+{
+    return sliceBlocks( a );
+}
+
+void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, double* partial_dev,
+                   int64_t nblocks, hipStream_t st )
+{
+    if ( a.nslices == 0 )
+        return;
+    auto fn = a.model == kLPS ? ( a.fracture ? kEnergy<kLPS, true> : kEnergy<kLPS, false> )
+                              : ( a.fracture ? kEnergy<kPMB, true> : kEnergy<kPMB, false> );
+    hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( nblocks ) ), dim3( kBlock ), 0, st, args_dev, parity,
+                        a.m, a.theta, partial_dev );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st )
+{
+    if ( a.n == 0 )
+        return;
+    hipLaunchKernelGGL( kInitialHalfStep, dim3( static_cast<unsigned>( ( a.n + 255 ) / 256 ) ), dim3( 256 ), 0, st,
+                        args_dev, parity );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchApplyForceTerms( const PDArgs* args_dev, const PDArgs& a, hipStream_t st )
+{
+    if ( a.n == 0 )
+        return;
+    hipLaunchKernelGGL( kApplyForceTerms, dim3( static_cast<unsigned>( ( a.n + 255 ) / 256 ) ), dim3( 256 ), 0,
+                        st, args_dev );
+    PDK_HIP( hipGetLastError() );
+}
+
+} // namespace pdk

@@ -1,0 +1,176 @@
+// pdk_internal.hpp -- host-side internals shared by the engine's translation
+// units (device buffers, error reporting, kernel launch wrappers).
+#ifndef PDK_INTERNAL_HPP
+#define PDK_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pdk_common.hpp"
+
+namespace pdk
+{
+// Thrown inside the library, turned into a status code at the C boundary.
+struct Error : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+#define PDK_HIP( call )                                                                        \
+    do                                                                                         \
+    {                                                                                          \
+        hipError_t pdk_e_ = ( call );                                                          \
+        if ( pdk_e_ != hipSuccess )                                                            \
+            throw ::pdk::Error( std::string( #call " failed: " ) + hipGetErrorString( pdk_e_ ) ); \
+    } while ( 0 )
+
+// Accounting hook: every device allocation of a spec goes through its pool so
+// the context can enforce the HBM budget.
+struct MemAccount
+{
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+};
+
+// Owning device buffer.
+class DevBuf
+{
+  public:
+    DevBuf() = default;
+    DevBuf( const DevBuf& ) = delete;
+    DevBuf& operator=( const DevBuf& ) = delete;
+    DevBuf( DevBuf&& o ) noexcept { swap( o ); }
+    DevBuf& operator=( DevBuf&& o ) noexcept
+    {
+        release();
+        swap( o );
+        return *this;
+    }
+    ~DevBuf() { release(); }
+
+    void alloc( size_t bytes, MemAccount* acct )
+    {
+        release();
+        if ( bytes == 0 )
+            return;
+        if ( acct && acct->budget && acct->in_use + bytes > acct->budget )
+            throw Error( "HBM budget exceeded: need " + std::to_string( bytes ) + " B more, " +
+                         std::to_string( acct->budget - acct->in_use ) + " B left" );
+        PDK_HIP( hipMalloc( &ptr_, bytes ) );
+        bytes_ = bytes;
+        acct_ = acct;
+        if ( acct_ )
+            acct_->in_use += bytes_;
+    }
+    void release()
+    {
+        if ( ptr_ )
+        {
+            (void)hipFree( ptr_ );
+            if ( acct_ )
+                acct_->in_use -= bytes_;
+        }
+        ptr_ = nullptr;
+        bytes_ = 0;
+        acct_ = nullptr;
+    }
+    template <class T>
+    T* as() const
+    {
+        return static_cast<T*>( ptr_ );
+    }
+    size_t bytes() const { return bytes_; }
+    explicit operator bool() const { return ptr_ != nullptr; }
+
+  private:
+    void swap( DevBuf& o )
+    {
+        std::swap( ptr_, o.ptr_ );
+        std::swap( bytes_, o.bytes_ );
+        std::swap( acct_, o.acct_ );
+    }
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+    MemAccount* acct_ = nullptr;
+};
+
+// ---- bond build (pdk_build.hip) ---------------------------------------------
+struct BuildScratch
+{
+    DevBuf keys, keys2, vals, vals2, tmp, cell, cell_start, cell_fill, cell_list, slice_len, bbox,
+        flag;
+};
+
+// Morton order of the caller's positions: perm[p] = caller id of sorted p.
+void launchMortonOrder( const double* x_aos_dev, int64_t n, const double lo[3], double inv_h,
+                        int64_t* perm_dev, int64_t* inv_dev, BuildScratch& s, MemAccount* acct,
+                        hipStream_t st );
+void launchGatherAos3( const double* src_aos, const int64_t* perm, double* dst_soa, int64_t n,
+                       hipStream_t st );
+void launchScatterAos3( const double* src_soa, const int64_t* perm, double* dst_aos, int64_t n,
+                        hipStream_t st );
+void launchGatherScalar( const double* src, const int64_t* perm, double* dst, int64_t n,
+                         hipStream_t st );
+void launchScatterScalar( const double* src, const int64_t* perm, double* dst, int64_t n,
+                          hipStream_t st );
+void launchGatherFlags( const int32_t* src, const int64_t* perm, uint8_t* dst, int64_t n,
+                        hipStream_t st );
+
+struct NeighborQuery
+{
+    const double* x;      // sorted SoA positions
+    const double* u;      // if non-null, positions are x + u
+    const double* radius; // polydisperse contact: pair cutoff R_i + R_j + extend
+    double rc;            // cutoff (uniform)
+    double extend;        // polydisperse: added to R_i + R_j
+    int64_t n;
+};
+
+// Builds the sliced CSR; (re)allocates col/intact when capacity is short.
+// Returns total slot rows.
+int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64_t* inv,
+                        int64_t* slice_ptr, int32_t* cnt, DevBuf& col, DevBuf& intact,
+                        int64_t& col_capacity_rows, BuildScratch& s, MemAccount* acct,
+                        hipStream_t st, int32_t* max_cnt_out );
+
+// Prenotch<N>::create over the sliced CSR.
+void launchPrenotch( const PDArgs& a, int32_t nnotch, const double* notch_dev, hipStream_t st );
+
+// ---- PD passes (pdk_force.hip) -----------------------------------------------
+void launchForce( const PDArgs* args_dev, const PDArgs& a, int32_t epilogue, int32_t parity,
+                  hipStream_t st );
+void launchLpsState( const PDArgs* args_dev, const PDArgs& a, bool weighted_volume, bool dilatation,
+                     int32_t parity, hipStream_t st );
+void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, double* partial_dev,
+                   int64_t nblocks, hipStream_t st );
+int64_t energyBlocks( const PDArgs& a );
+void launchInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
+void launchApplyForceTerms( const PDArgs* args_dev, const PDArgs& a, hipStream_t st );
+// Packed launches over several specs of one kernel variant (the variant is
+// taken from `variant`): tile t covers slices [tile_slice[t], +4) of
+// specs[tile_spec[t]]; each spec's epilogue and displacement buffer follow
+// from launch_index, its end_launch and parity_offset.
+void launchForceBatched( const PDArgs* specs_dev, const PDArgs& variant, const int32_t* tile_spec,
+                         const int64_t* tile_slice, int64_t ntiles, int64_t launch_index,
+                         hipStream_t st );
+void launchLpsStateBatched( const PDArgs* specs_dev, const PDArgs& variant, const int32_t* tile_spec,
+                            const int64_t* tile_slice, int64_t ntiles, int64_t launch_index,
+                            hipStream_t st );
+const char* forceKernelName( const PDArgs& a );
+const char* lpsKernelName( const PDArgs& a );
+const char* energyKernelName( const PDArgs& a );
+
+// ---- contact (pdk_contact.hip) -----------------------------------------------
+void launchContactForce( const PDArgs* args_dev, const PDArgs& a, int32_t epilogue, int32_t parity,
+                         hipStream_t st );
+void launchContactInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t parity,
+                                   double* partial_dev, int64_t nblocks, hipStream_t st );
+const char* contactKernelName( const PDArgs& a );
+
+} // namespace pdk
+
+#endif

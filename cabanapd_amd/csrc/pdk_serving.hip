@@ -1,0 +1,302 @@
+// pdk_serving.hip -- continuous batching of specimen solves on one GPU.
+//
+// Nothing in the reference corresponds to this: a CabanaPD run is one
+// specimen per process.  Here every running specimen advances one time step
+// per packed launch; all specimens of one kernel variant (model, fracture,
+// precision) share a launch whose tiles cover their concatenated slice
+// ranges.  A specimen's epilogue (final half step only on its last step) and
+// its displacement buffer follow from the launch index, so admission and
+// retirement only rewrite the small tile tables, never the specimen state.
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <set>
+#include <tuple>
+#include <vector>
+
+#include "pdk_serving.hpp"
+
+namespace pdk
+{
+struct BatchEngine::Impl
+{
+    struct Req
+    {
+        uint64_t ticket;
+        pdk_spec_s* s;
+        int32_t nsteps;
+    };
+    struct Run
+    {
+        uint64_t ticket;
+        pdk_spec_s* s;
+        int64_t end_launch;
+        int32_t parity_offset;
+        int32_t nsteps;
+    };
+    struct Group
+    {
+        PDArgs variant{};
+        std::vector<PDArgs> h_specs;
+        std::vector<int32_t> h_tspec;
+        std::vector<int64_t> h_tslice;
+        DevBuf specs, tspec, tslice;
+        int64_t ntiles = 0;
+        bool lps = false;
+    };
+
+    pdk_ctx_s* ctx;
+    hipStream_t st = nullptr;
+    int64_t L = 0;
+    uint64_t next_ticket = 1;
+    size_t max_slots = 512;
+    std::deque<Req> pending;
+    std::vector<Run> running;
+    std::map<uint64_t, hipEvent_t> done_events;
+    std::set<uint64_t> finished;
+    std::deque<hipEvent_t> inflight;
+    std::vector<Group> groups;
+    bool dirty = true;
+
+    explicit Impl( pdk_ctx_s* c ) : ctx( c )
+    {
+        PDK_HIP( hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) );
+    }
+    ~Impl()
+    {
+        (void)hipStreamSynchronize( st );
+        for ( auto& e : done_events )
+            (void)hipEventDestroy( e.second );
+        for ( auto e : inflight )
+            (void)hipEventDestroy( e );
+        (void)hipStreamDestroy( st );
+    }
+
+    void admit()
+    {
+        while ( !pending.empty() && running.size() < max_slots )
+        {
+            Req r = pending.front();
+            pending.pop_front();
+            SpecView v = view( r.s );
+            // The spec's own stream may still be finishing its setup.
+            hipEvent_t ready;
+            PDK_HIP( hipEventCreateWithFlags( &ready, hipEventDisableTiming ) );
+            PDK_HIP( hipEventRecord( ready, v.st ) );
+            PDK_HIP( hipStreamWaitEvent( st, ready, 0 ) );
+            PDK_HIP( hipEventDestroy( ready ) );
+            // Initial half step of the first step (CabanaPD_Integrate.hpp:89-116).
+            launchInitialHalfStep( v.dargs, *v.h, *v.cur, st );
+            *v.cur ^= 1;
+            Run run;
+            run.ticket = r.ticket;
+            run.s = r.s;
+            run.nsteps = r.nsteps;
+            run.end_launch = L + r.nsteps;
+            run.parity_offset = *v.cur ^ static_cast<int32_t>( L & 1 );
+            running.push_back( run );
+            dirty = true;
+        }
+    }
+
+    void rebuildGroups()
+    {
+        std::map<std::tuple<int, int, int>, size_t> key;
+        std::vector<Group> ng;
+        for ( const Run& r : running )
+        {
+            SpecView v = view( r.s );
+            auto k = std::make_tuple( v.model, v.fracture, v.precision );
+            auto it = key.find( k );
+            size_t gi;
+            if ( it == key.end() )
+            {
+                gi = ng.size();
+                key[k] = gi;
+                ng.emplace_back();
+                ng[gi].variant = *v.h;
+                ng[gi].lps = v.model == PDK_MODEL_LPS;
+            }
+            else
+                gi = it->second;
+            Group& g = ng[gi];
+            PDArgs a = *v.h;
+            a.parity_offset = r.parity_offset;
+            a.end_launch = r.end_launch;
+            const int32_t idx = static_cast<int32_t>( g.h_specs.size() );
+            g.h_specs.push_back( a );
+            for ( int64_t s0 = 0; s0 < v.nslices; s0 += kSlicesPerBlock )
+            {
+                g.h_tspec.push_back( idx );
+                g.h_tslice.push_back( s0 );
+            }
+        }
+        // Keep the old device tables alive until the stream has consumed them.
+        PDK_HIP( hipStreamSynchronize( st ) );
+        for ( Group& g : ng )
+        {
+            g.ntiles = static_cast<int64_t>( g.h_tspec.size() );
+            g.specs.alloc( g.h_specs.size() * sizeof( PDArgs ), &ctx_acct() );
+            g.tspec.alloc( g.h_tspec.size() * sizeof( int32_t ), &ctx_acct() );
+            g.tslice.alloc( g.h_tslice.size() * sizeof( int64_t ), &ctx_acct() );
+            PDK_HIP( hipMemcpyAsync( g.specs.as<PDArgs>(), g.h_specs.data(), g.h_specs.size() * sizeof( PDArgs ),
+                                     hipMemcpyHostToDevice, st ) );
+            PDK_HIP( hipMemcpyAsync( g.tspec.as<int32_t>(), g.h_tspec.data(), g.h_tspec.size() * sizeof( int32_t ),
+                                     hipMemcpyHostToDevice, st ) );
+            PDK_HIP( hipMemcpyAsync( g.tslice.as<int64_t>(), g.h_tslice.data(),
+                                     g.h_tslice.size() * sizeof( int64_t ), hipMemcpyHostToDevice, st ) );
+        }
+        groups.swap( ng );
+        dirty = false;
+    }
+
+    MemAccount& ctx_acct();
+
+    void retire()
+    {
+        std::vector<Run> keep;
+        hipEvent_t ev = nullptr;
+        for ( const Run& r : running )
+        {
+            if ( r.end_launch != L )
+            {
+                keep.push_back( r );
+                continue;
+            }
+            if ( !ev )
+            {
+                PDK_HIP( hipEventCreateWithFlags( &ev, hipEventDisableTiming ) );
+                PDK_HIP( hipEventRecord( ev, st ) );
+            }
+            SpecView v = view( r.s );
+            // The last launch (index end_launch - 1) did not flip the buffer.
+            *v.cur = static_cast<int32_t>( ( ( r.end_launch - 1 ) & 1 ) ) ^ r.parity_offset;
+            *v.steps_done += r.nsteps;
+            *v.in_batch = false;
+            PDK_HIP( hipStreamWaitEvent( v.st, ev, 0 ) );
+            hipEvent_t mine;
+            PDK_HIP( hipEventCreateWithFlags( &mine, hipEventDisableTiming ) );
+            PDK_HIP( hipEventRecord( mine, st ) );
+            done_events[r.ticket] = mine;
+            dirty = true;
+        }
+        if ( ev )
+            PDK_HIP( hipEventDestroy( ev ) );
+        running.swap( keep );
+    }
+
+    int32_t progress( int32_t max_steps )
+    {
+        int32_t launched = 0;
+        while ( max_steps <= 0 || launched < max_steps )
+        {
+            admit();
+            if ( running.empty() )
+                break;
+            if ( dirty )
+                rebuildGroups();
+            for ( Group& g : groups )
+            {
+                if ( g.lps )
+                    launchLpsStateBatched( g.specs.as<PDArgs>(), g.variant, g.tspec.as<int32_t>(),
+                                           g.tslice.as<int64_t>(), g.ntiles, L, st );
+                launchForceBatched( g.specs.as<PDArgs>(), g.variant, g.tspec.as<int32_t>(), g.tslice.as<int64_t>(),
+                                    g.ntiles, L, st );
+            }
+            L++;
+            retire();
+            launched++;
+            hipEvent_t e;
+            PDK_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
+            PDK_HIP( hipEventRecord( e, st ) );
+            inflight.push_back( e );
+            while ( inflight.size() > 4 )
+            {
+                PDK_HIP( hipEventSynchronize( inflight.front() ) );
+                PDK_HIP( hipEventDestroy( inflight.front() ) );
+                inflight.pop_front();
+            }
+        }
+        return launched;
+    }
+
+    bool done( uint64_t t )
+    {
+        if ( finished.count( t ) )
+            return true;
+        auto it = done_events.find( t );
+        if ( it == done_events.end() )
+            return false;
+        hipError_t q = hipEventQuery( it->second );
+        if ( q == hipSuccess )
+        {
+            (void)hipEventDestroy( it->second );
+            done_events.erase( it );
+            finished.insert( t );
+            return true;
+        }
+        if ( q != hipErrorNotReady )
+            throw Error( std::string( "batch engine: " ) + hipGetErrorString( q ) );
+        return false;
+    }
+
+    void forget( pdk_spec_s* s )
+    {
+        PDK_HIP( hipStreamSynchronize( st ) );
+        pending.erase( std::remove_if( pending.begin(), pending.end(), [&]( const Req& r ) { return r.s == s; } ),
+                       pending.end() );
+        size_t before = running.size();
+        running.erase( std::remove_if( running.begin(), running.end(), [&]( const Run& r ) { return r.s == s; } ),
+                       running.end() );
+        if ( running.size() != before )
+            dirty = true;
+    }
+};
+
+} // namespace pdk
+
+// The context owns the accounting record; reach it through the ctx pointer.
+namespace pdk
+{
+MemAccount& ctxAccount( pdk_ctx_s* c );
+MemAccount& BatchEngine::Impl::ctx_acct()
+{
+    return ctxAccount( ctx );
+}
+
+BatchEngine::BatchEngine( pdk_ctx_s* ctx ) : impl_( std::make_unique<Impl>( ctx ) ) {}
+BatchEngine::~BatchEngine() = default;
+
+uint64_t BatchEngine::submit( pdk_spec_s* s, int32_t nsteps )
+{
+    SpecView v = view( s );
+    if ( !v.inited )
+        throw Error( "call pdk_init before pdk_submit" );
+    if ( *v.in_batch )
+        throw Error( "spec already submitted" );
+    if ( v.model != PDK_MODEL_PMB && v.model != PDK_MODEL_LPS )
+        throw Error( "the batch engine runs PD specimens only" );
+    if ( nsteps <= 0 )
+        throw Error( "nsteps must be positive" );
+    *v.in_batch = true;
+    const uint64_t t = impl_->next_ticket++;
+    impl_->pending.push_back( { t, s, nsteps } );
+    return t;
+}
+
+bool BatchEngine::done( uint64_t ticket ) const
+{
+    return impl_->done( ticket );
+}
+
+int32_t BatchEngine::progress( int32_t max_steps )
+{
+    return impl_->progress( max_steps );
+}
+
+void BatchEngine::forget( pdk_spec_s* s )
+{
+    impl_->forget( s );
+}
+
+} // namespace pdk

@@ -1,0 +1,213 @@
+/*
+ * pdk.h -- C ABI of the MI355X-native peridynamics force/update engine.
+ *
+ * This is the drop-in boundary for CabanaPD's force/update hot path.  The
+ * reference exposes that path only as C++ templates instantiated in user
+ * translation units (no FFI).  The C++ drop-in headers under
+ * include/CabanaPD/ keep the reference's class and function names and call
+ * these entry points; the test suite calls them through ctypes.
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 = ok, nonzero = error; the message is
+ *     available from pdk_last_error() (thread-local).  No C++ exception
+ *     crosses this boundary (the reference throws std::runtime_error on the
+ *     host, e.g. CabanaPD_Input.hpp:64-106; here the drop-in headers turn a
+ *     nonzero status back into std::runtime_error).
+ *   - Host arrays are borrowed for the duration of the call only.  Vector
+ *     fields are AoS double[n][3] in the caller's particle order, exactly the
+ *     reference's slice layout (AoSoA vector length 1, CabanaPD_Particles.hpp:105-126).
+ *   - Device memory is owned by the opaque handles.  The engine keeps
+ *     particles in Morton order internally and translates on upload/download.
+ *   - Calls on one spec are synchronous with respect to the host unless the
+ *     name says otherwise (pdk_submit / pdk_poll), matching the reference's
+ *     Kokkos::fence() after every kernel (e.g. force/CabanaPD_PMB.hpp:121-123).
+ */
+#ifndef PDK_H
+#define PDK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- model selection (reference tags: CabanaPD_Types.hpp) --------------- */
+enum pdk_model_kind {
+    PDK_MODEL_PMB = 0,        /* ForceModel<PMB, Elastic, NoFracture|Fracture>  force_models/CabanaPD_PMB.hpp:149-231 */
+    PDK_MODEL_LPS = 1,        /* ForceModel<LPS, Elastic, NoFracture|Fracture>  force_models/CabanaPD_LPS.hpp:151-253 */
+    PDK_MODEL_HERTZ = 2,      /* HertzianModel     force_models/CabanaPD_Hertzian.hpp:23-88 */
+    PDK_MODEL_HERTZ_JKR = 3   /* HertzianJKRModel  force_models/CabanaPD_HertzianJKR.hpp:23-177 */
+};
+
+enum pdk_precision {
+    PDK_FP64 = 0,   /* plain FP64: the reference arithmetic, operation for operation */
+    PDK_MIXED = 1   /* FP32 bond math, FP64 accumulation/integration, FP64 guard band for breaks */
+};
+
+enum pdk_field {
+    PDK_FIELD_X = 0,      /* reference position  double[n][3] */
+    PDK_FIELD_U = 1,      /* displacement        double[n][3] */
+    PDK_FIELD_V = 2,      /* velocity            double[n][3] */
+    PDK_FIELD_F = 3,      /* force density       double[n][3] */
+    PDK_FIELD_VOL = 4,    /* volume              double[n] */
+    PDK_FIELD_RHO = 5,    /* density             double[n] */
+    PDK_FIELD_W = 6,      /* strain energy density double[n] */
+    PDK_FIELD_PHI = 7,    /* damage              double[n] */
+    PDK_FIELD_M = 8,      /* LPS weighted volume double[n] */
+    PDK_FIELD_THETA = 9,  /* LPS dilatation      double[n] */
+    PDK_FIELD_COUNT = 10
+};
+
+enum pdk_bc_kind {
+    PDK_BC_FORCE_VALUE = 0,   /* f(pid,d) = value   CabanaPD_Boundary.hpp:77-119 */
+    PDK_BC_FORCE_UPDATE = 1   /* f(pid,d) += value  CabanaPD_Boundary.hpp:121-163 */
+};
+
+/* Model/specimen description.  Derived constants (c, s0, bond_break_coeff,
+ * theta/s coefficients, Hertz/JKR constants) are computed by the engine with
+ * the reference's expressions (see pdk_model_constants). */
+typedef struct pdk_model_desc {
+    int32_t model;          /* pdk_model_kind */
+    int32_t fracture;       /* 0 = NoFracture, 1 = Fracture */
+    int32_t influence;      /* 0 or 1: BaseFractureModel s0 selector / LPS influence (1 => 1/|xi|) */
+    int32_t precision;      /* pdk_precision */
+    double horizon;         /* force horizon delta; PD neighbor cutoff = horizon + neighbor_tol */
+    double K;               /* bulk modulus */
+    double G;               /* shear modulus (LPS) */
+    double G0;              /* fracture energy (Fracture) */
+    double dt;              /* time step (VelocityVerlet, CabanaPD_Integrate.hpp:82-86) */
+    double neighbor_tol;    /* Neighbor ctor tolerance, reference default 1e-14 (CabanaPD_Neighbor.hpp:46) */
+    /* contact (DEM) */
+    double radius;          /* ContactModel::radius (uniform) */
+    double radius_extend;   /* ContactModel::radius_extend (list skin) */
+    double nu;              /* Poisson's ratio */
+    double E;               /* Young's modulus */
+    double restitution;     /* e */
+    double gamma;           /* JKR surface energy */
+    double mu_friction;     /* JKR Coulomb friction */
+    int32_t rebuild_every;  /* 0: rebuild when max|u-u_build| > extend (CabanaPD_Neighbor.hpp:116-119); k>0: every k steps (plus the same safety trigger) */
+    int32_t polydisperse;   /* 1: per-particle radii (extension; equal radii reduce to the reference) */
+} pdk_model_desc;
+
+/* Derived constants, for inspection and tests. */
+typedef struct pdk_model_constants {
+    double c, s0, bond_break_coeff, theta_coeff, s_coeff;
+    double Rs, Es, Gs, beta, coeff_h_n, coeff_h_d, Gamma, a0, delta_tear, fc;
+} pdk_model_constants;
+
+typedef struct pdk_ctx_s* pdk_ctx;
+typedef struct pdk_spec_s* pdk_spec;
+
+/* ---- errors --------------------------------------------------------------- */
+const char* pdk_last_error(void);
+
+/* ---- derived constants ----------------------------------------------------
+ * PMB: c = 18K/(pi delta^4)           force_models/CabanaPD_PMB.hpp:41-47
+ * s0, bond_break_coeff                CabanaPD_ForceModels.hpp:93-102, LPS override force_models/CabanaPD_LPS.hpp:209-223
+ * LPS theta/s coefficients            force_models/CabanaPD_LPS.hpp:73-83
+ * Hertz / JKR constants               force_models/CabanaPD_Hertzian.hpp:41-56, force_models/CabanaPD_HertzianJKR.hpp:51-80 */
+int pdk_model_constants_get(const pdk_model_desc* desc, pdk_model_constants* out);
+
+/* Inputs::computeCriticalTimeStep lattice sum (CabanaPD_Input.hpp:217-336),
+ * returns safety * dt_crit. */
+int pdk_critical_timestep(double dx, double dy, double dz, double K, double rho,
+                          double horizon, double safety_factor, double* dt_out);
+
+/* ---- context ---------------------------------------------------------------
+ * One context per GPU.  hbm_budget (bytes, 0 = 90% of free) bounds what specs
+ * and the serving admission may allocate. */
+int pdk_ctx_create(int device, uint64_t hbm_budget, pdk_ctx* ctx);
+int pdk_ctx_destroy(pdk_ctx ctx);
+int pdk_ctx_bytes_in_use(pdk_ctx ctx, uint64_t* bytes);
+
+/* ---- specimen lifetime -----------------------------------------------------
+ * pdk_spec_create + pdk_spec_upload replace Particles::create(x, vol)
+ * (CabanaPD_Particles.hpp:310-368) followed by Particles::update with the
+ * user's init functor (:429-443).  nofail and radius may be NULL. */
+int pdk_spec_create(pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spec* spec);
+int pdk_spec_destroy(pdk_spec spec);
+int pdk_spec_upload(pdk_spec spec, const double* x, const double* u, const double* v,
+                    const double* vol, const double* rho, const int32_t* nofail,
+                    const double* radius);
+/* Replace u, v (caller order) on an existing spec; f is reset to zero. */
+int pdk_spec_set_state(pdk_spec spec, const double* u, const double* v);
+int pdk_spec_bytes(pdk_spec spec, uint64_t* bytes);
+
+/* ---- bond family -------------------------------------------------------------
+ * Neighbor<NoFracture|Fracture> ctor (CabanaPD_Neighbor.hpp:44-66,199-210):
+ * full list of j != i with |x_j - x_i|^2 <= (horizon + tol)^2 (reference
+ * positions; current positions x+u for contact models).  Built on the device
+ * by cell binning into a Morton-ordered sliced CSR; every bond starts intact. */
+int pdk_build_bonds(pdk_spec spec);
+/* Neighbor statistics (Neighbor::getStatistics, CabanaPD_Neighbor.hpp:82-102). */
+int pdk_bond_stats(pdk_spec spec, int64_t* total_bonds, int32_t* max_neighbors);
+/* Prenotch<N>::create (CabanaPD_Prenotch.hpp:215-263): every bond whose
+ * segment crosses one of the n plane patches (p0 + a v1 + b v2) is broken.
+ * p0, v1, v2 are double[n][3]. */
+int pdk_prenotch(pdk_spec spec, int32_t n, const double* p0, const double* v1, const double* v2);
+/* Bonds in caller order: offsets[n+1], nbr[total] (caller ids, ascending per
+ * row), intact[total] (1 = mu 1, 0 = broken). Any pointer may be NULL. */
+int pdk_download_bonds(pdk_spec spec, int64_t* offsets, int64_t* nbr, uint8_t* intact);
+
+/* ---- boundary conditions / body terms --------------------------------------
+ * createBoundaryCondition(ForceValueBCTag|ForceUpdateBCTag, value, ...)
+ * (CabanaPD_Boundary.hpp:77-176): idx are caller particle ids.  Applied to f
+ * after the force pass, before the final half step (CabanaPD_Solver.hpp:365-367),
+ * and once at init (:272-274).  A second call replaces the first. */
+int pdk_set_bc(pdk_spec spec, int32_t kind, const int64_t* idx, int64_t n, double value);
+/* Constant per-particle body force density b[n][3] (caller order) added to f
+ * every step: BodyTerm with a time-independent functor (CabanaPD_BodyTerm.hpp:46-63),
+ * e.g. gravity f_z -= 9.8 rho, or a face traction.  NULL clears it. */
+int pdk_set_body_force(pdk_spec spec, const double* b);
+
+/* ---- force / energy (free functions, CabanaPD_Force.hpp:176-219) ---------- */
+int pdk_compute_weighted_volume(pdk_spec spec);   /* Force<LPS>::computeWeightedVolume; no-op for PMB */
+int pdk_compute_dilatation(pdk_spec spec);        /* Force<LPS>::computeDilatation; no-op for PMB */
+int pdk_compute_force(pdk_spec spec, int32_t reset);  /* computeForce(model, force, particles, neighbor, reset) */
+int pdk_compute_energy(pdk_spec spec, double* total_strain_energy, double* total_damage);  /* computeEnergy */
+
+/* ---- time stepping (Solver::init / runStep, CabanaPD_Solver.hpp:229-404) ----
+ * pdk_init: Solver::init(bc) order: weighted volume (LPS NoFracture), force,
+ * energy, force BC at t = 0.
+ * pdk_step: nsteps x runStep(step, bc) -- velocity-Verlet fused into the
+ * force pass epilogue; energy/damage are not recomputed (call
+ * pdk_compute_energy at output steps, as Solver::output does). */
+int pdk_init(pdk_spec spec);
+int pdk_step(pdk_spec spec, int32_t nsteps);
+int pdk_steps_done(pdk_spec spec, int64_t* steps);
+
+/* Download a field in caller order (double[n][3] or double[n]). */
+int pdk_download(pdk_spec spec, int32_t field, double* out);
+int pdk_sync(pdk_spec spec);
+
+/* ---- measurement -------------------------------------------------------------
+ * When enabled, every kernel launch of the spec is bracketed by HIP events on
+ * the spec's stream; pdk_kernel_time returns the summed duration (ms) and the
+ * launch count of one kernel class:
+ *   0 = force pass (PD force + fused VV epilogue, or contact force)
+ *   1 = LPS weighted-volume/dilatation pass
+ *   2 = energy/damage pass
+ *   3 = bond build
+ *   4 = other (prologue/VV-only, BC, contact list rebuild) */
+int pdk_kernel_timing(pdk_spec spec, int32_t enable);
+int pdk_kernel_time(pdk_spec spec, int32_t kernel_class, double* total_ms, int64_t* launches);
+/* Name of the device function behind a kernel class for the spec's current
+ * configuration (what rocprofv3 reports). */
+int pdk_kernel_name(pdk_spec spec, int32_t kernel_class, char* buf, int32_t buflen);
+
+/* ---- serving: continuous batching ---------------------------------------------
+ * pdk_submit queues nsteps of a spec on the context's batch engine; the batch
+ * engine packs all running specs into one launch per step over their
+ * concatenated slice ranges.  Requests are admitted as slots and HBM budget
+ * allow; pdk_poll reports completion (*done = 1) without blocking. */
+int pdk_submit(pdk_ctx ctx, pdk_spec spec, int32_t nsteps, uint64_t* ticket);
+int pdk_poll(pdk_ctx ctx, uint64_t ticket, int32_t* done);
+/* Advance the batch engine by up to max_steps packed steps (0 = until idle).
+ * Returns the number of packed steps launched in *launched. */
+int pdk_batch_progress(pdk_ctx ctx, int32_t max_steps, int32_t* launched);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDK_H */

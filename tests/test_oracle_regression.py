@@ -1,0 +1,76 @@
+"""End-to-end pins of the oracle against the reference's regression goldens.
+
+regression_test/tstKalthoffWinkler.hpp:53  max damaged-particle y = 7.42e-2 +- 1.1e-4
+regression_test/tstHertz.hpp:135           sqrt(KE_f / KE_i) = e +- 1e-3
+regression_test/tstHertzJKR.hpp:145        min pull-off force / f_c = -1 +- 5e-3
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cabanapd_amd import configs
+from cabanapd_amd._abi import make_desc
+from oracle import binding as orc
+
+
+def oracle_from(w):
+    s = orc.OracleSpec(w["desc"], w["x"].shape[0], w["x"], w["vol"], u=w["u"], v=w["v"], rho=w["rho"],
+                       nofail=w["nofail"], radius=w["radius"])
+    configs.setup(s, w)
+    return s
+
+
+def test_kalthoff_winkler_regression():
+    w = configs.kw_regression()
+    assert w["steps"] == 700
+    s = oracle_from(w)
+    series = []
+    for k in range(1, w["steps"] + 1):
+        s.step(1)
+        if k % w["output_every"] == 0:
+            s.compute_energy()  # Solver::output computes energy and damage
+            series.append(configs.kw_crack_y(w["x"], s.get("phi")))
+    assert len(series) == 14
+    assert abs(series[-1] - 7.42e-2) <= 1.1e-4
+
+
+def two_spheres(model, pos, rho, vol, radius, extend, nu, E, e, gamma, dt):
+    x = np.array([[pos, 0.0, 0.0], [-pos, 0.0, 0.0]])
+    v = np.array([[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    desc = make_desc(model, radius=radius, radius_extend=extend, nu=nu, E=E, restitution=e, gamma=gamma,
+                     mu_friction=0.0, dt=dt)
+    s = orc.OracleSpec(desc, 2, x, np.full(2, vol), v=v, rho=np.full(2, rho))
+    s.build_bonds()
+    s.init()
+    return s, desc
+
+
+def kinetic_energy(v, rho, vol):
+    return float(np.sum(0.5 * rho * vol * np.hypot.reduce(v, axis=1) ** 2))
+
+
+def test_hertz_restitution():
+    # regression_test/inputs/hertzian_contact.json
+    rho, vol, E, nu, e = 7.95e3, 5.236e-13, 195.6e9, 0.25, 0.5
+    dt, tf = 0.8e-8, 1e-5
+    s, _ = two_spheres("HERTZ", 5.1e-5, rho, vol, 5e-5, 1e-8, nu, E, e, 0.0, dt)
+    ke_i = kinetic_energy(np.array([[-1.0, 0, 0], [1.0, 0, 0]]), rho, vol)
+    s.step(int(tf / dt))
+    ke_f = kinetic_energy(s.get("v"), rho, vol)
+    assert abs(math.sqrt(ke_f / ke_i) - e) <= 1e-3
+
+
+def test_jkr_pulloff():
+    # regression_test/inputs/hertzian_jkr_contact.json
+    rho, vol, E, nu, e, gamma = 7.95e3, 5.236e-13, 195.6e9, 0.25, 1.0, 10.0
+    dt, tf = 1e-9, 5e-7
+    s, desc = two_spheres("HERTZ_JKR", 5.01e-5, rho, vol, 5e-5, 1e-8, nu, E, e, gamma, dt)
+    steps = int(tf / dt)
+    assert steps == 499  # Inputs truncation (CabanaPD_Input.hpp:109)
+    fx = []
+    for _ in range(steps):
+        s.step(1)
+        fx.append(s.get("f")[0, 0])
+    fc = orc.model_constants(desc)["fc"]
+    assert abs(min(fx) * vol / fc - (-1.0)) <= 5e-3
