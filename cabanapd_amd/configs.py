@@ -155,6 +155,39 @@ def lps_cube(n_side=128, steps=500, precision="fp64", seed=44758454, dt=None, si
     return w
 
 
+def dem_packing(n=1000, seed=7, model="HERTZ_JKR", polydisperse=True, dt=1.5e-7, steps=2000,
+                rebuild_every=20, speed=50.0, r_mean=1e-3, sigma=0.1, clip=0.3, E=70e9, nu=0.25, rho=2600.0,
+                restitution=0.5, gamma=0.05, extend_frac=0.1, gravity=True):
+    """Spheres on a jittered cubic lattice (non-overlapping: each centre stays
+    at least its radius inside its own cell), log-normal radii clipped to
+    +-clip around r_mean, random velocities of magnitude ~speed, gravity
+    f_z -= 9.8 rho (examples/dem/powder_fill.cpp:138-142)."""
+    rng = np.random.default_rng(seed)
+    if polydisperse:
+        r = np.exp(rng.normal(math.log(r_mean), sigma, size=n))
+        r = np.clip(r, (1 - clip) * r_mean, (1 + clip) * r_mean)
+    else:
+        r = np.full(n, r_mean)
+    a = 2.05 * r.max()
+    side = int(math.ceil(n ** (1.0 / 3.0)))
+    cells = np.stack(np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij"),
+                     axis=-1).reshape(-1, 3)[:n, ::-1].astype(np.float64)
+    slack = (a / 2.0 - r)[:, None]
+    x = (cells + 0.5) * a + rng.uniform(-1.0, 1.0, size=(n, 3)) * slack
+    vol = 4.0 / 3.0 * math.pi * r ** 3
+    desc = make_desc(model, radius=r_mean, radius_extend=extend_frac * r_mean, nu=nu, E=E,
+                     restitution=restitution, gamma=gamma, mu_friction=0.0, dt=dt, rebuild_every=rebuild_every,
+                     polydisperse=polydisperse)
+    w = _base(desc, x, vol, rho, "dem_%s_%d" % (model.lower(), n), a, steps)
+    w["v"] = rng.normal(0.0, speed / math.sqrt(3.0), size=(n, 3))
+    w["radius"] = r if polydisperse else None
+    if gravity:
+        b = np.zeros((n, 3))
+        b[:, 2] = -9.8 * rho
+        w["body"] = b
+    return w
+
+
 def lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.1, K=1.0, G=0.5, influence=None,
                      fracture=False, G0=1000.0):
     """Known-answer lattice of unit_test/tstForce.hpp:593-661: cell-centred

@@ -127,6 +127,7 @@ struct pdk_ctx_s
     int device = 0;
     MemAccount acct;
     std::unique_ptr<BatchEngine> batch;
+    int64_t live_specs = 0;
 };
 
 struct pdk_spec_s
@@ -144,6 +145,7 @@ struct pdk_spec_s
     bool uploaded = false, built = false, inited = false;
     int64_t col_rows_cap = 0, total_rows = 0, total_bonds = 0;
     int32_t max_cnt = 0;
+    bool stats_dirty = false; // contact list rebuilt since total_bonds/max_cnt were computed
     int64_t steps_done = 0;
     // contact bookkeeping
     int64_t steps_since_build = 0;
@@ -283,6 +285,24 @@ struct pdk_spec_s
 
     double* Ucur() { return cur ? U1.as<double>() : U0.as<double>(); }
 
+    // Bond count and longest row from the device row lengths.
+    void refreshStats()
+    {
+        std::vector<int32_t> c( n );
+        PDK_HIP( hipMemcpyAsync( c.data(), cnt.as<int32_t>(), n * sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
+        PDK_HIP( hipStreamSynchronize( st ) );
+        int64_t t = 0;
+        int32_t mx = 0;
+        for ( int32_t v : c )
+        {
+            t += v;
+            mx = std::max( mx, v );
+        }
+        total_bonds = t;
+        max_cnt = mx;
+        stats_dirty = false;
+    }
+
     // ---- PD orchestration ---------------------------------------------------
     bool lps() const { return d.model == PDK_MODEL_LPS; }
 
@@ -378,6 +398,7 @@ int pdk_ctx_destroy( pdk_ctx ctx )
     return guarded( [&] {
         if ( ctx )
         {
+            require( ctx->live_specs == 0, "destroy every spec of a context before the context" );
             ctx->batch.reset();
             delete ctx;
         }
@@ -443,6 +464,7 @@ int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spe
         }
         s->h.bc_kind = -1;
         *out = s.release();
+        ctx->live_specs++;
     } );
 }
 
@@ -459,6 +481,8 @@ int pdk_spec_destroy( pdk_spec s )
             (void)hipStreamSynchronize( s->st );
             (void)hipStreamDestroy( s->st );
         }
+        if ( s->ctx )
+            s->ctx->live_specs--;
         delete s;
     } );
 }
@@ -591,19 +615,7 @@ int pdk_build_bonds( pdk_spec s )
                                                 s->col_rows_cap, s->scratch, s->acct(), s->st, &s->max_cnt );
             } );
         }
-        std::vector<int32_t> c( s->n );
-        PDK_HIP( hipMemcpyAsync( c.data(), s->cnt.as<int32_t>(), s->n * sizeof( int32_t ), hipMemcpyDeviceToHost,
-                                 s->st ) );
-        PDK_HIP( hipStreamSynchronize( s->st ) );
-        int64_t t = 0;
-        int32_t mx = 0;
-        for ( int32_t v : c )
-        {
-            t += v;
-            mx = std::max( mx, v );
-        }
-        s->total_bonds = t;
-        s->max_cnt = mx;
+        s->refreshStats();
         s->built = true;
         s->inited = false;
         s->pushArgs();
@@ -615,6 +627,8 @@ int pdk_bond_stats( pdk_spec s, int64_t* total_bonds, int32_t* max_neighbors )
 {
     return guarded( [&] {
         require( s && s->built, "bonds not built" );
+        if ( s->stats_dirty )
+            s->refreshStats();
         if ( total_bonds )
             *total_bonds = s->total_bonds;
         if ( max_neighbors )
@@ -653,10 +667,14 @@ int pdk_prenotch( pdk_spec s, int32_t nn, const double* p0, const double* v1, co
     } );
 }
 
-int pdk_download_bonds( pdk_spec s, int64_t* offsets, int64_t* nbr, uint8_t* intact )
+int pdk_download_bonds( pdk_spec s, int64_t capacity, int64_t* offsets, int64_t* nbr, uint8_t* intact )
 {
     return guarded( [&] {
         require( s && s->built, "bonds not built" );
+        if ( s->stats_dirty )
+            s->refreshStats();
+        require( ( !nbr && !intact ) || capacity >= s->total_bonds,
+                 "pdk_download_bonds: capacity is smaller than the bond count" );
         const int64_t n = s->n, ns = s->nslices;
         std::vector<int64_t> sp( ns + 1 );
         std::vector<int32_t> c( n );
@@ -673,6 +691,10 @@ int pdk_download_bonds( pdk_spec s, int64_t* offsets, int64_t* nbr, uint8_t* int
                                      hipMemcpyDeviceToHost, s->st ) );
         }
         PDK_HIP( hipStreamSynchronize( s->st ) );
+        int64_t sum = 0;
+        for ( int32_t v : c )
+            sum += v;
+        require( ( !nbr && !intact ) || sum <= capacity, "pdk_download_bonds: capacity is smaller than the bond count" );
         int64_t off = 0;
         for ( int64_t cid = 0; cid < n; cid++ )
         {
@@ -995,6 +1017,7 @@ void contactBuildImpl( pdk_spec s, bool initial, double* max_disp, int64_t* sinc
                                     s->st, &s->max_cnt );
     PDK_HIP( hipMemcpyAsync( s->u_neigh.as<double>(), s->Ucur(), 3 * s->n * sizeof( double ), hipMemcpyDeviceToDevice,
                              s->st ) );
+    s->stats_dirty = true;
     s->pushArgs();
     *max_disp = 0.0;
     *since_build = 0;

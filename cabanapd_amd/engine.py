@@ -38,7 +38,7 @@ _SIGS = {
     "pdk_build_bonds": (C.c_int, [_VP]),
     "pdk_bond_stats": (C.c_int, [_VP, _I64, _I32]),
     "pdk_prenotch": (C.c_int, [_VP, C.c_int32, _D, _D, _D]),
-    "pdk_download_bonds": (C.c_int, [_VP, _I64, _I64, _U8]),
+    "pdk_download_bonds": (C.c_int, [_VP, C.c_int64, _I64, _I64, _U8]),
     "pdk_set_bc": (C.c_int, [_VP, C.c_int32, _I64, C.c_int64, C.c_double]),
     "pdk_set_body_force": (C.c_int, [_VP, _D]),
     "pdk_compute_weighted_volume": (C.c_int, [_VP]),
@@ -205,7 +205,7 @@ class Spec:
         off = np.zeros(self.n + 1, dtype=np.int64)
         nbr = np.zeros(max(total, 1), dtype=np.int64)
         intact = np.zeros(max(total, 1), dtype=np.uint8)
-        _check(lib().pdk_download_bonds(self.h, off.ctypes.data_as(_I64), nbr.ctypes.data_as(_I64),
+        _check(lib().pdk_download_bonds(self.h, nbr.size, off.ctypes.data_as(_I64), nbr.ctypes.data_as(_I64),
                                         intact.ctypes.data_as(_U8)))
         return off, nbr[:total], intact[:total]
 

@@ -1,0 +1,337 @@
+// CabanaPD_ForceModels.hpp -- ForceModel<PMB|LPS, Elastic, NoFracture|Fracture>
+// and the Hertz / Hertz-JKR contact models of the drop-in API.
+//
+// Constructors, deduction guides and public members follow the reference:
+//   force_models/CabanaPD_PMB.hpp:149-231,277-297
+//   force_models/CabanaPD_LPS.hpp:151-253,274-302
+//   CabanaPD_ForceModels.hpp:42-129
+//   force_models/CabanaPD_Contact.hpp:26-51, CabanaPD_Hertzian.hpp:23-56,
+//   CabanaPD_HertzianJKR.hpp:23-80
+// Derived constants come from the engine (pdk_model_constants_get), which
+// evaluates the reference's expressions.
+#ifndef CABANAPD_DROPIN_FORCEMODELS_HPP
+#define CABANAPD_DROPIN_FORCEMODELS_HPP
+
+#include <stdexcept>
+
+#include <pdk.h>
+
+#include "CabanaPD_Types.hpp"
+
+namespace CabanaPD
+{
+namespace detail
+{
+inline pdk_model_constants constantsOf( const pdk_model_desc& d )
+{
+    pdk_model_constants k{};
+    if ( pdk_model_constants_get( &d, &k ) != 0 )
+        throw std::runtime_error( pdk_last_error() );
+    return k;
+}
+inline pdk_model_desc blankDesc()
+{
+    pdk_model_desc d{};
+    d.neighbor_tol = 1e-14;
+    d.nu = 0.25;
+    d.restitution = 1.0;
+    return d;
+}
+} // namespace detail
+
+template <class ModelType, class MechanicsType = Elastic, class FractureType = Fracture,
+          class ThermalType = TemperatureIndependent>
+struct ForceModel;
+
+struct BaseForceModel
+{
+    using material_type = SingleMaterial;
+    using thermal_type = TemperatureIndependent;
+    double force_horizon = 0.0;
+    double K = 0.0;
+    auto cutoff() const { return force_horizon; }
+    auto extend() const { return 0.0; }
+    void updateBonds( const int, const int ) {}
+};
+
+template <class Tag, class FractureType>
+struct PDForceModel : BaseForceModel
+{
+    using model_tag = Tag;
+    using force_tag = Tag;
+    using base_model = Tag;
+    using fracture_type = FractureType;
+    using mechanics_type = Elastic;
+    double G = 0.0;
+    double G0 = 0.0;
+    double s0 = 0.0;
+    double bond_break_coeff = 0.0;
+    double c = 0.0;
+    double theta_coeff = 0.0;
+    double s_coeff = 0.0;
+    int influence_type = 0;
+
+    // The engine's description of this model for time step dt.
+    pdk_model_desc desc( double dt ) const
+    {
+        pdk_model_desc d = detail::blankDesc();
+        d.model = std::is_same_v<Tag, LPS> ? PDK_MODEL_LPS : PDK_MODEL_PMB;
+        d.fracture = std::is_same_v<FractureType, Fracture> ? 1 : 0;
+        d.influence = influence_type;
+        d.precision = PDK_FP64;
+        d.horizon = force_horizon;
+        d.K = K;
+        d.G = G;
+        d.G0 = G0;
+        d.dt = dt;
+        return d;
+    }
+
+  protected:
+    void derive()
+    {
+        pdk_model_constants k = detail::constantsOf( desc( 1.0 ) );
+        c = k.c;
+        s0 = k.s0;
+        bond_break_coeff = k.bond_break_coeff;
+        theta_coeff = k.theta_coeff;
+        s_coeff = k.s_coeff;
+    }
+};
+
+template <class F>
+struct is_fracture<ForceModel<PMB, Elastic, F>> : std::is_same<F, Fracture>
+{
+};
+template <class F>
+struct is_fracture<ForceModel<LPS, Elastic, F>> : std::is_same<F, Fracture>
+{
+};
+
+// ---- PMB -------------------------------------------------------------------
+template <>
+struct ForceModel<PMB, Elastic, NoFracture> : PDForceModel<PMB, NoFracture>
+{
+    ForceModel( PMB, NoFracture, const double force_horizon_, const double K_ ) { set( force_horizon_, K_ ); }
+    ForceModel( PMB, Elastic, NoFracture, const double force_horizon_, const double K_ )
+    {
+        set( force_horizon_, K_ );
+    }
+
+  private:
+    void set( double h, double k )
+    {
+        force_horizon = h;
+        K = k;
+        influence_type = 1;
+        derive();
+    }
+};
+
+template <>
+struct ForceModel<PMB, Elastic, Fracture> : PDForceModel<PMB, Fracture>
+{
+    ForceModel( PMB, const double h, const double k, const double g0, const int influence = 1 )
+    {
+        set( h, k, g0, influence );
+    }
+    ForceModel( PMB, Elastic, const double h, const double k, const double g0, const int influence = 1 )
+    {
+        set( h, k, g0, influence );
+    }
+    ForceModel( PMB, Fracture, const double h, const double k, const double g0, const int influence = 1 )
+    {
+        set( h, k, g0, influence );
+    }
+    ForceModel( PMB, Elastic, Fracture, const double h, const double k, const double g0, const int influence = 1 )
+    {
+        set( h, k, g0, influence );
+    }
+
+  private:
+    void set( double h, double k, double g0, int infl )
+    {
+        force_horizon = h;
+        K = k;
+        G0 = g0;
+        influence_type = infl;
+        derive();
+    }
+};
+
+// ---- LPS -------------------------------------------------------------------
+template <>
+struct ForceModel<LPS, Elastic, NoFracture> : PDForceModel<LPS, NoFracture>
+{
+    ForceModel( LPS, NoFracture, const double h, const double k, const double g, const int influence = 0 )
+    {
+        set( h, k, g, influence );
+    }
+    ForceModel( LPS, Elastic, NoFracture, const double h, const double k, const double g, const int influence = 0 )
+    {
+        set( h, k, g, influence );
+    }
+
+  private:
+    void set( double h, double k, double g, int infl )
+    {
+        force_horizon = h;
+        K = k;
+        G = g;
+        influence_type = infl;
+        derive();
+    }
+};
+
+template <>
+struct ForceModel<LPS, Elastic, Fracture> : PDForceModel<LPS, Fracture>
+{
+    ForceModel( LPS, const double h, const double k, const double g, const double g0, const int influence = 0 )
+    {
+        set( h, k, g, g0, influence );
+    }
+    ForceModel( LPS, Fracture, const double h, const double k, const double g, const double g0,
+                const int influence = 0 )
+    {
+        set( h, k, g, g0, influence );
+    }
+    ForceModel( LPS, Elastic, const double h, const double k, const double g, const double g0,
+                const int influence = 0 )
+    {
+        set( h, k, g, g0, influence );
+    }
+    ForceModel( LPS, Elastic, Fracture, const double h, const double k, const double g, const double g0,
+                const int influence = 0 )
+    {
+        set( h, k, g, g0, influence );
+    }
+
+  private:
+    void set( double h, double k, double g, double g0, int infl )
+    {
+        force_horizon = h;
+        K = k;
+        G = g;
+        G0 = g0;
+        influence_type = infl;
+        derive();
+    }
+};
+
+// Deduction guides (force_models/CabanaPD_PMB.hpp:277-297, force_models/CabanaPD_LPS.hpp:274-302)
+ForceModel( PMB, NoFracture, double, double ) -> ForceModel<PMB, Elastic, NoFracture>;
+ForceModel( PMB, Elastic, NoFracture, double, double ) -> ForceModel<PMB, Elastic, NoFracture>;
+ForceModel( PMB, double, double, double ) -> ForceModel<PMB, Elastic, Fracture>;
+ForceModel( PMB, double, double, double, int ) -> ForceModel<PMB, Elastic, Fracture>;
+ForceModel( PMB, Elastic, double, double, double ) -> ForceModel<PMB, Elastic, Fracture>;
+ForceModel( PMB, Fracture, double, double, double ) -> ForceModel<PMB, Elastic, Fracture>;
+ForceModel( PMB, Elastic, Fracture, double, double, double ) -> ForceModel<PMB, Elastic, Fracture>;
+ForceModel( LPS, NoFracture, double, double, double ) -> ForceModel<LPS, Elastic, NoFracture>;
+ForceModel( LPS, NoFracture, double, double, double, int ) -> ForceModel<LPS, Elastic, NoFracture>;
+ForceModel( LPS, Elastic, NoFracture, double, double, double ) -> ForceModel<LPS, Elastic, NoFracture>;
+ForceModel( LPS, Elastic, NoFracture, double, double, double, int ) -> ForceModel<LPS, Elastic, NoFracture>;
+ForceModel( LPS, double, double, double, double ) -> ForceModel<LPS, Elastic, Fracture>;
+ForceModel( LPS, double, double, double, double, int ) -> ForceModel<LPS, Elastic, Fracture>;
+ForceModel( LPS, Fracture, double, double, double, double ) -> ForceModel<LPS, Elastic, Fracture>;
+ForceModel( LPS, Fracture, double, double, double, double, int ) -> ForceModel<LPS, Elastic, Fracture>;
+ForceModel( LPS, Elastic, double, double, double, double ) -> ForceModel<LPS, Elastic, Fracture>;
+ForceModel( LPS, Elastic, double, double, double, double, int ) -> ForceModel<LPS, Elastic, Fracture>;
+ForceModel( LPS, Elastic, Fracture, double, double, double, double ) -> ForceModel<LPS, Elastic, Fracture>;
+ForceModel( LPS, Elastic, Fracture, double, double, double, double, int ) -> ForceModel<LPS, Elastic, Fracture>;
+
+// ---- contact ---------------------------------------------------------------
+struct ContactModel
+{
+    using model_tag = Contact;
+    using material_type = SingleMaterial;
+    using thermal_type = TemperatureIndependent;
+    using fracture_type = NoFracture;
+    double radius = 0.0;
+    double radius_extend = 0.0;
+    double K = 0.0;
+    ContactModel() = default;
+    ContactModel( const double r, const double e ) : radius( r ), radius_extend( e ) {}
+    auto cutoff() const { return 2.0 * radius + radius_extend; }
+    auto extend() const { return radius_extend; }
+    void updateBonds( const int, const int ) {}
+};
+
+struct HertzianModel : ContactModel
+{
+    using force_tag = HertzianModel;
+    using base_model = HertzianModel;
+    double nu = 0.25, E = 0.0, e = 1.0;
+    double Rs = 0.0, Es = 0.0, beta = 0.0, coeff_h_n = 0.0, coeff_h_d = 0.0;
+    HertzianModel() = default;
+    HertzianModel( const double r, const double ext, const double nu_, const double E_, const double e_ )
+        : ContactModel( r, ext ), nu( nu_ ), E( E_ ), e( e_ )
+    {
+        auto k = detail::constantsOf( desc( 1.0 ) );
+        Rs = k.Rs;
+        Es = k.Es;
+        beta = k.beta;
+        coeff_h_n = k.coeff_h_n;
+        coeff_h_d = k.coeff_h_d;
+    }
+    pdk_model_desc desc( double dt ) const
+    {
+        pdk_model_desc d = detail::blankDesc();
+        d.model = PDK_MODEL_HERTZ;
+        d.radius = radius;
+        d.radius_extend = radius_extend;
+        d.nu = nu;
+        d.E = E;
+        d.restitution = e;
+        d.dt = dt;
+        return d;
+    }
+};
+
+struct HertzianJKRModel : ContactModel
+{
+    using force_tag = HertzianModel; // reference: HertzianJKR.hpp:29
+    using base_model = HertzianJKRModel;
+    double nu = 0.25, E = 0.0, mu = 0.0, e = 1.0, gamma = 0.0;
+    double Rs = 0.0, Es = 0.0, Gs = 0.0, beta = 0.0, coeff_h_n = 0.0, coeff_h_d = 0.0, Gamma = 0.0, a0 = 0.0,
+           delta_tear = 0.0, fc = 0.0;
+    HertzianJKRModel() = default;
+    HertzianJKRModel( const double r, const double ext, const double nu_, const double E_, const double e_,
+                      const double gamma_, const double mu_ )
+        : ContactModel( r, ext ), nu( nu_ ), E( E_ ), mu( mu_ ), e( e_ ), gamma( gamma_ )
+    {
+        auto k = detail::constantsOf( desc( 1.0 ) );
+        Rs = k.Rs;
+        Es = k.Es;
+        Gs = k.Gs;
+        beta = k.beta;
+        coeff_h_n = k.coeff_h_n;
+        coeff_h_d = k.coeff_h_d;
+        Gamma = k.Gamma;
+        a0 = k.a0;
+        delta_tear = k.delta_tear;
+        fc = k.fc;
+    }
+    pdk_model_desc desc( double dt ) const
+    {
+        pdk_model_desc d = detail::blankDesc();
+        d.model = PDK_MODEL_HERTZ_JKR;
+        d.radius = radius;
+        d.radius_extend = radius_extend;
+        d.nu = nu;
+        d.E = E;
+        d.restitution = e;
+        d.gamma = gamma;
+        d.mu_friction = mu;
+        d.dt = dt;
+        return d;
+    }
+};
+
+template <class T>
+struct is_contact : std::integral_constant<bool, std::is_base_of_v<ContactModel, T>>
+{
+};
+
+} // namespace CabanaPD
+
+#endif

@@ -1,0 +1,371 @@
+// CabanaPD_Solver.hpp -- Neighbor, Force, computeForce/computeEnergy and
+// Solver of the drop-in API, calling the engine through include/pdk.h.
+//
+// Reference: src/CabanaPD_Neighbor.hpp:24-220, src/CabanaPD_Force.hpp:140-219,
+// src/CabanaPD_Solver.hpp:87-657.  The step order is the reference's
+// (Solver::runStep, :332-404); the engine runs the velocity-Verlet halves in
+// the force pass epilogue and keeps the state on the GPU between output steps.
+#ifndef CABANAPD_DROPIN_SOLVER_HPP
+#define CABANAPD_DROPIN_SOLVER_HPP
+
+#include <chrono>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <pdk.h>
+
+#include "CabanaPD_Boundary.hpp"
+#include "CabanaPD_ForceModels.hpp"
+#include "CabanaPD_Input.hpp"
+#include "CabanaPD_Particles.hpp"
+
+namespace CabanaPD
+{
+namespace detail
+{
+inline void check( int rc )
+{
+    if ( rc != 0 )
+        throw std::runtime_error( pdk_last_error() );
+}
+
+// One engine context per process (device 0).
+inline pdk_ctx context()
+{
+    static pdk_ctx ctx = [] {
+        pdk_ctx c = nullptr;
+        check( pdk_ctx_create( 0, 0, &c ) );
+        return c;
+    }();
+    return ctx;
+}
+
+// Owning handle of the engine-side specimen.
+struct DeviceSpec
+{
+    pdk_spec spec = nullptr;
+    bool lps = false;
+    bool fracture = false;
+    DeviceSpec() = default;
+    DeviceSpec( const DeviceSpec& ) = delete;
+    DeviceSpec& operator=( const DeviceSpec& ) = delete;
+    ~DeviceSpec()
+    {
+        if ( spec )
+            (void)pdk_spec_destroy( spec );
+    }
+
+    // Particles::create + update -> device; then the Neighbor build.
+    void attach( const pdk_model_desc& d, ParticleStorage& s )
+    {
+        lps = d.model == PDK_MODEL_LPS;
+        fracture = d.fracture != 0;
+        check( pdk_spec_create( context(), &d, static_cast<int64_t>( s.n ), &spec ) );
+        check( pdk_spec_upload( spec, s.x.data(), s.u.data(), s.v.data(), s.vol.data(), s.rho.data(),
+                                s.nofail.data(), nullptr ) );
+        check( pdk_build_bonds( spec ) );
+    }
+
+    void pull( ParticleStorage& s, std::initializer_list<int> fields )
+    {
+        for ( int f : fields )
+        {
+            double* dst = nullptr;
+            switch ( f )
+            {
+            case PDK_FIELD_U: dst = s.u.data(); break;
+            case PDK_FIELD_V: dst = s.v.data(); break;
+            case PDK_FIELD_F: dst = s.f.data(); break;
+            case PDK_FIELD_W: dst = s.W.data(); break;
+            case PDK_FIELD_PHI: dst = s.phi.data(); break;
+            case PDK_FIELD_M: dst = s.m.data(); break;
+            case PDK_FIELD_THETA: dst = s.theta.data(); break;
+            default: continue;
+            }
+            if ( ( f == PDK_FIELD_M || f == PDK_FIELD_THETA ) && !lps )
+                continue;
+            check( pdk_download( spec, f, dst ) );
+        }
+    }
+    void pullState( ParticleStorage& s )
+    {
+        pull( s, { PDK_FIELD_U, PDK_FIELD_V, PDK_FIELD_F, PDK_FIELD_W, PDK_FIELD_PHI, PDK_FIELD_M,
+                   PDK_FIELD_THETA } );
+    }
+};
+} // namespace detail
+
+// ---- Neighbor (CabanaPD_Neighbor.hpp:24-220) ----------------------------------
+template <class MemorySpace, class FractureType = Fracture>
+class Neighbor
+{
+  public:
+    template <class ParticleType, class ModelType>
+    Neighbor( const ModelType& model, const ParticleType& particles, const double tol = 1e-14, double dt = 1.0 )
+        : _dev( std::make_shared<detail::DeviceSpec>() )
+    {
+        pdk_model_desc d = model.desc( dt );
+        d.neighbor_tol = tol;
+        _dev->attach( d, particles.storage() );
+    }
+    unsigned getMaxLocal() const
+    {
+        int64_t t = 0;
+        int32_t m = 0;
+        detail::check( pdk_bond_stats( _dev->spec, &t, &m ) );
+        return static_cast<unsigned>( m );
+    }
+    void getStatistics( unsigned& max_neighbors, unsigned long long& total_neighbors ) const
+    {
+        int64_t t = 0;
+        int32_t m = 0;
+        detail::check( pdk_bond_stats( _dev->spec, &t, &m ) );
+        max_neighbors = static_cast<unsigned>( m );
+        total_neighbors = static_cast<unsigned long long>( t );
+    }
+    template <class ExecSpace, class ParticleType, class PrenotchType>
+    void prenotch( ExecSpace, const ParticleType&, PrenotchType& notch )
+    {
+        std::vector<double> p0, v1, v2;
+        notch.flatten( p0, v1, v2 );
+        detail::check( pdk_prenotch( _dev->spec, static_cast<int32_t>( PrenotchType::num_notch ), p0.data(),
+                                     v1.data(), v2.data() ) );
+    }
+    pdk_spec spec() const { return _dev->spec; }
+    detail::DeviceSpec& device() const { return *_dev; }
+    double time() const { return 0.0; }
+
+  private:
+    std::shared_ptr<detail::DeviceSpec> _dev;
+};
+
+// ---- Force (CabanaPD_Force.hpp:140-171, force/CabanaPD_PMB.hpp, force/CabanaPD_LPS.hpp) ----
+template <class MemorySpace, class ModelTag, class FractureTag>
+class Force
+{
+  public:
+    Force() = default;
+    template <class ModelType, class ParticleType, class NeighborType>
+    void computeWeightedVolume( const ModelType&, ParticleType& particles, const NeighborType& neighbor )
+    {
+        detail::check( pdk_compute_weighted_volume( neighbor.spec() ) );
+        neighbor.device().pull( particles.storage(), { PDK_FIELD_M } );
+    }
+    template <class ModelType, class ParticleType, class NeighborType>
+    void computeDilatation( const ModelType&, ParticleType& particles, const NeighborType& neighbor )
+    {
+        detail::check( pdk_compute_dilatation( neighbor.spec() ) );
+        neighbor.device().pull( particles.storage(), { PDK_FIELD_THETA } );
+    }
+    template <class ModelType, class ParticleType, class NeighborType>
+    void computeForceFull( const ModelType&, ParticleType& particles, NeighborType& neighbor, bool reset )
+    {
+        detail::check( pdk_compute_force( neighbor.spec(), reset ? 1 : 0 ) );
+        neighbor.device().pull( particles.storage(), { PDK_FIELD_F } );
+    }
+    template <class ModelType, class ParticleType, class NeighborType>
+    void computeEnergyFull( const ModelType&, ParticleType& particles, const NeighborType& neighbor )
+    {
+        detail::check( pdk_compute_energy( neighbor.spec(), &_total_strain_energy, &_total_damage ) );
+        neighbor.device().pull( particles.storage(), { PDK_FIELD_W, PDK_FIELD_PHI } );
+    }
+    double totalStrainEnergy() const { return _total_strain_energy; }
+    double totalDamage() const { return _total_damage; }
+    double time() const { return 0.0; }
+    double timeEnergy() const { return 0.0; }
+
+  private:
+    double _total_strain_energy = 0.0;
+    double _total_damage = 0.0;
+};
+
+// computeForce / computeEnergy (CabanaPD_Force.hpp:176-219)
+template <class ModelType, class ForceType, class ParticleType, class NeighborType>
+void computeForce( const ModelType& model, ForceType& force, ParticleType& particles, NeighborType& neighbor,
+                   const bool reset = true )
+{
+    force.computeForceFull( model, particles, neighbor, reset );
+}
+
+template <class ModelType, class ForceType, class ParticleType, class NeighborType>
+void computeEnergy( const ModelType& model, ForceType& force, ParticleType& particles, const NeighborType& neighbor )
+{
+    if constexpr ( is_energy_output<typename ParticleType::output_type>::value )
+        force.computeEnergyFull( model, particles, neighbor );
+}
+
+// ---- Solver (CabanaPD_Solver.hpp:87-657) ---------------------------------------
+template <class ParticleType, class ForceModelType, class ContactModelType = NoContact>
+class Solver
+{
+  public:
+    using force_fracture_type = typename ForceModelType::fracture_type;
+    using force_type = Force<typename ParticleType::memory_space, typename ForceModelType::force_tag, force_fracture_type>;
+    using neighbor_type = Neighbor<typename ParticleType::memory_space, force_fracture_type>;
+
+    Solver( Inputs _inputs, ParticleType _particles, ForceModelType _force_model )
+        : particles( _particles ), inputs( _inputs ), force_model( _force_model )
+    {
+        static_assert( std::is_same_v<ContactModelType, NoContact>,
+                       "PD + contact in one Solver is not supported by this build" );
+        setup();
+    }
+
+    void setup()
+    {
+        // Solver::setup (CabanaPD_Solver.hpp:153-227)
+        if constexpr ( !is_contact<ForceModelType>::value )
+            inputs.computeCriticalTimeStep( force_model );
+        num_steps = inputs["num_steps"];
+        output_frequency = inputs["output_frequency"];
+        output_reference = inputs["output_reference"];
+        dt = inputs["timestep"];
+        neighbor = std::make_shared<neighbor_type>( force_model, particles, 1e-14, dt );
+        force = std::make_shared<force_type>();
+        unsigned max_neighbors = 0;
+        unsigned long long total_neighbors = 0;
+        neighbor->getStatistics( max_neighbors, total_neighbors );
+        log( std::cout, "Local particles: ", particles.numLocal(), ", Maximum neighbors: ", max_neighbors );
+        log( std::cout, "#Timestep/Total-steps Simulation-time" );
+    }
+
+    // Solver::init overloads (CabanaPD_Solver.hpp:229-297)
+    void init( const bool initial_output = true )
+    {
+        detail::check( pdk_init( spec() ) );
+        pullAll();
+        (void)initial_output;
+    }
+    template <class BoundaryType, std::enable_if_t<is_boundary_condition<BoundaryType>::value, int> = 0>
+    void init( BoundaryType boundary_condition, const bool initial_output = true )
+    {
+        registerBoundary( boundary_condition );
+        init( initial_output );
+    }
+    template <std::size_t N, std::size_t V>
+    void init( Prenotch<N, V> prenotch, const bool initial_output = true )
+    {
+        neighbor->prenotch( 0, particles, prenotch );
+        init( initial_output );
+    }
+    template <class BoundaryType, std::size_t N, std::size_t V>
+    void init( BoundaryType boundary_condition, Prenotch<N, V> prenotch, const bool initial_output = true )
+    {
+        neighbor->prenotch( 0, particles, prenotch );
+        init( boundary_condition, initial_output );
+    }
+
+    // Solver::run (CabanaPD_Solver.hpp:406-445)
+    void run() { runImpl(); }
+    template <class First, class... Rest>
+    void run( First& first, Rest&... rest )
+    {
+        if constexpr ( is_boundary_condition<First>::value )
+        {
+            registerBoundary( first );
+            runImpl( rest... );
+        }
+        else
+            runImpl( first, rest... );
+    }
+
+    void runStep( const int step )
+    {
+        detail::check( pdk_step( spec(), 1 ) );
+        output( step );
+    }
+    template <class BoundaryType>
+    void runStep( const int step, BoundaryType boundary_condition )
+    {
+        registerBoundary( boundary_condition );
+        runStep( step );
+    }
+
+    // Solver::updateForce (CabanaPD_Solver.hpp:462-477)
+    void updateForce()
+    {
+        if constexpr ( is_fracture<ForceModelType>::value )
+            detail::check( pdk_compute_weighted_volume( spec() ) );
+        detail::check( pdk_compute_dilatation( spec() ) );
+        detail::check( pdk_compute_force( spec(), 1 ) );
+        neighbor->device().pull( particles.storage(), { PDK_FIELD_F } );
+    }
+
+    int num_steps = 0;
+    int output_frequency = 1;
+    bool output_reference = true;
+    double dt = 0.0;
+    ParticleType particles;
+
+  protected:
+    pdk_spec spec() const { return neighbor->spec(); }
+
+    void pullAll() { neighbor->device().pullState( particles.storage() ); }
+
+    template <class BC>
+    void registerBoundary( BC& bc )
+    {
+        if constexpr ( is_body_term<std::decay_t<BC>>::value )
+        {
+            std::vector<double> b = bc.evaluate( particles.storage() );
+            detail::check( pdk_set_body_force( spec(), b.data() ) );
+        }
+        else if constexpr ( is_boundary_condition<std::decay_t<BC>>::value )
+        {
+            detail::check( pdk_set_bc( spec(), bc.kind(), bc._indices._view.data(),
+                                       static_cast<int64_t>( bc._indices._view.size() ), bc._value ) );
+        }
+    }
+
+    void output( const int step )
+    {
+        if ( output_frequency > 0 && step % output_frequency == 0 )
+        {
+            double W = 0.0, D = 0.0;
+            detail::check( pdk_compute_energy( spec(), &W, &D ) );
+            pullAll();
+            _last_energy = W;
+            _last_damage = D;
+            auto now = std::chrono::steady_clock::now();
+            double sec = std::chrono::duration<double>( now - _t0 ).count();
+            double rate = sec > 0 ? static_cast<double>( particles.numGlobal() ) * step / sec : 0.0;
+            log( std::cout, step, "/", num_steps, " ", step * dt, " ", W, " ",
+                 D / static_cast<double>( particles.numGlobal() ), " ", rate, " particle-steps/s" );
+        }
+    }
+
+    template <class... OutputType>
+    void runImpl( OutputType&... region_output )
+    {
+        _t0 = std::chrono::steady_clock::now();
+        int step = 0;
+        while ( step < num_steps )
+        {
+            int chunk = num_steps - step;
+            if ( output_frequency > 0 )
+                chunk = std::min( chunk, output_frequency - step % output_frequency );
+            detail::check( pdk_step( spec(), chunk ) );
+            step += chunk;
+            output( step );
+            if ( output_frequency > 0 && step % output_frequency == 0 )
+                ( region_output.update( step * dt ), ... );
+        }
+        pullAll();
+        ( region_output.print( 0 ), ... );
+    }
+
+    Inputs inputs;
+    ForceModelType force_model;
+    std::shared_ptr<neighbor_type> neighbor;
+    std::shared_ptr<force_type> force;
+    double _last_energy = 0.0;
+    double _last_damage = 0.0;
+    std::chrono::steady_clock::time_point _t0;
+};
+
+} // namespace CabanaPD
+
+#endif

@@ -1,0 +1,123 @@
+// CabanaPD_Types.hpp -- tags and traits of the drop-in API
+// (reference: src/CabanaPD_Types.hpp:19-294, subset on the force/update path).
+#ifndef CABANAPD_DROPIN_TYPES_HPP
+#define CABANAPD_DROPIN_TYPES_HPP
+
+#include <array>
+#include <cstddef>
+#include <type_traits>
+
+// Host-side names that user drivers written against CabanaPD spell with
+// Kokkos: execution/memory space tags, fixed-size arrays, the lambda macro and
+// library init.  Nothing here runs device code; the compute path is the
+// engine behind include/pdk.h.
+#ifndef KOKKOS_LAMBDA
+#define KOKKOS_LAMBDA [=]
+#endif
+#ifndef KOKKOS_INLINE_FUNCTION
+#define KOKKOS_INLINE_FUNCTION inline
+#endif
+#ifndef KOKKOS_FUNCTION
+#define KOKKOS_FUNCTION
+#endif
+
+namespace Kokkos
+{
+struct HostSpace;
+struct DefaultExecutionSpace
+{
+    using memory_space = HostSpace;
+    using execution_space = DefaultExecutionSpace;
+};
+struct HostSpace
+{
+    using memory_space = HostSpace;
+    using execution_space = DefaultExecutionSpace;
+};
+using DefaultHostExecutionSpace = DefaultExecutionSpace;
+template <class T, std::size_t N>
+using Array = std::array<T, N>;
+inline void initialize( int&, char** ) {}
+inline void initialize() {}
+inline void finalize() {}
+template <class T>
+struct Max
+{
+    using value_type = T;
+};
+template <class T>
+struct Min
+{
+    using value_type = T;
+};
+template <class T>
+struct Sum
+{
+    using value_type = T;
+};
+} // namespace Kokkos
+
+namespace CabanaPD
+{
+// Model tags (src/CabanaPD_Types.hpp)
+struct PMB
+{
+    using base_type = PMB;
+};
+struct LPS
+{
+    using base_type = LPS;
+};
+struct Elastic
+{
+};
+struct NoFracture
+{
+};
+struct Fracture
+{
+};
+struct TemperatureIndependent
+{
+    using base_type = TemperatureIndependent;
+};
+struct SingleMaterial
+{
+};
+struct Contact
+{
+    using base_type = Contact;
+};
+struct NoContact
+{
+    using base_type = NoContact;
+    using fracture_type = NoFracture;
+    using force_tag = NoContact;
+};
+struct BaseOutput
+{
+};
+struct EnergyOutput
+{
+};
+struct EnergyStressOutput
+{
+};
+
+template <class T>
+struct is_fracture : std::false_type
+{
+};
+template <class T>
+struct is_state_based : std::is_same<typename T::base_type, LPS>
+{
+};
+template <class T>
+struct is_energy_output
+    : std::integral_constant<bool, std::is_same_v<T, EnergyOutput> || std::is_same_v<T, EnergyStressOutput>>
+{
+};
+
+} // namespace CabanaPD
+
+#endif

@@ -146,8 +146,10 @@ int pdk_bond_stats(pdk_spec spec, int64_t* total_bonds, int32_t* max_neighbors);
  * p0, v1, v2 are double[n][3]. */
 int pdk_prenotch(pdk_spec spec, int32_t n, const double* p0, const double* v1, const double* v2);
 /* Bonds in caller order: offsets[n+1], nbr[total] (caller ids, ascending per
- * row), intact[total] (1 = mu 1, 0 = broken). Any pointer may be NULL. */
-int pdk_download_bonds(pdk_spec spec, int64_t* offsets, int64_t* nbr, uint8_t* intact);
+ * row), intact[total] (1 = mu 1, 0 = broken). Any pointer may be NULL.
+ * capacity is the length of nbr/intact; the call fails (writing nothing) when
+ * the current bond count (pdk_bond_stats) exceeds it. */
+int pdk_download_bonds(pdk_spec spec, int64_t capacity, int64_t* offsets, int64_t* nbr, uint8_t* intact);
 
 /* ---- boundary conditions / body terms --------------------------------------
  * createBoundaryCondition(ForceValueBCTag|ForceUpdateBCTag, value, ...)

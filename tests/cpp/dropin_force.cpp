@@ -1,0 +1,140 @@
+// Force/energy known answers through the drop-in API, in the shape of the
+// reference's unit_test/tstForce.hpp: Particles on [-1,1]^3, u = s0 x,
+// Force + Neighbor objects, computeForce / computeEnergy, checks against the
+// single-particle lattice sums (tstForce.hpp:128-155, 238-287, 386-422) on
+// interior particles.
+#include <cmath>
+#include <cstdio>
+
+#include <CabanaPD.hpp>
+
+namespace
+{
+int failures = 0;
+void expect( bool c, const char* what, double a, double b )
+{
+    if ( !c )
+    {
+        std::printf( "FAIL %s: %.17g vs %.17g\n", what, a, b );
+        failures++;
+    }
+}
+
+template <class Model>
+double latticeW( const Model& model, int m, double s0, bool lps, double K, double G )
+{
+    const double h = model.force_horizon, dx = h / m, vol = dx * dx * dx;
+    const double c = 18.0 * K / ( M_PI * h * h * h * h );
+    double wv = 0.0, nn = 0.0;
+    for ( int i = -m; i <= m; i++ )
+        for ( int j = -m; j <= m; j++ )
+            for ( int k = -m; k <= m; k++ )
+            {
+                double xi = std::sqrt( double( i * i + j * j + k * k ) ) * dx;
+                if ( xi > 0.0 && xi < h + 1e-14 )
+                {
+                    wv += ( 1.0 / xi ) * xi * xi * vol;
+                    nn += 1.0;
+                }
+            }
+    double theta = 0.0, W = 0.0;
+    for ( int i = -m; i <= m; i++ )
+        for ( int j = -m; j <= m; j++ )
+            for ( int k = -m; k <= m; k++ )
+            {
+                double xi = std::sqrt( double( i * i + j * j + k * k ) ) * dx;
+                if ( xi > 0.0 && xi < h + 1e-14 )
+                    theta += 3.0 / wv * ( 1.0 / xi ) * s0 * xi * xi * vol;
+            }
+    for ( int i = -m; i <= m; i++ )
+        for ( int j = -m; j <= m; j++ )
+            for ( int k = -m; k <= m; k++ )
+            {
+                double xi = std::sqrt( double( i * i + j * j + k * k ) ) * dx;
+                if ( !( xi > 0.0 && xi < h + 1e-14 ) )
+                    continue;
+                if ( lps )
+                    W += 1.0 / nn * 0.5 * ( 3.0 * K - 5.0 * G ) / 3.0 * theta * theta +
+                         0.5 * ( 15.0 * G / wv ) * ( 1.0 / xi ) * s0 * s0 * xi * xi * vol;
+                else
+                    W += 0.25 * c * s0 * s0 * xi * vol;
+            }
+    return W;
+}
+
+template <class Model, class Tag>
+void run( const Model& model, Tag, int nc, double bw, bool lps, double K, double G )
+{
+    using exec_space = Kokkos::DefaultExecutionSpace;
+    using memory_space = exec_space::memory_space;
+    const int m = 3;
+    const double s0 = 0.1;
+    CabanaPD::Particles particles( memory_space{}, Tag{}, CabanaPD::EnergyOutput{} );
+    particles.domain( { -1.0, -1.0, -1.0 }, { 1.0, 1.0, 1.0 }, { nc, nc, nc }, 0 );
+    particles.create( exec_space{} );
+    auto x = particles.sliceReferencePosition();
+    auto u = particles.sliceDisplacement();
+    particles.update( exec_space{}, [=]( const int pid ) {
+        for ( int d = 0; d < 3; d++ )
+            u( pid, d ) = s0 * x( pid, d );
+    } );
+    CabanaPD::Force<memory_space, typename Model::force_tag, typename Model::fracture_type> force;
+    CabanaPD::Neighbor<memory_space, typename Model::fracture_type> neighbor( model, particles );
+    force.computeWeightedVolume( model, particles, neighbor );
+    force.computeDilatation( model, particles, neighbor );
+    CabanaPD::computeForce( model, force, particles, neighbor );
+    CabanaPD::computeEnergy( model, force, particles, neighbor );
+
+    auto f = particles.sliceForce();
+    auto W = particles.sliceStrainEnergy();
+    auto vol = particles.sliceVolume();
+    auto theta = particles.sliceDilatation();
+    const double h = model.force_horizon;
+    const double refW = latticeW( model, m, s0, lps, K, G );
+    double Phi = 0.0;
+    int checked = 0;
+    for ( std::size_t p = 0; p < particles.numLocal(); p++ )
+    {
+        bool in = true;
+        for ( int d = 0; d < 3; d++ )
+            in = in && x( p, d ) > -1.0 + h * bw && x( p, d ) < 1.0 - h * bw;
+        if ( in )
+        {
+            for ( int d = 0; d < 3; d++ )
+                expect( f( p, d ) <= 1e-13, "f ~ 0", f( p, d ), 0.0 );
+            expect( std::fabs( W( p ) - refW ) <= 1e-6 * std::fabs( refW ), "W lattice sum", W( p ), refW );
+            if ( lps )
+                expect( std::fabs( theta( p ) - 3 * s0 ) <= 1e-6, "theta = 3 s0", theta( p ), 3 * s0 );
+            checked++;
+        }
+        Phi += W( p ) * vol( p );
+    }
+    expect( std::fabs( Phi - force.totalStrainEnergy() ) <= 1e-5, "total strain energy", Phi,
+            force.totalStrainEnergy() );
+    expect( checked > 0, "particles checked", checked, 0 );
+    std::printf( "%s: %d interior particles checked\n", lps ? "LPS" : "PMB", checked );
+}
+} // namespace
+
+int main()
+{
+    const double K = 1.0, G = 0.5;
+    {
+        const int nc = 11;
+        const double h = 3 * 2.0 / nc;
+        CabanaPD::ForceModel model( CabanaPD::PMB{}, CabanaPD::Elastic{}, CabanaPD::NoFracture{}, h, K );
+        run( model, CabanaPD::PMB{}, nc, 1.1, false, K, G );
+        CabanaPD::ForceModel dmodel( CabanaPD::PMB{}, h, K, 1000.0 );
+        run( dmodel, CabanaPD::PMB{}, nc, 1.1, false, K, G );
+    }
+    {
+        const int nc = 15;
+        const double h = 3 * 2.0 / nc;
+        CabanaPD::ForceModel model( CabanaPD::LPS{}, CabanaPD::Elastic{}, CabanaPD::NoFracture{}, h, K, G, 1 );
+        run( model, CabanaPD::LPS{}, nc, 2.1, true, K, G );
+        CabanaPD::ForceModel dmodel( CabanaPD::LPS{}, h, K, G, 1000.0, 1 );
+        run( dmodel, CabanaPD::LPS{}, nc, 2.1, true, K, G );
+    }
+    std::printf( "%s\n", failures ? "DROPIN_FORCE_FAIL" : "DROPIN_FORCE_OK" );
+    return failures ? 1 : 0;
+}

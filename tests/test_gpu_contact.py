@@ -1,0 +1,108 @@
+"""DEM contact step on the GPU: the reference's two-sphere regression goldens
+(regression_test/tstHertz.hpp:135, tstHertzJKR.hpp:145) and parity with the
+oracle on a polydisperse packing.
+
+pow/log/tanh come from the device math library on the GPU and from libm in
+the oracle, so contact trajectories are compared to a tolerance (1e-9
+relative on u after the run) instead of bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cabanapd_amd import configs, engine
+from cabanapd_amd._abi import make_desc
+from tests.parity import ctx, oracle_spec, rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+
+def two_spheres(model, pos, rho, vol, radius, extend, nu, E, e, gamma, dt):
+    x = np.array([[pos, 0.0, 0.0], [-pos, 0.0, 0.0]])
+    v = np.array([[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    desc = make_desc(model, radius=radius, radius_extend=extend, nu=nu, E=E, restitution=e, gamma=gamma,
+                     mu_friction=0.0, dt=dt)
+    s = engine.Spec(ctx(), desc, 2)
+    s.upload(x, np.full(2, vol), v=v, rho=np.full(2, rho))
+    s.build_bonds()
+    s.init()
+    return s, desc
+
+
+def test_gpu_hertz_restitution():
+    say("case: Hertz two spheres e=0.5")
+    rho, vol, E, nu, e = 7.95e3, 5.236e-13, 195.6e9, 0.25, 0.5
+    dt, tf = 0.8e-8, 1e-5
+    s, _ = two_spheres("HERTZ", 5.1e-5, rho, vol, 5e-5, 1e-8, nu, E, e, 0.0, dt)
+    ke_i = 2 * 0.5 * rho * vol * 1.0
+    s.step(int(tf / dt))
+    v = s.get("v")
+    ke_f = float(np.sum(0.5 * rho * vol * np.sum(v * v, axis=1)))
+    say(f"  sqrt(KE_f/KE_i) = {math.sqrt(ke_f / ke_i)!r}")
+    assert abs(math.sqrt(ke_f / ke_i) - e) <= 1e-3
+
+
+def test_gpu_jkr_pulloff():
+    say("case: Hertz-JKR two spheres pull-off")
+    rho, vol, E, nu, e, gamma = 7.95e3, 5.236e-13, 195.6e9, 0.25, 1.0, 10.0
+    dt, tf = 1e-9, 5e-7
+    s, desc = two_spheres("HERTZ_JKR", 5.01e-5, rho, vol, 5e-5, 1e-8, nu, E, e, gamma, dt)
+    fx = []
+    for _ in range(int(tf / dt)):
+        s.step(1)
+        fx.append(s.get("f")[0, 0])
+    fc = engine.model_constants(desc)["fc"]
+    say(f"  min pull-off / fc = {min(fx) * vol / fc!r}")
+    assert abs(min(fx) * vol / fc + 1.0) <= 5e-3
+
+
+def compressed_packing(model, poly, speed):
+    # lattice spacing below 2 R: most face neighbours overlap, so every
+    # branch of the normal force (overlap, JKR tail, approach/recede) is hit
+    w = configs.dem_packing(n=3000, seed=7, model=model, polydisperse=poly, dt=1e-8, steps=20, rebuild_every=10,
+                            speed=speed)
+    w["x"] = w["x"] * 0.93
+    return w
+
+
+@pytest.mark.parametrize("model,poly", [("HERTZ", False), ("HERTZ_JKR", False), ("HERTZ_JKR", True)])
+def test_gpu_contact_force_parity(model, poly):
+    """One contact pass on overlapping spheres: forces within 1e-12 of the
+    oracle (the only differences are libm vs device pow/log/tanh ulps)."""
+    say(f"case: {model} force poly={poly} vs oracle")
+    w = compressed_packing(model, poly, speed=1.0)
+    g = engine.Spec(ctx(), w["desc"], w["x"].shape[0])
+    configs.upload(g, w)
+    configs.setup(g, w)
+    o = oracle_spec(w)
+    configs.setup(o, w)
+    go, gn, _ = g.bonds()
+    oo, on, _ = o.bonds()
+    assert np.array_equal(go, oo) and np.array_equal(gn, on)
+    ef = rel_err(g.get("f"), o.get("f"))
+    say(f"  contacts {gn.size} f rel err {ef:.3e}")
+    assert gn.size > 3000
+    assert ef <= 1e-12
+
+
+@pytest.mark.parametrize("model,poly", [("HERTZ", False), ("HERTZ_JKR", True)])
+def test_gpu_contact_trajectory_parity(model, poly):
+    """20 steps with two list rebuilds.  Dense contact dynamics amplify the
+    ulp-level pow differences exponentially, so the horizon is kept short."""
+    say(f"case: {model} trajectory poly={poly} vs oracle")
+    w = compressed_packing(model, poly, speed=0.5)
+    g = engine.Spec(ctx(), w["desc"], w["x"].shape[0])
+    configs.upload(g, w)
+    configs.setup(g, w)
+    o = oracle_spec(w)
+    configs.setup(o, w)
+    g.step(w["steps"])
+    o.step(w["steps"])
+    go, gn, _ = g.bonds()
+    oo, on, _ = o.bonds()
+    eu = rel_err(g.get("u"), o.get("u"))
+    ev = rel_err(g.get("v"), o.get("v"))
+    say(f"  contacts {gn.size} u rel err {eu:.3e} v rel err {ev:.3e}")
+    assert np.array_equal(go, oo) and np.array_equal(gn, on)
+    assert eu <= 1e-9 and ev <= 1e-9

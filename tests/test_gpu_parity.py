@@ -1,0 +1,158 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+FP64 form: the engine evaluates the reference's expressions in the oracle's
+order, so bonds, broken-bond sets, forces, displacements and velocities are
+compared bit for bit; energy totals (a reduction in a different order) to
+1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from cabanapd_amd import configs
+from tests.parity import bonds_equal, broken_set, pair, rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(a, b, what):
+    """Bit-for-bit equality; NaN matches NaN (the reference produces NaN for
+    LPS energy of a particle whose bonds are all broken, force/CabanaPD_LPS.hpp:470-487)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    both_nan = np.isnan(a) & np.isnan(b)
+    nd = int(np.sum((a != b) & ~both_nan))
+    fin = ~both_nan
+    err = rel_err(a[fin], b[fin]) if fin.any() else 0.0
+    say(f"  {what}: differing={nd} nan={int(both_nan.sum())} rel_err={err:.3e}")
+    assert nd == 0, f"{what}: {nd} values differ, rel err {err:.3e}"
+
+
+def assert_total(a, b, what, tol=1e-12):
+    say(f"  {what}: gpu {a!r} oracle {b!r}")
+    if np.isnan(b):
+        assert np.isnan(a)
+    else:
+        assert abs(a - b) <= tol * max(abs(b), 1e-300)
+
+
+def test_gpu_elastic_cube_small_bitwise():
+    say("case: PMB elastic cube 17^3 FP64, 20 steps")
+    w = configs.elastic_cube(n_side=17, steps=20)
+    g, o = pair(w)
+    nb_ok, _ = bonds_equal(g, o)
+    assert nb_ok
+    assert g.bond_stats() == o.bond_stats()
+    assert_same(g.get("f"), o.get("f"), "f at init")
+    Wg, _ = g.compute_energy()
+    Wo, _ = o.compute_energy()
+    assert abs(Wg - Wo) <= 1e-12 * abs(Wo)
+    assert_same(g.get("W"), o.get("W"), "W at init")
+    g.step(20)
+    o.step(20)
+    for fld in ("u", "v", "f"):
+        assert_same(g.get(fld), o.get(fld), fld + " after 20 steps")
+
+
+def test_gpu_step_splitting_is_transparent():
+    say("case: PMB elastic cube 13^3, 7 steps in one call vs 3+4")
+    w = configs.elastic_cube(n_side=13, steps=7)
+    g1, _ = pair(w)
+    g2, _ = pair(w)
+    g1.step(7)
+    g2.step(3)
+    g2.step(4)
+    for fld in ("u", "v", "f"):
+        assert np.array_equal(g1.get(fld), g2.get(fld))
+    assert g1.steps_done() == 7 and g2.steps_done() == 7
+
+
+def test_gpu_elastic_cube_config1():
+    say("case: PMB elastic cube 41^3 FP64, 500 steps (the CPU-runnable benchmark specimen)")
+    w = configs.elastic_cube(n_side=41, steps=500)
+    g, o = pair(w)
+    total, maxn = g.bond_stats()
+    say(f"  bonds {total} max {maxn}")
+    assert total == 7720394 and maxn == 122
+    g.step(500)
+    o.step(500)
+    for fld in ("u", "v", "f"):
+        assert_same(g.get(fld), o.get(fld), fld + " after 500 steps")
+    Wg, _ = g.compute_energy()
+    Wo, _ = o.compute_energy()
+    say(f"  total strain energy gpu {Wg!r} oracle {Wo!r}")
+    assert abs(Wg - Wo) <= 1e-12 * abs(Wo)
+
+
+@pytest.mark.parametrize("model,field,fracture", [
+    ("PMB", "linear", False), ("PMB", "quadratic", False), ("PMB", "quadratic", True),
+    ("LPS", "linear", False), ("LPS", "quadratic", False), ("LPS", "quadratic", True)])
+def test_gpu_lattice_known_answers(model, field, fracture):
+    say(f"case: lattice {model} {field} fracture={fracture}")
+    nc = 11 if model == "PMB" else 15
+    w = configs.lattice_sum_cube(model=model, m=3, nc=nc, field=field, coeff=0.1 if field == "linear" else 0.01,
+                                 influence=1 if model == "LPS" else None, fracture=fracture)
+    g, o = pair(w, init=False)
+    for s in (g, o):
+        s.compute_weighted_volume()
+        s.compute_dilatation()
+        s.compute_force(True)
+    Wg, _ = g.compute_energy()
+    Wo, _ = o.compute_energy()
+    assert_same(g.get("f"), o.get("f"), "f")
+    assert_same(g.get("W"), o.get("W"), "W")
+    if model == "LPS":
+        assert_same(g.get("m"), o.get("m"), "m")
+        assert_same(g.get("theta"), o.get("theta"), "theta")
+    assert abs(Wg - Wo) <= 1e-12 * abs(Wo)
+
+
+def test_gpu_kalthoff_winkler_regression():
+    say("case: Kalthoff-Winkler regression 75x151x7, 700 steps FP64")
+    w = configs.kw_regression()
+    g, o = pair(w)
+    assert broken_set(g) == broken_set(o)
+    series = []
+    for k in range(1, w["steps"] + 1, w["output_every"]):
+        g.step(w["output_every"])
+        o.step(w["output_every"])
+        g.compute_energy()
+        o.compute_energy()
+        series.append(configs.kw_crack_y(w["x"], g.get("phi")))
+    assert_same(g.get("phi"), o.get("phi"), "phi")
+    assert_same(g.get("u"), o.get("u"), "u")
+    assert broken_set(g) == broken_set(o)
+    say(f"  max crack y {series[-1]!r}")
+    assert abs(series[-1] - 7.42e-2) <= 1.1e-4
+
+
+def test_gpu_pmb_fracture_plate_small():
+    say("case: PMB fracture plate 40x80x9, 150 steps FP64")
+    w = configs.kw_plate(num_cells=(40, 80, 9), steps=150, dt=3.0e-7 * 0.25)
+    g, o = pair(w)
+    g.step(150)
+    o.step(150)
+    bg, bo = broken_set(g), broken_set(o)
+    say(f"  broken bonds gpu {len(bg)} oracle {len(bo)}")
+    assert bg == bo
+    for fld in ("u", "v"):
+        assert_same(g.get(fld), o.get(fld), fld)
+
+
+@pytest.mark.parametrize("fracture", [False, True])
+def test_gpu_lps_cube_small(fracture):
+    say(f"case: LPS cube 20^3 fracture={fracture}, 40 steps FP64")
+    w = configs.lps_cube(n_side=20, steps=40, fracture=fracture)
+    g, o = pair(w)
+    g.step(40)
+    o.step(40)
+    for fld in ("u", "v", "m", "theta"):
+        assert_same(g.get(fld), o.get(fld), fld)
+    if fracture:
+        assert broken_set(g) == broken_set(o)
+    Wg, Dg = g.compute_energy()
+    Wo, Do = o.compute_energy()
+    assert_same(g.get("W"), o.get("W"), "W")
+    assert_total(Wg, Wo, "total strain energy")
+    if fracture:
+        assert_same(g.get("phi"), o.get("phi"), "phi")
+        assert_total(Dg, Do, "total damage")
