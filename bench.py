@@ -38,9 +38,10 @@ def algorithmic_bytes_per_step(n_particles, bonds, passes=1, fracture=True, lps=
     return passes * bonds * (4.0 + f / 8.0) + n_particles * (24 + 48 + 48 + (16 if lps else 0))
 
 
-def make_spec(ctx, w, precision):
+def make_spec(ctx, w, precision, bond_layout=0):
     d = type(w["desc"]).from_buffer_copy(w["desc"])
     d.precision = {"fp64": 0, "mixed": 1}[precision]
+    d.bond_layout = bond_layout
     s = engine.Spec(ctx, d, w["x"].shape[0])
     configs.upload(s, w)
     configs.setup(s, w)
@@ -84,6 +85,91 @@ def cpu_baseline(w, bonds, budget_s=12.0):
             "sample": f"first {steps} steps of the same specimen (oracle, OpenMP, {threads} threads)"}
 
 
+def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
+    """Serving stream: Poisson arrivals, specimens of uneven size.  The mixed
+    form runs through the continuous-batching engine (specimens admitted as
+    they arrive, one packed launch per step for all running specimens); the
+    plain FP64 form serves the same stream one specimen at a time.  Latency is
+    arrival -> result, including specimen setup (upload, bond build, init)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
+    reqs = [configs.serving_request(rng, n_range=(2e4, max_particles)) for _ in range(num_requests)]
+
+    def make(w, precision, ctx):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.precision = {"fp64": 0, "mixed": 1}[precision]
+        s = engine.Spec(ctx, d, w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        return s
+
+    def summarize(lat, wall, bond_evals):
+        lat = np.sort(np.asarray(lat))
+        return {"solves_per_s": len(lat) / wall, "p50_s": float(np.percentile(lat, 50)),
+                "p99_s": float(np.percentile(lat, 99)), "wall_s": wall, "bond_evals_per_s": bond_evals / wall}
+
+    # continuous batching, mixed form
+    ctx = engine.Context(0)
+    t0 = time.perf_counter()
+    pending = list(range(num_requests))
+    running = {}
+    lat = [0.0] * num_requests
+    bond_evals = 0.0
+    while pending or running:
+        now = time.perf_counter() - t0
+        while pending and arrivals[pending[0]] <= now:
+            k = pending.pop(0)
+            s = make(reqs[k], "mixed", ctx)
+            bond_evals += s.bond_stats()[0] * reqs[k]["steps"]
+            running[k] = (s, ctx.submit(s, reqs[k]["steps"]))
+        if running:
+            ctx.progress(4)
+        for k in list(running):
+            s, t = running[k]
+            if ctx.poll(t):
+                lat[k] = time.perf_counter() - t0 - arrivals[k]
+                s.close()
+                del running[k]
+        if not running and pending:
+            time.sleep(max(0.0, arrivals[pending[0]] - (time.perf_counter() - t0)))
+    mixed = summarize(lat, time.perf_counter() - t0, bond_evals)
+    ctx.close()
+
+    # plain FP64, one specimen at a time
+    ctx = engine.Context(0)
+    t0 = time.perf_counter()
+    lat = [0.0] * num_requests
+    bond_evals = 0.0
+    for k in range(num_requests):
+        wait = arrivals[k] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        s = make(reqs[k], "fp64", ctx)
+        bond_evals += s.bond_stats()[0] * reqs[k]["steps"]
+        s.step(reqs[k]["steps"])
+        lat[k] = time.perf_counter() - t0 - arrivals[k]
+        s.close()
+    fp64 = summarize(lat, time.perf_counter() - t0, bond_evals)
+    ctx.close()
+    sizes = [r["x"].shape[0] for r in reqs]
+    return {"requests": num_requests, "arrival_rate_per_s": rate, "seed": seed,
+            "particles_min": int(min(sizes)), "particles_max": int(max(sizes)),
+            "mixed_batched": mixed, "fp64_per_specimen": fp64}
+
+
+def copy_kernel(mib, repeats=5):
+    """A plain device copy of a known size: rocprofv3's FETCH_SIZE / WRITE_SIZE
+    for this kernel calibrate the counters (bytes moved / counter reading)."""
+    import torch
+    n = int(mib * (1 << 20)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(repeats):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4, "repeats": repeats}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +177,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cells", type=str, default="200,400,18")
+    ap.add_argument("--unroll", type=int, default=0, help="force-pass slot grouping (0 = autotune)")
+    ap.add_argument("--requests", type=int, default=24, help="serving stream length (0 = skip serving)")
+    ap.add_argument("--max-particles", type=float, default=2e6)
+    ap.add_argument("--bond-layout", choices=("auto", "compact"), default="auto",
+                    help="auto: stencil slots on full lattices; compact: stored neighbor ids")
+    ap.add_argument("--copy-kernel", type=float, default=0.0,
+                    help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_kernel > 0:
+        copy_kernel(args.copy_kernel)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; sharding across GPUs is design-only "
                                    "(see DESIGN.md)", "n_gpus": args.gpus}))
@@ -103,11 +199,16 @@ def main():
     res = {}
     specs = {}
     for prec in ("fp64", "mixed"):
-        s = make_spec(ctx, w, prec)
+        s = make_spec(ctx, w, prec, {"auto": 0, "compact": 1}[args.bond_layout])
         bonds, _ = s.bond_stats()
+        if args.unroll:
+            s.set_tuning(args.unroll)
+            unroll = args.unroll
+        else:
+            unroll = s.autotune(3)
         wall, force_ms, launches = timed_run(s, args.warmup, args.steps)
         res[prec] = dict(wall=wall, force_ms=force_ms, launches=launches, bonds=bonds,
-                         kernel=s.kernel_name(KCLS_FORCE))
+                         kernel=s.kernel_name(KCLS_FORCE), unroll=unroll, layout=s.layout())
         specs[prec] = s
     bonds = res["mixed"]["bonds"]
     u64 = specs["fp64"].get("u")
@@ -142,7 +243,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "PMB brittle fracture pre-notched plate %dx%dx%d (%d particles, %d bonds), "
                                "edge impact 16.5 m/s, dt 7.5e-8" % (cells + (n, bonds)),
-                   "particles": n, "bonds": bonds, "precision": "mixed"},
+                   "particles": n, "bonds": bonds, "precision": "mixed", "layout": m["layout"],
+                   "unroll": m["unroll"]},
         "fp64": {"value": fp64_value, "unit": "bond-evals/s", "ms_per_step": res["fp64"]["wall"] / args.steps * 1e3,
                  "kernel": res["fp64"]["kernel"],
                  "force_kernel_avg_ms": res["fp64"]["force_ms"] / max(res["fp64"]["launches"], 1)},
@@ -153,6 +255,8 @@ def main():
                      "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": b_alg},
         "cpu_baseline": None,
     }
+    if args.requests > 0:
+        line["serving"] = serving(args.requests, max_particles=args.max_particles)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, bonds)
     print(json.dumps(line))

@@ -43,7 +43,16 @@ class ModelDesc(C.Structure):
         ("mu_friction", C.c_double),
         ("rebuild_every", C.c_int32),
         ("polydisperse", C.c_int32),
+        ("ordering", C.c_int32),
+        ("bond_layout", C.c_int32),
     ]
+
+
+PDK_ORDER_AUTO = 0
+PDK_ORDER_MORTON = 1
+PDK_ORDER_LATTICE_ROWS = 2
+PDK_BONDS_AUTO = 0
+PDK_BONDS_COMPACT = 1
 
 
 class ModelConstants(C.Structure):
@@ -56,7 +65,7 @@ class ModelConstants(C.Structure):
 def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
               G=0.0, G0=0.0, dt=1e-7, influence=None, neighbor_tol=1e-14,
               radius=0.0, radius_extend=0.0, nu=0.25, E=0.0, restitution=1.0,
-              gamma=0.0, mu_friction=0.0, rebuild_every=0, polydisperse=False):
+              gamma=0.0, mu_friction=0.0, rebuild_every=0, polydisperse=False, ordering=0, bond_layout=0):
     """Model description with the reference's constructor defaults.
 
     influence defaults: PMB fracture s0 uses influence_type 1
@@ -87,4 +96,6 @@ def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
     d.mu_friction = mu_friction
     d.rebuild_every = rebuild_every
     d.polydisperse = 1 if polydisperse else 0
+    d.ordering = ordering
+    d.bond_layout = bond_layout
     return d

@@ -155,6 +155,52 @@ def lps_cube(n_side=128, steps=500, precision="fp64", seed=44758454, dt=None, si
     return w
 
 
+def serving_request(rng, dx=5e-4, rho=8000.0, nu=0.25, precision="mixed", n_range=(2e4, 2e6)):
+    """One specimen of the serving stream: a rectangular PMB fracture plate
+    with log-uniform particle count, aspect ratio U[1,4], thickness U[3,20]
+    cells, E ~ U[50,210] GPa, G0 ~ logU[1, 5e4] J/m^2, 0-3 random straight
+    pre-notches (placement as in examples/mechanics/random_cracks.cpp:74-114),
+    an x-velocity U[1,30] m/s on the low-x edge and U[200,2000] steps.
+    dx, rho and nu are not drawn (the plate values of the fracture benchmark)."""
+    n_target = math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))
+    aspect = rng.uniform(1.0, 4.0)
+    nz = int(rng.integers(3, 21))
+    nx = max(8, int(round(math.sqrt(n_target / (nz * aspect)))))
+    ny = max(8, int(round(aspect * nx)))
+    size = (nx * dx, ny * dx, nz * dx)
+    E = rng.uniform(50e9, 210e9)
+    G0 = math.exp(rng.uniform(math.log(1.0), math.log(5e4)))
+    K = E / (3.0 * (1.0 - 2.0 * nu))
+    horizon = 3.01 * dx
+    from .engine import critical_timestep
+    dt = critical_timestep(dx, dx, dx, K, rho, horizon, 0.85)
+    low = -0.5 * np.asarray(size)
+    high = 0.5 * np.asarray(size)
+    x, vol, d = grid_cell_centred(low, high, (nx, ny, nz))
+    desc = make_desc("PMB", fracture=True, precision=precision, horizon=horizon, K=K, G0=G0, dt=dt)
+    steps = int(rng.integers(200, 2001))
+    w = _base(desc, x, vol, rho, "serving_%dx%dx%d" % (nx, ny, nz), dx, steps)
+    nnotch = int(rng.integers(0, 4))
+    if nnotch:
+        thickness = size[2]
+        maxl = 0.25 * min(size[0], size[1])
+        minl = 0.5 * maxl
+        p0, v1, v2 = [], [], []
+        for _ in range(nnotch):
+            rx, ry = rng.uniform(), rng.uniform()
+            xc = (low[0] + maxl) + ((high[0] - maxl) - (low[0] + maxl)) * rx
+            yc = (low[1] + maxl) + ((high[1] - maxl) - (low[1] + maxl)) * ry
+            length = minl + (maxl - minl) * rng.uniform()
+            theta = math.pi * rng.uniform()
+            p0.append([xc, yc, low[2]])
+            v1.append([length * math.cos(theta), length * math.sin(theta), 0.0])
+            v2.append([0.0, (-1.0 + 2.0 * rng.uniform()) * thickness, thickness])
+        w["notches"] = (np.array(p0), np.array(v1), np.array(v2))
+    v0 = rng.uniform(1.0, 30.0)
+    w["v"][x[:, 0] < low[0] + dx, 0] = v0
+    return w
+
+
 def dem_packing(n=1000, seed=7, model="HERTZ_JKR", polydisperse=True, dt=1.5e-7, steps=2000,
                 rebuild_every=20, speed=50.0, r_mean=1e-3, sigma=0.1, clip=0.3, E=70e9, nu=0.25, rho=2600.0,
                 restitution=0.5, gamma=0.05, extend_frac=0.1, gravity=True):

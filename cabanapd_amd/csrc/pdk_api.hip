@@ -138,11 +138,20 @@ struct pdk_spec_s
     int64_t n = 0, nslices = 0;
     hipStream_t st = nullptr;
     DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
-        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging;
+        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab;
+    double lat_h[3] = { 0, 0, 0 }; // lattice spacing when lattice_rows
+    int64_t lat_dims[3] = { 0, 0, 0 };
+    int32_t nsten = 0;             // stencil entries (0 = no stencil)
+    SlotForm slot;                 // stencil-slot form (slot.slots == 0: compact form)
     std::vector<int64_t> h_perm, h_inv;
     PDArgs h{};
     int32_t cur = 0;
     bool uploaded = false, built = false, inited = false;
+    bool lattice_rows = false; // device order is lattice rows (else Morton)
+    int32_t unroll = 4;        // force-pass slot grouping (tuning knob)
+    bool vol_uniform = false;  // every particle has volume vol_value
+    double vol_value = 0.0;
+    bool any_nofail = false;
     int64_t col_rows_cap = 0, total_rows = 0, total_bonds = 0;
     int32_t max_cnt = 0;
     bool stats_dirty = false; // contact list rebuilt since total_bonds/max_cnt were computed
@@ -192,6 +201,19 @@ struct pdk_spec_s
         h.phi = phi.as<double>();
         h.radius = radius ? radius.as<double>() : nullptr;
         h.u_neigh = u_neigh ? u_neigh.as<double>() : nullptr;
+        h.sid = sid ? sid.as<uint8_t>() : nullptr;
+        h.stencil = stab ? stab.as<float4>() : nullptr;
+        h.exist = nullptr;
+        h.soff = nullptr;
+        h.stencil_slots = 0;
+        if ( slot.slots > 0 )
+        {
+            h.sid = nullptr;
+            h.stencil = slot.table.as<float4>();
+            h.exist = slot.exist.as<uint64_t>();
+            h.soff = slot.soff.as<int32_t>();
+            h.stencil_slots = slot.slots;
+        }
         h.c = k.c;
         h.bcc = k.bond_break_coeff;
         h.s0 = k.s0;
@@ -219,6 +241,10 @@ struct pdk_spec_s
         h.influence = d.influence;
         h.precision = d.precision;
         h.polydisperse = d.polydisperse;
+        h.unroll = unroll;
+        h.vol_uniform = vol_uniform ? 1 : 0;
+        h.vol_value = vol_value;
+        h.any_nofail = any_nofail ? 1 : 0;
         PDK_HIP( hipMemcpyAsync( args.as<PDArgs>(), &h, sizeof( PDArgs ), hipMemcpyHostToDevice, st ) );
     }
 
@@ -507,9 +533,33 @@ int pdk_spec_upload( pdk_spec s, const double* x, const double* u, const double*
         double h = std::cbrt( vsum / n );
         if ( !( h > 0 ) || !std::isfinite( h ) )
             h = 1.0;
+        // Lattice specimens are ordered in lattice rows (x fastest): the 64
+        // lanes of a slice then read contiguous runs of x and u at every bond
+        // slot.  Other point sets use Morton order.
+        std::vector<int64_t> lperm;
+        bool lattice = false;
+        if ( s->d.ordering != PDK_ORDER_MORTON )
+            lattice = latticeOrder( x, n, lperm, s->lat_h, s->lat_dims );
+        require( lattice || s->d.ordering != PDK_ORDER_LATTICE_ROWS, "positions do not form a lattice" );
+        s->lattice_rows = lattice;
+        s->vol_uniform = true;
+        s->vol_value = vol[0];
+        for ( int64_t i = 1; i < n && s->vol_uniform; i++ )
+            s->vol_uniform = vol[i] == vol[0];
+        s->any_nofail = false;
+        if ( nofail )
+            for ( int64_t i = 0; i < n && !s->any_nofail; i++ )
+                s->any_nofail = nofail[i] != 0;
         s->stageUpload( x, 3 * n * sizeof( double ) );
-        launchMortonOrder( s->staging.as<double>(), n, lo, 1.0 / h, s->perm.as<int64_t>(), s->inv.as<int64_t>(),
-                           s->scratch, s->acct(), s->st );
+        if ( lattice )
+        {
+            PDK_HIP( hipMemcpyAsync( s->perm.as<int64_t>(), lperm.data(), n * sizeof( int64_t ), hipMemcpyHostToDevice,
+                                     s->st ) );
+            launchInvertPerm( s->perm.as<int64_t>(), n, s->inv.as<int64_t>(), s->st );
+        }
+        else
+            launchMortonOrder( s->staging.as<double>(), n, lo, 1.0 / h, s->perm.as<int64_t>(), s->inv.as<int64_t>(),
+                               s->scratch, s->acct(), s->st );
         s->h_perm.resize( n );
         s->h_inv.resize( n );
         PDK_HIP( hipMemcpyAsync( s->h_perm.data(), s->perm.as<int64_t>(), n * sizeof( int64_t ), hipMemcpyDeviceToHost,
@@ -585,7 +635,7 @@ int pdk_spec_bytes( pdk_spec s, uint64_t* bytes )
         for ( const DevBuf* b : { &s->x, &s->U0, &s->U1, &s->v, &s->f, &s->vol, &s->rho, &s->nofail, &s->bcflag,
                                   &s->body, &s->m, &s->theta, &s->lpsA, &s->lpsB, &s->W, &s->phi, &s->perm, &s->inv,
                                   &s->cnt, &s->slice_ptr, &s->col, &s->intact, &s->radius, &s->u_neigh,
-                                  &s->partial, &s->args } )
+                                  &s->partial, &s->args, &s->sid, &s->stab } )
             t += b->bytes();
         *bytes = t;
     } );
@@ -614,6 +664,30 @@ int pdk_build_bonds( pdk_spec s )
                                                 s->slice_ptr.as<int64_t>(), s->cnt.as<int32_t>(), s->col, s->intact,
                                                 s->col_rows_cap, s->scratch, s->acct(), s->st, &s->max_cnt );
             } );
+            s->sid.release();
+            s->stab.release();
+            s->nsten = 0;
+            s->slot = SlotForm{};
+            // Full lattice: slot q of every row is stencil entry q, so the
+            // force pass computes neighbor ids instead of streaming them.
+            if ( s->lattice_rows && s->d.bond_layout != PDK_BONDS_COMPACT )
+            {
+                s->pushArgs();
+                s->timed( kClsBuild, [&] {
+                    if ( buildStencilSlots( s->h, s->lat_dims, s->lat_h, q.rc, s->h_perm, s->col, s->intact,
+                                            s->slice_ptr.as<int64_t>(), s->total_rows, s->col_rows_cap, s->slot,
+                                            s->scratch, s->acct(), s->st ) )
+                        s->nsten = s->slot.slots;
+                } );
+            }
+            // Mixed form on other lattices: one stencil id per bond replaces the x_j gather.
+            if ( s->slot.slots == 0 && s->d.precision == PDK_MIXED && s->lattice_rows )
+            {
+                s->pushArgs();
+                if ( !buildStencil( s->h, s->lat_h, q.rc, s->total_rows, s->sid, s->stab, s->scratch, s->acct(),
+                                    s->st, &s->nsten ) )
+                    s->nsten = 0;
+            }
         }
         s->refreshStats();
         s->built = true;
@@ -680,6 +754,8 @@ int pdk_download_bonds( pdk_spec s, int64_t capacity, int64_t* offsets, int64_t*
         std::vector<int32_t> c( n );
         std::vector<int32_t> col( static_cast<size_t>( s->total_rows ) * kSlice );
         std::vector<uint64_t> bits( s->total_rows );
+        const bool slots = s->slot.slots > 0;
+        std::vector<uint64_t> ex( slots ? s->total_rows : 0 );
         PDK_HIP( hipMemcpyAsync( sp.data(), s->slice_ptr.as<int64_t>(), ( ns + 1 ) * sizeof( int64_t ),
                                  hipMemcpyDeviceToHost, s->st ) );
         PDK_HIP( hipMemcpyAsync( c.data(), s->cnt.as<int32_t>(), n * sizeof( int32_t ), hipMemcpyDeviceToHost, s->st ) );
@@ -689,6 +765,9 @@ int pdk_download_bonds( pdk_spec s, int64_t capacity, int64_t* offsets, int64_t*
                                      hipMemcpyDeviceToHost, s->st ) );
             PDK_HIP( hipMemcpyAsync( bits.data(), s->intact.as<uint64_t>(), bits.size() * sizeof( uint64_t ),
                                      hipMemcpyDeviceToHost, s->st ) );
+            if ( slots )
+                PDK_HIP( hipMemcpyAsync( ex.data(), s->slot.exist.as<uint64_t>(), ex.size() * sizeof( uint64_t ),
+                                         hipMemcpyDeviceToHost, s->st ) );
         }
         PDK_HIP( hipStreamSynchronize( s->st ) );
         int64_t sum = 0;
@@ -702,14 +781,23 @@ int pdk_download_bonds( pdk_spec s, int64_t capacity, int64_t* offsets, int64_t*
             const int64_t slice = p / kSlice, lane = p % kSlice;
             if ( offsets )
                 offsets[cid] = off;
-            for ( int32_t q = 0; q < c[p]; q++ )
+            // compact form: slots [0, cnt); stencil-slot form: every slot of
+            // the row whose exist bit is set, in slot order
+            const int64_t qend = slots ? sp[slice + 1] - sp[slice] : c[p];
+            int64_t k = 0;
+            for ( int64_t q = 0; q < qend; q++ )
             {
                 const int64_t row = sp[slice] + q;
+                if ( slots && !( ( ex[row] >> lane ) & 1ull ) )
+                    continue;
+                require( k < c[p], "pdk_download_bonds: inconsistent bond rows" );
                 if ( nbr )
-                    nbr[off + q] = s->h_perm[col[row * kSlice + lane]];
+                    nbr[off + k] = s->h_perm[col[row * kSlice + lane]];
                 if ( intact )
-                    intact[off + q] = static_cast<uint8_t>( ( bits[row] >> lane ) & 1ull );
+                    intact[off + k] = static_cast<uint8_t>( ( bits[row] >> lane ) & 1ull );
+                k++;
             }
+            require( k == c[p], "pdk_download_bonds: inconsistent bond rows" );
             off += c[p];
         }
         if ( offsets )
@@ -926,6 +1014,83 @@ int pdk_sync( pdk_spec s )
     return guarded( [&] {
         require( s != nullptr, "null spec" );
         PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_spec_layout( pdk_spec s, int32_t* lattice_rows, int32_t* stencil_entries, int32_t* stencil_slots )
+{
+    return guarded( [&] {
+        require( s != nullptr, "null spec" );
+        if ( lattice_rows )
+            *lattice_rows = s->lattice_rows ? 1 : 0;
+        if ( stencil_entries )
+            *stencil_entries = s->nsten;
+        if ( stencil_slots )
+            *stencil_slots = s->slot.slots > 0 ? 1 : 0;
+    } );
+}
+
+int pdk_set_tuning( pdk_spec s, int32_t unroll )
+{
+    return guarded( [&] {
+        require( s != nullptr, "null spec" );
+        require( unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, "unroll must be 1, 2, 4 or 8" );
+        s->unroll = unroll;
+        if ( s->uploaded )
+        {
+            s->pushArgs();
+            PDK_HIP( hipStreamSynchronize( s->st ) );
+        }
+    } );
+}
+
+int pdk_autotune( pdk_spec s, int32_t launches, int32_t* unroll_out )
+{
+    return guarded( [&] {
+        require( s && s->inited, "call pdk_init before pdk_autotune" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        if ( isContact( s->d ) )
+        {
+            if ( unroll_out )
+                *unroll_out = s->unroll;
+            return;
+        }
+        launches = std::max( launches, 1 );
+        hipEvent_t a, b;
+        PDK_HIP( hipEventCreate( &a ) );
+        PDK_HIP( hipEventCreate( &b ) );
+        float best = 1e30f;
+        int32_t best_u = s->unroll;
+        for ( int32_t u : { 1, 2, 4, 8 } )
+        {
+            s->unroll = u;
+            s->pushArgs();
+            // warm-up launch, then the timed ones; each repeats the init force
+            // pass on the unchanged state
+            launchForce( s->dargs(), s->h, kEpiStoreOnly, s->cur, s->st );
+            PDK_HIP( hipEventRecord( a, s->st ) );
+            for ( int32_t k = 0; k < launches; k++ )
+                launchForce( s->dargs(), s->h, kEpiStoreOnly, s->cur, s->st );
+            PDK_HIP( hipEventRecord( b, s->st ) );
+            PDK_HIP( hipEventSynchronize( b ) );
+            float ms = 0.f;
+            PDK_HIP( hipEventElapsedTime( &ms, a, b ) );
+            if ( ms < best )
+            {
+                best = ms;
+                best_u = u;
+            }
+        }
+        (void)hipEventDestroy( a );
+        (void)hipEventDestroy( b );
+        s->unroll = best_u;
+        s->pushArgs();
+        // restore f as pdk_init left it (force terms on the raw force)
+        launchForce( s->dargs(), s->h, kEpiStoreOnly, s->cur, s->st );
+        launchApplyForceTerms( s->dargs(), s->h, s->st );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+        if ( unroll_out )
+            *unroll_out = best_u;
     } );
 }
 

@@ -254,28 +254,52 @@ __global__ __launch_bounds__( kThreads ) void kCountNeighbors( NeighborQuery q, 
         slice_len[slice] = m;
 }
 
-__global__ __launch_bounds__( kThreads ) void kFillNeighbors( NeighborQuery q, CellGrid g,
-                                                              const int32_t* __restrict__ cell_start,
-                                                              const int32_t* __restrict__ list,
-                                                              const int64_t* __restrict__ perm,
-                                                              const int64_t* __restrict__ inv,
-                                                              const int64_t* __restrict__ slice_ptr,
-                                                              const int32_t* __restrict__ cnt,
-                                                              int32_t* __restrict__ col,
-                                                              uint64_t* __restrict__ intact )
+// Rows of up to kFillLds neighbors are sorted in LDS: every lane appends the
+// caller ids of its neighbors to its own LDS column, then places each id at
+// its rank (number of smaller ids of the row).  Each lane touches only its own
+// column, so no barrier is needed.  Longer rows fall back to an insertion sort
+// in the output column itself.
+constexpr int kFillLds = 128;
+
+// One wavefront (one workgroup) per slice.
+__global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, CellGrid g,
+                                                            const int32_t* __restrict__ cell_start,
+                                                            const int32_t* __restrict__ list,
+                                                            const int64_t* __restrict__ perm,
+                                                            const int64_t* __restrict__ inv,
+                                                            const int64_t* __restrict__ slice_ptr,
+                                                            const int32_t* __restrict__ cnt,
+                                                            int32_t* __restrict__ col,
+                                                            uint64_t* __restrict__ intact )
 {
-    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & ( kSlice - 1 );
-    const int64_t slice = p / kSlice;
-    const int64_t nslices = ( q.n + kSlice - 1 ) / kSlice;
-    if ( slice >= nslices )
-        return; // whole wavefront
+    __shared__ int32_t keys[kFillLds * kSlice];
+    const int lane = threadIdx.x;
+    const int64_t slice = blockIdx.x;
+    const int64_t p = slice * kSlice + lane;
     const int64_t base = slice_ptr[slice];
     const int32_t len = static_cast<int32_t>( slice_ptr[slice + 1] - base );
     int32_t* row = col + base * kSlice + lane;
     const bool valid = p < q.n;
     int32_t c = 0;
-    if ( valid )
+    if ( valid && len <= kFillLds )
+    {
+        int32_t* mine = keys + lane;
+        visitNeighbors( q, g, cell_start, list, p, [&]( int64_t j ) {
+            if ( c >= len )
+                return; // cannot happen: len is the slice's maximum count
+            mine[c * kSlice] = static_cast<int32_t>( perm[j] );
+            c++;
+        } );
+        for ( int32_t a = 0; a < c; a++ )
+        {
+            const int32_t key = mine[a * kSlice];
+            int32_t r = 0;
+            for ( int32_t b = 0; b < c; b++ )
+                r += mine[b * kSlice] < key ? 1 : 0;
+            row[r * kSlice] = static_cast<int32_t>( inv[key] );
+        }
+    }
+    else if ( valid )
     {
         // Collect caller ids, insertion-sort them, map back to sorted ids.
         visitNeighbors( q, g, cell_start, list, p, [&]( int64_t j ) {
@@ -410,7 +434,7 @@ __global__ __launch_bounds__( kThreads ) void kPrenotch( PDArgs a, int32_t nnotc
     for ( int32_t k = 0; k < len; k++ )
     {
         bool cut = false;
-        if ( k < mycnt )
+        if ( valid && slotBond( a, base, k, lane, mycnt ) )
         {
             const int64_t j = a.col[( base + k ) * kSlice + lane];
             D3 xj{ { a.x[j], a.x[n + j], a.x[2 * n + j] } };
@@ -425,6 +449,142 @@ __global__ __launch_bounds__( kThreads ) void kPrenotch( PDArgs a, int32_t nnotc
         const uint64_t m = __ballot( cut );
         if ( m && lane == 0 )
             a.intact[base + k] &= ~m;
+    }
+}
+
+// Lattice stencil ids: bond (p, slot) gets the compact id of its integer
+// offset (x_j - x_i) / h, looked up in a dense (2K+1)^3 table; 255 marks
+// padding.  Any bond whose offset is not a lattice vector of the table sets
+// *flag, and the caller then drops the stencil.
+__global__ __launch_bounds__( kThreads ) void kStencilIds( PDArgs a, double hx, double hy, double hz, int32_t kx,
+                                                           int32_t ky, int32_t kz,
+                                                           const int16_t* __restrict__ dense,
+                                                           uint8_t* __restrict__ sid, int32_t* __restrict__ flag )
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t slice = p / kSlice;
+    if ( slice >= a.nslices )
+        return;
+    const bool valid = p < a.n;
+    const int64_t base = a.slice_ptr[slice];
+    const int32_t len = static_cast<int32_t>( a.slice_ptr[slice + 1] - base );
+    const int32_t mycnt = valid ? a.cnt[p] : 0;
+    const int64_t n = a.n;
+    const int64_t pc = valid ? p : slice * kSlice;
+    const double xi0 = a.x[pc], xi1 = a.x[n + pc], xi2 = a.x[2 * n + pc];
+    for ( int32_t q = 0; q < len; q++ )
+    {
+        uint8_t id = 255;
+        if ( q < mycnt )
+        {
+            const int64_t j = a.col[( base + q ) * kSlice + lane];
+            const double gx = a.x[j] - xi0, gy = a.x[n + j] - xi1, gz = a.x[2 * n + j] - xi2;
+            const double fa = rint( gx / hx ), fb = rint( gy / hy ), fc = rint( gz / hz );
+            const bool on = fabs( gx - fa * hx ) <= 1e-6 * hx && fabs( gy - fb * hy ) <= 1e-6 * hy &&
+                            fabs( gz - fc * hz ) <= 1e-6 * hz && fabs( fa ) <= kx && fabs( fb ) <= ky && fabs( fc ) <= kz;
+            int16_t c = -1;
+            if ( on )
+            {
+                const int32_t di = ( ( static_cast<int32_t>( fc ) + kz ) * ( 2 * ky + 1 ) + ( static_cast<int32_t>( fb ) + ky ) ) *
+                                       ( 2 * kx + 1 ) +
+                                   ( static_cast<int32_t>( fa ) + kx );
+                c = dense[di];
+            }
+            if ( c < 0 )
+                atomicOr( flag, 1 );
+            else
+                id = static_cast<uint8_t>( c );
+        }
+        sid[( base + q ) * kSlice + lane] = id;
+    }
+}
+
+// ---- Stencil-slot form -------------------------------------------------------
+// A full lattice in lattice-row order (device id p = (iz*ny + iy)*nx + ix)
+// whose every row lists exactly its in-box stencil entries, in one fixed
+// order, is stored with slot q = stencil entry q for every particle: the
+// neighbor of slot q is p + off[q] and a per-slot-row word marks which lanes
+// have that bond.  ent[q] = (dx, dy, dz, off).
+struct LatticeBox
+{
+    int32_t nx, ny, nz;
+};
+
+__device__ __forceinline__ void latticeCoords( int64_t p, const LatticeBox& L, int32_t& ix, int32_t& iy,
+                                               int32_t& iz )
+{
+    const int64_t t = p / L.nx;
+    ix = static_cast<int32_t>( p - t * L.nx );
+    iy = static_cast<int32_t>( t % L.ny );
+    iz = static_cast<int32_t>( t / L.ny );
+}
+
+__device__ __forceinline__ bool inBox( const int4 e, int32_t ix, int32_t iy, int32_t iz, const LatticeBox& L )
+{
+    return static_cast<uint32_t>( ix + e.x ) < static_cast<uint32_t>( L.nx ) &&
+           static_cast<uint32_t>( iy + e.y ) < static_cast<uint32_t>( L.ny ) &&
+           static_cast<uint32_t>( iz + e.z ) < static_cast<uint32_t>( L.nz );
+}
+
+// Every compact row must equal its in-box stencil entries in slot order.
+__global__ __launch_bounds__( kThreads ) void kCheckStencilSlots( PDArgs a, LatticeBox L,
+                                                                  const int4* __restrict__ ent, int32_t S,
+                                                                  int32_t* __restrict__ flag )
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( p >= a.n )
+        return;
+    int32_t ix, iy, iz;
+    latticeCoords( p, L, ix, iy, iz );
+    const int64_t slice = p / kSlice;
+    const int lane = static_cast<int>( p - slice * kSlice );
+    const int64_t base = a.slice_ptr[slice];
+    const int32_t cnt = a.cnt[p];
+    int32_t q = 0;
+    bool bad = false;
+    for ( int32_t e = 0; e < S && !bad; e++ )
+    {
+        const int4 t = ent[e];
+        if ( inBox( t, ix, iy, iz, L ) )
+        {
+            if ( q >= cnt || a.col[( base + q ) * kSlice + lane] != p + t.w )
+                bad = true;
+            q++;
+        }
+    }
+    if ( bad || q != cnt )
+        *flag = 1;
+}
+
+// Fill the slot-form arrays: col (neighbor id or self), exist and intact words.
+__global__ __launch_bounds__( kThreads ) void kFillStencilSlots( int64_t n, int64_t nslices, LatticeBox L,
+                                                                 const int4* __restrict__ ent, int32_t S,
+                                                                 int32_t* __restrict__ col,
+                                                                 uint64_t* __restrict__ exist,
+                                                                 uint64_t* __restrict__ intact )
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t slice = p / kSlice;
+    if ( slice >= nslices )
+        return; // whole wavefront
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const bool valid = p < n;
+    int32_t ix = 0, iy = 0, iz = 0;
+    if ( valid )
+        latticeCoords( p, L, ix, iy, iz );
+    const int32_t self = static_cast<int32_t>( valid ? p : slice * kSlice );
+    for ( int32_t e = 0; e < S; e++ )
+    {
+        const int4 t = ent[e];
+        const bool ex = valid && inBox( t, ix, iy, iz, L );
+        col[( slice * S + e ) * kSlice + lane] = ex ? static_cast<int32_t>( p + t.w ) : self;
+        const uint64_t m = __ballot( ex );
+        if ( lane == 0 )
+        {
+            exist[slice * S + e] = m;
+            intact[slice * S + e] = m;
+        }
     }
 }
 
@@ -457,6 +617,76 @@ void launchMortonOrder( const double* x_aos_dev, int64_t n, const double lo[3], 
                                                  static_cast<int>( n ), 0, 63, st ) );
     kInvertPerm<<<gridFor( n ), kThreads, 0, st>>>( perm_dev, n, inv_dev );
     PDK_HIP( hipGetLastError() );
+}
+
+void launchInvertPerm( const int64_t* perm_dev, int64_t n, int64_t* inv_dev, hipStream_t st )
+{
+    if ( n == 0 )
+        return;
+    kInvertPerm<<<gridFor( n ), kThreads, 0, st>>>( perm_dev, n, inv_dev );
+    PDK_HIP( hipGetLastError() );
+}
+
+bool latticeOrder( const double* x, int64_t n, std::vector<int64_t>& perm, double h_out[3], int64_t dims_out[3] )
+{
+    if ( n < 2 )
+        return false;
+    double lo[3], h[3];
+    int64_t nk[3] = { 1, 1, 1 };
+    for ( int d = 0; d < 3; d++ )
+    {
+        lo[d] = x[d];
+        h[d] = 0.0;
+        for ( int64_t p = 0; p < n; p++ )
+            lo[d] = std::min( lo[d], x[3 * p + d] );
+        double best = 0.0;
+        for ( int64_t p = 1; p < n; p++ )
+        {
+            const double dd = std::fabs( x[3 * p + d] - x[3 * ( p - 1 ) + d] );
+            if ( dd > 0.0 && ( best == 0.0 || dd < best ) )
+                best = dd;
+        }
+        h[d] = best > 0.0 ? best : 1.0;
+    }
+    std::vector<int64_t> k( 3 * n );
+    for ( int64_t p = 0; p < n; p++ )
+        for ( int d = 0; d < 3; d++ )
+        {
+            const double q = ( x[3 * p + d] - lo[d] ) / h[d];
+            const double r = std::nearbyint( q );
+            if ( !( std::fabs( q - r ) <= 1e-6 ) || r > 2e6 )
+                return false;
+            k[3 * p + d] = static_cast<int64_t>( r );
+            nk[d] = std::max( nk[d], k[3 * p + d] + 1 );
+        }
+    // spacing from the extent (averages out the rounding of single steps)
+    for ( int d = 0; d < 3; d++ )
+    {
+        double hi = lo[d];
+        for ( int64_t p = 0; p < n; p++ )
+            hi = std::max( hi, x[3 * p + d] );
+        h_out[d] = nk[d] > 1 ? ( hi - lo[d] ) / static_cast<double>( nk[d] - 1 ) : h[d];
+        dims_out[d] = nk[d];
+    }
+    std::vector<uint64_t> key( n );
+    bool increasing = true;
+    for ( int64_t p = 0; p < n; p++ )
+    {
+        key[p] = static_cast<uint64_t>( ( k[3 * p + 2] * nk[1] + k[3 * p + 1] ) * nk[0] + k[3 * p] );
+        if ( p > 0 && key[p] <= key[p - 1] )
+            increasing = false;
+    }
+    perm.resize( n );
+    for ( int64_t p = 0; p < n; p++ )
+        perm[p] = p;
+    if ( !increasing )
+    {
+        std::stable_sort( perm.begin(), perm.end(), [&]( int64_t a, int64_t b ) { return key[a] < key[b]; } );
+        for ( int64_t p = 1; p < n; p++ )
+            if ( key[perm[p]] == key[perm[p - 1]] )
+                return false; // two particles on one lattice site
+    }
+    return true;
 }
 
 void launchGatherAos3( const double* src_aos, const int64_t* perm, double* dst_soa, int64_t n, hipStream_t st )
@@ -581,8 +811,9 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
         intact.alloc( static_cast<size_t>( cap ) * sizeof( uint64_t ), acct );
         col_capacity_rows = cap;
     }
-    kFillNeighbors<<<gslices, kThreads, 0, st>>>( q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), perm,
-                                                   inv, slice_ptr, cnt, col.as<int32_t>(), intact.as<uint64_t>() );
+    kFillNeighbors<<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>( q, g, s.cell_start.as<int32_t>(),
+                                                                          s.cell_list.as<int32_t>(), perm, inv, slice_ptr,
+                                                                          cnt, col.as<int32_t>(), intact.as<uint64_t>() );
     PDK_HIP( hipGetLastError() );
     if ( max_cnt_out )
     {
@@ -596,6 +827,178 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
         *max_cnt_out = static_cast<int32_t>( m );
     }
     return total_rows;
+}
+
+bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_rows, DevBuf& sid, DevBuf& table,
+                   BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out )
+{
+    int32_t K[3];
+    for ( int d = 0; d < 3; d++ )
+    {
+        if ( !( h[d] > 0 ) )
+            return false;
+        K[d] = static_cast<int32_t>( std::ceil( rc / h[d] ) );
+        if ( K[d] > 40 )
+            return false;
+    }
+    const int32_t D[3] = { 2 * K[0] + 1, 2 * K[1] + 1, 2 * K[2] + 1 };
+    std::vector<int16_t> dense( static_cast<size_t>( D[0] ) * D[1] * D[2], -1 );
+    std::vector<float4> tab;
+    const double rc2 = rc * rc * ( 1.0 + 1e-9 );
+    for ( int32_t c = -K[2]; c <= K[2]; c++ )
+        for ( int32_t b = -K[1]; b <= K[1]; b++ )
+            for ( int32_t aa = -K[0]; aa <= K[0]; aa++ )
+            {
+                const double gx = aa * h[0], gy = b * h[1], gz = c * h[2];
+                const double r2 = gx * gx + gy * gy + gz * gz;
+                if ( r2 == 0.0 || r2 > rc2 )
+                    continue;
+                if ( tab.size() >= 255 )
+                    return false;
+                dense[( ( c + K[2] ) * D[1] + ( b + K[1] ) ) * D[0] + ( aa + K[0] )] = static_cast<int16_t>( tab.size() );
+                tab.push_back( make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
+                                            static_cast<float>( std::sqrt( r2 ) ) ) );
+            }
+    while ( tab.size() < 256 )
+        tab.push_back( make_float4( 0.f, 0.f, 0.f, 1.f ) );
+    DevBuf ddense;
+    ddense.alloc( dense.size() * sizeof( int16_t ), acct );
+    ensure<int32_t>( s.flag, 1, acct );
+    PDK_HIP( hipMemcpyAsync( ddense.as<int16_t>(), dense.data(), dense.size() * sizeof( int16_t ), hipMemcpyHostToDevice, st ) );
+    PDK_HIP( hipMemsetAsync( s.flag.as<int32_t>(), 0, sizeof( int32_t ), st ) );
+    sid.alloc( static_cast<size_t>( std::max<int64_t>( total_rows, 1 ) ) * kSlice, acct );
+    table.alloc( 256 * sizeof( float4 ), acct );
+    PDK_HIP( hipMemcpyAsync( table.as<float4>(), tab.data(), 256 * sizeof( float4 ), hipMemcpyHostToDevice, st ) );
+    const unsigned g = static_cast<unsigned>( ( a.nslices * kSlice + kThreads - 1 ) / kThreads );
+    kStencilIds<<<g, kThreads, 0, st>>>( a, h[0], h[1], h[2], K[0], K[1], K[2], ddense.as<int16_t>(), sid.as<uint8_t>(),
+                                         s.flag.as<int32_t>() );
+    PDK_HIP( hipGetLastError() );
+    int32_t flag = 0;
+    PDK_HIP( hipMemcpyAsync( &flag, s.flag.as<int32_t>(), sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
+    PDK_HIP( hipStreamSynchronize( st ) );
+    if ( flag )
+    {
+        sid.release();
+        table.release();
+        return false;
+    }
+    if ( nsten_out )
+    {
+        int32_t cnt = 0;
+        for ( const float4& t : tab )
+            cnt += ( t.w != 1.f || t.x != 0.f || t.y != 0.f || t.z != 0.f ) ? 1 : 0;
+        *nsten_out = cnt;
+    }
+    return true;
+}
+
+bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3], double rc,
+                        const std::vector<int64_t>& perm_host, DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,
+                        int64_t& total_rows, int64_t& col_capacity_rows, SlotForm& out, BuildScratch& s,
+                        MemAccount* acct, hipStream_t st )
+{
+    const int64_t n = a.n;
+    if ( dims[0] * dims[1] * dims[2] != n || static_cast<int64_t>( perm_host.size() ) != n )
+        return false;
+    int32_t K[3];
+    for ( int d = 0; d < 3; d++ )
+    {
+        if ( !( h[d] > 0 ) )
+            return false;
+        // largest integer offset along d that can be within the cutoff
+        K[d] = static_cast<int32_t>( std::floor( rc / h[d] * ( 1.0 + 1e-9 ) ) );
+        // an interior particle (full stencil) must exist to fix the slot order
+        if ( K[d] > 40 || dims[d] <= 2 * K[d] )
+            return false;
+    }
+    const LatticeBox L{ static_cast<int32_t>( dims[0] ), static_cast<int32_t>( dims[1] ),
+                        static_cast<int32_t>( dims[2] ) };
+    struct Entry
+    {
+        int32_t a, b, c;
+        int64_t off;
+        double r2;
+    };
+    std::vector<Entry> es;
+    const double rc2 = rc * rc;
+    for ( int32_t c = -K[2]; c <= K[2]; c++ )
+        for ( int32_t b = -K[1]; b <= K[1]; b++ )
+            for ( int32_t aa = -K[0]; aa <= K[0]; aa++ )
+            {
+                const double gx = aa * h[0], gy = b * h[1], gz = c * h[2];
+                const double r2 = gx * gx + gy * gy + gz * gz;
+                if ( r2 == 0.0 )
+                    continue;
+                // a lattice vector this close to the cutoff is decided by the
+                // rounding of the actual positions, particle by particle
+                if ( std::fabs( r2 - rc2 ) <= 1e-6 * rc2 )
+                    return false;
+                if ( r2 > rc2 )
+                    continue;
+                es.push_back( { aa, b, c, ( static_cast<int64_t>( c ) * L.ny + b ) * L.nx + aa, r2 } );
+            }
+    const int32_t S = static_cast<int32_t>( es.size() );
+    if ( S == 0 || S > 255 )
+        return false;
+    // slot order: ascending caller id of an interior particle's neighbors
+    const int64_t p0 = ( static_cast<int64_t>( K[2] ) * L.ny + K[1] ) * L.nx + K[0];
+    std::stable_sort( es.begin(), es.end(), [&]( const Entry& u, const Entry& v ) {
+        return perm_host[p0 + u.off] < perm_host[p0 + v.off];
+    } );
+    std::vector<int4> ent( S );
+    for ( int32_t e = 0; e < S; e++ )
+        ent[e] = make_int4( es[e].a, es[e].b, es[e].c, static_cast<int32_t>( es[e].off ) );
+    DevBuf dent;
+    dent.alloc( S * sizeof( int4 ), acct );
+    ensure<int32_t>( s.flag, 1, acct );
+    PDK_HIP( hipMemcpyAsync( dent.as<int4>(), ent.data(), S * sizeof( int4 ), hipMemcpyHostToDevice, st ) );
+    PDK_HIP( hipMemsetAsync( s.flag.as<int32_t>(), 0, sizeof( int32_t ), st ) );
+    kCheckStencilSlots<<<gridFor( n ), kThreads, 0, st>>>( a, L, dent.as<int4>(), S, s.flag.as<int32_t>() );
+    PDK_HIP( hipGetLastError() );
+    int32_t flag = 0;
+    PDK_HIP( hipMemcpyAsync( &flag, s.flag.as<int32_t>(), sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
+    PDK_HIP( hipStreamSynchronize( st ) );
+    if ( flag )
+        return false;
+
+    // Rebuild the slot arrays.
+    const int64_t nslices = a.nslices;
+    const int64_t rows = nslices * S;
+    DevBuf ncol, nintact, nexist, dsoff, dtab;
+    ncol.alloc( static_cast<size_t>( rows ) * kSlice * sizeof( int32_t ), acct );
+    nintact.alloc( static_cast<size_t>( rows ) * sizeof( uint64_t ), acct );
+    nexist.alloc( static_cast<size_t>( rows ) * sizeof( uint64_t ), acct );
+    kFillStencilSlots<<<gridFor( nslices * kSlice ), kThreads, 0, st>>>( n, nslices, L, dent.as<int4>(), S,
+                                                                         ncol.as<int32_t>(), nexist.as<uint64_t>(),
+                                                                         nintact.as<uint64_t>() );
+    PDK_HIP( hipGetLastError() );
+    std::vector<int64_t> sp( nslices + 1 );
+    for ( int64_t k = 0; k <= nslices; k++ )
+        sp[k] = k * S;
+    PDK_HIP( hipMemcpyAsync( slice_ptr_dev, sp.data(), ( nslices + 1 ) * sizeof( int64_t ), hipMemcpyHostToDevice, st ) );
+    std::vector<int32_t> soff( 256, 0 );
+    std::vector<float4> tab( 256, make_float4( 0.f, 0.f, 0.f, 1.f ) );
+    for ( int32_t e = 0; e < S; e++ )
+    {
+        soff[e] = ent[e].w;
+        const double gx = es[e].a * h[0], gy = es[e].b * h[1], gz = es[e].c * h[2];
+        tab[e] = make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
+                              static_cast<float>( std::sqrt( es[e].r2 ) ) );
+    }
+    dsoff.alloc( 256 * sizeof( int32_t ), acct );
+    dtab.alloc( 256 * sizeof( float4 ), acct );
+    PDK_HIP( hipMemcpyAsync( dsoff.as<int32_t>(), soff.data(), 256 * sizeof( int32_t ), hipMemcpyHostToDevice, st ) );
+    PDK_HIP( hipMemcpyAsync( dtab.as<float4>(), tab.data(), 256 * sizeof( float4 ), hipMemcpyHostToDevice, st ) );
+    PDK_HIP( hipStreamSynchronize( st ) );
+    col = std::move( ncol );
+    intact = std::move( nintact );
+    out.exist = std::move( nexist );
+    out.soff = std::move( dsoff );
+    out.table = std::move( dtab );
+    out.slots = S;
+    total_rows = rows;
+    col_capacity_rows = rows;
+    return true;
 }
 
 void launchPrenotch( const PDArgs& a, int32_t nnotch, const double* notch_dev, hipStream_t st )

@@ -60,6 +60,12 @@ struct PDArgs
     uint64_t* intact;
     const uint8_t* sid;     // stencil ids or null
     const float4* stencil;  // stencil table (xi_x, xi_y, xi_z, |xi|) or null
+    // Stencil-slot form (full lattices): slot q of every slice is stencil
+    // entry q; neighbor j = p + soff[q]; bit lane of exist[slice * S + q] says
+    // whether the bond exists.  The stencil table is then indexed by q.
+    // Null in the compact form, where slot q holds a bond iff q < cnt[p].
+    const uint64_t* exist;
+    const int32_t* soff;
     double* m;
     double* theta;
     double* lpsA;           // LPS: theta_coeff * theta / m per particle
@@ -85,9 +91,27 @@ struct PDArgs
     int32_t bc_kind;        // -1 none
     int32_t polydisperse;
     int32_t parity_offset;  // batch: current U index = launch parity ^ parity_offset
-    int32_t pad0;
+    int32_t unroll;         // force pass: bond slots loaded per group (1, 2, 4, 8)
     int64_t end_launch;     // batch: launch index of this spec's last step
+    // Exact shortcuts: every vol[j] equals vol_value / no particle is no-fail.
+    int32_t vol_uniform;
+    int32_t any_nofail;
+    double vol_value;
+    int32_t stencil_slots;  // S in the stencil-slot form, 0 in the compact form
+    int32_t pad0;
 };
+
+// Bond-slot walk shared by both layouts: a lane visits slots [0, slotEnd) and
+// slot q holds one of its bonds iff slotBond (q < cnt in the compact form,
+// the exist bit in the stencil-slot form).  row0 = slice_ptr[slice].
+__device__ __forceinline__ int32_t slotEnd( const PDArgs& A, int32_t len, int32_t mycnt )
+{
+    return A.exist ? len : mycnt;
+}
+__device__ __forceinline__ bool slotBond( const PDArgs& A, int64_t row0, int32_t q, int lane, int32_t mycnt )
+{
+    return A.exist ? ( ( ( A.exist[row0 + q] >> lane ) & 1ull ) != 0 ) : ( q < mycnt );
+}
 
 // Block -> logical slice group, keeping consecutive slice groups on one XCD
 // so that their shared halo stays in that XCD's L2.

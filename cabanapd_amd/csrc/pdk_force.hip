@@ -19,6 +19,7 @@
 //           re-tested with the FP64 expression, so break decisions equal the
 //           FP64 decisions for the same state.
 #include <cmath>
+#include <cstdio>
 
 #include "pdk_internal.hpp"
 
@@ -101,6 +102,27 @@ __device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi
     const float num = 2.0f * xe + ee;
     b.s = num * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
     b.etan = __builtin_sqrtf( ee ) * xinv;
+}
+
+// Same stretch with the bond vector from the lattice stencil table
+// (g.xyz = FP32 xi, g.w = |xi|): no x_j gather.
+__device__ __forceinline__ void getDistance32s( const float4 g, double ui0, double ui1, double ui2, double uj0,
+                                                double uj1, double uj2, Bond32& b )
+{
+    const float ex = static_cast<float>( uj0 - ui0 );
+    const float ey = static_cast<float>( uj1 - ui1 );
+    const float ez = static_cast<float>( uj2 - ui2 );
+    b.rx = g.x + ex;
+    b.ry = g.y + ey;
+    b.rz = g.z + ez;
+    const float r2 = b.rx * b.rx + b.ry * b.ry + b.rz * b.rz;
+    b.rinv = __builtin_amdgcn_rsqf( r2 );
+    const float r = r2 * b.rinv;
+    b.xi = g.w;
+    const float xe = g.x * ex + g.y * ey + g.z * ez;
+    const float ee = ex * ex + ey * ey + ez * ez;
+    b.s = ( 2.0f * xe + ee ) * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
+    b.etan = __builtin_sqrtf( ee ) * __builtin_amdgcn_rcpf( b.xi );
 }
 
 // Uniform per-wave access to the spec of this wave's slice.
@@ -200,7 +222,12 @@ __device__ __forceinline__ void epilogue( const PDArgs& A, int64_t p, int32_t ep
 // per particle (bitwise the per-bond subexpressions of the reference's
 // ForceCoeffTag, force_models/CabanaPD_LPS.hpp:131-133).
 // ---------------------------------------------------------------------------
-template <int MODEL, bool FRAC, int PREC>
+// SLOTS selects the stencil-slot form (neighbor ids from the slot offsets,
+// existence bits per slot row) over the compact form (neighbor id stream).
+// ---------------------------------------------------------------------------
+constexpr int kBatch = 16; // bond slots whose ids / bit words are fetched ahead
+
+template <int MODEL, bool FRAC, int PREC, int UNR, bool SLOTS>
 __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ specs,
                                                     const int32_t* __restrict__ tile_spec,
                                                     const int64_t* __restrict__ tile_slice,
@@ -209,7 +236,26 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
 {
 #pragma clang fp contract( off )
     WaveCtx w;
-    if ( !waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w ) )
+    const bool active = waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w );
+    // All waves of a block belong to one spec: stage its stencil table (and
+    // slot offsets) in LDS before any wave leaves.
+    __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
+    __shared__ int32_t soffs[SLOTS ? 256 + kBatch : 1];
+    const bool useSten = ( PREC == kMixed ) && ( SLOTS || w.A->sid != nullptr );
+    if constexpr ( SLOTS )
+    {
+        soffs[threadIdx.x] = w.A->soff[threadIdx.x];
+        if ( threadIdx.x < kBatch )
+            soffs[256 + threadIdx.x] = 0;
+    }
+    if constexpr ( PREC == kMixed )
+    {
+        if ( useSten )
+            stab[threadIdx.x] = w.A->stencil[threadIdx.x];
+    }
+    if constexpr ( SLOTS || PREC == kMixed )
+        __syncthreads();
+    if ( !active )
         return;
     const PDArgs& A = *w.A;
     const int lane = threadIdx.x & ( kSlice - 1 );
@@ -221,10 +267,13 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     const double* __restrict__ u = A.U[w.cur];
     const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
     const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
+    const uint8_t* __restrict__ sidp = nullptr;
     const int32_t mycnt = valid ? A.cnt[p] : 0;
     const int64_t base = A.slice_ptr[w.slice];
     const int32_t len = static_cast<int32_t>( A.slice_ptr[w.slice + 1] - base );
     const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
+    if ( useSten && !SLOTS )
+        sidp = A.sid + base * kSlice + lane;
     uint64_t* __restrict__ bits = A.intact + base;
     const double* __restrict__ vol = A.vol;
     const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
@@ -240,6 +289,10 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     const double* __restrict__ Barr = A.lpsB;
     const int infl1 = A.influence;
 
+    const bool uvol = A.vol_uniform != 0;
+    const double volu = A.vol_value;
+    const bool any_nofail = A.any_nofail != 0;
+
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if ( accumulate && valid )
     {
@@ -247,90 +300,238 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         fy = A.f[n + p];
         fz = A.f[2 * n + p];
     }
-    for ( int32_t q = 0; q < len; q++ )
-    {
-        const int64_t j = colp[q * kSlice];
-        uint64_t word = 0;
-        if constexpr ( FRAC )
-            word = bits[q];
-        bool brk = false;
-        if ( q < mycnt )
+    // The neighbor ids and broken-bond words of kBatch slots are fetched one
+    // batch ahead (the index stream comes from HBM; keeping a batch in
+    // flight hides its latency).  Within a batch, slots are processed UNR at
+    // a time: the gathers of a group are issued together, then its bonds are
+    // evaluated in slot order (the summation order is unchanged).
+    static_assert( kBatch % UNR == 0, "batch must hold whole groups" );
+    int32_t jb[SLOTS ? 1 : kBatch], jn[SLOTS ? 1 : kBatch];
+    uint32_t sb[kBatch / 4], sn[kBatch / 4]; // stencil ids, four per word
+    uint32_t wlo = 0, whi = 0, nlo = 0, nhi = 0;
+    uint32_t elo = 0, ehi = 0, mlo = 0, mhi = 0; // existence words (stencil-slot form)
+    const int32_t self = static_cast<int32_t>( pc );
+    const uint64_t* __restrict__ exist = SLOTS ? A.exist + base : nullptr;
+    auto fetch = [&]( int32_t qb, int32_t( &dst )[SLOTS ? 1 : kBatch], uint32_t( &sdst )[kBatch / 4], uint32_t& lo,
+                      uint32_t& hi, uint32_t& xlo, uint32_t& xhi ) {
+        if constexpr ( SLOTS )
         {
-            const double xj0 = x[j], xj1 = x[n + j], xj2 = x[2 * n + j];
-            const double uj0 = u[j], uj1 = u[n + j], uj2 = u[2 * n + j];
-            bool intact = true;
-            if constexpr ( FRAC )
-                intact = ( word >> lane ) & 1ull;
-            if constexpr ( PREC == kF64 )
+            // lane k holds the existence word of slot qb + k
+            const uint64_t xv = ( lane < kBatch && qb + lane < len ) ? exist[qb + lane] : 0ull;
+            xlo = static_cast<uint32_t>( xv );
+            xhi = static_cast<uint32_t>( xv >> 32 );
+        }
+        else
+        {
+#pragma unroll
+            for ( int k = 0; k < kBatch; k++ )
+                dst[k] = ( qb + k < len ) ? colp[( qb + k ) * kSlice] : self;
+        }
+        if constexpr ( PREC == kMixed && !SLOTS )
+        {
+            if ( useSten )
             {
-                Bond64 b;
-                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
-                if constexpr ( FRAC )
+#pragma unroll
+                for ( int k4 = 0; k4 < kBatch / 4; k4++ )
                 {
-                    const bool nf_j = ( MODEL == kLPS ) ? nf_i : ( A.nofail[j] != 0 );
-                    if ( criticalStretch64( b.r, b.xi, bcc ) && !nf_i && !nf_j )
-                        brk = true;
-                }
-                if ( !brk && intact )
-                {
-                    double coeff;
-                    if constexpr ( MODEL == kPMB )
-                        coeff = c * b.s * vol[j];
-                    else
+                    uint32_t pk = 0;
+#pragma unroll
+                    for ( int e = 0; e < 4; e++ )
                     {
-                        const double infl = infl1 ? 1.0 / b.xi : 1.0;
-                        coeff = ( Ai + Aarr[j] + b.s * ( Bi + Barr[j] ) ) * infl * b.xi * vol[j];
+                        const int32_t q = qb + 4 * k4 + e;
+                        const uint32_t v = ( q < len ) ? sidp[q * kSlice] : 0u;
+                        pk |= v << ( 8 * e );
                     }
-                    fx += coeff * b.rx / b.r;
-                    fy += coeff * b.ry / b.r;
-                    fz += coeff * b.rz / b.r;
+                    sdst[k4] = pk;
                 }
+            }
+        }
+        if constexpr ( FRAC )
+        {
+            // lane k holds the word of slot qb + k
+            const uint64_t wv = ( lane < kBatch && qb + lane < len ) ? bits[qb + lane] : 0ull;
+            lo = static_cast<uint32_t>( wv );
+            hi = static_cast<uint32_t>( wv >> 32 );
+        }
+    };
+    fetch( 0, jb, sb, wlo, whi, elo, ehi );
+    for ( int32_t qb = 0; qb < len; qb += kBatch )
+    {
+        if ( qb + kBatch < len )
+            fetch( qb + kBatch, jn, sn, nlo, nhi, mlo, mhi );
+#pragma unroll
+        for ( int g = 0; g < kBatch; g += UNR )
+        {
+        const int32_t q0 = qb + g;
+        if ( q0 >= len )
+            break;
+        int64_t jj[UNR];
+        uint64_t ww[UNR];
+        bool ex[UNR];
+#pragma unroll
+        for ( int k = 0; k < UNR; k++ )
+        {
+            if constexpr ( SLOTS )
+            {
+                const uint32_t l32 = __builtin_amdgcn_readlane( elo, g + k );
+                const uint32_t h32 = __builtin_amdgcn_readlane( ehi, g + k );
+                const uint64_t e = ( static_cast<uint64_t>( h32 ) << 32 ) | l32;
+                ex[k] = ( e >> lane ) & 1ull;
+                jj[k] = ex[k] ? p + soffs[q0 + k] : pc;
             }
             else
             {
-                Bond32 b;
-                getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
-                if constexpr ( FRAC )
+                jj[k] = jb[g + k];
+                ex[k] = q0 + k < mycnt;
+            }
+            ww[k] = 0;
+            if constexpr ( FRAC )
+            {
+                const uint32_t l32 = __builtin_amdgcn_readlane( wlo, g + k );
+                const uint32_t h32 = __builtin_amdgcn_readlane( whi, g + k );
+                ww[k] = ( static_cast<uint64_t>( h32 ) << 32 ) | l32;
+            }
+        }
+        double xj[UNR][3], uj[UNR][3], vj[UNR], aj[UNR], bj[UNR];
+        bool nfj[UNR];
+#pragma unroll
+        for ( int k = 0; k < UNR; k++ )
+        {
+            if ( ex[k] )
+            {
+                const int64_t j = jj[k];
+                if ( !useSten )
                 {
-                    const bool nf_j = ( MODEL == kLPS ) ? nf_i : ( A.nofail[j] != 0 );
-                    if ( !nf_i && !nf_j )
-                    {
-                        const float band = A.bandf * ( b.etan + fabsf( b.s ) + A.s0f );
-                        if ( fabsf( b.s - A.s0f ) <= band )
-                        {
-                            Bond64 e;
-                            getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, e );
-                            brk = criticalStretch64( e.r, e.xi, bcc );
-                        }
-                        else
-                            brk = b.s > A.s0f;
-                    }
+                    xj[k][0] = x[j];
+                    xj[k][1] = x[n + j];
+                    xj[k][2] = x[2 * n + j];
                 }
-                if ( !brk && intact )
+                uj[k][0] = u[j];
+                uj[k][1] = u[n + j];
+                uj[k][2] = u[2 * n + j];
+                vj[k] = uvol ? volu : vol[j];
+                nfj[k] = ( FRAC && MODEL == kPMB && any_nofail ) ? ( A.nofail[j] != 0 ) : false;
+                if constexpr ( MODEL == kLPS )
                 {
-                    float coeff;
-                    if constexpr ( MODEL == kPMB )
-                        coeff = static_cast<float>( c ) * b.s * static_cast<float>( vol[j] );
-                    else
-                    {
-                        const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
-                        coeff = static_cast<float>( Ai + Aarr[j] ) +
-                                b.s * static_cast<float>( Bi + Barr[j] );
-                        coeff = coeff * infl * b.xi * static_cast<float>( vol[j] );
-                    }
-                    const float fr = coeff * b.rinv;
-                    fx += static_cast<double>( fr * b.rx );
-                    fy += static_cast<double>( fr * b.ry );
-                    fz += static_cast<double>( fr * b.rz );
+                    aj[k] = Aarr[j];
+                    bj[k] = Barr[j];
                 }
             }
         }
-        if constexpr ( FRAC )
+#pragma unroll
+        for ( int k = 0; k < UNR; k++ )
         {
-            const uint64_t m = __ballot( brk );
-            if ( ( m & word ) && lane == 0 )
-                bits[q] = word & ~m;
+            const int32_t q = q0 + k;
+            bool brk = false;
+            if ( ex[k] )
+            {
+                bool intact = true;
+                if constexpr ( FRAC )
+                    intact = ( ww[k] >> lane ) & 1ull;
+                const bool nf_j = ( MODEL == kLPS ) ? nf_i : nfj[k];
+                if constexpr ( PREC == kF64 )
+                {
+                    Bond64 b;
+                    getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0], uj[k][1],
+                                   uj[k][2], b );
+                    if constexpr ( FRAC )
+                    {
+                        if ( criticalStretch64( b.r, b.xi, bcc ) && !nf_i && !nf_j )
+                            brk = true;
+                    }
+                    if ( !brk && intact )
+                    {
+                        double coeff;
+                        if constexpr ( MODEL == kPMB )
+                            coeff = c * b.s * vj[k];
+                        else
+                        {
+                            const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                            coeff = ( Ai + aj[k] + b.s * ( Bi + bj[k] ) ) * infl * b.xi * vj[k];
+                        }
+                        fx += coeff * b.rx / b.r;
+                        fy += coeff * b.ry / b.r;
+                        fz += coeff * b.rz / b.r;
+                    }
+                }
+                else
+                {
+                    Bond32 b;
+                    if ( useSten )
+                    {
+                        const uint32_t id =
+                            SLOTS ? static_cast<uint32_t>( q ) : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
+                        getDistance32s( stab[id], ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2], b );
+                    }
+                    else
+                        getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
+                                       uj[k][1], uj[k][2], b );
+                    if constexpr ( FRAC )
+                    {
+                        if ( !nf_i && !nf_j )
+                        {
+                            const float band = A.bandf * ( b.etan + fabsf( b.s ) + A.s0f );
+                            if ( fabsf( b.s - A.s0f ) <= band )
+                            {
+                                // guard band: the reference's FP64 test on this bond
+                                const int64_t j = jj[k];
+                                if ( useSten )
+                                {
+                                    xj[k][0] = x[j];
+                                    xj[k][1] = x[n + j];
+                                    xj[k][2] = x[2 * n + j];
+                                }
+                                Bond64 e;
+                                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
+                                               uj[k][1], uj[k][2], e );
+                                brk = criticalStretch64( e.r, e.xi, bcc );
+                            }
+                            else
+                                brk = b.s > A.s0f;
+                        }
+                    }
+                    if ( !brk && intact )
+                    {
+                        float coeff;
+                        if constexpr ( MODEL == kPMB )
+                            coeff = static_cast<float>( c ) * b.s * static_cast<float>( vj[k] );
+                        else
+                        {
+                            const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
+                            coeff = static_cast<float>( Ai + aj[k] ) + b.s * static_cast<float>( Bi + bj[k] );
+                            coeff = coeff * infl * b.xi * static_cast<float>( vj[k] );
+                        }
+                        const float fr = coeff * b.rinv;
+                        fx += static_cast<double>( fr * b.rx );
+                        fy += static_cast<double>( fr * b.ry );
+                        fz += static_cast<double>( fr * b.rz );
+                    }
+                }
+            }
+            if constexpr ( FRAC )
+            {
+                if ( q < len )
+                {
+                    const uint64_t m = __ballot( brk );
+                    if ( ( m & ww[k] ) && lane == 0 )
+                        bits[q] = ww[k] & ~m;
+                }
+            }
         }
+        } // group
+        if constexpr ( !SLOTS )
+        {
+#pragma unroll
+            for ( int k = 0; k < kBatch; k++ )
+                jb[k] = jn[k];
+        }
+#pragma unroll
+        for ( int k4 = 0; k4 < kBatch / 4; k4++ )
+            sb[k4] = sn[k4];
+        wlo = nlo;
+        whi = nhi;
+        elo = mlo;
+        ehi = mhi;
     }
     if ( !valid )
         return;
@@ -367,6 +568,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
     const double ui0 = u[p], ui1 = u[n + p], ui2 = u[2 * n + p];
     const int32_t mycnt = A.cnt[p];
     const int64_t base = A.slice_ptr[w.slice];
+    const int32_t qend = slotEnd( A, static_cast<int32_t>( A.slice_ptr[w.slice + 1] - base ), mycnt );
     const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
     const uint64_t* __restrict__ bits = A.intact + base;
     const double* __restrict__ vol = A.vol;
@@ -376,8 +578,10 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
     if ( do_m )
     {
         m_i = 0.0;
-        for ( int32_t q = 0; q < mycnt; q++ )
+        for ( int32_t q = 0; q < qend; q++ )
         {
+            if ( !slotBond( A, base, q, lane, mycnt ) )
+                continue;
             if constexpr ( FRAC )
                 if ( !( ( bits[q] >> lane ) & 1ull ) )
                     continue;
@@ -410,8 +614,10 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
     const bool mpos = FRAC ? ( m_i > 0 ) : true;
     if ( mpos )
     {
-        for ( int32_t q = 0; q < mycnt; q++ )
+        for ( int32_t q = 0; q < qend; q++ )
         {
+            if ( !slotBond( A, base, q, lane, mycnt ) )
+                continue;
             if constexpr ( FRAC )
                 if ( !( ( bits[q] >> lane ) & 1ull ) )
                     continue;
@@ -473,6 +679,7 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
         const double ui0 = u[p], ui1 = u[n + p], ui2 = u[2 * n + p];
         const int32_t mycnt = A.cnt[p];
         const int64_t base = A.slice_ptr[slice];
+        const int32_t qend = slotEnd( A, static_cast<int32_t>( A.slice_ptr[slice + 1] - base ), mycnt );
         const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
         const uint64_t* __restrict__ bits = A.intact + base;
         const double* __restrict__ vol = A.vol;
@@ -485,16 +692,18 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
             theta_i = thetavals[p];
             if constexpr ( FRAC )
             {
-                for ( int32_t q = 0; q < mycnt; q++ )
-                    num_bonds += static_cast<double>( ( bits[q] >> lane ) & 1ull );
+                for ( int32_t q = 0; q < qend; q++ )
+                    num_bonds += static_cast<double>( ( bits[q] >> lane ) & 1ull ); // 0 on empty slots
             }
             else
                 num_bonds = static_cast<double>( mycnt );
         }
         double W = 0.0, phi_i = 0.0, vol_H_i = 0.0;
         const double vol_i = vol[p];
-        for ( int32_t q = 0; q < mycnt; q++ )
+        for ( int32_t q = 0; q < qend; q++ )
         {
+            if ( !slotBond( A, base, q, lane, mycnt ) )
+                continue;
             const int64_t j = colp[q * kSlice];
             Bond64 b;
             getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
@@ -600,31 +809,55 @@ __global__ __launch_bounds__( 256 ) void kApplyForceTerms( const PDArgs* __restr
 
 using ForceFn = void ( * )( const PDArgs*, const int32_t*, const int64_t*, int32_t, int32_t, int64_t, int32_t );
 
-ForceFn pickForce( const PDArgs& a )
+template <int MODEL, bool FRAC, int PREC, bool SLOTS>
+ForceFn pickUnroll( int unroll )
+{
+    switch ( unroll )
+    {
+    case 1: return kForce<MODEL, FRAC, PREC, 1, SLOTS>;
+    case 2: return kForce<MODEL, FRAC, PREC, 2, SLOTS>;
+    case 8: return kForce<MODEL, FRAC, PREC, 8, SLOTS>;
+    default: return kForce<MODEL, FRAC, PREC, 4, SLOTS>;
+    }
+}
+
+template <bool SLOTS>
+ForceFn pickForceForm( const PDArgs& a )
 {
     const int idx = a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 );
     switch ( idx )
     {
-    case 0: return kForce<kPMB, false, kF64>;
-    case 1: return kForce<kPMB, false, kMixed>;
-    case 2: return kForce<kPMB, true, kF64>;
-    case 3: return kForce<kPMB, true, kMixed>;
-    case 4: return kForce<kLPS, false, kF64>;
-    case 5: return kForce<kLPS, false, kMixed>;
-    case 6: return kForce<kLPS, true, kF64>;
-    default: return kForce<kLPS, true, kMixed>;
+    case 0: return pickUnroll<kPMB, false, kF64, SLOTS>( a.unroll );
+    case 1: return pickUnroll<kPMB, false, kMixed, SLOTS>( a.unroll );
+    case 2: return pickUnroll<kPMB, true, kF64, SLOTS>( a.unroll );
+    case 3: return pickUnroll<kPMB, true, kMixed, SLOTS>( a.unroll );
+    case 4: return pickUnroll<kLPS, false, kF64, SLOTS>( a.unroll );
+    case 5: return pickUnroll<kLPS, false, kMixed, SLOTS>( a.unroll );
+    case 6: return pickUnroll<kLPS, true, kF64, SLOTS>( a.unroll );
+    default: return pickUnroll<kLPS, true, kMixed, SLOTS>( a.unroll );
     }
 }
 
-const char* kForceNames[8] = {
-    "kForce<0, false, 0>", "kForce<0, false, 1>", "kForce<0, true, 0>", "kForce<0, true, 1>",
-    "kForce<1, false, 0>", "kForce<1, false, 1>", "kForce<1, true, 0>", "kForce<1, true, 1>" };
+ForceFn pickForce( const PDArgs& a )
+{
+    return a.exist ? pickForceForm<true>( a ) : pickForceForm<false>( a );
+}
+
+thread_local char g_force_name[64];
 
 } // namespace
 
+int normalizeUnroll( int u )
+{
+    return ( u == 1 || u == 2 || u == 8 ) ? u : 4;
+}
+
 const char* forceKernelName( const PDArgs& a )
 {
-    return kForceNames[a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 )];
+    std::snprintf( g_force_name, sizeof( g_force_name ), "kForce<%d, %s, %d, %d, %s>", a.model,
+                   a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0, normalizeUnroll( a.unroll ),
+                   a.exist ? "true" : "false" );
+    return g_force_name;
 }
 const char* lpsKernelName( const PDArgs& a )
 {

@@ -109,6 +109,31 @@ struct BuildScratch
 void launchMortonOrder( const double* x_aos_dev, int64_t n, const double lo[3], double inv_h,
                         int64_t* perm_dev, int64_t* inv_dev, BuildScratch& s, MemAccount* acct,
                         hipStream_t st );
+// Lattice detection: true when every position is lo + k h (k integer per
+// axis, one particle per site); perm then lists particles in lattice index
+// order with x fastest.
+bool latticeOrder( const double* x_aos_host, int64_t n, std::vector<int64_t>& perm, double h_out[3],
+                   int64_t dims_out[3] );
+// Stencil ids of a lattice specimen's bonds (mixed form); false when some
+// bond is not a lattice vector or the stencil has more than 255 entries.
+bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_rows, DevBuf& sid, DevBuf& table,
+                   BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out );
+// Stencil-slot form of a full lattice (dims[0]*dims[1]*dims[2] == n, lattice
+// rows): verifies that every compact row equals its in-box stencil entries in
+// one fixed order (the caller-id order of an interior particle's row), then
+// rebuilds col / intact / slice_ptr with slot q = stencil entry q and fills
+// exist, the slot offsets soff[256] and the FP32 stencil table[256] in slot
+// order.  False (arrays untouched) when the specimen does not qualify.
+struct SlotForm
+{
+    DevBuf exist, soff, table;
+    int32_t slots = 0;
+};
+bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3], double rc,
+                        const std::vector<int64_t>& perm_host, DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,
+                        int64_t& total_rows, int64_t& col_capacity_rows, SlotForm& out, BuildScratch& s,
+                        MemAccount* acct, hipStream_t st );
+void launchInvertPerm( const int64_t* perm_dev, int64_t n, int64_t* inv_dev, hipStream_t st );
 void launchGatherAos3( const double* src_aos, const int64_t* perm, double* dst_soa, int64_t n,
                        hipStream_t st );
 void launchScatterAos3( const double* src_soa, const int64_t* perm, double* dst_aos, int64_t n,

@@ -3,7 +3,7 @@
 // Nothing in the reference corresponds to this: a CabanaPD run is one
 // specimen per process.  Here every running specimen advances one time step
 // per packed launch; all specimens of one kernel variant (model, fracture,
-// precision) share a launch whose tiles cover their concatenated slice
+// precision, bond layout) share a launch whose tiles cover their concatenated slice
 // ranges.  A specimen's epilogue (final half step only on its last step) and
 // its displacement buffer follow from the launch index, so admission and
 // retirement only rewrite the small tile tables, never the specimen state.
@@ -101,12 +101,13 @@ struct BatchEngine::Impl
 
     void rebuildGroups()
     {
-        std::map<std::tuple<int, int, int>, size_t> key;
+        std::map<std::tuple<int, int, int, int>, size_t> key;
         std::vector<Group> ng;
         for ( const Run& r : running )
         {
             SpecView v = view( r.s );
-            auto k = std::make_tuple( v.model, v.fracture, v.precision );
+            // one kernel variant per group: model, fracture, precision, bond layout
+            auto k = std::make_tuple( v.model, v.fracture, v.precision, v.h->exist ? 1 : 0 );
             auto it = key.find( k );
             size_t gi;
             if ( it == key.end() )

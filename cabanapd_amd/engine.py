@@ -50,6 +50,9 @@ _SIGS = {
     "pdk_steps_done": (C.c_int, [_VP, _I64]),
     "pdk_download": (C.c_int, [_VP, C.c_int32, _D]),
     "pdk_sync": (C.c_int, [_VP]),
+    "pdk_set_tuning": (C.c_int, [_VP, C.c_int32]),
+    "pdk_spec_layout": (C.c_int, [_VP, _I32, _I32, _I32]),
+    "pdk_autotune": (C.c_int, [_VP, C.c_int32, _I32]),
     "pdk_kernel_timing": (C.c_int, [_VP, C.c_int32]),
     "pdk_kernel_time": (C.c_int, [_VP, C.c_int32, _D, _I64]),
     "pdk_kernel_name": (C.c_int, [_VP, C.c_int32, C.c_char_p, C.c_int32]),
@@ -255,6 +258,21 @@ class Spec:
         b = C.c_uint64()
         _check(lib().pdk_spec_bytes(self.h, C.byref(b)))
         return b.value
+
+    def layout(self):
+        lr = C.c_int32()
+        ns = C.c_int32()
+        sl = C.c_int32()
+        _check(lib().pdk_spec_layout(self.h, C.byref(lr), C.byref(ns), C.byref(sl)))
+        return {"lattice_rows": bool(lr.value), "stencil_entries": ns.value, "stencil_slots": bool(sl.value)}
+
+    def set_tuning(self, unroll):
+        _check(lib().pdk_set_tuning(self.h, unroll))
+
+    def autotune(self, launches=3):
+        u = C.c_int32()
+        _check(lib().pdk_autotune(self.h, launches, C.byref(u)))
+        return u.value
 
     def timing(self, enable=True):
         _check(lib().pdk_kernel_timing(self.h, 1 if enable else 0))

@@ -17,7 +17,9 @@
  *     fields are AoS double[n][3] in the caller's particle order, exactly the
  *     reference's slice layout (AoSoA vector length 1, CabanaPD_Particles.hpp:105-126).
  *   - Device memory is owned by the opaque handles.  The engine keeps
- *     particles in Morton order internally and translates on upload/download.
+ *     particles in its own device order (lattice rows, x fastest, for
+ *     lattice specimens; Morton order otherwise) and translates on
+ *     upload/download.
  *   - Calls on one spec are synchronous with respect to the host unless the
  *     name says otherwise (pdk_submit / pdk_poll), matching the reference's
  *     Kokkos::fence() after every kernel (e.g. force/CabanaPD_PMB.hpp:121-123).
@@ -87,7 +89,20 @@ typedef struct pdk_model_desc {
     double mu_friction;     /* JKR Coulomb friction */
     int32_t rebuild_every;  /* 0: rebuild when max|u-u_build| > extend (CabanaPD_Neighbor.hpp:116-119); k>0: every k steps (plus the same safety trigger) */
     int32_t polydisperse;   /* 1: per-particle radii (extension; equal radii reduce to the reference) */
+    int32_t ordering;       /* particle order on the device: PDK_ORDER_AUTO / _MORTON / _LATTICE_ROWS */
+    int32_t bond_layout;    /* PDK_BONDS_AUTO / PDK_BONDS_COMPACT */
 } pdk_model_desc;
+
+enum pdk_ordering {
+    PDK_ORDER_AUTO = 0,         /* lattice rows when the positions form a lattice, else Morton */
+    PDK_ORDER_MORTON = 1,       /* Morton order of the reference positions */
+    PDK_ORDER_LATTICE_ROWS = 2  /* lattice index order, x fastest (fails if not a lattice) */
+};
+
+enum pdk_bond_layout {
+    PDK_BONDS_AUTO = 0,     /* stencil slots on full lattices (neighbor ids computed, not stored), else compact */
+    PDK_BONDS_COMPACT = 1   /* always the compact form: one stored neighbor id per bond */
+};
 
 /* Derived constants, for inspection and tests. */
 typedef struct pdk_model_constants {
@@ -137,7 +152,7 @@ int pdk_spec_bytes(pdk_spec spec, uint64_t* bytes);
  * Neighbor<NoFracture|Fracture> ctor (CabanaPD_Neighbor.hpp:44-66,199-210):
  * full list of j != i with |x_j - x_i|^2 <= (horizon + tol)^2 (reference
  * positions; current positions x+u for contact models).  Built on the device
- * by cell binning into a Morton-ordered sliced CSR; every bond starts intact. */
+ * by cell binning into a sliced CSR in device order; every bond starts intact. */
 int pdk_build_bonds(pdk_spec spec);
 /* Neighbor statistics (Neighbor::getStatistics, CabanaPD_Neighbor.hpp:82-102). */
 int pdk_bond_stats(pdk_spec spec, int64_t* total_bonds, int32_t* max_neighbors);
@@ -181,6 +196,20 @@ int pdk_steps_done(pdk_spec spec, int64_t* steps);
 /* Download a field in caller order (double[n][3] or double[n]). */
 int pdk_download(pdk_spec spec, int32_t field, double* out);
 int pdk_sync(pdk_spec spec);
+
+/* ---- tuning --------------------------------------------------------------------
+ * The force pass loads `unroll` bond slots (1, 2, 4 or 8) per group before
+ * evaluating them; results do not depend on it.  pdk_autotune times the
+ * candidates on the current state (after pdk_init; the probes repeat the
+ * init force pass, which is idempotent) and keeps the fastest. */
+int pdk_set_tuning(pdk_spec spec, int32_t unroll);
+/* Device layout chosen at upload/build: lattice-row order (1) or Morton (0);
+ * the number of lattice stencil entries (0 = no stencil: the mixed form
+ * gathers x_j for the bond vectors); stencil_slots = 1 when the bonds are in
+ * the stencil-slot form (slot q of every row is stencil entry q), 0 for the
+ * compact form.  Any pointer may be NULL. */
+int pdk_spec_layout(pdk_spec spec, int32_t* lattice_rows, int32_t* stencil_entries, int32_t* stencil_slots);
+int pdk_autotune(pdk_spec spec, int32_t launches_per_candidate, int32_t* unroll_out);
 
 /* ---- measurement -------------------------------------------------------------
  * When enabled, every kernel launch of the spec is bracketed by HIP events on

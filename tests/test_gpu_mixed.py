@@ -123,3 +123,33 @@ def test_guard_band_break_decisions(model):
         nb = int((res["fp64"] == 0).sum())
         say(f"  rel {rel:+.0e}: fp64 broken {nb}/{res['fp64'].size} mismatches {(res['fp64'] != res['mixed']).sum()}")
         assert np.array_equal(res["fp64"], res["mixed"])
+
+
+def test_mixed_stencil_matches_gathered_bond_vectors():
+    """Mixed form with the bond vectors from the lattice stencil (stencil-slot
+    form, and compact form with stencil ids) vs the same form gathering x_j
+    (Morton order): same broken bonds, displacements within rounding."""
+    say("case: PMB fracture plate 40x80x9 (cubic cells) mixed, stencil vs gathered xi")
+    w = configs.kw_plate(num_cells=(40, 80, 9), size=(0.1, 0.2, 0.0225), steps=150)
+    res = {}
+    for name, order, layout in (("morton", 1, 0), ("lattice-compact", 2, 1), ("lattice-slots", 2, 0)):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.precision = 1
+        d.ordering = order
+        d.bond_layout = layout
+        s = engine.Spec(ctx(), d, w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        lay = s.layout()
+        say(f"  {name}: {lay}")
+        if order == 2:
+            assert lay["stencil_entries"] == 122
+            assert lay["stencil_slots"] == (layout == 0)
+        s.step(150)
+        res[name] = (s.get("u"), s.bonds()[2])
+        s.close()
+    for name in ("lattice-compact", "lattice-slots"):
+        e = rel_err(res[name][0], res["morton"][0])
+        say(f"  {name} vs morton: u rel err {e:.3e} mismatched bonds {(res[name][1] != res['morton'][1]).sum()}")
+        assert np.array_equal(res[name][1], res["morton"][1])
+        assert e <= 1e-6

@@ -156,3 +156,43 @@ def test_gpu_lps_cube_small(fracture):
     if fracture:
         assert_same(g.get("phi"), o.get("phi"), "phi")
         assert_total(Dg, Do, "total damage")
+
+
+def test_gpu_device_order_is_transparent():
+    """Morton order, lattice rows with stored neighbor ids and lattice rows in
+    the stencil-slot form give bit-identical FP64 results (each particle sums
+    its bonds in caller-id order every way)."""
+    say("case: PMB fracture plate 30x60x7, Morton vs lattice compact vs lattice slots, 60 steps")
+    from cabanapd_amd import engine
+    from tests.parity import ctx
+    w = configs.kw_plate(num_cells=(30, 60, 7), size=(0.09, 0.18, 0.021), steps=60)
+    out = {}
+    for name, order, layout in (("morton", 1, 0), ("compact", 2, 1), ("slots", 2, 0)):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.ordering = order
+        d.bond_layout = layout
+        s = engine.Spec(ctx(), d, w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        lay = s.layout()
+        say(f"  {name}: {lay}")
+        assert lay["lattice_rows"] == (order == 2)
+        assert lay["stencil_slots"] == (name == "slots")
+        s.step(60)
+        out[name] = (s.get("u"), s.get("v"), s.bonds()[2])
+        s.close()
+    for name in ("compact", "slots"):
+        for a, b, what in zip(out["morton"], out[name], ("u", "v", "intact")):
+            assert_same(a, b, f"{what} ({name} vs morton)")
+
+
+@pytest.mark.parametrize("unroll", [1, 2, 4, 8])
+def test_gpu_unroll_is_transparent(unroll):
+    say(f"case: PMB fracture plate 24x48x6 FP64, unroll {unroll} vs oracle")
+    w = configs.kw_plate(num_cells=(24, 48, 6), steps=40)
+    g, o = pair(w)
+    g.set_tuning(unroll)
+    g.step(40)
+    o.step(40)
+    assert_same(g.get("u"), o.get("u"), "u")
+    assert broken_set(g) == broken_set(o)
