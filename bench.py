@@ -28,6 +28,16 @@ from cabanapd_amd import configs, engine  # noqa: E402
 from cabanapd_amd._abi import KCLS_FORCE  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+DEFAULT_CELLS = (200, 400, 18)
+
+# HBM bytes per launch of the force pass on the default plate, measured with
+# rocprofv3 --pmc (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE; the factors come from a
+# plain 1 GiB device copy read through the same counters).  Summaries with
+# the commands are kept under profiles/.
+MEASURED_TRAFFIC = {
+    "kForce<0, true, 1, 2, true>": 503.2e6,
+    "kForce<0, true, 0, 2, true>": 744.5e6,
+}
 
 
 def algorithmic_bytes_per_step(n_particles, bonds, passes=1, fracture=True, lps=False):
@@ -177,7 +187,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cells", type=str, default="200,400,18")
-    ap.add_argument("--unroll", type=int, default=0, help="force-pass slot grouping (0 = autotune)")
+    ap.add_argument("--unroll", type=int, default=-1,
+                    help="force-pass variant: 0 row windows in LDS, 1/2/4/8 gathers in slot groups, -1 autotune")
     ap.add_argument("--requests", type=int, default=24, help="serving stream length (0 = skip serving)")
     ap.add_argument("--max-particles", type=float, default=2e6)
     ap.add_argument("--bond-layout", choices=("auto", "compact"), default="auto",
@@ -201,7 +212,7 @@ def main():
     for prec in ("fp64", "mixed"):
         s = make_spec(ctx, w, prec, {"auto": 0, "compact": 1}[args.bond_layout])
         bonds, _ = s.bond_stats()
-        if args.unroll:
+        if args.unroll >= 0:
             s.set_tuning(args.unroll)
             unroll = args.unroll
         else:
@@ -251,7 +262,9 @@ def main():
         "mixed_vs_fp64": {"speedup": value / fp64_value, "u_rel_err": u_err,
                           "broken_bonds_fp64": broken64, "broken_bond_mismatches": broken_diff},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": m["kernel"],
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get(m["kernel"]) if cells == DEFAULT_CELLS else None,
+                     "kernel": m["kernel"],
                      "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": b_alg},
         "cpu_baseline": None,
     }

@@ -148,7 +148,7 @@ struct pdk_spec_s
     int32_t cur = 0;
     bool uploaded = false, built = false, inited = false;
     bool lattice_rows = false; // device order is lattice rows (else Morton)
-    int32_t unroll = 4;        // force-pass slot grouping (tuning knob)
+    int32_t unroll = -1;       // force-pass variant (tuning knob): -1 default, 0 row windows, 1/2/4/8 gathers
     bool vol_uniform = false;  // every particle has volume vol_value
     double vol_value = 0.0;
     bool any_nofail = false;
@@ -213,7 +213,22 @@ struct pdk_spec_s
             h.exist = slot.exist.as<uint64_t>();
             h.soff = slot.soff.as<int32_t>();
             h.stencil_slots = slot.slots;
+            h.rows = slot.nrows > 0 ? slot.rows.as<int4>() : nullptr;
+            h.nrows = slot.nrows;
+            for ( int dd = 0; dd < 3; dd++ )
+                h.lat_n[dd] = static_cast<int32_t>( lat_dims[dd] );
         }
+        else
+        {
+            h.rows = nullptr;
+            h.nrows = 0;
+        }
+        // tuning knob: -1 = default (row windows where available, else groups of 4)
+        h.unroll = unroll;
+        if ( h.unroll < 0 )
+            h.unroll = h.nrows > 0 ? 0 : 4;
+        if ( h.unroll == 0 && h.nrows == 0 )
+            h.unroll = 4;
         h.c = k.c;
         h.bcc = k.bond_break_coeff;
         h.s0 = k.s0;
@@ -241,7 +256,6 @@ struct pdk_spec_s
         h.influence = d.influence;
         h.precision = d.precision;
         h.polydisperse = d.polydisperse;
-        h.unroll = unroll;
         h.vol_uniform = vol_uniform ? 1 : 0;
         h.vol_value = vol_value;
         h.any_nofail = any_nofail ? 1 : 0;
@@ -1034,7 +1048,10 @@ int pdk_set_tuning( pdk_spec s, int32_t unroll )
 {
     return guarded( [&] {
         require( s != nullptr, "null spec" );
-        require( unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8, "unroll must be 1, 2, 4 or 8" );
+        require( unroll == 0 || unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8,
+                 "unroll must be 0 (row windows), 1, 2, 4 or 8" );
+        require( unroll != 0 || !s->built || s->slot.nrows > 0,
+                 "row windows need the stencil-slot layout of a full lattice" );
         s->unroll = unroll;
         if ( s->uploaded )
         {
@@ -1052,7 +1069,7 @@ int pdk_autotune( pdk_spec s, int32_t launches, int32_t* unroll_out )
         if ( isContact( s->d ) )
         {
             if ( unroll_out )
-                *unroll_out = s->unroll;
+                *unroll_out = s->h.unroll;
             return;
         }
         launches = std::max( launches, 1 );
@@ -1061,7 +1078,12 @@ int pdk_autotune( pdk_spec s, int32_t launches, int32_t* unroll_out )
         PDK_HIP( hipEventCreate( &b ) );
         float best = 1e30f;
         int32_t best_u = s->unroll;
+        std::vector<int32_t> cand;
+        if ( s->slot.nrows > 0 )
+            cand.push_back( 0 ); // row windows staged in LDS
         for ( int32_t u : { 1, 2, 4, 8 } )
+            cand.push_back( u );
+        for ( int32_t u : cand )
         {
             s->unroll = u;
             s->pushArgs();

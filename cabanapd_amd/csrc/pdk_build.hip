@@ -900,6 +900,12 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
     const int64_t n = a.n;
     if ( dims[0] * dims[1] * dims[2] != n || static_cast<int64_t>( perm_host.size() ) != n )
         return false;
+    // Caller order equal to the lattice order: slots sorted by id offset are
+    // sorted by caller id for every particle.  Otherwise the slot order is
+    // taken from an interior particle (which must exist) and verified below.
+    bool identity = true;
+    for ( int64_t p = 0; p < n && identity; p++ )
+        identity = perm_host[p] == p;
     int32_t K[3];
     for ( int d = 0; d < 3; d++ )
     {
@@ -907,8 +913,7 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
             return false;
         // largest integer offset along d that can be within the cutoff
         K[d] = static_cast<int32_t>( std::floor( rc / h[d] * ( 1.0 + 1e-9 ) ) );
-        // an interior particle (full stencil) must exist to fix the slot order
-        if ( K[d] > 40 || dims[d] <= 2 * K[d] )
+        if ( K[d] > 40 || ( !identity && dims[d] <= 2 * K[d] ) )
             return false;
     }
     const LatticeBox L{ static_cast<int32_t>( dims[0] ), static_cast<int32_t>( dims[1] ),
@@ -935,16 +940,24 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
                     return false;
                 if ( r2 > rc2 )
                     continue;
+                // offsets that no particle of the box can have
+                if ( std::abs( aa ) >= dims[0] || std::abs( b ) >= dims[1] || std::abs( c ) >= dims[2] )
+                    continue;
                 es.push_back( { aa, b, c, ( static_cast<int64_t>( c ) * L.ny + b ) * L.nx + aa, r2 } );
             }
     const int32_t S = static_cast<int32_t>( es.size() );
     if ( S == 0 || S > 255 )
         return false;
-    // slot order: ascending caller id of an interior particle's neighbors
-    const int64_t p0 = ( static_cast<int64_t>( K[2] ) * L.ny + K[1] ) * L.nx + K[0];
-    std::stable_sort( es.begin(), es.end(), [&]( const Entry& u, const Entry& v ) {
-        return perm_host[p0 + u.off] < perm_host[p0 + v.off];
-    } );
+    if ( identity )
+        std::stable_sort( es.begin(), es.end(), []( const Entry& u, const Entry& v ) { return u.off < v.off; } );
+    else
+    {
+        // ascending caller id of an interior particle's neighbors
+        const int64_t p0 = ( static_cast<int64_t>( K[2] ) * L.ny + K[1] ) * L.nx + K[0];
+        std::stable_sort( es.begin(), es.end(), [&]( const Entry& u, const Entry& v ) {
+            return perm_host[p0 + u.off] < perm_host[p0 + v.off];
+        } );
+    }
     std::vector<int4> ent( S );
     for ( int32_t e = 0; e < S; e++ )
         ent[e] = make_int4( es[e].a, es[e].b, es[e].c, static_cast<int32_t>( es[e].off ) );
@@ -985,6 +998,32 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
         tab[e] = make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
                               static_cast<float>( std::sqrt( es[e].r2 ) ) );
     }
+    // Row windows: maximal runs of slots with one (dy, dz) and consecutive dx.
+    std::vector<int4> rtab;
+    for ( int32_t e = 0; e < S; e++ )
+    {
+        if ( !rtab.empty() )
+        {
+            int4& r = rtab.back();
+            const int32_t dx0 = ( r.w & 0xff ) - 128, dy = ( ( r.w >> 8 ) & 0xff ) - 128,
+                          dz = ( ( r.w >> 16 ) & 0xff ) - 128;
+            if ( es[e].b == dy && es[e].c == dz && es[e].a == dx0 + r.y && r.y < kRowMaxSlots )
+            {
+                r.y++;
+                continue;
+            }
+        }
+        const int32_t packed = ( es[e].a + 128 ) | ( ( es[e].b + 128 ) << 8 ) | ( ( es[e].c + 128 ) << 16 );
+        rtab.push_back( make_int4( e, 1, static_cast<int32_t>( es[e].off ), packed ) );
+    }
+    DevBuf drows;
+    int32_t nrows = 0;
+    if ( static_cast<int>( rtab.size() ) <= kMaxRows )
+    {
+        nrows = static_cast<int32_t>( rtab.size() );
+        drows.alloc( nrows * sizeof( int4 ), acct );
+        PDK_HIP( hipMemcpyAsync( drows.as<int4>(), rtab.data(), nrows * sizeof( int4 ), hipMemcpyHostToDevice, st ) );
+    }
     dsoff.alloc( 256 * sizeof( int32_t ), acct );
     dtab.alloc( 256 * sizeof( float4 ), acct );
     PDK_HIP( hipMemcpyAsync( dsoff.as<int32_t>(), soff.data(), 256 * sizeof( int32_t ), hipMemcpyHostToDevice, st ) );
@@ -995,6 +1034,8 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
     out.exist = std::move( nexist );
     out.soff = std::move( dsoff );
     out.table = std::move( dtab );
+    out.rows = std::move( drows );
+    out.nrows = nrows;
     out.slots = S;
     total_rows = rows;
     col_capacity_rows = rows;

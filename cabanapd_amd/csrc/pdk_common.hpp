@@ -1,7 +1,8 @@
 // pdk_common.hpp -- device-side data layout shared by the engine's kernels.
 //
 // Layout in HBM (one specimen):
-//   particles in Morton order of their reference positions, SoA:
+//   particles in lattice-row order (x fastest) when the positions form a
+//   lattice, else in Morton order of their reference positions, SoA:
 //     x[3][n] (FP64), U0[3][n] / U1[3][n] (FP64 displacement, double
 //     buffered: the force pass reads U[cur] and its fused velocity-Verlet
 //     epilogue writes U[cur^1]), v[3][n], f[3][n], vol[n], rho[n],
@@ -16,6 +17,10 @@
 //     sid[total_slots][64] (u8 lattice stencil id, optional).
 //   Row order within a slot row is ascending caller id, so a lane's serial
 //   sum visits bonds in the oracle's order.
+//   Stencil-slot form (full lattices): every slice has S slot rows, slot q =
+//   stencil entry q for every lane, plus exist[total_slots] (u64, bit lane =
+//   bond exists); the neighbor id is p + soff[q], so the force pass reads no
+//   col stream.
 #ifndef PDK_COMMON_HPP
 #define PDK_COMMON_HPP
 
@@ -98,8 +103,23 @@ struct PDArgs
     int32_t any_nofail;
     double vol_value;
     int32_t stencil_slots;  // S in the stencil-slot form, 0 in the compact form
-    int32_t pad0;
+    // Row windows of the stencil-slot form: the slots of one lattice row
+    // (dy, dz) are consecutive and their neighbors of a slice form one
+    // contiguous id window.  rows[r] = (first slot, slot count, id offset of
+    // the first slot's neighbor, packed (dx0, dy, dz) + 128 per byte).
+    const int4* rows;
+    int32_t nrows;          // 0: no row table
+    int32_t lat_n[3];       // lattice extent (stencil-slot form)
 };
+
+constexpr int kRowMaxSlots = 9;   // slots per row window (|dx| <= 4)
+constexpr int kMaxRows = 64;      // rows per stencil
+// Force-pass variant selected by the tuning knob: 0 = row windows staged in
+// LDS (needs a row table), 1/2/4/8 = per-slot gathers in groups of that size.
+__host__ __device__ __forceinline__ bool useRowWindows( const PDArgs& a )
+{
+    return a.unroll == 0 && a.rows != nullptr && a.nrows > 0;
+}
 
 // Bond-slot walk shared by both layouts: a lane visits slots [0, slotEnd) and
 // slot q holds one of its bonds iff slotBond (q < cnt in the compact form,

@@ -539,6 +539,254 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
 }
 
 // ---------------------------------------------------------------------------
+// The same force pass with the neighbor data staged in LDS (stencil-slot form
+// on lattice rows).  For a lattice row r = (dy, dz) of the stencil, slots
+// q0 .. q0+c-1 of the 64 lanes of a slice have their neighbors at the
+// contiguous ids g .. g+63+c-1 (g = first id of the slice + the row's offset):
+// the wave loads that window once with coalesced loads, stores it in its LDS
+// buffer, and slot q0+t of lane L reads window[L + t].  The window of the next
+// row is loaded while the current row is evaluated.  Bond existence follows
+// from the lattice coordinates (the build verified that it equals the stored
+// existence bits).  Evaluation order and arithmetic are kForce's, so results
+// are identical.
+// ---------------------------------------------------------------------------
+template <int MODEL, bool FRAC, int PREC>
+__global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict__ specs,
+                                                        const int32_t* __restrict__ tile_spec,
+                                                        const int64_t* __restrict__ tile_slice,
+                                                        int32_t epi, int32_t parity, int64_t launch,
+                                                        int32_t accumulate )
+{
+#pragma clang fp contract( off )
+    constexpr int kW = kSlice + kRowMaxSlots - 1;  // window width
+    constexpr int NX = ( PREC == kF64 ) ? 3 : 0;    // staged x components
+    constexpr int NAB = ( MODEL == kLPS ) ? 2 : 0;  // staged lpsA, lpsB
+    constexpr int NC = 3 + NX + NAB;
+    __shared__ double win[kSlicesPerBlock][2][NC][kW];
+    __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
+    __shared__ int4 rtab[kMaxRows];
+    WaveCtx w;
+    const bool active = waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w );
+    // All waves of a block belong to one spec: stage its tables before any
+    // wave leaves.
+    if constexpr ( PREC == kMixed )
+        stab[threadIdx.x] = w.A->stencil[threadIdx.x];
+    if ( static_cast<int>( threadIdx.x ) < w.A->nrows )
+        rtab[threadIdx.x] = w.A->rows[threadIdx.x];
+    __syncthreads();
+    if ( !active )
+        return;
+    const PDArgs& A = *w.A;
+    const int wv = threadIdx.x / kSlice;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t n = A.n;
+    const int64_t p0 = w.slice * kSlice;
+    const int64_t p = p0 + lane;
+    const bool valid = p < n;
+    const int64_t pc = valid ? p : p0;
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ u = A.U[w.cur];
+    const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
+    const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
+    const int32_t nx = A.lat_n[0], ny = A.lat_n[1], nz = A.lat_n[2];
+    const int64_t t1 = pc / nx;
+    const int32_t ix = static_cast<int32_t>( pc - t1 * nx );
+    const int32_t iy = static_cast<int32_t>( t1 % ny );
+    const int32_t iz = static_cast<int32_t>( t1 / ny );
+    uint64_t* __restrict__ bits = A.intact + A.slice_ptr[w.slice];
+    const double* __restrict__ vol = A.vol;
+    const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
+    const double c = A.c;
+    const double bcc = A.bcc;
+    double Ai = 0.0, Bi = 0.0;
+    if constexpr ( MODEL == kLPS )
+    {
+        Ai = A.lpsA[pc]; // theta_coeff * theta_i / m_i
+        Bi = A.lpsB[pc]; // s_coeff / m_i
+    }
+    const int infl1 = A.influence;
+    const bool uvol = A.vol_uniform != 0;
+    const double volu = A.vol_value;
+    const bool any_nofail = A.any_nofail != 0;
+
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if ( accumulate && valid )
+    {
+        fx = A.f[p];
+        fy = A.f[n + p];
+        fz = A.f[2 * n + p];
+    }
+
+    const double* src[NC];
+    src[0] = u;
+    src[1] = u + n;
+    src[2] = u + 2 * n;
+    if constexpr ( NX )
+    {
+        src[3] = x;
+        src[4] = x + n;
+        src[5] = x + 2 * n;
+    }
+    if constexpr ( NAB )
+    {
+        src[3 + NX] = A.lpsA;
+        src[4 + NX] = A.lpsB;
+    }
+    double v0[NC], v1[NC];
+    uint32_t blo = 0, bhi = 0; // intact words of the row's slots: lane t holds slot q0 + t
+    auto loadRow = [&]( int r ) {
+        const int4 R = rtab[r];
+        const int64_t g = p0 + R.z;
+        const int64_t g0 = min( max( g + lane, int64_t( 0 ) ), n - 1 );
+        const int64_t g1 = min( max( g + kSlice + lane, int64_t( 0 ) ), n - 1 );
+        const bool second = lane < R.y - 1;
+#pragma unroll
+        for ( int k = 0; k < NC; k++ )
+        {
+            v0[k] = src[k][g0];
+            v1[k] = second ? src[k][g1] : 0.0;
+        }
+        if constexpr ( FRAC )
+        {
+            const uint64_t wd = ( lane < R.y ) ? bits[R.x + lane] : 0ull;
+            blo = static_cast<uint32_t>( wd );
+            bhi = static_cast<uint32_t>( wd >> 32 );
+        }
+    };
+
+    const int nrows = A.nrows;
+    loadRow( 0 );
+    for ( int r = 0; r < nrows; r++ )
+    {
+        const int4 R = rtab[r];
+        double* __restrict__ wb = &win[wv][r & 1][0][0];
+#pragma unroll
+        for ( int k = 0; k < NC; k++ )
+        {
+            wb[k * kW + lane] = v0[k];
+            if ( lane < R.y - 1 )
+                wb[k * kW + kSlice + lane] = v1[k];
+        }
+        const uint32_t rlo = blo, rhi = bhi;
+        // LDS operations of one wavefront complete in order; the fences keep
+        // the compiler from moving the window reads above the stores.
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+        if ( r + 1 < nrows )
+            loadRow( r + 1 );
+        const int32_t dx0 = ( R.w & 0xff ) - 128;
+        const int32_t dy = ( ( R.w >> 8 ) & 0xff ) - 128;
+        const int32_t dz = ( ( R.w >> 16 ) & 0xff ) - 128;
+        const bool rowok = valid && static_cast<uint32_t>( iy + dy ) < static_cast<uint32_t>( ny ) &&
+                           static_cast<uint32_t>( iz + dz ) < static_cast<uint32_t>( nz );
+        for ( int t = 0; t < R.y; t++ )
+        {
+            const int32_t q = R.x + t;
+            const bool ex = rowok && static_cast<uint32_t>( ix + dx0 + t ) < static_cast<uint32_t>( nx );
+            uint64_t ww = 0;
+            if constexpr ( FRAC )
+            {
+                const uint32_t l32 = __builtin_amdgcn_readlane( rlo, t );
+                const uint32_t h32 = __builtin_amdgcn_readlane( rhi, t );
+                ww = ( static_cast<uint64_t>( h32 ) << 32 ) | l32;
+            }
+            bool brk = false;
+            if ( ex )
+            {
+                const double* __restrict__ wr = wb + lane + t;
+                const int64_t j = p + R.z + t;
+                const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
+                const double vj = uvol ? volu : vol[j];
+                const bool nf_j =
+                    ( MODEL == kLPS ) ? nf_i : ( ( FRAC && any_nofail ) ? ( A.nofail[j] != 0 ) : false );
+                double aj = 0.0, bj = 0.0;
+                if constexpr ( NAB )
+                {
+                    aj = wr[( 3 + NX ) * kW];
+                    bj = wr[( 4 + NX ) * kW];
+                }
+                bool intact = true;
+                if constexpr ( FRAC )
+                    intact = ( ww >> lane ) & 1ull;
+                if constexpr ( PREC == kF64 )
+                {
+                    Bond64 b;
+                    getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
+                                   b );
+                    if constexpr ( FRAC )
+                    {
+                        if ( criticalStretch64( b.r, b.xi, bcc ) && !nf_i && !nf_j )
+                            brk = true;
+                    }
+                    if ( !brk && intact )
+                    {
+                        double coeff;
+                        if constexpr ( MODEL == kPMB )
+                            coeff = c * b.s * vj;
+                        else
+                        {
+                            const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                            coeff = ( Ai + aj + b.s * ( Bi + bj ) ) * infl * b.xi * vj;
+                        }
+                        fx += coeff * b.rx / b.r;
+                        fy += coeff * b.ry / b.r;
+                        fz += coeff * b.rz / b.r;
+                    }
+                }
+                else
+                {
+                    Bond32 b;
+                    getDistance32s( stab[q], ui0, ui1, ui2, uj0, uj1, uj2, b );
+                    if constexpr ( FRAC )
+                    {
+                        if ( !nf_i && !nf_j )
+                        {
+                            const float band = A.bandf * ( b.etan + fabsf( b.s ) + A.s0f );
+                            if ( fabsf( b.s - A.s0f ) <= band )
+                            {
+                                // guard band: the reference's FP64 test on this bond
+                                Bond64 e;
+                                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], uj0, uj1,
+                                               uj2, e );
+                                brk = criticalStretch64( e.r, e.xi, bcc );
+                            }
+                            else
+                                brk = b.s > A.s0f;
+                        }
+                    }
+                    if ( !brk && intact )
+                    {
+                        float coeff;
+                        if constexpr ( MODEL == kPMB )
+                            coeff = static_cast<float>( c ) * b.s * static_cast<float>( vj );
+                        else
+                        {
+                            const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
+                            coeff = static_cast<float>( Ai + aj ) + b.s * static_cast<float>( Bi + bj );
+                            coeff = coeff * infl * b.xi * static_cast<float>( vj );
+                        }
+                        const float fr = coeff * b.rinv;
+                        fx += static_cast<double>( fr * b.rx );
+                        fy += static_cast<double>( fr * b.ry );
+                        fz += static_cast<double>( fr * b.rz );
+                    }
+                }
+            }
+            if constexpr ( FRAC )
+            {
+                const uint64_t m = __ballot( brk );
+                if ( ( m & ww ) && lane == 0 )
+                    bits[q] = ww & ~m;
+            }
+        }
+    }
+    if ( !valid )
+        return;
+    epilogue( A, p, w.epi, w.cur, fx, fy, fz );
+}
+
+// ---------------------------------------------------------------------------
 // LPS state pass: Force<LPS,*>::computeWeightedVolume + computeDilatation
 // (force/CabanaPD_LPS.hpp:89-141, 287-378), fused: m_i and theta_i need only
 // particle i's own bonds.  Also stores the per-particle force-coefficient
@@ -838,8 +1086,26 @@ ForceFn pickForceForm( const PDArgs& a )
     }
 }
 
+ForceFn pickForceRows( const PDArgs& a )
+{
+    const int idx = a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 );
+    switch ( idx )
+    {
+    case 0: return kForceRows<kPMB, false, kF64>;
+    case 1: return kForceRows<kPMB, false, kMixed>;
+    case 2: return kForceRows<kPMB, true, kF64>;
+    case 3: return kForceRows<kPMB, true, kMixed>;
+    case 4: return kForceRows<kLPS, false, kF64>;
+    case 5: return kForceRows<kLPS, false, kMixed>;
+    case 6: return kForceRows<kLPS, true, kF64>;
+    default: return kForceRows<kLPS, true, kMixed>;
+    }
+}
+
 ForceFn pickForce( const PDArgs& a )
 {
+    if ( useRowWindows( a ) )
+        return pickForceRows( a );
     return a.exist ? pickForceForm<true>( a ) : pickForceForm<false>( a );
 }
 
@@ -854,6 +1120,12 @@ int normalizeUnroll( int u )
 
 const char* forceKernelName( const PDArgs& a )
 {
+    if ( useRowWindows( a ) )
+    {
+        std::snprintf( g_force_name, sizeof( g_force_name ), "kForceRows<%d, %s, %d>", a.model,
+                       a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0 );
+        return g_force_name;
+    }
     std::snprintf( g_force_name, sizeof( g_force_name ), "kForce<%d, %s, %d, %d, %s>", a.model,
                    a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0, normalizeUnroll( a.unroll ),
                    a.exist ? "true" : "false" );

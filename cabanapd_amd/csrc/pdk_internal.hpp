@@ -122,12 +122,14 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
 // rows): verifies that every compact row equals its in-box stencil entries in
 // one fixed order (the caller-id order of an interior particle's row), then
 // rebuilds col / intact / slice_ptr with slot q = stencil entry q and fills
-// exist, the slot offsets soff[256] and the FP32 stencil table[256] in slot
-// order.  False (arrays untouched) when the specimen does not qualify.
+// exist, the slot offsets soff[256], the FP32 stencil table[256] in slot
+// order and the row-window table.  False (arrays untouched) when the
+// specimen does not qualify.
 struct SlotForm
 {
-    DevBuf exist, soff, table;
+    DevBuf exist, soff, table, rows;
     int32_t slots = 0;
+    int32_t nrows = 0; // row windows (0: the stencil does not split into <= kMaxRows rows)
 };
 bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3], double rc,
                         const std::vector<int64_t>& perm_host, DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,

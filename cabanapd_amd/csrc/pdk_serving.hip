@@ -107,7 +107,8 @@ struct BatchEngine::Impl
         {
             SpecView v = view( r.s );
             // one kernel variant per group: model, fracture, precision, bond layout
-            auto k = std::make_tuple( v.model, v.fracture, v.precision, v.h->exist ? 1 : 0 );
+            const int form = useRowWindows( *v.h ) ? 2 : ( v.h->exist ? 1 : 0 );
+            auto k = std::make_tuple( v.model, v.fracture, v.precision, form );
             auto it = key.find( k );
             size_t gi;
             if ( it == key.end() )

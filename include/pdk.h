@@ -198,10 +198,13 @@ int pdk_download(pdk_spec spec, int32_t field, double* out);
 int pdk_sync(pdk_spec spec);
 
 /* ---- tuning --------------------------------------------------------------------
- * The force pass loads `unroll` bond slots (1, 2, 4 or 8) per group before
- * evaluating them; results do not depend on it.  pdk_autotune times the
- * candidates on the current state (after pdk_init; the probes repeat the
- * init force pass, which is idempotent) and keeps the fastest. */
+ * Force-pass variant: unroll = 1, 2, 4 or 8 gathers the neighbor data of that
+ * many bond slots per group; unroll = 0 (stencil-slot layout only) stages
+ * the neighbor data of each lattice row window in LDS instead.  Results do
+ * not depend on the choice.  The default is 0 where available, else 4.
+ * pdk_autotune times the candidates on the current state (after pdk_init;
+ * the probes repeat the init force pass, which is idempotent) and keeps the
+ * fastest. */
 int pdk_set_tuning(pdk_spec spec, int32_t unroll);
 /* Device layout chosen at upload/build: lattice-row order (1) or Morton (0);
  * the number of lattice stencil entries (0 = no stencil: the mixed form

@@ -153,3 +153,30 @@ def test_mixed_stencil_matches_gathered_bond_vectors():
         say(f"  {name} vs morton: u rel err {e:.3e} mismatched bonds {(res[name][1] != res['morton'][1]).sum()}")
         assert np.array_equal(res[name][1], res["morton"][1])
         assert e <= 1e-6
+
+
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_mixed_row_windows_match_gathers(model):
+    """The mixed force pass with neighbor data staged in LDS row windows and
+    the one gathering it per slot evaluate the same expressions in the same
+    order: bit-identical results."""
+    if model == "PMB":
+        say("case: PMB fracture plate 40x80x9 mixed, row windows vs gathers, 100 steps")
+        w = configs.kw_plate(num_cells=(40, 80, 9), size=(0.1, 0.2, 0.0225), steps=100)
+    else:
+        say("case: LPS cube 20^3 fracture mixed, row windows vs gathers, 40 steps")
+        w = configs.lps_cube(n_side=20, steps=40, fracture=True)
+    out = {}
+    for variant in (0, 2):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.precision = 1
+        s = engine.Spec(ctx(), d, w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        s.set_tuning(variant)
+        say(f"  variant {variant}: {s.kernel_name(0)} {s.layout()}")
+        s.step(w["steps"])
+        out[variant] = (s.get("u"), s.get("v"), s.bonds()[2])
+        s.close()
+    for a, b in zip(out[0], out[2]):
+        assert np.array_equal(a, b)
