@@ -101,13 +101,14 @@ __device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi
     const float ee = ex * ex + ey * ey + ez * ez;
     const float num = 2.0f * xe + ee;
     b.s = num * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
-    b.etan = __builtin_sqrtf( ee ) * xinv;
+    // guard-band scale: (|ex| + |ey| + |ez|) / |xi| >= |eta| / |xi|
+    b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
 }
 
 // Same stretch with the bond vector from the lattice stencil table
-// (g.xyz = FP32 xi, g.w = |xi|): no x_j gather.
-__device__ __forceinline__ void getDistance32s( const float4 g, double ui0, double ui1, double ui2, double uj0,
-                                                double uj1, double uj2, Bond32& b )
+// (g.xyz = FP32 xi, g.w = |xi|, xinv = rcp(|xi|)): no x_j gather.
+__device__ __forceinline__ void getDistance32s( const float4 g, float xinv, double ui0, double ui1, double ui2,
+                                                double uj0, double uj1, double uj2, Bond32& b )
 {
     const float ex = static_cast<float>( uj0 - ui0 );
     const float ey = static_cast<float>( uj1 - ui1 );
@@ -122,7 +123,7 @@ __device__ __forceinline__ void getDistance32s( const float4 g, double ui0, doub
     const float xe = g.x * ex + g.y * ey + g.z * ez;
     const float ee = ex * ex + ey * ey + ez * ez;
     b.s = ( 2.0f * xe + ee ) * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
-    b.etan = __builtin_sqrtf( ee ) * __builtin_amdgcn_rcpf( b.xi );
+    b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
 }
 
 // Uniform per-wave access to the spec of this wave's slice.
@@ -461,7 +462,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     {
                         const uint32_t id =
                             SLOTS ? static_cast<uint32_t>( q ) : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
-                        getDistance32s( stab[id], ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2], b );
+                        const float4 g = stab[id];
+                        getDistance32s( g, __builtin_amdgcn_rcpf( g.w ), ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2],
+                                        b );
                     }
                     else
                         getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
@@ -550,7 +553,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
 // existence bits).  Evaluation order and arithmetic are kForce's, so results
 // are identical.
 // ---------------------------------------------------------------------------
-template <int MODEL, bool FRAC, int PREC>
+template <int MODEL, bool FRAC, int PREC, bool GEN>
 __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict__ specs,
                                                         const int32_t* __restrict__ tile_spec,
                                                         const int64_t* __restrict__ tile_slice,
@@ -561,16 +564,23 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     constexpr int kW = kSlice + kRowMaxSlots - 1;  // window width
     constexpr int NX = ( PREC == kF64 ) ? 3 : 0;    // staged x components
     constexpr int NAB = ( MODEL == kLPS ) ? 2 : 0;  // staged lpsA, lpsB
-    constexpr int NC = 3 + NX + NAB;
+    constexpr int KV = 3 + NX + NAB;                // staged vol (GEN)
+    constexpr int KNF = KV + 1;                     // staged no-fail flag (GEN)
+    constexpr int NC = KV + ( GEN ? 2 : 0 );
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
     __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
+    __shared__ float sinv[( PREC == kMixed ) ? 256 : 1];
     __shared__ int4 rtab[kMaxRows];
     WaveCtx w;
     const bool active = waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w );
     // All waves of a block belong to one spec: stage its tables before any
     // wave leaves.
     if constexpr ( PREC == kMixed )
-        stab[threadIdx.x] = w.A->stencil[threadIdx.x];
+    {
+        const float4 g = w.A->stencil[threadIdx.x];
+        stab[threadIdx.x] = g;
+        sinv[threadIdx.x] = __builtin_amdgcn_rcpf( g.w );
+    }
     if ( static_cast<int>( threadIdx.x ) < w.A->nrows )
         rtab[threadIdx.x] = w.A->rows[threadIdx.x];
     __syncthreads();
@@ -594,20 +604,23 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const int32_t iy = static_cast<int32_t>( t1 % ny );
     const int32_t iz = static_cast<int32_t>( t1 / ny );
     uint64_t* __restrict__ bits = A.intact + A.slice_ptr[w.slice];
-    const double* __restrict__ vol = A.vol;
     const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
+    // Loop-invariant scalars held in registers (the args block lives in
+    // global memory, which the bit-word stores could alias).
     const double c = A.c;
     const double bcc = A.bcc;
+    const float c32 = static_cast<float>( A.c );
+    const float vol32 = static_cast<float>( A.vol_value );
+    const double volu = A.vol_value;
+    const float s0f = A.s0f;
+    const float bandf = A.bandf;
+    const int infl1 = A.influence;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( MODEL == kLPS )
     {
         Ai = A.lpsA[pc]; // theta_coeff * theta_i / m_i
         Bi = A.lpsB[pc]; // s_coeff / m_i
     }
-    const int infl1 = A.influence;
-    const bool uvol = A.vol_uniform != 0;
-    const double volu = A.vol_value;
-    const bool any_nofail = A.any_nofail != 0;
 
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if ( accumulate && valid )
@@ -617,7 +630,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
         fz = A.f[2 * n + p];
     }
 
-    const double* src[NC];
+    const double* src[KV];
     src[0] = u;
     src[1] = u + n;
     src[2] = u + 2 * n;
@@ -632,6 +645,8 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
         src[3 + NX] = A.lpsA;
         src[4 + NX] = A.lpsB;
     }
+    const double* __restrict__ volp = A.vol;
+    const uint8_t* __restrict__ nfp = A.nofail;
     double v0[NC], v1[NC];
     uint32_t blo = 0, bhi = 0; // intact words of the row's slots: lane t holds slot q0 + t
     auto loadRow = [&]( int r ) {
@@ -641,10 +656,17 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
         const int64_t g1 = min( max( g + kSlice + lane, int64_t( 0 ) ), n - 1 );
         const bool second = lane < R.y - 1;
 #pragma unroll
-        for ( int k = 0; k < NC; k++ )
+        for ( int k = 0; k < KV; k++ )
         {
             v0[k] = src[k][g0];
             v1[k] = second ? src[k][g1] : 0.0;
+        }
+        if constexpr ( GEN )
+        {
+            v0[KV] = volp[g0];
+            v1[KV] = second ? volp[g1] : 0.0;
+            v0[KNF] = ( FRAC && nfp[g0] != 0 ) ? 1.0 : 0.0;
+            v1[KNF] = ( FRAC && second && nfp[g1] != 0 ) ? 1.0 : 0.0;
         }
         if constexpr ( FRAC )
         {
@@ -685,32 +707,29 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
             const int32_t q = R.x + t;
             const bool ex = rowok && static_cast<uint32_t>( ix + dx0 + t ) < static_cast<uint32_t>( nx );
             uint64_t ww = 0;
+            bool intact = true;
             if constexpr ( FRAC )
             {
                 const uint32_t l32 = __builtin_amdgcn_readlane( rlo, t );
                 const uint32_t h32 = __builtin_amdgcn_readlane( rhi, t );
                 ww = ( static_cast<uint64_t>( h32 ) << 32 ) | l32;
+                intact = __builtin_amdgcn_inverse_ballot_w64( ww );
             }
+            // The window always holds readable values; bonds that do not
+            // exist are masked out of the sums below.
+            const double* __restrict__ wr = wb + lane + t;
+            const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
+            bool nf_j = false;
+            if constexpr ( MODEL == kLPS )
+                nf_j = nf_i;
+            else if constexpr ( GEN && FRAC )
+                nf_j = wr[KNF * kW] != 0.0;
             bool brk = false;
-            if ( ex )
+            if constexpr ( PREC == kF64 )
             {
-                const double* __restrict__ wr = wb + lane + t;
-                const int64_t j = p + R.z + t;
-                const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
-                const double vj = uvol ? volu : vol[j];
-                const bool nf_j =
-                    ( MODEL == kLPS ) ? nf_i : ( ( FRAC && any_nofail ) ? ( A.nofail[j] != 0 ) : false );
-                double aj = 0.0, bj = 0.0;
-                if constexpr ( NAB )
+                if ( ex )
                 {
-                    aj = wr[( 3 + NX ) * kW];
-                    bj = wr[( 4 + NX ) * kW];
-                }
-                bool intact = true;
-                if constexpr ( FRAC )
-                    intact = ( ww >> lane ) & 1ull;
-                if constexpr ( PREC == kF64 )
-                {
+                    const double vj = GEN ? wr[KV * kW] : volu;
                     Bond64 b;
                     getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
                                    b );
@@ -727,51 +746,50 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                         else
                         {
                             const double infl = infl1 ? 1.0 / b.xi : 1.0;
-                            coeff = ( Ai + aj + b.s * ( Bi + bj ) ) * infl * b.xi * vj;
+                            coeff = ( Ai + wr[( 3 + NX ) * kW] + b.s * ( Bi + wr[( 4 + NX ) * kW] ) ) * infl * b.xi * vj;
                         }
                         fx += coeff * b.rx / b.r;
                         fy += coeff * b.ry / b.r;
                         fz += coeff * b.rz / b.r;
                     }
                 }
-                else
+            }
+            else
+            {
+                const float4 g = stab[q];
+                const float xinv = sinv[q];
+                Bond32 b;
+                getDistance32s( g, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
+                if constexpr ( FRAC )
                 {
-                    Bond32 b;
-                    getDistance32s( stab[q], ui0, ui1, ui2, uj0, uj1, uj2, b );
-                    if constexpr ( FRAC )
+                    const bool cand = ex && !nf_i && !nf_j;
+                    const float band = bandf * ( b.etan + fabsf( b.s ) + s0f );
+                    brk = cand && ( b.s > s0f );
+                    if ( cand && fabsf( b.s - s0f ) <= band )
                     {
-                        if ( !nf_i && !nf_j )
-                        {
-                            const float band = A.bandf * ( b.etan + fabsf( b.s ) + A.s0f );
-                            if ( fabsf( b.s - A.s0f ) <= band )
-                            {
-                                // guard band: the reference's FP64 test on this bond
-                                Bond64 e;
-                                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], uj0, uj1,
-                                               uj2, e );
-                                brk = criticalStretch64( e.r, e.xi, bcc );
-                            }
-                            else
-                                brk = b.s > A.s0f;
-                        }
-                    }
-                    if ( !brk && intact )
-                    {
-                        float coeff;
-                        if constexpr ( MODEL == kPMB )
-                            coeff = static_cast<float>( c ) * b.s * static_cast<float>( vj );
-                        else
-                        {
-                            const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
-                            coeff = static_cast<float>( Ai + aj ) + b.s * static_cast<float>( Bi + bj );
-                            coeff = coeff * infl * b.xi * static_cast<float>( vj );
-                        }
-                        const float fr = coeff * b.rinv;
-                        fx += static_cast<double>( fr * b.rx );
-                        fy += static_cast<double>( fr * b.ry );
-                        fz += static_cast<double>( fr * b.rz );
+                        // guard band: the reference's FP64 test on this bond
+                        const int64_t j = p + R.z + t;
+                        Bond64 e;
+                        getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], uj0, uj1, uj2, e );
+                        brk = criticalStretch64( e.r, e.xi, bcc );
                     }
                 }
+                const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
+                float coeff;
+                if constexpr ( MODEL == kPMB )
+                    coeff = c32 * b.s * vj32;
+                else
+                {
+                    const float infl = infl1 ? xinv : 1.0f;
+                    coeff = static_cast<float>( Ai + wr[( 3 + NX ) * kW] ) +
+                            b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kW] );
+                    coeff = coeff * infl * b.xi * vj32;
+                }
+                const bool live = ex && intact && !brk;
+                const float fr = live ? coeff * b.rinv : 0.0f;
+                fx += static_cast<double>( fr * b.rx );
+                fy += static_cast<double>( fr * b.ry );
+                fz += static_cast<double>( fr * b.rz );
             }
             if constexpr ( FRAC )
             {
@@ -1086,20 +1104,34 @@ ForceFn pickForceForm( const PDArgs& a )
     }
 }
 
-ForceFn pickForceRows( const PDArgs& a )
+template <bool GEN>
+ForceFn pickForceRowsGen( const PDArgs& a )
 {
     const int idx = a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 );
     switch ( idx )
     {
-    case 0: return kForceRows<kPMB, false, kF64>;
-    case 1: return kForceRows<kPMB, false, kMixed>;
-    case 2: return kForceRows<kPMB, true, kF64>;
-    case 3: return kForceRows<kPMB, true, kMixed>;
-    case 4: return kForceRows<kLPS, false, kF64>;
-    case 5: return kForceRows<kLPS, false, kMixed>;
-    case 6: return kForceRows<kLPS, true, kF64>;
-    default: return kForceRows<kLPS, true, kMixed>;
+    case 0: return kForceRows<kPMB, false, kF64, GEN>;
+    case 1: return kForceRows<kPMB, false, kMixed, GEN>;
+    case 2: return kForceRows<kPMB, true, kF64, GEN>;
+    case 3: return kForceRows<kPMB, true, kMixed, GEN>;
+    case 4: return kForceRows<kLPS, false, kF64, GEN>;
+    case 5: return kForceRows<kLPS, false, kMixed, GEN>;
+    case 6: return kForceRows<kLPS, true, kF64, GEN>;
+    default: return kForceRows<kLPS, true, kMixed, GEN>;
     }
+}
+
+// The general variant stages per-particle volumes and no-fail flags; the
+// other one uses the uniform volume and no no-fail particles (exact
+// shortcuts detected at upload).
+bool rowsGeneral( const PDArgs& a )
+{
+    return !a.vol_uniform || ( a.fracture && a.any_nofail );
+}
+
+ForceFn pickForceRows( const PDArgs& a )
+{
+    return rowsGeneral( a ) ? pickForceRowsGen<true>( a ) : pickForceRowsGen<false>( a );
 }
 
 ForceFn pickForce( const PDArgs& a )
@@ -1122,8 +1154,9 @@ const char* forceKernelName( const PDArgs& a )
 {
     if ( useRowWindows( a ) )
     {
-        std::snprintf( g_force_name, sizeof( g_force_name ), "kForceRows<%d, %s, %d>", a.model,
-                       a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0 );
+        std::snprintf( g_force_name, sizeof( g_force_name ), "kForceRows<%d, %s, %d, %s>", a.model,
+                       a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0,
+                       rowsGeneral( a ) ? "true" : "false" );
         return g_force_name;
     }
     std::snprintf( g_force_name, sizeof( g_force_name ), "kForce<%d, %s, %d, %d, %s>", a.model,
