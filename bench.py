@@ -35,6 +35,8 @@ DEFAULT_CELLS = (200, 400, 18)
 # plain 1 GiB device copy read through the same counters).  Summaries with
 # the commands are kept under profiles/.
 MEASURED_TRAFFIC = {
+    "kForceRows<0, true, 1, false>": 474.3e6,
+    "kForceRows<0, true, 0, false>": 715.9e6,
     "kForce<0, true, 1, 2, true>": 503.2e6,
     "kForce<0, true, 0, 2, true>": 744.5e6,
 }
@@ -96,11 +98,15 @@ def cpu_baseline(w, bonds, budget_s=12.0):
 
 
 def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
-    """Serving stream: Poisson arrivals, specimens of uneven size.  The mixed
-    form runs through the continuous-batching engine (specimens admitted as
-    they arrive, one packed launch per step for all running specimens); the
-    plain FP64 form serves the same stream one specimen at a time.  Latency is
-    arrival -> result, including specimen setup (upload, bond build, init)."""
+    """Serving stream: Poisson arrivals, specimens of uneven size.  Specimen
+    setup (upload, bond build, prenotch, init) runs on two host worker threads
+    as requests arrive.  The mixed form runs through the continuous-batching
+    engine (specimens admitted first come, first served as slots free up, one
+    packed launch per step for all running specimens); the plain FP64 form
+    serves the same stream one specimen at a time.  Latency is arrival ->
+    result, setup included."""
+    from concurrent.futures import ThreadPoolExecutor
+
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
     reqs = [configs.serving_request(rng, n_range=(2e4, max_particles)) for _ in range(num_requests)]
@@ -118,52 +124,59 @@ def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
         return {"solves_per_s": len(lat) / wall, "p50_s": float(np.percentile(lat, 50)),
                 "p99_s": float(np.percentile(lat, 99)), "wall_s": wall, "bond_evals_per_s": bond_evals / wall}
 
-    # continuous batching, mixed form
-    ctx = engine.Context(0)
-    t0 = time.perf_counter()
-    pending = list(range(num_requests))
-    running = {}
-    lat = [0.0] * num_requests
-    bond_evals = 0.0
-    while pending or running:
-        now = time.perf_counter() - t0
-        while pending and arrivals[pending[0]] <= now:
-            k = pending.pop(0)
-            s = make(reqs[k], "mixed", ctx)
-            bond_evals += s.bond_stats()[0] * reqs[k]["steps"]
-            running[k] = (s, ctx.submit(s, reqs[k]["steps"]))
-        if running:
-            ctx.progress(4)
-        for k in list(running):
-            s, t = running[k]
-            if ctx.poll(t):
-                lat[k] = time.perf_counter() - t0 - arrivals[k]
-                s.close()
-                del running[k]
-        if not running and pending:
-            time.sleep(max(0.0, arrivals[pending[0]] - (time.perf_counter() - t0)))
-    mixed = summarize(lat, time.perf_counter() - t0, bond_evals)
-    ctx.close()
+    def stream(precision, batched):
+        ctx = engine.Context(0)
+        pool = ThreadPoolExecutor(max_workers=2)
+        t0 = time.perf_counter()
+        setups = {}
+        arrived = 0
+        started = 0
+        running = {}
+        lat = [0.0] * num_requests
+        finished = 0
+        bond_evals = 0.0
+        while finished < num_requests:
+            now = time.perf_counter() - t0
+            while arrived < num_requests and arrivals[arrived] <= now:
+                setups[arrived] = pool.submit(make, reqs[arrived], precision, ctx)
+                arrived += 1
+            # start specimens in arrival order once their setup is done
+            while started < arrived and (setups[started].done() or not batched):
+                s = setups.pop(started).result()
+                bond_evals += s.bond_stats()[0] * reqs[started]["steps"]
+                if batched:
+                    running[started] = (s, ctx.submit(s, reqs[started]["steps"]))
+                else:
+                    s.step(reqs[started]["steps"])  # one specimen at a time
+                    lat[started] = time.perf_counter() - t0 - arrivals[started]
+                    s.close()
+                    finished += 1
+                    started += 1
+                    break  # take in the arrivals of that solve first
+                started += 1
+            if running:
+                ctx.progress(16)
+                for k in list(running):
+                    s, t = running[k]
+                    if ctx.poll(t):
+                        lat[k] = time.perf_counter() - t0 - arrivals[k]
+                        s.close()
+                        del running[k]
+                        finished += 1
+            elif finished < num_requests:
+                nxt = arrivals[arrived] if arrived < num_requests else now + 1e-3
+                time.sleep(min(1e-3, max(0.0, nxt - (time.perf_counter() - t0))))
+        wall = time.perf_counter() - t0
+        pool.shutdown()
+        ctx.close()
+        return summarize(lat, wall, bond_evals)
 
-    # plain FP64, one specimen at a time
-    ctx = engine.Context(0)
-    t0 = time.perf_counter()
-    lat = [0.0] * num_requests
-    bond_evals = 0.0
-    for k in range(num_requests):
-        wait = arrivals[k] - (time.perf_counter() - t0)
-        if wait > 0:
-            time.sleep(wait)
-        s = make(reqs[k], "fp64", ctx)
-        bond_evals += s.bond_stats()[0] * reqs[k]["steps"]
-        s.step(reqs[k]["steps"])
-        lat[k] = time.perf_counter() - t0 - arrivals[k]
-        s.close()
-    fp64 = summarize(lat, time.perf_counter() - t0, bond_evals)
-    ctx.close()
+    mixed = stream("mixed", True)
+    fp64 = stream("fp64", False)
     sizes = [r["x"].shape[0] for r in reqs]
     return {"requests": num_requests, "arrival_rate_per_s": rate, "seed": seed,
             "particles_min": int(min(sizes)), "particles_max": int(max(sizes)),
+            "steps_total": int(sum(r["steps"] for r in reqs)),
             "mixed_batched": mixed, "fp64_per_specimen": fp64}
 
 

@@ -1,6 +1,7 @@
 // pdk_api.hip -- the C ABI (include/pdk.h): specimen state on one GPU and the
 // host orchestration of the reference's Solver step order.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -127,7 +128,7 @@ struct pdk_ctx_s
     int device = 0;
     MemAccount acct;
     std::unique_ptr<BatchEngine> batch;
-    int64_t live_specs = 0;
+    std::atomic<int64_t> live_specs{ 0 };
 };
 
 struct pdk_spec_s
@@ -214,6 +215,7 @@ struct pdk_spec_s
             h.soff = slot.soff.as<int32_t>();
             h.stencil_slots = slot.slots;
             h.rows = slot.nrows > 0 ? slot.rows.as<int4>() : nullptr;
+            h.rowstart = slot.nrows > 0 ? slot.rowstart.as<uint32_t>() : nullptr;
             h.nrows = slot.nrows;
             for ( int dd = 0; dd < 3; dd++ )
                 h.lat_n[dd] = static_cast<int32_t>( lat_dims[dd] );
@@ -221,6 +223,7 @@ struct pdk_spec_s
         else
         {
             h.rows = nullptr;
+            h.rowstart = nullptr;
             h.nrows = 0;
         }
         // tuning knob: -1 = default (row windows where available, else groups of 4)
@@ -449,7 +452,7 @@ int pdk_ctx_bytes_in_use( pdk_ctx ctx, uint64_t* bytes )
 {
     return guarded( [&] {
         require( ctx && bytes, "null argument" );
-        *bytes = ctx->acct.in_use;
+        *bytes = ctx->acct.used();
     } );
 }
 

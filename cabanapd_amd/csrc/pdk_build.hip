@@ -1016,13 +1016,19 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
         const int32_t packed = ( es[e].a + 128 ) | ( ( es[e].b + 128 ) << 8 ) | ( ( es[e].c + 128 ) << 16 );
         rtab.push_back( make_int4( e, 1, static_cast<int32_t>( es[e].off ), packed ) );
     }
-    DevBuf drows;
+    DevBuf drows, drstart;
     int32_t nrows = 0;
-    if ( static_cast<int>( rtab.size() ) <= kMaxRows )
+    // the row-window pass addresses particles with 32-bit byte offsets
+    if ( static_cast<int>( rtab.size() ) <= kMaxRows && n < ( int64_t( 1 ) << 29 ) )
     {
         nrows = static_cast<int32_t>( rtab.size() );
         drows.alloc( nrows * sizeof( int4 ), acct );
         PDK_HIP( hipMemcpyAsync( drows.as<int4>(), rtab.data(), nrows * sizeof( int4 ), hipMemcpyHostToDevice, st ) );
+        std::vector<uint32_t> rs( 9, 0u );
+        for ( const int4& r : rtab )
+            rs[r.x >> 5] |= 1u << ( r.x & 31 );
+        drstart.alloc( 9 * sizeof( uint32_t ), acct );
+        PDK_HIP( hipMemcpyAsync( drstart.as<uint32_t>(), rs.data(), 9 * sizeof( uint32_t ), hipMemcpyHostToDevice, st ) );
     }
     dsoff.alloc( 256 * sizeof( int32_t ), acct );
     dtab.alloc( 256 * sizeof( float4 ), acct );
@@ -1035,6 +1041,7 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
     out.soff = std::move( dsoff );
     out.table = std::move( dtab );
     out.rows = std::move( drows );
+    out.rowstart = std::move( drstart );
     out.nrows = nrows;
     out.slots = S;
     total_rows = rows;

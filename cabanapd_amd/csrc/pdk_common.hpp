@@ -108,6 +108,7 @@ struct PDArgs
     // contiguous id window.  rows[r] = (first slot, slot count, id offset of
     // the first slot's neighbor, packed (dx0, dy, dz) + 128 per byte).
     const int4* rows;
+    const uint32_t* rowstart; // bit q set: slot q starts a row (9 words), or null
     int32_t nrows;          // 0: no row table
     int32_t lat_n[3];       // lattice extent (stencil-slot form)
 };

@@ -242,7 +242,13 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     // slot offsets) in LDS before any wave leaves.
     __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
     __shared__ int32_t soffs[SLOTS ? 256 + kBatch : 1];
+    __shared__ uint32_t rsm[( PREC == kMixed ) ? 9 : 1]; // row-start bits of the slots (stencil-slot form)
     const bool useSten = ( PREC == kMixed ) && ( SLOTS || w.A->sid != nullptr );
+    if constexpr ( PREC == kMixed )
+    {
+        if ( threadIdx.x < 9 )
+            rsm[threadIdx.x] = ( SLOTS && w.A->rowstart ) ? w.A->rowstart[threadIdx.x] : 0xffffffffu;
+    }
     if constexpr ( SLOTS )
     {
         soffs[threadIdx.x] = w.A->soff[threadIdx.x];
@@ -301,6 +307,11 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         fy = A.f[n + p];
         fz = A.f[2 * n + p];
     }
+    // Mixed form: bond forces are summed in FP32 within a lattice row of the
+    // stencil (at most kRowMaxSlots bonds) and the row sums in FP64; without a
+    // row table every bond is its own row.
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    const float cv32u = static_cast<float>( c ) * static_cast<float>( A.vol_value );
     // The neighbor ids and broken-bond words of kBatch slots are fetched one
     // batch ahead (the index stream comes from HBM; keeping a batch in
     // flight hides its latency).  Within a batch, slots are processed UNR at
@@ -424,6 +435,16 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         {
             const int32_t q = q0 + k;
             bool brk = false;
+            if constexpr ( PREC == kMixed )
+            {
+                if ( ( rsm[q >> 5] >> ( q & 31 ) ) & 1u )
+                {
+                    fx += static_cast<double>( px );
+                    fy += static_cast<double>( py );
+                    fz += static_cast<double>( pz );
+                    px = py = pz = 0.0f;
+                }
+            }
             if ( ex[k] )
             {
                 bool intact = true;
@@ -497,7 +518,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     {
                         float coeff;
                         if constexpr ( MODEL == kPMB )
-                            coeff = static_cast<float>( c ) * b.s * static_cast<float>( vj[k] );
+                            coeff = ( uvol ? cv32u : static_cast<float>( c ) * static_cast<float>( vj[k] ) ) * b.s;
                         else
                         {
                             const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
@@ -505,9 +526,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                             coeff = coeff * infl * b.xi * static_cast<float>( vj[k] );
                         }
                         const float fr = coeff * b.rinv;
-                        fx += static_cast<double>( fr * b.rx );
-                        fy += static_cast<double>( fr * b.ry );
-                        fz += static_cast<double>( fr * b.rz );
+                        px = fmaf( fr, b.rx, px );
+                        py = fmaf( fr, b.ry, py );
+                        pz = fmaf( fr, b.rz, pz );
                     }
                 }
             }
@@ -535,6 +556,12 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         whi = nhi;
         elo = mlo;
         ehi = mhi;
+    }
+    if constexpr ( PREC == kMixed )
+    {
+        fx += static_cast<double>( px );
+        fy += static_cast<double>( py );
+        fz += static_cast<double>( pz );
     }
     if ( !valid )
         return;
@@ -611,6 +638,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const double bcc = A.bcc;
     const float c32 = static_cast<float>( A.c );
     const float vol32 = static_cast<float>( A.vol_value );
+    const float cv32u = c32 * vol32;
     const double volu = A.vol_value;
     const float s0f = A.s0f;
     const float bandf = A.bandf;
@@ -649,22 +677,29 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const uint8_t* __restrict__ nfp = A.nofail;
     double v0[NC], v1[NC];
     uint32_t blo = 0, bhi = 0; // intact words of the row's slots: lane t holds slot q0 + t
+    // Row tables exist only for n < 2^29, so ids and byte offsets fit 32 bits.
+    const int32_t n32 = static_cast<int32_t>( n );
+    const int32_t p032 = static_cast<int32_t>( p0 );
+    auto ld = []( const double* base, uint32_t byte_off ) {
+        return *reinterpret_cast<const double*>( reinterpret_cast<const char*>( base ) + byte_off );
+    };
     auto loadRow = [&]( int r ) {
         const int4 R = rtab[r];
-        const int64_t g = p0 + R.z;
-        const int64_t g0 = min( max( g + lane, int64_t( 0 ) ), n - 1 );
-        const int64_t g1 = min( max( g + kSlice + lane, int64_t( 0 ) ), n - 1 );
+        const int32_t g = p032 + R.z;
+        const int32_t g0 = min( max( g + lane, 0 ), n32 - 1 );
+        const int32_t g1 = min( max( g + kSlice + lane, 0 ), n32 - 1 );
+        const uint32_t o0 = static_cast<uint32_t>( g0 ) * 8u, o1 = static_cast<uint32_t>( g1 ) * 8u;
         const bool second = lane < R.y - 1;
 #pragma unroll
         for ( int k = 0; k < KV; k++ )
         {
-            v0[k] = src[k][g0];
-            v1[k] = second ? src[k][g1] : 0.0;
+            v0[k] = ld( src[k], o0 );
+            v1[k] = second ? ld( src[k], o1 ) : 0.0;
         }
         if constexpr ( GEN )
         {
-            v0[KV] = volp[g0];
-            v1[KV] = second ? volp[g1] : 0.0;
+            v0[KV] = ld( volp, o0 );
+            v1[KV] = second ? ld( volp, o1 ) : 0.0;
             v0[KNF] = ( FRAC && nfp[g0] != 0 ) ? 1.0 : 0.0;
             v1[KNF] = ( FRAC && second && nfp[g1] != 0 ) ? 1.0 : 0.0;
         }
@@ -702,6 +737,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
         const int32_t dz = ( ( R.w >> 16 ) & 0xff ) - 128;
         const bool rowok = valid && static_cast<uint32_t>( iy + dy ) < static_cast<uint32_t>( ny ) &&
                            static_cast<uint32_t>( iz + dz ) < static_cast<uint32_t>( nz );
+        float px = 0.0f, py = 0.0f, pz = 0.0f; // FP32 sum of this row's bond forces (mixed form)
         for ( int t = 0; t < R.y; t++ )
         {
             const int32_t q = R.x + t;
@@ -777,7 +813,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                 const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
                 float coeff;
                 if constexpr ( MODEL == kPMB )
-                    coeff = c32 * b.s * vj32;
+                    coeff = ( GEN ? c32 * vj32 : cv32u ) * b.s;
                 else
                 {
                     const float infl = infl1 ? xinv : 1.0f;
@@ -787,9 +823,9 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                 }
                 const bool live = ex && intact && !brk;
                 const float fr = live ? coeff * b.rinv : 0.0f;
-                fx += static_cast<double>( fr * b.rx );
-                fy += static_cast<double>( fr * b.ry );
-                fz += static_cast<double>( fr * b.rz );
+                px = fmaf( fr, b.rx, px );
+                py = fmaf( fr, b.ry, py );
+                pz = fmaf( fr, b.rz, pz );
             }
             if constexpr ( FRAC )
             {
@@ -797,6 +833,12 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                 if ( ( m & ww ) && lane == 0 )
                     bits[q] = ww & ~m;
             }
+        }
+        if constexpr ( PREC == kMixed )
+        {
+            fx += static_cast<double>( px );
+            fy += static_cast<double>( py );
+            fz += static_cast<double>( pz );
         }
     }
     if ( !valid )

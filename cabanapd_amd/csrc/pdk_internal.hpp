@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,10 +31,32 @@ struct Error : std::runtime_error
 
 // Accounting hook: every device allocation of a spec goes through its pool so
 // the context can enforce the HBM budget.
+// HBM accounting of one context.  Specimens may be set up on one host thread
+// while another drives the batch engine, so updates are serialized.
 struct MemAccount
 {
     uint64_t budget = 0;
     uint64_t in_use = 0;
+    std::mutex mu;
+
+    void reserve( uint64_t bytes )
+    {
+        std::lock_guard<std::mutex> g( mu );
+        if ( budget && in_use + bytes > budget )
+            throw Error( "HBM budget exceeded: need " + std::to_string( bytes ) + " B more, " +
+                         std::to_string( budget - in_use ) + " B left" );
+        in_use += bytes;
+    }
+    void unreserve( uint64_t bytes )
+    {
+        std::lock_guard<std::mutex> g( mu );
+        in_use -= bytes;
+    }
+    uint64_t used()
+    {
+        std::lock_guard<std::mutex> g( mu );
+        return in_use;
+    }
 };
 
 // Owning device buffer.
@@ -57,14 +80,18 @@ class DevBuf
         release();
         if ( bytes == 0 )
             return;
-        if ( acct && acct->budget && acct->in_use + bytes > acct->budget )
-            throw Error( "HBM budget exceeded: need " + std::to_string( bytes ) + " B more, " +
-                         std::to_string( acct->budget - acct->in_use ) + " B left" );
-        PDK_HIP( hipMalloc( &ptr_, bytes ) );
+        if ( acct )
+            acct->reserve( bytes );
+        const hipError_t e = hipMalloc( &ptr_, bytes );
+        if ( e != hipSuccess )
+        {
+            ptr_ = nullptr;
+            if ( acct )
+                acct->unreserve( bytes );
+            PDK_HIP( e );
+        }
         bytes_ = bytes;
         acct_ = acct;
-        if ( acct_ )
-            acct_->in_use += bytes_;
     }
     void release()
     {
@@ -72,7 +99,7 @@ class DevBuf
         {
             (void)hipFree( ptr_ );
             if ( acct_ )
-                acct_->in_use -= bytes_;
+                acct_->unreserve( bytes_ );
         }
         ptr_ = nullptr;
         bytes_ = 0;
@@ -127,7 +154,7 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
 // specimen does not qualify.
 struct SlotForm
 {
-    DevBuf exist, soff, table, rows;
+    DevBuf exist, soff, table, rows, rowstart;
     int32_t slots = 0;
     int32_t nrows = 0; // row windows (0: the stencil does not split into <= kMaxRows rows)
 };

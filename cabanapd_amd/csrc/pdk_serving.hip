@@ -50,6 +50,13 @@ struct BatchEngine::Impl
     int64_t L = 0;
     uint64_t next_ticket = 1;
     size_t max_slots = 512;
+    // Admission keeps the running specimens within about this many slices
+    // (enough wavefronts to fill the GPU several times over): arrivals are
+    // admitted first come, first served, and small specimens share launches
+    // while a large one runs nearly alone.
+    int64_t max_slices = 16384;
+    int64_t running_slices = 0;
+    int32_t max_inflight = 16;  // packed steps queued ahead of the host
     std::deque<Req> pending;
     std::vector<Run> running;
     std::map<uint64_t, hipEvent_t> done_events;
@@ -61,6 +68,10 @@ struct BatchEngine::Impl
     explicit Impl( pdk_ctx_s* c ) : ctx( c )
     {
         PDK_HIP( hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) );
+        int dev = 0, cus = 0;
+        PDK_HIP( hipGetDevice( &dev ) );
+        if ( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) == hipSuccess && cus > 0 )
+            max_slices = static_cast<int64_t>( cus ) * 64;
     }
     ~Impl()
     {
@@ -76,9 +87,12 @@ struct BatchEngine::Impl
     {
         while ( !pending.empty() && running.size() < max_slots )
         {
+            SpecView v = view( pending.front().s );
+            if ( !running.empty() && running_slices + v.nslices > max_slices )
+                break;
             Req r = pending.front();
             pending.pop_front();
-            SpecView v = view( r.s );
+            running_slices += v.nslices;
             // The spec's own stream may still be finishing its setup.
             hipEvent_t ready;
             PDK_HIP( hipEventCreateWithFlags( &ready, hipEventDisableTiming ) );
@@ -171,6 +185,7 @@ struct BatchEngine::Impl
                 PDK_HIP( hipEventRecord( ev, st ) );
             }
             SpecView v = view( r.s );
+            running_slices -= v.nslices;
             // The last launch (index end_launch - 1) did not flip the buffer.
             *v.cur = static_cast<int32_t>( ( ( r.end_launch - 1 ) & 1 ) ) ^ r.parity_offset;
             *v.steps_done += r.nsteps;
@@ -212,7 +227,7 @@ struct BatchEngine::Impl
             PDK_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
             PDK_HIP( hipEventRecord( e, st ) );
             inflight.push_back( e );
-            while ( inflight.size() > 4 )
+            while ( inflight.size() > static_cast<size_t>( max_inflight ) )
             {
                 PDK_HIP( hipEventSynchronize( inflight.front() ) );
                 PDK_HIP( hipEventDestroy( inflight.front() ) );
@@ -248,6 +263,9 @@ struct BatchEngine::Impl
         pending.erase( std::remove_if( pending.begin(), pending.end(), [&]( const Req& r ) { return r.s == s; } ),
                        pending.end() );
         size_t before = running.size();
+        for ( const Run& r : running )
+            if ( r.s == s )
+                running_slices -= view( s ).nslices;
         running.erase( std::remove_if( running.begin(), running.end(), [&]( const Run& r ) { return r.s == s; } ),
                        running.end() );
         if ( running.size() != before )
