@@ -54,9 +54,12 @@ def make_spec(ctx, w, precision, bond_layout=0):
     d = type(w["desc"]).from_buffer_copy(w["desc"])
     d.precision = {"fp64": 0, "mixed": 1}[precision]
     d.bond_layout = bond_layout
+    t0 = time.perf_counter()
     s = engine.Spec(ctx, d, w["x"].shape[0])
     configs.upload(s, w)
     configs.setup(s, w)
+    s.sync()
+    s.setup_s = time.perf_counter() - t0
     return s
 
 
@@ -193,6 +196,83 @@ def copy_kernel(mib, repeats=5):
     print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4, "repeats": repeats}))
 
 
+def lps_leg(args, n_side=128):
+    """LPS elastic-brittle cube 128^3 (2.1M particles): weighted volume,
+    dilatation and force passes every step, both precision forms."""
+    from cabanapd_amd._abi import KCLS_LPS
+    w = configs.lps_cube(n_side=n_side, steps=args.steps, fracture=True)
+    n = w["x"].shape[0]
+    ctx = engine.Context(0)
+    res, specs = {}, {}
+    for prec in ("fp64", "mixed"):
+        s = make_spec(ctx, w, prec)
+        bonds, _ = s.bond_stats()
+        unroll = s.autotune(3) if args.unroll < 0 else (s.set_tuning(args.unroll) or args.unroll)
+        s.step(args.warmup) if args.warmup else None
+        s.sync()
+        s.timing(True)
+        t0 = time.perf_counter()
+        s.step(args.steps)
+        s.sync()
+        wall = time.perf_counter() - t0
+        fms, fl = s.kernel_time(KCLS_FORCE)
+        lms, ll = s.kernel_time(KCLS_LPS)
+        s.timing(False)
+        res[prec] = dict(value=bonds * args.steps / wall, ms_per_step=wall / args.steps * 1e3,
+                         force_kernel=s.kernel_name(KCLS_FORCE), force_avg_ms=fms / max(fl, 1),
+                         state_kernel=s.kernel_name(KCLS_LPS), state_avg_ms=lms / max(ll, 1), unroll=unroll)
+        specs[prec] = s
+    u64, u32 = specs["fp64"].get("u"), specs["mixed"].get("u")
+    den = float(np.max(np.abs(u64)))
+    _, _, i64 = specs["fp64"].bonds()
+    _, _, i32 = specs["mixed"].bonds()
+    for s in specs.values():
+        s.close()
+    ctx.close()
+    b_alg = algorithmic_bytes_per_step(n, bonds, passes=3, fracture=True, lps=True)
+    return {"workload": "LPS elastic-brittle cube %d^3 (%d particles, %d bonds), seeded perturbation, "
+                        "+-10 MPa y-face traction, %d steps" % (n_side, n, bonds, args.steps),
+            "metric": "bond-evals/s", "mixed": res["mixed"], "fp64": res["fp64"],
+            "speedup": res["mixed"]["value"] / res["fp64"]["value"],
+            "u_rel_err": float(np.max(np.abs(u32 - u64)) / den) if den > 0 else 0.0,
+            "broken_bonds_fp64": int((i64 == 0).sum()), "broken_bond_mismatches": int((i64 != i32).sum()),
+            "alg_bytes_per_step": b_alg,
+            "achieved_GBs_mixed": b_alg / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9}
+
+
+def dem_leg(args, n=1000000):
+    """Hertz-JKR contact box: 1e6 polydisperse spheres (radii log-normal,
+    sigma 0.1, clipped to +-30 %) on a jittered lattice, gravity, restitution
+    0.5, surface energy 0.05 J/m^2, contact list rebuilt every 20 steps."""
+    w = configs.dem_packing(n=n, model="HERTZ_JKR", polydisperse=True, steps=args.steps)
+    ctx = engine.Context(0)
+    s = engine.Spec(ctx, w["desc"], w["x"].shape[0])
+    configs.upload(s, w)
+    configs.setup(s, w)
+    s.step(args.warmup) if args.warmup else None
+    s.sync()
+    s.timing(True)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    fms, fl = s.kernel_time(KCLS_FORCE)
+    bms, bl = s.kernel_time(3)
+    oms, ol = s.kernel_time(4)
+    s.timing(False)
+    pairs, maxn = s.bond_stats()
+    r = w["radius"]
+    frac = float(np.sum(4.0 / 3.0 * np.pi * r ** 3) / np.prod(w["x"].max(0) - w["x"].min(0) + 2 * r.max()))
+    s.close()
+    ctx.close()
+    return {"workload": "Hertz-JKR contact box, %d polydisperse spheres (solid fraction %.3f), gravity, "
+                        "list rebuilt every 20 steps, %d steps" % (n, frac, args.steps),
+            "metric": "contact-pair evals/s", "value": pairs * args.steps / wall,
+            "steps_per_s": args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+            "list_pairs": pairs, "max_neighbors": maxn,
+            "force_avg_ms": fms / max(fl, 1), "build_ms_total": bms, "builds": bl, "other_ms_total": oms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +286,9 @@ def main():
     ap.add_argument("--max-particles", type=float, default=2e6)
     ap.add_argument("--bond-layout", choices=("auto", "compact"), default="auto",
                     help="auto: stencil slots on full lattices; compact: stored neighbor ids")
+    ap.add_argument("--workload", choices=("plate", "lps", "dem"), default="plate",
+                    help="plate: the PMB fracture plate (the benchmark line); lps: LPS fracture cube "
+                         "128^3; dem: Hertz-JKR contact box of 1e6 spheres")
     ap.add_argument("--copy-kernel", type=float, default=0.0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -213,9 +296,15 @@ def main():
         copy_kernel(args.copy_kernel)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; sharding across GPUs is design-only "
-                                   "(see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; sharding across GPUs is a design, "
+                                   "not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "lps":
+        print(json.dumps(lps_leg(args)))
+        return
+    if args.workload == "dem":
+        print(json.dumps(dem_leg(args)))
+        return
     cells = tuple(int(c) for c in args.cells.split(","))
     w = configs.kw_plate(num_cells=cells, steps=args.steps)
     n = w["x"].shape[0]
@@ -232,7 +321,7 @@ def main():
             unroll = s.autotune(3)
         wall, force_ms, launches = timed_run(s, args.warmup, args.steps)
         res[prec] = dict(wall=wall, force_ms=force_ms, launches=launches, bonds=bonds,
-                         kernel=s.kernel_name(KCLS_FORCE), unroll=unroll, layout=s.layout())
+                         kernel=s.kernel_name(KCLS_FORCE), unroll=unroll, layout=s.layout(), setup_s=s.setup_s)
         specs[prec] = s
     bonds = res["mixed"]["bonds"]
     u64 = specs["fp64"].get("u")
@@ -268,7 +357,7 @@ def main():
         "config": {"workload": "PMB brittle fracture pre-notched plate %dx%dx%d (%d particles, %d bonds), "
                                "edge impact 16.5 m/s, dt 7.5e-8" % (cells + (n, bonds)),
                    "particles": n, "bonds": bonds, "precision": "mixed", "layout": m["layout"],
-                   "unroll": m["unroll"]},
+                   "unroll": m["unroll"], "setup_s": m["setup_s"]},
         "fp64": {"value": fp64_value, "unit": "bond-evals/s", "ms_per_step": res["fp64"]["wall"] / args.steps * 1e3,
                  "kernel": res["fp64"]["kernel"],
                  "force_kernel_avg_ms": res["fp64"]["force_ms"] / max(res["fp64"]["launches"], 1)},

@@ -676,18 +676,32 @@ int pdk_build_bonds( pdk_spec s )
             q.rc = s->d.horizon + s->d.neighbor_tol;
             q.extend = 0.0;
             q.n = s->n;
-            s->timed( kClsBuild, [&] {
-                s->total_rows = buildSlicedCsr( q, s->perm.as<int64_t>(), s->inv.as<int64_t>(),
-                                                s->slice_ptr.as<int64_t>(), s->cnt.as<int32_t>(), s->col, s->intact,
-                                                s->col_rows_cap, s->scratch, s->acct(), s->st, &s->max_cnt );
-            } );
             s->sid.release();
             s->stab.release();
             s->nsten = 0;
             s->slot = SlotForm{};
             // Full lattice: slot q of every row is stencil entry q, so the
             // force pass computes neighbor ids instead of streaming them.
+            // Built straight from the lattice where the stencil is clear of
+            // the cutoff, else from the cell search below.
+            bool direct = false;
             if ( s->lattice_rows && s->d.bond_layout != PDK_BONDS_COMPACT )
+                s->timed( kClsBuild, [&] {
+                    direct = buildStencilSlotsDirect( s->x.as<double>(), s->n, s->lat_dims, s->lat_h, q.rc, s->h_perm,
+                                                      s->perm.as<int64_t>(), s->cnt.as<int32_t>(), s->col, s->intact,
+                                                      s->slice_ptr.as<int64_t>(), s->total_rows, s->col_rows_cap,
+                                                      s->slot, s->scratch, s->acct(), s->st );
+                } );
+            if ( direct )
+                s->nsten = s->slot.slots;
+            else
+                s->timed( kClsBuild, [&] {
+                    s->total_rows = buildSlicedCsr( q, s->perm.as<int64_t>(), s->inv.as<int64_t>(),
+                                                    s->slice_ptr.as<int64_t>(), s->cnt.as<int32_t>(), s->col,
+                                                    s->intact, s->col_rows_cap, s->scratch, s->acct(), s->st,
+                                                    &s->max_cnt );
+                } );
+            if ( !direct && s->lattice_rows && s->d.bond_layout != PDK_BONDS_COMPACT )
             {
                 s->pushArgs();
                 s->timed( kClsBuild, [&] {

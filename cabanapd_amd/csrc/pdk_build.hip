@@ -892,18 +892,28 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
     return true;
 }
 
-bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3], double rc,
-                        const std::vector<int64_t>& perm_host, DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,
-                        int64_t& total_rows, int64_t& col_capacity_rows, SlotForm& out, BuildScratch& s,
-                        MemAccount* acct, hipStream_t st )
+namespace
 {
-    const int64_t n = a.n;
+struct SlotEntry
+{
+    int32_t a, b, c;
+    int64_t off;
+    double r2;
+};
+
+// Stencil entries of a full lattice in slot order.  margin: relative band
+// around the cutoff (in squared distance) inside which no lattice vector may
+// lie, since there the rounding of the actual positions decides.
+bool slotStencil( int64_t n, const int64_t dims[3], const double h[3], double rc,
+                  const std::vector<int64_t>& perm_host, double margin, std::vector<SlotEntry>& es,
+                  LatticeBox& L, bool& identity )
+{
     if ( dims[0] * dims[1] * dims[2] != n || static_cast<int64_t>( perm_host.size() ) != n )
         return false;
     // Caller order equal to the lattice order: slots sorted by id offset are
     // sorted by caller id for every particle.  Otherwise the slot order is
-    // taken from an interior particle (which must exist) and verified below.
-    bool identity = true;
+    // taken from an interior particle (which must exist) and verified.
+    identity = true;
     for ( int64_t p = 0; p < n && identity; p++ )
         identity = perm_host[p] == p;
     int32_t K[3];
@@ -912,19 +922,12 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
         if ( !( h[d] > 0 ) )
             return false;
         // largest integer offset along d that can be within the cutoff
-        K[d] = static_cast<int32_t>( std::floor( rc / h[d] * ( 1.0 + 1e-9 ) ) );
+        K[d] = static_cast<int32_t>( std::floor( rc / h[d] * ( 1.0 + margin ) ) );
         if ( K[d] > 40 || ( !identity && dims[d] <= 2 * K[d] ) )
             return false;
     }
-    const LatticeBox L{ static_cast<int32_t>( dims[0] ), static_cast<int32_t>( dims[1] ),
-                        static_cast<int32_t>( dims[2] ) };
-    struct Entry
-    {
-        int32_t a, b, c;
-        int64_t off;
-        double r2;
-    };
-    std::vector<Entry> es;
+    L = LatticeBox{ static_cast<int32_t>( dims[0] ), static_cast<int32_t>( dims[1] ), static_cast<int32_t>( dims[2] ) };
+    es.clear();
     const double rc2 = rc * rc;
     for ( int32_t c = -K[2]; c <= K[2]; c++ )
         for ( int32_t b = -K[1]; b <= K[1]; b++ )
@@ -934,9 +937,7 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
                 const double r2 = gx * gx + gy * gy + gz * gz;
                 if ( r2 == 0.0 )
                     continue;
-                // a lattice vector this close to the cutoff is decided by the
-                // rounding of the actual positions, particle by particle
-                if ( std::fabs( r2 - rc2 ) <= 1e-6 * rc2 )
+                if ( std::fabs( r2 - rc2 ) <= margin * rc2 )
                     return false;
                 if ( r2 > rc2 )
                     continue;
@@ -949,55 +950,46 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
     if ( S == 0 || S > 255 )
         return false;
     if ( identity )
-        std::stable_sort( es.begin(), es.end(), []( const Entry& u, const Entry& v ) { return u.off < v.off; } );
+        std::stable_sort( es.begin(), es.end(), []( const SlotEntry& u, const SlotEntry& v ) { return u.off < v.off; } );
     else
     {
         // ascending caller id of an interior particle's neighbors
         const int64_t p0 = ( static_cast<int64_t>( K[2] ) * L.ny + K[1] ) * L.nx + K[0];
-        std::stable_sort( es.begin(), es.end(), [&]( const Entry& u, const Entry& v ) {
+        std::stable_sort( es.begin(), es.end(), [&]( const SlotEntry& u, const SlotEntry& v ) {
             return perm_host[p0 + u.off] < perm_host[p0 + v.off];
         } );
     }
-    std::vector<int4> ent( S );
-    for ( int32_t e = 0; e < S; e++ )
-        ent[e] = make_int4( es[e].a, es[e].b, es[e].c, static_cast<int32_t>( es[e].off ) );
-    DevBuf dent;
-    dent.alloc( S * sizeof( int4 ), acct );
-    ensure<int32_t>( s.flag, 1, acct );
-    PDK_HIP( hipMemcpyAsync( dent.as<int4>(), ent.data(), S * sizeof( int4 ), hipMemcpyHostToDevice, st ) );
-    PDK_HIP( hipMemsetAsync( s.flag.as<int32_t>(), 0, sizeof( int32_t ), st ) );
-    kCheckStencilSlots<<<gridFor( n ), kThreads, 0, st>>>( a, L, dent.as<int4>(), S, s.flag.as<int32_t>() );
-    PDK_HIP( hipGetLastError() );
-    int32_t flag = 0;
-    PDK_HIP( hipMemcpyAsync( &flag, s.flag.as<int32_t>(), sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
-    PDK_HIP( hipStreamSynchronize( st ) );
-    if ( flag )
-        return false;
+    return true;
+}
 
-    // Rebuild the slot arrays.
-    const int64_t nslices = a.nslices;
+std::vector<int4> slotEntries( const std::vector<SlotEntry>& es )
+{
+    std::vector<int4> ent( es.size() );
+    for ( size_t e = 0; e < es.size(); e++ )
+        ent[e] = make_int4( es[e].a, es[e].b, es[e].c, static_cast<int32_t>( es[e].off ) );
+    return ent;
+}
+
+// Fill the slot arrays, the row table, offsets and FP32 stencil table and
+// move them into place.
+void finishSlots( int64_t n, int64_t nslices, const LatticeBox& L, const std::vector<SlotEntry>& es,
+                  const int4* dent, const double h[3], DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,
+                  int64_t& total_rows, int64_t& col_capacity_rows, SlotForm& out, MemAccount* acct,
+                  hipStream_t st )
+{
+    const int32_t S = static_cast<int32_t>( es.size() );
     const int64_t rows = nslices * S;
     DevBuf ncol, nintact, nexist, dsoff, dtab;
     ncol.alloc( static_cast<size_t>( rows ) * kSlice * sizeof( int32_t ), acct );
     nintact.alloc( static_cast<size_t>( rows ) * sizeof( uint64_t ), acct );
     nexist.alloc( static_cast<size_t>( rows ) * sizeof( uint64_t ), acct );
-    kFillStencilSlots<<<gridFor( nslices * kSlice ), kThreads, 0, st>>>( n, nslices, L, dent.as<int4>(), S,
-                                                                         ncol.as<int32_t>(), nexist.as<uint64_t>(),
-                                                                         nintact.as<uint64_t>() );
+    kFillStencilSlots<<<gridFor( nslices * kSlice ), kThreads, 0, st>>>( n, nslices, L, dent, S, ncol.as<int32_t>(),
+                                                                         nexist.as<uint64_t>(), nintact.as<uint64_t>() );
     PDK_HIP( hipGetLastError() );
     std::vector<int64_t> sp( nslices + 1 );
     for ( int64_t k = 0; k <= nslices; k++ )
         sp[k] = k * S;
     PDK_HIP( hipMemcpyAsync( slice_ptr_dev, sp.data(), ( nslices + 1 ) * sizeof( int64_t ), hipMemcpyHostToDevice, st ) );
-    std::vector<int32_t> soff( 256, 0 );
-    std::vector<float4> tab( 256, make_float4( 0.f, 0.f, 0.f, 1.f ) );
-    for ( int32_t e = 0; e < S; e++ )
-    {
-        soff[e] = ent[e].w;
-        const double gx = es[e].a * h[0], gy = es[e].b * h[1], gz = es[e].c * h[2];
-        tab[e] = make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
-                              static_cast<float>( std::sqrt( es[e].r2 ) ) );
-    }
     // Row windows: maximal runs of slots with one (dy, dz) and consecutive dx.
     std::vector<int4> rtab;
     for ( int32_t e = 0; e < S; e++ )
@@ -1030,6 +1022,15 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
         drstart.alloc( 9 * sizeof( uint32_t ), acct );
         PDK_HIP( hipMemcpyAsync( drstart.as<uint32_t>(), rs.data(), 9 * sizeof( uint32_t ), hipMemcpyHostToDevice, st ) );
     }
+    std::vector<int32_t> soff( 256, 0 );
+    std::vector<float4> tab( 256, make_float4( 0.f, 0.f, 0.f, 1.f ) );
+    for ( int32_t e = 0; e < S; e++ )
+    {
+        soff[e] = static_cast<int32_t>( es[e].off );
+        const double gx = es[e].a * h[0], gy = es[e].b * h[1], gz = es[e].c * h[2];
+        tab[e] = make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
+                              static_cast<float>( std::sqrt( es[e].r2 ) ) );
+    }
     dsoff.alloc( 256 * sizeof( int32_t ), acct );
     dtab.alloc( 256 * sizeof( float4 ), acct );
     PDK_HIP( hipMemcpyAsync( dsoff.as<int32_t>(), soff.data(), 256 * sizeof( int32_t ), hipMemcpyHostToDevice, st ) );
@@ -1046,6 +1047,115 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
     out.slots = S;
     total_rows = rows;
     col_capacity_rows = rows;
+}
+
+// Direct build check: every in-box stencil bond passes the exact FP64 cutoff
+// test (the cell search's, CabanaPD_Neighbor.hpp:46,56-59) and, for a caller
+// order other than the lattice order, lists its neighbors in ascending
+// caller id; also writes the bond count of each particle.
+__global__ __launch_bounds__( kThreads ) void kDirectSlotCheck( const double* __restrict__ x, int64_t n, LatticeBox L,
+                                                                const int4* __restrict__ ent, int32_t S, double rc2,
+                                                                const int64_t* __restrict__ perm, int32_t* __restrict__ cnt,
+                                                                int32_t* __restrict__ flag )
+{
+#pragma clang fp contract( off )
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( p >= n )
+        return;
+    int32_t ix, iy, iz;
+    latticeCoords( p, L, ix, iy, iz );
+    const double pi0 = x[p], pi1 = x[n + p], pi2 = x[2 * n + p];
+    int32_t c = 0;
+    int64_t last = -1;
+    bool bad = false;
+    for ( int32_t e = 0; e < S; e++ )
+    {
+        const int4 t = ent[e];
+        if ( !inBox( t, ix, iy, iz, L ) )
+            continue;
+        const int64_t j = p + t.w;
+        const double dx = x[j] - pi0;
+        const double dy = x[n + j] - pi1;
+        const double dz = x[2 * n + j] - pi2;
+        if ( !( dx * dx + dy * dy + dz * dz <= rc2 ) )
+            bad = true;
+        if ( perm )
+        {
+            const int64_t cid = perm[j];
+            if ( cid <= last )
+                bad = true;
+            last = cid;
+        }
+        c++;
+    }
+    cnt[p] = c;
+    if ( bad )
+        *flag = 1;
+}
+} // namespace
+
+bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3], double rc,
+                        const std::vector<int64_t>& perm_host, DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,
+                        int64_t& total_rows, int64_t& col_capacity_rows, SlotForm& out, BuildScratch& s,
+                        MemAccount* acct, hipStream_t st )
+{
+    const int64_t n = a.n;
+    std::vector<SlotEntry> es;
+    LatticeBox L{};
+    bool identity = false;
+    if ( !slotStencil( n, dims, h, rc, perm_host, 1e-6, es, L, identity ) )
+        return false;
+    const int32_t S = static_cast<int32_t>( es.size() );
+    const std::vector<int4> ent = slotEntries( es );
+    DevBuf dent;
+    dent.alloc( S * sizeof( int4 ), acct );
+    ensure<int32_t>( s.flag, 1, acct );
+    PDK_HIP( hipMemcpyAsync( dent.as<int4>(), ent.data(), S * sizeof( int4 ), hipMemcpyHostToDevice, st ) );
+    PDK_HIP( hipMemsetAsync( s.flag.as<int32_t>(), 0, sizeof( int32_t ), st ) );
+    kCheckStencilSlots<<<gridFor( n ), kThreads, 0, st>>>( a, L, dent.as<int4>(), S, s.flag.as<int32_t>() );
+    PDK_HIP( hipGetLastError() );
+    int32_t flag = 0;
+    PDK_HIP( hipMemcpyAsync( &flag, s.flag.as<int32_t>(), sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
+    PDK_HIP( hipStreamSynchronize( st ) );
+    if ( flag )
+        return false;
+    finishSlots( n, a.nslices, L, es, dent.as<int4>(), h, col, intact, slice_ptr_dev, total_rows, col_capacity_rows,
+                 out, acct, st );
+    return true;
+}
+
+bool buildStencilSlotsDirect( const double* x_dev, int64_t n, const int64_t dims[3], const double h[3], double rc,
+                              const std::vector<int64_t>& perm_host, const int64_t* perm_dev, int32_t* cnt_dev,
+                              DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev, int64_t& total_rows,
+                              int64_t& col_capacity_rows, SlotForm& out, BuildScratch& s, MemAccount* acct,
+                              hipStream_t st )
+{
+    // A wider band than the verified build's: positions deviate from the
+    // lattice by at most ~1e-6 h per axis, so no pair of particles whose
+    // lattice vector lies outside the band can fall on the other side of
+    // the cutoff; inside the stencil the exact test is applied anyway.
+    std::vector<SlotEntry> es;
+    LatticeBox L{};
+    bool identity = false;
+    if ( !slotStencil( n, dims, h, rc, perm_host, 2e-5, es, L, identity ) )
+        return false;
+    const int32_t S = static_cast<int32_t>( es.size() );
+    const std::vector<int4> ent = slotEntries( es );
+    DevBuf dent;
+    dent.alloc( S * sizeof( int4 ), acct );
+    ensure<int32_t>( s.flag, 1, acct );
+    PDK_HIP( hipMemcpyAsync( dent.as<int4>(), ent.data(), S * sizeof( int4 ), hipMemcpyHostToDevice, st ) );
+    PDK_HIP( hipMemsetAsync( s.flag.as<int32_t>(), 0, sizeof( int32_t ), st ) );
+    kDirectSlotCheck<<<gridFor( n ), kThreads, 0, st>>>( x_dev, n, L, dent.as<int4>(), S, rc * rc,
+                                                         identity ? nullptr : perm_dev, cnt_dev, s.flag.as<int32_t>() );
+    PDK_HIP( hipGetLastError() );
+    int32_t flag = 0;
+    PDK_HIP( hipMemcpyAsync( &flag, s.flag.as<int32_t>(), sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
+    PDK_HIP( hipStreamSynchronize( st ) );
+    if ( flag )
+        return false;
+    finishSlots( n, ( n + kSlice - 1 ) / kSlice, L, es, dent.as<int4>(), h, col, intact, slice_ptr_dev, total_rows,
+                 col_capacity_rows, out, acct, st );
     return true;
 }
 

@@ -162,6 +162,14 @@ bool buildStencilSlots( const PDArgs& a, const int64_t dims[3], const double h[3
                         const std::vector<int64_t>& perm_host, DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,
                         int64_t& total_rows, int64_t& col_capacity_rows, SlotForm& out, BuildScratch& s,
                         MemAccount* acct, hipStream_t st );
+// The same form built without the cell search: every in-box stencil bond is
+// checked with the exact cutoff test; fills cnt.  False (nothing written but
+// cnt) when the lattice does not qualify.
+bool buildStencilSlotsDirect( const double* x_dev, int64_t n, const int64_t dims[3], const double h[3], double rc,
+                              const std::vector<int64_t>& perm_host, const int64_t* perm_dev, int32_t* cnt_dev,
+                              DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev, int64_t& total_rows,
+                              int64_t& col_capacity_rows, SlotForm& out, BuildScratch& s, MemAccount* acct,
+                              hipStream_t st );
 void launchInvertPerm( const int64_t* perm_dev, int64_t n, int64_t* inv_dev, hipStream_t st );
 void launchGatherAos3( const double* src_aos, const int64_t* perm, double* dst_soa, int64_t n,
                        hipStream_t st );
