@@ -881,6 +881,9 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
     const uint64_t* __restrict__ bits = A.intact + base;
     const double* __restrict__ vol = A.vol;
     const int infl1 = A.influence;
+    // mixed form in the stencil-slot layout: bond vectors from the stencil
+    // table, as in the row-window state pass
+    const bool sten = ( PREC == kMixed ) && A.exist != nullptr;
 
     double m_i;
     if ( do_m )
@@ -904,9 +907,21 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
             }
             else
             {
-                const float gx = static_cast<float>( x[j] - xi0 );
-                const float gy = static_cast<float>( x[n + j] - xi1 );
-                const float gz = static_cast<float>( x[2 * n + j] - xi2 );
+                float gx, gy, gz;
+                if ( sten )
+                {
+                    // stencil-slot form: slot q is stencil entry q
+                    const float4 g = A.stencil[q];
+                    gx = g.x;
+                    gy = g.y;
+                    gz = g.z;
+                }
+                else
+                {
+                    gx = static_cast<float>( x[j] - xi0 );
+                    gy = static_cast<float>( x[n + j] - xi1 );
+                    gz = static_cast<float>( x[2 * n + j] - xi2 );
+                }
                 const float xi2f = gx * gx + gy * gy + gz * gz;
                 const float wv = infl1 ? ( xi2f * __builtin_amdgcn_rsqf( xi2f ) ) : xi2f;
                 m_i += static_cast<double>( wv * static_cast<float>( vol[j] ) );
@@ -942,8 +957,14 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
             else
             {
                 Bond32 b;
-                getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
-                               u[2 * n + j], b );
+                if ( sten )
+                {
+                    const float4 g = A.stencil[q];
+                    getDistance32s( g, __builtin_amdgcn_rcpf( g.w ), ui0, ui1, ui2, u[j], u[n + j], u[2 * n + j], b );
+                }
+                else
+                    getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
+                                   u[2 * n + j], b );
                 const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
                 th += static_cast<double>( infl * b.s * b.xi * b.xi * static_cast<float>( vol[j] ) );
             }
@@ -952,6 +973,231 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
             if ( mycnt > 0 )
                 th = 3.0 * th / m_i;
     }
+    A.theta[p] = th;
+    // ForceCoeffTag per-particle subexpressions (force_models/CabanaPD_LPS.hpp:131-132)
+    A.lpsA[p] = A.theta_coeff * th / m_i;
+    A.lpsB[p] = A.s_coeff / m_i;
+}
+
+// ---------------------------------------------------------------------------
+// The LPS state pass with neighbor data staged in LDS row windows (stencil-slot
+// form on lattice rows), as kForceRows does for the force pass.  Same bonds,
+// order and expressions as kLpsState, so the results are identical; the FP64
+// form walks the rows twice (m_i must be complete before each bond's share of
+// theta_i is divided by it), the mixed form once.
+// ---------------------------------------------------------------------------
+template <bool FRAC, int PREC, bool GEN>
+__global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restrict__ specs,
+                                                           const int32_t* __restrict__ tile_spec,
+                                                           const int64_t* __restrict__ tile_slice,
+                                                           int32_t parity, int64_t launch,
+                                                           int32_t do_m, int32_t do_theta )
+{
+#pragma clang fp contract( off )
+    constexpr int kW = kSlice + kRowMaxSlots - 1;
+    constexpr int NX = ( PREC == kF64 ) ? 3 : 0; // staged x components
+    constexpr int KV = 3 + NX;                   // staged vol (GEN)
+    constexpr int NC = KV + ( GEN ? 1 : 0 );
+    __shared__ double win[kSlicesPerBlock][2][NC][kW];
+    __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
+    __shared__ float sinv[( PREC == kMixed ) ? 256 : 1];
+    __shared__ int4 rtab[kMaxRows];
+    WaveCtx w;
+    const bool active = waveSetup( specs, tile_spec, tile_slice, 0, parity, launch, w );
+    if constexpr ( PREC == kMixed )
+    {
+        const float4 g = w.A->stencil[threadIdx.x];
+        stab[threadIdx.x] = g;
+        sinv[threadIdx.x] = __builtin_amdgcn_rcpf( g.w );
+    }
+    if ( static_cast<int>( threadIdx.x ) < w.A->nrows )
+        rtab[threadIdx.x] = w.A->rows[threadIdx.x];
+    __syncthreads();
+    if ( !active )
+        return;
+    const PDArgs& A = *w.A;
+    const int wv = threadIdx.x / kSlice;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t n = A.n;
+    const int64_t p0 = w.slice * kSlice;
+    const int64_t p = p0 + lane;
+    const bool valid = p < n;
+    const int64_t pc = valid ? p : p0;
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ u = A.U[w.cur];
+    const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
+    const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
+    const int32_t nx = A.lat_n[0], ny = A.lat_n[1], nz = A.lat_n[2];
+    const int64_t t1 = pc / nx;
+    const int32_t ix = static_cast<int32_t>( pc - t1 * nx );
+    const int32_t iy = static_cast<int32_t>( t1 % ny );
+    const int32_t iz = static_cast<int32_t>( t1 / ny );
+    const uint64_t* __restrict__ bits = A.intact + A.slice_ptr[w.slice];
+    const int infl1 = A.influence;
+    const double volu = A.vol_value;
+    const float vol32 = static_cast<float>( A.vol_value );
+
+    const double* src[KV];
+    src[0] = u;
+    src[1] = u + n;
+    src[2] = u + 2 * n;
+    if constexpr ( NX )
+    {
+        src[3] = x;
+        src[4] = x + n;
+        src[5] = x + 2 * n;
+    }
+    const double* __restrict__ volp = A.vol;
+    const int32_t n32 = static_cast<int32_t>( n );
+    const int32_t p032 = static_cast<int32_t>( p0 );
+    auto ld = []( const double* base, uint32_t byte_off ) {
+        return *reinterpret_cast<const double*>( reinterpret_cast<const char*>( base ) + byte_off );
+    };
+    double v0[NC], v1[NC];
+    uint32_t blo = 0, bhi = 0;
+    auto loadRow = [&]( int r ) {
+        const int4 R = rtab[r];
+        const int32_t g = p032 + R.z;
+        const int32_t g0 = min( max( g + lane, 0 ), n32 - 1 );
+        const int32_t g1 = min( max( g + kSlice + lane, 0 ), n32 - 1 );
+        const uint32_t o0 = static_cast<uint32_t>( g0 ) * 8u, o1 = static_cast<uint32_t>( g1 ) * 8u;
+        const bool second = lane < R.y - 1;
+#pragma unroll
+        for ( int k = 0; k < KV; k++ )
+        {
+            v0[k] = ld( src[k], o0 );
+            v1[k] = second ? ld( src[k], o1 ) : 0.0;
+        }
+        if constexpr ( GEN )
+        {
+            v0[KV] = ld( volp, o0 );
+            v1[KV] = second ? ld( volp, o1 ) : 0.0;
+        }
+        if constexpr ( FRAC )
+        {
+            const uint64_t wd = ( lane < R.y ) ? bits[R.x + lane] : 0ull;
+            blo = static_cast<uint32_t>( wd );
+            bhi = static_cast<uint32_t>( wd >> 32 );
+        }
+    };
+    // Walk every bond slot in slot order; bond(live, wr, q) gets the window
+    // position of the neighbor's staged values.
+    auto walk = [&]( auto&& bond ) {
+        const int nrows = A.nrows;
+        loadRow( 0 );
+        for ( int r = 0; r < nrows; r++ )
+        {
+            const int4 R = rtab[r];
+            double* __restrict__ wb = &win[wv][r & 1][0][0];
+#pragma unroll
+            for ( int k = 0; k < NC; k++ )
+            {
+                wb[k * kW + lane] = v0[k];
+                if ( lane < R.y - 1 )
+                    wb[k * kW + kSlice + lane] = v1[k];
+            }
+            const uint32_t rlo = blo, rhi = bhi;
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+            if ( r + 1 < nrows )
+                loadRow( r + 1 );
+            const int32_t dx0 = ( R.w & 0xff ) - 128;
+            const int32_t dy = ( ( R.w >> 8 ) & 0xff ) - 128;
+            const int32_t dz = ( ( R.w >> 16 ) & 0xff ) - 128;
+            const bool rowok = valid && static_cast<uint32_t>( iy + dy ) < static_cast<uint32_t>( ny ) &&
+                               static_cast<uint32_t>( iz + dz ) < static_cast<uint32_t>( nz );
+            for ( int t = 0; t < R.y; t++ )
+            {
+                bool live = rowok && static_cast<uint32_t>( ix + dx0 + t ) < static_cast<uint32_t>( nx );
+                if constexpr ( FRAC )
+                {
+                    const uint32_t l32 = __builtin_amdgcn_readlane( rlo, t );
+                    const uint32_t h32 = __builtin_amdgcn_readlane( rhi, t );
+                    live = live && __builtin_amdgcn_inverse_ballot_w64( ( static_cast<uint64_t>( h32 ) << 32 ) | l32 );
+                }
+                bond( live, wb + lane + t, R.x + t );
+            }
+        }
+    };
+
+    double m_i = 0.0;
+    double th = 0.0;
+    if constexpr ( PREC == kF64 )
+    {
+        if ( do_m )
+        {
+            walk( [&]( bool live, const double* wr, int32_t ) {
+                if ( live )
+                {
+                    Bond64 b;
+                    getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], wr[0], wr[kW],
+                                   wr[2 * kW], b );
+                    const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                    m_i += infl * b.xi * b.xi * ( GEN ? wr[KV * kW] : volu );
+                }
+            } );
+        }
+        else if ( valid )
+            m_i = A.m[p];
+        const bool mpos = FRAC ? ( m_i > 0 ) : true;
+        if ( do_theta )
+        {
+            walk( [&]( bool live, const double* wr, int32_t ) {
+                if ( live && mpos )
+                {
+                    Bond64 b;
+                    getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], wr[0], wr[kW],
+                                   wr[2 * kW], b );
+                    const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                    const double theta_b = infl * b.s * b.xi * b.xi * ( GEN ? wr[KV * kW] : volu );
+                    th += 3.0 * theta_b / m_i;
+                }
+            } );
+        }
+    }
+    else
+    {
+        if ( !do_m && valid )
+            m_i = A.m[p];
+        // m_i for the theta condition is only known after the walk: sum both,
+        // apply the condition at the end (the sums are the same per-bond terms)
+        walk( [&]( bool live, const double* wr, int32_t q ) {
+            if ( live )
+            {
+                const float4 g = stab[q];
+                const float xinv = sinv[q];
+                const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
+                if ( do_m )
+                {
+                    const float xi2f = g.x * g.x + g.y * g.y + g.z * g.z;
+                    const float wvb = infl1 ? ( xi2f * __builtin_amdgcn_rsqf( xi2f ) ) : xi2f;
+                    m_i += static_cast<double>( wvb * vj32 );
+                }
+                if ( do_theta )
+                {
+                    Bond32 b;
+                    getDistance32s( g, xinv, ui0, ui1, ui2, wr[0], wr[kW], wr[2 * kW], b );
+                    const float infl = infl1 ? xinv : 1.0f;
+                    th += static_cast<double>( infl * b.s * b.xi * b.xi * vj32 );
+                }
+            }
+        } );
+        if ( do_theta )
+        {
+            const bool mpos = FRAC ? ( m_i > 0 ) : true;
+            if ( !mpos )
+                th = 0.0;
+            else if ( valid && A.cnt[p] > 0 )
+                th = 3.0 * th / m_i;
+        }
+    }
+    if ( !valid )
+        return;
+    if ( do_m )
+        A.m[p] = m_i;
+    if ( !do_theta )
+        return;
     A.theta[p] = th;
     // ForceCoeffTag per-particle subexpressions (force_models/CabanaPD_LPS.hpp:131-132)
     A.lpsA[p] = A.theta_coeff * th / m_i;
@@ -1184,6 +1430,7 @@ ForceFn pickForce( const PDArgs& a )
 }
 
 thread_local char g_force_name[64];
+thread_local char g_lps_name[64];
 
 } // namespace
 
@@ -1208,6 +1455,13 @@ const char* forceKernelName( const PDArgs& a )
 }
 const char* lpsKernelName( const PDArgs& a )
 {
+    if ( a.rows != nullptr && a.nrows > 0 )
+    {
+        std::snprintf( g_lps_name, sizeof( g_lps_name ), "kLpsStateRows<%s, %d, %s>",
+                       a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0,
+                       a.vol_uniform ? "false" : "true" );
+        return g_lps_name;
+    }
     return a.fracture ? ( a.precision == PDK_MIXED ? "kLpsState<true, 1>" : "kLpsState<true, 0>" )
                       : ( a.precision == PDK_MIXED ? "kLpsState<false, 1>" : "kLpsState<false, 0>" );
 }
@@ -1235,13 +1489,32 @@ void launchForce( const PDArgs* args_dev, const PDArgs& a, int32_t epilogue_mode
     PDK_HIP( hipGetLastError() );
 }
 
+using LpsFn = void ( * )( const PDArgs*, const int32_t*, const int64_t*, int32_t, int64_t, int32_t, int32_t );
+
+template <bool GEN>
+LpsFn pickLpsRows( const PDArgs& a )
+{
+    return a.fracture ? ( a.precision == PDK_MIXED ? kLpsStateRows<true, kMixed, GEN> : kLpsStateRows<true, kF64, GEN> )
+                      : ( a.precision == PDK_MIXED ? kLpsStateRows<false, kMixed, GEN>
+                                                   : kLpsStateRows<false, kF64, GEN> );
+}
+
+// The LPS state pass follows the layout (row windows wherever a row table
+// exists), not the force-pass tuning knob; both kernels give the same results.
+LpsFn pickLps( const PDArgs& a )
+{
+    if ( a.rows != nullptr && a.nrows > 0 )
+        return a.vol_uniform ? pickLpsRows<false>( a ) : pickLpsRows<true>( a );
+    return a.fracture ? ( a.precision == PDK_MIXED ? kLpsState<true, kMixed> : kLpsState<true, kF64> )
+                      : ( a.precision == PDK_MIXED ? kLpsState<false, kMixed> : kLpsState<false, kF64> );
+}
+
 void launchLpsState( const PDArgs* args_dev, const PDArgs& a, bool weighted_volume, bool dilatation,
                      int32_t parity, hipStream_t st )
 {
     if ( a.nslices == 0 )
         return;
-    auto fn = a.fracture ? ( a.precision == PDK_MIXED ? kLpsState<true, kMixed> : kLpsState<true, kF64> )
-                         : ( a.precision == PDK_MIXED ? kLpsState<false, kMixed> : kLpsState<false, kF64> );
+    LpsFn fn = pickLps( a );
     hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, nullptr, nullptr, parity,
                         int64_t( 0 ), int32_t( weighted_volume ), int32_t( dilatation ) );
     PDK_HIP( hipGetLastError() );
@@ -1264,9 +1537,7 @@ void launchLpsStateBatched( const PDArgs* specs_dev, const PDArgs& variant, cons
 {
     if ( ntiles == 0 )
         return;
-    auto fn = variant.fracture
-                  ? ( variant.precision == PDK_MIXED ? kLpsState<true, kMixed> : kLpsState<true, kF64> )
-                  : ( variant.precision == PDK_MIXED ? kLpsState<false, kMixed> : kLpsState<false, kF64> );
+    LpsFn fn = pickLps( variant );
     hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( ntiles ) ), dim3( kBlock ), 0, st, specs_dev, tile_spec,
                         tile_slice, static_cast<int32_t>( launch_index & 1 ), launch_index,
                         int32_t( variant.fracture ? 1 : 0 ), int32_t( 1 ) );

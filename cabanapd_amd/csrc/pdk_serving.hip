@@ -121,7 +121,9 @@ struct BatchEngine::Impl
         {
             SpecView v = view( r.s );
             // one kernel variant per group: model, fracture, precision, bond layout
-            const int form = useRowWindows( *v.h ) ? 2 : ( v.h->exist ? 1 : 0 );
+            const int form = ( useRowWindows( *v.h ) ? 2 : ( v.h->exist ? 1 : 0 ) ) +
+                             ( v.h->rows != nullptr ? 4 : 0 ) + ( v.h->vol_uniform ? 0 : 8 ) +
+                             ( v.h->any_nofail ? 16 : 0 );
             auto k = std::make_tuple( v.model, v.fracture, v.precision, form );
             auto it = key.find( k );
             size_t gi;
