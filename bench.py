@@ -242,9 +242,10 @@ def lps_leg(args, n_side=128):
 
 def dem_leg(args, n=1000000):
     """Hertz-JKR contact box: 1e6 polydisperse spheres (radii log-normal,
-    sigma 0.1, clipped to +-30 %) on a jittered lattice, gravity, restitution
-    0.5, surface energy 0.05 J/m^2, contact list rebuilt every 20 steps."""
-    w = configs.dem_packing(n=n, model="HERTZ_JKR", polydisperse=True, steps=args.steps)
+    sigma 0.1) on a face-centred cubic packing of solid fraction ~0.50,
+    random velocities ~1 m/s, gravity, restitution 0.5, surface energy
+    0.05 J/m^2, contact list rebuilt every 20 steps."""
+    w = configs.dem_fcc(n=n, model="HERTZ_JKR", steps=args.steps)
     ctx = engine.Context(0)
     s = engine.Spec(ctx, w["desc"], w["x"].shape[0])
     configs.upload(s, w)
@@ -261,8 +262,7 @@ def dem_leg(args, n=1000000):
     oms, ol = s.kernel_time(4)
     s.timing(False)
     pairs, maxn = s.bond_stats()
-    r = w["radius"]
-    frac = float(np.sum(4.0 / 3.0 * np.pi * r ** 3) / np.prod(w["x"].max(0) - w["x"].min(0) + 2 * r.max()))
+    frac = w["solid_fraction"]
     s.close()
     ctx.close()
     return {"workload": "Hertz-JKR contact box, %d polydisperse spheres (solid fraction %.3f), gravity, "

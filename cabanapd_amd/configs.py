@@ -234,6 +234,40 @@ def dem_packing(n=1000, seed=7, model="HERTZ_JKR", polydisperse=True, dt=1.5e-7,
     return w
 
 
+def dem_fcc(n=1000000, seed=7, model="HERTZ_JKR", dt=1.5e-7, steps=2000, rebuild_every=20, speed=1.0,
+            r_mean=1e-3, sigma=0.1, spacing=2.3, E=70e9, nu=0.25, rho=2600.0, restitution=0.5, gamma=0.05,
+            extend_frac=0.1, gravity=True):
+    """Dense granular box: n spheres on a face-centred cubic lattice with
+    nearest-neighbour distance spacing * r_mean, radii log-normal (sigma) about
+    r_mean clipped to [0.7, spacing / 2 * 0.999] r_mean so that no two spheres
+    overlap (solid fraction about 0.50 for spacing 2.3), seeded random
+    velocities of magnitude ~speed, gravity f_z -= 9.8 rho."""
+    rng = np.random.default_rng(seed)
+    r = np.exp(rng.normal(math.log(r_mean), sigma, size=n))
+    r = np.clip(r, 0.7 * r_mean, 0.5 * spacing * 0.999 * r_mean)
+    d = spacing * r_mean
+    a = d * math.sqrt(2.0)  # cubic cell of the FCC lattice
+    m = int(math.ceil((n / 4.0) ** (1.0 / 3.0)))
+    basis = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    cz, cy, cx = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    corners = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1).astype(np.float64)
+    x = ((corners[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a)[:n]
+    vol = 4.0 / 3.0 * math.pi * r ** 3
+    desc = make_desc(model, radius=r_mean, radius_extend=extend_frac * r_mean, nu=nu, E=E,
+                     restitution=restitution, gamma=gamma, mu_friction=0.0, dt=dt, rebuild_every=rebuild_every,
+                     polydisperse=True)
+    w = _base(desc, x, vol, rho, "dem_fcc_%s_%d" % (model.lower(), n), d, steps)
+    w["v"] = rng.normal(0.0, speed / math.sqrt(3.0), size=(n, 3))
+    w["radius"] = r
+    if gravity:
+        b = np.zeros((n, 3))
+        b[:, 2] = -9.8 * rho
+        w["body"] = b
+    box = np.prod(x.max(0) - x.min(0) + d)
+    w["solid_fraction"] = float(vol.sum() / box)
+    return w
+
+
 def lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.1, K=1.0, G=0.5, influence=None,
                      fracture=False, G0=1000.0):
     """Known-answer lattice of unit_test/tstForce.hpp:593-661: cell-centred

@@ -160,6 +160,7 @@ struct pdk_spec_s
     // contact bookkeeping
     int64_t steps_since_build = 0;
     double max_disp = 0.0;
+    double radius_max = 0.0; // largest per-particle radius (polydisperse contact)
     // timing
     bool timing = false;
     struct Ev
@@ -612,6 +613,9 @@ int pdk_spec_upload( pdk_spec s, const double* x, const double* u, const double*
         {
             require( radius != nullptr, "polydisperse contact needs per-particle radii" );
             s->uploadScalar( radius, s->radius.as<double>() );
+            s->radius_max = 0.0;
+            for ( int64_t i = 0; i < n; i++ )
+                s->radius_max = std::max( s->radius_max, radius[i] );
         }
         PDK_HIP( hipMemsetAsync( s->W.as<double>(), 0, n * sizeof( double ), s->st ) );
         PDK_HIP( hipMemsetAsync( s->phi.as<double>(), 0, n * sizeof( double ), s->st ) );
@@ -1216,6 +1220,7 @@ void contactBuildImpl( pdk_spec s, bool initial, double* max_disp, int64_t* sinc
     q.rc = 2.0 * s->d.radius + s->d.radius_extend + tol;
     q.extend = s->d.radius_extend + tol;
     q.n = s->n;
+    q.rmax = s->radius_max;
     s->total_rows = buildSlicedCsr( q, s->perm.as<int64_t>(), s->inv.as<int64_t>(), s->slice_ptr.as<int64_t>(),
                                     s->cnt.as<int32_t>(), s->col, s->intact, s->col_rows_cap, s->scratch, s->acct(),
                                     s->st, &s->max_cnt );

@@ -751,12 +751,15 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
     double cell = q.rc;
     if ( q.radius )
     {
-        std::vector<double> rh( n );
-        PDK_HIP( hipMemcpyAsync( rh.data(), q.radius, n * sizeof( double ), hipMemcpyDeviceToHost, st ) );
-        PDK_HIP( hipStreamSynchronize( st ) );
-        double rmax = 0;
-        for ( double r : rh )
-            rmax = std::max( rmax, r );
+        double rmax = q.rmax;
+        if ( !( rmax > 0 ) )
+        {
+            std::vector<double> rh( n );
+            PDK_HIP( hipMemcpyAsync( rh.data(), q.radius, n * sizeof( double ), hipMemcpyDeviceToHost, st ) );
+            PDK_HIP( hipStreamSynchronize( st ) );
+            for ( double r : rh )
+                rmax = std::max( rmax, r );
+        }
         cell = 2.0 * rmax + q.extend;
     }
     CellGrid g;

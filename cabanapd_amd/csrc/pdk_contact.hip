@@ -237,6 +237,26 @@ __global__ __launch_bounds__( 256 ) void kContactInitial( const PDArgs* __restri
         partial[blockIdx.x] = red[0];
 }
 
+// Maximum of the per-block maxima, written to out (one block).
+__global__ __launch_bounds__( 256 ) void kMaxReduce( const double* __restrict__ partial, int64_t nb,
+                                                     double* __restrict__ out )
+{
+    __shared__ double red[256];
+    double mx = 0.0;
+    for ( int64_t b = threadIdx.x; b < nb; b += 256 )
+        mx = fmax( mx, partial[b] );
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for ( int w = 128; w > 0; w >>= 1 )
+    {
+        if ( (int)threadIdx.x < w )
+            red[threadIdx.x] = fmax( red[threadIdx.x], red[threadIdx.x + w] );
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        *out = red[0];
+}
+
 using ContactFn = void ( * )( const PDArgs*, int32_t, int32_t );
 ContactFn pickContact( const PDArgs& a )
 {
@@ -298,17 +318,19 @@ void contactStep( pdk_spec_s* s, int32_t nsteps )
     {
         SpecView v = view( s );
         const int64_t nb = ( v.n + 255 ) / 256;
-        if ( cs.partial->bytes() < static_cast<size_t>( nb ) * sizeof( double ) )
-            cs.partial->alloc( nb * sizeof( double ), cs.acct );
-        specTimed( s, 4, [&] { launchContactInitialHalfStep( v.dargs, *v.h, *v.cur, cs.partial->as<double>(), nb, v.st ); } );
+        if ( cs.partial->bytes() < static_cast<size_t>( nb + 1 ) * sizeof( double ) )
+            cs.partial->alloc( ( nb + 1 ) * sizeof( double ), cs.acct );
+        double* part = cs.partial->as<double>();
+        specTimed( s, 4, [&] {
+            launchContactInitialHalfStep( v.dargs, *v.h, *v.cur, part, nb, v.st );
+            hipLaunchKernelGGL( kMaxReduce, dim3( 1 ), dim3( 256 ), 0, v.st, part, nb, part + nb );
+            PDK_HIP( hipGetLastError() );
+        } );
         *v.cur ^= 1;
-        std::vector<double> part( nb );
-        PDK_HIP( hipMemcpyAsync( part.data(), cs.partial->as<double>(), nb * sizeof( double ), hipMemcpyDeviceToHost,
-                                 v.st ) );
-        PDK_HIP( hipStreamSynchronize( v.st ) );
+        // the host decides on a rebuild (CabanaPD_Neighbor.hpp:116-119)
         double mx = 0.0;
-        for ( double m : part )
-            mx = std::max( mx, m );
+        PDK_HIP( hipMemcpyAsync( &mx, part + nb, sizeof( double ), hipMemcpyDeviceToHost, v.st ) );
+        PDK_HIP( hipStreamSynchronize( v.st ) );
         *cs.max_disp = mx;
         *cs.since_build += 1;
         const bool periodic = cs.rebuild_every > 0 && *cs.since_build >= cs.rebuild_every;

@@ -190,6 +190,7 @@ struct NeighborQuery
     double rc;            // cutoff (uniform)
     double extend;        // polydisperse: added to R_i + R_j
     int64_t n;
+    double rmax = 0.0;    // polydisperse: largest radius if known (else read back from the device)
 };
 
 // Builds the sliced CSR; (re)allocates col/intact when capacity is short.
