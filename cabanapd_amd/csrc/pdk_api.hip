@@ -177,8 +177,48 @@ struct pdk_spec_s
 
     MemAccount* acct() { return &ctx->acct; }
 
+    // Two consecutive middle steps of pdk_step captured as one HIP graph per
+    // starting displacement buffer (after two steps the buffer index is back
+    // where it started).  Dropped whenever the launch configuration may have
+    // changed.
+    hipGraphExec_t step_graph[2] = { nullptr, nullptr };
+    bool use_graph = true;
+    void dropGraphs()
+    {
+        for ( auto& g : step_graph )
+            if ( g )
+            {
+                (void)hipGraphExecDestroy( g );
+                g = nullptr;
+            }
+    }
+    // The launches of one middle step (initial half step of the next step
+    // fused into the force epilogue), displacement buffer c.
+    void middleStep( int32_t c )
+    {
+        if ( lps() )
+            launchLpsState( dargs(), h, d.fracture != 0, true, c, st );
+        launchForce( dargs(), h, kEpiFinalInitial, c, st );
+    }
+    hipGraphExec_t stepPairGraph( int32_t c )
+    {
+        if ( !step_graph[c] )
+        {
+            hipGraph_t g = nullptr;
+            PDK_HIP( hipStreamBeginCapture( st, hipStreamCaptureModeThreadLocal ) );
+            middleStep( c );
+            middleStep( c ^ 1 );
+            PDK_HIP( hipStreamEndCapture( st, &g ) );
+            const hipError_t e = hipGraphInstantiate( &step_graph[c], g, nullptr, nullptr, 0 );
+            (void)hipGraphDestroy( g );
+            PDK_HIP( e );
+        }
+        return step_graph[c];
+    }
+
     void pushArgs()
     {
+        dropGraphs();
         h.n = n;
         h.nslices = nslices;
         h.x = x.as<double>();
@@ -521,10 +561,10 @@ int pdk_spec_destroy( pdk_spec s )
             s->ctx->batch->forget( s );
         s->resolveEvents();
         if ( s->st )
-        {
             (void)hipStreamSynchronize( s->st );
+        s->dropGraphs();
+        if ( s->st )
             (void)hipStreamDestroy( s->st );
-        }
         if ( s->ctx )
             s->ctx->live_specs--;
         delete s;
@@ -983,7 +1023,13 @@ int pdk_step( pdk_spec s, int32_t nsteps )
             // the previous step's force epilogue.
             s->timed( kClsOther, [&] { launchInitialHalfStep( s->dargs(), s->h, s->cur, s->st ); } );
             s->cur ^= 1;
-            for ( int32_t k = 1; k <= nsteps; k++ )
+            int32_t k = 1;
+            // middle steps in pairs from a captured graph (not while kernels
+            // are being timed: events are recorded around each launch)
+            if ( s->use_graph && !s->timing )
+                for ( ; k + 1 < nsteps; k += 2 )
+                    PDK_HIP( hipGraphLaunch( s->stepPairGraph( s->cur ), s->st ) );
+            for ( ; k <= nsteps; k++ )
             {
                 if ( s->lps() )
                     s->lpsState( s->d.fracture != 0, true );
@@ -1062,6 +1108,15 @@ int pdk_spec_layout( pdk_spec s, int32_t* lattice_rows, int32_t* stencil_entries
             *stencil_entries = s->nsten;
         if ( stencil_slots )
             *stencil_slots = s->slot.slots > 0 ? 1 : 0;
+    } );
+}
+
+int pdk_set_graph( pdk_spec s, int32_t enable )
+{
+    return guarded( [&] {
+        require( s != nullptr, "null spec" );
+        s->use_graph = enable != 0;
+        s->dropGraphs();
     } );
 }
 

@@ -53,6 +53,7 @@ _SIGS = {
     "pdk_set_tuning": (C.c_int, [_VP, C.c_int32]),
     "pdk_spec_layout": (C.c_int, [_VP, _I32, _I32, _I32]),
     "pdk_autotune": (C.c_int, [_VP, C.c_int32, _I32]),
+    "pdk_set_graph": (C.c_int, [_VP, C.c_int32]),
     "pdk_kernel_timing": (C.c_int, [_VP, C.c_int32]),
     "pdk_kernel_time": (C.c_int, [_VP, C.c_int32, _D, _I64]),
     "pdk_kernel_name": (C.c_int, [_VP, C.c_int32, C.c_char_p, C.c_int32]),
@@ -265,6 +266,9 @@ class Spec:
         sl = C.c_int32()
         _check(lib().pdk_spec_layout(self.h, C.byref(lr), C.byref(ns), C.byref(sl)))
         return {"lattice_rows": bool(lr.value), "stencil_entries": ns.value, "stencil_slots": bool(sl.value)}
+
+    def set_graph(self, enable):
+        _check(lib().pdk_set_graph(self.h, 1 if enable else 0))
 
     def set_tuning(self, unroll):
         _check(lib().pdk_set_tuning(self.h, unroll))

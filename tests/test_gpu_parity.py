@@ -200,3 +200,32 @@ def test_gpu_unroll_is_transparent(unroll):
     o.step(40)
     assert_same(g.get("u"), o.get("u"), "u")
     assert broken_set(g) == broken_set(o)
+
+
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_gpu_step_graph_matches_eager(model):
+    """pdk_step replays captured two-step graphs; between replays the host
+    downloads fields (eager kernels and copies on the same stream).  Every
+    chunk must equal the same chunk run with every kernel launched eagerly."""
+    from cabanapd_amd import engine
+    from tests.parity import ctx
+    say(f"case: {model} graph-captured steps vs eager launches, 4 chunks of 7 steps")
+    if model == "PMB":
+        w = configs.kw_plate(num_cells=(24, 48, 8), size=(0.06, 0.12, 0.02), steps=28)
+    else:
+        w = configs.lps_cube(n_side=16, steps=28, fracture=True)
+    specs = []
+    for graph in (True, False):
+        s = engine.Spec(ctx(), w["desc"], w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        s.set_graph(graph)
+        specs.append(s)
+    for chunk in range(4):
+        for s in specs:
+            s.step(7)
+        for fld in ("u", "v", "f"):
+            assert_same(specs[0].get(fld), specs[1].get(fld), f"{fld} after chunk {chunk}")
+        assert np.array_equal(specs[0].bonds()[2], specs[1].bonds()[2])
+    for s in specs:
+        s.close()
