@@ -240,6 +240,38 @@ def lps_leg(args, n_side=128):
             "achieved_GBs_mixed": b_alg / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9}
 
 
+def cube_leg(args):
+    """PMB elastic cube 41^3 (68,921 particles, 7,720,394 bonds), Gaussian
+    radial displacement, dt 1e-7: both precision forms, with the captured
+    step graphs and with every kernel launched on its own."""
+    w = configs.elastic_cube(n_side=41, steps=args.steps)
+    ctx = engine.Context(0)
+    out = {}
+    u = {}
+    for prec in ("fp64", "mixed"):
+        for graph in (True, False):
+            s = make_spec(ctx, w, prec)
+            bonds, _ = s.bond_stats()
+            s.set_graph(graph)
+            s.step(args.warmup) if args.warmup else None
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(args.steps)
+            s.sync()
+            wall = time.perf_counter() - t0
+            out["%s_%s" % (prec, "graph" if graph else "eager")] = {
+                "value": bonds * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+                "kernel": s.kernel_name(KCLS_FORCE)}
+            if graph:
+                u[prec] = s.get("u")
+            s.close()
+    ctx.close()
+    den = float(np.max(np.abs(u["fp64"])))
+    return {"workload": "PMB elastic cube 41^3 (68921 particles, %d bonds), %d steps" % (bonds, args.steps),
+            "metric": "bond-evals/s", "runs": out,
+            "u_rel_err": float(np.max(np.abs(u["mixed"] - u["fp64"])) / den)}
+
+
 def dem_leg(args, n=1000000):
     """Hertz-JKR contact box: 1e6 polydisperse spheres (radii log-normal,
     sigma 0.1) on a face-centred cubic packing of solid fraction ~0.50,
@@ -286,9 +318,9 @@ def main():
     ap.add_argument("--max-particles", type=float, default=2e6)
     ap.add_argument("--bond-layout", choices=("auto", "compact"), default="auto",
                     help="auto: stencil slots on full lattices; compact: stored neighbor ids")
-    ap.add_argument("--workload", choices=("plate", "lps", "dem"), default="plate",
-                    help="plate: the PMB fracture plate (the benchmark line); lps: LPS fracture cube "
-                         "128^3; dem: Hertz-JKR contact box of 1e6 spheres")
+    ap.add_argument("--workload", choices=("plate", "cube", "lps", "dem"), default="plate",
+                    help="plate: the PMB fracture plate (the benchmark line); cube: PMB elastic cube 41^3; "
+                         "lps: LPS fracture cube 128^3; dem: Hertz-JKR contact box of 1e6 spheres")
     ap.add_argument("--copy-kernel", type=float, default=0.0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -301,6 +333,9 @@ def main():
         sys.exit(2)
     if args.workload == "lps":
         print(json.dumps(lps_leg(args)))
+        return
+    if args.workload == "cube":
+        print(json.dumps(cube_leg(args)))
         return
     if args.workload == "dem":
         print(json.dumps(dem_leg(args)))
