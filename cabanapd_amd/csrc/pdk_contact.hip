@@ -152,13 +152,18 @@ __global__ __launch_bounds__( kBlock ) void kContact( const PDArgs* __restrict__
         if constexpr ( JKR )
         {
             // force_models/CabanaPD_HertzianJKR.hpp:114-129
-            const double vtx = vx - vn * nx, vty = vy - vn * ny, vtz = vz - vn * nz;
-            const double vt = sqrt( vtx * vtx + vty * vty + vtz * vtz );
+            // Without friction or normal force every term is a signed zero,
+            // which leaves the sum unchanged: skip the three tanh.
             const double fna = fabs( fn );
             const double mu = A.mu_f;
-            ftx = ( vt > 0 ) ? fabs( mu * tanh( 10 * vtx ) ) * fna * vtx / vt : 0.0;
-            fty = ( vt > 0 ) ? fabs( mu * tanh( 10 * vty ) ) * fna * vty / vt : 0.0;
-            ftz = ( vt > 0 ) ? fabs( mu * tanh( 10 * vtz ) ) * fna * vtz / vt : 0.0;
+            if ( mu != 0.0 && fna != 0.0 )
+            {
+                const double vtx = vx - vn * nx, vty = vy - vn * ny, vtz = vz - vn * nz;
+                const double vt = sqrt( vtx * vtx + vty * vty + vtz * vtz );
+                ftx = ( vt > 0 ) ? fabs( mu * tanh( 10 * vtx ) ) * fna * vtx / vt : 0.0;
+                fty = ( vt > 0 ) ? fabs( mu * tanh( 10 * vty ) ) * fna * vty / vt : 0.0;
+                ftz = ( vt > 0 ) ? fabs( mu * tanh( 10 * vtz ) ) * fna * vtz / vt : 0.0;
+            }
         }
         fx += ftx;
         fy += fty;

@@ -66,12 +66,14 @@ def compressed_packing(model, poly, speed):
     return w
 
 
-@pytest.mark.parametrize("model,poly", [("HERTZ", False), ("HERTZ_JKR", False), ("HERTZ_JKR", True)])
-def test_gpu_contact_force_parity(model, poly):
+@pytest.mark.parametrize("model,poly,mu", [("HERTZ", False, 0.0), ("HERTZ_JKR", False, 0.0),
+                                           ("HERTZ_JKR", True, 0.0), ("HERTZ_JKR", True, 0.3)])
+def test_gpu_contact_force_parity(model, poly, mu):
     """One contact pass on overlapping spheres: forces within 1e-12 of the
     oracle (the only differences are libm vs device pow/log/tanh ulps)."""
-    say(f"case: {model} force poly={poly} vs oracle")
+    say(f"case: {model} force poly={poly} friction={mu} vs oracle")
     w = compressed_packing(model, poly, speed=1.0)
+    w["desc"].mu_friction = mu
     g = engine.Spec(ctx(), w["desc"], w["x"].shape[0])
     configs.upload(g, w)
     configs.setup(g, w)
