@@ -5,6 +5,8 @@ PDK_MODEL_PMB = 0
 PDK_MODEL_LPS = 1
 PDK_MODEL_HERTZ = 2
 PDK_MODEL_HERTZ_JKR = 3
+PDK_MODEL_LINEAR_PMB = 4
+PDK_MODEL_LINEAR_LPS = 5
 
 PDK_FP64 = 0
 PDK_MIXED = 1
@@ -71,11 +73,12 @@ def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
     influence defaults: PMB fracture s0 uses influence_type 1
     (CabanaPD_ForceModels.hpp:93-101), LPS uses 0
     (force_models/CabanaPD_LPS.hpp:46-47)."""
-    kinds = {"PMB": PDK_MODEL_PMB, "LPS": PDK_MODEL_LPS,
+    kinds = {"PMB": PDK_MODEL_PMB, "LPS": PDK_MODEL_LPS, "LINEAR_PMB": PDK_MODEL_LINEAR_PMB,
+             "LINEAR_LPS": PDK_MODEL_LINEAR_LPS,
              "HERTZ": PDK_MODEL_HERTZ, "HERTZ_JKR": PDK_MODEL_HERTZ_JKR}
     m = kinds[model.upper()]
     if influence is None:
-        influence = 0 if m == PDK_MODEL_LPS else 1
+        influence = 0 if m in (PDK_MODEL_LPS, PDK_MODEL_LINEAR_LPS) else 1
     d = ModelDesc()
     d.model = m
     d.fracture = 1 if fracture else 0

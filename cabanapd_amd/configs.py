@@ -279,7 +279,7 @@ def lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.1, K=1.0, 
     kw = dict(horizon=horizon, K=K, dt=1e-7, fracture=fracture)
     if fracture:
         kw["G0"] = G0
-    if model == "LPS":
+    if model in ("LPS", "LINEAR_LPS"):
         kw["G"] = G
     desc = make_desc(model, influence=influence, **kw)
     w = _base(desc, x, vol, 1.0, "lattice_%s_%s" % (model, field), dx, 0)

@@ -53,6 +53,16 @@ bool isContact( const pdk_model_desc& d )
 {
     return d.model == PDK_MODEL_HERTZ || d.model == PDK_MODEL_HERTZ_JKR;
 }
+// LinearLPS / LinearPMB derive from the LPS / PMB force models
+// (force_models/CabanaPD_LPS.hpp:255-273, force_models/CabanaPD_PMB.hpp:575-597).
+bool isLpsModel( int32_t m )
+{
+    return m == PDK_MODEL_LPS || m == PDK_MODEL_LINEAR_LPS;
+}
+bool isLinearModel( int32_t m )
+{
+    return m == PDK_MODEL_LINEAR_PMB || m == PDK_MODEL_LINEAR_LPS;
+}
 
 // Derived constants with the reference's expressions.
 void modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
@@ -60,9 +70,12 @@ void modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
     std::memset( &k, 0, sizeof( k ) );
     const double delta = d.horizon;
     const double K = d.K;
-    if ( d.model == PDK_MODEL_PMB || d.model == PDK_MODEL_LPS )
+    if ( d.model == PDK_MODEL_PMB || d.model == PDK_MODEL_LPS || isLinearModel( d.model ) )
     {
         require( delta > 0 && K > 0, "horizon and K must be positive" );
+        // Force<LinearPMB|LinearLPS> exists for NoFracture only
+        // (force/CabanaPD_PMB.hpp:406-409, force/CabanaPD_LPS.hpp:577-580)
+        require( !( isLinearModel( d.model ) && d.fracture ), "the linearized models have no fracture form" );
         // force_models/CabanaPD_PMB.hpp:44-46
         k.c = 18.0 * K / ( kPi * delta * delta * delta * delta );
         if ( d.fracture )
@@ -77,7 +90,7 @@ void modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
         // force_models/CabanaPD_LPS.hpp:75-82
         k.theta_coeff = 3.0 * K - 5.0 * d.G;
         k.s_coeff = 15.0 * d.G;
-        if ( d.model == PDK_MODEL_LPS )
+        if ( isLpsModel( d.model ) )
             require( d.influence == 0 || d.influence == 1, "Influence function type must be 0 or 1." );
     }
     else if ( d.model == PDK_MODEL_HERTZ )
@@ -388,7 +401,7 @@ struct pdk_spec_s
     }
 
     // ---- PD orchestration ---------------------------------------------------
-    bool lps() const { return d.model == PDK_MODEL_LPS; }
+    bool lps() const { return isLpsModel( d.model ); }
 
     void lpsState( bool wv, bool dil )
     {
@@ -527,7 +540,7 @@ int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spe
         s->bcflag.alloc( n, a );
         s->W.alloc( d1, a );
         s->phi.alloc( d1, a );
-        if ( desc->model == PDK_MODEL_LPS )
+        if ( isLpsModel( desc->model ) )
         {
             s->m.alloc( d1, a );
             s->theta.alloc( d1, a );

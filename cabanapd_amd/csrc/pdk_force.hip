@@ -27,11 +27,20 @@ namespace pdk
 {
 namespace
 {
+// Values equal the ABI's model kinds.  The linearized models take the force
+// model of their parent (LinearPMB: PMB, LinearLPS: LPS) and change only the
+// bond geometry of the force and energy passes.
 enum ModelT : int
 {
     kPMB = 0,
-    kLPS = 1
+    kLPS = 1,
+    kLinPMB = 4,
+    kLinLPS = 5
 };
+template <int M>
+constexpr bool kIsLps = ( M == kLPS || M == kLinLPS );
+template <int M>
+constexpr bool kIsLin = ( M == kLinPMB || M == kLinLPS );
 enum PrecT : int
 {
     kF64 = 0,
@@ -124,6 +133,84 @@ __device__ __forceinline__ void getDistance32s( const float4 g, float xinv, doub
     const float ee = ex * ex + ey * ey + ez * ez;
     b.s = ( 2.0f * xe + ee ) * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
     b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
+}
+
+// getLinearizedDistance (CabanaPD_Force.hpp:114-138), operation for
+// operation: stretch linearized in eta, force along the reference bond.
+// Stored so that the force passes' coeff * rx / r is coeff * xi_x / xi.
+__device__ __forceinline__ void getLinearizedDistance64( double xi0, double xi1, double xi2, double ui0,
+                                                         double ui1, double ui2, double xj0, double xj1,
+                                                         double xj2, double uj0, double uj1, double uj2,
+                                                         Bond64& b )
+{
+#pragma clang fp contract( off )
+    const double xi_x = xj0 - xi0;
+    const double eta_u = uj0 - ui0;
+    const double xi_y = xj1 - xi1;
+    const double eta_v = uj1 - ui1;
+    const double xi_z = xj2 - xi2;
+    const double eta_w = uj2 - ui2;
+    b.xi = sqrt( xi_x * xi_x + xi_y * xi_y + xi_z * xi_z );
+    b.s = ( xi_x * eta_u + xi_y * eta_v + xi_z * eta_w ) / ( b.xi * b.xi );
+    b.rx = xi_x;
+    b.ry = xi_y;
+    b.rz = xi_z;
+    b.r = b.xi;
+}
+
+// Mixed form of the linearized stretch: s = xi.eta / |xi|^2 in FP32 from the
+// FP64 differences; direction and 1/r are those of the reference bond.
+__device__ __forceinline__ void linearized32( float gx, float gy, float gz, float xi, float xinv, double ui0,
+                                              double ui1, double ui2, double uj0, double uj1, double uj2, Bond32& b )
+{
+    const float ex = static_cast<float>( uj0 - ui0 );
+    const float ey = static_cast<float>( uj1 - ui1 );
+    const float ez = static_cast<float>( uj2 - ui2 );
+    b.rx = gx;
+    b.ry = gy;
+    b.rz = gz;
+    b.xi = xi;
+    b.rinv = xinv;
+    b.s = ( gx * ex + gy * ey + gz * ez ) * ( xinv * xinv );
+    b.etan = 0.0f;
+}
+
+// Bond geometry of the force and energy passes for the model.
+template <int MODEL>
+__device__ __forceinline__ void bond64( double xi0, double xi1, double xi2, double ui0, double ui1, double ui2,
+                                        double xj0, double xj1, double xj2, double uj0, double uj1, double uj2,
+                                        Bond64& b )
+{
+    if constexpr ( kIsLin<MODEL> )
+        getLinearizedDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
+    else
+        getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
+}
+template <int MODEL>
+__device__ __forceinline__ void bond32( double xi0, double xi1, double xi2, double ui0, double ui1, double ui2,
+                                        double xj0, double xj1, double xj2, double uj0, double uj1, double uj2,
+                                        Bond32& b )
+{
+    if constexpr ( kIsLin<MODEL> )
+    {
+        const float gx = static_cast<float>( xj0 - xi0 );
+        const float gy = static_cast<float>( xj1 - xi1 );
+        const float gz = static_cast<float>( xj2 - xi2 );
+        const float g2 = gx * gx + gy * gy + gz * gz;
+        const float xinv = __builtin_amdgcn_rsqf( g2 );
+        linearized32( gx, gy, gz, g2 * xinv, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
+    }
+    else
+        getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
+}
+template <int MODEL>
+__device__ __forceinline__ void bond32s( const float4 g, float xinv, double ui0, double ui1, double ui2, double uj0,
+                                         double uj1, double uj2, Bond32& b )
+{
+    if constexpr ( kIsLin<MODEL> )
+        linearized32( g.x, g.y, g.z, g.w, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
+    else
+        getDistance32s( g, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
 }
 
 // Uniform per-wave access to the spec of this wave's slice.
@@ -287,7 +374,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     const double c = A.c;
     const double bcc = A.bcc;
     double Ai = 0.0, Bi = 0.0;
-    if constexpr ( MODEL == kLPS )
+    if constexpr ( kIsLps<MODEL> )
     {
         Ai = A.lpsA[pc]; // theta_coeff * theta_i / m_i
         Bi = A.lpsB[pc]; // s_coeff / m_i
@@ -422,8 +509,8 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                 uj[k][1] = u[n + j];
                 uj[k][2] = u[2 * n + j];
                 vj[k] = uvol ? volu : vol[j];
-                nfj[k] = ( FRAC && MODEL == kPMB && any_nofail ) ? ( A.nofail[j] != 0 ) : false;
-                if constexpr ( MODEL == kLPS )
+                nfj[k] = ( FRAC && !kIsLps<MODEL> && any_nofail ) ? ( A.nofail[j] != 0 ) : false;
+                if constexpr ( kIsLps<MODEL> )
                 {
                     aj[k] = Aarr[j];
                     bj[k] = Barr[j];
@@ -450,11 +537,11 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                 bool intact = true;
                 if constexpr ( FRAC )
                     intact = ( ww[k] >> lane ) & 1ull;
-                const bool nf_j = ( MODEL == kLPS ) ? nf_i : nfj[k];
+                const bool nf_j = kIsLps<MODEL> ? nf_i : nfj[k];
                 if constexpr ( PREC == kF64 )
                 {
                     Bond64 b;
-                    getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0], uj[k][1],
+                    bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0], uj[k][1],
                                    uj[k][2], b );
                     if constexpr ( FRAC )
                     {
@@ -464,7 +551,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     if ( !brk && intact )
                     {
                         double coeff;
-                        if constexpr ( MODEL == kPMB )
+                        if constexpr ( !kIsLps<MODEL> )
                             coeff = c * b.s * vj[k];
                         else
                         {
@@ -484,11 +571,11 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                         const uint32_t id =
                             SLOTS ? static_cast<uint32_t>( q ) : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
                         const float4 g = stab[id];
-                        getDistance32s( g, __builtin_amdgcn_rcpf( g.w ), ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2],
+                        bond32s<MODEL>( g, __builtin_amdgcn_rcpf( g.w ), ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2],
                                         b );
                     }
                     else
-                        getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
+                        bond32<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
                                        uj[k][1], uj[k][2], b );
                     if constexpr ( FRAC )
                     {
@@ -517,7 +604,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     if ( !brk && intact )
                     {
                         float coeff;
-                        if constexpr ( MODEL == kPMB )
+                        if constexpr ( !kIsLps<MODEL> )
                             coeff = ( uvol ? cv32u : static_cast<float>( c ) * static_cast<float>( vj[k] ) ) * b.s;
                         else
                         {
@@ -590,7 +677,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
 #pragma clang fp contract( off )
     constexpr int kW = kSlice + kRowMaxSlots - 1;  // window width
     constexpr int NX = ( PREC == kF64 ) ? 3 : 0;    // staged x components
-    constexpr int NAB = ( MODEL == kLPS ) ? 2 : 0;  // staged lpsA, lpsB
+    constexpr int NAB = kIsLps<MODEL> ? 2 : 0;  // staged lpsA, lpsB
     constexpr int KV = 3 + NX + NAB;                // staged vol (GEN)
     constexpr int KNF = KV + 1;                     // staged no-fail flag (GEN)
     constexpr int NC = KV + ( GEN ? 2 : 0 );
@@ -644,7 +731,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const float bandf = A.bandf;
     const int infl1 = A.influence;
     double Ai = 0.0, Bi = 0.0;
-    if constexpr ( MODEL == kLPS )
+    if constexpr ( kIsLps<MODEL> )
     {
         Ai = A.lpsA[pc]; // theta_coeff * theta_i / m_i
         Bi = A.lpsB[pc]; // s_coeff / m_i
@@ -756,7 +843,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
             const double* __restrict__ wr = wb + lane + t;
             const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
             bool nf_j = false;
-            if constexpr ( MODEL == kLPS )
+            if constexpr ( kIsLps<MODEL> )
                 nf_j = nf_i;
             else if constexpr ( GEN && FRAC )
                 nf_j = wr[KNF * kW] != 0.0;
@@ -767,7 +854,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                 {
                     const double vj = GEN ? wr[KV * kW] : volu;
                     Bond64 b;
-                    getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
+                    bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
                                    b );
                     if constexpr ( FRAC )
                     {
@@ -777,7 +864,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                     if ( !brk && intact )
                     {
                         double coeff;
-                        if constexpr ( MODEL == kPMB )
+                        if constexpr ( !kIsLps<MODEL> )
                             coeff = c * b.s * vj;
                         else
                         {
@@ -795,7 +882,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                 const float4 g = stab[q];
                 const float xinv = sinv[q];
                 Bond32 b;
-                getDistance32s( g, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
+                bond32s<MODEL>( g, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
                 if constexpr ( FRAC )
                 {
                     const bool cand = ex && !nf_i && !nf_j;
@@ -812,7 +899,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                 }
                 const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
                 float coeff;
-                if constexpr ( MODEL == kPMB )
+                if constexpr ( !kIsLps<MODEL> )
                     coeff = ( GEN ? c32 * vj32 : cv32u ) * b.s;
                 else
                 {
@@ -1240,7 +1327,7 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
         const double c = A.c;
         double num_bonds = 0.0;
         double m_i = 0.0, theta_i = 0.0;
-        if constexpr ( MODEL == kLPS )
+        if constexpr ( kIsLps<MODEL> )
         {
             m_i = mvals[p];
             theta_i = thetavals[p];
@@ -1260,10 +1347,10 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
                 continue;
             const int64_t j = colp[q * kSlice];
             Bond64 b;
-            getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
+            bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
                            u[2 * n + j], b );
             double wb;
-            if constexpr ( MODEL == kLPS )
+            if constexpr ( kIsLps<MODEL> )
             {
                 const double infl = A.influence ? 1.0 / b.xi : 1.0;
                 wb = 1.0 / num_bonds * 0.5 * A.theta_coeff / 3.0 * ( theta_i * theta_i ) +
@@ -1375,37 +1462,54 @@ ForceFn pickUnroll( int unroll )
     }
 }
 
+template <int MODEL, bool SLOTS>
+ForceFn pickForceModel( const PDArgs& a )
+{
+    const bool mixed = a.precision == PDK_MIXED;
+    if constexpr ( kIsLin<MODEL> ) // no fracture form
+        return mixed ? pickUnroll<MODEL, false, kMixed, SLOTS>( a.unroll )
+                     : pickUnroll<MODEL, false, kF64, SLOTS>( a.unroll );
+    else if ( a.fracture )
+        return mixed ? pickUnroll<MODEL, true, kMixed, SLOTS>( a.unroll )
+                     : pickUnroll<MODEL, true, kF64, SLOTS>( a.unroll );
+    else
+        return mixed ? pickUnroll<MODEL, false, kMixed, SLOTS>( a.unroll )
+                     : pickUnroll<MODEL, false, kF64, SLOTS>( a.unroll );
+}
+
 template <bool SLOTS>
 ForceFn pickForceForm( const PDArgs& a )
 {
-    const int idx = a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 );
-    switch ( idx )
+    switch ( a.model )
     {
-    case 0: return pickUnroll<kPMB, false, kF64, SLOTS>( a.unroll );
-    case 1: return pickUnroll<kPMB, false, kMixed, SLOTS>( a.unroll );
-    case 2: return pickUnroll<kPMB, true, kF64, SLOTS>( a.unroll );
-    case 3: return pickUnroll<kPMB, true, kMixed, SLOTS>( a.unroll );
-    case 4: return pickUnroll<kLPS, false, kF64, SLOTS>( a.unroll );
-    case 5: return pickUnroll<kLPS, false, kMixed, SLOTS>( a.unroll );
-    case 6: return pickUnroll<kLPS, true, kF64, SLOTS>( a.unroll );
-    default: return pickUnroll<kLPS, true, kMixed, SLOTS>( a.unroll );
+    case kPMB: return pickForceModel<kPMB, SLOTS>( a );
+    case kLPS: return pickForceModel<kLPS, SLOTS>( a );
+    case kLinPMB: return pickForceModel<kLinPMB, SLOTS>( a );
+    default: return pickForceModel<kLinLPS, SLOTS>( a );
     }
+}
+
+template <int MODEL, bool GEN>
+ForceFn pickRowsModel( const PDArgs& a )
+{
+    const bool mixed = a.precision == PDK_MIXED;
+    if constexpr ( kIsLin<MODEL> ) // no fracture form
+        return mixed ? kForceRows<MODEL, false, kMixed, GEN> : kForceRows<MODEL, false, kF64, GEN>;
+    else if ( a.fracture )
+        return mixed ? kForceRows<MODEL, true, kMixed, GEN> : kForceRows<MODEL, true, kF64, GEN>;
+    else
+        return mixed ? kForceRows<MODEL, false, kMixed, GEN> : kForceRows<MODEL, false, kF64, GEN>;
 }
 
 template <bool GEN>
 ForceFn pickForceRowsGen( const PDArgs& a )
 {
-    const int idx = a.model * 4 + ( a.fracture ? 2 : 0 ) + ( a.precision == PDK_MIXED ? 1 : 0 );
-    switch ( idx )
+    switch ( a.model )
     {
-    case 0: return kForceRows<kPMB, false, kF64, GEN>;
-    case 1: return kForceRows<kPMB, false, kMixed, GEN>;
-    case 2: return kForceRows<kPMB, true, kF64, GEN>;
-    case 3: return kForceRows<kPMB, true, kMixed, GEN>;
-    case 4: return kForceRows<kLPS, false, kF64, GEN>;
-    case 5: return kForceRows<kLPS, false, kMixed, GEN>;
-    case 6: return kForceRows<kLPS, true, kF64, GEN>;
-    default: return kForceRows<kLPS, true, kMixed, GEN>;
+    case kPMB: return pickRowsModel<kPMB, GEN>( a );
+    case kLPS: return pickRowsModel<kLPS, GEN>( a );
+    case kLinPMB: return pickRowsModel<kLinPMB, GEN>( a );
+    default: return pickRowsModel<kLinLPS, GEN>( a );
     }
 }
 
@@ -1467,6 +1571,10 @@ const char* lpsKernelName( const PDArgs& a )
 }
 const char* energyKernelName( const PDArgs& a )
 {
+    if ( a.model == kLinPMB )
+        return "kEnergy<4, false>";
+    if ( a.model == kLinLPS )
+        return "kEnergy<5, false>";
     return a.model == kLPS ? ( a.fracture ? "kEnergy<1, true>" : "kEnergy<1, false>" )
                            : ( a.fracture ? "kEnergy<0, true>" : "kEnergy<0, false>" );
 }
@@ -1554,8 +1662,10 @@ void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, doub
 {
     if ( a.nslices == 0 )
         return;
-    auto fn = a.model == kLPS ? ( a.fracture ? kEnergy<kLPS, true> : kEnergy<kLPS, false> )
-                              : ( a.fracture ? kEnergy<kPMB, true> : kEnergy<kPMB, false> );
+    auto fn = a.model == kLinPMB ? kEnergy<kLinPMB, false>
+              : a.model == kLinLPS ? kEnergy<kLinLPS, false>
+              : a.model == kLPS    ? ( a.fracture ? kEnergy<kLPS, true> : kEnergy<kLPS, false> )
+                                   : ( a.fracture ? kEnergy<kPMB, true> : kEnergy<kPMB, false> );
     hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( nblocks ) ), dim3( kBlock ), 0, st, args_dev, parity,
                         a.m, a.theta, partial_dev );
     PDK_HIP( hipGetLastError() );

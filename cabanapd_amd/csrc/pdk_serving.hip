@@ -133,7 +133,7 @@ struct BatchEngine::Impl
                 key[k] = gi;
                 ng.emplace_back();
                 ng[gi].variant = *v.h;
-                ng[gi].lps = v.model == PDK_MODEL_LPS;
+                ng[gi].lps = v.model == PDK_MODEL_LPS || v.model == PDK_MODEL_LINEAR_LPS;
             }
             else
                 gi = it->second;
@@ -296,7 +296,7 @@ uint64_t BatchEngine::submit( pdk_spec_s* s, int32_t nsteps )
         throw Error( "call pdk_init before pdk_submit" );
     if ( *v.in_batch )
         throw Error( "spec already submitted" );
-    if ( v.model != PDK_MODEL_PMB && v.model != PDK_MODEL_LPS )
+    if ( v.model == PDK_MODEL_HERTZ || v.model == PDK_MODEL_HERTZ_JKR )
         throw Error( "the batch engine runs PD specimens only" );
     if ( nsteps <= 0 )
         throw Error( "nsteps must be positive" );

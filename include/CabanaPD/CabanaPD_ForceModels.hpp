@@ -1,4 +1,5 @@
-// CabanaPD_ForceModels.hpp -- ForceModel<PMB|LPS, Elastic, NoFracture|Fracture>
+// CabanaPD_ForceModels.hpp -- ForceModel<PMB|LPS, Elastic, NoFracture|Fracture>,
+// ForceModel<LinearPMB|LinearLPS, Elastic, NoFracture>
 // and the Hertz / Hertz-JKR contact models of the drop-in API.
 //
 // Constructors, deduction guides and public members follow the reference:
@@ -75,7 +76,10 @@ struct PDForceModel : BaseForceModel
     pdk_model_desc desc( double dt ) const
     {
         pdk_model_desc d = detail::blankDesc();
-        d.model = std::is_same_v<Tag, LPS> ? PDK_MODEL_LPS : PDK_MODEL_PMB;
+        d.model = std::is_same_v<Tag, LPS>         ? PDK_MODEL_LPS
+                  : std::is_same_v<Tag, LinearPMB> ? PDK_MODEL_LINEAR_PMB
+                  : std::is_same_v<Tag, LinearLPS> ? PDK_MODEL_LINEAR_LPS
+                                                   : PDK_MODEL_PMB;
         d.fracture = std::is_same_v<FractureType, Fracture> ? 1 : 0;
         d.influence = influence_type;
         d.precision = PDK_FP64;
@@ -218,7 +222,56 @@ struct ForceModel<LPS, Elastic, Fracture> : PDForceModel<LPS, Fracture>
     }
 };
 
+// ---- linearized models (force_models/CabanaPD_PMB.hpp:575-597,
+//      force_models/CabanaPD_LPS.hpp:255-273): NoFracture only -----------------
+template <>
+struct ForceModel<LinearPMB, Elastic, NoFracture> : PDForceModel<LinearPMB, NoFracture>
+{
+    ForceModel( LinearPMB, NoFracture, const double h, const double k ) { set( h, k ); }
+    ForceModel( LinearPMB, Elastic, NoFracture, const double h, const double k ) { set( h, k ); }
+
+  private:
+    void set( double h, double k )
+    {
+        force_horizon = h;
+        K = k;
+        influence_type = 1;
+        derive();
+    }
+};
+
+template <>
+struct ForceModel<LinearLPS, Elastic, NoFracture> : PDForceModel<LinearLPS, NoFracture>
+{
+    ForceModel( LinearLPS, NoFracture, const double h, const double k, const double g, const int influence = 0 )
+    {
+        set( h, k, g, influence );
+    }
+    ForceModel( LinearLPS, Elastic, NoFracture, const double h, const double k, const double g,
+                const int influence = 0 )
+    {
+        set( h, k, g, influence );
+    }
+
+  private:
+    void set( double h, double k, double g, int infl )
+    {
+        force_horizon = h;
+        K = k;
+        G = g;
+        influence_type = infl;
+        derive();
+    }
+};
+
 // Deduction guides (force_models/CabanaPD_PMB.hpp:277-297, force_models/CabanaPD_LPS.hpp:274-302)
+ForceModel( LinearPMB, NoFracture, double, double ) -> ForceModel<LinearPMB, Elastic, NoFracture>;
+ForceModel( LinearPMB, Elastic, NoFracture, double, double ) -> ForceModel<LinearPMB, Elastic, NoFracture>;
+ForceModel( LinearLPS, NoFracture, double, double, double ) -> ForceModel<LinearLPS, Elastic, NoFracture>;
+ForceModel( LinearLPS, NoFracture, double, double, double, int ) -> ForceModel<LinearLPS, Elastic, NoFracture>;
+ForceModel( LinearLPS, Elastic, NoFracture, double, double, double ) -> ForceModel<LinearLPS, Elastic, NoFracture>;
+ForceModel( LinearLPS, Elastic, NoFracture, double, double, double, int )
+    -> ForceModel<LinearLPS, Elastic, NoFracture>;
 ForceModel( PMB, NoFracture, double, double ) -> ForceModel<PMB, Elastic, NoFracture>;
 ForceModel( PMB, Elastic, NoFracture, double, double ) -> ForceModel<PMB, Elastic, NoFracture>;
 ForceModel( PMB, double, double, double ) -> ForceModel<PMB, Elastic, Fracture>;

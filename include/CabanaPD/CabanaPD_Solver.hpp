@@ -61,7 +61,7 @@ struct DeviceSpec
     // Particles::create + update -> device; then the Neighbor build.
     void attach( const pdk_model_desc& d, ParticleStorage& s )
     {
-        lps = d.model == PDK_MODEL_LPS;
+        lps = d.model == PDK_MODEL_LPS || d.model == PDK_MODEL_LINEAR_LPS;
         fracture = d.fracture != 0;
         check( pdk_spec_create( context(), &d, static_cast<int64_t>( s.n ), &spec ) );
         check( pdk_spec_upload( spec, s.x.data(), s.u.data(), s.v.data(), s.vol.data(), s.rho.data(),

@@ -68,6 +68,18 @@ struct LPS
 {
     using base_type = LPS;
 };
+// Linearized variants (src/CabanaPD_Types.hpp:147-161): the force model of
+// the parent, linearized stretch in the force and energy passes.
+struct LinearPMB
+{
+    using base_type = PMB;
+    using model_tag = PMB;
+};
+struct LinearLPS
+{
+    using base_type = LPS;
+    using model_tag = LPS;
+};
 struct Elastic
 {
 };

@@ -38,7 +38,12 @@ enum pdk_model_kind {
     PDK_MODEL_PMB = 0,        /* ForceModel<PMB, Elastic, NoFracture|Fracture>  force_models/CabanaPD_PMB.hpp:149-231 */
     PDK_MODEL_LPS = 1,        /* ForceModel<LPS, Elastic, NoFracture|Fracture>  force_models/CabanaPD_LPS.hpp:151-253 */
     PDK_MODEL_HERTZ = 2,      /* HertzianModel     force_models/CabanaPD_Hertzian.hpp:23-88 */
-    PDK_MODEL_HERTZ_JKR = 3   /* HertzianJKRModel  force_models/CabanaPD_HertzianJKR.hpp:23-177 */
+    PDK_MODEL_HERTZ_JKR = 3,  /* HertzianJKRModel  force_models/CabanaPD_HertzianJKR.hpp:23-177 */
+    PDK_MODEL_LINEAR_PMB = 4, /* ForceModel<LinearPMB, Elastic, NoFracture>  force_models/CabanaPD_PMB.hpp:575-597,
+                                 Force<LinearPMB> force/CabanaPD_PMB.hpp:406-538 (linearized stretch) */
+    PDK_MODEL_LINEAR_LPS = 5  /* ForceModel<LinearLPS, Elastic, NoFracture>  force_models/CabanaPD_LPS.hpp:255-273,
+                                 Force<LinearLPS> force/CabanaPD_LPS.hpp:577-724 (linearized stretch in the
+                                 force and energy; weighted volume and dilatation as LPS) */
 };
 
 enum pdk_precision {

@@ -74,6 +74,17 @@ bool isContact( const pdk_model_desc& d )
 {
     return d.model == PDK_MODEL_HERTZ || d.model == PDK_MODEL_HERTZ_JKR;
 }
+// LinearLPS is an LPS model (its force model derives from LPS's,
+// force_models/CabanaPD_LPS.hpp:255-273); LinearPMB from PMB's
+// (force_models/CabanaPD_PMB.hpp:575-597).
+bool isLps( const pdk_model_desc& d )
+{
+    return d.model == PDK_MODEL_LPS || d.model == PDK_MODEL_LINEAR_LPS;
+}
+bool isLinear( const pdk_model_desc& d )
+{
+    return d.model == PDK_MODEL_LINEAR_PMB || d.model == PDK_MODEL_LINEAR_LPS;
+}
 
 // ---------------------------------------------------------------------------
 // Derived model constants.
@@ -83,8 +94,10 @@ int modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
     std::memset( &k, 0, sizeof( k ) );
     const double delta = d.horizon;
     const double K = d.K;
-    if ( d.model == PDK_MODEL_PMB || d.model == PDK_MODEL_LPS )
+    if ( d.model == PDK_MODEL_PMB || d.model == PDK_MODEL_LPS || isLinear( d ) )
     {
+        if ( isLinear( d ) && d.fracture )
+            return 1; // the linearized models exist without fracture only
         // force_models/CabanaPD_PMB.hpp:44-46
         k.c = 18.0 * K / ( kPi * delta * delta * delta * delta );
         if ( d.fracture )
@@ -93,7 +106,7 @@ int modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
             k.s0 = std::sqrt( 5.0 * d.G0 / 9.0 / K / delta );
             if ( d.influence == 0 )
                 k.s0 = std::sqrt( 8.0 * d.G0 / 15.0 / K / delta );
-            if ( d.model == PDK_MODEL_LPS )
+            if ( isLps( d ) )
             {
                 // force_models/CabanaPD_LPS.hpp:214-222
                 if ( d.influence == 1 )
@@ -106,7 +119,7 @@ int modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
         // force_models/CabanaPD_LPS.hpp:75-76
         k.theta_coeff = 3.0 * K - 5.0 * d.G;
         k.s_coeff = 15.0 * d.G;
-        if ( d.model == PDK_MODEL_LPS && ( d.influence < 0 || d.influence > 1 ) )
+        if ( isLps( d ) && ( d.influence < 0 || d.influence > 1 ) )
             return 1;
     }
     else if ( d.model == PDK_MODEL_HERTZ )
@@ -159,6 +172,39 @@ inline void getDistance( const Sys& s, int64_t i, int64_t j, double& xi, double&
     r = std::sqrt( rx * rx + ry * ry + rz * rz );
     xi = std::sqrt( xi_x * xi_x + xi_y * xi_y + xi_z * xi_z );
     st = ( r - xi ) / xi;
+}
+
+// getLinearizedDistance (CabanaPD_Force.hpp:114-138): stretch linearized in
+// eta, force along the reference bond; returned as r = xi and (rx, ry, rz) =
+// xi so that the force passes' coeff * rx / r is the reference's
+// coeff * xi_x / xi.
+inline void getLinearizedDistance( const Sys& s, int64_t i, int64_t j, double& xi, double& r, double& st,
+                                   double& rx, double& ry, double& rz )
+{
+    const double* x = s.x.data();
+    const double* u = s.u.data();
+    const double xi_x = x[3 * j + 0] - x[3 * i + 0];
+    const double eta_u = u[3 * j + 0] - u[3 * i + 0];
+    const double xi_y = x[3 * j + 1] - x[3 * i + 1];
+    const double eta_v = u[3 * j + 1] - u[3 * i + 1];
+    const double xi_z = x[3 * j + 2] - x[3 * i + 2];
+    const double eta_w = u[3 * j + 2] - u[3 * i + 2];
+    xi = std::sqrt( xi_x * xi_x + xi_y * xi_y + xi_z * xi_z );
+    st = ( xi_x * eta_u + xi_y * eta_v + xi_z * eta_w ) / ( xi * xi );
+    rx = xi_x;
+    ry = xi_y;
+    rz = xi_z;
+    r = xi;
+}
+
+// The bond geometry of the force and energy passes for the model.
+inline void bondGeometry( const Sys& s, int64_t i, int64_t j, double& xi, double& r, double& st, double& rx,
+                          double& ry, double& rz )
+{
+    if ( isLinear( s.d ) )
+        getLinearizedDistance( s, i, j, xi, r, st, rx, ry, rz );
+    else
+        getDistance( s, i, j, xi, r, st, rx, ry, rz );
 }
 
 // ---------------------------------------------------------------------------
@@ -367,7 +413,7 @@ inline double influence( const Sys& s, double xi )
 // Force<LPS,*>::computeWeightedVolume (force/CabanaPD_LPS.hpp:89-112, 287-329)
 void computeWeightedVolume( Sys& s )
 {
-    if ( s.d.model != PDK_MODEL_LPS )
+    if ( !isLps( s.d ) )
         return;
     const bool frac = s.d.fracture != 0;
     std::fill( s.m.begin(), s.m.end(), 0.0 );
@@ -393,7 +439,7 @@ void computeWeightedVolume( Sys& s )
 // Force<LPS,*>::computeDilatation (force/CabanaPD_LPS.hpp:114-141, 331-378)
 void computeDilatation( Sys& s )
 {
-    if ( s.d.model != PDK_MODEL_LPS )
+    if ( !isLps( s.d ) )
         return;
     const bool frac = s.d.fracture != 0;
     std::fill( s.theta.begin(), s.theta.end(), 0.0 );
@@ -438,7 +484,7 @@ inline double lpsForceCoeff( const Sys& s, double st, double xi, double vol, dou
 void computeForceFull( Sys& s )
 {
     const bool frac = s.d.fracture != 0;
-    const bool lps = s.d.model == PDK_MODEL_LPS;
+    const bool lps = isLps( s.d );
     const double c = s.k.c;
     const double bcc = s.k.bond_break_coeff;
 #pragma omp parallel for schedule( static )
@@ -451,7 +497,7 @@ void computeForceFull( Sys& s )
             const int64_t j = row[q];
             double fx_i = 0.0, fy_i = 0.0, fz_i = 0.0;
             double xi, r, st, rx, ry, rz;
-            getDistance( s, i, j, xi, r, st, rx, ry, rz );
+            bondGeometry( s, i, j, xi, r, st, rx, ry, rz );
             if ( !frac )
             {
                 // force/CabanaPD_PMB.hpp:105-118, force/CabanaPD_LPS.hpp:161-174
@@ -501,7 +547,7 @@ void computeForceFull( Sys& s )
 void computeEnergyFull( Sys& s )
 {
     const bool frac = s.d.fracture != 0;
-    const bool lps = s.d.model == PDK_MODEL_LPS;
+    const bool lps = isLps( s.d );
     const double c = s.k.c;
     double Phi = 0.0, D = 0.0;
     std::fill( s.W.begin(), s.W.end(), 0.0 );
@@ -525,7 +571,7 @@ void computeEnergyFull( Sys& s )
         {
             const int64_t j = row[q];
             double xi, r, st, rx, ry, rz;
-            getDistance( s, i, j, xi, r, st, rx, ry, rz );
+            bondGeometry( s, i, j, xi, r, st, rx, ry, rz );
             double w;
             if ( lps )
             {
@@ -815,7 +861,7 @@ void computeForce( Sys& s, bool reset )
 void updateForce( Sys& s )
 {
     // Solver::updateForce (CabanaPD_Solver.hpp:462-477)
-    if ( s.d.fracture && s.d.model == PDK_MODEL_LPS )
+    if ( s.d.fracture && isLps( s.d ) )
         computeWeightedVolume( s );
     computeDilatation( s );
     computeForce( s, true );
@@ -1077,7 +1123,7 @@ int orc_compute_energy( orc_sys s, double* W, double* D )
 int orc_init( orc_sys s )
 {
     // Solver::init(bc) (CabanaPD_Solver.hpp:229-279)
-    if ( !s->d.fracture && s->d.model == PDK_MODEL_LPS )
+    if ( !s->d.fracture && isLps( s->d ) )
         computeWeightedVolume( *s );
     updateForce( *s );
     if ( !isContact( s->d ) )

@@ -251,3 +251,105 @@ def test_model_constants():
     assert k["theta_coeff"] == 3 * K - 5 * G
     assert k["s_coeff"] == 15 * G
     assert k["bond_break_coeff"] == pytest.approx((1 + k["s0"]) ** 2, rel=1e-15)
+
+
+def _rotation(x, angle):
+    """u = R x - x for a rotation by angle about z."""
+    c, s = np.cos(angle), np.sin(angle)
+    u = np.zeros_like(x)
+    u[:, 0] = (c - 1.0) * x[:, 0] - s * x[:, 1]
+    u[:, 1] = s * x[:, 0] + (c - 1.0) * x[:, 1]
+    return u
+
+
+def _lattice_offsets(m):
+    r = np.arange(-m, m + 1)
+    a, b, c = np.meshgrid(r, r, r, indexing="ij")
+    off = np.stack([a.ravel(), b.ravel(), c.ravel()], axis=1).astype(np.float64)
+    d2 = np.sum(off * off, axis=1)
+    return off[(d2 > 0) & (d2 <= m * m + 1e-9)]
+
+
+def _run_field(model, u, m, nc, K=1.0, G=0.5, influence=None):
+    w = configs.lattice_sum_cube(model=model, m=m, nc=nc, field="linear", coeff=0.0, K=K, G=G,
+                                 influence=influence)
+    w["u"] = u(w["x"])
+    s = orc.OracleSpec(w["desc"], w["x"].shape[0], w["x"], w["vol"], u=w["u"], v=w["v"], rho=w["rho"])
+    s.build_bonds()
+    s.compute_weighted_volume()
+    s.compute_dilatation()
+    s.compute_force(True)
+    Phi, _ = s.compute_energy()
+    return w, s, Phi
+
+
+@pytest.mark.parametrize("model", ["LINEAR_PMB", "PMB"])
+def test_linear_pmb_rotation(model):
+    """test_force_linear_pmb's model (tstForce.hpp:1157-1167) on a rigid
+    rotation: the linearized stretch of offset (a, b, c) is exactly
+    (cos t - 1)(a^2 + b^2)/(a^2 + b^2 + c^2), the non-linear one 0.  W_i is the
+    lattice sum of c s^2 |xi| V / 4 on interior particles; f_i = 0 by
+    symmetry."""
+    m, nc, K, angle = 3, 11, 1.0, 0.1
+    w, s, Phi = _run_field(model, lambda x: _rotation(x, angle), m, nc, K=K)
+    horizon = w["desc"].horizon
+    dx = 2.0 / nc
+    c = 18.0 * K / (np.pi * horizon ** 4)
+    off = _lattice_offsets(m)
+    a2b2 = off[:, 0] ** 2 + off[:, 1] ** 2
+    xi = np.sqrt(np.sum(off * off, axis=1)) * dx
+    s_lin = (np.cos(angle) - 1.0) * a2b2 / np.sum(off * off, axis=1)
+    W_ref = np.sum(0.25 * c * s_lin * s_lin * xi * dx ** 3) if model == "LINEAR_PMB" else 0.0
+    x, f, W = w["x"], s.get("f"), s.get("W")
+    sel = interior(x, horizon, 1.1)
+    assert sel.sum() == 125
+    assert np.all(np.abs(f[sel]) <= 1e-12)
+    assert np.all(np.abs(W[sel] - W_ref) <= 1e-12 * max(W_ref, 1e-300) + 1e-18)
+    if model == "LINEAR_PMB":
+        assert W_ref > 1e-6  # the linearized model sees the rotation
+
+
+def test_linear_pmb_stretch():
+    """Uniaxial stretch u = (e x, 0, 0): linearized stretch e a^2/(a^2+b^2+c^2)
+    per offset, f_i = 0 and W_i the lattice sum on the interior; differs from
+    PMB at O(e^2)."""
+    m, nc, K, e = 3, 11, 1.0, 0.1
+    field = lambda x: np.stack([e * x[:, 0], 0 * x[:, 0], 0 * x[:, 0]], axis=1)  # noqa: E731
+    w, s, _ = _run_field("LINEAR_PMB", field, m, nc, K=K)
+    horizon = w["desc"].horizon
+    dx = 2.0 / nc
+    c = 18.0 * K / (np.pi * horizon ** 4)
+    off = _lattice_offsets(m)
+    n2 = np.sum(off * off, axis=1)
+    s_lin = e * off[:, 0] ** 2 / n2
+    W_ref = np.sum(0.25 * c * s_lin * s_lin * np.sqrt(n2) * dx * dx ** 3)
+    sel = interior(w["x"], horizon, 1.1)
+    assert np.all(np.abs(s.get("f")[sel]) <= 1e-12)
+    assert np.all(np.abs(s.get("W")[sel] - W_ref) <= 1e-12 * W_ref)
+    _, s2, _ = _run_field("PMB", field, m, nc, K=K)
+    assert np.all(np.abs(s2.get("W")[sel] - W_ref) > 1e-3 * W_ref)
+
+
+def test_linear_lps_rotation():
+    """test_force_linear_lps's model (tstForce.hpp:1186-1198, influence 1) on a
+    rigid rotation: the dilatation uses the non-linear stretch (the weighted
+    volume and dilatation passes are LPS's), so theta_i = 0 and
+    W_i = sum (s_coeff / m_i) / 2 * (1/|xi|) s_lin^2 |xi|^2 V; m_i the
+    lattice sum of |xi| V (influence 1/|xi|)."""
+    m, nc, K, G, angle = 3, 15, 1.0, 0.5, 0.1
+    w, s, _ = _run_field("LINEAR_LPS", lambda x: _rotation(x, angle), m, nc, K=K, G=G, influence=1)
+    horizon = w["desc"].horizon
+    dx = 2.0 / nc
+    off = _lattice_offsets(m)
+    n2 = np.sum(off * off, axis=1)
+    xi = np.sqrt(n2) * dx
+    vol = dx ** 3
+    m_ref = np.sum(xi * vol)
+    s_lin = (np.cos(angle) - 1.0) * (off[:, 0] ** 2 + off[:, 1] ** 2) / n2
+    W_ref = np.sum(0.5 * (15.0 * G / m_ref) * (1.0 / xi) * s_lin * s_lin * xi * xi * vol)
+    sel = interior(w["x"], horizon, 2.1)
+    assert sel.sum() > 0
+    assert np.all(np.abs(s.get("m")[sel] - m_ref) <= 1e-12 * m_ref)
+    assert np.all(np.abs(s.get("theta")[sel]) <= 1e-12)
+    assert np.all(np.abs(s.get("f")[sel]) <= 1e-12)
+    assert np.all(np.abs(s.get("W")[sel] - W_ref) <= 1e-10 * W_ref)
