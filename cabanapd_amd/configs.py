@@ -50,6 +50,61 @@ def _base(desc, x, vol, rho, name, dx, steps, output_every=0):
                 bc=None, body=None, steps=steps, output_every=output_every, name=name, dx=dx)
 
 
+def elastic_wave(precision="fp64"):
+    """regression_test/inputs/elastic_wave.json run through
+    examples/mechanics/elastic_wave.cpp: 41^3 cell-centred particles on
+    [-0.5, 0.5]^3, rho 100, K 1, G 0.5, horizon 0.075 (+1e-10), influence 0,
+    NoFracture, dt 0.01, int(0.6 / 0.01) = 60 steps, radial Gaussian shell
+    u = 0.001 exp(-(|x| - 0.25)^2 / 0.07^2) x/|x|.
+
+    The example asks for ForceModel<LinearLPS>, which keeps the LPS force tag
+    (force_models/CabanaPD_LPS.hpp:31-32,255-272; only LinearPMB replaces it,
+    force_models/CabanaPD_PMB.hpp:584), and the solver picks its force class
+    by that tag (CabanaPD_Solver.hpp:96-99): the reference run, and the
+    polynomial its regression test fits, is the LPS force."""
+    x, vol, d = grid_cell_centred([-0.5, -0.5, -0.5], [0.5, 0.5, 0.5], (41, 41, 41))
+    horizon = 0.075 + 1e-10
+    desc = make_desc("LPS", precision=precision, horizon=horizon, K=1.0, G=0.5, dt=0.01, influence=0)
+    steps = int(0.6 / 0.01)  # Inputs truncation (CabanaPD_Input.hpp:109)
+    w = _base(desc, x, vol, 100.0, "elastic_wave", d[0], steps)
+    norm = np.sqrt(np.sum(x * x, axis=1))
+    amp = 0.001 * np.exp(-((norm - 0.25) ** 2) / 0.07 / 0.07)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        comp = np.where(norm[:, None] > 0.0, x / norm[:, None], 0.0)
+    w["u"] = amp[:, None] * comp
+    return w
+
+
+def line_profile(x, dx, profile_dim=0):
+    """Particles of Region<Line> (CabanaPD_Geometry.hpp:137-168): the other two
+    coordinates within +-dx/2 of 0."""
+    o = [(profile_dim + 1) % 3, (profile_dim + 2) % 3]
+    sel = np.ones(x.shape[0], dtype=bool)
+    for k in o:
+        sel &= (x[:, k] < dx / 2.0) & (x[:, k] > -dx / 2.0)
+    return np.nonzero(sel)[0]
+
+
+def polyfit_check(xs, us, fixture):
+    """regression_test/tstElasticWave.hpp:62-93: relative error to the fitted
+    polynomial and the L2 norm of the differences inside the fitted window.
+    Returns (max relative error, L2 norm, points checked)."""
+    c = fixture["coefficients"]
+    lo, hi = fixture["x_window"]
+    elo, ehi = fixture["x_exclude"]
+    worst, total, k = 0.0, 0.0, 0
+    for xv, uv in zip(xs, us):
+        if xv > hi or xv < lo or (elo < xv < ehi):
+            continue
+        fit = 0.0
+        for ci in c:
+            fit = fit * xv + ci
+        worst = max(worst, abs((uv - fit) / fit))
+        total += (uv - fit) ** 2
+        k += 1
+    return worst, math.sqrt(total), k
+
+
 def elastic_cube(n_side=41, steps=500, precision="fp64", dt=1e-7):
     """Plain FP64 PMB elastic cube: 41^3 node-centred particles, dx = 1/40,
     delta = 3.01 dx, E = 72 GPa, nu = 0.25, rho = 2440, Gaussian radial

@@ -1,16 +1,17 @@
 // CabanaPD_Boundary.hpp -- regions, boundary conditions, body terms,
-// pre-notches and region time series of the drop-in API.
+// pre-notches, line profiles and region time series of the drop-in API.
 //
 // Reference: src/CabanaPD_Geometry.hpp:24-168 (Region), :170-273
 // (ParticleSteeringVector), src/CabanaPD_Boundary.hpp:26-204
 // (BoundaryCondition, createBoundaryCondition), src/CabanaPD_BodyTerm.hpp:25-69,
-// src/CabanaPD_Prenotch.hpp:174-281, src/CabanaPD_OutputProfiles.hpp:113-215.
+// src/CabanaPD_Prenotch.hpp:174-281, src/CabanaPD_OutputProfiles.hpp:26-215.
 #ifndef CABANAPD_DROPIN_BOUNDARY_HPP
 #define CABANAPD_DROPIN_BOUNDARY_HPP
 
 #include <algorithm>
 #include <array>
 #include <cassert>
+#include <cmath>
 #include <fstream>
 #include <functional>
 #include <iomanip>
@@ -86,6 +87,32 @@ struct Region<Cylinder>
                      ( x( pid, 1 ) - y_center ) * ( x( pid, 1 ) - y_center );
         return rsq >= radius_in * radius_in && rsq <= radius_out * radius_out && x( pid, 2 ) >= low_z &&
                x( pid, 2 ) <= high_z;
+    }
+};
+
+struct Line
+{
+};
+
+// Particles on the line through the origin along profile_dim: the other two
+// reference coordinates strictly within half a cell of zero.
+template <>
+struct Region<Line>
+{
+    Kokkos::Array<double, 2> _half_dx;
+    Kokkos::Array<int, 2> _dims;
+    template <typename ArrayType>
+    Region( const double profile_dim, const ArrayType dx )
+    {
+        const int p = static_cast<int>( profile_dim );
+        _dims = { ( p + 1 ) % 3, ( p + 2 ) % 3 };
+        _half_dx = { dx[_dims[0]] / 2.0, dx[_dims[1]] / 2.0 };
+    }
+    template <class PositionType>
+    bool inside( const PositionType& x, const int pid ) const
+    {
+        return x( pid, _dims[0] ) < _half_dx[0] && x( pid, _dims[0] ) > -_half_dx[0] &&
+               x( pid, _dims[1] ) < _half_dx[1] && x( pid, _dims[1] ) > -_half_dx[1];
     }
 };
 
@@ -295,6 +322,44 @@ auto createOutputTimeSeries( std::string name, const Inputs& inputs, ExecSpace e
 {
     return OutputTimeSeries<ReducerType, FunctorType>( name, inputs, ParticleSteeringVector( exec, particles, geom ),
                                                        user );
+}
+
+// ---- profiles along a line (src/CabanaPD_OutputProfiles.hpp:26-93) ----------
+// Appends "rank x value" per particle of Region<Line> to file_name, in
+// particle order, with the stream's default precision as the reference does.
+// The values are read from the host copy the solver refreshes after run().
+template <typename ParticlesT, typename UserFunctor>
+void createOutputProfile( std::string file_name, ParticlesT particles, const int profile_dim, UserFunctor user )
+{
+    Region<Line> line( profile_dim, particles.dx );
+    ParticleSteeringVector sv( Kokkos::DefaultExecutionSpace{}, particles, line );
+    auto x = particles.sliceReferencePosition();
+    std::fstream fout( file_name, std::ios::app | std::ios::out );
+    for ( std::size_t b = 0; b < sv.size(); b++ )
+    {
+        const int p = static_cast<int>( sv._view[b] );
+        fout << 0 << " " << x( p, profile_dim ) << " " << user( p ) << std::endl;
+    }
+}
+
+template <typename ParticlesT>
+void createDisplacementProfile( std::string file_name, ParticlesT particles, const int profile_dim,
+                                int displacement_dim = -1 )
+{
+    if ( displacement_dim == -1 )
+        displacement_dim = profile_dim;
+    auto u = particles.sliceDisplacement();
+    createOutputProfile( file_name, particles, profile_dim,
+                         [=]( const int pid ) { return u( pid, displacement_dim ); } );
+}
+
+template <typename ParticlesT>
+void createDisplacementMagnitudeProfile( std::string file_name, ParticlesT particles, const int profile_dim )
+{
+    auto u = particles.sliceDisplacement();
+    createOutputProfile( file_name, particles, profile_dim, [=]( const int pid ) {
+        return std::sqrt( u( pid, 0 ) * u( pid, 0 ) + u( pid, 1 ) * u( pid, 1 ) + u( pid, 2 ) * u( pid, 2 ) );
+    } );
 }
 
 } // namespace CabanaPD

@@ -14,6 +14,7 @@
 #define CABANAPD_DROPIN_FORCEMODELS_HPP
 
 #include <stdexcept>
+#include <type_traits>
 
 #include <pdk.h>
 
@@ -23,6 +24,23 @@ namespace CabanaPD
 {
 namespace detail
 {
+// Parent model of a (possibly linearized) model tag.
+template <class Tag>
+struct ParentTag
+{
+    using type = Tag;
+};
+template <>
+struct ParentTag<LinearPMB>
+{
+    using type = PMB;
+};
+template <>
+struct ParentTag<LinearLPS>
+{
+    using type = LPS;
+};
+
 inline pdk_model_constants constantsOf( const pdk_model_desc& d )
 {
     pdk_model_constants k{};
@@ -58,8 +76,13 @@ struct BaseForceModel
 template <class Tag, class FractureType>
 struct PDForceModel : BaseForceModel
 {
-    using model_tag = Tag;
-    using force_tag = Tag;
+    // As in the reference, a linearized model keeps its parent's model_tag
+    // (src/CabanaPD_Types.hpp:147-161) and only LinearPMB replaces the force
+    // tag (force_models/CabanaPD_PMB.hpp:584); ForceModel<LinearLPS> keeps
+    // force_tag = LPS (force_models/CabanaPD_LPS.hpp:31-32, 255-272), so the
+    // solver runs the LPS force for it (CabanaPD_Solver.hpp:96-99).
+    using model_tag = typename detail::ParentTag<Tag>::type;
+    using force_tag = std::conditional_t<std::is_same_v<Tag, LinearLPS>, LPS, Tag>;
     using base_model = Tag;
     using fracture_type = FractureType;
     using mechanics_type = Elastic;
@@ -76,10 +99,10 @@ struct PDForceModel : BaseForceModel
     pdk_model_desc desc( double dt ) const
     {
         pdk_model_desc d = detail::blankDesc();
-        d.model = std::is_same_v<Tag, LPS>         ? PDK_MODEL_LPS
-                  : std::is_same_v<Tag, LinearPMB> ? PDK_MODEL_LINEAR_PMB
-                  : std::is_same_v<Tag, LinearLPS> ? PDK_MODEL_LINEAR_LPS
-                                                   : PDK_MODEL_PMB;
+        d.model = std::is_same_v<force_tag, LPS>         ? PDK_MODEL_LPS
+                  : std::is_same_v<force_tag, LinearPMB> ? PDK_MODEL_LINEAR_PMB
+                  : std::is_same_v<force_tag, LinearLPS> ? PDK_MODEL_LINEAR_LPS
+                                                         : PDK_MODEL_PMB;
         d.fracture = std::is_same_v<FractureType, Fracture> ? 1 : 0;
         d.influence = influence_type;
         d.precision = PDK_FP64;

@@ -4,8 +4,12 @@ dropin_kalthoff_winkler is the reference's Kalthoff-Winkler example
 (examples/mechanics/kalthoff_winkler.cpp) written against the drop-in headers;
 it checks the reference's regression value (7.42e-2 +- 1.1e-4,
 regression_test/tstKalthoffWinkler.hpp:53) on the input deck the reference
-ships (tests/golden/kalthoff_winkler.json).  dropin_force runs the tstForce
-flow (Force, Neighbor, computeForce, computeEnergy) against lattice sums.
+ships (tests/golden/kalthoff_winkler.json).  dropin_elastic_wave is
+examples/mechanics/elastic_wave.cpp on the deck of
+regression_test/tstElasticWave.hpp; its displacement profile is checked
+against that test's polynomial fit and against the oracle.  dropin_force runs
+the tstForce flow (Force, Neighbor, computeForce, computeEnergy) against
+lattice sums.
 
 The drivers are built on the CPU beforehand (`make -C tests/cpp`, also done by
 __graft_entry__.build()); the test builds them only when they are missing.
@@ -47,3 +51,33 @@ def test_dropin_kalthoff_winkler_example(tmp_path):
     deck = os.path.join(HERE, "golden", "kalthoff_winkler.json")
     rc, out = run([driver("dropin_kalthoff_winkler"), deck, str(tmp_path / "crack_y.txt")], str(tmp_path), 110)
     assert rc == 0 and "KW_REGRESSION_OK" in out
+
+
+def test_dropin_elastic_wave_example(tmp_path):
+    import json
+
+    import numpy as np
+
+    from cabanapd_amd import configs
+    from tests.parity import oracle_spec
+    deck = os.path.join(HERE, "golden", "elastic_wave.json")
+    prof = tmp_path / "displacement_profile.txt"
+    rc, out = run([driver("dropin_elastic_wave"), deck, str(prof)], str(tmp_path), 110)
+    assert rc == 0 and "EW_DONE" in out
+    rows = np.array([[float(t) for t in ln.split()] for ln in prof.read_text().splitlines() if ln.strip()])
+    xs, us = rows[:, 1], rows[:, 2]
+    fx = json.load(open(os.path.join(HERE, "golden", "elastic_wave_polyfit.json")))
+    worst, l2, k = configs.polyfit_check(xs, us, fx)
+    say(f"  polyfit: worst rel {worst:.4f} l2 {l2:.3e} points {k}")
+    assert k >= 20 and worst < fx["max_rel_err"] and l2 < fx["max_l2_err"]
+    # the same run in the oracle; the file holds six significant digits
+    w = configs.elastic_wave()
+    o = oracle_spec(w)
+    configs.setup(o, w)
+    o.step(w["steps"])
+    line = configs.line_profile(w["x"], w["dx"])
+    xo, uo = w["x"][line, 0], o.get("u")[line, 0]
+    assert rows.shape[0] == line.size
+    a, b = np.argsort(xs), np.argsort(xo)
+    assert np.allclose(xs[a], xo[b], rtol=1e-5, atol=1e-7)
+    assert np.max(np.abs(us[a] - uo[b])) <= 1e-5 * np.max(np.abs(uo))

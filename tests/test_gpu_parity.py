@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from cabanapd_amd import configs
-from tests.parity import bonds_equal, broken_set, pair, rel_err, say
+from tests.parity import bonds_equal, broken_set, gpu_spec, oracle_spec, pair, rel_err, say
 
 pytestmark = pytest.mark.gpu
 
@@ -277,3 +277,35 @@ def test_gpu_linearized_models_bitwise(model):
     say(f"  mixed vs FP64 u rel err {e:.3e} ({mx.kernel_name(0)})")
     assert e <= 1e-5
     mx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_elastic_wave_regression(precision):
+    """regression_test/tstElasticWave.hpp on its own deck (41^3, 60 steps):
+    FP64 equals the oracle bit for bit, the mixed form stays within 1e-5 of
+    it, and u_x along the x axis meets the reference's polynomial check
+    (relative error < 3.5 %, L2 deviation < 3e-5)."""
+    import json
+    import os
+    say(f"case: elastic wave 41^3 60 steps {precision}")
+    w = configs.elastic_wave(precision=precision)
+    g = gpu_spec(w)
+    configs.setup(g, w)
+    wo = configs.elastic_wave()
+    o = oracle_spec(wo)
+    configs.setup(o, wo)
+    g.step(w["steps"])
+    o.step(wo["steps"])
+    ug, uo = g.get("u"), o.get("u")
+    if precision == "fp64":
+        assert_same(ug, uo, "u")
+        assert_same(g.get("theta"), o.get("theta"), "theta")
+    else:
+        e = rel_err(ug, uo)
+        say(f"  u rel err vs FP64 oracle {e:.3e}")
+        assert e <= 1e-5
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "elastic_wave_polyfit.json")))
+    line = configs.line_profile(w["x"], w["dx"])
+    worst, l2, k = configs.polyfit_check(w["x"][line, 0], ug[line, 0], fx)
+    say(f"  polyfit: worst rel {worst:.4f} l2 {l2:.3e} points {k}")
+    assert k >= 20 and worst < fx["max_rel_err"] and l2 < fx["max_l2_err"]

@@ -3,6 +3,7 @@
 regression_test/tstKalthoffWinkler.hpp:53  max damaged-particle y = 7.42e-2 +- 1.1e-4
 regression_test/tstHertz.hpp:135           sqrt(KE_f / KE_i) = e +- 1e-3
 regression_test/tstHertzJKR.hpp:145        min pull-off force / f_c = -1 +- 5e-3
+regression_test/tstElasticWave.hpp:54-93   u_x profile vs polynomial fit: rel < 3.5 %, L2 < 3e-5
 """
 import math
 
@@ -74,3 +75,24 @@ def test_jkr_pulloff():
         fx.append(s.get("f")[0, 0])
     fc = orc.model_constants(desc)["fc"]
     assert abs(min(fx) * vol / fc - (-1.0)) <= 5e-3
+
+
+def test_oracle_elastic_wave_regression():
+    """regression_test/tstElasticWave.hpp: the 41^3 elastic-wave deck (the
+    LPS force, which the example's ForceModel<LinearLPS> dispatches to); u_x
+    along the x axis after 60 steps (t = 0.6) within 3.5 % of the reference's
+    polynomial fit and its L2 deviation below 3e-5."""
+    import json
+    import os
+    w = configs.elastic_wave()
+    s = orc.OracleSpec(w["desc"], w["x"].shape[0], w["x"], w["vol"], u=w["u"], v=w["v"], rho=w["rho"])
+    configs.setup(s, w)
+    s.step(w["steps"])
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "elastic_wave_polyfit.json")))
+    line = configs.line_profile(w["x"], w["dx"])
+    u = s.get("u")
+    worst, l2, k = configs.polyfit_check(w["x"][line, 0], u[line, 0], fx)
+    s.close()
+    assert k >= 20
+    assert worst < fx["max_rel_err"], worst
+    assert l2 < fx["max_l2_err"], l2
