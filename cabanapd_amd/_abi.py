@@ -20,6 +20,60 @@ VECTOR_FIELDS = {"x", "u", "v", "f"}
 PDK_BC_FORCE_VALUE = 0
 PDK_BC_FORCE_UPDATE = 1
 
+PDK_REGION_ALL, PDK_REGION_BOX, PDK_REGION_CYLINDER, PDK_REGION_LINE, PDK_REGION_INDICES = range(5)
+REDUCER = {"sum": 0, "max": 1, "min": 2}
+
+
+class RegionDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("dim", C.c_int32),
+        ("low", C.c_double * 3), ("high", C.c_double * 3),
+        ("r_in", C.c_double), ("r_out", C.c_double), ("low_z", C.c_double), ("high_z", C.c_double),
+        ("x_center", C.c_double), ("y_center", C.c_double),
+        ("half", C.c_double * 2),
+        ("ids", C.POINTER(C.c_int64)), ("count", C.c_int64),
+    ]
+
+
+class Quantity(C.Structure):
+    _fields_ = [
+        ("field", C.c_int32), ("component", C.c_int32), ("gate_field", C.c_int32),
+        ("weight_volume", C.c_int32), ("gate_threshold", C.c_double),
+    ]
+
+
+def box_region(low, high):
+    r = RegionDesc()
+    r.kind = PDK_REGION_BOX
+    for d in range(3):
+        r.low[d] = low[d]
+        r.high[d] = high[d]
+    return r
+
+
+def cylinder_region(r_in, r_out, low_z, high_z, x_center=0.0, y_center=0.0):
+    r = RegionDesc()
+    r.kind = PDK_REGION_CYLINDER
+    r.r_in, r.r_out, r.low_z, r.high_z, r.x_center, r.y_center = r_in, r_out, low_z, high_z, x_center, y_center
+    return r
+
+
+def line_region(dim, dx):
+    """Region<Line>(profile_dim, dx): half a cell in the two other dimensions."""
+    r = RegionDesc()
+    r.kind = PDK_REGION_LINE
+    r.dim = dim
+    r.half[0] = dx[(dim + 1) % 3] / 2.0
+    r.half[1] = dx[(dim + 2) % 3] / 2.0
+    return r
+
+
+def all_region():
+    r = RegionDesc()
+    r.kind = PDK_REGION_ALL
+    return r
+
+
 # Kernel classes of pdk_kernel_time.
 KCLS_FORCE, KCLS_LPS, KCLS_ENERGY, KCLS_BUILD, KCLS_OTHER = range(5)
 

@@ -144,6 +144,8 @@ struct pdk_ctx_s
     std::atomic<int64_t> live_specs{ 0 };
 };
 
+struct pdk_series_s;
+
 struct pdk_spec_s
 {
     pdk_ctx ctx = nullptr;
@@ -158,6 +160,8 @@ struct pdk_spec_s
     int32_t nsten = 0;             // stencil entries (0 = no stencil)
     SlotForm slot;                 // stencil-slot form (slot.slots == 0: compact form)
     std::vector<int64_t> h_perm, h_inv;
+    int64_t layout_version = 0;            // bumped whenever the device order changes
+    std::vector<pdk_series_s*> series_list; // live region series on this spec
     PDArgs h{};
     int32_t cur = 0;
     bool uploaded = false, built = false, inited = false;
@@ -382,6 +386,27 @@ struct pdk_spec_s
 
     double* Ucur() { return cur ? U1.as<double>() : U0.as<double>(); }
 
+    // Device array of a pdk_field ([3][n] when *vec, else [n]); null when the
+    // spec does not hold that field.
+    const double* fieldDev( int32_t field, bool* vec )
+    {
+        *vec = false;
+        switch ( field )
+        {
+        case PDK_FIELD_X: *vec = true; return x.as<double>();
+        case PDK_FIELD_U: *vec = true; return Ucur();
+        case PDK_FIELD_V: *vec = true; return v.as<double>();
+        case PDK_FIELD_F: *vec = true; return f.as<double>();
+        case PDK_FIELD_VOL: return vol.as<double>();
+        case PDK_FIELD_RHO: return rho.as<double>();
+        case PDK_FIELD_W: return W.as<double>();
+        case PDK_FIELD_PHI: return phi.as<double>();
+        case PDK_FIELD_M: return m.as<double>();
+        case PDK_FIELD_THETA: return theta.as<double>();
+        default: throw Error( "unknown field" );
+        }
+    }
+
     // Bond count and longest row from the device row lengths.
     void refreshStats()
     {
@@ -565,6 +590,75 @@ int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spe
     } );
 }
 
+// ---- region series -------------------------------------------------------------
+struct pdk_series_s
+{
+    pdk_spec spec = nullptr;
+    pdk_quantity q{};
+    int32_t red = 0;
+    int64_t capacity = 0, used = 0;
+    std::vector<int64_t> ids;  // members, ascending caller id
+    int64_t version = -1;      // spec layout the device ids were made for
+    DevBuf dev_ids, partial, values;
+    std::vector<double> times;
+
+    // Device ids of the members in the spec's current order.
+    void mapIds()
+    {
+        if ( version == spec->layout_version )
+            return;
+        std::vector<int64_t> d( ids.size() );
+        for ( size_t k = 0; k < ids.size(); k++ )
+            d[k] = spec->h_inv[ids[k]];
+        if ( dev_ids.bytes() < d.size() * sizeof( int64_t ) )
+            dev_ids.alloc( d.size() * sizeof( int64_t ), spec->acct() );
+        if ( !d.empty() )
+        {
+            PDK_HIP( hipMemcpyAsync( dev_ids.as<int64_t>(), d.data(), d.size() * sizeof( int64_t ),
+                                     hipMemcpyHostToDevice, spec->st ) );
+            PDK_HIP( hipStreamSynchronize( spec->st ) );
+        }
+        version = spec->layout_version;
+    }
+
+    // Device pointers of the quantity as of now (the displacement buffer
+    // alternates between steps).
+    QuantityArgs args()
+    {
+        QuantityArgs Q{};
+        bool vec = false;
+        const double* src = spec->fieldDev( q.field, &vec );
+        require( src != nullptr, "the series' field is not held by this spec" );
+        Q.src = src + ( vec ? q.component * spec->n : 0 );
+        if ( q.gate_field >= 0 )
+        {
+            Q.gate = spec->fieldDev( q.gate_field, &vec );
+            require( Q.gate != nullptr, "the series' gate field is not held by this spec" );
+            Q.threshold = q.gate_threshold;
+        }
+        if ( q.weight_volume )
+            Q.vol = spec->vol.as<double>();
+        return Q;
+    }
+};
+
+namespace
+{
+// The spec is going away: its series keep their host data but lose their
+// device buffers (accounted to the spec's context) and refuse further work.
+void detachSeries( pdk_spec s )
+{
+    for ( pdk_series_s* ser : s->series_list )
+    {
+        ser->dev_ids.release();
+        ser->partial.release();
+        ser->values.release();
+        ser->spec = nullptr;
+    }
+    s->series_list.clear();
+}
+} // namespace
+
 int pdk_spec_destroy( pdk_spec s )
 {
     return guarded( [&] {
@@ -576,6 +670,7 @@ int pdk_spec_destroy( pdk_spec s )
         if ( s->st )
             (void)hipStreamSynchronize( s->st );
         s->dropGraphs();
+        detachSeries( s );
         if ( s->st )
             (void)hipStreamDestroy( s->st );
         if ( s->ctx )
@@ -633,6 +728,7 @@ int pdk_spec_upload( pdk_spec s, const double* x, const double* u, const double*
                                s->scratch, s->acct(), s->st );
         s->h_perm.resize( n );
         s->h_inv.resize( n );
+        s->layout_version++;
         PDK_HIP( hipMemcpyAsync( s->h_perm.data(), s->perm.as<int64_t>(), n * sizeof( int64_t ), hipMemcpyDeviceToHost,
                                  s->st ) );
         PDK_HIP( hipMemcpyAsync( s->h_inv.data(), s->inv.as<int64_t>(), n * sizeof( int64_t ), hipMemcpyDeviceToHost,
@@ -991,6 +1087,20 @@ int pdk_compute_energy( pdk_spec s, double* total_strain_energy, double* total_d
     } );
 }
 
+int pdk_compute_energy_async( pdk_spec s )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        if ( isContact( s->d ) )
+            return;
+        const int64_t nb = energyBlocks( s->h );
+        if ( s->partial.bytes() < static_cast<size_t>( 2 * nb ) * sizeof( double ) )
+            s->partial.alloc( 2 * nb * sizeof( double ), s->acct() );
+        s->timed( kClsEnergy, [&] { launchEnergy( s->dargs(), s->h, s->cur, s->partial.as<double>(), nb, s->st ); } );
+    } );
+}
+
 int pdk_init( pdk_spec s )
 {
     return guarded( [&] {
@@ -1070,22 +1180,8 @@ int pdk_download( pdk_spec s, int32_t field, double* out )
     return guarded( [&] {
         require( s && s->uploaded && out, "spec not uploaded" );
         const int64_t n = s->n;
-        const double* src = nullptr;
         bool vec = false;
-        switch ( field )
-        {
-        case PDK_FIELD_X: src = s->x.as<double>(); vec = true; break;
-        case PDK_FIELD_U: src = s->Ucur(); vec = true; break;
-        case PDK_FIELD_V: src = s->v.as<double>(); vec = true; break;
-        case PDK_FIELD_F: src = s->f.as<double>(); vec = true; break;
-        case PDK_FIELD_VOL: src = s->vol.as<double>(); break;
-        case PDK_FIELD_RHO: src = s->rho.as<double>(); break;
-        case PDK_FIELD_W: src = s->W.as<double>(); break;
-        case PDK_FIELD_PHI: src = s->phi.as<double>(); break;
-        case PDK_FIELD_M: src = s->m.as<double>(); break;
-        case PDK_FIELD_THETA: src = s->theta.as<double>(); break;
-        default: throw Error( "unknown field" );
-        }
+        const double* src = s->fieldDev( field, &vec );
         if ( !src )
         {
             std::fill( out, out + n, 0.0 );
@@ -1108,6 +1204,185 @@ int pdk_sync( pdk_spec s )
     return guarded( [&] {
         require( s != nullptr, "null spec" );
         PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_series_create( pdk_spec s, const pdk_region_desc* r, const pdk_quantity* q, int32_t reducer,
+                       int64_t capacity, pdk_series* out )
+{
+    return guarded( [&] {
+        require( s && r && q && out, "null argument" );
+        *out = nullptr;
+        require( s->uploaded, "spec not uploaded" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( reducer >= PDK_REDUCE_SUM && reducer <= PDK_REDUCE_MIN, "unknown reducer" );
+        require( capacity >= 0, "capacity must be >= 0" );
+        bool vec = false;
+        (void)s->fieldDev( q->field, &vec );
+        require( q->component >= 0 && q->component < ( vec ? 3 : 1 ), "component out of range for the field" );
+        if ( q->gate_field >= 0 )
+        {
+            bool gvec = false;
+            (void)s->fieldDev( q->gate_field, &gvec );
+            require( !gvec, "the gate field must be a scalar field" );
+        }
+        require( q->weight_volume == 0 || q->weight_volume == 1, "weight_volume must be 0 or 1" );
+        const int64_t n = s->n;
+        auto ser = std::make_unique<pdk_series_s>();
+        switch ( r->kind )
+        {
+        case PDK_REGION_ALL:
+            ser->ids.resize( n );
+            for ( int64_t c = 0; c < n; c++ )
+                ser->ids[c] = c;
+            break;
+        case PDK_REGION_INDICES:
+        {
+            require( r->count >= 0 && ( r->ids || r->count == 0 ), "INDICES region needs ids[count]" );
+            ser->ids.assign( r->ids, r->ids + r->count );
+            std::sort( ser->ids.begin(), ser->ids.end() );
+            for ( size_t k = 0; k < ser->ids.size(); k++ )
+            {
+                require( ser->ids[k] >= 0 && ser->ids[k] < n, "region id out of range" );
+                require( k == 0 || ser->ids[k] != ser->ids[k - 1], "repeated region id" );
+            }
+            break;
+        }
+        case PDK_REGION_BOX:
+        case PDK_REGION_CYLINDER:
+        case PDK_REGION_LINE:
+        {
+            require( r->kind != PDK_REGION_LINE || ( r->dim >= 0 && r->dim < 3 ), "line dim must be 0, 1 or 2" );
+            RegionParams P{};
+            P.kind = r->kind;
+            P.dim = r->dim;
+            for ( int d = 0; d < 3; d++ )
+            {
+                P.low[d] = r->low[d];
+                P.high[d] = r->high[d];
+            }
+            P.r_in = r->r_in;
+            P.r_out = r->r_out;
+            P.low_z = r->low_z;
+            P.high_z = r->high_z;
+            P.x_center = r->x_center;
+            P.y_center = r->y_center;
+            P.half[0] = r->half[0];
+            P.half[1] = r->half[1];
+            DevBuf mask;
+            mask.alloc( std::max<int64_t>( n, 1 ), s->acct() );
+            launchRegionMask( P, s->x.as<double>(), s->inv.as<int64_t>(), n, mask.as<uint8_t>(), s->st );
+            std::vector<uint8_t> hm( n );
+            if ( n )
+                PDK_HIP( hipMemcpyAsync( hm.data(), mask.as<uint8_t>(), n, hipMemcpyDeviceToHost, s->st ) );
+            PDK_HIP( hipStreamSynchronize( s->st ) );
+            for ( int64_t c = 0; c < n; c++ )
+                if ( hm[c] )
+                    ser->ids.push_back( c );
+            break;
+        }
+        default:
+            throw Error( "unknown region kind" );
+        }
+        ser->spec = s;
+        ser->q = *q;
+        ser->red = reducer;
+        ser->capacity = capacity;
+        ser->partial.alloc( kSeriesMaxPartials * sizeof( double ), s->acct() );
+        if ( capacity )
+            ser->values.alloc( capacity * sizeof( double ), s->acct() );
+        ser->times.reserve( static_cast<size_t>( std::min<int64_t>( capacity, 1 << 20 ) ) );
+        ser->mapIds();
+        s->series_list.push_back( ser.get() );
+        *out = ser.release();
+    } );
+}
+
+int pdk_series_destroy( pdk_series ser )
+{
+    return guarded( [&] {
+        if ( !ser )
+            return;
+        if ( pdk_spec s = ser->spec )
+        {
+            (void)hipStreamSynchronize( s->st ); // pending updates write into ser's buffers
+            auto& l = s->series_list;
+            l.erase( std::remove( l.begin(), l.end(), ser ), l.end() );
+        }
+        delete ser;
+    } );
+}
+
+int pdk_series_members( pdk_series ser, int64_t* count )
+{
+    return guarded( [&] {
+        require( ser && count, "null argument" );
+        *count = static_cast<int64_t>( ser->ids.size() );
+    } );
+}
+
+int pdk_series_update( pdk_series ser, double time )
+{
+    return guarded( [&] {
+        require( ser != nullptr, "null series" );
+        pdk_spec s = ser->spec;
+        require( s != nullptr, "the series' spec was destroyed" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( ser->used < ser->capacity, "series is full" );
+        ser->mapIds();
+        const QuantityArgs Q = ser->args();
+        s->timed( kClsOther, [&] {
+            launchSeriesReduce( ser->dev_ids.as<int64_t>(), static_cast<int64_t>( ser->ids.size() ), Q, ser->red,
+                                ser->partial.as<double>(), ser->values.as<double>() + ser->used, s->st );
+        } );
+        ser->times.push_back( time );
+        ser->used++;
+    } );
+}
+
+int pdk_series_values( pdk_series ser, int64_t capacity, double* times, double* values, int64_t* count )
+{
+    return guarded( [&] {
+        require( ser != nullptr, "null series" );
+        const int64_t k = std::min( std::max<int64_t>( capacity, 0 ), ser->used );
+        if ( k > 0 && values )
+        {
+            pdk_spec s = ser->spec;
+            require( s != nullptr, "the series' spec was destroyed" );
+            PDK_HIP( hipMemcpyAsync( values, ser->values.as<double>(), k * sizeof( double ), hipMemcpyDeviceToHost,
+                                     s->st ) );
+            PDK_HIP( hipStreamSynchronize( s->st ) );
+        }
+        if ( k > 0 && times )
+            std::copy( ser->times.begin(), ser->times.begin() + k, times );
+        if ( count )
+            *count = ser->used;
+    } );
+}
+
+int pdk_series_gather( pdk_series ser, int64_t capacity, int64_t* ids, double* values, int64_t* count )
+{
+    return guarded( [&] {
+        require( ser != nullptr, "null series" );
+        pdk_spec s = ser->spec;
+        require( s != nullptr, "the series' spec was destroyed" );
+        const int64_t m = static_cast<int64_t>( ser->ids.size() );
+        const int64_t k = std::min( std::max<int64_t>( capacity, 0 ), m );
+        if ( k > 0 && values )
+        {
+            ser->mapIds();
+            const QuantityArgs Q = ser->args();
+            const size_t bytes = static_cast<size_t>( k ) * sizeof( double );
+            if ( s->staging.bytes() < bytes )
+                s->staging.alloc( bytes, s->acct() );
+            launchGatherQuantity( ser->dev_ids.as<int64_t>(), k, Q, s->staging.as<double>(), s->st );
+            PDK_HIP( hipMemcpyAsync( values, s->staging.as<double>(), bytes, hipMemcpyDeviceToHost, s->st ) );
+            PDK_HIP( hipStreamSynchronize( s->st ) );
+        }
+        if ( k > 0 && ids )
+            std::copy( ser->ids.begin(), ser->ids.begin() + k, ids );
+        if ( count )
+            *count = m;
     } );
 }
 

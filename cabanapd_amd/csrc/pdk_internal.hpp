@@ -234,6 +234,46 @@ void launchContactInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int3
                                    double* partial_dev, int64_t nblocks, hipStream_t st );
 const char* contactKernelName( const PDArgs& a );
 
+// ---- region outputs (pdk_output.hip) ----------------------------------------
+enum RegionKind : int32_t
+{
+    kRegionAll = 0,
+    kRegionBox = 1,
+    kRegionCylinder = 2,
+    kRegionLine = 3,
+    kRegionIndices = 4
+};
+enum ReduceKind : int32_t
+{
+    kReduceSum = 0,
+    kReduceMax = 1,
+    kReduceMin = 2
+};
+struct RegionParams
+{
+    int32_t kind, dim;
+    double low[3], high[3];
+    double r_in, r_out, low_z, high_z, x_center, y_center;
+    double half[2];
+};
+// Per-member value: src[p], zeroed where gate[p] <= threshold (gate set),
+// times vol[p] (vol set).  src already points at the component's row.
+struct QuantityArgs
+{
+    const double* src;
+    const double* gate;
+    const double* vol;
+    double threshold;
+};
+constexpr int32_t kSeriesMaxPartials = 256;
+void launchRegionMask( const RegionParams& R, const double* x_dev, const int64_t* inv_dev, int64_t n,
+                       uint8_t* mask_dev, hipStream_t st );
+int32_t seriesPartials( int64_t count );
+void launchSeriesReduce( const int64_t* idx_dev, int64_t count, const QuantityArgs& Q, int32_t red,
+                         double* partial_dev, double* out_dev, hipStream_t st );
+void launchGatherQuantity( const int64_t* idx_dev, int64_t count, const QuantityArgs& Q, double* out_dev,
+                           hipStream_t st );
+
 } // namespace pdk
 
 #endif

@@ -60,6 +60,14 @@ _SIGS = {
     "pdk_submit": (C.c_int, [_VP, _VP, C.c_int32, C.POINTER(C.c_uint64)]),
     "pdk_poll": (C.c_int, [_VP, C.c_uint64, _I32]),
     "pdk_batch_progress": (C.c_int, [_VP, C.c_int32, _I32]),
+    "pdk_series_create": (C.c_int, [_VP, C.POINTER(_abi.RegionDesc), C.POINTER(_abi.Quantity), C.c_int32,
+                                    C.c_int64, C.POINTER(_VP)]),
+    "pdk_series_destroy": (C.c_int, [_VP]),
+    "pdk_series_members": (C.c_int, [_VP, _I64]),
+    "pdk_series_update": (C.c_int, [_VP, C.c_double]),
+    "pdk_series_values": (C.c_int, [_VP, C.c_int64, _D, _D, _I64]),
+    "pdk_series_gather": (C.c_int, [_VP, C.c_int64, _I64, _D, _I64]),
+    "pdk_compute_energy_async": (C.c_int, [_VP]),
 }
 
 
@@ -241,6 +249,15 @@ class Spec:
     def step(self, nsteps=1):
         _check(lib().pdk_step(self.h, nsteps))
 
+    def compute_energy_async(self):
+        _check(lib().pdk_compute_energy_async(self.h))
+
+    def series(self, region, field, component=0, reducer="max", capacity=1024, gate=None, threshold=0.0,
+               weight_volume=False, ids=None):
+        """Device-resident time series of field[component] reduced over a
+        region (an _abi.*_region descriptor, or ids= for an explicit list)."""
+        return Series(self, region, field, component, reducer, capacity, gate, threshold, weight_volume, ids)
+
     def steps_done(self):
         s = C.c_int64()
         _check(lib().pdk_steps_done(self.h, C.byref(s)))
@@ -291,3 +308,63 @@ class Spec:
         buf = C.create_string_buffer(256)
         _check(lib().pdk_kernel_name(self.h, cls, buf, 256))
         return buf.value.decode()
+
+
+class Series:
+    """pdk_series: a region, a per-particle quantity and a reducer
+    (OutputTimeSeries, CabanaPD_OutputProfiles.hpp:113-215)."""
+
+    def __init__(self, spec, region, field, component, reducer, capacity, gate, threshold, weight_volume, ids):
+        self.spec = spec
+        r = region if region is not None else _abi.RegionDesc()
+        self._ids = None
+        if ids is not None:
+            self._ids = np.ascontiguousarray(ids, dtype=np.int64)
+            r.kind = _abi.PDK_REGION_INDICES
+            r.ids = self._ids.ctypes.data_as(_I64)
+            r.count = self._ids.size
+        q = _abi.Quantity()
+        q.field = FIELD[field]
+        q.component = component
+        q.gate_field = -1 if gate is None else FIELD[gate]
+        q.gate_threshold = threshold
+        q.weight_volume = 1 if weight_volume else 0
+        self.h = _VP()
+        _check(lib().pdk_series_create(spec.h, C.byref(r), C.byref(q), _abi.REDUCER[reducer], capacity,
+                                       C.byref(self.h)))
+        self._ids = None
+
+    def close(self):
+        if self.h:
+            _check(lib().pdk_series_destroy(self.h))
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def members(self):
+        c = C.c_int64()
+        _check(lib().pdk_series_members(self.h, C.byref(c)))
+        return c.value
+
+    def update(self, time=0.0):
+        _check(lib().pdk_series_update(self.h, time))
+
+    def values(self):
+        c = C.c_int64()
+        _check(lib().pdk_series_values(self.h, 0, None, None, C.byref(c)))
+        t = np.empty(c.value)
+        v = np.empty(c.value)
+        _check(lib().pdk_series_values(self.h, c.value, _dp(t), _dp(v), C.byref(c)))
+        return t, v
+
+    def gather(self):
+        m = self.members()
+        ids = np.empty(m, dtype=np.int64)
+        v = np.empty(m)
+        c = C.c_int64()
+        _check(lib().pdk_series_gather(self.h, m, ids.ctypes.data_as(_I64), _dp(v), C.byref(c)))
+        return ids, v

@@ -202,6 +202,67 @@ int pdk_steps_done(pdk_spec spec, int64_t* steps);
 int pdk_download(pdk_spec spec, int32_t field, double* out);
 int pdk_sync(pdk_spec spec);
 
+/* ---- region outputs on the device ----------------------------------------------
+ * A series binds a region of the specimen (its members fixed at creation,
+ * from the reference positions), a per-particle quantity and a reducer.
+ * pdk_series_update reduces the quantity over the members into the next slot
+ * of a device-resident time series -- OutputTimeSeries::update
+ * (CabanaPD_OutputProfiles.hpp:146-163) -- without copying anything to the
+ * host: it is enqueued on the spec's stream and returns at once (the one
+ * asynchronous call besides the serving calls).  pdk_series_values downloads
+ * the series (OutputTimeSeries::print, :165-187); pdk_series_gather reads the
+ * quantity of every member now, members in ascending caller id
+ * (createOutputProfile, :26-62).  The members of a series are in ascending
+ * caller id, the reference's steering-vector order (CabanaPD_Geometry.hpp:170-273).
+ * Sum reduces in a fixed order of its own (reproducible, not the reference's
+ * order: compare sums to rounding); Max/Min are exact. */
+enum pdk_region_kind {
+    PDK_REGION_ALL = 0,       /* every particle */
+    PDK_REGION_BOX = 1,       /* Region<RectangularPrism>: low[d] <= x_d <= high[d]  CabanaPD_Geometry.hpp:54-82 */
+    PDK_REGION_CYLINDER = 2,  /* Region<Cylinder>: r_in^2 <= (x-xc)^2+(y-yc)^2 <= r_out^2, low_z <= z <= high_z  :84-135 */
+    PDK_REGION_LINE = 3,      /* Region<Line>: the coordinates (dim+1)%3 and (dim+2)%3 strictly within
+                                 +-half[0], +-half[1]  :137-168 */
+    PDK_REGION_INDICES = 4    /* the caller ids in ids[count] (a region given by a user functor,
+                                 evaluated by the caller), any order, no repeats */
+};
+typedef struct pdk_region_desc {
+    int32_t kind;             /* pdk_region_kind */
+    int32_t dim;              /* LINE: profile dimension */
+    double low[3], high[3];   /* BOX */
+    double r_in, r_out, low_z, high_z, x_center, y_center;  /* CYLINDER */
+    double half[2];           /* LINE */
+    const int64_t* ids;       /* INDICES: copied at creation */
+    int64_t count;            /* INDICES */
+} pdk_region_desc;
+enum pdk_reducer {
+    PDK_REDUCE_SUM = 0,       /* Kokkos::Sum<double> */
+    PDK_REDUCE_MAX = 1,       /* Kokkos::Max<double>: identity -DBL_MAX */
+    PDK_REDUCE_MIN = 2        /* Kokkos::Min<double>: identity +DBL_MAX */
+};
+/* value(p) = field(p, component); where gate_field >= 0 it is value(p) if
+ * gate_field(p) > gate_threshold, else 0 (the crack-extent functor of
+ * examples/mechanics/kalthoff_winkler.cpp:135-142: damage > 0.3 ? y : 0);
+ * weight_volume = 1 multiplies by the particle volume (sum of W: total strain
+ * energy).  Energy and damage are the values of the last energy pass. */
+typedef struct pdk_quantity {
+    int32_t field;            /* pdk_field */
+    int32_t component;        /* 0..2 for vector fields, 0 for scalars */
+    int32_t gate_field;       /* -1 or a scalar pdk_field */
+    int32_t weight_volume;    /* 0 or 1 */
+    double gate_threshold;
+} pdk_quantity;
+typedef struct pdk_series_s* pdk_series;
+int pdk_series_create(pdk_spec spec, const pdk_region_desc* region, const pdk_quantity* q, int32_t reducer,
+                      int64_t capacity, pdk_series* series);
+int pdk_series_destroy(pdk_series series);
+int pdk_series_members(pdk_series series, int64_t* count);
+int pdk_series_update(pdk_series series, double time);
+int pdk_series_values(pdk_series series, int64_t capacity, double* times, double* values, int64_t* count);
+int pdk_series_gather(pdk_series series, int64_t capacity, int64_t* ids, double* values, int64_t* count);
+/* computeEnergy (CabanaPD_Force.hpp:200-219) without reading the totals
+ * back: refreshes W and damage on the device for the series that follow. */
+int pdk_compute_energy_async(pdk_spec spec);
+
 /* ---- tuning --------------------------------------------------------------------
  * Force-pass variant: unroll = 1, 2, 4 or 8 gathers the neighbor data of that
  * many bond slots per group; unroll = 0 (stencil-slot layout only) stages

@@ -10,9 +10,10 @@ import math
 import numpy as np
 import pytest
 
-from cabanapd_amd import configs
+from cabanapd_amd import _abi, configs
 from cabanapd_amd._abi import make_desc
 from oracle import binding as orc
+from oracle import regions
 
 
 def oracle_from(w):
@@ -27,11 +28,17 @@ def test_kalthoff_winkler_regression():
     assert w["steps"] == 700
     s = oracle_from(w)
     series = []
+    box = regions.members(w["x"], _abi.box_region(w["x"].min(axis=0), w["x"].max(axis=0)))
+    assert box.size == w["x"].shape[0]
     for k in range(1, w["steps"] + 1):
         s.step(1)
         if k % w["output_every"] == 0:
             s.compute_energy()  # Solver::output computes energy and damage
-            series.append(configs.kw_crack_y(w["x"], s.get("phi")))
+            phi = s.get("phi")
+            series.append(configs.kw_crack_y(w["x"], phi))
+            # the same observable through the region-output restatement
+            q = regions.quantity(w["x"][box, 1], gate=phi[box], threshold=0.3)
+            assert regions.reduce(q, "max") == series[-1]
     assert len(series) == 14
     assert abs(series[-1] - 7.42e-2) <= 1.1e-4
 
