@@ -13,9 +13,18 @@ PDK_MIXED = 1
 
 FIELD = {
     "x": 0, "u": 1, "v": 2, "f": 3, "vol": 4, "rho": 5,
-    "W": 6, "phi": 7, "m": 8, "theta": 9,
+    "W": 6, "phi": 7, "m": 8, "theta": 9, "stress": 10,
 }
 VECTOR_FIELDS = {"x", "u", "v", "f"}
+
+
+def field_shape(field, n):
+    """Host shape of a downloaded field: [n][3], [n][3][3] (stress) or [n]."""
+    if field in VECTOR_FIELDS:
+        return (n, 3)
+    if field == "stress":
+        return (n, 3, 3)
+    return (n,)
 
 PDK_BC_FORCE_VALUE = 0
 PDK_BC_FORCE_UPDATE = 1

@@ -154,7 +154,7 @@ struct pdk_spec_s
     int64_t n = 0, nslices = 0;
     hipStream_t st = nullptr;
     DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
-        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab;
+        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stress;
     double lat_h[3] = { 0, 0, 0 }; // lattice spacing when lattice_rows
     int64_t lat_dims[3] = { 0, 0, 0 };
     int32_t nsten = 0;             // stencil entries (0 = no stencil)
@@ -258,6 +258,7 @@ struct pdk_spec_s
         h.lpsB = lpsB.as<double>();
         h.W = W.as<double>();
         h.phi = phi.as<double>();
+        h.stress = stress.as<double>();
         h.radius = radius ? radius.as<double>() : nullptr;
         h.u_neigh = u_neigh ? u_neigh.as<double>() : nullptr;
         h.sid = sid ? sid.as<uint8_t>() : nullptr;
@@ -386,17 +387,18 @@ struct pdk_spec_s
 
     double* Ucur() { return cur ? U1.as<double>() : U0.as<double>(); }
 
-    // Device array of a pdk_field ([3][n] when *vec, else [n]); null when the
+    // Device array of a pdk_field, [ncomp][n] in device order; null when the
     // spec does not hold that field.
-    const double* fieldDev( int32_t field, bool* vec )
+    const double* fieldDev( int32_t field, int32_t* ncomp )
     {
-        *vec = false;
+        *ncomp = 1;
         switch ( field )
         {
-        case PDK_FIELD_X: *vec = true; return x.as<double>();
-        case PDK_FIELD_U: *vec = true; return Ucur();
-        case PDK_FIELD_V: *vec = true; return v.as<double>();
-        case PDK_FIELD_F: *vec = true; return f.as<double>();
+        case PDK_FIELD_X: *ncomp = 3; return x.as<double>();
+        case PDK_FIELD_U: *ncomp = 3; return Ucur();
+        case PDK_FIELD_V: *ncomp = 3; return v.as<double>();
+        case PDK_FIELD_F: *ncomp = 3; return f.as<double>();
+        case PDK_FIELD_STRESS: *ncomp = 9; return stress.as<double>();
         case PDK_FIELD_VOL: return vol.as<double>();
         case PDK_FIELD_RHO: return rho.as<double>();
         case PDK_FIELD_W: return W.as<double>();
@@ -626,13 +628,13 @@ struct pdk_series_s
     QuantityArgs args()
     {
         QuantityArgs Q{};
-        bool vec = false;
-        const double* src = spec->fieldDev( q.field, &vec );
+        int32_t nc = 1;
+        const double* src = spec->fieldDev( q.field, &nc );
         require( src != nullptr, "the series' field is not held by this spec" );
-        Q.src = src + ( vec ? q.component * spec->n : 0 );
+        Q.src = src + q.component * spec->n;
         if ( q.gate_field >= 0 )
         {
-            Q.gate = spec->fieldDev( q.gate_field, &vec );
+            Q.gate = spec->fieldDev( q.gate_field, &nc );
             require( Q.gate != nullptr, "the series' gate field is not held by this spec" );
             Q.threshold = q.gate_threshold;
         }
@@ -1087,6 +1089,22 @@ int pdk_compute_energy( pdk_spec s, double* total_strain_energy, double* total_d
     } );
 }
 
+int pdk_compute_stress( pdk_spec s )
+{
+    return guarded( [&] {
+        require( s && s->built, "bonds not built" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( !isContact( s->d ), "the contact models have no stress pass" );
+        if ( !s->stress )
+        {
+            s->stress.alloc( static_cast<size_t>( 9 ) * std::max<int64_t>( s->n, 1 ) * sizeof( double ), s->acct() );
+            s->pushArgs();
+        }
+        s->timed( kClsOther, [&] { launchStress( s->dargs(), s->h, s->cur, s->st ); } );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
 int pdk_compute_energy_async( pdk_spec s )
 {
     return guarded( [&] {
@@ -1180,20 +1198,22 @@ int pdk_download( pdk_spec s, int32_t field, double* out )
     return guarded( [&] {
         require( s && s->uploaded && out, "spec not uploaded" );
         const int64_t n = s->n;
-        bool vec = false;
-        const double* src = s->fieldDev( field, &vec );
+        int32_t nc = 1;
+        const double* src = s->fieldDev( field, &nc );
         if ( !src )
         {
-            std::fill( out, out + n, 0.0 );
+            std::fill( out, out + nc * n, 0.0 );
             return;
         }
-        const size_t bytes = ( vec ? 3 : 1 ) * n * sizeof( double );
+        const size_t bytes = static_cast<size_t>( nc ) * n * sizeof( double );
         if ( s->staging.bytes() < bytes )
             s->staging.alloc( bytes, s->acct() );
-        if ( vec )
+        if ( nc == 3 )
             launchScatterAos3( src, s->perm.as<int64_t>(), s->staging.as<double>(), n, s->st );
-        else
+        else if ( nc == 1 )
             launchScatterScalar( src, s->perm.as<int64_t>(), s->staging.as<double>(), n, s->st );
+        else
+            launchScatterRows( src, s->perm.as<int64_t>(), s->staging.as<double>(), n, nc, s->st );
         PDK_HIP( hipMemcpyAsync( out, s->staging.as<double>(), bytes, hipMemcpyDeviceToHost, s->st ) );
         PDK_HIP( hipStreamSynchronize( s->st ) );
     } );
@@ -1217,14 +1237,14 @@ int pdk_series_create( pdk_spec s, const pdk_region_desc* r, const pdk_quantity*
         require( !s->in_batch, "spec is running in the batch engine" );
         require( reducer >= PDK_REDUCE_SUM && reducer <= PDK_REDUCE_MIN, "unknown reducer" );
         require( capacity >= 0, "capacity must be >= 0" );
-        bool vec = false;
-        (void)s->fieldDev( q->field, &vec );
-        require( q->component >= 0 && q->component < ( vec ? 3 : 1 ), "component out of range for the field" );
+        int32_t nc = 1;
+        (void)s->fieldDev( q->field, &nc );
+        require( q->component >= 0 && q->component < nc, "component out of range for the field" );
         if ( q->gate_field >= 0 )
         {
-            bool gvec = false;
-            (void)s->fieldDev( q->gate_field, &gvec );
-            require( !gvec, "the gate field must be a scalar field" );
+            int32_t gnc = 1;
+            (void)s->fieldDev( q->gate_field, &gnc );
+            require( gnc == 1, "the gate field must be a scalar field" );
         }
         require( q->weight_volume == 0 || q->weight_volume == 1, "weight_volume must be 0 or 1" );
         const int64_t n = s->n;

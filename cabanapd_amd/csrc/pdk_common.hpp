@@ -77,6 +77,7 @@ struct PDArgs
     double* lpsB;           // LPS: s_coeff / m per particle
     double* W;
     double* phi;
+    double* stress;         // [9][n], component 3 a + b (null until the first stress pass)
     const double* radius;   // contact: per-particle radius or null
     double* u_neigh;        // contact: u at last list build [3][n]
     double* block_partial;  // energy / max-displacement partials

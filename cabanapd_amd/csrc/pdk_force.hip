@@ -1405,6 +1405,107 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stress: Force<*>::computeStressFull (force/CabanaPD_PMB.hpp:157-202,
+// 341-401, 492-536; force/CabanaPD_LPS.hpp:221-267, 506-572, 677-723) after
+// computeStress's reset (CabanaPD_Force.hpp:221-236).  Per bond, f is half the
+// bond force (PMB: 0.5 * (c s V_j), LPS: the force coefficient times 0.5; the
+// fracture forms multiply by mu, LPS fracture visits unbroken bonds only) and
+// stress(i, a, b) += f_a xi_b with xi the reference bond.  FP64 in every
+// precision; one lane per particle, nine accumulators, written [9][n].
+// ---------------------------------------------------------------------------
+template <int MODEL, bool FRAC>
+__global__ __launch_bounds__( kBlock ) void kStress( const PDArgs* __restrict__ specs, int32_t parity )
+{
+#pragma clang fp contract( off )
+    const PDArgs& A = *specs;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int wave = threadIdx.x / kSlice;
+    const int64_t slice = blockIdx.x * (int64_t)kSlicesPerBlock + wave;
+    const int64_t n = A.n;
+    const int64_t p = slice * kSlice + lane;
+    if ( slice >= A.nslices || p >= n )
+        return;
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ u = A.U[parity];
+    const double xi0 = x[p], xi1 = x[n + p], xi2 = x[2 * n + p];
+    const double ui0 = u[p], ui1 = u[n + p], ui2 = u[2 * n + p];
+    const int32_t mycnt = A.cnt[p];
+    const int64_t base = A.slice_ptr[slice];
+    const int32_t qend = slotEnd( A, static_cast<int32_t>( A.slice_ptr[slice + 1] - base ), mycnt );
+    const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
+    const uint64_t* __restrict__ bits = A.intact + base;
+    const double* __restrict__ vol = A.vol;
+    const double c = A.c;
+    double Ai = 0.0, Bi = 0.0;
+    if constexpr ( kIsLps<MODEL> )
+    {
+        Ai = A.lpsA[p];
+        Bi = A.lpsB[p];
+    }
+    double s00 = 0.0, s01 = 0.0, s02 = 0.0, s10 = 0.0, s11 = 0.0, s12 = 0.0, s20 = 0.0, s21 = 0.0, s22 = 0.0;
+    for ( int32_t q = 0; q < qend; q++ )
+    {
+        if ( !slotBond( A, base, q, lane, mycnt ) )
+            continue;
+        double mu = 1.0;
+        if constexpr ( FRAC )
+        {
+            mu = static_cast<double>( ( bits[q] >> lane ) & 1ull );
+            if ( kIsLps<MODEL> && !( mu > 0 ) )
+                continue;
+        }
+        const int64_t j = colp[q * kSlice];
+        const double xj0 = x[j], xj1 = x[n + j], xj2 = x[2 * n + j];
+        Bond64 b;
+        bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, u[j], u[n + j], u[2 * n + j], b );
+        const double xi_x = xj0 - xi0;
+        const double xi_y = xj1 - xi1;
+        const double xi_z = xj2 - xi2;
+        double coeff;
+        if constexpr ( kIsLps<MODEL> )
+        {
+            const double infl = A.influence ? 1.0 / b.xi : 1.0;
+            coeff = ( Ai + A.lpsA[j] + b.s * ( Bi + A.lpsB[j] ) ) * infl * b.xi * vol[j];
+            coeff *= 0.5;
+        }
+        else
+            coeff = 0.5 * ( c * b.s * vol[j] );
+        double fx, fy, fz;
+        if constexpr ( FRAC )
+        {
+            fx = mu * coeff * b.rx / b.r;
+            fy = mu * coeff * b.ry / b.r;
+            fz = mu * coeff * b.rz / b.r;
+        }
+        else
+        {
+            fx = coeff * b.rx / b.r;
+            fy = coeff * b.ry / b.r;
+            fz = coeff * b.rz / b.r;
+        }
+        s00 += fx * xi_x;
+        s11 += fy * xi_y;
+        s22 += fz * xi_z;
+        s01 += fx * xi_y;
+        s10 += fy * xi_x;
+        s02 += fx * xi_z;
+        s20 += fz * xi_x;
+        s12 += fy * xi_z;
+        s21 += fz * xi_y;
+    }
+    double* __restrict__ st = A.stress;
+    st[p] = s00;
+    st[n + p] = s01;
+    st[2 * n + p] = s02;
+    st[3 * n + p] = s10;
+    st[4 * n + p] = s11;
+    st[5 * n + p] = s12;
+    st[6 * n + p] = s20;
+    st[7 * n + p] = s21;
+    st[8 * n + p] = s22;
+}
+
 // Velocity-Verlet initial half step on its own (CabanaPD_Integrate.hpp:89-116):
 // the first step of a pdk_step call; later steps run it in the force epilogue.
 __global__ __launch_bounds__( 256 ) void kInitialHalfStep( const PDArgs* __restrict__ specs,
@@ -1669,6 +1770,28 @@ void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, doub
     hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( nblocks ) ), dim3( kBlock ), 0, st, args_dev, parity,
                         a.m, a.theta, partial_dev );
     PDK_HIP( hipGetLastError() );
+}
+
+void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st )
+{
+    if ( a.nslices == 0 )
+        return;
+    auto fn = a.model == kLinPMB ? kStress<kLinPMB, false>
+              : a.model == kLinLPS ? kStress<kLinLPS, false>
+              : a.model == kLPS    ? ( a.fracture ? kStress<kLPS, true> : kStress<kLPS, false> )
+                                   : ( a.fracture ? kStress<kPMB, true> : kStress<kPMB, false> );
+    hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, parity );
+    PDK_HIP( hipGetLastError() );
+}
+
+const char* stressKernelName( const PDArgs& a )
+{
+    if ( a.model == kLinPMB )
+        return "kStress<4, false>";
+    if ( a.model == kLinLPS )
+        return "kStress<5, false>";
+    return a.model == kLPS ? ( a.fracture ? "kStress<1, true>" : "kStress<1, false>" )
+                           : ( a.fracture ? "kStress<0, true>" : "kStress<0, false>" );
 }
 
 void launchInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st )

@@ -211,6 +211,12 @@ void launchLpsState( const PDArgs* args_dev, const PDArgs& a, bool weighted_volu
 void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, double* partial_dev,
                    int64_t nblocks, hipStream_t st );
 int64_t energyBlocks( const PDArgs& a );
+// Stress tensor pass into a.stress ([9][n], device order).
+void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
+const char* stressKernelName( const PDArgs& a );
+// Scatter a [ncomp][n] device-order array to a caller-order AoS [n][ncomp].
+void launchScatterRows( const double* src_soa, const int64_t* perm, double* dst_aos, int64_t n, int32_t ncomp,
+                        hipStream_t st );
 void launchInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
 void launchApplyForceTerms( const PDArgs* args_dev, const PDArgs& a, hipStream_t st );
 // Packed launches over several specs of one kernel variant (the variant is

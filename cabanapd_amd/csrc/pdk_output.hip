@@ -135,6 +135,17 @@ __global__ void kGatherQuantity( const int64_t* __restrict__ idx, int64_t count,
         out[k] = quantityAt( Q, idx[k] );
 }
 
+__global__ void kScatterRows( const double* __restrict__ src, const int64_t* __restrict__ perm,
+                              double* __restrict__ dst, int64_t n, int32_t ncomp )
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( p >= n )
+        return;
+    const int64_t c = perm[p];
+    for ( int32_t k = 0; k < ncomp; k++ )
+        dst[ncomp * c + k] = src[k * n + p];
+}
+
 unsigned gridOf( int64_t n )
 {
     return static_cast<unsigned>( ( n + kOutThreads - 1 ) / kOutThreads );
@@ -163,6 +174,15 @@ void launchSeriesReduce( const int64_t* idx_dev, int64_t count, const QuantityAr
     kSeriesPartial<<<nparts, kOutThreads, 0, st>>>( idx_dev, count, Q, red, partial_dev );
     PDK_HIP( hipGetLastError() );
     kSeriesFinal<<<1, kOutThreads, 0, st>>>( partial_dev, nparts, red, out_dev );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchScatterRows( const double* src_soa, const int64_t* perm, double* dst_aos, int64_t n, int32_t ncomp,
+                        hipStream_t st )
+{
+    if ( n == 0 )
+        return;
+    kScatterRows<<<gridOf( n ), kOutThreads, 0, st>>>( src_soa, perm, dst_aos, n, ncomp );
     PDK_HIP( hipGetLastError() );
 }
 

@@ -68,6 +68,7 @@ _SIGS = {
     "pdk_series_values": (C.c_int, [_VP, C.c_int64, _D, _D, _I64]),
     "pdk_series_gather": (C.c_int, [_VP, C.c_int64, _I64, _D, _I64]),
     "pdk_compute_energy_async": (C.c_int, [_VP]),
+    "pdk_compute_stress": (C.c_int, [_VP]),
 }
 
 
@@ -249,6 +250,9 @@ class Spec:
     def step(self, nsteps=1):
         _check(lib().pdk_step(self.h, nsteps))
 
+    def compute_stress(self):
+        _check(lib().pdk_compute_stress(self.h))
+
     def compute_energy_async(self):
         _check(lib().pdk_compute_energy_async(self.h))
 
@@ -264,7 +268,7 @@ class Spec:
         return s.value
 
     def get(self, field):
-        shape = (self.n, 3) if field in VECTOR_FIELDS else (self.n,)
+        shape = _abi.field_shape(field, self.n)
         out = np.empty(shape, dtype=np.float64)
         _check(lib().pdk_download(self.h, FIELD[field], _dp(out)))
         return out

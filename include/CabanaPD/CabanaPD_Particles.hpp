@@ -13,6 +13,7 @@
 #include <cmath>
 #include <memory>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 #include "CabanaPD_Input.hpp"
@@ -32,10 +33,19 @@ struct Slice
     std::size_t extent( int ) const { return n; }
 };
 
+// Stress view: s(i, a, b) over double[n][3][3].
+struct StressSlice
+{
+    double* data = nullptr;
+    std::size_t n = 0;
+    double& operator()( const std::size_t i, const int a, const int b ) const { return data[9 * i + 3 * a + b]; }
+    std::size_t size() const { return n; }
+};
+
 struct ParticleStorage
 {
     std::size_t n = 0;
-    std::vector<double> x, u, v, f, vol, rho, W, phi, m, theta, y;
+    std::vector<double> x, u, v, f, vol, rho, W, phi, m, theta, y, stress;
     std::vector<int> nofail, type;
     void resize( std::size_t nn )
     {
@@ -44,6 +54,7 @@ struct ParticleStorage
             a->assign( 3 * nn, 0.0 );
         for ( auto* a : { &vol, &W, &phi, &m, &theta } )
             a->assign( nn, 0.0 );
+        stress.assign( 9 * nn, 0.0 );
         rho.assign( nn, 1.0 );
         nofail.assign( nn, 0 );
         type.assign( nn, 0 );
@@ -187,6 +198,8 @@ class Particles
     Slice<double, 1> sliceDamage() const { return { _s->phi.data(), _s->n }; }
     Slice<double, 1> sliceWeightedVolume() const { return { _s->m.data(), _s->n }; }
     Slice<double, 1> sliceDilatation() const { return { _s->theta.data(), _s->n }; }
+    // Particles<..., EnergyStressOutput> (CabanaPD_Particles.hpp:1233-1356)
+    StressSlice sliceStress() const { return { _s->stress.data(), _s->n }; }
     Slice<int, 1> sliceNoFail() const { return { _s->nofail.data(), _s->n }; }
     Slice<int, 1> sliceType() const { return { _s->type.data(), _s->n }; }
 
@@ -235,9 +248,26 @@ class Particles
     bool _host_newer = true;
 };
 
-// Deduction: Particles( memory_space{}, model_tag{} [, tags...] )
+namespace detail
+{
+// Output type named among the constructor tags (EnergyOutput when none is).
+template <class... Tags>
+struct OutputOf
+{
+    using type = EnergyOutput;
+};
+template <class T, class... Rest>
+struct OutputOf<T, Rest...>
+{
+    using type = std::conditional_t<is_output<T>::value, T, typename OutputOf<Rest...>::type>;
+};
+} // namespace detail
+
+// Deduction (CabanaPD_Particles.hpp:1364-1395): Particles( memory_space{},
+// model{} [, thermal] [, output] ), energy output by default.
 template <class MemorySpace, class ModelType, class... Tags>
-Particles( MemorySpace, ModelType, Tags... ) -> Particles<MemorySpace, ModelType>;
+Particles( MemorySpace, ModelType, Tags... )
+    -> Particles<MemorySpace, ModelType, TemperatureIndependent, typename detail::OutputOf<Tags...>::type>;
 
 } // namespace CabanaPD
 

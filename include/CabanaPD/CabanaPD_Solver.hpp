@@ -49,6 +49,7 @@ struct DeviceSpec
     pdk_spec spec = nullptr;
     bool lps = false;
     bool fracture = false;
+    bool stress = false; // a stress pass has run
     DeviceSpec() = default;
     DeviceSpec( const DeviceSpec& ) = delete;
     DeviceSpec& operator=( const DeviceSpec& ) = delete;
@@ -83,8 +84,11 @@ struct DeviceSpec
             case PDK_FIELD_PHI: dst = s.phi.data(); break;
             case PDK_FIELD_M: dst = s.m.data(); break;
             case PDK_FIELD_THETA: dst = s.theta.data(); break;
+            case PDK_FIELD_STRESS: dst = s.stress.data(); break;
             default: continue;
             }
+            if ( f == PDK_FIELD_STRESS && !stress )
+                continue;
             if ( ( f == PDK_FIELD_M || f == PDK_FIELD_THETA ) && !lps )
                 continue;
             check( pdk_download( spec, f, dst ) );
@@ -93,7 +97,13 @@ struct DeviceSpec
     void pullState( ParticleStorage& s )
     {
         pull( s, { PDK_FIELD_U, PDK_FIELD_V, PDK_FIELD_F, PDK_FIELD_W, PDK_FIELD_PHI, PDK_FIELD_M,
-                   PDK_FIELD_THETA } );
+                   PDK_FIELD_THETA, PDK_FIELD_STRESS } );
+    }
+    // Stress pass (computeStress); pulled from then on.
+    void computeStress()
+    {
+        check( pdk_compute_stress( spec ) );
+        stress = true;
     }
 };
 } // namespace detail
@@ -167,6 +177,12 @@ class Force
         neighbor.device().pull( particles.storage(), { PDK_FIELD_F } );
     }
     template <class ModelType, class ParticleType, class NeighborType>
+    void computeStressFull( const ModelType&, ParticleType& particles, const NeighborType& neighbor )
+    {
+        neighbor.device().computeStress();
+        neighbor.device().pull( particles.storage(), { PDK_FIELD_STRESS } );
+    }
+    template <class ModelType, class ParticleType, class NeighborType>
     void computeEnergyFull( const ModelType&, ParticleType& particles, const NeighborType& neighbor )
     {
         detail::check( pdk_compute_energy( neighbor.spec(), &_total_strain_energy, &_total_damage ) );
@@ -195,6 +211,14 @@ void computeEnergy( const ModelType& model, ForceType& force, ParticleType& part
 {
     if constexpr ( is_energy_output<typename ParticleType::output_type>::value )
         force.computeEnergyFull( model, particles, neighbor );
+}
+
+// computeStress (CabanaPD_Force.hpp:221-236): only for particles with stress output.
+template <class ModelType, class ForceType, class ParticleType, class NeighborType>
+void computeStress( const ModelType& model, ForceType& force, ParticleType& particles, const NeighborType& neighbor )
+{
+    if constexpr ( is_stress_output<typename ParticleType::output_type>::value )
+        force.computeStressFull( model, particles, neighbor );
 }
 
 // ---- Solver (CabanaPD_Solver.hpp:87-657) ---------------------------------------
@@ -236,6 +260,8 @@ class Solver
     void init( const bool initial_output = true )
     {
         detail::check( pdk_init( spec() ) );
+        if constexpr ( is_stress_output<typename ParticleType::output_type>::value )
+            neighbor->device().computeStress();
         pullAll();
         (void)initial_output;
     }
@@ -326,6 +352,8 @@ class Solver
         {
             double W = 0.0, D = 0.0;
             detail::check( pdk_compute_energy( spec(), &W, &D ) );
+            if constexpr ( is_stress_output<typename ParticleType::output_type>::value )
+                neighbor->device().computeStress();
             pullAll();
             _last_energy = W;
             _last_damage = D;

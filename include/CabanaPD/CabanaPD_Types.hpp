@@ -129,6 +129,16 @@ struct is_energy_output
     : std::integral_constant<bool, std::is_same_v<T, EnergyOutput> || std::is_same_v<T, EnergyStressOutput>>
 {
 };
+template <class T>
+struct is_stress_output : std::is_same<T, EnergyStressOutput>
+{
+};
+template <class T>
+struct is_output
+    : std::integral_constant<bool, std::is_same_v<T, BaseOutput> || std::is_same_v<T, EnergyOutput> ||
+                                       std::is_same_v<T, EnergyStressOutput>>
+{
+};
 
 } // namespace CabanaPD
 

@@ -62,7 +62,9 @@ enum pdk_field {
     PDK_FIELD_PHI = 7,    /* damage              double[n] */
     PDK_FIELD_M = 8,      /* LPS weighted volume double[n] */
     PDK_FIELD_THETA = 9,  /* LPS dilatation      double[n] */
-    PDK_FIELD_COUNT = 10
+    PDK_FIELD_STRESS = 10, /* stress tensor      double[n][3][3], stress(i, a, b) at [9 i + 3 a + b];
+                              the value of the last pdk_compute_stress */
+    PDK_FIELD_COUNT = 11
 };
 
 enum pdk_bc_kind {
@@ -187,6 +189,12 @@ int pdk_compute_weighted_volume(pdk_spec spec);   /* Force<LPS>::computeWeighted
 int pdk_compute_dilatation(pdk_spec spec);        /* Force<LPS>::computeDilatation; no-op for PMB */
 int pdk_compute_force(pdk_spec spec, int32_t reset);  /* computeForce(model, force, particles, neighbor, reset) */
 int pdk_compute_energy(pdk_spec spec, double* total_strain_energy, double* total_damage);  /* computeEnergy */
+/* computeStress (CabanaPD_Force.hpp:221-236): reset, then per bond
+ * stress(i, a, b) += f_a xi_b with f half the bond force
+ * (Force<*>::computeStressFull, force/CabanaPD_PMB.hpp:157-202, 341-401, 492-536;
+ * force/CabanaPD_LPS.hpp:221-267, 506-572, 677-723).  FP64 in every precision
+ * (an output pass); PD models only. */
+int pdk_compute_stress(pdk_spec spec);
 
 /* ---- time stepping (Solver::init / runStep, CabanaPD_Solver.hpp:229-404) ----
  * pdk_init: Solver::init(bc) order: weighted volume (LPS NoFracture), force,

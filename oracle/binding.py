@@ -40,6 +40,7 @@ _SIGS = {
     "orc_compute_dilatation": (C.c_int, [_VP]),
     "orc_compute_force": (C.c_int, [_VP, C.c_int32]),
     "orc_compute_energy": (C.c_int, [_VP, _D, _D]),
+    "orc_compute_stress": (C.c_int, [_VP]),
     "orc_init": (C.c_int, [_VP]),
     "orc_step": (C.c_int, [_VP, C.c_int32]),
     "orc_get": (C.c_int, [_VP, C.c_int32, _D]),
@@ -210,6 +211,9 @@ class OracleSpec:
         _check(lib().orc_compute_energy(self.h, C.byref(W), C.byref(D)))
         return W.value, D.value
 
+    def compute_stress(self):
+        _check(lib().orc_compute_stress(self.h))
+
     def init(self):
         _check(lib().orc_init(self.h))
 
@@ -220,7 +224,7 @@ class OracleSpec:
         _check(lib().orc_set_state(self.h, _dp(_f64(u, (self.n, 3))), _dp(_f64(v, (self.n, 3)))))
 
     def get(self, field):
-        shape = (self.n, 3) if field in VECTOR_FIELDS else (self.n,)
+        shape = _abi.field_shape(field, self.n)
         out = np.empty(shape, dtype=np.float64)
         _check(lib().orc_get(self.h, FIELD[field], _dp(out)))
         return out

@@ -51,7 +51,7 @@ struct Sys
     pdk_model_desc d;
     pdk_model_constants k;
     int64_t n = 0;
-    std::vector<double> x, u, v, f, vol, rho, W, phi, m, theta, radius, u_neigh;
+    std::vector<double> x, u, v, f, vol, rho, W, phi, m, theta, radius, u_neigh, stress;
     std::vector<int32_t> nofail;
     // VerletLayout2D-style table: nbr[i * maxn + slot], cnt[i]
     int32_t maxn = 0;
@@ -611,6 +611,68 @@ void computeEnergyFull( Sys& s )
     s.total_D = frac ? D : 0.0;
 }
 
+// Force<*>::computeStressFull (force/CabanaPD_PMB.hpp:157-202, 341-401,
+// 492-536; force/CabanaPD_LPS.hpp:221-267, 506-572, 677-723) after
+// computeStress's reset (CabanaPD_Force.hpp:221-236); stress(i, a, b) at
+// stress[9 i + 3 a + b].
+void computeStressFull( Sys& s )
+{
+    const bool frac = s.d.fracture != 0;
+    const bool lps = isLps( s.d );
+    const double c = s.k.c;
+    s.stress.assign( 9 * s.n, 0.0 );
+#pragma omp parallel for schedule( static )
+    for ( int64_t i = 0; i < s.n; i++ )
+    {
+        const int32_t* row = s.nbr.data() + i * s.maxn;
+        const uint8_t* mu = frac ? s.mu.data() + i * s.maxn : nullptr;
+        double* st9 = s.stress.data() + 9 * i;
+        for ( int32_t q = 0; q < s.cnt[i]; q++ )
+        {
+            const int64_t j = row[q];
+            // LPS fracture: unbroken bonds only (force/CabanaPD_LPS.hpp:535)
+            if ( lps && frac && !( mu[q] > 0 ) )
+                continue;
+            double xi, r, st, rx, ry, rz;
+            bondGeometry( s, i, j, xi, r, st, rx, ry, rz );
+            const double xi_x = s.x[3 * j + 0] - s.x[3 * i + 0];
+            const double xi_y = s.x[3 * j + 1] - s.x[3 * i + 1];
+            const double xi_z = s.x[3 * j + 2] - s.x[3 * i + 2];
+            double coeff;
+            if ( lps )
+            {
+                coeff = lpsForceCoeff( s, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
+                coeff *= 0.5;
+            }
+            else
+                coeff = 0.5 * ( c * st * s.vol[j] ); // 0.5 * model( ForceCoeffTag, ... )
+            double fx_i, fy_i, fz_i;
+            if ( frac )
+            {
+                const double muij = mu[q];
+                fx_i = muij * coeff * rx / r;
+                fy_i = muij * coeff * ry / r;
+                fz_i = muij * coeff * rz / r;
+            }
+            else
+            {
+                fx_i = coeff * rx / r;
+                fy_i = coeff * ry / r;
+                fz_i = coeff * rz / r;
+            }
+            st9[0] += fx_i * xi_x;
+            st9[4] += fy_i * xi_y;
+            st9[8] += fz_i * xi_z;
+            st9[1] += fx_i * xi_y;
+            st9[3] += fy_i * xi_x;
+            st9[2] += fx_i * xi_z;
+            st9[6] += fz_i * xi_x;
+            st9[5] += fy_i * xi_z;
+            st9[7] += fz_i * xi_y;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Contact (force/CabanaPD_Contact.hpp:109-172).
 // ---------------------------------------------------------------------------
@@ -1147,11 +1209,27 @@ int orc_step( orc_sys s, int32_t nsteps )
     return 0;
 }
 
+int orc_compute_stress( orc_sys s )
+{
+    if ( isContact( s->d ) || !s->built )
+        return 1;
+    computeStressFull( *s );
+    return 0;
+}
+
 int orc_get( orc_sys s, int32_t field, double* out )
 {
     const std::vector<double>* src = nullptr;
     switch ( field )
     {
+    case PDK_FIELD_STRESS:
+        if ( s->stress.empty() )
+        {
+            std::fill( out, out + 9 * s->n, 0.0 );
+            return 0;
+        }
+        src = &s->stress;
+        break;
     case PDK_FIELD_X: src = &s->x; break;
     case PDK_FIELD_U: src = &s->u; break;
     case PDK_FIELD_V: src = &s->v; break;

@@ -1,8 +1,8 @@
 // Force/energy known answers through the drop-in API, in the shape of the
 // reference's unit_test/tstForce.hpp: Particles on [-1,1]^3, u = s0 x,
-// Force + Neighbor objects, computeForce / computeEnergy, checks against the
-// single-particle lattice sums (tstForce.hpp:128-155, 238-287, 386-422) on
-// interior particles.
+// Force + Neighbor objects, computeForce / computeEnergy / computeStress,
+// checks against the single-particle lattice sums (tstForce.hpp:91-155,
+// 238-287, 345-422) on interior particles.
 #include <cmath>
 #include <cstdio>
 
@@ -62,6 +62,24 @@ double latticeW( const Model& model, int m, double s0, bool lps, double K, doubl
     return W;
 }
 
+// Diagonal stress of the PMB lattice sum (tstForce.hpp:91-126); LPS gives
+// exactly 3 K s0 on a full neighborhood.
+double latticeStressPMB( double h, int m, double s0, double K )
+{
+    const double dx = h / m, vol = dx * dx * dx;
+    const double c = 18.0 * K / ( M_PI * h * h * h * h );
+    double sxx = 0.0;
+    for ( int i = -m; i <= m; i++ )
+        for ( int j = -m; j <= m; j++ )
+            for ( int k = -m; k <= m; k++ )
+            {
+                double xi = std::sqrt( double( i * i + j * j + k * k ) ) * dx;
+                if ( xi > 0.0 && xi < h + 1e-14 )
+                    sxx += 0.5 * ( c * s0 * vol * ( i * dx ) / xi ) * ( i * dx );
+            }
+    return sxx;
+}
+
 template <class Model, class Tag>
 void run( const Model& model, Tag, int nc, double bw, bool lps, double K, double G )
 {
@@ -69,7 +87,8 @@ void run( const Model& model, Tag, int nc, double bw, bool lps, double K, double
     using memory_space = exec_space::memory_space;
     const int m = 3;
     const double s0 = 0.1;
-    CabanaPD::Particles particles( memory_space{}, Tag{}, CabanaPD::EnergyOutput{} );
+    CabanaPD::Particles particles( memory_space{}, Tag{}, CabanaPD::EnergyStressOutput{} );
+    static_assert( CabanaPD::is_stress_output<typename decltype( particles )::output_type>::value );
     particles.domain( { -1.0, -1.0, -1.0 }, { 1.0, 1.0, 1.0 }, { nc, nc, nc }, 0 );
     particles.create( exec_space{} );
     auto x = particles.sliceReferencePosition();
@@ -84,13 +103,16 @@ void run( const Model& model, Tag, int nc, double bw, bool lps, double K, double
     force.computeDilatation( model, particles, neighbor );
     CabanaPD::computeForce( model, force, particles, neighbor );
     CabanaPD::computeEnergy( model, force, particles, neighbor );
+    CabanaPD::computeStress( model, force, particles, neighbor );
 
     auto f = particles.sliceForce();
+    auto sigma = particles.sliceStress();
     auto W = particles.sliceStrainEnergy();
     auto vol = particles.sliceVolume();
     auto theta = particles.sliceDilatation();
     const double h = model.force_horizon;
     const double refW = latticeW( model, m, s0, lps, K, G );
+    const double refS = lps ? 3.0 * K * s0 : latticeStressPMB( model.force_horizon, m, s0, K );
     double Phi = 0.0;
     int checked = 0;
     for ( std::size_t p = 0; p < particles.numLocal(); p++ )
@@ -105,6 +127,16 @@ void run( const Model& model, Tag, int nc, double bw, bool lps, double K, double
             expect( std::fabs( W( p ) - refW ) <= 1e-6 * std::fabs( refW ), "W lattice sum", W( p ), refW );
             if ( lps )
                 expect( std::fabs( theta( p ) - 3 * s0 ) <= 1e-6, "theta = 3 s0", theta( p ), 3 * s0 );
+            for ( int a = 0; a < 3; a++ )
+                for ( int b = 0; b < 3; b++ )
+                {
+                    if ( a == b )
+                        expect( std::fabs( sigma( p, a, b ) - refS ) <= 1e-6 * std::fabs( refS ), "stress diagonal",
+                                sigma( p, a, b ), refS );
+                    else
+                        expect( std::fabs( sigma( p, a, b ) ) <= 1e-13, "stress off-diagonal", sigma( p, a, b ),
+                                0.0 );
+                }
             checked++;
         }
         Phi += W( p ) * vol( p );

@@ -32,6 +32,16 @@ def ref_neighbors(horizon, m):
     return offsets(m, dx, horizon)[0].shape[0]
 
 
+def pmb_stress_linear(K, horizon, m, s0):
+    """computeReferenceStress, LinearTag, PMB (tstForce.hpp:91-126): diagonal
+    of the stress tensor, sum over the lattice neighborhood of 0.5 f_a xi_a."""
+    dx = horizon / m
+    vol = dx ** 3
+    xiv, xi = offsets(m, dx, horizon)
+    f_mag = pmb_c(K, horizon) * s0 * vol
+    return [float(np.sum(0.5 * (f_mag * xiv[:, d] / xi) * xiv[:, d])) for d in range(3)]
+
+
 def pmb_W_linear(K, horizon, m, s0):
     """computeReferenceStrainEnergyDensity, LinearTag, PMB (tstForce.hpp:128-155)."""
     dx = horizon / m
@@ -145,3 +155,16 @@ def lps_fx_quadratic(K, G, horizon, m, u11, x, influence_type):
         coeff = (tc * theta_i / wv + tc * theta_j / wv + s[b] * (sc / wv + sc / wv)) * inf_b * xi[b] * vol
         fx += coeff * rx[b] / rr[b]
     return fx
+
+
+def lps_stress_linear(K, G, horizon, m, s0, influence_type):
+    """computeReferenceStress, LinearTag, LPS (tstForce.hpp:345-379): constant
+    weighted volume and dilatation on the full lattice neighborhood."""
+    dx = horizon / m
+    vol = dx ** 3
+    wv = lps_weighted_volume(horizon, m, vol, influence_type)
+    theta = lps_theta_linear(horizon, m, s0, vol, wv, influence_type)
+    tc, sc = lps_coeffs(K, G)
+    xiv, xi = offsets(m, dx, horizon)
+    f_mag = (tc * theta / wv + tc * theta / wv + s0 * (sc / wv + sc / wv)) * influence(xi, influence_type) * xi * vol
+    return [float(np.sum(0.5 * (f_mag * xiv[:, d] / xi) * xiv[:, d])) for d in range(3)]

@@ -98,8 +98,11 @@ def test_gpu_lattice_known_answers(model, field, fracture):
         s.compute_force(True)
     Wg, _ = g.compute_energy()
     Wo, _ = o.compute_energy()
+    g.compute_stress()
+    o.compute_stress()
     assert_same(g.get("f"), o.get("f"), "f")
     assert_same(g.get("W"), o.get("W"), "W")
+    assert_same(g.get("stress"), o.get("stress"), "stress")
     if model == "LPS":
         assert_same(g.get("m"), o.get("m"), "m")
         assert_same(g.get("theta"), o.get("theta"), "theta")
@@ -136,6 +139,9 @@ def test_gpu_pmb_fracture_plate_small():
     assert bg == bo
     for fld in ("u", "v"):
         assert_same(g.get(fld), o.get(fld), fld)
+    g.compute_stress()
+    o.compute_stress()
+    assert_same(g.get("stress"), o.get("stress"), "stress (broken bonds contribute mu = 0)")
 
 
 @pytest.mark.parametrize("fracture", [False, True])
@@ -156,6 +162,9 @@ def test_gpu_lps_cube_small(fracture):
     if fracture:
         assert_same(g.get("phi"), o.get("phi"), "phi")
         assert_total(Dg, Do, "total damage")
+    g.compute_stress()
+    o.compute_stress()
+    assert_same(g.get("stress"), o.get("stress"), "stress")
 
 
 def test_gpu_device_order_is_transparent():
@@ -264,6 +273,9 @@ def test_gpu_linearized_models_bitwise(model):
     Wo, _ = o.compute_energy()
     assert_same(g.get("W"), o.get("W"), "W")
     assert_total(Wg, Wo, "total strain energy")
+    g.compute_stress()
+    o.compute_stress()
+    assert_same(g.get("stress"), o.get("stress"), "stress")
     # mixed form of the same run
     from cabanapd_amd import engine
     from tests.parity import ctx
@@ -276,6 +288,10 @@ def test_gpu_linearized_models_bitwise(model):
     e = rel_err(mx.get("u"), g.get("u"))
     say(f"  mixed vs FP64 u rel err {e:.3e} ({mx.kernel_name(0)})")
     assert e <= 1e-5
+    mx.compute_stress()
+    es = rel_err(mx.get("stress"), g.get("stress"))
+    say(f"  mixed vs FP64 stress rel err {es:.3e}")
+    assert es <= 1e-4
     mx.close()
 
 
