@@ -83,8 +83,43 @@ __device__ __forceinline__ bool criticalStretch64( double r, double xi, double b
 struct Bond32
 {
     float s, rinv, rx, ry, rz, xi;
-    float etan; // |eta| / |xi|, for the guard band
+    float etan;       // |eta|_1 / |xi|, for the guard band
+    float ex, ey, ez; // eta in FP32
 };
+
+// Guard band of the mixed form: true when the FP32 stretch s lies within
+// bandf * (|eta|_1 / |xi| + |s| + s0) of s0, evaluated as one FMA chain with
+// bx = bandf / |xi| and bs0 = bandf * s0 per slot / per launch.  The band is
+// 2^-13 relative, far above the FP32 error of s and of this evaluation.
+__device__ __forceinline__ bool inGuardBand( const Bond32& b, float s0f, float bandf, float bx, float bs0 )
+{
+    float t = fmaf( -bandf, fabsf( b.s ), fabsf( b.s - s0f ) );
+    t = fmaf( -bx, fabsf( b.ex ), t );
+    t = fmaf( -bx, fabsf( b.ey ), t );
+    t = fmaf( -bx, fabsf( b.ez ), t );
+    return t <= bs0;
+}
+
+// Wave-uniform read through the scalar data cache (p must be the same in
+// every lane).  Used for the stencil tables, which no kernel writes, and for
+// the per-slot bond-mask words, which a wave reads before it writes them (a
+// vector store) and which no other wave of the launch touches; the scalar
+// cache starts every kernel dispatch invalidated.
+template <class T>
+__device__ __forceinline__ T scalarLoad( const T* p )
+{
+    return *( const __attribute__( ( address_space( 4 ) ) ) T* )( p );
+}
+__device__ __forceinline__ float4 scalarLoad4( const float4* p )
+{
+    const auto* f = ( const __attribute__( ( address_space( 4 ) ) ) float* )( p );
+    return make_float4( f[0], f[1], f[2], f[3] );
+}
+__device__ __forceinline__ float2 scalarLoad2( const float2* p )
+{
+    const auto* f = ( const __attribute__( ( address_space( 4 ) ) ) float* )( p );
+    return make_float2( f[0], f[1] );
+}
 
 __device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi2, double ui0,
                                                double ui1, double ui2, double xj0, double xj1,
@@ -112,6 +147,9 @@ __device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi
     b.s = num * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
     // guard-band scale: (|ex| + |ey| + |ez|) / |xi| >= |eta| / |xi|
     b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
+    b.ex = ex;
+    b.ey = ey;
+    b.ez = ez;
 }
 
 // Same stretch with the bond vector from the lattice stencil table
@@ -133,6 +171,9 @@ __device__ __forceinline__ void getDistance32s( const float4 g, float xinv, doub
     const float ee = ex * ex + ey * ey + ez * ez;
     b.s = ( 2.0f * xe + ee ) * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
     b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
+    b.ex = ex;
+    b.ey = ey;
+    b.ez = ez;
 }
 
 // getLinearizedDistance (CabanaPD_Force.hpp:114-138), operation for
@@ -173,6 +214,9 @@ __device__ __forceinline__ void linearized32( float gx, float gy, float gz, floa
     b.rinv = xinv;
     b.s = ( gx * ex + gy * ey + gz * ez ) * ( xinv * xinv );
     b.etan = 0.0f;
+    b.ex = ex;
+    b.ey = ey;
+    b.ez = ez;
 }
 
 // Bond geometry of the force and energy passes for the model.
@@ -227,7 +271,9 @@ __device__ __forceinline__ bool waveSetup( const PDArgs* __restrict__ specs,
                                            const int64_t* __restrict__ tile_slice, int32_t epi,
                                            int32_t parity, int64_t launch, WaveCtx& w )
 {
-    const int wave = threadIdx.x / kSlice;
+    // uniform per wavefront; readfirstlane lets the compiler keep it (and the
+    // addresses derived from it) in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
     const int64_t t = xcdRemap( blockIdx.x, gridDim.x );
     if ( tile_spec == nullptr )
     {
@@ -238,8 +284,9 @@ __device__ __forceinline__ bool waveSetup( const PDArgs* __restrict__ specs,
     }
     else
     {
-        w.A = specs + tile_spec[t];
-        w.slice = tile_slice[t] + wave;
+        // the tile tables are written by the host before the launch
+        w.A = specs + scalarLoad( tile_spec + t );
+        w.slice = scalarLoad( tile_slice + t ) + wave;
         w.cur = parity ^ w.A->parity_offset;
         w.epi = ( launch + 1 < w.A->end_launch ) ? kEpiFinalInitial : kEpiFinal;
     }
@@ -682,19 +729,11 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     constexpr int KNF = KV + 1;                     // staged no-fail flag (GEN)
     constexpr int NC = KV + ( GEN ? 2 : 0 );
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
-    __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
-    __shared__ float sinv[( PREC == kMixed ) ? 256 : 1];
     __shared__ int4 rtab[kMaxRows];
     WaveCtx w;
     const bool active = waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w );
-    // All waves of a block belong to one spec: stage its tables before any
+    // All waves of a block belong to one spec: stage its row table before any
     // wave leaves.
-    if constexpr ( PREC == kMixed )
-    {
-        const float4 g = w.A->stencil[threadIdx.x];
-        stab[threadIdx.x] = g;
-        sinv[threadIdx.x] = __builtin_amdgcn_rcpf( g.w );
-    }
     if ( static_cast<int>( threadIdx.x ) < w.A->nrows )
         rtab[threadIdx.x] = w.A->rows[threadIdx.x];
     __syncthreads();
@@ -712,12 +751,15 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const double* __restrict__ u = A.U[w.cur];
     const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
     const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
-    const int32_t nx = A.lat_n[0], ny = A.lat_n[1], nz = A.lat_n[2];
-    const int64_t t1 = pc / nx;
-    const int32_t ix = static_cast<int32_t>( pc - t1 * nx );
-    const int32_t iy = static_cast<int32_t>( t1 % ny );
-    const int32_t iz = static_cast<int32_t>( t1 / ny );
-    uint64_t* __restrict__ bits = A.intact + A.slice_ptr[w.slice];
+    // Per-slot lane masks of the slice: the intact words (a subset of the
+    // existence words, so they alone gate the fracture forms) or, without
+    // fracture, the existence words.  Lanes past n hold no bonds.
+    // (the argument block and the slice table are written by the host only)
+    const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + w.slice );
+    uint64_t* __restrict__ bits = scalarLoad( &A.intact ) + row0;
+    const uint64_t* __restrict__ mwords = FRAC ? bits : scalarLoad( &A.exist ) + row0;
+    const float4* __restrict__ sten = scalarLoad( &A.stencil );
+    const float2* __restrict__ saux = scalarLoad( &A.stencil_aux );
     const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
     // Loop-invariant scalars held in registers (the args block lives in
     // global memory, which the bit-word stores could alias).
@@ -729,6 +771,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const double volu = A.vol_value;
     const float s0f = A.s0f;
     const float bandf = A.bandf;
+    const float bs0 = bandf * s0f;
     const int infl1 = A.influence;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( kIsLps<MODEL> )
@@ -763,7 +806,6 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const double* __restrict__ volp = A.vol;
     const uint8_t* __restrict__ nfp = A.nofail;
     double v0[NC], v1[NC];
-    uint32_t blo = 0, bhi = 0; // intact words of the row's slots: lane t holds slot q0 + t
     // Row tables exist only for n < 2^29, so ids and byte offsets fit 32 bits.
     const int32_t n32 = static_cast<int32_t>( n );
     const int32_t p032 = static_cast<int32_t>( p0 );
@@ -790,12 +832,6 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
             v0[KNF] = ( FRAC && nfp[g0] != 0 ) ? 1.0 : 0.0;
             v1[KNF] = ( FRAC && second && nfp[g1] != 0 ) ? 1.0 : 0.0;
         }
-        if constexpr ( FRAC )
-        {
-            const uint64_t wd = ( lane < R.y ) ? bits[R.x + lane] : 0ull;
-            blo = static_cast<uint32_t>( wd );
-            bhi = static_cast<uint32_t>( wd >> 32 );
-        }
     };
 
     const int nrows = A.nrows;
@@ -811,7 +847,6 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
             if ( lane < R.y - 1 )
                 wb[k * kW + kSlice + lane] = v1[k];
         }
-        const uint32_t rlo = blo, rhi = bhi;
         // LDS operations of one wavefront complete in order; the fences keep
         // the compiler from moving the window reads above the stores.
         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
@@ -819,25 +854,13 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
         if ( r + 1 < nrows )
             loadRow( r + 1 );
-        const int32_t dx0 = ( R.w & 0xff ) - 128;
-        const int32_t dy = ( ( R.w >> 8 ) & 0xff ) - 128;
-        const int32_t dz = ( ( R.w >> 16 ) & 0xff ) - 128;
-        const bool rowok = valid && static_cast<uint32_t>( iy + dy ) < static_cast<uint32_t>( ny ) &&
-                           static_cast<uint32_t>( iz + dz ) < static_cast<uint32_t>( nz );
         float px = 0.0f, py = 0.0f, pz = 0.0f; // FP32 sum of this row's bond forces (mixed form)
         for ( int t = 0; t < R.y; t++ )
         {
             const int32_t q = R.x + t;
-            const bool ex = rowok && static_cast<uint32_t>( ix + dx0 + t ) < static_cast<uint32_t>( nx );
-            uint64_t ww = 0;
-            bool intact = true;
-            if constexpr ( FRAC )
-            {
-                const uint32_t l32 = __builtin_amdgcn_readlane( rlo, t );
-                const uint32_t h32 = __builtin_amdgcn_readlane( rhi, t );
-                ww = ( static_cast<uint64_t>( h32 ) << 32 ) | l32;
-                intact = __builtin_amdgcn_inverse_ballot_w64( ww );
-            }
+            const uint64_t ww = scalarLoad( mwords + q );
+            const bool ex = __builtin_amdgcn_inverse_ballot_w64( ww ); // intact (fracture) or exists
+            const bool intact = ex;
             // The window always holds readable values; bonds that do not
             // exist are masked out of the sums below.
             const double* __restrict__ wr = wb + lane + t;
@@ -879,16 +902,16 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
             }
             else
             {
-                const float4 g = stab[q];
-                const float xinv = sinv[q];
+                const float4 g = scalarLoad4( sten + q );
+                const float2 ga = scalarLoad2( saux + q ); // (1/|xi|, bandf/|xi|)
+                const float xinv = ga.x;
                 Bond32 b;
                 bond32s<MODEL>( g, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
                 if constexpr ( FRAC )
                 {
-                    const bool cand = ex && !nf_i && !nf_j;
-                    const float band = bandf * ( b.etan + fabsf( b.s ) + s0f );
+                    const bool cand = intact && !nf_i && !nf_j;
                     brk = cand && ( b.s > s0f );
-                    if ( cand && fabsf( b.s - s0f ) <= band )
+                    if ( cand && inGuardBand( b, s0f, bandf, ga.y, bs0 ) )
                     {
                         // guard band: the reference's FP64 test on this bond
                         const int64_t j = p + R.z + t;
@@ -908,7 +931,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                             b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kW] );
                     coeff = coeff * infl * b.xi * vj32;
                 }
-                const bool live = ex && intact && !brk;
+                const bool live = intact && !brk;
                 const float fr = live ? coeff * b.rinv : 0.0f;
                 px = fmaf( fr, b.rx, px );
                 py = fmaf( fr, b.ry, py );
@@ -916,8 +939,8 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
             }
             if constexpr ( FRAC )
             {
-                const uint64_t m = __ballot( brk );
-                if ( ( m & ww ) && lane == 0 )
+                const uint64_t m = __ballot( brk ) & ww;
+                if ( m && lane == 0 )
                     bits[q] = ww & ~m;
             }
         }
@@ -1781,6 +1804,27 @@ void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipS
               : a.model == kLPS    ? ( a.fracture ? kStress<kLPS, true> : kStress<kLPS, false> )
                                    : ( a.fracture ? kStress<kPMB, true> : kStress<kPMB, false> );
     hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, parity );
+    PDK_HIP( hipGetLastError() );
+}
+
+namespace
+{
+__global__ void kStencilAux( const float4* __restrict__ table, int32_t S, float bandf, float2* __restrict__ aux )
+{
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( q < S )
+    {
+        const float xinv = __builtin_amdgcn_rcpf( table[q].w );
+        aux[q] = make_float2( xinv, bandf * xinv );
+    }
+}
+} // namespace
+
+void launchStencilAux( const float4* table, int32_t S, float bandf, float2* aux, hipStream_t st )
+{
+    if ( S <= 0 )
+        return;
+    hipLaunchKernelGGL( kStencilAux, dim3( ( S + 255 ) / 256 ), dim3( 256 ), 0, st, table, S, bandf, aux );
     PDK_HIP( hipGetLastError() );
 }
 
