@@ -328,10 +328,10 @@ struct pdk_spec_s
         {
             if ( !slot.aux )
             {
-                slot.aux.alloc( static_cast<size_t>( slot.slots ) * sizeof( float2 ), acct() );
-                launchStencilAux( slot.table.as<float4>(), slot.slots, h.bandf, slot.aux.as<float2>(), st );
+                slot.aux.alloc( static_cast<size_t>( slot.slots ) * sizeof( float4 ), acct() );
+                launchStencilAux( slot.table.as<float4>(), slot.slots, h.bandf, slot.aux.as<float4>(), st );
             }
-            h.stencil_aux = slot.aux.as<float2>();
+            h.stencil_aux = slot.aux.as<float4>();
         }
         PDK_HIP( hipMemcpyAsync( args.as<PDArgs>(), &h, sizeof( PDArgs ), hipMemcpyHostToDevice, st ) );
     }

@@ -115,11 +115,6 @@ __device__ __forceinline__ float4 scalarLoad4( const float4* p )
     const auto* f = ( const __attribute__( ( address_space( 4 ) ) ) float* )( p );
     return make_float4( f[0], f[1], f[2], f[3] );
 }
-__device__ __forceinline__ float2 scalarLoad2( const float2* p )
-{
-    const auto* f = ( const __attribute__( ( address_space( 4 ) ) ) float* )( p );
-    return make_float2( f[0], f[1] );
-}
 
 __device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi2, double ui0,
                                                double ui1, double ui2, double xj0, double xj1,
@@ -154,8 +149,8 @@ __device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi
 
 // Same stretch with the bond vector from the lattice stencil table
 // (g.xyz = FP32 xi, g.w = |xi|, xinv = rcp(|xi|)): no x_j gather.
-__device__ __forceinline__ void getDistance32s( const float4 g, float xinv, double ui0, double ui1, double ui2,
-                                                double uj0, double uj1, double uj2, Bond32& b )
+__device__ __forceinline__ void getDistance32s( const float4 g, float xinv, float xi2, double ui0, double ui1,
+                                                double ui2, double uj0, double uj1, double uj2, Bond32& b )
 {
     const float ex = static_cast<float>( uj0 - ui0 );
     const float ey = static_cast<float>( uj1 - ui1 );
@@ -163,13 +158,15 @@ __device__ __forceinline__ void getDistance32s( const float4 g, float xinv, doub
     b.rx = g.x + ex;
     b.ry = g.y + ey;
     b.rz = g.z + ez;
-    const float r2 = b.rx * b.rx + b.ry * b.ry + b.rz * b.rz;
-    b.rinv = __builtin_amdgcn_rsqf( r2 );
-    const float r = r2 * b.rinv;
     b.xi = g.w;
     const float xe = g.x * ex + g.y * ey + g.z * ez;
     const float ee = ex * ex + ey * ey + ez * ez;
-    b.s = ( 2.0f * xe + ee ) * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
+    const float num = 2.0f * xe + ee; // |xi + eta|^2 - |xi|^2
+    // |xi + eta|^2 from |xi|^2 = fl(g.w * g.w) and the numerator of s
+    const float r2 = xi2 + num;
+    b.rinv = __builtin_amdgcn_rsqf( r2 );
+    const float r = r2 * b.rinv;
+    b.s = num * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
     b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
     b.ex = ex;
     b.ey = ey;
@@ -248,13 +245,13 @@ __device__ __forceinline__ void bond32( double xi0, double xi1, double xi2, doub
         getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
 }
 template <int MODEL>
-__device__ __forceinline__ void bond32s( const float4 g, float xinv, double ui0, double ui1, double ui2, double uj0,
-                                         double uj1, double uj2, Bond32& b )
+__device__ __forceinline__ void bond32s( const float4 g, float xinv, float xi2, double ui0, double ui1, double ui2,
+                                         double uj0, double uj1, double uj2, Bond32& b )
 {
     if constexpr ( kIsLin<MODEL> )
         linearized32( g.x, g.y, g.z, g.w, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
     else
-        getDistance32s( g, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
+        getDistance32s( g, xinv, xi2, ui0, ui1, ui2, uj0, uj1, uj2, b );
 }
 
 // Uniform per-wave access to the spec of this wave's slice.
@@ -618,7 +615,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                         const uint32_t id =
                             SLOTS ? static_cast<uint32_t>( q ) : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
                         const float4 g = stab[id];
-                        bond32s<MODEL>( g, __builtin_amdgcn_rcpf( g.w ), ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2],
+                        bond32s<MODEL>( g, __builtin_amdgcn_rcpf( g.w ), g.w * g.w, ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2],
                                         b );
                     }
                     else
@@ -709,10 +706,11 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
 // contiguous ids g .. g+63+c-1 (g = first id of the slice + the row's offset):
 // the wave loads that window once with coalesced loads, stores it in its LDS
 // buffer, and slot q0+t of lane L reads window[L + t].  The window of the next
-// row is loaded while the current row is evaluated.  Bond existence follows
-// from the lattice coordinates (the build verified that it equals the stored
-// existence bits).  Evaluation order and arithmetic are kForce's, so results
-// are identical.
+// row is loaded while the current row is evaluated.  Per slot, one
+// wave-uniform 64-bit word (scalar load) gates the lanes: the intact word
+// under fracture (breaks and pre-notches only clear bits, so it is a subset
+// of the existence word) or the existence word.  Evaluation order and
+// arithmetic are kForce's, so results are identical.
 // ---------------------------------------------------------------------------
 template <int MODEL, bool FRAC, int PREC, bool GEN>
 __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict__ specs,
@@ -759,7 +757,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     uint64_t* __restrict__ bits = scalarLoad( &A.intact ) + row0;
     const uint64_t* __restrict__ mwords = FRAC ? bits : scalarLoad( &A.exist ) + row0;
     const float4* __restrict__ sten = scalarLoad( &A.stencil );
-    const float2* __restrict__ saux = scalarLoad( &A.stencil_aux );
+    const float4* __restrict__ saux = scalarLoad( &A.stencil_aux );
     const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
     // Loop-invariant scalars held in registers (the args block lives in
     // global memory, which the bit-word stores could alias).
@@ -773,6 +771,8 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     const float bandf = A.bandf;
     const float bs0 = bandf * s0f;
     const int infl1 = A.influence;
+    // lanes whose particle may break bonds (every lane of the wave is active here)
+    const uint64_t nfi_ok = FRAC ? __ballot( !nf_i ) : 0ull;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( kIsLps<MODEL> )
     {
@@ -858,33 +858,35 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
         for ( int t = 0; t < R.y; t++ )
         {
             const int32_t q = R.x + t;
+            // intact word (fracture) or existence word of the slot
             const uint64_t ww = scalarLoad( mwords + q );
-            const bool ex = __builtin_amdgcn_inverse_ballot_w64( ww ); // intact (fracture) or exists
-            const bool intact = ex;
             // The window always holds readable values; bonds that do not
-            // exist are masked out of the sums below.
+            // exist are masked out below.
             const double* __restrict__ wr = wb + lane + t;
             const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
-            bool nf_j = false;
-            if constexpr ( kIsLps<MODEL> )
-                nf_j = nf_i;
-            else if constexpr ( GEN && FRAC )
-                nf_j = wr[KNF * kW] != 0.0;
-            bool brk = false;
+            // candidates for the break test: intact, neither end no-fail
+            // (LPS tests nofail(i) twice, force/CabanaPD_LPS.hpp:417-418)
+            uint64_t cand = 0;
+            if constexpr ( FRAC )
+            {
+                cand = ww & nfi_ok;
+                if constexpr ( !kIsLps<MODEL> && GEN )
+                    cand &= __ballot( wr[KNF * kW] == 0.0 );
+            }
+            // Masks below are wave-uniform (ballots in converged code).
+            uint64_t brkm = 0; // bonds that break in this pass
             if constexpr ( PREC == kF64 )
             {
-                if ( ex )
+                bool brk = false;
+                if ( __builtin_amdgcn_inverse_ballot_w64( ww ) )
                 {
                     const double vj = GEN ? wr[KV * kW] : volu;
                     Bond64 b;
                     bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
                                    b );
                     if constexpr ( FRAC )
-                    {
-                        if ( criticalStretch64( b.r, b.xi, bcc ) && !nf_i && !nf_j )
-                            brk = true;
-                    }
-                    if ( !brk && intact )
+                        brk = criticalStretch64( b.r, b.xi, bcc ) && __builtin_amdgcn_inverse_ballot_w64( cand );
+                    if ( !brk )
                     {
                         double coeff;
                         if constexpr ( !kIsLps<MODEL> )
@@ -899,39 +901,49 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                         fz += coeff * b.rz / b.r;
                     }
                 }
+                if constexpr ( FRAC )
+                    brkm = __ballot( brk );
             }
             else
             {
                 const float4 g = scalarLoad4( sten + q );
-                const float2 ga = scalarLoad2( saux + q ); // (1/|xi|, bandf/|xi|)
-                const float xinv = ga.x;
+                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, bandf/|xi|, |xi|^2, 0)
                 Bond32 b;
-                bond32s<MODEL>( g, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
+                bond32s<MODEL>( g, ga.x, ga.z, ui0, ui1, ui2, uj0, uj1, uj2, b );
                 if constexpr ( FRAC )
                 {
-                    const bool cand = intact && !nf_i && !nf_j;
-                    brk = cand && ( b.s > s0f );
-                    if ( cand && inGuardBand( b, s0f, bandf, ga.y, bs0 ) )
+                    // FP32 decision, except inside the guard band where the
+                    // reference's FP64 test decides
+                    const uint64_t inb = __ballot( inGuardBand( b, s0f, bandf, ga.y, bs0 ) ) & cand;
+                    brkm = __ballot( b.s > s0f ) & cand & ~inb;
+                    if ( inb )
                     {
-                        // guard band: the reference's FP64 test on this bond
-                        const int64_t j = p + R.z + t;
-                        Bond64 e;
-                        getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], uj0, uj1, uj2, e );
-                        brk = criticalStretch64( e.r, e.xi, bcc );
+                        bool crit = false;
+                        if ( __builtin_amdgcn_inverse_ballot_w64( inb ) )
+                        {
+                            const int64_t j = p + R.z + t;
+                            Bond64 e;
+                            getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], uj0, uj1, uj2,
+                                           e );
+                            crit = criticalStretch64( e.r, e.xi, bcc );
+                        }
+                        brkm |= __ballot( crit ) & inb;
                     }
                 }
+                // dead lanes (absent, broken or breaking bonds) add exactly
+                // zero: their window values are finite, fr is selected to 0
+                const bool live = __builtin_amdgcn_inverse_ballot_w64( ww & ~brkm );
                 const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
                 float coeff;
                 if constexpr ( !kIsLps<MODEL> )
                     coeff = ( GEN ? c32 * vj32 : cv32u ) * b.s;
                 else
                 {
-                    const float infl = infl1 ? xinv : 1.0f;
+                    const float infl = infl1 ? ga.x : 1.0f;
                     coeff = static_cast<float>( Ai + wr[( 3 + NX ) * kW] ) +
                             b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kW] );
                     coeff = coeff * infl * b.xi * vj32;
                 }
-                const bool live = intact && !brk;
                 const float fr = live ? coeff * b.rinv : 0.0f;
                 px = fmaf( fr, b.rx, px );
                 py = fmaf( fr, b.ry, py );
@@ -939,9 +951,8 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
             }
             if constexpr ( FRAC )
             {
-                const uint64_t m = __ballot( brk ) & ww;
-                if ( m && lane == 0 )
-                    bits[q] = ww & ~m;
+                if ( brkm && lane == 0 )
+                    bits[q] = ww & ~brkm;
             }
         }
         if constexpr ( PREC == kMixed )
@@ -1070,7 +1081,8 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
                 if ( sten )
                 {
                     const float4 g = A.stencil[q];
-                    getDistance32s( g, __builtin_amdgcn_rcpf( g.w ), ui0, ui1, ui2, u[j], u[n + j], u[2 * n + j], b );
+                    getDistance32s( g, __builtin_amdgcn_rcpf( g.w ), g.w * g.w, ui0, ui1, ui2, u[j], u[n + j],
+                                    u[2 * n + j], b );
                 }
                 else
                     getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
@@ -1109,17 +1121,9 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     constexpr int KV = 3 + NX;                   // staged vol (GEN)
     constexpr int NC = KV + ( GEN ? 1 : 0 );
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
-    __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
-    __shared__ float sinv[( PREC == kMixed ) ? 256 : 1];
     __shared__ int4 rtab[kMaxRows];
     WaveCtx w;
     const bool active = waveSetup( specs, tile_spec, tile_slice, 0, parity, launch, w );
-    if constexpr ( PREC == kMixed )
-    {
-        const float4 g = w.A->stencil[threadIdx.x];
-        stab[threadIdx.x] = g;
-        sinv[threadIdx.x] = __builtin_amdgcn_rcpf( g.w );
-    }
     if ( static_cast<int>( threadIdx.x ) < w.A->nrows )
         rtab[threadIdx.x] = w.A->rows[threadIdx.x];
     __syncthreads();
@@ -1137,12 +1141,13 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     const double* __restrict__ u = A.U[w.cur];
     const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
     const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
-    const int32_t nx = A.lat_n[0], ny = A.lat_n[1], nz = A.lat_n[2];
-    const int64_t t1 = pc / nx;
-    const int32_t ix = static_cast<int32_t>( pc - t1 * nx );
-    const int32_t iy = static_cast<int32_t>( t1 % ny );
-    const int32_t iz = static_cast<int32_t>( t1 / ny );
-    const uint64_t* __restrict__ bits = A.intact + A.slice_ptr[w.slice];
+    // per-slot lane masks (intact words, a subset of the existence words,
+    // or the existence words without fracture) and the stencil tables, all
+    // wave-uniform: scalar loads
+    const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + w.slice );
+    const uint64_t* __restrict__ mwords = scalarLoad( FRAC ? &A.intact : &A.exist ) + row0;
+    const float4* __restrict__ sten = scalarLoad( &A.stencil );
+    const float4* __restrict__ saux = scalarLoad( &A.stencil_aux );
     const int infl1 = A.influence;
     const double volu = A.vol_value;
     const float vol32 = static_cast<float>( A.vol_value );
@@ -1164,7 +1169,6 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
         return *reinterpret_cast<const double*>( reinterpret_cast<const char*>( base ) + byte_off );
     };
     double v0[NC], v1[NC];
-    uint32_t blo = 0, bhi = 0;
     auto loadRow = [&]( int r ) {
         const int4 R = rtab[r];
         const int32_t g = p032 + R.z;
@@ -1183,12 +1187,6 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
             v0[KV] = ld( volp, o0 );
             v1[KV] = second ? ld( volp, o1 ) : 0.0;
         }
-        if constexpr ( FRAC )
-        {
-            const uint64_t wd = ( lane < R.y ) ? bits[R.x + lane] : 0ull;
-            blo = static_cast<uint32_t>( wd );
-            bhi = static_cast<uint32_t>( wd >> 32 );
-        }
     };
     // Walk every bond slot in slot order; bond(live, wr, q) gets the window
     // position of the neighbor's staged values.
@@ -1206,27 +1204,16 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
                 if ( lane < R.y - 1 )
                     wb[k * kW + kSlice + lane] = v1[k];
             }
-            const uint32_t rlo = blo, rhi = bhi;
             __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
             if ( r + 1 < nrows )
                 loadRow( r + 1 );
-            const int32_t dx0 = ( R.w & 0xff ) - 128;
-            const int32_t dy = ( ( R.w >> 8 ) & 0xff ) - 128;
-            const int32_t dz = ( ( R.w >> 16 ) & 0xff ) - 128;
-            const bool rowok = valid && static_cast<uint32_t>( iy + dy ) < static_cast<uint32_t>( ny ) &&
-                               static_cast<uint32_t>( iz + dz ) < static_cast<uint32_t>( nz );
             for ( int t = 0; t < R.y; t++ )
             {
-                bool live = rowok && static_cast<uint32_t>( ix + dx0 + t ) < static_cast<uint32_t>( nx );
-                if constexpr ( FRAC )
-                {
-                    const uint32_t l32 = __builtin_amdgcn_readlane( rlo, t );
-                    const uint32_t h32 = __builtin_amdgcn_readlane( rhi, t );
-                    live = live && __builtin_amdgcn_inverse_ballot_w64( ( static_cast<uint64_t>( h32 ) << 32 ) | l32 );
-                }
-                bond( live, wb + lane + t, R.x + t );
+                const int32_t q = R.x + t;
+                const bool live = __builtin_amdgcn_inverse_ballot_w64( scalarLoad( mwords + q ) );
+                bond( live, wb + lane + t, q );
             }
         }
     };
@@ -1275,8 +1262,9 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
         walk( [&]( bool live, const double* wr, int32_t q ) {
             if ( live )
             {
-                const float4 g = stab[q];
-                const float xinv = sinv[q];
+                const float4 g = scalarLoad4( sten + q );
+                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, bandf/|xi|, |xi|^2, 0)
+                const float xinv = ga.x;
                 const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
                 if ( do_m )
                 {
@@ -1287,7 +1275,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
                 if ( do_theta )
                 {
                     Bond32 b;
-                    getDistance32s( g, xinv, ui0, ui1, ui2, wr[0], wr[kW], wr[2 * kW], b );
+                    getDistance32s( g, xinv, ga.z, ui0, ui1, ui2, wr[0], wr[kW], wr[2 * kW], b );
                     const float infl = infl1 ? xinv : 1.0f;
                     th += static_cast<double>( infl * b.s * b.xi * b.xi * vj32 );
                 }
@@ -1809,18 +1797,19 @@ void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipS
 
 namespace
 {
-__global__ void kStencilAux( const float4* __restrict__ table, int32_t S, float bandf, float2* __restrict__ aux )
+__global__ void kStencilAux( const float4* __restrict__ table, int32_t S, float bandf, float4* __restrict__ aux )
 {
     const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if ( q < S )
     {
-        const float xinv = __builtin_amdgcn_rcpf( table[q].w );
-        aux[q] = make_float2( xinv, bandf * xinv );
+        const float w = table[q].w;
+        const float xinv = __builtin_amdgcn_rcpf( w );
+        aux[q] = make_float4( xinv, bandf * xinv, w * w, 0.0f );
     }
 }
 } // namespace
 
-void launchStencilAux( const float4* table, int32_t S, float bandf, float2* aux, hipStream_t st )
+void launchStencilAux( const float4* table, int32_t S, float bandf, float4* aux, hipStream_t st )
 {
     if ( S <= 0 )
         return;
