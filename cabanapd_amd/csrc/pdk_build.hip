@@ -1013,8 +1013,9 @@ void finishSlots( int64_t n, int64_t nslices, const LatticeBox& L, const std::ve
     }
     DevBuf drows, drstart;
     int32_t nrows = 0;
-    // the row-window pass addresses particles with 32-bit byte offsets
-    if ( static_cast<int>( rtab.size() ) <= kMaxRows && n < ( int64_t( 1 ) << 29 ) )
+    // the row-window passes address their windows with 32-bit byte offsets
+    // checked against 32-bit buffer sizes: n < 2^28
+    if ( static_cast<int>( rtab.size() ) <= kMaxRows && n < ( int64_t( 1 ) << 28 ) )
     {
         nrows = static_cast<int32_t>( rtab.size() );
         drows.alloc( nrows * sizeof( int4 ), acct );

@@ -115,6 +115,37 @@ __device__ __forceinline__ float4 scalarLoad4( const float4* p )
     const auto* f = ( const __attribute__( ( address_space( 4 ) ) ) float* )( p );
     return make_float4( f[0], f[1], f[2], f[3] );
 }
+// Raw buffer access for the row windows: an SGPR resource (uniform base and
+// byte size), a VGPR byte offset (the lane) and an SGPR byte offset (the
+// window start), so no address arithmetic runs on the vector ALUs.  Offsets
+// at or past the size read 0 (hardware range check), which also covers
+// windows that start before id 0 (their negative offsets wrap to values
+// above the size for any specimen of fewer than 2^28 particles).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf( const void* base, uint32_t bytes )
+{
+    // base is wave-uniform; readfirstlane keeps the resource in SGPRs
+    const uint64_t b = reinterpret_cast<uint64_t>( base );
+    const uint32_t lo = __builtin_amdgcn_readfirstlane( static_cast<uint32_t>( b ) );
+    const uint32_t hi = __builtin_amdgcn_readfirstlane( static_cast<uint32_t>( b >> 32 ) );
+    void* ub = reinterpret_cast<void*>( ( static_cast<uint64_t>( hi ) << 32 ) | lo );
+    // dword 3 of a raw (untyped) buffer resource on gfx9-family GPUs
+    return __builtin_amdgcn_make_buffer_rsrc( ub, 0, static_cast<int>( __builtin_amdgcn_readfirstlane( bytes ) ),
+                                              0x00020000 );
+}
+__device__ __forceinline__ double bufferLoad64( __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff )
+{
+    return __builtin_bit_cast( double, __builtin_amdgcn_raw_buffer_load_b64( r, voff, soff, 0 ) );
+}
+__device__ __forceinline__ uint8_t bufferLoad8( __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff )
+{
+    return __builtin_amdgcn_raw_buffer_load_b8( r, voff, soff, 0 );
+}
+
+__device__ __forceinline__ int4 scalarLoad4( const int4* p )
+{
+    const auto* f = ( const __attribute__( ( address_space( 4 ) ) ) int* )( p );
+    return make_int4( f[0], f[1], f[2], f[3] );
+}
 
 __device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi2, double ui0,
                                                double ui1, double ui2, double xj0, double xj1,
@@ -712,8 +743,14 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
 // of the existence word) or the existence word.  Evaluation order and
 // arithmetic are kForce's, so results are identical.
 // ---------------------------------------------------------------------------
+// Waves per SIMD the register allocator must leave room for (the attribute
+// below): 6 for the PMB passes (80 VGPRs, one spill outside the loops;
+// measured 6 % faster on the fracture plate than the unconstrained
+// allocation at 5 waves); LPS (models 1 and 5) stays unconstrained, since
+// forcing it spills inside the slot loop.
 template <int MODEL, bool FRAC, int PREC, bool GEN>
-__global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict__ specs,
+__global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( MODEL == 1 || MODEL == 5 ) ? 1 : 6, 8 ) ) ) void
+kForceRows( const PDArgs* __restrict__ specs,
                                                         const int32_t* __restrict__ tile_spec,
                                                         const int64_t* __restrict__ tile_slice,
                                                         int32_t epi, int32_t parity, int64_t launch,
@@ -727,27 +764,30 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     constexpr int KNF = KV + 1;                     // staged no-fail flag (GEN)
     constexpr int NC = KV + ( GEN ? 2 : 0 );
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
-    __shared__ int4 rtab[kMaxRows];
     WaveCtx w;
-    const bool active = waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w );
-    // All waves of a block belong to one spec: stage its row table before any
-    // wave leaves.
-    if ( static_cast<int>( threadIdx.x ) < w.A->nrows )
-        rtab[threadIdx.x] = w.A->rows[threadIdx.x];
-    __syncthreads();
-    if ( !active )
-        return;
+    if ( !waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w ) )
+        return; // waves are independent: no block-level synchronisation
     const PDArgs& A = *w.A;
-    const int wv = threadIdx.x / kSlice;
+    const int wv = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
     const int lane = threadIdx.x & ( kSlice - 1 );
-    const int64_t n = A.n;
+    const int64_t n = scalarLoad( &A.n );
     const int64_t p0 = w.slice * kSlice;
     const int64_t p = p0 + lane;
     const bool valid = p < n;
     const int64_t pc = valid ? p : p0;
-    const double* __restrict__ x = A.x;
-    const double* __restrict__ u = A.U[w.cur];
-    const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
+    // Base pointers, the row table and the slice table through scalar loads
+    // (the argument block and the tables are written by the host only), so
+    // window addresses are an SGPR base plus the lane offset.
+    const double* __restrict__ x = scalarLoad( &A.x );
+    const double* __restrict__ u = scalarLoad( &A.U[w.cur] );
+    const int4* __restrict__ rows = scalarLoad( &A.rows );
+    double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0;
+    if constexpr ( PREC == kF64 )
+    {
+        xi0 = x[pc];
+        xi1 = x[n + pc];
+        xi2 = x[2 * n + pc];
+    }
     const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
     // Per-slot lane masks of the slice: the intact words (a subset of the
     // existence words, so they alone gate the fracture forms) or, without
@@ -800,45 +840,56 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
     }
     if constexpr ( NAB )
     {
-        src[3 + NX] = A.lpsA;
-        src[4 + NX] = A.lpsB;
+        src[3 + NX] = scalarLoad( &A.lpsA );
+        src[4 + NX] = scalarLoad( &A.lpsB );
     }
-    const double* __restrict__ volp = A.vol;
-    const uint8_t* __restrict__ nfp = A.nofail;
+    const double* __restrict__ volp = scalarLoad( &A.vol );
+    const uint8_t* __restrict__ nfp = scalarLoad( &A.nofail );
     double v0[NC], v1[NC];
-    // Row tables exist only for n < 2^29, so ids and byte offsets fit 32 bits.
-    const int32_t n32 = static_cast<int32_t>( n );
+#pragma unroll
+    for ( int k = 0; k < NC; k++ )
+        v1[k] = 0.0;
+    // Row tables exist only for n < 2^28 (byte sizes and offsets fit 32 bits).
     const int32_t p032 = static_cast<int32_t>( p0 );
-    auto ld = []( const double* base, uint32_t byte_off ) {
-        return *reinterpret_cast<const double*>( reinterpret_cast<const char*>( base ) + byte_off );
-    };
-    auto loadRow = [&]( int r ) {
-        const int4 R = rtab[r];
-        const int32_t g = p032 + R.z;
-        const int32_t g0 = min( max( g + lane, 0 ), n32 - 1 );
-        const int32_t g1 = min( max( g + kSlice + lane, 0 ), n32 - 1 );
-        const uint32_t o0 = static_cast<uint32_t>( g0 ) * 8u, o1 = static_cast<uint32_t>( g1 ) * 8u;
-        const bool second = lane < R.y - 1;
+    const uint32_t nbytes = static_cast<uint32_t>( n ) * 8u;
+    __amdgpu_buffer_rsrc_t rs[KV + ( GEN ? 1 : 0 )];
+#pragma unroll
+    for ( int k = 0; k < KV; k++ )
+        rs[k] = bufferOf( src[k], nbytes );
+    const __amdgpu_buffer_rsrc_t rnf = bufferOf( nfp, GEN ? static_cast<uint32_t>( n ) : 0u );
+    if constexpr ( GEN )
+        rs[KV] = bufferOf( volp, nbytes );
+    const uint32_t lane8 = static_cast<uint32_t>( lane ) * 8u;
+    // 64 window values starting at id g (uniform) into dst.
+    auto loadHalf = [&]( double* dst, int32_t g ) {
+        // g >= 0: lane offset + uniform start; g < 0: per-lane offsets, the
+        // negative ones wrap out of range and read 0
+        const uint32_t vo = g >= 0 ? lane8 : static_cast<uint32_t>( g + lane ) * 8u;
+        const uint32_t so = g >= 0 ? static_cast<uint32_t>( g ) * 8u : 0u;
 #pragma unroll
         for ( int k = 0; k < KV; k++ )
-        {
-            v0[k] = ld( src[k], o0 );
-            v1[k] = second ? ld( src[k], o1 ) : 0.0;
-        }
+            dst[k] = bufferLoad64( rs[k], vo, so );
         if constexpr ( GEN )
         {
-            v0[KV] = ld( volp, o0 );
-            v1[KV] = second ? ld( volp, o1 ) : 0.0;
-            v0[KNF] = ( FRAC && nfp[g0] != 0 ) ? 1.0 : 0.0;
-            v1[KNF] = ( FRAC && second && nfp[g1] != 0 ) ? 1.0 : 0.0;
+            dst[KV] = bufferLoad64( rs[KV], vo, so );
+            dst[KNF] = ( FRAC && bufferLoad8( rnf, vo >> 3, so >> 3 ) != 0 ) ? 1.0 : 0.0;
         }
     };
+    // Window of row r: ids g .. g + 63 (every lane) and g + 64 .. g + 64 + c - 2
+    // (lanes < c - 1).
+    auto loadRow = [&]( int r ) {
+        const int4 R = scalarLoad4( rows + r );
+        const int32_t g = p032 + R.z; // uniform
+        loadHalf( v0, g );
+        if ( lane < R.y - 1 )
+            loadHalf( v1, g + kSlice );
+    };
 
-    const int nrows = A.nrows;
+    const int nrows = scalarLoad( &A.nrows );
     loadRow( 0 );
     for ( int r = 0; r < nrows; r++ )
     {
-        const int4 R = rtab[r];
+        const int4 R = scalarLoad4( rows + r );
         double* __restrict__ wb = &win[wv][r & 1][0][0];
 #pragma unroll
         for ( int k = 0; k < NC; k++ )
@@ -921,10 +972,14 @@ __global__ __launch_bounds__( kBlock ) void kForceRows( const PDArgs* __restrict
                         bool crit = false;
                         if ( __builtin_amdgcn_inverse_ballot_w64( inb ) )
                         {
-                            const int64_t j = p + R.z + t;
+                            // neighbor j = p + R.z + t: uniform base + lane; the
+                            // coordinates of i are reloaded here (rare path)
+                            // rather than held in registers for the whole pass
+                            const double* __restrict__ xj = x + ( p0 + R.z + t );
+                            const double* __restrict__ xp = x + p0;
                             Bond64 e;
-                            getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], uj0, uj1, uj2,
-                                           e );
+                            getDistance64( xp[lane], ( xp + n )[lane], ( xp + 2 * n )[lane], ui0, ui1, ui2, xj[lane],
+                                           ( xj + n )[lane], ( xj + 2 * n )[lane], uj0, uj1, uj2, e );
                             crit = criticalStretch64( e.r, e.xi, bcc );
                         }
                         brkm |= __ballot( crit ) & inb;
@@ -1121,24 +1176,20 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     constexpr int KV = 3 + NX;                   // staged vol (GEN)
     constexpr int NC = KV + ( GEN ? 1 : 0 );
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
-    __shared__ int4 rtab[kMaxRows];
     WaveCtx w;
-    const bool active = waveSetup( specs, tile_spec, tile_slice, 0, parity, launch, w );
-    if ( static_cast<int>( threadIdx.x ) < w.A->nrows )
-        rtab[threadIdx.x] = w.A->rows[threadIdx.x];
-    __syncthreads();
-    if ( !active )
-        return;
+    if ( !waveSetup( specs, tile_spec, tile_slice, 0, parity, launch, w ) )
+        return; // waves are independent: no block-level synchronisation
     const PDArgs& A = *w.A;
-    const int wv = threadIdx.x / kSlice;
+    const int wv = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
     const int lane = threadIdx.x & ( kSlice - 1 );
-    const int64_t n = A.n;
+    const int64_t n = scalarLoad( &A.n );
     const int64_t p0 = w.slice * kSlice;
     const int64_t p = p0 + lane;
     const bool valid = p < n;
     const int64_t pc = valid ? p : p0;
-    const double* __restrict__ x = A.x;
-    const double* __restrict__ u = A.U[w.cur];
+    const double* __restrict__ x = scalarLoad( &A.x );
+    const double* __restrict__ u = scalarLoad( &A.U[w.cur] );
+    const int4* __restrict__ rows = scalarLoad( &A.rows );
     const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
     const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
     // per-slot lane masks (intact words, a subset of the existence words,
@@ -1162,40 +1213,43 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
         src[4] = x + n;
         src[5] = x + 2 * n;
     }
-    const double* __restrict__ volp = A.vol;
-    const int32_t n32 = static_cast<int32_t>( n );
+    const double* __restrict__ volp = scalarLoad( &A.vol );
     const int32_t p032 = static_cast<int32_t>( p0 );
-    auto ld = []( const double* base, uint32_t byte_off ) {
-        return *reinterpret_cast<const double*>( reinterpret_cast<const char*>( base ) + byte_off );
-    };
     double v0[NC], v1[NC];
-    auto loadRow = [&]( int r ) {
-        const int4 R = rtab[r];
-        const int32_t g = p032 + R.z;
-        const int32_t g0 = min( max( g + lane, 0 ), n32 - 1 );
-        const int32_t g1 = min( max( g + kSlice + lane, 0 ), n32 - 1 );
-        const uint32_t o0 = static_cast<uint32_t>( g0 ) * 8u, o1 = static_cast<uint32_t>( g1 ) * 8u;
-        const bool second = lane < R.y - 1;
 #pragma unroll
-        for ( int k = 0; k < KV; k++ )
-        {
-            v0[k] = ld( src[k], o0 );
-            v1[k] = second ? ld( src[k], o1 ) : 0.0;
-        }
-        if constexpr ( GEN )
-        {
-            v0[KV] = ld( volp, o0 );
-            v1[KV] = second ? ld( volp, o1 ) : 0.0;
-        }
+    for ( int k = 0; k < NC; k++ )
+        v1[k] = 0.0;
+    // Row windows through raw buffer loads, as in kForceRows (n < 2^28).
+    const uint32_t nbytes = static_cast<uint32_t>( n ) * 8u;
+    __amdgpu_buffer_rsrc_t rs[NC];
+#pragma unroll
+    for ( int k = 0; k < KV; k++ )
+        rs[k] = bufferOf( src[k], nbytes );
+    if constexpr ( GEN )
+        rs[KV] = bufferOf( volp, nbytes );
+    const uint32_t lane8 = static_cast<uint32_t>( lane ) * 8u;
+    auto loadHalf = [&]( double* dst, int32_t g ) {
+        const uint32_t vo = g >= 0 ? lane8 : static_cast<uint32_t>( g + lane ) * 8u;
+        const uint32_t so = g >= 0 ? static_cast<uint32_t>( g ) * 8u : 0u;
+#pragma unroll
+        for ( int k = 0; k < NC; k++ )
+            dst[k] = bufferLoad64( rs[k], vo, so );
+    };
+    auto loadRow = [&]( int r ) {
+        const int4 R = scalarLoad4( rows + r );
+        const int32_t g = p032 + R.z; // uniform
+        loadHalf( v0, g );
+        if ( lane < R.y - 1 )
+            loadHalf( v1, g + kSlice );
     };
     // Walk every bond slot in slot order; bond(live, wr, q) gets the window
     // position of the neighbor's staged values.
     auto walk = [&]( auto&& bond ) {
-        const int nrows = A.nrows;
+        const int nrows = scalarLoad( &A.nrows );
         loadRow( 0 );
         for ( int r = 0; r < nrows; r++ )
         {
-            const int4 R = rtab[r];
+            const int4 R = scalarLoad4( rows + r );
             double* __restrict__ wb = &win[wv][r & 1][0][0];
 #pragma unroll
             for ( int k = 0; k < NC; k++ )
