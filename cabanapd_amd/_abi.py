@@ -120,6 +120,18 @@ PDK_BONDS_AUTO = 0
 PDK_BONDS_COMPACT = 1
 
 
+class Material(C.Structure):
+    _fields_ = [("K", C.c_double), ("G", C.c_double), ("G0", C.c_double)]
+
+
+def materials_array(mats):
+    """(K, G, G0) tuples -> a ctypes array of Material."""
+    arr = (Material * len(mats))()
+    for t, (K, G, G0) in enumerate(mats):
+        arr[t].K, arr[t].G, arr[t].G0 = K, G, G0
+    return arr
+
+
 class ModelConstants(C.Structure):
     _fields_ = [(name, C.c_double) for name in (
         "c", "s0", "bond_break_coeff", "theta_coeff", "s_coeff",

@@ -171,6 +171,60 @@ def kw_regression(precision="fp64"):
                     horizon_extra=1e-10, output_every=50, name="kw_regression")
 
 
+def crack_inclusion(precision="fp64"):
+    """regression_test/inputs/crack_inclusion.json run through
+    examples/mechanics/crack_inclusion.cpp: LPS fracture (influence 0) on
+    80x40x4 cells of a 0.04 x 0.02 x 0.001 plate, two materials (plate
+    E 72 GPa, G0 3.8; inclusion E 144 GPa, G0 38; nu 0.22, rho 2440) averaged
+    across the interface (createMultiForceModel with AverageTag), inclusion
+    radius 0.005 at (0.015, 0); horizon 0.001 (+1e-10); one pre-notch of
+    length height/2 through the thickness at mid-height from the low-x edge;
+    no-fail bands of one horizon inside the loaded rows; traction 2 MPa on the
+    low/high y rows as f_y += sign(y) sigma/dy; dt 3.4e-8,
+    int(22.5e-6 / 3.4e-8) = 661 steps, output every 50."""
+    size = np.array([0.04, 0.02, 0.001])
+    low, high = -0.5 * size, 0.5 * size
+    x, vol, d = grid_cell_centred(low, high, (80, 40, 4))
+    dy = d[1]
+    rho0 = 2440.0
+    E = (72e9, 144e9)
+    nu = (0.22, 0.22)
+    G0 = (3.8, 38.0)
+    mats = []
+    for t in range(2):
+        K = E[t] / (3 * (1 - 2 * nu[t]))
+        G = E[t] / (2 * (1 + nu[t]))
+        mats.append((K, G, G0[t]))
+    horizon = 0.001 + 1e-10
+    tf, dt = 22.5e-6, 3.4e-8
+    steps = int(tf / dt)  # Inputs truncation (CabanaPD_Input.hpp:109)
+    desc = make_desc("LPS", fracture=True, precision=precision, horizon=horizon, K=mats[0][0], G=mats[0][1],
+                     G0=mats[0][2], dt=dt, influence=0)
+    w = _base(desc, x, vol, rho0, "crack_inclusion_80x40x4", d[0], steps, output_every=50)
+    height, thickness = size[0], size[2]
+    y_notch = 0.5 * (low[1] + high[1])
+    w["notches"] = (np.array([[low[0], y_notch, low[2]]]), np.array([[height / 2.0, 0.0, 0.0]]),
+                    np.array([[0.0, 0.0, thickness]]))
+    # BC planes (crack_inclusion.cpp:98-105) and the no-fail bands (:122-125)
+    p1_lo_y, p1_hi_y = low[1] - dy, low[1] + dy
+    p2_lo_y, p2_hi_y = high[1] - dy, high[1] + dy
+    w["nofail"] = ((x[:, 1] <= p1_lo_y + horizon + 1e-10) | (x[:, 1] >= p2_hi_y - horizon - 1e-10)).astype(np.int32)
+    rsq = (x[:, 0] - 0.015) ** 2 + (x[:, 1] - 0.0) ** 2
+    mtype = (rsq < 0.005 * 0.005).astype(np.int32)
+    w["materials"] = (mats, mtype)
+    # the force BC functor f(pid, 1) += b0 * sign(y) on both planes, applied
+    # after the force pass every step: a constant per-particle force term
+    b0 = 2e6 / dy
+    inx = (x[:, 0] >= low[0]) & (x[:, 0] <= high[0]) & (x[:, 2] >= low[2]) & (x[:, 2] <= high[2])
+    in1 = inx & (x[:, 1] >= p1_lo_y) & (x[:, 1] <= p1_hi_y)
+    in2 = inx & (x[:, 1] >= p2_lo_y) & (x[:, 1] <= p2_hi_y)
+    body = np.zeros_like(x)
+    sel = in1 | in2
+    body[sel, 1] = b0 * (np.abs(x[sel, 1]) / x[sel, 1])
+    w["body"] = body
+    return w
+
+
 def kw_crack_y(x, phi):
     """Output time series value of kalthoff_winkler.cpp:134-145: max over
     particles of (phi > 0.3 ? y : 0)."""
@@ -355,6 +409,8 @@ def upload(spec, w):
 def setup(spec, w, init=True):
     """Build bonds, prenotch, BC/body term and Solver::init for a workload."""
     spec.build_bonds()
+    if w.get("materials") is not None:
+        spec.set_materials(*w["materials"])
     if w["notches"] is not None:
         spec.prenotch(*w["notches"])
     if w["bc"] is not None:

@@ -154,7 +154,10 @@ struct pdk_spec_s
     int64_t n = 0, nslices = 0;
     hipStream_t st = nullptr;
     DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
-        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stress;
+        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stress, mtype;
+    // multi-material (pdk_set_materials): materials and pair tables
+    int32_t nmat = 1;
+    double mat_c[16] = {}, mat_bcc[16] = {}, mat_tc[4] = {}, mat_sc[4] = {};
     double lat_h[3] = { 0, 0, 0 }; // lattice spacing when lattice_rows
     int64_t lat_dims[3] = { 0, 0, 0 };
     int32_t nsten = 0;             // stencil entries (0 = no stencil)
@@ -287,12 +290,25 @@ struct pdk_spec_s
             h.rowstart = nullptr;
             h.nrows = 0;
         }
-        // tuning knob: -1 = default (row windows where available, else groups of 4)
+        // tuning knob: -1 = default (row windows where available, else groups
+        // of 4); multi-material specimens use the gather variant
         h.unroll = unroll;
         if ( h.unroll < 0 )
-            h.unroll = h.nrows > 0 ? 0 : 4;
-        if ( h.unroll == 0 && h.nrows == 0 )
+            h.unroll = ( h.nrows > 0 && nmat == 1 ) ? 0 : 4;
+        if ( h.unroll == 0 && ( h.nrows == 0 || nmat > 1 ) )
             h.unroll = 4;
+        h.nmat = nmat;
+        h.mtype = nmat > 1 ? mtype.as<uint8_t>() : nullptr;
+        for ( int t = 0; t < 16; t++ )
+        {
+            h.mat_c[t] = mat_c[t];
+            h.mat_bcc[t] = mat_bcc[t];
+        }
+        for ( int t = 0; t < 4; t++ )
+        {
+            h.mat_tc[t] = mat_tc[t];
+            h.mat_sc[t] = mat_sc[t];
+        }
         h.c = k.c;
         h.bcc = k.bond_break_coeff;
         h.s0 = k.s0;
@@ -1101,6 +1117,69 @@ int pdk_compute_energy( pdk_spec s, double* total_strain_energy, double* total_d
     } );
 }
 
+int pdk_set_materials( pdk_spec s, int32_t nmat, const pdk_material* mats, const int32_t* type )
+{
+    return guarded( [&] {
+        require( s && mats, "null argument" );
+        require( s->uploaded, "call pdk_spec_upload first" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( !s->inited, "set the materials before pdk_init" );
+        require( nmat >= 1 && nmat <= 4, "1 to 4 materials" );
+        require( !isContact( s->d ), "the contact models have no materials" );
+        require( nmat == 1 || s->d.precision == PDK_FP64, "multi-material specimens are FP64 only" );
+        require( nmat == 1 || type != nullptr, "type[] is required for several materials" );
+        // createMultiForceModel( ..., AverageTag, m_0, .. ) (CabanaPD_ForceModelsMulti.hpp:338-516):
+        // pair (a, a) -> material a's model; a != b -> the averaged model
+        // (force_models/CabanaPD_PMB.hpp:51-55, CabanaPD_ForceModels.hpp:114-121).
+        double K[4], c[4], s0[4], bcc[4];
+        for ( int32_t t = 0; t < nmat; t++ )
+        {
+            pdk_model_desc d = s->d;
+            d.K = mats[t].K;
+            d.G = mats[t].G;
+            d.G0 = mats[t].G0;
+            pdk_model_constants k;
+            modelConstants( d, k ); // validates K, G, G0 as for one model
+            K[t] = d.K;
+            c[t] = k.c;
+            s0[t] = k.s0;
+            bcc[t] = k.bond_break_coeff;
+            s->mat_tc[t] = k.theta_coeff;
+            s->mat_sc[t] = k.s_coeff;
+        }
+        for ( int32_t a = 0; a < nmat; a++ )
+            for ( int32_t b = 0; b < nmat; b++ )
+            {
+                if ( a == b )
+                {
+                    s->mat_c[4 * a + b] = c[a];
+                    s->mat_bcc[4 * a + b] = bcc[a];
+                }
+                else
+                {
+                    s->mat_c[4 * a + b] = ( c[a] + c[b] ) / 2.0;
+                    const double s0ab = std::sqrt( ( s0[a] * s0[a] * K[a] + s0[b] * s0[b] * K[b] ) / ( K[a] + K[b] ) );
+                    s->mat_bcc[4 * a + b] = ( 1.0 + s0ab ) * ( 1.0 + s0ab );
+                }
+            }
+        if ( nmat > 1 )
+        {
+            std::vector<uint8_t> dev( s->n );
+            for ( int64_t cidx = 0; cidx < s->n; cidx++ )
+            {
+                require( type[cidx] >= 0 && type[cidx] < nmat, "material type out of range" );
+                dev[s->h_inv[cidx]] = static_cast<uint8_t>( type[cidx] );
+            }
+            if ( s->mtype.bytes() < static_cast<size_t>( s->n ) )
+                s->mtype.alloc( std::max<int64_t>( s->n, 1 ), s->acct() );
+            PDK_HIP( hipMemcpyAsync( s->mtype.as<uint8_t>(), dev.data(), s->n, hipMemcpyHostToDevice, s->st ) );
+        }
+        s->nmat = nmat;
+        s->pushArgs();
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
 int pdk_compute_stress( pdk_spec s )
 {
     return guarded( [&] {
@@ -1448,6 +1527,7 @@ int pdk_set_tuning( pdk_spec s, int32_t unroll )
                  "unroll must be 0 (row windows), 1, 2, 4 or 8" );
         require( unroll != 0 || !s->built || s->slot.nrows > 0,
                  "row windows need the stencil-slot layout of a full lattice" );
+        require( unroll != 0 || s->nmat == 1, "multi-material specimens use the gather variant" );
         s->unroll = unroll;
         if ( s->uploaded )
         {
@@ -1475,7 +1555,7 @@ int pdk_autotune( pdk_spec s, int32_t launches, int32_t* unroll_out )
         float best = 1e30f;
         int32_t best_u = s->unroll;
         std::vector<int32_t> cand;
-        if ( s->slot.nrows > 0 )
+        if ( s->slot.nrows > 0 && s->nmat == 1 )
             cand.push_back( 0 ); // row windows staged in LDS
         for ( int32_t u : { 1, 2, 4, 8 } )
             cand.push_back( u );
@@ -1639,6 +1719,7 @@ int pdk_submit( pdk_ctx ctx, pdk_spec s, int32_t nsteps, uint64_t* ticket )
     return guarded( [&] {
         require( ctx && s && ticket, "null argument" );
         require( s->ctx == ctx, "spec belongs to another context" );
+        require( s->nmat == 1, "multi-material specimens are stepped with pdk_step, not the batch engine" );
         *ticket = ctx->batch->submit( s, nsteps );
     } );
 }

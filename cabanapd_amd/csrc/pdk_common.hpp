@@ -113,6 +113,13 @@ struct PDArgs
     const uint32_t* rowstart; // bit q set: slot q starts a row (9 words), or null
     int32_t nrows;          // 0: no row table
     int32_t lat_n[3];       // lattice extent (stencil-slot form)
+    // Multi-material specimens (FP64 gather passes): material of each
+    // particle (null: one material), the PMB coefficient and the critical
+    // stretch coefficient of each material pair [4 a + b], and each
+    // material's LPS theta / s coefficients.
+    const uint8_t* mtype;
+    int32_t nmat;
+    double mat_c[16], mat_bcc[16], mat_tc[4], mat_sc[4];
 };
 
 constexpr int kRowMaxSlots = 9;   // slots per row window (|dx| <= 4)

@@ -448,6 +448,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
     const double c = A.c;
     const double bcc = A.bcc;
+    // multi-material (FP64): model of the material pair; null with one material
+    const uint8_t* __restrict__ mt = ( PREC == kF64 ) ? A.mtype : nullptr;
+    const int ti4 = mt ? 4 * static_cast<int>( mt[pc] ) : 0;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( kIsLps<MODEL> )
     {
@@ -568,12 +571,16 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         }
         double xj[UNR][3], uj[UNR][3], vj[UNR], aj[UNR], bj[UNR];
         bool nfj[UNR];
+        int mj[UNR];
 #pragma unroll
         for ( int k = 0; k < UNR; k++ )
         {
+            mj[k] = 0;
             if ( ex[k] )
             {
                 const int64_t j = jj[k];
+                if ( mt )
+                    mj[k] = mt[j];
                 if ( !useSten )
                 {
                     xj[k][0] = x[j];
@@ -620,14 +627,14 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                                    uj[k][2], b );
                     if constexpr ( FRAC )
                     {
-                        if ( criticalStretch64( b.r, b.xi, bcc ) && !nf_i && !nf_j )
+                        if ( criticalStretch64( b.r, b.xi, mt ? A.mat_bcc[ti4 + mj[k]] : bcc ) && !nf_i && !nf_j )
                             brk = true;
                     }
                     if ( !brk && intact )
                     {
                         double coeff;
                         if constexpr ( !kIsLps<MODEL> )
-                            coeff = c * b.s * vj[k];
+                            coeff = ( mt ? A.mat_c[ti4 + mj[k]] : c ) * b.s * vj[k];
                         else
                         {
                             const double infl = infl1 ? 1.0 / b.xi : 1.0;
@@ -1151,9 +1158,12 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
                 th = 3.0 * th / m_i;
     }
     A.theta[p] = th;
-    // ForceCoeffTag per-particle subexpressions (force_models/CabanaPD_LPS.hpp:131-132)
-    A.lpsA[p] = A.theta_coeff * th / m_i;
-    A.lpsB[p] = A.s_coeff / m_i;
+    // ForceCoeffTag per-particle subexpressions (force_models/CabanaPD_LPS.hpp:131-132),
+    // with the particle's own material coefficients in a multi-material specimen
+    const double tc = A.mtype ? A.mat_tc[A.mtype[p]] : A.theta_coeff;
+    const double sc = A.mtype ? A.mat_sc[A.mtype[p]] : A.s_coeff;
+    A.lpsA[p] = tc * th / m_i;
+    A.lpsB[p] = sc / m_i;
 }
 
 // ---------------------------------------------------------------------------
@@ -1351,9 +1361,12 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     if ( !do_theta )
         return;
     A.theta[p] = th;
-    // ForceCoeffTag per-particle subexpressions (force_models/CabanaPD_LPS.hpp:131-132)
-    A.lpsA[p] = A.theta_coeff * th / m_i;
-    A.lpsB[p] = A.s_coeff / m_i;
+    // ForceCoeffTag per-particle subexpressions (force_models/CabanaPD_LPS.hpp:131-132),
+    // with the particle's own material coefficients in a multi-material specimen
+    const double tc = A.mtype ? A.mat_tc[A.mtype[p]] : A.theta_coeff;
+    const double sc = A.mtype ? A.mat_sc[A.mtype[p]] : A.s_coeff;
+    A.lpsA[p] = tc * th / m_i;
+    A.lpsB[p] = sc / m_i;
 }
 
 // ---------------------------------------------------------------------------
@@ -1390,6 +1403,10 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
         const uint64_t* __restrict__ bits = A.intact + base;
         const double* __restrict__ vol = A.vol;
         const double c = A.c;
+        const uint8_t* __restrict__ mt = A.mtype; // multi-material: pair / own-material constants
+        const int ti4 = mt ? 4 * static_cast<int>( mt[p] ) : 0;
+        const double tc_i = mt ? A.mat_tc[mt[p]] : A.theta_coeff;
+        const double sc_i = mt ? A.mat_sc[mt[p]] : A.s_coeff;
         double num_bonds = 0.0;
         double m_i = 0.0, theta_i = 0.0;
         if constexpr ( kIsLps<MODEL> )
@@ -1418,11 +1435,11 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
             if constexpr ( kIsLps<MODEL> )
             {
                 const double infl = A.influence ? 1.0 / b.xi : 1.0;
-                wb = 1.0 / num_bonds * 0.5 * A.theta_coeff / 3.0 * ( theta_i * theta_i ) +
-                     0.5 * ( A.s_coeff / m_i ) * infl * b.s * b.s * b.xi * b.xi * vol[j];
+                wb = 1.0 / num_bonds * 0.5 * tc_i / 3.0 * ( theta_i * theta_i ) +
+                     0.5 * ( sc_i / m_i ) * infl * b.s * b.s * b.xi * b.xi * vol[j];
             }
             else
-                wb = 0.25 * c * b.s * b.s * b.xi * vol[j];
+                wb = 0.25 * ( mt ? A.mat_c[ti4 + mt[j]] : c ) * b.s * b.s * b.xi * vol[j];
             if constexpr ( FRAC )
             {
                 const double mu = static_cast<double>( ( bits[q] >> lane ) & 1ull );
@@ -1502,6 +1519,8 @@ __global__ __launch_bounds__( kBlock ) void kStress( const PDArgs* __restrict__ 
     const uint64_t* __restrict__ bits = A.intact + base;
     const double* __restrict__ vol = A.vol;
     const double c = A.c;
+    const uint8_t* __restrict__ mt = A.mtype; // multi-material: pair constants
+    const int ti4 = mt ? 4 * static_cast<int>( mt[p] ) : 0;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( kIsLps<MODEL> )
     {
@@ -1535,7 +1554,7 @@ __global__ __launch_bounds__( kBlock ) void kStress( const PDArgs* __restrict__ 
             coeff *= 0.5;
         }
         else
-            coeff = 0.5 * ( c * b.s * vol[j] );
+            coeff = 0.5 * ( ( mt ? A.mat_c[ti4 + mt[j]] : c ) * b.s * vol[j] );
         double fx, fy, fz;
         if constexpr ( FRAC )
         {

@@ -69,6 +69,7 @@ _SIGS = {
     "pdk_series_gather": (C.c_int, [_VP, C.c_int64, _I64, _D, _I64]),
     "pdk_compute_energy_async": (C.c_int, [_VP]),
     "pdk_compute_stress": (C.c_int, [_VP]),
+    "pdk_set_materials": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.Material), _I32]),
 }
 
 
@@ -252,6 +253,13 @@ class Spec:
 
     def compute_stress(self):
         _check(lib().pdk_compute_stress(self.h))
+
+    def set_materials(self, mats, mtype):
+        """createMultiForceModel(particles, AverageTag, ...): mats = [(K, G, G0), ...],
+        mtype[n] the material of each particle."""
+        arr = _abi.materials_array(mats)
+        t = None if mtype is None else np.ascontiguousarray(mtype, dtype=np.int32)
+        _check(lib().pdk_set_materials(self.h, len(mats), arr, None if t is None else t.ctypes.data_as(_I32)))
 
     def compute_energy_async(self):
         _check(lib().pdk_compute_energy_async(self.h))

@@ -184,6 +184,23 @@ int pdk_set_bc(pdk_spec spec, int32_t kind, const int64_t* idx, int64_t n, doubl
  * e.g. gravity f_z -= 9.8 rho, or a face traction.  NULL clears it. */
 int pdk_set_body_force(pdk_spec spec, const double* b);
 
+/* ---- multi-material specimens ---------------------------------------------------
+ * createMultiForceModel( particles, AverageTag, m_0, .., m_{nmat-1} )
+ * (CabanaPD_ForceModelsMulti.hpp:338-516) for 1 <= nmat <= 4 materials that
+ * share the spec's model kind, fracture form, horizon and influence (as the
+ * reference requires): material t has its own K, G, G0; type[n] (caller
+ * order) holds each particle's material (Particles::sliceType).  A bond
+ * between materials a != b uses the averaged model: PMB c = (c_a + c_b) / 2
+ * (force_models/CabanaPD_PMB.hpp:51-55); s0 = sqrt((s0_a^2 K_a + s0_b^2 K_b) /
+ * (K_a + K_b)), bond_break_coeff = (1 + s0)^2 (CabanaPD_ForceModels.hpp:114-121);
+ * LPS keeps each material's theta/s coefficients on its own particle's terms
+ * (force_models/CabanaPD_LPS.hpp:55-69).  FP64 only; the force pass then uses
+ * the gather variant.  Call after pdk_spec_upload and before pdk_init. */
+typedef struct pdk_material {
+    double K, G, G0;
+} pdk_material;
+int pdk_set_materials(pdk_spec spec, int32_t nmat, const pdk_material* materials, const int32_t* type);
+
 /* ---- force / energy (free functions, CabanaPD_Force.hpp:176-219) ---------- */
 int pdk_compute_weighted_volume(pdk_spec spec);   /* Force<LPS>::computeWeightedVolume; no-op for PMB */
 int pdk_compute_dilatation(pdk_spec spec);        /* Force<LPS>::computeDilatation; no-op for PMB */

@@ -41,6 +41,7 @@ _SIGS = {
     "orc_compute_force": (C.c_int, [_VP, C.c_int32]),
     "orc_compute_energy": (C.c_int, [_VP, _D, _D]),
     "orc_compute_stress": (C.c_int, [_VP]),
+    "orc_set_materials": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.Material), _I32]),
     "orc_init": (C.c_int, [_VP]),
     "orc_step": (C.c_int, [_VP, C.c_int32]),
     "orc_get": (C.c_int, [_VP, C.c_int32, _D]),
@@ -213,6 +214,11 @@ class OracleSpec:
 
     def compute_stress(self):
         _check(lib().orc_compute_stress(self.h))
+
+    def set_materials(self, mats, mtype):
+        arr = _abi.materials_array(mats)
+        t = np.ascontiguousarray(mtype, dtype=np.int32)
+        _check(lib().orc_set_materials(self.h, len(mats), arr, t.ctypes.data_as(_I32)))
 
     def init(self):
         _check(lib().orc_init(self.h))

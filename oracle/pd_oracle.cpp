@@ -68,6 +68,11 @@ struct Sys
     double max_disp = 0.0;
     int64_t steps_since_build = 0;
     double total_W = 0.0, total_D = 0.0;
+    // multi-material (createMultiForceModel with AverageTag): material per
+    // particle, constants per material pair and per material
+    int32_t nmat = 1;
+    std::vector<int32_t> mtype;
+    double pair_c[16] = {}, pair_bcc[16] = {}, mat_tc[4] = {}, mat_sc[4] = {};
 };
 
 bool isContact( const pdk_model_desc& d )
@@ -467,14 +472,41 @@ void computeDilatation( Sys& s )
 }
 
 // model( ForceCoeffTag ) for LPS (force_models/CabanaPD_LPS.hpp:118-134)
-inline double lpsForceCoeff( const Sys& s, double st, double xi, double vol, double m_i,
+// Model of the bond (i, j) in a multi-material specimen: the model of the
+// material pair, the averaged model between different materials
+// (CabanaPD_ForceModelsMulti.hpp:257-273, 338-372).
+inline int32_t matOf( const Sys& s, int64_t i )
+{
+    return s.nmat > 1 ? s.mtype[i] : 0;
+}
+inline double pairC( const Sys& s, int64_t i, int64_t j )
+{
+    return s.nmat > 1 ? s.pair_c[4 * matOf( s, i ) + matOf( s, j )] : s.k.c;
+}
+inline double pairBcc( const Sys& s, int64_t i, int64_t j )
+{
+    return s.nmat > 1 ? s.pair_bcc[4 * matOf( s, i ) + matOf( s, j )] : s.k.bond_break_coeff;
+}
+// LPS coefficients of particle i's terms: i's material, also in an averaged
+// pair model (force_models/CabanaPD_LPS.hpp:55-69: theta_coeff[0]/s_coeff[0]
+// from the first model, [1] from the second).
+inline double thetaCoeff( const Sys& s, int64_t i )
+{
+    return s.nmat > 1 ? s.mat_tc[matOf( s, i )] : s.k.theta_coeff;
+}
+inline double sCoeff( const Sys& s, int64_t i )
+{
+    return s.nmat > 1 ? s.mat_sc[matOf( s, i )] : s.k.s_coeff;
+}
+
+inline double lpsForceCoeff( const Sys& s, int64_t i, int64_t j, double st, double xi, double vol, double m_i,
                              double m_j, double theta_i, double theta_j )
 {
     const double infl = influence( s, xi );
-    const double theta_coeff_i = s.k.theta_coeff;
-    const double theta_coeff_j = s.k.theta_coeff;
-    const double s_coeff_i = s.k.s_coeff;
-    const double s_coeff_j = s.k.s_coeff;
+    const double theta_coeff_i = thetaCoeff( s, i );
+    const double theta_coeff_j = thetaCoeff( s, j );
+    const double s_coeff_i = sCoeff( s, i );
+    const double s_coeff_j = sCoeff( s, j );
     return ( theta_coeff_i * theta_i / m_i + theta_coeff_j * theta_j / m_j +
              st * ( s_coeff_i / m_i + s_coeff_j / m_j ) ) *
            infl * xi * vol;
@@ -485,8 +517,6 @@ void computeForceFull( Sys& s )
 {
     const bool frac = s.d.fracture != 0;
     const bool lps = isLps( s.d );
-    const double c = s.k.c;
-    const double bcc = s.k.bond_break_coeff;
 #pragma omp parallel for schedule( static )
     for ( int64_t i = 0; i < s.n; i++ )
     {
@@ -503,9 +533,9 @@ void computeForceFull( Sys& s )
                 // force/CabanaPD_PMB.hpp:105-118, force/CabanaPD_LPS.hpp:161-174
                 double coeff;
                 if ( lps )
-                    coeff = lpsForceCoeff( s, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
+                    coeff = lpsForceCoeff( s, i, j, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
                 else
-                    coeff = c * st * s.vol[j];
+                    coeff = pairC( s, i, j ) * st * s.vol[j];
                 fx_i = coeff * rx / r;
                 fy_i = coeff * ry / r;
                 fz_i = coeff * rz / r;
@@ -518,7 +548,7 @@ void computeForceFull( Sys& s )
                 // force/CabanaPD_PMB.hpp:259-279; force/CabanaPD_LPS.hpp:416-438
                 // (the LPS no-fail test reads nofail(i) twice, :417-418).
                 const bool nofail_j = lps ? ( s.nofail[i] != 0 ) : ( s.nofail[j] != 0 );
-                if ( r * r >= bcc * xi * xi && !( s.nofail[i] != 0 ) && !nofail_j )
+                if ( r * r >= pairBcc( s, i, j ) * xi * xi && !( s.nofail[i] != 0 ) && !nofail_j )
                 {
                     mu[q] = 0;
                 }
@@ -526,9 +556,9 @@ void computeForceFull( Sys& s )
                 {
                     double coeff;
                     if ( lps )
-                        coeff = lpsForceCoeff( s, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
+                        coeff = lpsForceCoeff( s, i, j, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
                     else
-                        coeff = c * st * s.vol[j];
+                        coeff = pairC( s, i, j ) * st * s.vol[j];
                     double muij = mu[q];
                     fx_i = muij * coeff * rx / r;
                     fy_i = muij * coeff * ry / r;
@@ -548,7 +578,6 @@ void computeEnergyFull( Sys& s )
 {
     const bool frac = s.d.fracture != 0;
     const bool lps = isLps( s.d );
-    const double c = s.k.c;
     double Phi = 0.0, D = 0.0;
     std::fill( s.W.begin(), s.W.end(), 0.0 );
 #pragma omp parallel for schedule( static ) reduction( + : Phi, D )
@@ -577,15 +606,15 @@ void computeEnergyFull( Sys& s )
             {
                 // model( EnergyTag ) (force_models/CabanaPD_LPS.hpp:136-148)
                 const double infl = influence( s, xi );
-                const double theta_coeff_i = s.k.theta_coeff;
-                const double s_coeff_i = s.k.s_coeff;
+                const double theta_coeff_i = thetaCoeff( s, i );
+                const double s_coeff_i = sCoeff( s, i );
                 w = 1.0 / num_bonds * 0.5 * theta_coeff_i / 3.0 * ( s.theta[i] * s.theta[i] ) +
                     0.5 * ( s_coeff_i / s.m[i] ) * infl * st * st * xi * xi * s.vol[j];
             }
             else
             {
                 // model( EnergyTag ) (force_models/CabanaPD_PMB.hpp:64-71)
-                w = 0.25 * c * st * st * xi * s.vol[j];
+                w = 0.25 * pairC( s, i, j ) * st * st * xi * s.vol[j];
             }
             if ( frac )
             {
@@ -619,7 +648,6 @@ void computeStressFull( Sys& s )
 {
     const bool frac = s.d.fracture != 0;
     const bool lps = isLps( s.d );
-    const double c = s.k.c;
     s.stress.assign( 9 * s.n, 0.0 );
 #pragma omp parallel for schedule( static )
     for ( int64_t i = 0; i < s.n; i++ )
@@ -641,11 +669,11 @@ void computeStressFull( Sys& s )
             double coeff;
             if ( lps )
             {
-                coeff = lpsForceCoeff( s, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
+                coeff = lpsForceCoeff( s, i, j, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
                 coeff *= 0.5;
             }
             else
-                coeff = 0.5 * ( c * st * s.vol[j] ); // 0.5 * model( ForceCoeffTag, ... )
+                coeff = 0.5 * ( pairC( s, i, j ) * st * s.vol[j] ); // 0.5 * model( ForceCoeffTag, ... )
             double fx_i, fy_i, fz_i;
             if ( frac )
             {
@@ -1206,6 +1234,61 @@ int orc_step( orc_sys s, int32_t nsteps )
         applyForceTerms( *s );
         finalHalfStep( *s );
     }
+    return 0;
+}
+
+int orc_set_materials( orc_sys s, int32_t nmat, const pdk_material* mats, const int32_t* type )
+{
+    // createMultiForceModel( particles, AverageTag, m_0, .., m_{nmat-1} )
+    // (CabanaPD_ForceModelsMulti.hpp:338-516): the pair (a, a) uses material
+    // a's model, a pair a != b the averaged model: PMB c = (c_a + c_b) / 2
+    // (force_models/CabanaPD_PMB.hpp:51-55), s0 = sqrt((s0_a^2 K_a + s0_b^2
+    // K_b) / (K_a + K_b)), bond_break_coeff = (1 + s0)^2
+    // (CabanaPD_ForceModels.hpp:114-121); LPS keeps each material's
+    // coefficients on its own particle's terms.
+    if ( !s || nmat < 1 || nmat > 4 || !mats || ( nmat > 1 && !type ) || isContact( s->d ) )
+        return 1;
+    double K[4], c[4], s0[4], bcc[4];
+    for ( int32_t t = 0; t < nmat; t++ )
+    {
+        pdk_model_desc d = s->d;
+        d.K = mats[t].K;
+        d.G = mats[t].G;
+        d.G0 = mats[t].G0;
+        pdk_model_constants k;
+        if ( modelConstants( d, k ) )
+            return 1;
+        K[t] = d.K;
+        c[t] = k.c;
+        s0[t] = k.s0;
+        bcc[t] = k.bond_break_coeff;
+        s->mat_tc[t] = k.theta_coeff;
+        s->mat_sc[t] = k.s_coeff;
+    }
+    for ( int32_t a = 0; a < nmat; a++ )
+        for ( int32_t b = 0; b < nmat; b++ )
+        {
+            if ( a == b )
+            {
+                s->pair_c[4 * a + b] = c[a];
+                s->pair_bcc[4 * a + b] = bcc[a];
+            }
+            else
+            {
+                s->pair_c[4 * a + b] = ( c[a] + c[b] ) / 2.0;
+                const double s0ab = std::sqrt( ( s0[a] * s0[a] * K[a] + s0[b] * s0[b] * K[b] ) / ( K[a] + K[b] ) );
+                s->pair_bcc[4 * a + b] = ( 1.0 + s0ab ) * ( 1.0 + s0ab );
+            }
+        }
+    s->nmat = nmat;
+    s->mtype.assign( s->n, 0 );
+    if ( nmat > 1 )
+        for ( int64_t i = 0; i < s->n; i++ )
+        {
+            if ( type[i] < 0 || type[i] >= nmat )
+                return 1;
+            s->mtype[i] = type[i];
+        }
     return 0;
 }
 

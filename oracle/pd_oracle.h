@@ -45,6 +45,7 @@ int orc_compute_dilatation(orc_sys s);
 int orc_compute_force(orc_sys s, int32_t reset);
 int orc_compute_energy(orc_sys s, double* W, double* D);
 int orc_compute_stress(orc_sys s);
+int orc_set_materials(orc_sys s, int32_t nmat, const pdk_material* mats, const int32_t* type);
 int orc_init(orc_sys s);
 int orc_step(orc_sys s, int32_t nsteps);
 int orc_get(orc_sys s, int32_t field, double* out);

@@ -325,3 +325,78 @@ def test_gpu_elastic_wave_regression(precision):
     worst, l2, k = configs.polyfit_check(w["x"][line, 0], ug[line, 0], fx)
     say(f"  polyfit: worst rel {worst:.4f} l2 {l2:.3e} points {k}")
     assert k >= 20 and worst < fx["max_rel_err"] and l2 < fx["max_l2_err"]
+
+
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_gpu_multi_material_bitwise(model):
+    """createMultiForceModel(AverageTag) with two materials scattered at
+    random over a cube with fracture: FP64 bonds, u, v, f, W, phi and stress
+    equal the oracle's bit for bit after 20 steps."""
+    say(f"case: {model} two materials, cube 11^3, 20 steps")
+    w = configs.lattice_sum_cube(model=model, m=3, nc=11, field="linear", coeff=0.0, K=1e9, G=0.5e9,
+                                 fracture=True, G0=1.0)
+    x = w["x"]
+    rng = np.random.default_rng(4)
+    w["u"] = rng.uniform(-2e-3, 2e-3, size=x.shape) * (2.0 / 11)
+    w["rho"] = np.full(x.shape[0], 1000.0)
+    w["desc"].dt = 1e-6
+    w["materials"] = ([(1e9, 0.5e9, 1.0), (3e9, 1.2e9, 4.0)], rng.integers(0, 2, x.shape[0]).astype(np.int32))
+    g, o = pair(w)
+    g.step(20)
+    o.step(20)
+    assert broken_set(g) == broken_set(o)
+    for fld in ("u", "v", "f"):
+        assert_same(g.get(fld), o.get(fld), fld)
+    Wg, Dg = g.compute_energy()
+    Wo, Do = o.compute_energy()
+    assert_same(g.get("W"), o.get("W"), "W")
+    assert_same(g.get("phi"), o.get("phi"), "phi")
+    assert_total(Wg, Wo, "total strain energy")
+    g.compute_stress()
+    o.compute_stress()
+    assert_same(g.get("stress"), o.get("stress"), "stress")
+    say(f"  broken bonds {len(broken_set(g))}")
+    assert len(broken_set(g)) > 0
+
+
+def test_gpu_crack_inclusion_regression():
+    """regression_test/tstCrackInclusion.hpp:55 on its own deck (two
+    materials, LPS fracture, 661 steps): the crack-extent series kept on the
+    device equals the oracle's at every output step; the last value is
+    5.75e-3 +- 5e-5."""
+    from cabanapd_amd import _abi
+    say("case: crack inclusion 80x40x4, 661 steps FP64")
+    w = configs.crack_inclusion()
+    g, o = pair(w)
+    x = w["x"]
+    box = _abi.box_region(x.min(axis=0), x.max(axis=0))
+    crack = g.series(box, "x", component=1, reducer="max", capacity=16, gate="phi", threshold=0.3)
+    host = []
+    for k in range(1, w["steps"] + 1):
+        g.step(1)
+        o.step(1)
+        if k % w["output_every"] == 0:
+            g.compute_energy_async()
+            crack.update(k * w["desc"].dt)
+            o.compute_energy()
+            host.append(configs.kw_crack_y(x, o.get("phi")))
+    _, v = crack.values()
+    say(f"  crack y series {v.tolist()}")
+    assert np.array_equal(v, np.array(host))
+    assert abs(v[-1] - 5.75e-3) <= 5e-5
+    assert_same(g.get("u"), o.get("u"), "u")
+    assert broken_set(g) == broken_set(o)
+    crack.close()
+
+
+def test_gpu_multi_material_rejects_mixed():
+    from cabanapd_amd import engine
+    from tests.parity import ctx
+    w = configs.lattice_sum_cube(model="PMB", m=3, nc=7, field="linear", coeff=0.0, K=1e9, fracture=True, G0=1.0)
+    d = type(w["desc"]).from_buffer_copy(w["desc"])
+    d.precision = 1
+    s = engine.Spec(ctx(), d, w["x"].shape[0])
+    configs.upload(s, w)
+    with pytest.raises(engine.PdkError, match="FP64 only"):
+        s.set_materials([(1e9, 0.5e9, 1.0), (2e9, 0.5e9, 1.0)], np.zeros(w["x"].shape[0], dtype=np.int32))
+    s.close()

@@ -416,3 +416,52 @@ def test_stress_two_particle_closed_form():
     assert abs(sig[1, 0, 0] - 0.5 * c * s0 * vol[0] * dx) <= 1e-12 * abs(sig[1, 0, 0])
     assert np.count_nonzero(sig[:, ~np.eye(3, dtype=bool)]) == 0
     assert np.count_nonzero(sig[:, 1:, 1:]) == 0
+
+
+def multi_material_run(model, mats, mtype, steps=20):
+    w = configs.lattice_sum_cube(model=model, m=3, nc=9, field="linear", coeff=0.0, K=1e9, G=0.5e9,
+                                 fracture=True, G0=1.0)
+    x = w["x"]
+    rng = np.random.default_rng(3)
+    u = rng.uniform(-2e-3, 2e-3, size=x.shape) * (2.0 / 9)
+    s = orc.OracleSpec(w["desc"], x.shape[0], x, w["vol"], u=u, rho=np.full(x.shape[0], 1000.0))
+    s.build_bonds()
+    if mats is not None:
+        s.set_materials(mats, mtype)
+    w["desc"].dt = 1e-6
+    s.init()
+    s.step(steps)
+    out = (s.get("u"), s.get("f"), s.bonds()[2])
+    s.compute_energy()
+    out = out + (s.get("W"), s.get("phi"))
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_multi_material_identical_materials_equal_single(model):
+    # createMultiForceModel with two copies of one model: every pair model
+    # equals the single model (averages of equal values), so the run is the
+    # single-material run bit for bit
+    K, G, G0 = 1e9, 0.5e9, 1.0
+    n = 9 ** 3
+    mtype = np.random.default_rng(1).integers(0, 2, n).astype(np.int32)
+    ref = multi_material_run(model, None, None)
+    got = multi_material_run(model, [(K, G, G0), (K, G, G0)], mtype)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+    assert (ref[2] == 0).sum() > 0  # bonds break: the break test is exercised
+
+
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_multi_material_label_swap(model):
+    # relabelling the materials (swap the list and the types) changes nothing
+    mats = [(1e9, 0.5e9, 1.0), (3e9, 1.2e9, 4.0)]
+    n = 9 ** 3
+    mtype = np.random.default_rng(2).integers(0, 2, n).astype(np.int32)
+    a = multi_material_run(model, mats, mtype)
+    b = multi_material_run(model, mats[::-1], (1 - mtype).astype(np.int32))
+    for p, q in zip(a, b):
+        assert np.array_equal(p, q)
+    single = multi_material_run(model, None, None)
+    assert not np.array_equal(a[0], single[0])

@@ -4,6 +4,7 @@ regression_test/tstKalthoffWinkler.hpp:53  max damaged-particle y = 7.42e-2 +- 1
 regression_test/tstHertz.hpp:135           sqrt(KE_f / KE_i) = e +- 1e-3
 regression_test/tstHertzJKR.hpp:145        min pull-off force / f_c = -1 +- 5e-3
 regression_test/tstElasticWave.hpp:54-93   u_x profile vs polynomial fit: rel < 3.5 %, L2 < 3e-5
+regression_test/tstCrackInclusion.hpp:55   max damaged-particle y = 5.75e-3 +- 5e-5 (two materials)
 """
 import math
 
@@ -103,3 +104,31 @@ def test_oracle_elastic_wave_regression():
     assert k >= 20
     assert worst < fx["max_rel_err"], worst
     assert l2 < fx["max_l2_err"], l2
+
+
+def crack_inclusion_series(w):
+    s = oracle_from(w)
+    series = []
+    for k in range(1, w["steps"] + 1):
+        s.step(1)
+        if k % w["output_every"] == 0:
+            s.compute_energy()
+            series.append(configs.kw_crack_y(w["x"], s.get("phi")))
+    s.close()
+    return series
+
+
+def test_crack_inclusion_regression():
+    """regression_test/tstCrackInclusion.hpp:55 on its own deck: the crack
+    branches around the stiffer inclusion; the last output (step 650 of 661)
+    of the crack-extent series is 5.75e-3 +- 5e-5.  The same deck with one
+    material stays at the notch row (2.5e-4), so the value pins the averaged
+    cross-material bonds."""
+    w = configs.crack_inclusion()
+    assert w["steps"] == 661
+    series = crack_inclusion_series(w)
+    assert len(series) == 13
+    assert abs(series[-1] - 5.75e-3) <= 5e-5
+    w1 = configs.crack_inclusion()
+    w1["materials"] = None
+    assert abs(crack_inclusion_series(w1)[-1] - 5.75e-3) > 1e-3
