@@ -166,6 +166,50 @@ auto createBoundaryCondition( BCTag tag, const double value, ExecSpace exec, con
     return BoundaryCondition<BCTag>( tag, value, ParticleSteeringVector( exec, particles, regions... ) );
 }
 
+// Boundary condition with a user functor (pid, time) over the particles of
+// the regions (CabanaPD_Boundary.hpp:26-73, 177-185).  As for BodyTerm, the
+// engine applies a constant per-particle force term after the force pass
+// each step: the functor is evaluated once, at time 0, against zeroed force
+// storage on the region's particles, so it must add a time-independent force
+// (a traction such as f(pid, 1) += b0 * sign(y)).  Functors applied before
+// the force pass (force_update = false: displacement or velocity
+// conditions) are rejected.
+template <class UserFunctor>
+struct FunctorBoundaryCondition
+{
+    UserFunctor _user_functor;
+    ParticleSteeringVector _indices;
+    bool _force_update;
+    FunctorBoundaryCondition( ParticleSteeringVector idx, UserFunctor user, const bool force_update )
+        : _user_functor( user ), _indices( idx ), _force_update( force_update )
+    {
+    }
+    bool forceUpdate() const { return _force_update; }
+    double timeInit() const { return 0.0; }
+    std::vector<double> evaluate( ParticleStorage& s ) const
+    {
+        if ( !_force_update )
+            throw std::runtime_error( "boundary functors applied before the force pass (force_update = false) "
+                                      "are not supported by this build" );
+        std::vector<double> saved = s.f;
+        std::fill( s.f.begin(), s.f.end(), 0.0 );
+        for ( int64_t p : _indices._view )
+            _user_functor( static_cast<int>( p ), 0.0 );
+        std::vector<double> b = s.f;
+        s.f = saved;
+        return b;
+    }
+};
+
+template <class UserFunctor, class ExecSpace, class ParticlesT, class... RegionType,
+          std::enable_if_t<!std::is_arithmetic_v<ExecSpace>, int> = 0>
+auto createBoundaryCondition( UserFunctor user, ExecSpace exec, const ParticlesT& particles, const bool force_update,
+                              RegionType... regions )
+{
+    return FunctorBoundaryCondition<UserFunctor>( ParticleSteeringVector( exec, particles, regions... ), user,
+                                                  force_update );
+}
+
 // BodyTerm with a user functor (pid, time).  The engine applies a constant
 // per-particle body force density each step; the functor is evaluated once
 // against zeroed force storage to obtain it, so it must add a
@@ -207,6 +251,14 @@ struct is_body_term<BodyTerm<F>> : std::true_type
 };
 template <class T>
 struct is_boundary_condition : std::false_type
+{
+};
+template <class F>
+struct is_body_term<FunctorBoundaryCondition<F>> : std::true_type
+{
+};
+template <class F>
+struct is_boundary_condition<FunctorBoundaryCondition<F>> : std::true_type
 {
 };
 template <class T>

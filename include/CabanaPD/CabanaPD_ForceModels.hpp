@@ -13,8 +13,10 @@
 #ifndef CABANAPD_DROPIN_FORCEMODELS_HPP
 #define CABANAPD_DROPIN_FORCEMODELS_HPP
 
+#include <cmath>
 #include <stdexcept>
 #include <type_traits>
+#include <vector>
 
 #include <pdk.h>
 
@@ -407,6 +409,83 @@ template <class T>
 struct is_contact : std::integral_constant<bool, std::is_base_of_v<ContactModel, T>>
 {
 };
+
+// ---- multi-material models (CabanaPD_ForceModelsMulti.hpp:321-516) ------------
+// createMultiForceModel( particles, AverageTag{}, m_0, .., m_{n-1} ), 2 to 4
+// models of one model kind, fracture form and horizon: particle p uses the
+// model of material type(p) (Particles::sliceType), a bond between different
+// materials the averaged model.  The engine builds the pair tables
+// (pdk_set_materials); the explicit-indexing overloads of the reference
+// (createMultiForceModel( particles, indexing, models... )) are not provided.
+struct AverageTag
+{
+};
+
+template <class M0, class... Ms>
+struct ForceModels
+{
+    static_assert( sizeof...( Ms ) >= 1 && sizeof...( Ms ) <= 3, "2 to 4 materials" );
+    static_assert( ( std::is_same_v<typename M0::force_tag, typename Ms::force_tag> && ... ),
+                   "All forces need the same base type" );
+    static_assert( ( std::is_same_v<typename M0::fracture_type, typename Ms::fracture_type> && ... ),
+                   "this build needs one fracture form for all materials" );
+    using model_tag = typename M0::model_tag;
+    using force_tag = typename M0::force_tag;
+    using base_model = typename M0::base_model;
+    using fracture_type = typename M0::fracture_type;
+    using mechanics_type = Elastic;
+    using material_type = MultiMaterial;
+    using thermal_type = TemperatureIndependent;
+    double force_horizon = 0.0;
+    double K = 0.0;
+    std::vector<pdk_material> materials;
+    M0 first;
+
+    ForceModels( const M0& m0, const Ms&... ms ) : force_horizon( m0.force_horizon ), K( m0.K ), first( m0 )
+    {
+        add( m0 );
+        ( add( ms ), ... );
+    }
+    auto cutoff() const { return force_horizon; }
+    void updateBonds( const int, const int ) {}
+    pdk_model_desc desc( double dt ) const { return first.desc( dt ); }
+
+  private:
+    template <class M>
+    void add( const M& m )
+    {
+        // CabanaPD_ForceModelsMulti.hpp:295-311: equal horizons required
+        if ( std::abs( m.force_horizon - force_horizon ) > 1e-10 )
+            throw std::runtime_error( "Horizon for each model must match for multi-material systems." );
+        materials.push_back( pdk_material{ m.K, m.G, m.G0 } );
+    }
+};
+
+template <class M0, class... Ms>
+struct is_fracture<ForceModels<M0, Ms...>> : std::is_same<typename M0::fracture_type, Fracture>
+{
+};
+
+template <class ParticleType, class M0, class... Ms>
+auto createMultiForceModel( ParticleType, AverageTag, M0 m0, Ms... ms )
+{
+    return ForceModels<M0, Ms...>( m0, ms... );
+}
+
+namespace detail
+{
+// Materials of a model for pdk_set_materials (empty: one material).
+template <class Model>
+std::vector<pdk_material> materialsOf( const Model& )
+{
+    return {};
+}
+template <class M0, class... Ms>
+std::vector<pdk_material> materialsOf( const ForceModels<M0, Ms...>& m )
+{
+    return m.materials;
+}
+} // namespace detail
 
 } // namespace CabanaPD
 

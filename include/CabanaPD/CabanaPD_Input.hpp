@@ -398,11 +398,30 @@ class Inputs
     // Critical time step (CabanaPD_Input.hpp:217-336, 385-399): warns when
     // dt exceeds the estimate and stores it.
     template <class ForceModel>
-    void computeCriticalTimeStep( const ForceModel& model )
+    void computeCriticalTimeStep( const ForceModel& )
     {
+        // Inputs::computeCriticalTimeStep (CabanaPD_Input.hpp:217-300): K from
+        // bulk_modulus, else from elastic_modulus with nu = 0.25; scalars or
+        // one value per material, the material of least rho / K decides.
         const json::Value& dxv = inputs.at( "dx" ).at( "value" );
-        double K = model.K;
-        double rho = ( *this )["density"];
+        std::vector<double> Kv;
+        if ( inputs.contains( "bulk_modulus" ) )
+            Kv = getVector( "bulk_modulus" );
+        else
+        {
+            Kv = getVector( "elastic_modulus" );
+            for ( double& k : Kv )
+                k = k / ( 3.0 * ( 1.0 - 2.0 * 0.25 ) );
+        }
+        const std::vector<double> rhov = getVector( "density" );
+        if ( rhov.empty() || rhov.size() != Kv.size() )
+            throw std::runtime_error( "density and moduli need one value per material" );
+        std::size_t mi = 0;
+        for ( std::size_t t = 1; t < rhov.size(); t++ )
+            if ( rhov[t] / Kv[t] < rhov[mi] / Kv[mi] )
+                mi = t;
+        double K = Kv[mi];
+        double rho = rhov[mi];
         double horizon = ( *this )["horizon"];
         double safety = ( *this )["timestep_safety_factor"];
         double dt_crit = 0.0;
@@ -416,6 +435,19 @@ class Inputs
     }
 
   private:
+    // A scalar input or each entry of an array input.
+    std::vector<double> getVector( const std::string& key ) const
+    {
+        const json::Value& v = inputs.at( key ).at( "value" );
+        std::vector<double> out;
+        if ( v.is_array() )
+            for ( std::size_t i = 0; i < v.arr.size(); i++ )
+                out.push_back( double( v.arr[i] ) );
+        else
+            out.push_back( double( v ) );
+        return out;
+    }
+
     void setupSize()
     {
         if ( inputs.contains( "system_size" ) )

@@ -59,14 +59,20 @@ struct DeviceSpec
             (void)pdk_spec_destroy( spec );
     }
 
-    // Particles::create + update -> device; then the Neighbor build.
-    void attach( const pdk_model_desc& d, ParticleStorage& s )
+    // Particles::create + update -> device; materials of a multi-material
+    // model (Particles::sliceType); then the Neighbor build.
+    void attach( const pdk_model_desc& d, ParticleStorage& s, const std::vector<pdk_material>& mats = {} )
     {
         lps = d.model == PDK_MODEL_LPS || d.model == PDK_MODEL_LINEAR_LPS;
         fracture = d.fracture != 0;
         check( pdk_spec_create( context(), &d, static_cast<int64_t>( s.n ), &spec ) );
         check( pdk_spec_upload( spec, s.x.data(), s.u.data(), s.v.data(), s.vol.data(), s.rho.data(),
                                 s.nofail.data(), nullptr ) );
+        if ( !mats.empty() )
+        {
+            std::vector<int32_t> type( s.type.begin(), s.type.end() );
+            check( pdk_set_materials( spec, static_cast<int32_t>( mats.size() ), mats.data(), type.data() ) );
+        }
         check( pdk_build_bonds( spec ) );
     }
 
@@ -119,7 +125,7 @@ class Neighbor
     {
         pdk_model_desc d = model.desc( dt );
         d.neighbor_tol = tol;
-        _dev->attach( d, particles.storage() );
+        _dev->attach( d, particles.storage(), detail::materialsOf( model ) );
     }
     unsigned getMaxLocal() const
     {

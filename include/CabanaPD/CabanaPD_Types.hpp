@@ -96,6 +96,9 @@ struct TemperatureIndependent
 struct SingleMaterial
 {
 };
+struct MultiMaterial
+{
+};
 struct Contact
 {
     using base_type = Contact;

@@ -9,7 +9,9 @@ examples/mechanics/elastic_wave.cpp on the deck of
 regression_test/tstElasticWave.hpp; its displacement profile is checked
 against that test's polynomial fit and against the oracle.  dropin_force runs
 the tstForce flow (Force, Neighbor, computeForce, computeEnergy) against
-lattice sums.
+lattice sums.  dropin_crack_inclusion is examples/mechanics/crack_inclusion.cpp
+(two materials through createMultiForceModel, a functor force boundary
+condition) checking regression_test/tstCrackInclusion.hpp:55.
 
 The drivers are built on the CPU beforehand (`make -C tests/cpp`, also done by
 __graft_entry__.build()); the test builds them only when they are missing.
@@ -81,3 +83,9 @@ def test_dropin_elastic_wave_example(tmp_path):
     a, b = np.argsort(xs), np.argsort(xo)
     assert np.allclose(xs[a], xo[b], rtol=1e-5, atol=1e-7)
     assert np.max(np.abs(us[a] - uo[b])) <= 1e-5 * np.max(np.abs(uo))
+
+
+def test_dropin_crack_inclusion_example(tmp_path):
+    deck = os.path.join(HERE, "golden", "crack_inclusion.json")
+    rc, out = run([driver("dropin_crack_inclusion"), deck, str(tmp_path / "incl_crack_y.txt")], str(tmp_path), 110)
+    assert rc == 0 and "CI_REGRESSION_OK" in out
