@@ -13,8 +13,12 @@ PDK_MIXED = 1
 
 FIELD = {
     "x": 0, "u": 1, "v": 2, "f": 3, "vol": 4, "rho": 5,
-    "W": 6, "phi": 7, "m": 8, "theta": 9, "stress": 10,
+    "W": 6, "phi": 7, "m": 8, "theta": 9, "stress": 10, "temperature": 11,
 }
+PDK_MECH_ELASTIC = 0
+PDK_MECH_EPP = 1
+PDK_TEMP_INDEPENDENT = 0
+PDK_TEMP_DEPENDENT = 1
 VECTOR_FIELDS = {"x", "u", "v", "f"}
 
 
@@ -110,6 +114,11 @@ class ModelDesc(C.Structure):
         ("polydisperse", C.c_int32),
         ("ordering", C.c_int32),
         ("bond_layout", C.c_int32),
+        ("mechanics", C.c_int32),
+        ("thermal", C.c_int32),
+        ("sigma_y", C.c_double),
+        ("alpha", C.c_double),
+        ("temp0", C.c_double),
     ]
 
 
@@ -136,18 +145,22 @@ class ModelConstants(C.Structure):
     _fields_ = [(name, C.c_double) for name in (
         "c", "s0", "bond_break_coeff", "theta_coeff", "s_coeff",
         "Rs", "Es", "Gs", "beta", "coeff_h_n", "coeff_h_d", "Gamma", "a0",
-        "delta_tear", "fc")]
+        "delta_tear", "fc", "s_Y")]
 
 
 def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
               G=0.0, G0=0.0, dt=1e-7, influence=None, neighbor_tol=1e-14,
               radius=0.0, radius_extend=0.0, nu=0.25, E=0.0, restitution=1.0,
-              gamma=0.0, mu_friction=0.0, rebuild_every=0, polydisperse=False, ordering=0, bond_layout=0):
+              gamma=0.0, mu_friction=0.0, rebuild_every=0, polydisperse=False, ordering=0, bond_layout=0,
+              mechanics="elastic", sigma_y=0.0, thermal=False, alpha=0.0, temp0=0.0):
     """Model description with the reference's constructor defaults.
 
     influence defaults: PMB fracture s0 uses influence_type 1
     (CabanaPD_ForceModels.hpp:93-101), LPS uses 0
-    (force_models/CabanaPD_LPS.hpp:46-47)."""
+    (force_models/CabanaPD_LPS.hpp:46-47).  mechanics "epp" selects
+    ElasticPerfectlyPlastic (PMB with fracture, yield stress sigma_y);
+    thermal=True selects TemperatureDependent (expansion alpha, reference
+    temperature temp0)."""
     kinds = {"PMB": PDK_MODEL_PMB, "LPS": PDK_MODEL_LPS, "LINEAR_PMB": PDK_MODEL_LINEAR_PMB,
              "LINEAR_LPS": PDK_MODEL_LINEAR_LPS,
              "HERTZ": PDK_MODEL_HERTZ, "HERTZ_JKR": PDK_MODEL_HERTZ_JKR}
@@ -176,4 +189,9 @@ def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
     d.polydisperse = 1 if polydisperse else 0
     d.ordering = ordering
     d.bond_layout = bond_layout
+    d.mechanics = {"elastic": PDK_MECH_ELASTIC, "epp": PDK_MECH_EPP}[mechanics.lower()]
+    d.sigma_y = sigma_y
+    d.thermal = PDK_TEMP_DEPENDENT if thermal else PDK_TEMP_INDEPENDENT
+    d.alpha = alpha
+    d.temp0 = temp0
     return d

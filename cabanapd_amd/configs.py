@@ -6,6 +6,8 @@ Every generator returns a dict with keys
   nofail (n,) or None, radius (n,) or None,
   notches: (p0, v1, v2) arrays or None,
   bc: (kind, idx, value) or None, body (n,3) or None,
+  prescribed: list of (field, idx, a, b) set before each force pass,
+  temperature (n,) initial temperature or None,
   steps, output_every, name, dx.
 Positions are in the caller order the reference's grid creation produces:
 lexicographic with x fastest.
@@ -47,7 +49,8 @@ def _base(desc, x, vol, rho, name, dx, steps, output_every=0):
     n = x.shape[0]
     return dict(desc=desc, x=x, vol=vol, rho=np.full(n, rho) if np.isscalar(rho) else rho,
                 u=np.zeros((n, 3)), v=np.zeros((n, 3)), nofail=None, radius=None, notches=None,
-                bc=None, body=None, steps=steps, output_every=output_every, name=name, dx=dx)
+                bc=None, body=None, prescribed=[], temperature=None, steps=steps, output_every=output_every,
+                name=name, dx=dx)
 
 
 def elastic_wave(precision="fp64"):
@@ -222,6 +225,85 @@ def crack_inclusion(precision="fp64"):
     sel = in1 | in2
     body[sel, 1] = b0 * (np.abs(x[sel, 1]) / x[sel, 1])
     w["body"] = body
+    return w
+
+
+def dogbone_inside(x, low, high, gage=0.050, width=0.013, fillet=0.076):
+    """The ASTM D638 type I specimen of examples/mechanics/dogbone_tensile_test.cpp:71-139:
+    outside the gage section's width band and outside the four fillet circles
+    (strict comparisons, as there)."""
+    midx = 0.5 * (low[0] + high[0])
+    midy = 0.5 * (low[1] + high[1])
+    rsq = fillet * fillet
+    xc_l, xc_r = midx - 0.5 * gage, midx + 0.5 * gage
+    yc_b, yc_t = midy - 0.5 * width - fillet, midy + 0.5 * width + fillet
+    keep = ~((np.abs(x[:, 0] - midx) < 0.5 * gage) & (np.abs(x[:, 1] - midy) > 0.5 * width))
+    for xc, yc in ((xc_l, yc_b), (xc_r, yc_b), (xc_l, yc_t), (xc_r, yc_t)):
+        keep &= ~(np.abs(x[:, 0] - xc) * np.abs(x[:, 0] - xc) + np.abs(x[:, 1] - yc) * np.abs(x[:, 1] - yc) < rsq)
+    return keep
+
+
+def dogbone(grip_velocity=0.1, steps=None, num_cells=(251, 29, 5)):
+    """examples/mechanics/inputs/dogbone_tensile_test.json run through
+    dogbone_tensile_test.cpp: PMB ElasticPerfectlyPlastic with fracture,
+    E 3 GPa, nu 0.25 (K = E / (3 (1 - 2 nu))), G0 3000, sigma_y 30 MPa,
+    rho 1200, horizon 0.002 (+1e-10), dt 3e-7, final time 0.015
+    (int(0.015 / 3e-7) = 50000 steps); 251x29x5 cells of 0.165 x 0.019 x
+    0.0032 cut to the dogbone.  Grips (inclusive boxes beyond the grip
+    distance 0.115 about the centre): u = (v0 t, 0, 0) on the right grip and
+    u = 0 on the left, set after each initial half step (force_update =
+    false).  The example seeds the particles randomly within their cells
+    (Cabana::InitRandom); here they sit at the cell centres, so the specimen
+    is the same shape but not the same particle set."""
+    size = np.array([0.165, 0.019, 0.0032])
+    low, high = -0.5 * size, 0.5 * size
+    x, vol, d = grid_cell_centred(low, high, num_cells)
+    keep = dogbone_inside(x, low, high)
+    x, vol = x[keep], vol[keep]
+    E, nu = 3e9, 0.25
+    K = E / (3 * (1 - 2 * nu))
+    horizon = 0.002 + 1e-10
+    dt = 3.0e-7
+    if steps is None:
+        steps = int(1.5e-2 / dt)
+    desc = make_desc("PMB", fracture=True, horizon=horizon, K=K, G0=3000.0, dt=dt, mechanics="epp",
+                     sigma_y=30e6)
+    w = _base(desc, x, vol, 1200.0, "dogbone_%dx%dx%d" % tuple(num_cells), d[0], steps, output_every=200)
+    midx = 0.5 * (low[0] + high[0])
+    D = 0.115
+    right = (x[:, 0] >= midx + 0.5 * D) & (x[:, 0] <= high[0])
+    left = (x[:, 0] >= low[0]) & (x[:, 0] <= midx - 0.5 * D)
+    grips = np.flatnonzero(right | left)
+    a = np.zeros((grips.size, 3))
+    b = np.zeros((grips.size, 3))
+    b[right[grips], 0] = grip_velocity
+    w["prescribed"] = [("u", grips, a, b)]
+    w["right_grip"] = np.flatnonzero(right)
+    return w
+
+
+def thermal_deformation(steps=None):
+    """examples/thermomechanics/inputs/thermal_deformation.json run through
+    thermal_deformation.cpp: PMB NoFracture TemperatureDependent, E 370 GPa,
+    nu 0.25, rho 3980, alpha 7.5e-6, temp0 20, horizon 0.03 (+1e-10),
+    dt 7.5e-7, final time 0.01 (int(0.01 / 7.5e-7) = 13333 steps); 101x31x3
+    cells of 1.0 x 0.3 x 0.03.  Temperature imposed on every particle before
+    each force pass (BodyTerm with force_update = false):
+    T = temp0 + 5000 (y - y_low) t."""
+    size = np.array([1.0, 0.3, 0.03])
+    low, high = -0.5 * size, 0.5 * size
+    x, vol, d = grid_cell_centred(low, high, (101, 31, 3))
+    E, nu = 370e9, 0.25
+    K = E / (3 * (1 - 2 * nu))
+    horizon = 0.03 + 1e-10
+    dt = 7.5e-7
+    if steps is None:
+        steps = int(0.01 / dt)
+    temp0 = 20.0
+    desc = make_desc("PMB", horizon=horizon, K=K, dt=dt, thermal=True, alpha=7.5e-6, temp0=temp0)
+    w = _base(desc, x, vol, 3980.0, "thermal_deformation_101x31x3", d[0], steps, output_every=100)
+    n = x.shape[0]
+    w["prescribed"] = [("temperature", np.arange(n), np.full(n, temp0), 5000.0 * (x[:, 1] - low[1]))]
     return w
 
 
@@ -417,5 +499,9 @@ def setup(spec, w, init=True):
         spec.set_bc(*w["bc"])
     if w["body"] is not None:
         spec.set_body_force(w["body"])
+    if w.get("temperature") is not None:
+        spec.set_field("temperature", w["temperature"])
+    for field, idx, a, b in w.get("prescribed", []):
+        spec.set_prescribed(field, idx, a, b)
     if init:
         spec.init()

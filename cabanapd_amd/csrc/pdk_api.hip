@@ -92,7 +92,39 @@ void modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
         k.s_coeff = 15.0 * d.G;
         if ( isLpsModel( d.model ) )
             require( d.influence == 0 || d.influence == 1, "Influence function type must be 0 or 1." );
+        require( d.mechanics == PDK_MECH_ELASTIC || d.mechanics == PDK_MECH_EPP, "unknown mechanics" );
+        require( d.thermal == PDK_TEMP_INDEPENDENT || d.thermal == PDK_TEMP_DEPENDENT, "unknown thermal type" );
+        if ( d.mechanics == PDK_MECH_EPP )
+        {
+            // ForceModel<PMB, ElasticPerfectlyPlastic, Fracture> only
+            // (force_models/CabanaPD_PMB.hpp:232-284; tstForce.hpp:1048)
+            require( d.model == PDK_MODEL_PMB && d.fracture, "ElasticPerfectlyPlastic exists for PMB with fracture only" );
+            require( d.sigma_y > 0, "ElasticPerfectlyPlastic needs sigma_y > 0" );
+            // force_models/CabanaPD_PMB.hpp:90-96
+            k.s_Y = d.sigma_y / 3.0 / K;
+            if ( d.thermal != PDK_TEMP_DEPENDENT )
+            {
+                // force_models/CabanaPD_PMB.hpp:255-264
+                k.s0 = ( 5.0 * d.G0 / d.sigma_y / delta + d.sigma_y / K ) / 6.0;
+                k.bond_break_coeff = ( 1.0 + k.s0 ) * ( 1.0 + k.s0 );
+            }
+        }
+        if ( d.thermal == PDK_TEMP_DEPENDENT )
+        {
+            // PMB and LinearPMB (force_models/CabanaPD_PMB.hpp:306-568, 575-597)
+            require( d.model == PDK_MODEL_PMB || d.model == PDK_MODEL_LINEAR_PMB,
+                     "TemperatureDependent exists for the PMB models only" );
+            if ( d.fracture )
+            {
+                // ThermalFractureModel: BaseFractureModel( delta, K, G0 ),
+                // influence_type 1 (CabanaPD_ForceModels.hpp:228-236, 93-102)
+                k.s0 = std::sqrt( 5.0 * d.G0 / 9.0 / K / delta );
+                k.bond_break_coeff = ( 1.0 + k.s0 ) * ( 1.0 + k.s0 );
+            }
+        }
     }
+    else if ( d.mechanics != PDK_MECH_ELASTIC || d.thermal != PDK_TEMP_INDEPENDENT )
+        throw Error( "the contact models are elastic and temperature independent" );
     else if ( d.model == PDK_MODEL_HERTZ )
     {
         require( d.radius > 0 && d.E > 0 && d.restitution > 0, "Hertz needs radius, E, restitution > 0" );
@@ -154,7 +186,18 @@ struct pdk_spec_s
     int64_t n = 0, nslices = 0;
     hipStream_t st = nullptr;
     DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
-        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stress, mtype;
+        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stress, mtype, splas,
+        temp;
+    // Prescribed values (force_update == false conditions): members (device
+    // ids), a and b ([count][ncomp]); one set for U and one for TEMPERATURE.
+    struct Prescribed
+    {
+        DevBuf idx, a, b;
+        int64_t count = 0;
+        int32_t ncomp = 0;
+        bool has_b = false;
+    };
+    Prescribed pre_u, pre_temp;
     // multi-material (pdk_set_materials): materials and pair tables
     int32_t nmat = 1;
     double mat_c[16] = {}, mat_bcc[16] = {}, mat_tc[4] = {}, mat_sc[4] = {};
@@ -196,6 +239,24 @@ struct pdk_spec_s
     bool in_batch = false;
 
     MemAccount* acct() { return &ctx->acct; }
+
+    // Specs whose force passes need the FP64 gather variant: several
+    // materials, EPP (per-bond state) or a temperature field.
+    bool gatherOnly() const
+    {
+        return nmat > 1 || d.mechanics == PDK_MECH_EPP || d.thermal == PDK_TEMP_DEPENDENT;
+    }
+    bool hasPrescribed() const { return pre_u.count > 0 || pre_temp.count > 0; }
+    // The prescribed values at time t, before a force pass.
+    void applyPrescribed( double t )
+    {
+        if ( pre_u.count )
+            launchPrescribed( pre_u.idx.as<int64_t>(), pre_u.count, pre_u.a.as<double>(),
+                              pre_u.has_b ? pre_u.b.as<double>() : nullptr, 3, t, Ucur(), n, st );
+        if ( pre_temp.count )
+            launchPrescribed( pre_temp.idx.as<int64_t>(), pre_temp.count, pre_temp.a.as<double>(),
+                              pre_temp.has_b ? pre_temp.b.as<double>() : nullptr, 1, t, temp.as<double>(), n, st );
+    }
 
     // Two consecutive middle steps of pdk_step captured as one HIP graph per
     // starting displacement buffer (after two steps the buffer index is back
@@ -294,9 +355,14 @@ struct pdk_spec_s
         // of 4); multi-material specimens use the gather variant
         h.unroll = unroll;
         if ( h.unroll < 0 )
-            h.unroll = ( h.nrows > 0 && nmat == 1 ) ? 0 : 4;
-        if ( h.unroll == 0 && ( h.nrows == 0 || nmat > 1 ) )
+            h.unroll = ( h.nrows > 0 && !gatherOnly() ) ? 0 : 4;
+        if ( h.unroll == 0 && ( h.nrows == 0 || gatherOnly() ) )
             h.unroll = 4;
+        h.splas = splas ? splas.as<double>() : nullptr;
+        h.sY = k.s_Y;
+        h.temp = temp ? temp.as<double>() : nullptr;
+        h.alpha = d.alpha;
+        h.temp0 = d.temp0;
         h.nmat = nmat;
         h.mtype = nmat > 1 ? mtype.as<uint8_t>() : nullptr;
         for ( int t = 0; t < 16; t++ )
@@ -433,6 +499,7 @@ struct pdk_spec_s
         case PDK_FIELD_PHI: return phi.as<double>();
         case PDK_FIELD_M: return m.as<double>();
         case PDK_FIELD_THETA: return theta.as<double>();
+        case PDK_FIELD_TEMPERATURE: return temp.as<double>();
         default: throw Error( "unknown field" );
         }
     }
@@ -574,6 +641,9 @@ int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spe
         require( desc->precision == PDK_FP64 || desc->precision == PDK_MIXED, "unknown precision" );
         require( !( isContact( *desc ) && desc->precision != PDK_FP64 ), "contact models run in FP64" );
         require( desc->dt > 0, "dt must be positive" );
+        require( ( desc->mechanics == PDK_MECH_ELASTIC && desc->thermal == PDK_TEMP_INDEPENDENT ) ||
+                     desc->precision == PDK_FP64,
+                 "ElasticPerfectlyPlastic and TemperatureDependent specimens are FP64 only" );
         PDK_HIP( hipSetDevice( ctx->device ) );
         auto s = std::make_unique<pdk_spec_s>();
         s->ctx = ctx;
@@ -608,6 +678,12 @@ int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spe
         s->slice_ptr.alloc( ( s->nslices + 1 ) * sizeof( int64_t ), a );
         s->partial.alloc( std::max<int64_t>( 2 * ( s->nslices / kSlicesPerBlock + 2 ), 64 ) * sizeof( double ), a );
         s->args.alloc( sizeof( PDArgs ), a );
+        if ( desc->thermal == PDK_TEMP_DEPENDENT )
+        {
+            // zero at creation (CabanaPD_Particles.hpp:860-864)
+            s->temp.alloc( d1, a );
+            PDK_HIP( hipMemsetAsync( s->temp.as<double>(), 0, d1, s->st ) );
+        }
         if ( isContact( *desc ) )
         {
             s->u_neigh.alloc( d3, a );
@@ -803,6 +879,11 @@ int pdk_spec_upload( pdk_spec s, const double* x, const double* u, const double*
                 PDK_HIP( hipMemsetAsync( b->as<double>(), 0, n * sizeof( double ), s->st ) );
         if ( s->u_neigh )
             PDK_HIP( hipMemsetAsync( s->u_neigh.as<double>(), 0, 3 * n * sizeof( double ), s->st ) );
+        if ( s->temp )
+            PDK_HIP( hipMemsetAsync( s->temp.as<double>(), 0, n * sizeof( double ), s->st ) );
+        // prescribed members were mapped to the previous device order
+        s->pre_u = pdk_spec_s::Prescribed{};
+        s->pre_temp = pdk_spec_s::Prescribed{};
         s->cur = 0;
         s->uploaded = true;
         s->built = false;
@@ -904,6 +985,14 @@ int pdk_build_bonds( pdk_spec s )
             }
         }
         s->refreshStats();
+        if ( s->d.mechanics == PDK_MECH_EPP )
+        {
+            // BasePlasticity::updateBonds after the neighbor list: zero s_p
+            // per bond (CabanaPD_Solver.hpp:183-185, CabanaPD_ForceModels.hpp:157-161)
+            const size_t b = static_cast<size_t>( std::max<int64_t>( s->total_rows, 1 ) ) * kSlice * sizeof( double );
+            s->splas.alloc( b, s->acct() );
+            PDK_HIP( hipMemsetAsync( s->splas.as<double>(), 0, b, s->st ) );
+        }
         s->built = true;
         s->inited = false;
         s->pushArgs();
@@ -1052,6 +1141,80 @@ int pdk_set_body_force( pdk_spec s, const double* b )
         else
             s->body.release();
         s->pushArgs();
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+namespace
+{
+int32_t prescribedComponents( pdk_spec s, int32_t field )
+{
+    if ( field == PDK_FIELD_U )
+        return 3;
+    require( field == PDK_FIELD_TEMPERATURE, "prescribed values are for PDK_FIELD_U or PDK_FIELD_TEMPERATURE" );
+    require( s->d.thermal == PDK_TEMP_DEPENDENT, "the spec has no temperature field (TemperatureIndependent)" );
+    return 1;
+}
+} // namespace
+
+int pdk_set_prescribed( pdk_spec s, int32_t field, const int64_t* idx, int64_t count, const double* a,
+                        const double* b )
+{
+    return guarded( [&] {
+        require( s && s->uploaded, "spec not uploaded" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( !isContact( s->d ), "prescribed values are for the PD models" );
+        const int32_t nc = prescribedComponents( s, field );
+        require( count >= 0 && ( count == 0 || ( idx && a ) ), "invalid prescribed arguments" );
+        auto& pr = field == PDK_FIELD_U ? s->pre_u : s->pre_temp;
+        pr.count = 0;
+        if ( count == 0 )
+            return;
+        std::vector<int64_t> dev( count );
+        std::vector<uint8_t> seen( s->n, 0 );
+        for ( int64_t m = 0; m < count; m++ )
+        {
+            require( idx[m] >= 0 && idx[m] < s->n, "prescribed particle id out of range" );
+            require( !seen[idx[m]], "prescribed particle ids are repeated" );
+            seen[idx[m]] = 1;
+            dev[m] = s->h_inv[idx[m]];
+        }
+        // Buffers are reused while they are large enough (a caller may set
+        // new values every step); copies are ordered on the spec's stream
+        // after the launches that read the previous values.
+        const size_t vb = static_cast<size_t>( count ) * nc * sizeof( double );
+        const size_t ib = static_cast<size_t>( count ) * sizeof( int64_t );
+        auto ensure = [&]( DevBuf& buf, size_t bytes ) {
+            if ( buf.bytes() < bytes )
+            {
+                PDK_HIP( hipStreamSynchronize( s->st ) ); // the old buffer may still be read
+                buf.alloc( bytes, s->acct() );
+            }
+        };
+        ensure( pr.idx, ib );
+        ensure( pr.a, vb );
+        PDK_HIP( hipMemcpyAsync( pr.idx.as<int64_t>(), dev.data(), ib, hipMemcpyHostToDevice, s->st ) );
+        PDK_HIP( hipMemcpyAsync( pr.a.as<double>(), a, vb, hipMemcpyHostToDevice, s->st ) );
+        pr.has_b = b != nullptr;
+        if ( b )
+        {
+            ensure( pr.b, vb );
+            PDK_HIP( hipMemcpyAsync( pr.b.as<double>(), b, vb, hipMemcpyHostToDevice, s->st ) );
+        }
+        pr.count = count;
+        pr.ncomp = nc;
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_set_field( pdk_spec s, int32_t field, const double* values )
+{
+    return guarded( [&] {
+        require( s && s->uploaded && values, "spec not uploaded" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( field == PDK_FIELD_TEMPERATURE, "pdk_set_field sets PDK_FIELD_TEMPERATURE" );
+        require( s->d.thermal == PDK_TEMP_DEPENDENT, "the spec has no temperature field (TemperatureIndependent)" );
+        s->uploadScalar( values, s->temp.as<double>() );
         PDK_HIP( hipStreamSynchronize( s->st ) );
     } );
 }
@@ -1216,6 +1379,7 @@ int pdk_init( pdk_spec s )
         require( s && s->built, "bonds not built" );
         require( !s->in_batch, "spec is running in the batch engine" );
         // Solver::init(bc) (CabanaPD_Solver.hpp:229-279)
+        s->applyPrescribed( 0.0 );
         if ( isContact( s->d ) )
         {
             contactForceStore( s );
@@ -1255,10 +1419,14 @@ int pdk_step( pdk_spec s, int32_t nsteps )
             // the previous step's force epilogue.
             s->timed( kClsOther, [&] { launchInitialHalfStep( s->dargs(), s->h, s->cur, s->st ); } );
             s->cur ^= 1;
+            // prescribed values of step g at t = g dt, after its initial half step
+            const bool pre = s->hasPrescribed();
+            if ( pre )
+                s->applyPrescribed( static_cast<double>( s->steps_done + 1 ) * s->d.dt );
             int32_t k = 1;
             // middle steps in pairs from a captured graph (not while kernels
             // are being timed: events are recorded around each launch)
-            if ( s->use_graph && !s->timing )
+            if ( s->use_graph && !s->timing && !pre )
                 for ( ; k + 1 < nsteps; k += 2 )
                     PDK_HIP( hipGraphLaunch( s->stepPairGraph( s->cur ), s->st ) );
             for ( ; k <= nsteps; k++ )
@@ -1268,7 +1436,11 @@ int pdk_step( pdk_spec s, int32_t nsteps )
                 const int32_t epi = k < nsteps ? kEpiFinalInitial : kEpiFinal;
                 s->timed( kClsForce, [&] { launchForce( s->dargs(), s->h, epi, s->cur, s->st ); } );
                 if ( epi == kEpiFinalInitial )
+                {
                     s->cur ^= 1;
+                    if ( pre )
+                        s->applyPrescribed( static_cast<double>( s->steps_done + k + 1 ) * s->d.dt );
+                }
             }
         }
         s->steps_done += nsteps;
@@ -1525,9 +1697,10 @@ int pdk_set_tuning( pdk_spec s, int32_t unroll )
         require( s != nullptr, "null spec" );
         require( unroll == 0 || unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8,
                  "unroll must be 0 (row windows), 1, 2, 4 or 8" );
+        require( unroll != 0 || !s->gatherOnly(),
+                 "multi-material, EPP and temperature-dependent specimens use the gather variant" );
         require( unroll != 0 || !s->built || s->slot.nrows > 0,
                  "row windows need the stencil-slot layout of a full lattice" );
-        require( unroll != 0 || s->nmat == 1, "multi-material specimens use the gather variant" );
         s->unroll = unroll;
         if ( s->uploaded )
         {
@@ -1555,7 +1728,7 @@ int pdk_autotune( pdk_spec s, int32_t launches, int32_t* unroll_out )
         float best = 1e30f;
         int32_t best_u = s->unroll;
         std::vector<int32_t> cand;
-        if ( s->slot.nrows > 0 && s->nmat == 1 )
+        if ( s->slot.nrows > 0 && !s->gatherOnly() )
             cand.push_back( 0 ); // row windows staged in LDS
         for ( int32_t u : { 1, 2, 4, 8 } )
             cand.push_back( u );
@@ -1719,7 +1892,9 @@ int pdk_submit( pdk_ctx ctx, pdk_spec s, int32_t nsteps, uint64_t* ticket )
     return guarded( [&] {
         require( ctx && s && ticket, "null argument" );
         require( s->ctx == ctx, "spec belongs to another context" );
-        require( s->nmat == 1, "multi-material specimens are stepped with pdk_step, not the batch engine" );
+        require( !s->gatherOnly(),
+                 "multi-material, EPP and temperature-dependent specimens are stepped with pdk_step, not the batch engine" );
+        require( !s->hasPrescribed(), "specimens with prescribed values are stepped with pdk_step, not the batch engine" );
         *ticket = ctx->batch->submit( s, nsteps );
     } );
 }

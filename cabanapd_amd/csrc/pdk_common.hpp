@@ -120,6 +120,15 @@ struct PDArgs
     const uint8_t* mtype;
     int32_t nmat;
     double mat_c[16], mat_bcc[16], mat_tc[4], mat_sc[4];
+    // PMB family extensions (FP64 gather passes).  EPP: plastic stretch of
+    // bond slot q of lane L of a slice at splas[(slice_ptr[slice] + q) *
+    // kSlice + L] (null: Elastic) and the yield stretch sY.
+    // TemperatureDependent: temperature per particle (null: independent),
+    // expansion coefficient and reference temperature.
+    double* splas;
+    double sY;
+    const double* temp;
+    double alpha, temp0;
 };
 
 constexpr int kRowMaxSlots = 9;   // slots per row window (|dx| <= 4)

@@ -80,6 +80,48 @@ __device__ __forceinline__ bool criticalStretch64( double r, double xi, double b
     return r * r >= bcc * xi * xi;
 }
 
+// TemperatureDependent (CabanaPD_ForceModels.hpp:203-211): the bond's
+// temp_avg; the stretch becomes s - alpha * temp_avg.
+__device__ __forceinline__ double thermalAvg( const PDArgs& A, double Ti, double Tj )
+{
+#pragma clang fp contract( off )
+    return 0.5 * ( Ti + Tj ) - A.temp0;
+}
+// ThermalFractureModel CriticalStretchTag coefficient (CabanaPD_ForceModels.hpp:246-255).
+__device__ __forceinline__ double thermalBcc( const PDArgs& A, double temp_avg )
+{
+#pragma clang fp contract( off )
+    return ( 1.0 + A.s0 + A.alpha * temp_avg ) * ( 1.0 + A.s0 + A.alpha * temp_avg );
+}
+// EPP ForceCoeffTag (force_models/CabanaPD_PMB.hpp:104-121): update the bond's
+// plastic stretch (written only when it changes) and return s - s_p.
+__device__ __forceinline__ double plasticStretch( double* slot, double s, double sY )
+{
+#pragma clang fp contract( off )
+    double sp = *slot;
+    if ( s >= sp + sY )
+    {
+        sp = s - sY;
+        *slot = sp;
+    }
+    else if ( s <= sp - sY )
+    {
+        sp = s + sY;
+        *slot = sp;
+    }
+    return s - sp;
+}
+// EPP EnergyTag stretch term (force_models/CabanaPD_PMB.hpp:123-146).
+__device__ __forceinline__ double plasticEnergyTerm( double s, double sp, double sY )
+{
+#pragma clang fp contract( off )
+    if ( s >= sp + sY )
+        return sY * ( 2.0 * s - sY );
+    if ( s <= sp - sY )
+        return sY * ( -2.0 * s - sY );
+    return s * s;
+}
+
 struct Bond32
 {
     float s, rinv, rx, ry, rz, xi;
@@ -451,6 +493,10 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     // multi-material (FP64): model of the material pair; null with one material
     const uint8_t* __restrict__ mt = ( PREC == kF64 ) ? A.mtype : nullptr;
     const int ti4 = mt ? 4 * static_cast<int>( mt[pc] ) : 0;
+    // PMB family extensions (FP64): temperature, plastic stretch per bond slot
+    const double* __restrict__ tmp = ( PREC == kF64 && !kIsLps<MODEL> ) ? A.temp : nullptr;
+    double* __restrict__ spl = ( PREC == kF64 && !kIsLps<MODEL> ) ? A.splas : nullptr;
+    const double Ti = tmp ? tmp[pc] : 0.0;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( kIsLps<MODEL> )
     {
@@ -569,18 +615,21 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                 ww[k] = ( static_cast<uint64_t>( h32 ) << 32 ) | l32;
             }
         }
-        double xj[UNR][3], uj[UNR][3], vj[UNR], aj[UNR], bj[UNR];
+        double xj[UNR][3], uj[UNR][3], vj[UNR], aj[UNR], bj[UNR], tj[UNR];
         bool nfj[UNR];
         int mj[UNR];
 #pragma unroll
         for ( int k = 0; k < UNR; k++ )
         {
             mj[k] = 0;
+            tj[k] = 0.0;
             if ( ex[k] )
             {
                 const int64_t j = jj[k];
                 if ( mt )
                     mj[k] = mt[j];
+                if ( tmp )
+                    tj[k] = tmp[j];
                 if ( !useSten )
                 {
                     xj[k][0] = x[j];
@@ -625,16 +674,27 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     Bond64 b;
                     bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0], uj[k][1],
                                    uj[k][2], b );
+                    double temp_avg = 0.0;
+                    if ( tmp )
+                    {
+                        temp_avg = thermalAvg( A, Ti, tj[k] );
+                        b.s = b.s - ( A.alpha * temp_avg );
+                    }
                     if constexpr ( FRAC )
                     {
-                        if ( criticalStretch64( b.r, b.xi, mt ? A.mat_bcc[ti4 + mj[k]] : bcc ) && !nf_i && !nf_j )
+                        const double bcc_ij = tmp ? thermalBcc( A, temp_avg ) : ( mt ? A.mat_bcc[ti4 + mj[k]] : bcc );
+                        if ( criticalStretch64( b.r, b.xi, bcc_ij ) && !nf_i && !nf_j )
                             brk = true;
                     }
                     if ( !brk && intact )
                     {
                         double coeff;
                         if constexpr ( !kIsLps<MODEL> )
-                            coeff = ( mt ? A.mat_c[ti4 + mj[k]] : c ) * b.s * vj[k];
+                        {
+                            const double sd =
+                                spl ? plasticStretch( spl + ( base + q ) * kSlice + lane, b.s, A.sY ) : b.s;
+                            coeff = ( mt ? A.mat_c[ti4 + mj[k]] : c ) * sd * vj[k];
+                        }
                         else
                         {
                             const double infl = infl1 ? 1.0 / b.xi : 1.0;
@@ -1407,6 +1467,9 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
         const int ti4 = mt ? 4 * static_cast<int>( mt[p] ) : 0;
         const double tc_i = mt ? A.mat_tc[mt[p]] : A.theta_coeff;
         const double sc_i = mt ? A.mat_sc[mt[p]] : A.s_coeff;
+        const double* __restrict__ tmp = kIsLps<MODEL> ? nullptr : A.temp;
+        const double* __restrict__ spl = kIsLps<MODEL> ? nullptr : A.splas;
+        const double Ti = tmp ? tmp[p] : 0.0;
         double num_bonds = 0.0;
         double m_i = 0.0, theta_i = 0.0;
         if constexpr ( kIsLps<MODEL> )
@@ -1439,7 +1502,15 @@ __global__ __launch_bounds__( kBlock ) void kEnergy( const PDArgs* __restrict__ 
                      0.5 * ( sc_i / m_i ) * infl * b.s * b.s * b.xi * b.xi * vol[j];
             }
             else
-                wb = 0.25 * ( mt ? A.mat_c[ti4 + mt[j]] : c ) * b.s * b.s * b.xi * vol[j];
+            {
+                if ( tmp )
+                    b.s = b.s - ( A.alpha * thermalAvg( A, Ti, tmp[j] ) );
+                const double cc = mt ? A.mat_c[ti4 + mt[j]] : c;
+                if ( spl )
+                    wb = 0.25 * cc * plasticEnergyTerm( b.s, spl[( base + q ) * kSlice + lane], A.sY ) * b.xi * vol[j];
+                else
+                    wb = 0.25 * cc * b.s * b.s * b.xi * vol[j];
+            }
             if constexpr ( FRAC )
             {
                 const double mu = static_cast<double>( ( bits[q] >> lane ) & 1ull );
@@ -1521,6 +1592,9 @@ __global__ __launch_bounds__( kBlock ) void kStress( const PDArgs* __restrict__ 
     const double c = A.c;
     const uint8_t* __restrict__ mt = A.mtype; // multi-material: pair constants
     const int ti4 = mt ? 4 * static_cast<int>( mt[p] ) : 0;
+    const double* __restrict__ tmp = kIsLps<MODEL> ? nullptr : A.temp;
+    double* __restrict__ spl = kIsLps<MODEL> ? nullptr : A.splas;
+    const double Ti = tmp ? tmp[p] : 0.0;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( kIsLps<MODEL> )
     {
@@ -1554,7 +1628,13 @@ __global__ __launch_bounds__( kBlock ) void kStress( const PDArgs* __restrict__ 
             coeff *= 0.5;
         }
         else
-            coeff = 0.5 * ( ( mt ? A.mat_c[ti4 + mt[j]] : c ) * b.s * vol[j] );
+        {
+            if ( tmp )
+                b.s = b.s - ( A.alpha * thermalAvg( A, Ti, tmp[j] ) );
+            // EPP: the reference's stress pass updates s_p of every bond (:373-376)
+            const double sd = spl ? plasticStretch( spl + ( base + q ) * kSlice + lane, b.s, A.sY ) : b.s;
+            coeff = 0.5 * ( ( mt ? A.mat_c[ti4 + mt[j]] : c ) * sd * vol[j] );
+        }
         double fx, fy, fz;
         if constexpr ( FRAC )
         {
@@ -1630,6 +1710,25 @@ __global__ __launch_bounds__( 256 ) void kApplyForceTerms( const PDArgs* __restr
         if ( A.bc_kind >= 0 && A.bcflag[p] )
             fv = ( A.bc_kind == PDK_BC_FORCE_VALUE ) ? A.bc_value : fv + A.bc_value;
         A.f[d * n + p] = fv;
+    }
+}
+
+// Prescribed values (boundary conditions and body terms with force_update ==
+// false, CabanaPD_Solver.hpp:255-257, 350-352): member m (device index
+// idx[m]) gets a[m][c] + b[m][c] t in component c of dst ([ncomp][n]).
+__global__ __launch_bounds__( 256 ) void kPrescribe( const int64_t* __restrict__ idx, int64_t count,
+                                                     const double* __restrict__ a, const double* __restrict__ b,
+                                                     int32_t ncomp, double t, double* __restrict__ dst, int64_t n )
+{
+#pragma clang fp contract( off )
+    const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( m >= count )
+        return;
+    const int64_t p = idx[m];
+    for ( int32_t c = 0; c < ncomp; c++ )
+    {
+        const int64_t e = m * ncomp + c;
+        dst[c * n + p] = b ? a[e] + b[e] * t : a[e];
     }
 }
 
@@ -1915,6 +2014,16 @@ void launchApplyForceTerms( const PDArgs* args_dev, const PDArgs& a, hipStream_t
         return;
     hipLaunchKernelGGL( kApplyForceTerms, dim3( static_cast<unsigned>( ( a.n + 255 ) / 256 ) ), dim3( 256 ), 0,
                         st, args_dev );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchPrescribed( const int64_t* idx_dev, int64_t count, const double* a_dev, const double* b_dev, int32_t ncomp,
+                       double t, double* dst_dev, int64_t n, hipStream_t st )
+{
+    if ( count == 0 )
+        return;
+    hipLaunchKernelGGL( kPrescribe, dim3( static_cast<unsigned>( ( count + 255 ) / 256 ) ), dim3( 256 ), 0, st,
+                        idx_dev, count, a_dev, b_dev, ncomp, t, dst_dev, n );
     PDK_HIP( hipGetLastError() );
 }
 

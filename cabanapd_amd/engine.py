@@ -70,6 +70,8 @@ _SIGS = {
     "pdk_compute_energy_async": (C.c_int, [_VP]),
     "pdk_compute_stress": (C.c_int, [_VP]),
     "pdk_set_materials": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.Material), _I32]),
+    "pdk_set_prescribed": (C.c_int, [_VP, C.c_int32, _I64, C.c_int64, _D, _D]),
+    "pdk_set_field": (C.c_int, [_VP, C.c_int32, _D]),
 }
 
 
@@ -263,6 +265,19 @@ class Spec:
 
     def compute_energy_async(self):
         _check(lib().pdk_compute_energy_async(self.h))
+
+    def set_prescribed(self, field, idx, a, b=None):
+        """Values a + b t set before each force pass (and at t = 0 in init):
+        field "u" (a, b [count][3]) or "temperature" (a, b [count])."""
+        nc = 3 if field == "u" else 1
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        a = _f64(a, (idx.size, nc)) if idx.size else np.zeros((0, nc))
+        b = _f64(b, (idx.size, nc)) if (b is not None and idx.size) else None
+        _check(lib().pdk_set_prescribed(self.h, _abi.FIELD[field], idx.ctypes.data_as(_I64), idx.size, _dp(a),
+                                        _dp(b)))
+
+    def set_field(self, field, values):
+        _check(lib().pdk_set_field(self.h, _abi.FIELD[field], _dp(_f64(values, (self.n,)))))
 
     def series(self, region, field, component=0, reducer="max", capacity=1024, gate=None, threshold=0.0,
                weight_volume=False, ids=None):

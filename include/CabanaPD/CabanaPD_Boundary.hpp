@@ -167,13 +167,17 @@ auto createBoundaryCondition( BCTag tag, const double value, ExecSpace exec, con
 }
 
 // Boundary condition with a user functor (pid, time) over the particles of
-// the regions (CabanaPD_Boundary.hpp:26-73, 177-185).  As for BodyTerm, the
-// engine applies a constant per-particle force term after the force pass
-// each step: the functor is evaluated once, at time 0, against zeroed force
-// storage on the region's particles, so it must add a time-independent force
-// (a traction such as f(pid, 1) += b0 * sign(y)).  Functors applied before
-// the force pass (force_update = false: displacement or velocity
-// conditions) are rejected.
+// the regions (CabanaPD_Boundary.hpp:26-73, 177-185).
+// force_update = true: as for BodyTerm, the engine applies a constant
+// per-particle force term after the force pass each step: the functor is
+// evaluated once, at time 0, against zeroed force storage on the region's
+// particles, so it must add a time-independent force (a traction such as
+// f(pid, 1) += b0 * sign(y)).
+// force_update = false (displacement or temperature conditions, applied after
+// the initial half step and before the force pass): the Solver evaluates the
+// functor on the host at every step's time and hands the displacements (all
+// three components) or temperatures it wrote to the engine (pdk_set_prescribed);
+// such functors may read reference positions and time, not the evolving state.
 template <class UserFunctor>
 struct FunctorBoundaryCondition
 {
@@ -186,11 +190,17 @@ struct FunctorBoundaryCondition
     }
     bool forceUpdate() const { return _force_update; }
     double timeInit() const { return 0.0; }
+    // Particles the functor may write, and one application at time t.
+    std::vector<int64_t> members( std::size_t ) const { return _indices._view; }
+    void applyAt( double t ) const
+    {
+        for ( int64_t p : _indices._view )
+            _user_functor( static_cast<int>( p ), t );
+    }
     std::vector<double> evaluate( ParticleStorage& s ) const
     {
         if ( !_force_update )
-            throw std::runtime_error( "boundary functors applied before the force pass (force_update = false) "
-                                      "are not supported by this build" );
+            throw std::runtime_error( "a force_update = false functor is applied through the Solver" );
         std::vector<double> saved = s.f;
         std::fill( s.f.begin(), s.f.end(), 0.0 );
         for ( int64_t p : _indices._view )
@@ -210,10 +220,13 @@ auto createBoundaryCondition( UserFunctor user, ExecSpace exec, const ParticlesT
                                                   force_update );
 }
 
-// BodyTerm with a user functor (pid, time).  The engine applies a constant
-// per-particle body force density each step; the functor is evaluated once
-// against zeroed force storage to obtain it, so it must add a
-// time-independent term (gravity, constant traction).
+// BodyTerm with a user functor (pid, time).  force = true: the engine
+// applies a constant per-particle body force density each step; the functor
+// is evaluated once against zeroed force storage to obtain it, so it must add
+// a time-independent term (gravity, constant traction).  force = false: an
+// imposed field (such as the temperature of
+// examples/thermomechanics/thermal_deformation.cpp) set before each force
+// pass, as FunctorBoundaryCondition with force_update = false.
 template <class UserFunctor>
 struct BodyTerm
 {
@@ -228,6 +241,20 @@ struct BodyTerm
     }
     bool forceUpdate() const { return _force_update; }
     double timeInit() const { return 0.0; }
+    // Particles the functor may write (force = false: applied before the
+    // force pass like a displacement condition), and one application at t.
+    std::vector<int64_t> members( std::size_t ) const
+    {
+        std::vector<int64_t> all( _particle_count );
+        for ( std::size_t p = 0; p < _particle_count; p++ )
+            all[p] = static_cast<int64_t>( p );
+        return all;
+    }
+    void applyAt( double t ) const
+    {
+        for ( std::size_t p = 0; p < _particle_count; p++ )
+            _user_functor( static_cast<int>( p ), t );
+    }
     // Evaluate the constant body force density into b[n][3].
     std::vector<double> evaluate( ParticleStorage& s ) const
     {

@@ -1,11 +1,13 @@
 // CabanaPD_ForceModels.hpp -- ForceModel<PMB|LPS, Elastic, NoFracture|Fracture>,
-// ForceModel<LinearPMB|LinearLPS, Elastic, NoFracture>
+// ForceModel<LinearPMB|LinearLPS, Elastic, NoFracture>, the PMB
+// ElasticPerfectlyPlastic and TemperatureDependent forms,
 // and the Hertz / Hertz-JKR contact models of the drop-in API.
 //
 // Constructors, deduction guides and public members follow the reference:
 //   force_models/CabanaPD_PMB.hpp:149-231,277-297
 //   force_models/CabanaPD_LPS.hpp:151-253,274-302
-//   CabanaPD_ForceModels.hpp:42-129
+//   CabanaPD_ForceModels.hpp:42-129,149-256
+//   force_models/CabanaPD_PMB.hpp:74-147,232-305,306-568 (EPP, thermal)
 //   force_models/CabanaPD_Contact.hpp:26-51, CabanaPD_Hertzian.hpp:23-56,
 //   CabanaPD_HertzianJKR.hpp:23-80
 // Derived constants come from the engine (pdk_model_constants_get), which
@@ -61,7 +63,7 @@ inline pdk_model_desc blankDesc()
 } // namespace detail
 
 template <class ModelType, class MechanicsType = Elastic, class FractureType = Fracture,
-          class ThermalType = TemperatureIndependent>
+          class ThermalType = TemperatureIndependent, class... FieldTypes>
 struct ForceModel;
 
 struct BaseForceModel
@@ -113,10 +115,20 @@ struct PDForceModel : BaseForceModel
         d.G = G;
         d.G0 = G0;
         d.dt = dt;
+        d.mechanics = mechanics_;
+        d.thermal = thermal_;
+        d.sigma_y = sigma_y_;
+        d.alpha = alpha_;
+        d.temp0 = temp0_;
         return d;
     }
 
   protected:
+    // EPP and thermal coupling of the PMB forms (the engine's description)
+    int32_t mechanics_ = PDK_MECH_ELASTIC;
+    int32_t thermal_ = PDK_TEMP_INDEPENDENT;
+    double sigma_y_ = 0.0, alpha_ = 0.0, temp0_ = 0.0;
+
     void derive()
     {
         pdk_model_constants k = detail::constantsOf( desc( 1.0 ) );
@@ -128,12 +140,8 @@ struct PDForceModel : BaseForceModel
     }
 };
 
-template <class F>
-struct is_fracture<ForceModel<PMB, Elastic, F>> : std::is_same<F, Fracture>
-{
-};
-template <class F>
-struct is_fracture<ForceModel<LPS, Elastic, F>> : std::is_same<F, Fracture>
+template <class Model, class Mech, class F, class Thermal, class... Fields>
+struct is_fracture<ForceModel<Model, Mech, F, Thermal, Fields...>> : std::is_same<F, Fracture>
 {
 };
 
@@ -187,6 +195,155 @@ struct ForceModel<PMB, Elastic, Fracture> : PDForceModel<PMB, Fracture>
         derive();
     }
 };
+
+// ---- PMB ElasticPerfectlyPlastic (force_models/CabanaPD_PMB.hpp:74-147, 232-284) ----
+// One plastic stretch per bond, held by the engine (zero when the bonds are
+// built); s_Y = sigma_y / (3 K), s0 = (5 G0 / sigma_y / delta + sigma_y / K) / 6.
+template <class MemorySpace>
+struct ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureIndependent, MemorySpace>
+    : PDForceModel<PMB, Fracture>
+{
+    using mechanics_type = ElasticPerfectlyPlastic;
+    double s_Y = 0.0;
+    ForceModel( PMB, ElasticPerfectlyPlastic, MemorySpace, const double h, const double k, const double g0,
+                const double sigma_y )
+    {
+        set( h, k, g0, sigma_y );
+    }
+    ForceModel( PMB, ElasticPerfectlyPlastic, Fracture, MemorySpace, const double h, const double k, const double g0,
+                const double sigma_y )
+    {
+        set( h, k, g0, sigma_y );
+    }
+    void updateBonds( const int, const int ) {}
+
+  private:
+    void set( double h, double k, double g0, double sy )
+    {
+        force_horizon = h;
+        K = k;
+        G0 = g0;
+        influence_type = 1;
+        mechanics_ = PDK_MECH_EPP;
+        sigma_y_ = sy;
+        derive();
+        s_Y = detail::constantsOf( desc( 1.0 ) ).s_Y;
+    }
+};
+
+// ---- PMB TemperatureDependent (force_models/CabanaPD_PMB.hpp:306-568) ------------
+// The temperature field lives in the particles (sliceTemperature) and on the
+// device; alpha, temp0 and the slice are kept as the reference's members.
+template <class Tag, class Mech, class F, class TemperatureType>
+struct ThermalPMB : PDForceModel<Tag, F>
+{
+    using thermal_type = TemperatureDependent;
+    using mechanics_type = Mech;
+    double alpha = 0.0;
+    double temp0 = 0.0;
+    double s_Y = 0.0;
+    TemperatureType temperature;
+
+  protected:
+    void setThermal( double h, double k, double g0, double sy, const TemperatureType& t, double a, double t0 )
+    {
+        this->force_horizon = h;
+        this->K = k;
+        this->G0 = g0;
+        this->influence_type = 1; // ThermalFractureModel's default (CabanaPD_ForceModels.hpp:228-236)
+        this->thermal_ = PDK_TEMP_DEPENDENT;
+        this->alpha_ = a;
+        this->temp0_ = t0;
+        if ( std::is_same_v<Mech, ElasticPerfectlyPlastic> )
+        {
+            this->mechanics_ = PDK_MECH_EPP;
+            this->sigma_y_ = sy;
+        }
+        alpha = a;
+        temp0 = t0;
+        temperature = t;
+        this->derive();
+        s_Y = detail::constantsOf( this->desc( 1.0 ) ).s_Y;
+    }
+};
+
+template <class TemperatureType>
+struct ForceModel<PMB, Elastic, NoFracture, TemperatureDependent, TemperatureType>
+    : ThermalPMB<PMB, Elastic, NoFracture, TemperatureType>
+{
+    ForceModel( PMB, NoFracture, const double h, const double k, const TemperatureType t, const double a,
+                const double t0 = 0.0 )
+    {
+        this->setThermal( h, k, 0.0, 0.0, t, a, t0 );
+    }
+};
+template <class TemperatureType>
+struct ForceModel<PMB, Elastic, Fracture, TemperatureDependent, TemperatureType>
+    : ThermalPMB<PMB, Elastic, Fracture, TemperatureType>
+{
+    ForceModel( PMB, const double h, const double k, const double g0, const TemperatureType t, const double a,
+                const double t0 = 0.0 )
+    {
+        this->setThermal( h, k, g0, 0.0, t, a, t0 );
+    }
+};
+template <class TemperatureType>
+struct ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureDependent, TemperatureType>
+    : ThermalPMB<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureType>
+{
+    ForceModel( PMB, ElasticPerfectlyPlastic, const double h, const double k, const double g0, const double sigma_y,
+                const TemperatureType t, const double a, const double t0 = 0.0 )
+    {
+        this->setThermal( h, k, g0, sigma_y, t, a, t0 );
+    }
+};
+// ForceModel<LinearPMB, ...> derives from the PMB form with force_tag
+// LinearPMB (force_models/CabanaPD_PMB.hpp:575-597).
+template <class TemperatureType>
+struct ForceModel<LinearPMB, Elastic, NoFracture, TemperatureDependent, TemperatureType>
+    : ThermalPMB<LinearPMB, Elastic, NoFracture, TemperatureType>
+{
+    ForceModel( LinearPMB, NoFracture, const double h, const double k, const TemperatureType t, const double a,
+                const double t0 = 0.0 )
+    {
+        this->setThermal( h, k, 0.0, 0.0, t, a, t0 );
+    }
+};
+
+namespace detail
+{
+template <class T>
+using IfField = std::enable_if_t<!std::is_arithmetic_v<T>, int>;
+}
+
+template <class MS, detail::IfField<MS> = 0>
+ForceModel( PMB, ElasticPerfectlyPlastic, MS, double, double, double, double )
+    -> ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureIndependent, MS>;
+template <class MS, detail::IfField<MS> = 0>
+ForceModel( PMB, ElasticPerfectlyPlastic, Fracture, MS, double, double, double, double )
+    -> ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureIndependent, MS>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, NoFracture, double, double, TT, double ) -> ForceModel<PMB, Elastic, NoFracture, TemperatureDependent, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, NoFracture, double, double, TT, double, double )
+    -> ForceModel<PMB, Elastic, NoFracture, TemperatureDependent, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, double, double, double, TT, double ) -> ForceModel<PMB, Elastic, Fracture, TemperatureDependent, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, double, double, double, TT, double, double )
+    -> ForceModel<PMB, Elastic, Fracture, TemperatureDependent, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, ElasticPerfectlyPlastic, double, double, double, double, TT, double )
+    -> ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureDependent, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, ElasticPerfectlyPlastic, double, double, double, double, TT, double, double )
+    -> ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureDependent, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( LinearPMB, NoFracture, double, double, TT, double )
+    -> ForceModel<LinearPMB, Elastic, NoFracture, TemperatureDependent, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( LinearPMB, NoFracture, double, double, TT, double, double )
+    -> ForceModel<LinearPMB, Elastic, NoFracture, TemperatureDependent, TT>;
 
 // ---- LPS -------------------------------------------------------------------
 template <>

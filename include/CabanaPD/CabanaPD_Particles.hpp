@@ -45,15 +45,15 @@ struct StressSlice
 struct ParticleStorage
 {
     std::size_t n = 0;
-    std::vector<double> x, u, v, f, vol, rho, W, phi, m, theta, y, stress;
+    std::vector<double> x, u, v, f, vol, rho, W, phi, m, theta, y, stress, temp;
     std::vector<int> nofail, type;
     void resize( std::size_t nn )
     {
         n = nn;
         for ( auto* a : { &x, &u, &v, &f, &y } )
             a->assign( 3 * nn, 0.0 );
-        for ( auto* a : { &vol, &W, &phi, &m, &theta } )
-            a->assign( nn, 0.0 );
+        for ( auto* a : { &vol, &W, &phi, &m, &theta, &temp } )
+            a->assign( nn, 0.0 ); // temperature zero at creation (CabanaPD_Particles.hpp:860-864)
         stress.assign( 9 * nn, 0.0 );
         rho.assign( nn, 1.0 );
         nofail.assign( nn, 0 );
@@ -200,6 +200,8 @@ class Particles
     Slice<double, 1> sliceDilatation() const { return { _s->theta.data(), _s->n }; }
     // Particles<..., EnergyStressOutput> (CabanaPD_Particles.hpp:1233-1356)
     StressSlice sliceStress() const { return { _s->stress.data(), _s->n }; }
+    // TemperatureDependent particles (CabanaPD_Particles.hpp:884-887)
+    Slice<double, 1> sliceTemperature() const { return { _s->temp.data(), _s->n }; }
     Slice<int, 1> sliceNoFail() const { return { _s->nofail.data(), _s->n }; }
     Slice<int, 1> sliceType() const { return { _s->type.data(), _s->n }; }
 
@@ -231,6 +233,7 @@ class Particles
             }
             s.vol[i] = old.vol[i];
             s.rho[i] = old.rho[i];
+            s.temp[i] = old.temp[i];
             s.nofail[i] = old.nofail[i];
         }
         for ( std::size_t i = 0; i < pv.size(); i++ )
@@ -261,13 +264,25 @@ struct OutputOf<T, Rest...>
 {
     using type = std::conditional_t<is_output<T>::value, T, typename OutputOf<Rest...>::type>;
 };
+// Thermal type named among the constructor tags (TemperatureIndependent when none is).
+template <class... Tags>
+struct ThermalOf
+{
+    using type = TemperatureIndependent;
+};
+template <class T, class... Rest>
+struct ThermalOf<T, Rest...>
+{
+    using type = std::conditional_t<std::is_same_v<T, TemperatureDependent>, T, typename ThermalOf<Rest...>::type>;
+};
 } // namespace detail
 
 // Deduction (CabanaPD_Particles.hpp:1364-1395): Particles( memory_space{},
 // model{} [, thermal] [, output] ), energy output by default.
 template <class MemorySpace, class ModelType, class... Tags>
 Particles( MemorySpace, ModelType, Tags... )
-    -> Particles<MemorySpace, ModelType, TemperatureIndependent, typename detail::OutputOf<Tags...>::type>;
+    -> Particles<MemorySpace, ModelType, typename detail::ThermalOf<Tags...>::type,
+                 typename detail::OutputOf<Tags...>::type>;
 
 } // namespace CabanaPD
 

@@ -9,7 +9,10 @@
 #define CABANAPD_DROPIN_SOLVER_HPP
 
 #include <chrono>
+#include <cmath>
+#include <functional>
 #include <iostream>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -49,6 +52,7 @@ struct DeviceSpec
     pdk_spec spec = nullptr;
     bool lps = false;
     bool fracture = false;
+    bool thermal = false;
     bool stress = false; // a stress pass has run
     DeviceSpec() = default;
     DeviceSpec( const DeviceSpec& ) = delete;
@@ -65,9 +69,12 @@ struct DeviceSpec
     {
         lps = d.model == PDK_MODEL_LPS || d.model == PDK_MODEL_LINEAR_LPS;
         fracture = d.fracture != 0;
+        thermal = d.thermal == PDK_TEMP_DEPENDENT;
         check( pdk_spec_create( context(), &d, static_cast<int64_t>( s.n ), &spec ) );
         check( pdk_spec_upload( spec, s.x.data(), s.u.data(), s.v.data(), s.vol.data(), s.rho.data(),
                                 s.nofail.data(), nullptr ) );
+        if ( thermal )
+            check( pdk_set_field( spec, PDK_FIELD_TEMPERATURE, s.temp.data() ) );
         if ( !mats.empty() )
         {
             std::vector<int32_t> type( s.type.begin(), s.type.end() );
@@ -91,8 +98,11 @@ struct DeviceSpec
             case PDK_FIELD_M: dst = s.m.data(); break;
             case PDK_FIELD_THETA: dst = s.theta.data(); break;
             case PDK_FIELD_STRESS: dst = s.stress.data(); break;
+            case PDK_FIELD_TEMPERATURE: dst = s.temp.data(); break;
             default: continue;
             }
+            if ( f == PDK_FIELD_TEMPERATURE && !thermal )
+                continue;
             if ( f == PDK_FIELD_STRESS && !stress )
                 continue;
             if ( ( f == PDK_FIELD_M || f == PDK_FIELD_THETA ) && !lps )
@@ -103,7 +113,7 @@ struct DeviceSpec
     void pullState( ParticleStorage& s )
     {
         pull( s, { PDK_FIELD_U, PDK_FIELD_V, PDK_FIELD_F, PDK_FIELD_W, PDK_FIELD_PHI, PDK_FIELD_M,
-                   PDK_FIELD_THETA, PDK_FIELD_STRESS } );
+                   PDK_FIELD_THETA, PDK_FIELD_STRESS, PDK_FIELD_TEMPERATURE } );
     }
     // Stress pass (computeStress); pulled from then on.
     void computeStress()
@@ -241,6 +251,9 @@ class Solver
     {
         static_assert( std::is_same_v<ContactModelType, NoContact>,
                        "PD + contact in one Solver is not supported by this build" );
+        static_assert( !is_temperature_dependent<typename ForceModelType::thermal_type>::value ||
+                           is_temperature_dependent<typename ParticleType::thermal_type>::value,
+                       "a TemperatureDependent force model needs TemperatureDependent particles" );
         setup();
     }
 
@@ -265,6 +278,8 @@ class Solver
     // Solver::init overloads (CabanaPD_Solver.hpp:229-297)
     void init( const bool initial_output = true )
     {
+        if ( _pre )
+            _pre( 0.0 ); // force_update = false conditions at t = 0 (CabanaPD_Solver.hpp:255-257)
         detail::check( pdk_init( spec() ) );
         if constexpr ( is_stress_output<typename ParticleType::output_type>::value )
             neighbor->device().computeStress();
@@ -306,6 +321,8 @@ class Solver
 
     void runStep( const int step )
     {
+        if ( _pre )
+            _pre( step * dt );
         detail::check( pdk_step( spec(), 1 ) );
         output( step );
     }
@@ -342,6 +359,12 @@ class Solver
     {
         if constexpr ( is_body_term<std::decay_t<BC>>::value )
         {
+            if ( !bc.forceUpdate() )
+            {
+                // applied before every force pass at that step's time
+                _pre = [this, bc]( double t ) { prescribeFrom( bc, t ); };
+                return;
+            }
             std::vector<double> b = bc.evaluate( particles.storage() );
             detail::check( pdk_set_body_force( spec(), b.data() ) );
         }
@@ -350,6 +373,64 @@ class Solver
             detail::check( pdk_set_bc( spec(), bc.kind(), bc._indices._view.data(),
                                        static_cast<int64_t>( bc._indices._view.size() ), bc._value ) );
         }
+    }
+
+    // Evaluate a force_update = false functor at time t on the host: the
+    // entries it writes (marked by overwriting NaN placeholders) become the
+    // engine's prescribed displacements / temperatures for the next force pass.
+    template <class BC>
+    void prescribeFrom( const BC& bc, double t )
+    {
+        ParticleStorage& s = particles.storage();
+        const std::vector<int64_t> cand = bc.members( s.n );
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        std::vector<double> su( 3 * cand.size() ), st( cand.size() );
+        for ( std::size_t m = 0; m < cand.size(); m++ )
+        {
+            const int64_t p = cand[m];
+            for ( int d = 0; d < 3; d++ )
+            {
+                su[3 * m + d] = s.u[3 * p + d];
+                s.u[3 * p + d] = nan;
+            }
+            st[m] = s.temp[p];
+            s.temp[p] = nan;
+        }
+        bc.applyAt( t );
+        std::vector<int64_t> iu, it;
+        std::vector<double> au, at;
+        for ( std::size_t m = 0; m < cand.size(); m++ )
+        {
+            const int64_t p = cand[m];
+            int written = 0;
+            for ( int d = 0; d < 3; d++ )
+                written += std::isnan( s.u[3 * p + d] ) ? 0 : 1;
+            if ( written != 0 && written != 3 )
+                throw std::runtime_error( "a force_update = false functor must set all three displacement components" );
+            if ( written == 3 )
+            {
+                iu.push_back( p );
+                for ( int d = 0; d < 3; d++ )
+                    au.push_back( s.u[3 * p + d] );
+            }
+            else
+                for ( int d = 0; d < 3; d++ )
+                    s.u[3 * p + d] = su[3 * m + d];
+            if ( !std::isnan( s.temp[p] ) )
+            {
+                it.push_back( p );
+                at.push_back( s.temp[p] );
+            }
+            else
+                s.temp[p] = st[m];
+        }
+        if ( !it.empty() && !neighbor->device().thermal )
+            throw std::runtime_error( "the functor sets temperatures but the force model is TemperatureIndependent" );
+        detail::check( pdk_set_prescribed( spec(), PDK_FIELD_U, iu.data(), static_cast<int64_t>( iu.size() ),
+                                           au.data(), nullptr ) );
+        if ( neighbor->device().thermal )
+            detail::check( pdk_set_prescribed( spec(), PDK_FIELD_TEMPERATURE, it.data(),
+                                               static_cast<int64_t>( it.size() ), at.data(), nullptr ) );
     }
 
     void output( const int step )
@@ -381,6 +462,12 @@ class Solver
             int chunk = num_steps - step;
             if ( output_frequency > 0 )
                 chunk = std::min( chunk, output_frequency - step % output_frequency );
+            if ( _pre )
+            {
+                // host-evaluated conditions: one step per call
+                chunk = 1;
+                _pre( ( step + 1 ) * dt );
+            }
             detail::check( pdk_step( spec(), chunk ) );
             step += chunk;
             output( step );
@@ -398,6 +485,7 @@ class Solver
     double _last_energy = 0.0;
     double _last_damage = 0.0;
     std::chrono::steady_clock::time_point _t0;
+    std::function<void( double )> _pre; // force_update = false condition, if any
 };
 
 } // namespace CabanaPD

@@ -4,6 +4,7 @@
 #define CABANAPD_DROPIN_TYPES_HPP
 
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <type_traits>
 
@@ -40,6 +41,17 @@ using Array = std::array<T, N>;
 inline void initialize( int&, char** ) {}
 inline void initialize() {}
 inline void finalize() {}
+// Kokkos math functions used by user drivers
+template <class T>
+inline T abs( const T v )
+{
+    return std::abs( v );
+}
+template <class T>
+inline T sqrt( const T v )
+{
+    return std::sqrt( v );
+}
 template <class T>
 struct Max
 {
@@ -83,6 +95,10 @@ struct LinearLPS
 struct Elastic
 {
 };
+// ElasticPerfectlyPlastic (CabanaPD_Types.hpp:52): PMB with fracture
+struct ElasticPerfectlyPlastic
+{
+};
 struct NoFracture
 {
 };
@@ -92,6 +108,15 @@ struct Fracture
 struct TemperatureIndependent
 {
     using base_type = TemperatureIndependent;
+};
+// TemperatureDependent: an imposed temperature field (CabanaPD_Types.hpp)
+struct TemperatureDependent
+{
+    using base_type = TemperatureDependent;
+};
+template <class T>
+struct is_temperature_dependent : std::is_same<typename T::base_type, TemperatureDependent>
+{
 };
 struct SingleMaterial
 {

@@ -64,7 +64,21 @@ enum pdk_field {
     PDK_FIELD_THETA = 9,  /* LPS dilatation      double[n] */
     PDK_FIELD_STRESS = 10, /* stress tensor      double[n][3][3], stress(i, a, b) at [9 i + 3 a + b];
                               the value of the last pdk_compute_stress */
-    PDK_FIELD_COUNT = 11
+    PDK_FIELD_TEMPERATURE = 11, /* temperature  double[n] (thermal models; Particles::sliceTemperature) */
+    PDK_FIELD_COUNT = 12
+};
+
+enum pdk_mechanics {
+    PDK_MECH_ELASTIC = 0,  /* Elastic */
+    PDK_MECH_EPP = 1       /* ElasticPerfectlyPlastic: PMB with Fracture only (force_models/CabanaPD_PMB.hpp:74-147,
+                              232-284); one plastic stretch per bond, zero when the bonds are built */
+};
+
+enum pdk_thermal {
+    PDK_TEMP_INDEPENDENT = 0,  /* TemperatureIndependent */
+    PDK_TEMP_DEPENDENT = 1     /* TemperatureDependent: PMB / LinearPMB stretch s - alpha (T_avg - temp0)
+                                  (CabanaPD_ForceModels.hpp:165-211); with Fracture the critical stretch
+                                  test uses (1 + s0 + alpha (T_avg - temp0))^2 (:213-256) */
 };
 
 enum pdk_bc_kind {
@@ -98,6 +112,12 @@ typedef struct pdk_model_desc {
     int32_t polydisperse;   /* 1: per-particle radii (extension; equal radii reduce to the reference) */
     int32_t ordering;       /* particle order on the device: PDK_ORDER_AUTO / _MORTON / _LATTICE_ROWS */
     int32_t bond_layout;    /* PDK_BONDS_AUTO / PDK_BONDS_COMPACT */
+    /* mechanics and thermal coupling (PMB family; zero = Elastic, TemperatureIndependent) */
+    int32_t mechanics;      /* pdk_mechanics */
+    int32_t thermal;        /* pdk_thermal */
+    double sigma_y;         /* yield stress (EPP): s_Y = sigma_y / (3 K) */
+    double alpha;           /* thermal expansion coefficient (TemperatureDependent) */
+    double temp0;           /* reference temperature (TemperatureDependent) */
 } pdk_model_desc;
 
 enum pdk_ordering {
@@ -115,6 +135,7 @@ enum pdk_bond_layout {
 typedef struct pdk_model_constants {
     double c, s0, bond_break_coeff, theta_coeff, s_coeff;
     double Rs, Es, Gs, beta, coeff_h_n, coeff_h_d, Gamma, a0, delta_tear, fc;
+    double s_Y;  /* EPP yield stretch sigma_y / 3 / K (force_models/CabanaPD_PMB.hpp:90-96) */
 } pdk_model_constants;
 
 typedef struct pdk_ctx_s* pdk_ctx;
@@ -126,6 +147,9 @@ const char* pdk_last_error(void);
 /* ---- derived constants ----------------------------------------------------
  * PMB: c = 18K/(pi delta^4)           force_models/CabanaPD_PMB.hpp:41-47
  * s0, bond_break_coeff                CabanaPD_ForceModels.hpp:93-102, LPS override force_models/CabanaPD_LPS.hpp:209-223
+ * EPP s0 = (5 G0 / sigma_y / delta + sigma_y / K) / 6  force_models/CabanaPD_PMB.hpp:255-264
+ *   (with TemperatureDependent the thermal fracture model's s0 = sqrt(5 G0 / 9 / K / delta),
+ *   influence 1, force_models/CabanaPD_PMB.hpp:366-396, CabanaPD_ForceModels.hpp:228-236)
  * LPS theta/s coefficients            force_models/CabanaPD_LPS.hpp:73-83
  * Hertz / JKR constants               force_models/CabanaPD_Hertzian.hpp:41-56, force_models/CabanaPD_HertzianJKR.hpp:51-80 */
 int pdk_model_constants_get(const pdk_model_desc* desc, pdk_model_constants* out);
@@ -183,6 +207,24 @@ int pdk_set_bc(pdk_spec spec, int32_t kind, const int64_t* idx, int64_t n, doubl
  * every step: BodyTerm with a time-independent functor (CabanaPD_BodyTerm.hpp:46-63),
  * e.g. gravity f_z -= 9.8 rho, or a face traction.  NULL clears it. */
 int pdk_set_body_force(pdk_spec spec, const double* b);
+/* Prescribed values where the reference applies boundary conditions and body
+ * terms built with force_update = false: after the velocity-Verlet initial
+ * half step and before the force pass of step k at t = k dt
+ * (CabanaPD_Solver.hpp:350-352), and at t = 0 at the start of pdk_init
+ * (:255-257).  field is PDK_FIELD_U (a, b double[count][3]: the three
+ * displacement components, as the dogbone grips of
+ * examples/mechanics/dogbone_tensile_test.cpp set them) or
+ * PDK_FIELD_TEMPERATURE (a, b double[count]: the imposed temperature of
+ * examples/thermomechanics/thermal_deformation.cpp).  Member idx[m] (caller
+ * id, no repeats) gets a[m] + b[m] * t, evaluated in that order without
+ * contraction; b may be NULL (constant values).  One set per field: a second
+ * call replaces the first, count = 0 removes it.  Specs with prescribed
+ * values are stepped with pdk_step (not the batch engine). */
+int pdk_set_prescribed(pdk_spec spec, int32_t field, const int64_t* idx, int64_t count, const double* a,
+                       const double* b);
+/* Set a per-particle field from caller-order values: PDK_FIELD_TEMPERATURE
+ * (thermal models; initially zero, as the reference's particle fields). */
+int pdk_set_field(pdk_spec spec, int32_t field, const double* values);
 
 /* ---- multi-material specimens ---------------------------------------------------
  * createMultiForceModel( particles, AverageTag, m_0, .., m_{nmat-1} )

@@ -42,6 +42,9 @@ _SIGS = {
     "orc_compute_energy": (C.c_int, [_VP, _D, _D]),
     "orc_compute_stress": (C.c_int, [_VP]),
     "orc_set_materials": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.Material), _I32]),
+    "orc_set_prescribed": (C.c_int, [_VP, C.c_int32, _I64, C.c_int64, _D, _D]),
+    "orc_set_field": (C.c_int, [_VP, C.c_int32, _D]),
+    "orc_get_plastic": (C.c_int, [_VP, _D]),
     "orc_init": (C.c_int, [_VP]),
     "orc_step": (C.c_int, [_VP, C.c_int32]),
     "orc_get": (C.c_int, [_VP, C.c_int32, _D]),
@@ -219,6 +222,25 @@ class OracleSpec:
         arr = _abi.materials_array(mats)
         t = np.ascontiguousarray(mtype, dtype=np.int32)
         _check(lib().orc_set_materials(self.h, len(mats), arr, t.ctypes.data_as(_I32)))
+
+    def set_prescribed(self, field, idx, a, b=None):
+        """Values a + b t applied before each force pass (and at t = 0 in init)."""
+        nc = 3 if field == "u" else 1
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        a = _f64(a, (idx.size, nc)) if idx.size else np.zeros((0, nc))
+        b = _f64(b, (idx.size, nc)) if (b is not None and idx.size) else None
+        _check(lib().orc_set_prescribed(self.h, FIELD[field], idx.ctypes.data_as(_I64), idx.size, _dp(a),
+                                        _dp(b)))
+
+    def set_field(self, field, values):
+        _check(lib().orc_set_field(self.h, FIELD[field], _dp(_f64(values, (self.n,)))))
+
+    def plastic(self):
+        """EPP plastic stretch per bond, in the order of bonds()."""
+        total, _ = self.bond_stats()
+        out = np.zeros(max(total, 1), dtype=np.float64)
+        _check(lib().orc_get_plastic(self.h, _dp(out)))
+        return out[:total]
 
     def init(self):
         _check(lib().orc_init(self.h))

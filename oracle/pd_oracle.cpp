@@ -22,7 +22,10 @@
 // the break test; geometric special cases for the prenotch; closed forms of
 // velocity-Verlet; the Kalthoff-Winkler regression value
 // (regression_test/tstKalthoffWinkler.hpp:53); Hertz restitution and JKR
-// pull-off (regression_test/tstHertz.hpp:135, tstHertzJKR.hpp:145).
+// pull-off (regression_test/tstHertz.hpp:135, tstHertzJKR.hpp:145); EPP
+// load-cycle closed forms, uniform-stretch scalings and the elastic limit;
+// thermal free expansion, held-body equivalence and break threshold;
+// prescribed values (tests/test_oracle_plastic_thermal.py).
 // Parity unpinned (no reference-held number exists): LPS fracture energy when
 // all bonds of a particle are broken, the LPS no-fail asymmetry, and the
 // polydisperse contact extension away from equal radii.
@@ -73,6 +76,22 @@ struct Sys
     int32_t nmat = 1;
     std::vector<int32_t> mtype;
     double pair_c[16] = {}, pair_bcc[16] = {}, mat_tc[4] = {}, mat_sc[4] = {};
+    // EPP: plastic stretch per bond sp[i * maxn + slot] (BasePlasticity::_s_p,
+    // CabanaPD_ForceModels.hpp:149-163, zeroed when the bonds are built)
+    std::vector<double> sp;
+    // TemperatureDependent: temperature per particle (zero at creation,
+    // CabanaPD_Particles.hpp:860-864)
+    std::vector<double> temp;
+    // values set before each force pass (boundary conditions and body terms
+    // with force_update == false): a + b t per member, field U or TEMPERATURE
+    struct Prescribed
+    {
+        std::vector<int64_t> idx;
+        std::vector<double> a, b;
+        int32_t ncomp = 0;
+    };
+    Prescribed pre_u, pre_temp;
+    int64_t steps_done = 0;
 };
 
 bool isContact( const pdk_model_desc& d )
@@ -89,6 +108,14 @@ bool isLps( const pdk_model_desc& d )
 bool isLinear( const pdk_model_desc& d )
 {
     return d.model == PDK_MODEL_LINEAR_PMB || d.model == PDK_MODEL_LINEAR_LPS;
+}
+bool isEpp( const pdk_model_desc& d )
+{
+    return d.mechanics == PDK_MECH_EPP;
+}
+bool isThermal( const pdk_model_desc& d )
+{
+    return d.thermal == PDK_TEMP_DEPENDENT;
 }
 
 // ---------------------------------------------------------------------------
@@ -126,7 +153,41 @@ int modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
         k.s_coeff = 15.0 * d.G;
         if ( isLps( d ) && ( d.influence < 0 || d.influence > 1 ) )
             return 1;
+        if ( d.mechanics != PDK_MECH_ELASTIC && d.mechanics != PDK_MECH_EPP )
+            return 1;
+        if ( d.thermal != PDK_TEMP_INDEPENDENT && d.thermal != PDK_TEMP_DEPENDENT )
+            return 1;
+        if ( isEpp( d ) )
+        {
+            // ForceModel<PMB, ElasticPerfectlyPlastic, Fracture> only
+            // (force_models/CabanaPD_PMB.hpp:232-284)
+            if ( d.model != PDK_MODEL_PMB || !d.fracture || !( d.sigma_y > 0 ) )
+                return 1;
+            // BaseForceModelPMB<ElasticPerfectlyPlastic> (force_models/CabanaPD_PMB.hpp:90-96)
+            k.s_Y = d.sigma_y / 3.0 / K;
+            if ( !isThermal( d ) )
+            {
+                // force_models/CabanaPD_PMB.hpp:255-264
+                k.s0 = ( 5.0 * d.G0 / d.sigma_y / delta + d.sigma_y / K ) / 6.0;
+                k.bond_break_coeff = ( 1.0 + k.s0 ) * ( 1.0 + k.s0 );
+            }
+        }
+        if ( isThermal( d ) )
+        {
+            // PMB and LinearPMB only (force_models/CabanaPD_PMB.hpp:306-568, 575-597)
+            if ( d.model != PDK_MODEL_PMB && d.model != PDK_MODEL_LINEAR_PMB )
+                return 1;
+            if ( d.fracture )
+            {
+                // ThermalFractureModel: BaseFractureModel( delta, K, G0 ) with
+                // influence_type 1 (CabanaPD_ForceModels.hpp:228-236, 93-102)
+                k.s0 = std::sqrt( 5.0 * d.G0 / 9.0 / K / delta );
+                k.bond_break_coeff = ( 1.0 + k.s0 ) * ( 1.0 + k.s0 );
+            }
+        }
     }
+    else if ( d.mechanics != PDK_MECH_ELASTIC || d.thermal != PDK_TEMP_INDEPENDENT )
+        return 1;
     else if ( d.model == PDK_MODEL_HERTZ )
     {
         // force_models/CabanaPD_Hertzian.hpp:46-55
@@ -512,6 +573,68 @@ inline double lpsForceCoeff( const Sys& s, int64_t i, int64_t j, double st, doub
            infl * xi * vol;
 }
 
+// BaseTemperatureModel<TemperatureDependent>: temp_avg and the stretch
+// correction (CabanaPD_ForceModels.hpp:203-211); the identity when
+// TemperatureIndependent.
+inline double thermalStretch( const Sys& s, int64_t i, int64_t j, double st )
+{
+    if ( !isThermal( s.d ) )
+        return st;
+    double temp_avg = 0.5 * ( s.temp[i] + s.temp[j] ) - s.d.temp0;
+    return st - ( s.d.alpha * temp_avg );
+}
+
+// model( CriticalStretchTag, i, j, r, xi ): BaseFractureModel
+// (CabanaPD_ForceModels.hpp:131-136), the averaged pair coefficient of a
+// multi-material specimen, or ThermalFractureModel (:246-255).
+inline bool criticalStretch( const Sys& s, int64_t i, int64_t j, double r, double xi )
+{
+    if ( isThermal( s.d ) )
+    {
+        double temp_avg = 0.5 * ( s.temp[i] + s.temp[j] ) - s.d.temp0;
+        double bond_break_coeff =
+            ( 1.0 + s.k.s0 + s.d.alpha * temp_avg ) * ( 1.0 + s.k.s0 + s.d.alpha * temp_avg );
+        return r * r >= bond_break_coeff * xi * xi;
+    }
+    return r * r >= pairBcc( s, i, j ) * xi * xi;
+}
+
+// model( ForceCoeffTag, i, j, s, vol, n ) of the PMB family: Elastic
+// (force_models/CabanaPD_PMB.hpp:57-62), or EPP with the update of bond
+// (i, slot)'s plastic stretch (:104-121).
+inline double pmbForceCoeff( Sys& s, int64_t i, int64_t j, int32_t slot, double st, double vol )
+{
+    const double c = pairC( s, i, j );
+    if ( !isEpp( s.d ) )
+        return c * st * vol;
+    const double s_Y = s.k.s_Y;
+    double& s_p = s.sp[i * s.maxn + slot];
+    if ( st >= s_p + s_Y )
+        s_p = st - s_Y;
+    else if ( st <= s_p - s_Y )
+        s_p = st + s_Y;
+    return c * ( st - s_p ) * vol;
+}
+
+// model( EnergyTag, i, j, s, xi, vol, n ) of the PMB family: Elastic
+// (force_models/CabanaPD_PMB.hpp:64-71) or EPP (:123-146).
+inline double pmbEnergy( const Sys& s, int64_t i, int64_t j, int32_t slot, double st, double xi, double vol )
+{
+    const double c = pairC( s, i, j );
+    if ( !isEpp( s.d ) )
+        return 0.25 * c * st * st * xi * vol;
+    const double s_Y = s.k.s_Y;
+    const double s_p = s.sp[i * s.maxn + slot];
+    double stretch_term;
+    if ( st >= s_p + s_Y )
+        stretch_term = s_Y * ( 2.0 * st - s_Y );
+    else if ( st <= s_p - s_Y )
+        stretch_term = s_Y * ( -2.0 * st - s_Y );
+    else
+        stretch_term = st * st;
+    return 0.25 * c * stretch_term * xi * vol;
+}
+
 // Force<PMB|LPS, NoFracture|Fracture>::computeForceFull
 void computeForceFull( Sys& s )
 {
@@ -528,6 +651,7 @@ void computeForceFull( Sys& s )
             double fx_i = 0.0, fy_i = 0.0, fz_i = 0.0;
             double xi, r, st, rx, ry, rz;
             bondGeometry( s, i, j, xi, r, st, rx, ry, rz );
+            st = thermalStretch( s, i, j, st ); // force/CabanaPD_PMB.hpp:109, 257, 442
             if ( !frac )
             {
                 // force/CabanaPD_PMB.hpp:105-118, force/CabanaPD_LPS.hpp:161-174
@@ -535,7 +659,7 @@ void computeForceFull( Sys& s )
                 if ( lps )
                     coeff = lpsForceCoeff( s, i, j, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
                 else
-                    coeff = pairC( s, i, j ) * st * s.vol[j];
+                    coeff = pmbForceCoeff( s, i, j, q, st, s.vol[j] );
                 fx_i = coeff * rx / r;
                 fy_i = coeff * ry / r;
                 fz_i = coeff * rz / r;
@@ -548,7 +672,7 @@ void computeForceFull( Sys& s )
                 // force/CabanaPD_PMB.hpp:259-279; force/CabanaPD_LPS.hpp:416-438
                 // (the LPS no-fail test reads nofail(i) twice, :417-418).
                 const bool nofail_j = lps ? ( s.nofail[i] != 0 ) : ( s.nofail[j] != 0 );
-                if ( r * r >= pairBcc( s, i, j ) * xi * xi && !( s.nofail[i] != 0 ) && !nofail_j )
+                if ( criticalStretch( s, i, j, r, xi ) && !( s.nofail[i] != 0 ) && !nofail_j )
                 {
                     mu[q] = 0;
                 }
@@ -558,7 +682,7 @@ void computeForceFull( Sys& s )
                     if ( lps )
                         coeff = lpsForceCoeff( s, i, j, st, xi, s.vol[j], s.m[i], s.m[j], s.theta[i], s.theta[j] );
                     else
-                        coeff = pairC( s, i, j ) * st * s.vol[j];
+                        coeff = pmbForceCoeff( s, i, j, q, st, s.vol[j] );
                     double muij = mu[q];
                     fx_i = muij * coeff * rx / r;
                     fy_i = muij * coeff * ry / r;
@@ -601,6 +725,7 @@ void computeEnergyFull( Sys& s )
             const int64_t j = row[q];
             double xi, r, st, rx, ry, rz;
             bondGeometry( s, i, j, xi, r, st, rx, ry, rz );
+            st = thermalStretch( s, i, j, st ); // force/CabanaPD_PMB.hpp:143, 319, 478
             double w;
             if ( lps )
             {
@@ -613,8 +738,8 @@ void computeEnergyFull( Sys& s )
             }
             else
             {
-                // model( EnergyTag ) (force_models/CabanaPD_PMB.hpp:64-71)
-                w = 0.25 * pairC( s, i, j ) * st * st * xi * s.vol[j];
+                // model( EnergyTag ) (force_models/CabanaPD_PMB.hpp:64-71, 123-146)
+                w = pmbEnergy( s, i, j, q, st, xi, s.vol[j] );
             }
             if ( frac )
             {
@@ -663,6 +788,7 @@ void computeStressFull( Sys& s )
                 continue;
             double xi, r, st, rx, ry, rz;
             bondGeometry( s, i, j, xi, r, st, rx, ry, rz );
+            st = thermalStretch( s, i, j, st ); // force/CabanaPD_PMB.hpp:176, 373, 510
             const double xi_x = s.x[3 * j + 0] - s.x[3 * i + 0];
             const double xi_y = s.x[3 * j + 1] - s.x[3 * i + 1];
             const double xi_z = s.x[3 * j + 2] - s.x[3 * i + 2];
@@ -673,7 +799,7 @@ void computeStressFull( Sys& s )
                 coeff *= 0.5;
             }
             else
-                coeff = 0.5 * ( pairC( s, i, j ) * st * s.vol[j] ); // 0.5 * model( ForceCoeffTag, ... )
+                coeff = 0.5 * pmbForceCoeff( s, i, j, q, st, s.vol[j] ); // EPP: updates s_p of every bond
             double fx_i, fy_i, fz_i;
             if ( frac )
             {
@@ -948,6 +1074,25 @@ void computeForce( Sys& s, bool reset )
         computeForceFull( s );
 }
 
+// A boundary condition or body term with force_update == false applied at
+// time t (CabanaPD_Solver.hpp:255-257, 350-352): member m gets a + b t.
+void applyPrescribed( Sys::Prescribed& pr, std::vector<double>& dst, double t )
+{
+    const int32_t nc = pr.ncomp;
+    for ( size_t m = 0; m < pr.idx.size(); m++ )
+        for ( int32_t c = 0; c < nc; c++ )
+        {
+            const size_t e = m * nc + c;
+            dst[nc * pr.idx[m] + c] = pr.b.empty() ? pr.a[e] : pr.a[e] + pr.b[e] * t;
+        }
+}
+
+void applyPrescribed( Sys& s, double t )
+{
+    applyPrescribed( s.pre_u, s.u, t );
+    applyPrescribed( s.pre_temp, s.temp, t );
+}
+
 void updateForce( Sys& s )
 {
     // Solver::updateForce (CabanaPD_Solver.hpp:462-477)
@@ -1062,6 +1207,7 @@ int orc_create( const pdk_model_desc* desc, int64_t n, const double* x, const do
     s->m.assign( n, 0.0 );
     s->theta.assign( n, 0.0 );
     s->u_neigh.assign( 3 * n, 0.0 );
+    s->temp.assign( n, 0.0 );
     *out = s;
     return 0;
 }
@@ -1103,6 +1249,9 @@ int orc_build_neighbors( orc_sys s )
     }
     if ( s->d.fracture )
         s->mu.assign( static_cast<size_t>( s->n ) * std::max( s->maxn, 1 ), 1 );
+    // BasePlasticity::updateBonds after the neighbor list (CabanaPD_Solver.hpp:183-185)
+    if ( isEpp( s->d ) )
+        s->sp.assign( static_cast<size_t>( s->n ) * std::max( s->maxn, 1 ), 0.0 );
     return 0;
 }
 
@@ -1213,6 +1362,7 @@ int orc_compute_energy( orc_sys s, double* W, double* D )
 int orc_init( orc_sys s )
 {
     // Solver::init(bc) (CabanaPD_Solver.hpp:229-279)
+    applyPrescribed( *s, 0.0 );
     if ( !s->d.fracture && isLps( s->d ) )
         computeWeightedVolume( *s );
     updateForce( *s );
@@ -1227,9 +1377,11 @@ int orc_step( orc_sys s, int32_t nsteps )
     // Solver::runStep(step, bc) (CabanaPD_Solver.hpp:332-375)
     for ( int32_t k = 0; k < nsteps; k++ )
     {
+        s->steps_done++;
         initialHalfStep( *s );
         if ( isContact( s->d ) )
             s->steps_since_build++;
+        applyPrescribed( *s, s->steps_done * s->d.dt );
         updateForce( *s );
         applyForceTerms( *s );
         finalHalfStep( *s );
@@ -1292,6 +1444,62 @@ int orc_set_materials( orc_sys s, int32_t nmat, const pdk_material* mats, const 
     return 0;
 }
 
+int orc_set_prescribed( orc_sys s, int32_t field, const int64_t* idx, int64_t count, const double* a,
+                        const double* b )
+{
+    if ( !s || count < 0 || ( count > 0 && ( !idx || !a ) ) )
+        return 1;
+    Sys::Prescribed* pr;
+    int32_t nc;
+    if ( field == PDK_FIELD_U )
+    {
+        pr = &s->pre_u;
+        nc = 3;
+    }
+    else if ( field == PDK_FIELD_TEMPERATURE && isThermal( s->d ) )
+    {
+        pr = &s->pre_temp;
+        nc = 1;
+    }
+    else
+        return 1;
+    std::vector<uint8_t> seen( s->n, 0 );
+    for ( int64_t m = 0; m < count; m++ )
+    {
+        if ( idx[m] < 0 || idx[m] >= s->n || seen[idx[m]] )
+            return 1;
+        seen[idx[m]] = 1;
+    }
+    pr->ncomp = nc;
+    pr->idx.assign( idx, idx + count );
+    pr->a.assign( a, a + nc * count );
+    if ( b )
+        pr->b.assign( b, b + nc * count );
+    else
+        pr->b.clear();
+    return 0;
+}
+
+int orc_set_field( orc_sys s, int32_t field, const double* values )
+{
+    if ( !s || !values || field != PDK_FIELD_TEMPERATURE || !isThermal( s->d ) )
+        return 1;
+    s->temp.assign( values, values + s->n );
+    return 0;
+}
+
+int orc_get_plastic( orc_sys s, double* out )
+{
+    // EPP plastic stretch per bond, in the order of orc_get_bonds
+    if ( !s || !out || !isEpp( s->d ) || !s->built )
+        return 1;
+    int64_t off = 0;
+    for ( int64_t i = 0; i < s->n; i++ )
+        for ( int32_t q = 0; q < s->cnt[i]; q++ )
+            out[off++] = s->sp[i * s->maxn + q];
+    return 0;
+}
+
 int orc_compute_stress( orc_sys s )
 {
     if ( isContact( s->d ) || !s->built )
@@ -1323,6 +1531,7 @@ int orc_get( orc_sys s, int32_t field, double* out )
     case PDK_FIELD_PHI: src = &s->phi; break;
     case PDK_FIELD_M: src = &s->m; break;
     case PDK_FIELD_THETA: src = &s->theta; break;
+    case PDK_FIELD_TEMPERATURE: src = &s->temp; break;
     default: return 1;
     }
     std::copy( src->begin(), src->end(), out );

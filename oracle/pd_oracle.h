@@ -46,6 +46,10 @@ int orc_compute_force(orc_sys s, int32_t reset);
 int orc_compute_energy(orc_sys s, double* W, double* D);
 int orc_compute_stress(orc_sys s);
 int orc_set_materials(orc_sys s, int32_t nmat, const pdk_material* mats, const int32_t* type);
+int orc_set_prescribed(orc_sys s, int32_t field, const int64_t* idx, int64_t count, const double* a,
+                       const double* b);
+int orc_set_field(orc_sys s, int32_t field, const double* values);
+int orc_get_plastic(orc_sys s, double* out);
 int orc_init(orc_sys s);
 int orc_step(orc_sys s, int32_t nsteps);
 int orc_get(orc_sys s, int32_t field, double* out);

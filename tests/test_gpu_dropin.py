@@ -11,7 +11,11 @@ against that test's polynomial fit and against the oracle.  dropin_force runs
 the tstForce flow (Force, Neighbor, computeForce, computeEnergy) against
 lattice sums.  dropin_crack_inclusion is examples/mechanics/crack_inclusion.cpp
 (two materials through createMultiForceModel, a functor force boundary
-condition) checking regression_test/tstCrackInclusion.hpp:55.
+condition) checking regression_test/tstCrackInclusion.hpp:55.  dropin_dogbone
+(ElasticPerfectlyPlastic, displacement grips) and dropin_thermal_deformation
+(TemperatureDependent, imposed temperature) are the reference's examples on
+their decks; their force_update = false functors run on the host each step,
+and the results equal the engine's prescribed-value runs bit for bit.
 
 The drivers are built on the CPU beforehand (`make -C tests/cpp`, also done by
 __graft_entry__.build()); the test builds them only when they are missing.
@@ -89,3 +93,68 @@ def test_dropin_crack_inclusion_example(tmp_path):
     deck = os.path.join(HERE, "golden", "crack_inclusion.json")
     rc, out = run([driver("dropin_crack_inclusion"), deck, str(tmp_path / "incl_crack_y.txt")], str(tmp_path), 110)
     assert rc == 0 and "CI_REGRESSION_OK" in out
+
+
+def _deck_variant(tmp_path, name, **changes):
+    import json
+    deck = json.load(open(os.path.join(HERE, "golden", name)))
+    for k, v in changes.items():
+        deck[k]["value"] = v
+    path = tmp_path / name
+    path.write_text(json.dumps(deck))
+    return str(path)
+
+
+def test_dropin_dogbone_example(tmp_path):
+    """examples/mechanics/dogbone_tensile_test.cpp (EPP, displacement grips
+    through a force_update = false functor evaluated on the host each step)
+    on its deck with the grips at 20 m/s for 300 steps: the final
+    displacement equals the engine run of the same specimen with the grips as
+    prescribed a + b t values, bit for bit."""
+    import numpy as np
+
+    from cabanapd_amd import configs
+    from tests.parity import gpu_spec
+    steps = 300
+    deck = _deck_variant(tmp_path, "dogbone_tensile_test.json", grip_velocity=20.0, final_time=(steps + 0.5) * 3.0e-7,
+                         output_frequency=100)
+    out = tmp_path / "dogbone_u.bin"
+    rc, log = run([driver("dropin_dogbone"), deck, str(out)], str(tmp_path), 110)
+    assert rc == 0 and "DB_DONE" in log
+    w = configs.dogbone(grip_velocity=20.0, steps=steps)
+    u_drop = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
+    assert u_drop.shape == w["x"].shape
+    g = gpu_spec(w)
+    configs.setup(g, w)
+    g.step(steps)
+    u = g.get("u")
+    nd = int(np.sum(u_drop != u))
+    say(f"  drop-in vs engine: differing={nd} max|u|={np.max(np.abs(u)):.3e}")
+    assert nd == 0
+    g.close()
+
+
+def test_dropin_thermal_deformation_example(tmp_path):
+    """examples/thermomechanics/thermal_deformation.cpp on its deck (13333
+    steps; the temperature BodyTerm evaluated on the host each step): the
+    final displacement and temperature equal the engine run with the
+    temperature as prescribed a + b t values, bit for bit."""
+    import numpy as np
+
+    from cabanapd_amd import configs
+    from tests.parity import gpu_spec
+    deck = os.path.join(HERE, "golden", "thermal_deformation.json")
+    out = tmp_path / "thermal_u.bin"
+    rc, log = run([driver("dropin_thermal_deformation"), deck, str(out)], str(tmp_path), 110)
+    assert rc == 0 and "TD_DONE" in log
+    w = configs.thermal_deformation()
+    u_drop = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
+    t_drop = np.fromfile(str(out) + ".temp", dtype=np.float64)
+    g = gpu_spec(w)
+    configs.setup(g, w)
+    g.step(w["steps"])
+    u, T = g.get("u"), g.get("temperature")
+    nd = int(np.sum(u_drop != u)) + int(np.sum(t_drop != T))
+    say(f"  drop-in vs engine: differing={nd} max|u|={np.max(np.abs(u)):.3e} T max {T.max():.4f}")
+    assert nd == 0
+    g.close()

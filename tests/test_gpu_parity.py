@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from cabanapd_amd import configs
-from tests.parity import bonds_equal, broken_set, gpu_spec, oracle_spec, pair, rel_err, say
+from tests.parity import bonds_equal, broken_set, ctx, gpu_spec, oracle_spec, pair, rel_err, say
 
 pytestmark = pytest.mark.gpu
 
@@ -400,3 +400,116 @@ def test_gpu_multi_material_rejects_mixed():
     with pytest.raises(engine.PdkError, match="FP64 only"):
         s.set_materials([(1e9, 0.5e9, 1.0), (2e9, 0.5e9, 1.0)], np.zeros(w["x"].shape[0], dtype=np.int32))
     s.close()
+
+
+def _full_compare(g, o, fields=("u", "v", "f")):
+    assert broken_set(g) == broken_set(o)
+    for fld in fields:
+        assert_same(g.get(fld), o.get(fld), fld)
+    Wg, Dg = g.compute_energy()
+    Wo, Do = o.compute_energy()
+    assert_same(g.get("W"), o.get("W"), "W")
+    assert_same(g.get("phi"), o.get("phi"), "phi")
+    assert_total(Wg, Wo, "total strain energy")
+    g.compute_stress()
+    o.compute_stress()
+    assert_same(g.get("stress"), o.get("stress"), "stress")
+
+
+def test_gpu_dogbone_epp_bitwise():
+    """ElasticPerfectlyPlastic PMB with fracture on the dogbone specimen of
+    examples/mechanics/dogbone_tensile_test.cpp, grips pulled at 20 m/s so
+    that bonds yield and some break within 300 steps: bonds, u, v, f, W, phi
+    and stress (which also updates the plastic stretch) equal the oracle's
+    bit for bit; the right grip sits at u_x = v0 t exactly."""
+    say("case: dogbone EPP 251x29x5 cut, v0 20 m/s, 300 steps FP64")
+    w = configs.dogbone(grip_velocity=20.0, steps=300)
+    g, o = pair(w)
+    g.step(w["steps"])
+    o.step(w["steps"])
+    _full_compare(g, o)
+    ux = g.get("u")[w["right_grip"], 0]
+    assert np.all(ux == 20.0 * (w["steps"] * w["desc"].dt))
+    say(f"  broken bonds {len(broken_set(g))}")
+    assert len(broken_set(g)) > 0
+
+
+def test_gpu_thermal_deformation_bitwise():
+    """examples/thermomechanics/thermal_deformation.cpp (PMB, NoFracture,
+    TemperatureDependent, T = temp0 + 5000 (y - y_low) t imposed before each
+    force pass) for 400 steps: temperature, u, v, f, W and stress equal the
+    oracle's bit for bit."""
+    say("case: thermal deformation 101x31x3, 400 steps FP64")
+    w = configs.thermal_deformation(steps=400)
+    g, o = pair(w)
+    g.step(w["steps"])
+    o.step(w["steps"])
+    assert_same(g.get("temperature"), o.get("temperature"), "temperature")
+    _full_compare(g, o)
+    assert np.max(np.abs(g.get("u"))) > 0.0
+
+
+@pytest.mark.parametrize("model,mech,frac", [("PMB", "elastic", True), ("PMB", "epp", True),
+                                             ("LINEAR_PMB", "elastic", False)])
+def test_gpu_thermal_cube_bitwise(model, mech, frac):
+    """TemperatureDependent PMB / LinearPMB, with the thermal critical
+    stretch and EPP, on a cube with a random initial temperature field and a
+    temperature ramp imposed on half the particles: 25 steps bit for bit."""
+    from cabanapd_amd._abi import make_desc
+    say(f"case: thermal {model} {mech} fracture={frac}, cube 11^3, 25 steps")
+    w = configs.lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.0, K=1e9)
+    x = w["x"]
+    n = x.shape[0]
+    rng = np.random.default_rng(9)
+    kw = dict(fracture=True, G0=1.0) if frac else {}
+    if mech == "epp":
+        kw["mechanics"] = "epp"
+        kw["sigma_y"] = 3e6
+    w["desc"] = make_desc(model, horizon=w["desc"].horizon, K=1e9, dt=1e-6, thermal=True, alpha=2e-4, temp0=10.0,
+                          **kw)
+    w["u"] = rng.uniform(-2e-3, 2e-3, size=x.shape) * (2.0 / 11)
+    w["rho"] = np.full(n, 1000.0)
+    w["temperature"] = rng.uniform(0.0, 40.0, size=n)
+    hot = np.flatnonzero(x[:, 2] > 0.0)
+    w["prescribed"] = [("temperature", hot, np.full(hot.size, 10.0), np.full(hot.size, 4e5))]
+    g, o = pair(w)
+    g.step(25)
+    o.step(25)
+    assert_same(g.get("temperature"), o.get("temperature"), "temperature")
+    _full_compare(g, o)
+    if frac:
+        say(f"  broken bonds {len(broken_set(g))}")
+
+
+def test_gpu_plastic_thermal_rejections():
+    """FP64 only; the gather variant only; prescribed specs stay out of the
+    batch engine; prescribed ids are checked."""
+    from cabanapd_amd import engine
+    from cabanapd_amd._abi import make_desc
+    say("case: EPP / thermal argument checks")
+    w = configs.lattice_sum_cube(model="PMB", m=2, nc=7, field="linear", coeff=0.0, K=1e9)
+    base = dict(horizon=w["desc"].horizon, K=1e9, dt=1e-6)
+    with pytest.raises(engine.PdkError, match="FP64"):
+        gpu_spec(w, desc=make_desc("PMB", precision="mixed", thermal=True, alpha=1e-5, **base))
+    with pytest.raises(engine.PdkError, match="fracture"):
+        gpu_spec(w, desc=make_desc("PMB", mechanics="epp", sigma_y=1e6, **base))
+    w["desc"] = make_desc("PMB", fracture=True, G0=1.0, mechanics="epp", sigma_y=1e6, **base)
+    g = gpu_spec(w)
+    configs.setup(g, w)
+    with pytest.raises(engine.PdkError, match="gather"):
+        g.set_tuning(0)
+    with pytest.raises(engine.PdkError, match="repeated"):
+        g.set_prescribed("u", [1, 1], np.zeros((2, 3)))
+    with pytest.raises(engine.PdkError, match="temperature"):
+        g.set_prescribed("temperature", [1], np.zeros(1))
+    with pytest.raises(engine.PdkError, match="batch"):
+        ctx().submit(g, 1)
+    g.close()
+    # an elastic spec with prescribed values stays out of the batch engine too
+    w["desc"] = make_desc("PMB", **base)
+    e = gpu_spec(w)
+    configs.setup(e, w)
+    e.set_prescribed("u", [0], np.zeros((1, 3)))
+    with pytest.raises(engine.PdkError, match="prescribed"):
+        ctx().submit(e, 1)
+    e.close()

@@ -1,0 +1,248 @@
+"""Pins of the oracle's elastic-perfectly-plastic (EPP) and temperature-dependent
+PMB terms and of prescribed values (boundary conditions and body terms with
+force_update == false).
+
+The reference's own tests construct these models without checking numbers
+(unit_test/tstForce.hpp:1047-1102), so the pins come from what the mechanics
+fixes:
+- EPP (force_models/CabanaPD_PMB.hpp:74-147): a bond loaded past s_p + s_Y
+  carries c s_Y V; elastic unloading keeps s_p; the load cycle
+  tension -> unload -> compression -> unload -> tension moves s_p through
+  s_Y, -s_Y, 2 s_Y; the energy branches give s_Y (2 s - s_Y), s^2 and
+  s_Y (-2 s - s_Y); on a uniformly stretched lattice every particle's force
+  and energy scale by s_Y / s and s_Y (2 s - s_Y) / s^2 against the elastic
+  model; a yield stress far above the loading reproduces the elastic model
+  bit for bit.
+- TemperatureDependent (CabanaPD_ForceModels.hpp:165-256): free thermal
+  expansion u = alpha dT x is force and energy free; a heated body held at
+  u = 0 carries the forces of the elastic model at u = -alpha dT x; heating
+  moves the break threshold to (1 + s0 + alpha dT)^2.
+- Prescribed values: a + b t at t = k dt exactly, set before the force pass
+  (the stored force equals a fresh force pass at the final state).
+"""
+import numpy as np
+import pytest
+
+from cabanapd_amd import configs
+from cabanapd_amd._abi import make_desc
+from oracle import binding as orc
+
+
+def two_particles(desc, dx, vols=(0.125, 0.25)):
+    x = np.array([[0.0, 0.0, 0.0], [dx, 0.0, 0.0]])
+    s = orc.OracleSpec(desc, 2, x, np.array(vols, dtype=float))
+    s.build_bonds()
+    return s
+
+
+def stretch_state(s, dx, st):
+    u = np.zeros((2, 3))
+    u[1, 0] = st * dx
+    s.set_state(u=u, v=np.zeros((2, 3)))
+
+
+def test_epp_constants_and_forms():
+    K, G0, sy, horizon = 2.0e9, 1.0e3, 3.0e7, 2.0e-3
+    d = make_desc("PMB", fracture=True, horizon=horizon, K=K, G0=G0, mechanics="epp", sigma_y=sy)
+    k = orc.model_constants(d)
+    assert k["s_Y"] == sy / 3.0 / K
+    s0 = (5.0 * G0 / sy / horizon + sy / K) / 6.0
+    assert k["s0"] == s0 and k["bond_break_coeff"] == (1.0 + s0) * (1.0 + s0)
+    # with TemperatureDependent the thermal fracture model's s0 (influence 1)
+    dt = make_desc("PMB", fracture=True, horizon=horizon, K=K, G0=G0, mechanics="epp", sigma_y=sy,
+                   thermal=True, alpha=1e-5)
+    assert orc.model_constants(dt)["s0"] == np.sqrt(5.0 * G0 / 9.0 / K / horizon)
+    # forms the reference does not have
+    for bad in (make_desc("PMB", horizon=horizon, K=K, mechanics="epp", sigma_y=sy),
+                make_desc("LPS", fracture=True, horizon=horizon, K=K, G=K, G0=G0, mechanics="epp", sigma_y=sy),
+                make_desc("PMB", fracture=True, horizon=horizon, K=K, G0=G0, mechanics="epp", sigma_y=0.0),
+                make_desc("LPS", horizon=horizon, K=K, G=K, thermal=True),
+                make_desc("LINEAR_LPS", horizon=horizon, K=K, G=K, thermal=True)):
+        with pytest.raises(RuntimeError):
+            orc.model_constants(bad)
+
+
+def test_epp_two_particle_load_cycle():
+    dx, K, sy = 0.5, 1.0, 0.003
+    horizon = 1.01 * dx
+    d = make_desc("PMB", fracture=True, horizon=horizon, K=K, G0=1e3, mechanics="epp", sigma_y=sy, dt=1e-3)
+    s = two_particles(d, dx)
+    c = 18.0 * K / (np.pi * horizon ** 4)
+    sY = sy / 3.0 / K
+    V0, V1 = 0.125, 0.25
+    # (stretch, expected s_p after the force pass, expected s - s_p)
+    cycle = [(2.0 * sY, sY, sY), (0.5 * sY, sY, -0.5 * sY), (-2.0 * sY, -sY, -sY),
+             (-0.5 * sY, -sY, 0.5 * sY), (3.0 * sY, 2.0 * sY, sY)]
+    for st, sp_exp, ds in cycle:
+        stretch_state(s, dx, st)
+        s.compute_force(True)
+        f = s.get("f")
+        assert f[0, 0] == pytest.approx(c * ds * V1, rel=1e-9), st
+        assert f[1, 0] == pytest.approx(-c * ds * V0, rel=1e-9), st
+        assert np.all(f[:, 1:] == 0.0)
+        assert np.allclose(s.plastic(), sp_exp, rtol=1e-9, atol=0), st
+    _, _, intact = s.bonds()
+    assert np.all(intact == 1)
+    # energy branches at s_p = 2 s_Y, without a force pass (no update)
+    for st, term in [(4.0 * sY, sY * (2.0 * 4.0 * sY - sY)), (2.0 * sY, 4.0 * sY * sY),
+                     (0.0, sY * (-2.0 * 0.0 - sY))]:
+        stretch_state(s, dx, st)
+        s.compute_energy()
+        W = s.get("W")
+        assert W[0] == pytest.approx(0.25 * c * term * dx * V1, rel=1e-9), st
+        assert W[1] == pytest.approx(0.25 * c * term * dx * V0, rel=1e-9), st
+        assert np.allclose(s.plastic(), 2.0 * sY, rtol=1e-12, atol=0)
+    # the stress pass updates s_p as the force pass does (force/CabanaPD_PMB.hpp:373-376)
+    stretch_state(s, dx, 4.0 * sY)
+    s.compute_stress()
+    sig = s.get("stress")
+    assert np.allclose(s.plastic(), 3.0 * sY, rtol=1e-9, atol=0)
+    assert sig[0, 0, 0] == pytest.approx(0.5 * c * sY * V1 * dx, rel=1e-9)
+
+
+def _lattice(desc_kw, u_fn, nc=11, m=3):
+    w = configs.lattice_sum_cube(model="PMB", m=m, nc=nc, field="linear", coeff=0.0, K=1.0)
+    horizon = w["desc"].horizon
+    d = make_desc("PMB", horizon=horizon, K=1.0, dt=1e-7, **desc_kw)
+    x = w["x"]
+    s = orc.OracleSpec(d, x.shape[0], x, w["vol"], u=u_fn(x))
+    s.build_bonds()
+    return s, x
+
+
+def test_epp_uniform_stretch_scales_elastic():
+    sY = 1e-3
+    st = 5.0 * sY
+    epp, x = _lattice(dict(fracture=True, G0=1e3, mechanics="epp", sigma_y=3.0 * sY), lambda x: st * x)
+    el, _ = _lattice(dict(fracture=True, G0=1e3), lambda x: st * x)
+    # energy before any force pass (s_p = 0): tension branch on every bond
+    epp.compute_energy()
+    el.compute_energy()
+    ratio = epp.get("W") / el.get("W")
+    assert np.allclose(ratio, sY * (2.0 * st - sY) / (st * st), rtol=1e-12, atol=0)
+    # force pass: every bond yields, s_p = s - s_Y, each bond carries c s_Y V
+    epp.compute_force(True)
+    el.compute_force(True)
+    fe, fl = epp.get("f"), el.get("f")
+    scale = np.max(np.abs(fl))
+    assert scale > 0
+    assert np.max(np.abs(fe - (sY / st) * fl)) <= 1e-9 * scale
+    assert np.allclose(epp.plastic(), st - sY, rtol=1e-9, atol=0)
+    assert np.all(epp.bonds()[2] == 1)
+
+
+def test_epp_high_yield_is_elastic_bitwise():
+    rng = np.random.default_rng(5)
+    u0 = None
+
+    def u_fn(x):
+        nonlocal u0
+        u0 = rng.uniform(-1e-4, 1e-4, size=x.shape)
+        return u0
+
+    epp, x = _lattice(dict(fracture=True, G0=1e3, mechanics="epp", sigma_y=1e200), u_fn, nc=9, m=2)
+    el, _ = _lattice(dict(fracture=True, G0=1e3), lambda x: u0, nc=9, m=2)
+    for s in (epp, el):
+        s.init()
+        s.step(15)
+    assert np.array_equal(epp.get("u"), el.get("u"))
+    assert np.array_equal(epp.get("f"), el.get("f"))
+    assert np.all(epp.plastic() == 0.0)
+    We, Wl = epp.compute_energy()[0], el.compute_energy()[0]
+    assert We == pytest.approx(Wl, rel=1e-13)
+
+
+def test_thermal_free_expansion_is_force_free():
+    alpha, T0, dT = 1e-5, 20.0, 150.0
+    kw = dict(thermal=True, alpha=alpha, temp0=T0)
+    th, x = _lattice(kw, lambda x: alpha * dT * x)
+    th.set_field("temperature", np.full(x.shape[0], T0 + dT))
+    th.compute_force(True)
+    th.compute_energy()
+    el, _ = _lattice({}, lambda x: alpha * dT * x)
+    el.compute_force(True)
+    el.compute_energy()
+    assert np.max(np.abs(th.get("f"))) <= 1e-9 * np.max(np.abs(el.get("f")))
+    assert np.max(np.abs(th.get("W"))) <= 1e-9 * np.max(np.abs(el.get("W")))
+    assert np.array_equal(th.get("temperature"), np.full(x.shape[0], T0 + dT))
+
+
+def test_thermal_held_body_matches_elastic_contraction():
+    alpha, T0, dT = 1e-5, 20.0, 150.0
+    th, x = _lattice(dict(thermal=True, alpha=alpha, temp0=T0), lambda x: np.zeros_like(x))
+    th.set_field("temperature", np.full(x.shape[0], T0 + dT))
+    th.compute_force(True)
+    el, _ = _lattice({}, lambda x: -alpha * dT * x)
+    el.compute_force(True)
+    fl = el.get("f")
+    assert np.max(np.abs(th.get("f") - fl)) <= 1e-7 * np.max(np.abs(fl))
+    # stress: the same bonds, the same half forces along the reference bond
+    th.compute_stress()
+    el.compute_stress()
+    sl = el.get("stress")
+    assert np.max(np.abs(th.get("stress") - sl)) <= 1e-7 * np.max(np.abs(sl))
+
+
+def test_thermal_break_threshold_moves_with_temperature():
+    dx, K, G0, alpha, T0 = 0.5, 1.0, 1e-4, 1e-3, 0.0
+    horizon = 1.01 * dx
+    d = make_desc("PMB", fracture=True, horizon=horizon, K=K, G0=G0, thermal=True, alpha=alpha, temp0=T0, dt=1e-3)
+    s0 = orc.model_constants(d)["s0"]
+    assert s0 == np.sqrt(5.0 * G0 / 9.0 / K / horizon)
+    for dT, st, expect in [(0.0, 1.01 * s0, 0), (0.0, 0.99 * s0, 1), (10.0, s0 + 0.5 * alpha * 10.0, 1),
+                           (10.0, s0 + 1.5 * alpha * 10.0, 0)]:
+        s = two_particles(d, dx)
+        s.set_field("temperature", np.full(2, T0 + dT))
+        stretch_state(s, dx, st)
+        s.compute_force(True)
+        assert np.all(s.bonds()[2] == expect), (dT, st)
+        if expect:
+            # intact: force from the thermal stretch s - alpha dT
+            c = 18.0 * K / (np.pi * horizon ** 4)
+            assert s.get("f")[0, 0] == pytest.approx(c * (st - alpha * dT) * 0.25, rel=1e-9)
+
+
+@pytest.mark.parametrize("thermal", [False, True])
+def test_prescribed_values_affine_in_time(thermal):
+    w = configs.lattice_sum_cube(model="PMB", m=2, nc=7, field="linear", coeff=0.0, K=1.0)
+    x = w["x"]
+    n = x.shape[0]
+    kw = dict(thermal=True, alpha=1e-4, temp0=5.0) if thermal else {}
+    dt = 1e-3
+    d = make_desc("PMB", horizon=w["desc"].horizon, K=1.0, dt=dt, **kw)
+    s = orc.OracleSpec(d, n, x, w["vol"], rho=np.full(n, 1.0))
+    s.build_bonds()
+    grip = np.flatnonzero(x[:, 0] > 0.6)
+    a = np.zeros((grip.size, 3))
+    b = np.zeros((grip.size, 3))
+    b[:, 0] = 0.01
+    s.set_prescribed("u", grip, a, b)
+    if thermal:
+        Ta = np.full(n, 5.0)
+        Tb = 50.0 * (x[:, 1] + 1.0)
+        s.set_prescribed("temperature", np.arange(n), Ta, Tb)
+    s.init()
+    assert np.all(s.get("u")[grip] == 0.0)
+    steps = 12
+    s.step(steps)
+    t = steps * dt
+    u = s.get("u")
+    assert np.array_equal(u[grip, 0], 0.0 + 0.01 * t * np.ones(grip.size))
+    assert np.all(u[grip, 1:] == 0.0)
+    free = np.setdiff1d(np.arange(n), grip)
+    assert np.any(u[free] != 0.0)
+    if thermal:
+        assert np.array_equal(s.get("temperature"), Ta + Tb * t)
+    # the stored force was computed after the values were set
+    f = s.get("f")
+    s.compute_force(True)
+    assert np.array_equal(s.get("f"), f)
+    # a second call replaces the set; count 0 removes it
+    s.set_prescribed("u", [], np.zeros((0, 3)))
+    s.step(1)
+    assert not np.array_equal(s.get("u")[grip, 0], u[grip, 0])
+    with pytest.raises(RuntimeError):
+        s.set_prescribed("u", [0, 0], np.zeros((2, 3)))
+    if not thermal:
+        with pytest.raises(RuntimeError):
+            s.set_prescribed("temperature", [0], np.zeros(1))
