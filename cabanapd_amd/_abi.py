@@ -19,6 +19,7 @@ PDK_MECH_ELASTIC = 0
 PDK_MECH_EPP = 1
 PDK_TEMP_INDEPENDENT = 0
 PDK_TEMP_DEPENDENT = 1
+PDK_TEMP_DYNAMIC = 2
 VECTOR_FIELDS = {"x", "u", "v", "f"}
 
 
@@ -119,6 +120,10 @@ class ModelDesc(C.Structure):
         ("sigma_y", C.c_double),
         ("alpha", C.c_double),
         ("temp0", C.c_double),
+        ("kappa", C.c_double),
+        ("cp", C.c_double),
+        ("thermal_subcycle", C.c_int32),
+        ("linear_microconductivity", C.c_int32),
     ]
 
 
@@ -145,14 +150,15 @@ class ModelConstants(C.Structure):
     _fields_ = [(name, C.c_double) for name in (
         "c", "s0", "bond_break_coeff", "theta_coeff", "s_coeff",
         "Rs", "Es", "Gs", "beta", "coeff_h_n", "coeff_h_d", "Gamma", "a0",
-        "delta_tear", "fc", "s_Y")]
+        "delta_tear", "fc", "s_Y", "thermal_coeff")]
 
 
 def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
               G=0.0, G0=0.0, dt=1e-7, influence=None, neighbor_tol=1e-14,
               radius=0.0, radius_extend=0.0, nu=0.25, E=0.0, restitution=1.0,
               gamma=0.0, mu_friction=0.0, rebuild_every=0, polydisperse=False, ordering=0, bond_layout=0,
-              mechanics="elastic", sigma_y=0.0, thermal=False, alpha=0.0, temp0=0.0):
+              mechanics="elastic", sigma_y=0.0, thermal=False, alpha=0.0, temp0=0.0, kappa=0.0, cp=0.0,
+              thermal_subcycle=1, linear_microconductivity=False):
     """Model description with the reference's constructor defaults.
 
     influence defaults: PMB fracture s0 uses influence_type 1
@@ -160,7 +166,8 @@ def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
     (force_models/CabanaPD_LPS.hpp:46-47).  mechanics "epp" selects
     ElasticPerfectlyPlastic (PMB with fracture, yield stress sigma_y);
     thermal=True selects TemperatureDependent (expansion alpha, reference
-    temperature temp0)."""
+    temperature temp0); thermal="dynamic" adds heat conduction (kappa, cp,
+    every thermal_subcycle steps)."""
     kinds = {"PMB": PDK_MODEL_PMB, "LPS": PDK_MODEL_LPS, "LINEAR_PMB": PDK_MODEL_LINEAR_PMB,
              "LINEAR_LPS": PDK_MODEL_LINEAR_LPS,
              "HERTZ": PDK_MODEL_HERTZ, "HERTZ_JKR": PDK_MODEL_HERTZ_JKR}
@@ -191,7 +198,11 @@ def make_desc(model="PMB", fracture=False, precision="fp64", horizon=0.0, K=0.0,
     d.bond_layout = bond_layout
     d.mechanics = {"elastic": PDK_MECH_ELASTIC, "epp": PDK_MECH_EPP}[mechanics.lower()]
     d.sigma_y = sigma_y
-    d.thermal = PDK_TEMP_DEPENDENT if thermal else PDK_TEMP_INDEPENDENT
+    d.thermal = (PDK_TEMP_DYNAMIC if thermal == "dynamic" else PDK_TEMP_DEPENDENT) if thermal else PDK_TEMP_INDEPENDENT
+    d.kappa = kappa
+    d.cp = cp
+    d.thermal_subcycle = thermal_subcycle
+    d.linear_microconductivity = 1 if linear_microconductivity else 0
     d.alpha = alpha
     d.temp0 = temp0
     return d

@@ -307,6 +307,42 @@ def thermal_deformation(steps=None):
     return w
 
 
+def thermal_deformation_heat_transfer(steps=None, thermal_subcycle=None, num_cells=(49, 49, 49)):
+    """examples/thermomechanics/inputs/thermal_deformation_heat_transfer.json run
+    through thermal_deformation_heat_transfer.cpp: PMB NoFracture
+    DynamicTemperature, E 115 GPa, nu 0.25, rho 8915, alpha 17e-6, kappa 387,
+    cp 385, temp0 100 (also the initial temperature), horizon 0.00615
+    (+1e-10), dt 4.2e-7, heat conduction every 45000 steps (final time 8 s);
+    49^3 cells of 0.1^3.  The temperature is held at 0 on the two planes
+    within one spacing of the low and high y faces (a force_update = false
+    functor before each force pass)."""
+    size = np.array([0.1, 0.1, 0.1])
+    low, high = -0.5 * size, 0.5 * size
+    x, vol, d = grid_cell_centred(low, high, num_cells)
+    E, nu = 115e9, 0.25
+    K = E / (3 * (1 - 2 * nu))
+    horizon = 0.00615 + 1e-10
+    dt = 4.2e-7
+    if steps is None:
+        steps = int(8.0 / dt)
+    if thermal_subcycle is None:
+        thermal_subcycle = 45000
+    temp0 = 100.0
+    desc = make_desc("PMB", horizon=horizon, K=K, dt=dt, thermal="dynamic", alpha=17e-6, temp0=temp0, kappa=387.0,
+                     cp=385.0, thermal_subcycle=thermal_subcycle)
+    w = _base(desc, x, vol, 8915.0, "thermal_heat_transfer_%dx%dx%d" % tuple(num_cells), d[0], steps,
+              output_every=10)
+    n = x.shape[0]
+    w["temperature"] = np.full(n, temp0)
+    dy = d[1]
+    full = (x[:, 0] >= low[0]) & (x[:, 0] <= high[0]) & (x[:, 2] >= low[2]) & (x[:, 2] <= high[2])
+    plane1 = full & (x[:, 1] >= high[1] - dy) & (x[:, 1] <= high[1] + dy)
+    plane2 = full & (x[:, 1] >= low[1] - dy) & (x[:, 1] <= low[1] + dy)
+    cold = np.flatnonzero(plane1 | plane2)
+    w["prescribed"] = [("temperature", cold, np.zeros(cold.size), None)]
+    return w
+
+
 def kw_crack_y(x, phi):
     """Output time series value of kalthoff_winkler.cpp:134-145: max over
     particles of (phi > 0.3 ? y : 0)."""

@@ -93,7 +93,17 @@ void modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
         if ( isLpsModel( d.model ) )
             require( d.influence == 0 || d.influence == 1, "Influence function type must be 0 or 1." );
         require( d.mechanics == PDK_MECH_ELASTIC || d.mechanics == PDK_MECH_EPP, "unknown mechanics" );
-        require( d.thermal == PDK_TEMP_INDEPENDENT || d.thermal == PDK_TEMP_DEPENDENT, "unknown thermal type" );
+        require( d.thermal == PDK_TEMP_INDEPENDENT || d.thermal == PDK_TEMP_DEPENDENT || d.thermal == PDK_TEMP_DYNAMIC,
+                 "unknown thermal type" );
+        if ( d.thermal == PDK_TEMP_DYNAMIC )
+        {
+            require( d.kappa > 0 && d.cp > 0, "DynamicTemperature needs kappa > 0 and cp > 0" );
+            require( d.thermal_subcycle >= 1, "thermal_subcycle must be >= 1" );
+            // BaseDynamicTemperatureModel (CabanaPD_ForceModels.hpp:277-288),
+            // thermal horizon = force horizon
+            const double d3 = delta * delta * delta;
+            k.thermal_coeff = 9.0 / 2.0 * d.kappa / kPi / d3;
+        }
         if ( d.mechanics == PDK_MECH_EPP )
         {
             // ForceModel<PMB, ElasticPerfectlyPlastic, Fracture> only
@@ -109,7 +119,7 @@ void modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
                 k.bond_break_coeff = ( 1.0 + k.s0 ) * ( 1.0 + k.s0 );
             }
         }
-        if ( d.thermal == PDK_TEMP_DEPENDENT )
+        if ( d.thermal != PDK_TEMP_INDEPENDENT )
         {
             // PMB and LinearPMB (force_models/CabanaPD_PMB.hpp:306-568, 575-597)
             require( d.model == PDK_MODEL_PMB || d.model == PDK_MODEL_LINEAR_PMB,
@@ -187,7 +197,7 @@ struct pdk_spec_s
     hipStream_t st = nullptr;
     DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
         slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stress, mtype, splas,
-        temp;
+        temp, cond;
     // Prescribed values (force_update == false conditions): members (device
     // ids), a and b ([count][ncomp]); one set for U and one for TEMPERATURE.
     struct Prescribed
@@ -242,11 +252,20 @@ struct pdk_spec_s
 
     // Specs whose force passes need the FP64 gather variant: several
     // materials, EPP (per-bond state) or a temperature field.
-    bool gatherOnly() const
-    {
-        return nmat > 1 || d.mechanics == PDK_MECH_EPP || d.thermal == PDK_TEMP_DEPENDENT;
-    }
+    bool gatherOnly() const { return nmat > 1 || d.mechanics == PDK_MECH_EPP || d.thermal != PDK_TEMP_INDEPENDENT; }
     bool hasPrescribed() const { return pre_u.count > 0 || pre_temp.count > 0; }
+    bool dynamicTemp() const { return d.thermal == PDK_TEMP_DYNAMIC; }
+    // Work between the initial half step and the force pass of step g:
+    // heat transfer every thermal_subcycle steps (CabanaPD_Solver.hpp:340-347),
+    // then the prescribed values at t = g dt (:350-352).
+    bool hasPreForce() const { return hasPrescribed() || dynamicTemp(); }
+    void preForce( int64_t g )
+    {
+        if ( dynamicTemp() && g % d.thermal_subcycle == 0 )
+            timed( kClsOther, [&] { launchHeatTransfer( dargs(), h, d.thermal_subcycle * d.dt, st ); } );
+        if ( hasPrescribed() )
+            applyPrescribed( static_cast<double>( g ) * d.dt );
+    }
     // The prescribed values at time t, before a force pass.
     void applyPrescribed( double t )
     {
@@ -363,6 +382,11 @@ struct pdk_spec_s
         h.temp = temp ? temp.as<double>() : nullptr;
         h.alpha = d.alpha;
         h.temp0 = d.temp0;
+        h.cond = cond ? cond.as<double>() : nullptr;
+        h.thermal_coeff = k.thermal_coeff;
+        h.cp = d.cp;
+        h.thermal_horizon = d.horizon;
+        h.lin_micro = d.linear_microconductivity;
         h.nmat = nmat;
         h.mtype = nmat > 1 ? mtype.as<uint8_t>() : nullptr;
         for ( int t = 0; t < 16; t++ )
@@ -678,11 +702,13 @@ int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spe
         s->slice_ptr.alloc( ( s->nslices + 1 ) * sizeof( int64_t ), a );
         s->partial.alloc( std::max<int64_t>( 2 * ( s->nslices / kSlicesPerBlock + 2 ), 64 ) * sizeof( double ), a );
         s->args.alloc( sizeof( PDArgs ), a );
-        if ( desc->thermal == PDK_TEMP_DEPENDENT )
+        if ( desc->thermal != PDK_TEMP_INDEPENDENT )
         {
             // zero at creation (CabanaPD_Particles.hpp:860-864)
             s->temp.alloc( d1, a );
             PDK_HIP( hipMemsetAsync( s->temp.as<double>(), 0, d1, s->st ) );
+            if ( desc->thermal == PDK_TEMP_DYNAMIC )
+                s->cond.alloc( d1, a );
         }
         if ( isContact( *desc ) )
         {
@@ -1152,7 +1178,7 @@ int32_t prescribedComponents( pdk_spec s, int32_t field )
     if ( field == PDK_FIELD_U )
         return 3;
     require( field == PDK_FIELD_TEMPERATURE, "prescribed values are for PDK_FIELD_U or PDK_FIELD_TEMPERATURE" );
-    require( s->d.thermal == PDK_TEMP_DEPENDENT, "the spec has no temperature field (TemperatureIndependent)" );
+    require( s->d.thermal != PDK_TEMP_INDEPENDENT, "the spec has no temperature field (TemperatureIndependent)" );
     return 1;
 }
 } // namespace
@@ -1213,7 +1239,7 @@ int pdk_set_field( pdk_spec s, int32_t field, const double* values )
         require( s && s->uploaded && values, "spec not uploaded" );
         require( !s->in_batch, "spec is running in the batch engine" );
         require( field == PDK_FIELD_TEMPERATURE, "pdk_set_field sets PDK_FIELD_TEMPERATURE" );
-        require( s->d.thermal == PDK_TEMP_DEPENDENT, "the spec has no temperature field (TemperatureIndependent)" );
+        require( s->d.thermal != PDK_TEMP_INDEPENDENT, "the spec has no temperature field (TemperatureIndependent)" );
         s->uploadScalar( values, s->temp.as<double>() );
         PDK_HIP( hipStreamSynchronize( s->st ) );
     } );
@@ -1419,10 +1445,10 @@ int pdk_step( pdk_spec s, int32_t nsteps )
             // the previous step's force epilogue.
             s->timed( kClsOther, [&] { launchInitialHalfStep( s->dargs(), s->h, s->cur, s->st ); } );
             s->cur ^= 1;
-            // prescribed values of step g at t = g dt, after its initial half step
-            const bool pre = s->hasPrescribed();
+            // heat transfer / prescribed values of step g, after its initial half step
+            const bool pre = s->hasPreForce();
             if ( pre )
-                s->applyPrescribed( static_cast<double>( s->steps_done + 1 ) * s->d.dt );
+                s->preForce( s->steps_done + 1 );
             int32_t k = 1;
             // middle steps in pairs from a captured graph (not while kernels
             // are being timed: events are recorded around each launch)
@@ -1439,7 +1465,7 @@ int pdk_step( pdk_spec s, int32_t nsteps )
                 {
                     s->cur ^= 1;
                     if ( pre )
-                        s->applyPrescribed( static_cast<double>( s->steps_done + k + 1 ) * s->d.dt );
+                        s->preForce( s->steps_done + k + 1 );
                 }
             }
         }
@@ -1894,7 +1920,8 @@ int pdk_submit( pdk_ctx ctx, pdk_spec s, int32_t nsteps, uint64_t* ticket )
         require( s->ctx == ctx, "spec belongs to another context" );
         require( !s->gatherOnly(),
                  "multi-material, EPP and temperature-dependent specimens are stepped with pdk_step, not the batch engine" );
-        require( !s->hasPrescribed(), "specimens with prescribed values are stepped with pdk_step, not the batch engine" );
+        require( !s->hasPreForce(),
+                 "specimens with prescribed values or heat transfer are stepped with pdk_step, not the batch engine" );
         *ticket = ctx->batch->submit( s, nsteps );
     } );
 }

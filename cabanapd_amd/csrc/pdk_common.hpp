@@ -127,8 +127,13 @@ struct PDArgs
     // expansion coefficient and reference temperature.
     double* splas;
     double sY;
-    const double* temp;
+    double* temp;
     double alpha, temp0;
+    // DynamicTemperature: conduction per particle (heat pass scratch),
+    // microconductivity constant, cp, horizon, linear microconductivity flag
+    double* cond;
+    double thermal_coeff, cp, thermal_horizon;
+    int32_t lin_micro;
 };
 
 constexpr int kRowMaxSlots = 9;   // slots per row window (|dx| <= 4)

@@ -1670,6 +1670,68 @@ __global__ __launch_bounds__( kBlock ) void kStress( const PDArgs* __restrict__ 
     st[8 * n + p] = s22;
 }
 
+// ---------------------------------------------------------------------------
+// Heat transfer (DynamicTemperature): HeatTransfer::computeHeatTransferFull
+// (CabanaPD_HeatTransfer.hpp:41-62, 101-140) -- conduction of particle i over
+// its bonds (intact bonds only with fracture), in the bond order of the other
+// passes -- then forwardEuler (:66-83) in a second launch, since every
+// conduction reads the neighbours' temperatures before the update.  FP64,
+// one lane per particle, like the energy pass.
+// ---------------------------------------------------------------------------
+template <bool FRAC>
+__global__ __launch_bounds__( kBlock ) void kHeatConduction( const PDArgs* __restrict__ specs )
+{
+#pragma clang fp contract( off )
+    const PDArgs& A = *specs;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int wave = threadIdx.x / kSlice;
+    const int64_t slice = blockIdx.x * (int64_t)kSlicesPerBlock + wave;
+    const int64_t n = A.n;
+    const int64_t p = slice * kSlice + lane;
+    if ( slice >= A.nslices || p >= n )
+        return;
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ T = A.temp;
+    const double* __restrict__ vol = A.vol;
+    const double xi0 = x[p], xi1 = x[n + p], xi2 = x[2 * n + p];
+    const double Ti = T[p];
+    const int32_t mycnt = A.cnt[p];
+    const int64_t base = A.slice_ptr[slice];
+    const int32_t qend = slotEnd( A, static_cast<int32_t>( A.slice_ptr[slice + 1] - base ), mycnt );
+    const int32_t* __restrict__ colp = A.col + base * kSlice + lane;
+    const uint64_t* __restrict__ bits = A.intact + base;
+    const double tc = A.thermal_coeff;
+    const double th = A.thermal_horizon;
+    double conduction = 0.0;
+    for ( int32_t q = 0; q < qend; q++ )
+    {
+        if ( !slotBond( A, base, q, lane, mycnt ) )
+            continue;
+        if constexpr ( FRAC )
+            if ( !( ( bits[q] >> lane ) & 1ull ) )
+                continue;
+        const int64_t j = colp[q * kSlice];
+        const double xi_x = x[j] - xi0;
+        const double xi_y = x[n + j] - xi1;
+        const double xi_z = x[2 * n + j] - xi2;
+        const double xi = sqrt( xi_x * xi_x + xi_y * xi_y + xi_z * xi_z );
+        // microconductivity_function (CabanaPD_ForceModels.hpp:302-308)
+        const double coeff = A.lin_micro ? 4.0 * tc * ( 1.0 - xi / th ) : tc;
+        conduction += coeff * ( T[j] - Ti ) / xi / xi * vol[j];
+    }
+    A.cond[p] = conduction;
+}
+
+__global__ __launch_bounds__( 256 ) void kHeatEuler( const PDArgs* __restrict__ specs, double dt )
+{
+#pragma clang fp contract( off )
+    const PDArgs& A = *specs;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( p >= A.n )
+        return;
+    A.temp[p] += dt / A.rho[p] / A.cp * A.cond[p];
+}
+
 // Velocity-Verlet initial half step on its own (CabanaPD_Integrate.hpp:89-116):
 // the first step of a pdk_step call; later steps run it in the force epilogue.
 __global__ __launch_bounds__( 256 ) void kInitialHalfStep( const PDArgs* __restrict__ specs,
@@ -1952,6 +2014,18 @@ void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, doub
                                    : ( a.fracture ? kEnergy<kPMB, true> : kEnergy<kPMB, false> );
     hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( nblocks ) ), dim3( kBlock ), 0, st, args_dev, parity,
                         a.m, a.theta, partial_dev );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchHeatTransfer( const PDArgs* args_dev, const PDArgs& a, double dt, hipStream_t st )
+{
+    if ( a.nslices == 0 )
+        return;
+    hipLaunchKernelGGL( a.fracture ? kHeatConduction<true> : kHeatConduction<false>, dim3( sliceBlocks( a ) ),
+                        dim3( kBlock ), 0, st, args_dev );
+    PDK_HIP( hipGetLastError() );
+    hipLaunchKernelGGL( kHeatEuler, dim3( static_cast<unsigned>( ( a.n + 255 ) / 256 ) ), dim3( 256 ), 0, st, args_dev,
+                        dt );
     PDK_HIP( hipGetLastError() );
 }
 

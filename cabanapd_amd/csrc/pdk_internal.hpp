@@ -221,6 +221,8 @@ void launchScatterRows( const double* src_soa, const int64_t* perm, double* dst_
                         hipStream_t st );
 void launchInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
 void launchApplyForceTerms( const PDArgs* args_dev, const PDArgs& a, hipStream_t st );
+// Heat conduction over the bonds, then forward Euler of the temperature with dt.
+void launchHeatTransfer( const PDArgs* args_dev, const PDArgs& a, double dt, hipStream_t st );
 // dst[c * n + idx[m]] = a[m * ncomp + c] (+ b[m * ncomp + c] * t when b is set)
 void launchPrescribed( const int64_t* idx_dev, int64_t count, const double* a_dev, const double* b_dev, int32_t ncomp,
                        double t, double* dst_dev, int64_t n, hipStream_t st );

@@ -234,24 +234,50 @@ struct ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureIndependent
 // ---- PMB TemperatureDependent (force_models/CabanaPD_PMB.hpp:306-568) ------------
 // The temperature field lives in the particles (sliceTemperature) and on the
 // device; alpha, temp0 and the slice are kept as the reference's members.
-template <class Tag, class Mech, class F, class TemperatureType>
+// DynamicTemperature adds BaseDynamicTemperatureModel's kappa, cp,
+// microconductivity (CabanaPD_ForceModels.hpp:262-310); the Solver sets
+// thermal_subcycle from the deck's "thermal_subcycle_steps".
+template <class Tag, class Mech, class F, class TemperatureType, class ThermalTag = TemperatureDependent>
 struct ThermalPMB : PDForceModel<Tag, F>
 {
-    using thermal_type = TemperatureDependent;
+    using thermal_type = ThermalTag;
     using mechanics_type = Mech;
     double alpha = 0.0;
     double temp0 = 0.0;
     double s_Y = 0.0;
     TemperatureType temperature;
+    // DynamicTemperature
+    double thermal_horizon = 0.0;
+    double thermal_coeff = 0.0;
+    double kappa = 0.0;
+    double cp = 0.0;
+    bool constant_microconductivity = true;
+    int thermal_subcycle = 1;
+
+    pdk_model_desc desc( double dt ) const
+    {
+        pdk_model_desc d = PDForceModel<Tag, F>::desc( dt );
+        d.kappa = kappa;
+        d.cp = cp;
+        d.linear_microconductivity = constant_microconductivity ? 0 : 1;
+        d.thermal_subcycle = thermal_subcycle;
+        return d;
+    }
 
   protected:
+    void setHeat( double k, double c, bool constant )
+    {
+        kappa = k;
+        cp = c;
+        constant_microconductivity = constant;
+    }
     void setThermal( double h, double k, double g0, double sy, const TemperatureType& t, double a, double t0 )
     {
         this->force_horizon = h;
         this->K = k;
         this->G0 = g0;
         this->influence_type = 1; // ThermalFractureModel's default (CabanaPD_ForceModels.hpp:228-236)
-        this->thermal_ = PDK_TEMP_DEPENDENT;
+        this->thermal_ = std::is_same_v<ThermalTag, DynamicTemperature> ? PDK_TEMP_DYNAMIC : PDK_TEMP_DEPENDENT;
         this->alpha_ = a;
         this->temp0_ = t0;
         if ( std::is_same_v<Mech, ElasticPerfectlyPlastic> )
@@ -262,8 +288,14 @@ struct ThermalPMB : PDForceModel<Tag, F>
         alpha = a;
         temp0 = t0;
         temperature = t;
-        this->derive();
-        s_Y = detail::constantsOf( this->desc( 1.0 ) ).s_Y;
+        thermal_horizon = h;
+        // constants from the full description (the heat terms included)
+        const pdk_model_constants kc = detail::constantsOf( desc( 1.0 ) );
+        this->c = kc.c;
+        this->s0 = kc.s0;
+        this->bond_break_coeff = kc.bond_break_coeff;
+        s_Y = kc.s_Y;
+        thermal_coeff = kc.thermal_coeff;
     }
 };
 
@@ -310,11 +342,75 @@ struct ForceModel<LinearPMB, Elastic, NoFracture, TemperatureDependent, Temperat
     }
 };
 
+// DynamicTemperature forms (force_models/CabanaPD_PMB.hpp:421-568)
+template <class TemperatureType>
+struct ForceModel<PMB, Elastic, NoFracture, DynamicTemperature, TemperatureType>
+    : ThermalPMB<PMB, Elastic, NoFracture, TemperatureType, DynamicTemperature>
+{
+    ForceModel( PMB, NoFracture, const double h, const double k, const TemperatureType& t, const double kap,
+                const double c, const double a, const double t0 = 0.0, const bool constant = true )
+    {
+        this->setHeat( kap, c, constant );
+        this->setThermal( h, k, 0.0, 0.0, t, a, t0 );
+    }
+};
+template <class TemperatureType>
+struct ForceModel<PMB, Elastic, Fracture, DynamicTemperature, TemperatureType>
+    : ThermalPMB<PMB, Elastic, Fracture, TemperatureType, DynamicTemperature>
+{
+    ForceModel( PMB, const double h, const double k, const double g0, const TemperatureType t, const double kap,
+                const double c, const double a, const double t0 = 0.0, const bool constant = true )
+    {
+        this->setHeat( kap, c, constant );
+        this->setThermal( h, k, g0, 0.0, t, a, t0 );
+    }
+};
+template <class TemperatureType>
+struct ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, DynamicTemperature, TemperatureType>
+    : ThermalPMB<PMB, ElasticPerfectlyPlastic, Fracture, TemperatureType, DynamicTemperature>
+{
+    ForceModel( PMB, ElasticPerfectlyPlastic, const double h, const double k, const double g0, const double sigma_y,
+                const TemperatureType& t, const double kap, const double c, const double a, const double t0 = 0.0,
+                const bool constant = true )
+    {
+        this->setHeat( kap, c, constant );
+        this->setThermal( h, k, g0, sigma_y, t, a, t0 );
+    }
+};
+
 namespace detail
 {
 template <class T>
 using IfField = std::enable_if_t<!std::is_arithmetic_v<T>, int>;
 }
+
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, NoFracture, double, double, TT, double, double, double )
+    -> ForceModel<PMB, Elastic, NoFracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, NoFracture, double, double, TT, double, double, double, double )
+    -> ForceModel<PMB, Elastic, NoFracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, NoFracture, double, double, TT, double, double, double, double, bool )
+    -> ForceModel<PMB, Elastic, NoFracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, double, double, double, TT, double, double, double )
+    -> ForceModel<PMB, Elastic, Fracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, double, double, double, TT, double, double, double, double )
+    -> ForceModel<PMB, Elastic, Fracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, double, double, double, TT, double, double, double, double, bool )
+    -> ForceModel<PMB, Elastic, Fracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, ElasticPerfectlyPlastic, double, double, double, double, TT, double, double, double )
+    -> ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, ElasticPerfectlyPlastic, double, double, double, double, TT, double, double, double, double )
+    -> ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, DynamicTemperature, TT>;
+template <class TT, detail::IfField<TT> = 0>
+ForceModel( PMB, ElasticPerfectlyPlastic, double, double, double, double, TT, double, double, double, double, bool )
+    -> ForceModel<PMB, ElasticPerfectlyPlastic, Fracture, DynamicTemperature, TT>;
 
 template <class MS, detail::IfField<MS> = 0>
 ForceModel( PMB, ElasticPerfectlyPlastic, MS, double, double, double, double )

@@ -273,7 +273,8 @@ struct ThermalOf
 template <class T, class... Rest>
 struct ThermalOf<T, Rest...>
 {
-    using type = std::conditional_t<std::is_same_v<T, TemperatureDependent>, T, typename ThermalOf<Rest...>::type>;
+    using type = std::conditional_t<std::is_same_v<T, TemperatureDependent> || std::is_same_v<T, DynamicTemperature>, T,
+                                    typename ThermalOf<Rest...>::type>;
 };
 } // namespace detail
 

@@ -69,7 +69,7 @@ struct DeviceSpec
     {
         lps = d.model == PDK_MODEL_LPS || d.model == PDK_MODEL_LINEAR_LPS;
         fracture = d.fracture != 0;
-        thermal = d.thermal == PDK_TEMP_DEPENDENT;
+        thermal = d.thermal != PDK_TEMP_INDEPENDENT;
         check( pdk_spec_create( context(), &d, static_cast<int64_t>( s.n ), &spec ) );
         check( pdk_spec_upload( spec, s.x.data(), s.u.data(), s.v.data(), s.vol.data(), s.rho.data(),
                                 s.nofail.data(), nullptr ) );
@@ -262,6 +262,12 @@ class Solver
         // Solver::setup (CabanaPD_Solver.hpp:153-227)
         if constexpr ( !is_contact<ForceModelType>::value )
             inputs.computeCriticalTimeStep( force_model );
+        if constexpr ( is_heat_transfer<typename ForceModelType::thermal_type>::value )
+        {
+            // heat transfer every thermal_subcycle_steps steps (CabanaPD_Solver.hpp:186-197, 340-347)
+            double sub = inputs["thermal_subcycle_steps"];
+            force_model.thermal_subcycle = static_cast<int>( sub );
+        }
         num_steps = inputs["num_steps"];
         output_frequency = inputs["output_frequency"];
         output_reference = inputs["output_reference"];

@@ -114,8 +114,17 @@ struct TemperatureDependent
 {
     using base_type = TemperatureDependent;
 };
+// DynamicTemperature: TemperatureDependent plus heat conduction (CabanaPD_Types.hpp:98-101)
+struct DynamicTemperature : public TemperatureDependent
+{
+    using base_type = TemperatureDependent;
+};
 template <class T>
 struct is_temperature_dependent : std::is_same<typename T::base_type, TemperatureDependent>
+{
+};
+template <class T>
+struct is_heat_transfer : std::is_same<T, DynamicTemperature>
 {
 };
 struct SingleMaterial

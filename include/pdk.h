@@ -76,9 +76,13 @@ enum pdk_mechanics {
 
 enum pdk_thermal {
     PDK_TEMP_INDEPENDENT = 0,  /* TemperatureIndependent */
-    PDK_TEMP_DEPENDENT = 1     /* TemperatureDependent: PMB / LinearPMB stretch s - alpha (T_avg - temp0)
+    PDK_TEMP_DEPENDENT = 1,    /* TemperatureDependent: PMB / LinearPMB stretch s - alpha (T_avg - temp0)
                                   (CabanaPD_ForceModels.hpp:165-211); with Fracture the critical stretch
                                   test uses (1 + s0 + alpha (T_avg - temp0))^2 (:213-256) */
+    PDK_TEMP_DYNAMIC = 2       /* DynamicTemperature: TemperatureDependent PMB plus peridynamic heat
+                                  conduction with forward Euler every thermal_subcycle steps
+                                  (CabanaPD_HeatTransfer.hpp:24-182, CabanaPD_ForceModels.hpp:262-310,
+                                  CabanaPD_Solver.hpp:340-347) */
 };
 
 enum pdk_bc_kind {
@@ -118,6 +122,13 @@ typedef struct pdk_model_desc {
     double sigma_y;         /* yield stress (EPP): s_Y = sigma_y / (3 K) */
     double alpha;           /* thermal expansion coefficient (TemperatureDependent) */
     double temp0;           /* reference temperature (TemperatureDependent) */
+    /* heat transfer (DynamicTemperature) */
+    double kappa;           /* thermal conductivity */
+    double cp;              /* specific heat capacity */
+    int32_t thermal_subcycle;  /* heat conduction at steps k with k % thermal_subcycle == 0, with dt_th =
+                                  thermal_subcycle * dt (inputs "thermal_subcycle_steps") */
+    int32_t linear_microconductivity; /* 0: constant microconductivity (the reference's default);
+                                         1: 4 thermal_coeff (1 - |xi| / horizon) */
 } pdk_model_desc;
 
 enum pdk_ordering {
@@ -136,6 +147,7 @@ typedef struct pdk_model_constants {
     double c, s0, bond_break_coeff, theta_coeff, s_coeff;
     double Rs, Es, Gs, beta, coeff_h_n, coeff_h_d, Gamma, a0, delta_tear, fc;
     double s_Y;  /* EPP yield stretch sigma_y / 3 / K (force_models/CabanaPD_PMB.hpp:90-96) */
+    double thermal_coeff;  /* 9/2 kappa / pi / horizon^3 (CabanaPD_ForceModels.hpp:277-288) */
 } pdk_model_constants;
 
 typedef struct pdk_ctx_s* pdk_ctx;

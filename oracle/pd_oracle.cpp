@@ -115,7 +115,11 @@ bool isEpp( const pdk_model_desc& d )
 }
 bool isThermal( const pdk_model_desc& d )
 {
-    return d.thermal == PDK_TEMP_DEPENDENT;
+    return d.thermal == PDK_TEMP_DEPENDENT || d.thermal == PDK_TEMP_DYNAMIC;
+}
+bool isDynamic( const pdk_model_desc& d )
+{
+    return d.thermal == PDK_TEMP_DYNAMIC;
 }
 
 // ---------------------------------------------------------------------------
@@ -155,8 +159,17 @@ int modelConstants( const pdk_model_desc& d, pdk_model_constants& k )
             return 1;
         if ( d.mechanics != PDK_MECH_ELASTIC && d.mechanics != PDK_MECH_EPP )
             return 1;
-        if ( d.thermal != PDK_TEMP_INDEPENDENT && d.thermal != PDK_TEMP_DEPENDENT )
+        if ( d.thermal != PDK_TEMP_INDEPENDENT && d.thermal != PDK_TEMP_DEPENDENT && d.thermal != PDK_TEMP_DYNAMIC )
             return 1;
+        if ( isDynamic( d ) )
+        {
+            if ( !( d.kappa > 0 ) || !( d.cp > 0 ) || d.thermal_subcycle < 1 )
+                return 1;
+            // BaseDynamicTemperatureModel (CabanaPD_ForceModels.hpp:277-288):
+            // thermal horizon = force horizon
+            const double d3 = delta * delta * delta;
+            k.thermal_coeff = 9.0 / 2.0 * d.kappa / kPi / d3;
+        }
         if ( isEpp( d ) )
         {
             // ForceModel<PMB, ElasticPerfectlyPlastic, Fracture> only
@@ -1074,6 +1087,38 @@ void computeForce( Sys& s, bool reset )
         computeForceFull( s );
 }
 
+// computeHeatTransfer (CabanaPD_HeatTransfer.hpp:155-182): conduction per
+// particle over its (intact, with fracture :126) bonds, then forward Euler
+// with dt_th (HeatTransfer::forwardEuler, :66-83).
+void computeHeatTransfer( Sys& s, double dt )
+{
+    const bool frac = s.d.fracture != 0;
+    const double thermal_coeff = s.k.thermal_coeff;
+    const double thermal_horizon = s.d.horizon;
+    std::vector<double> conduction( s.n, 0.0 );
+#pragma omp parallel for schedule( static )
+    for ( int64_t i = 0; i < s.n; i++ )
+    {
+        const int32_t* row = s.nbr.data() + i * s.maxn;
+        const uint8_t* mu = frac ? s.mu.data() + i * s.maxn : nullptr;
+        for ( int32_t q = 0; q < s.cnt[i]; q++ )
+        {
+            const int64_t j = row[q];
+            double xi, r, st, rx, ry, rz;
+            getDistance( s, i, j, xi, r, st, rx, ry, rz );
+            if ( frac && !( mu[q] > 0 ) )
+                continue;
+            // microconductivity_function (CabanaPD_ForceModels.hpp:302-308)
+            const double coeff =
+                s.d.linear_microconductivity ? 4.0 * thermal_coeff * ( 1.0 - xi / thermal_horizon ) : thermal_coeff;
+            conduction[i] += coeff * ( s.temp[j] - s.temp[i] ) / xi / xi * s.vol[j];
+        }
+    }
+#pragma omp parallel for schedule( static )
+    for ( int64_t i = 0; i < s.n; i++ )
+        s.temp[i] += dt / s.rho[i] / s.d.cp * conduction[i];
+}
+
 // A boundary condition or body term with force_update == false applied at
 // time t (CabanaPD_Solver.hpp:255-257, 350-352): member m gets a + b t.
 void applyPrescribed( Sys::Prescribed& pr, std::vector<double>& dst, double t )
@@ -1381,6 +1426,9 @@ int orc_step( orc_sys s, int32_t nsteps )
         initialHalfStep( *s );
         if ( isContact( s->d ) )
             s->steps_since_build++;
+        // heat transfer every thermal_subcycle steps (CabanaPD_Solver.hpp:340-347)
+        if ( isDynamic( s->d ) && s->steps_done % s->d.thermal_subcycle == 0 )
+            computeHeatTransfer( *s, s->d.thermal_subcycle * s->d.dt );
         applyPrescribed( *s, s->steps_done * s->d.dt );
         updateForce( *s );
         applyForceTerms( *s );

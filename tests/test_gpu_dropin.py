@@ -12,8 +12,9 @@ the tstForce flow (Force, Neighbor, computeForce, computeEnergy) against
 lattice sums.  dropin_crack_inclusion is examples/mechanics/crack_inclusion.cpp
 (two materials through createMultiForceModel, a functor force boundary
 condition) checking regression_test/tstCrackInclusion.hpp:55.  dropin_dogbone
-(ElasticPerfectlyPlastic, displacement grips) and dropin_thermal_deformation
-(TemperatureDependent, imposed temperature) are the reference's examples on
+(ElasticPerfectlyPlastic, displacement grips), dropin_thermal_deformation
+(TemperatureDependent, imposed temperature) and dropin_thermal_heat_transfer
+(DynamicTemperature, cold planes) are the reference's examples on
 their decks; their force_update = false functors run on the host each step,
 and the results equal the engine's prescribed-value runs bit for bit.
 
@@ -156,5 +157,33 @@ def test_dropin_thermal_deformation_example(tmp_path):
     u, T = g.get("u"), g.get("temperature")
     nd = int(np.sum(u_drop != u)) + int(np.sum(t_drop != T))
     say(f"  drop-in vs engine: differing={nd} max|u|={np.max(np.abs(u)):.3e} T max {T.max():.4f}")
+    assert nd == 0
+    g.close()
+
+
+def test_dropin_thermal_heat_transfer_example(tmp_path):
+    """examples/thermomechanics/thermal_deformation_heat_transfer.cpp on its
+    deck with heat conduction every 20 steps for 100 steps (the deck's own
+    cadence, 45000, needs 19M steps): the final displacement and temperature
+    equal the engine run, bit for bit."""
+    import numpy as np
+
+    from cabanapd_amd import configs
+    from tests.parity import gpu_spec
+    steps, sub = 100, 20
+    deck = _deck_variant(tmp_path, "thermal_deformation_heat_transfer.json", final_time=(steps + 0.5) * 4.2e-7,
+                         thermal_subcycle_steps=sub, output_frequency=50)
+    out = tmp_path / "heat_u.bin"
+    rc, log = run([driver("dropin_thermal_heat_transfer"), deck, str(out)], str(tmp_path), 110)
+    assert rc == 0 and "HT_DONE" in log
+    w = configs.thermal_deformation_heat_transfer(steps=steps, thermal_subcycle=sub)
+    u_drop = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
+    t_drop = np.fromfile(str(out) + ".temp", dtype=np.float64)
+    g = gpu_spec(w)
+    configs.setup(g, w)
+    g.step(steps)
+    u, T = g.get("u"), g.get("temperature")
+    nd = int(np.sum(u_drop != u)) + int(np.sum(t_drop != T))
+    say(f"  drop-in vs engine: differing={nd} max|u|={np.max(np.abs(u)):.3e} T in [{T.min():.4f}, {T.max():.4f}]")
     assert nd == 0
     g.close()

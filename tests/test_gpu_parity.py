@@ -449,14 +449,18 @@ def test_gpu_thermal_deformation_bitwise():
     assert np.max(np.abs(g.get("u"))) > 0.0
 
 
-@pytest.mark.parametrize("model,mech,frac", [("PMB", "elastic", True), ("PMB", "epp", True),
-                                             ("LINEAR_PMB", "elastic", False)])
-def test_gpu_thermal_cube_bitwise(model, mech, frac):
+@pytest.mark.parametrize("model,mech,frac,thermal", [("PMB", "elastic", True, True), ("PMB", "epp", True, True),
+                                                     ("LINEAR_PMB", "elastic", False, True),
+                                                     ("PMB", "elastic", True, "dynamic"),
+                                                     ("PMB", "epp", True, "dynamic")])
+def test_gpu_thermal_cube_bitwise(model, mech, frac, thermal):
     """TemperatureDependent PMB / LinearPMB, with the thermal critical
-    stretch and EPP, on a cube with a random initial temperature field and a
-    temperature ramp imposed on half the particles: 25 steps bit for bit."""
+    stretch and EPP, and DynamicTemperature (heat conduction over the intact
+    bonds every 4 steps), on a cube with a random initial temperature field
+    and a temperature ramp imposed on half the particles: 25 steps bit for
+    bit."""
     from cabanapd_amd._abi import make_desc
-    say(f"case: thermal {model} {mech} fracture={frac}, cube 11^3, 25 steps")
+    say(f"case: thermal={thermal} {model} {mech} fracture={frac}, cube 11^3, 25 steps")
     w = configs.lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.0, K=1e9)
     x = w["x"]
     n = x.shape[0]
@@ -465,7 +469,9 @@ def test_gpu_thermal_cube_bitwise(model, mech, frac):
     if mech == "epp":
         kw["mechanics"] = "epp"
         kw["sigma_y"] = 3e6
-    w["desc"] = make_desc(model, horizon=w["desc"].horizon, K=1e9, dt=1e-6, thermal=True, alpha=2e-4, temp0=10.0,
+    if thermal == "dynamic":
+        kw.update(kappa=5e4, cp=10.0, thermal_subcycle=4)
+    w["desc"] = make_desc(model, horizon=w["desc"].horizon, K=1e9, dt=1e-6, thermal=thermal, alpha=2e-4, temp0=10.0,
                           **kw)
     w["u"] = rng.uniform(-2e-3, 2e-3, size=x.shape) * (2.0 / 11)
     w["rho"] = np.full(n, 1000.0)
@@ -479,6 +485,22 @@ def test_gpu_thermal_cube_bitwise(model, mech, frac):
     _full_compare(g, o)
     if frac:
         say(f"  broken bonds {len(broken_set(g))}")
+
+
+def test_gpu_heat_transfer_bitwise():
+    """examples/thermomechanics/thermal_deformation_heat_transfer.cpp's
+    specimen (49^3, DynamicTemperature, the y faces held at 0) with heat
+    conduction every 20 steps for 100 steps: temperature, u, v, f, W and
+    stress equal the oracle's bit for bit."""
+    say("case: thermal deformation with heat transfer 49^3, subcycle 20, 100 steps FP64")
+    w = configs.thermal_deformation_heat_transfer(steps=100, thermal_subcycle=20)
+    g, o = pair(w)
+    g.step(w["steps"])
+    o.step(w["steps"])
+    T = g.get("temperature")
+    assert_same(T, o.get("temperature"), "temperature")
+    assert np.sum((T != 100.0) & (T != 0.0)) > 0
+    _full_compare(g, o)
 
 
 def test_gpu_plastic_thermal_rejections():

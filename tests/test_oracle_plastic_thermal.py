@@ -19,6 +19,10 @@ fixes:
   moves the break threshold to (1 + s0 + alpha dT)^2.
 - Prescribed values: a + b t at t = k dt exactly, set before the force pass
   (the stored force equals a fresh force pass at the final state).
+- Heat transfer (CabanaPD_HeatTransfer.hpp): closed-form lattice sums for a
+  quadratic temperature field with either microconductivity (within 10 % of
+  the continuum kappa T''), the subcycle cadence, and conservation of
+  sum V rho cp T with and without broken bonds.
 """
 import numpy as np
 import pytest
@@ -246,3 +250,77 @@ def test_prescribed_values_affine_in_time(thermal):
     if not thermal:
         with pytest.raises(RuntimeError):
             s.set_prescribed("temperature", [0], np.zeros(1))
+
+
+# ---- heat transfer (DynamicTemperature) -------------------------------------
+# CabanaPD_HeatTransfer.hpp: conduction_i = sum_j mc(xi) (T_j - T_i) / xi^2 V_j
+# and T_i += dt_th / rho_i / cp conduction_i every thermal_subcycle steps.
+# For T = x^2 the linear part cancels over a symmetric neighbourhood and
+# sum_j xi_x^2 / |xi|^2 = N / 3 by cubic symmetry, so an interior particle's
+# conduction is mc V N / 3 (constant microconductivity) or
+# (4 mc V / 3) sum_j (1 - |xi| / delta) (linear); both approach kappa T'' =
+# 2 kappa, the continuum limit of the 9 kappa / (2 pi delta^3) constant.
+# With equal volumes every bond term is antisymmetric, so sum_i V_i rho_i cp
+# dT_i = 0 for any field.
+
+from tests import lattice_reference as lref
+
+
+def _heat_lattice(kappa=2.0, cp=3.0, rho=5.0, sub=1, linear=False, fracture=False, nc=13, m=3):
+    w = configs.lattice_sum_cube(model="PMB", m=m, nc=nc, field="linear", coeff=0.0, K=1.0)
+    horizon = w["desc"].horizon
+    kw = dict(fracture=True, G0=1e-6) if fracture else {}
+    d = make_desc("PMB", horizon=horizon, K=1.0, dt=1e-3, thermal="dynamic", alpha=0.0, kappa=kappa, cp=cp,
+                  thermal_subcycle=sub, linear_microconductivity=linear, **kw)
+    x = w["x"]
+    s = orc.OracleSpec(d, x.shape[0], x, w["vol"], rho=np.full(x.shape[0], rho))
+    s.build_bonds()
+    return s, x, w["vol"][0], horizon, w["dx"]
+
+
+@pytest.mark.parametrize("linear", [False, True])
+def test_heat_quadratic_field_lattice_sum(linear):
+    kappa, cp, rho, sub, m = 2.0, 3.0, 5.0, 3, 3
+    s, x, V, horizon, dx = _heat_lattice(kappa, cp, rho, sub=sub, linear=linear, m=m)
+    T0 = x[:, 0] ** 2
+    s.set_field("temperature", T0)
+    k = orc.model_constants(s.desc)
+    mc = k["thermal_coeff"]
+    assert mc == pytest.approx(9.0 / 2.0 * kappa / np.pi / horizon ** 3, rel=1e-14)
+    s.init()
+    s.step(sub - 1)
+    assert np.array_equal(s.get("temperature"), T0)  # no heat pass before step `sub`
+    s.step(1)
+    dT = s.get("temperature") - T0
+    xi, mag = lref.offsets(m, dx, horizon)
+    N = xi.shape[0]
+    if linear:
+        cond = 4.0 * mc * V / 3.0 * np.sum(1.0 - mag / horizon)
+    else:
+        cond = mc * V * N / 3.0
+    expect = sub * 1e-3 / rho / cp * cond
+    lo, hi = -1.0 + horizon * 1.01, 1.0 - horizon * 1.01
+    inner = np.all((x > lo) & (x < hi), axis=1)
+    assert inner.sum() > 0
+    assert np.allclose(dT[inner], expect, rtol=1e-9, atol=0)
+    # the continuum limit kappa T'' = 2 kappa, within the lattice's discretisation
+    assert abs(cond - 2.0 * kappa) <= 0.1 * 2.0 * kappa
+
+
+@pytest.mark.parametrize("fracture", [False, True])
+def test_heat_conserves_energy(fracture):
+    rho, cp = 5.0, 3.0
+    s, x, V, horizon, dx = _heat_lattice(rho=rho, cp=cp, fracture=fracture, nc=11)
+    rng = np.random.default_rng(3)
+    T0 = rng.uniform(0.0, 50.0, size=x.shape[0])
+    s.set_field("temperature", T0)
+    if fracture:
+        # break some bonds first: random displacements beyond the tiny s0
+        s.set_state(u=rng.uniform(-1e-3, 1e-3, size=x.shape))
+        s.compute_force(True)
+        assert np.any(s.bonds()[2] == 0)
+    s.step(1)
+    dT = s.get("temperature") - T0
+    terms = V * rho * cp * np.abs(dT)
+    assert terms.sum() > 0
+    assert abs(np.sum(V * rho * cp * dT)) <= 1e-12 * terms.sum()
