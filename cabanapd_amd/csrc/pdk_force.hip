@@ -978,6 +978,14 @@ kForceRows( const PDArgs* __restrict__ specs,
             const int32_t q = R.x + t;
             // intact word (fracture) or existence word of the slot
             const uint64_t ww = scalarLoad( mwords + q );
+            // A slot with no live bond in the wave (the slots past a face of
+            // the specimen, fully broken slots) adds nothing: the LPS passes
+            // skip it (measured 8 % faster on the 128^3 LPS fracture cube; the
+            // branch made the PMB plate pass 3 % slower, so PMB evaluates
+            // and masks).  The FP64 form skips through its exec mask.
+            if constexpr ( PREC == kMixed && kIsLps<MODEL> )
+                if ( ww == 0 )
+                    continue;
             // The window always holds readable values; bonds that do not
             // exist are masked out below.
             const double* __restrict__ wr = wb + lane + t;
