@@ -11,7 +11,11 @@
 #include <array>
 #include <cassert>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <fstream>
 #include <memory>
+#include <string>
 #include <stdexcept>
 #include <type_traits>
 #include <vector>
@@ -211,9 +215,100 @@ class Particles
     double timeInit() { return 0.0; }
     double time() { return 0.0; }
     double timeOutput() { return 0.0; }
-    template <class... Args>
-    void output( Args&&... )
+    // Particles::output (CabanaPD_Particles.hpp:622-652 and the overrides at
+    // :787-792, :926-931, :1194-1199, :1325-1330): position (reference or
+    // current), force, displacement, velocity, type, plus the LPS weighted
+    // volume and dilatation, the temperature, strain energy and damage, and
+    // stress, as the particle type carries them.  The reference writes them
+    // with Cabana's HDF5 or SILO particle output when Cabana is built with
+    // one of those; HDF5 and SILO are not part of this build, which writes the
+    // same fields to particles_<step>.vtk (legacy VTK, binary PolyData) when
+    // CABANAPD_ENABLE_VTK_OUTPUT is defined and otherwise prints the
+    // reference's "No particle output enabled."  The host copy is refreshed by
+    // the Solver before each output.
+    template <class... OtherFields>
+    void output( const int output_step, const double output_time, const bool use_reference, OtherFields&&... )
     {
+#ifdef CABANAPD_ENABLE_VTK_OUTPUT
+        writeVtk( "particles_" + std::to_string( output_step ) + ".vtk", output_time, use_reference );
+#else
+        (void)output_step;
+        (void)output_time;
+        (void)use_reference;
+        log( std::cout, "No particle output enabled." );
+#endif
+    }
+
+    // Legacy VTK (big-endian binary) PolyData of the output fields.
+    void writeVtk( const std::string& name, const double time, const bool use_reference ) const
+    {
+        const ParticleStorage& s = *_s;
+        const std::size_t n = s.n;
+        std::ofstream out( name, std::ios::binary );
+        if ( !out )
+            throw std::runtime_error( "cannot write " + name );
+        auto be64 = [&]( double v ) {
+            uint64_t b;
+            std::memcpy( &b, &v, 8 );
+            for ( int k = 7; k >= 0; k-- )
+                out.put( static_cast<char>( ( b >> ( 8 * k ) ) & 0xff ) );
+        };
+        auto be32 = [&]( int32_t v ) {
+            const uint32_t b = static_cast<uint32_t>( v );
+            for ( int k = 3; k >= 0; k-- )
+                out.put( static_cast<char>( ( b >> ( 8 * k ) ) & 0xff ) );
+        };
+        auto vec3 = [&]( const char* nm, const std::vector<double>& a ) {
+            out << "VECTORS " << nm << " double\n";
+            for ( std::size_t i = 0; i < 3 * n; i++ )
+                be64( a[i] );
+            out << "\n";
+        };
+        auto scalar = [&]( const char* nm, const std::vector<double>& a ) {
+            out << "SCALARS " << nm << " double 1\nLOOKUP_TABLE default\n";
+            for ( std::size_t i = 0; i < n; i++ )
+                be64( a[i] );
+            out << "\n";
+        };
+        out << "# vtk DataFile Version 3.0\nCabanaPD particles\nBINARY\nDATASET POLYDATA\n";
+        out << "FIELD FieldData 1\nTIME 1 1 double\n";
+        be64( time );
+        out << "\nPOINTS " << n << " double\n";
+        for ( std::size_t i = 0; i < 3 * n; i++ )
+            be64( use_reference ? s.x[i] : s.x[i] + s.u[i] );
+        out << "\nVERTICES " << n << " " << 2 * n << "\n";
+        for ( std::size_t i = 0; i < n; i++ )
+        {
+            be32( 1 );
+            be32( static_cast<int32_t>( i ) );
+        }
+        out << "\nPOINT_DATA " << n << "\n";
+        vec3( "force", s.f );
+        vec3( "displacement", s.u );
+        vec3( "velocity", s.v );
+        out << "SCALARS type int 1\nLOOKUP_TABLE default\n";
+        for ( std::size_t i = 0; i < n; i++ )
+            be32( s.type[i] );
+        out << "\n";
+        if constexpr ( std::is_same_v<typename ModelType::base_type, LPS> )
+        {
+            scalar( "weighted_volume", s.m );
+            scalar( "dilatation", s.theta );
+        }
+        if constexpr ( is_temperature_dependent<ThermalType>::value )
+            scalar( "temperature", s.temp );
+        if constexpr ( is_energy_output<OutputType>::value )
+        {
+            scalar( "strain_energy", s.W );
+            scalar( "damage", s.phi );
+        }
+        if constexpr ( is_stress_output<OutputType>::value )
+        {
+            out << "TENSORS stress double\n";
+            for ( std::size_t i = 0; i < 9 * n; i++ )
+                be64( s.stress[i] );
+            out << "\n";
+        }
     }
 
   private:

@@ -290,7 +290,8 @@ class Solver
         if constexpr ( is_stress_output<typename ParticleType::output_type>::value )
             neighbor->device().computeStress();
         pullAll();
-        (void)initial_output;
+        if ( initial_output )
+            particles.output( 0, 0.0, output_reference ); // CabanaPD_Solver.hpp:245-246, 276-277
     }
     template <class BoundaryType, std::enable_if_t<is_boundary_condition<BoundaryType>::value, int> = 0>
     void init( BoundaryType boundary_condition, const bool initial_output = true )
@@ -448,6 +449,7 @@ class Solver
             if constexpr ( is_stress_output<typename ParticleType::output_type>::value )
                 neighbor->device().computeStress();
             pullAll();
+            particles.output( step / output_frequency, step * dt, output_reference ); // :488-489
             _last_energy = W;
             _last_damage = D;
             auto now = std::chrono::steady_clock::now();

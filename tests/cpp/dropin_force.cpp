@@ -4,6 +4,10 @@
 // checks against the single-particle lattice sums (tstForce.hpp:91-155,
 // 238-287, 345-422) on interior particles.
 #include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iterator>
+#include <string>
 #include <cstdio>
 
 #include <CabanaPD.hpp>
@@ -145,6 +149,42 @@ void run( const Model& model, Tag, int nc, double bw, bool lps, double K, double
             force.totalStrainEnergy() );
     expect( checked > 0, "particles checked", checked, 0 );
     std::printf( "%s: %d interior particles checked\n", lps ? "LPS" : "PMB", checked );
+
+    // Particle field output (built with CABANAPD_ENABLE_VTK_OUTPUT): read the
+    // file back and compare the current position, force and stress of the
+    // last particle with the slices (legacy VTK stores big-endian doubles).
+    particles.output( 7, 0.25, false );
+    std::ifstream in( "particles_7.vtk", std::ios::binary );
+    std::string text( ( std::istreambuf_iterator<char>( in ) ), std::istreambuf_iterator<char>() );
+    auto readBe = [&]( std::size_t at ) {
+        uint64_t b = 0;
+        for ( int k = 0; k < 8; k++ )
+            b = ( b << 8 ) | static_cast<unsigned char>( text[at + k] );
+        double v;
+        std::memcpy( &v, &b, 8 );
+        return v;
+    };
+    const std::size_t n = particles.numLocal(), last = n - 1;
+    auto fieldAt = [&]( const std::string& header ) {
+        const std::size_t pos = text.find( header );
+        return pos == std::string::npos ? std::string::npos : pos + header.size();
+    };
+    const std::size_t pp = fieldAt( "POINTS " + std::to_string( n ) + " double\n" );
+    const std::size_t pf = fieldAt( "VECTORS force double\n" );
+    const std::size_t ps = fieldAt( "TENSORS stress double\n" );
+    const std::size_t pt = fieldAt( "TIME 1 1 double\n" );
+    const bool found = pp != std::string::npos && pf != std::string::npos && ps != std::string::npos &&
+                       pt != std::string::npos && text.find( lps ? "weighted_volume" : "strain_energy" ) != std::string::npos;
+    expect( found, "vtk fields present", found ? 1.0 : 0.0, 1.0 );
+    if ( found )
+    {
+        expect( readBe( pt ) == 0.25, "vtk time", readBe( pt ), 0.25 );
+        expect( readBe( pp + 24 * last ) == x( last, 0 ) + u( last, 0 ), "vtk current position", readBe( pp + 24 * last ),
+                x( last, 0 ) + u( last, 0 ) );
+        expect( readBe( pf + 24 * last + 16 ) == f( last, 2 ), "vtk force", readBe( pf + 24 * last + 16 ), f( last, 2 ) );
+        expect( readBe( ps + 72 * last + 8 ) == sigma( last, 0, 1 ), "vtk stress", readBe( ps + 72 * last + 8 ),
+                sigma( last, 0, 1 ) );
+    }
 }
 } // namespace
 
