@@ -196,6 +196,161 @@ def copy_kernel(mib, repeats=5):
     print(json.dumps({"copy_kernel_bytes_read": n * 4, "copy_kernel_bytes_written": n * 4, "repeats": repeats}))
 
 
+def rows_leg(args):
+    """The rows beyond the benchmark path, each on the benchmark plate's
+    geometry (200 x 400 x 18 by default): linearized PMB / LPS (no fracture),
+    the stress and energy output passes, a device-resident region series (the
+    crack-extent max over every particle), and the FP64 gather forms of two
+    materials, ElasticPerfectlyPlastic, TemperatureDependent and heat
+    transfer.  Per row: kernel time from HIP events on the specimen's stream,
+    rates, the kernel's algorithmic HBM bytes against 8 TB/s (the same
+    yardstick as the main line), and the oracle on the host cores over two
+    steps (or calls) of the same workload."""
+    from cabanapd_amd import _abi
+    from cabanapd_amd._abi import KCLS_ENERGY, KCLS_LPS, KCLS_OTHER, make_desc
+    from oracle import binding as orc
+    from oracle import regions as oregions
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16)
+    orc.set_threads(threads)
+    cells = tuple(int(c) for c in args.cells.split(","))
+    steps = min(args.steps, 200)
+    base = configs.kw_plate(num_cells=cells, steps=steps)
+    x = base["x"]
+    n = x.shape[0]
+    bd = base["desc"]
+    ctx = engine.Context(0)
+    out = {"workload": "rows on the plate %dx%dx%d (%d particles), %d steps each" % (cells + (n, steps)),
+           "cpu_threads": threads, "rows": {}}
+
+    def workload(model="PMB", fracture=True, **kw):
+        w = dict(base)
+        w["desc"] = make_desc(model, fracture=fracture, horizon=bd.horizon, K=bd.K, G=kw.pop("G", 0.0),
+                              G0=bd.G0, dt=bd.dt, **kw)
+        if not fracture:
+            w["notches"] = None
+        w["prescribed"] = []
+        return w
+
+    def frac_of(b_alg, ms):
+        gbs = b_alg / (ms * 1e-3) / 1e9
+        return {"achieved_GBs": gbs, "frac_of_8TBs": gbs / HBM_PEAK_GBS, "alg_bytes": b_alg}
+
+    def oracle_rate(w, bonds, nsteps=2):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.precision = 0
+        o = orc.OracleSpec(d, n, w["x"], w["vol"], u=w["u"], v=w["v"], rho=w["rho"])
+        configs.setup(o, w)
+        t0 = time.perf_counter()
+        o.step(nsteps)
+        dt_ = (time.perf_counter() - t0) / nsteps
+        return o, {"ms_per_step": dt_ * 1e3, "bond_evals_per_s": bonds / dt_}
+
+    def force_row(name, w, precisions=("fp64", "mixed"), lps=False, extra_cls=None):
+        row = {}
+        bonds = 0
+        for prec in precisions:
+            s = make_spec(ctx, w, prec)
+            bonds, _ = s.bond_stats()
+            s.step(5)
+            s.sync()
+            s.timing(True)
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            wall = time.perf_counter() - t0
+            fms, fl = s.kernel_time(KCLS_FORCE)
+            r = {"ms_per_step": wall / steps * 1e3, "bond_evals_per_s": bonds * steps / wall,
+                 "force_kernel": s.kernel_name(KCLS_FORCE), "force_kernel_ms": fms / max(fl, 1)}
+            r.update(frac_of(algorithmic_bytes_per_step(n, bonds, fracture=w["desc"].fracture != 0, lps=lps),
+                             r["force_kernel_ms"]))
+            if lps:
+                lms, ll = s.kernel_time(KCLS_LPS)
+                r["state_kernel_ms"] = lms / max(ll, 1)
+            if extra_cls is not None:
+                oms, ol = s.kernel_time(extra_cls[0])
+                r[extra_cls[1] + "_ms_total"] = oms
+                r[extra_cls[1] + "_launches"] = ol
+            s.timing(False)
+            s.close()
+            row[prec] = r
+        _, row["cpu_oracle"] = oracle_rate(w, bonds)
+        row["bonds"] = bonds
+        out["rows"][name] = row
+        return bonds
+
+    # rank 1: linearized models (NoFracture only)
+    force_row("linear_pmb", workload("LINEAR_PMB", fracture=False))
+    force_row("linear_lps", workload("LINEAR_LPS", fracture=False, G=bd.K * 0.6), lps=True)
+
+    # ranks 2-3: output passes and a region series on the fracturing plate (mixed spec)
+    w = workload()
+    s = make_spec(ctx, w, "mixed")
+    bonds, _ = s.bond_stats()
+    s.step(50)
+    s.sync()
+    s.timing(True)
+    for _ in range(20):
+        s.compute_stress()
+    sms, sl = s.kernel_time(KCLS_OTHER)
+    for _ in range(20):
+        s.compute_energy_async()
+    s.sync()
+    ems, el = s.kernel_time(KCLS_ENERGY)
+    s.timing(False)
+    box = _abi.box_region(x.min(axis=0), x.max(axis=0))
+    ser = s.series(box, "x", component=1, reducer="max", capacity=256, gate="phi", threshold=0.3)
+    ser.update(0.0)
+    s.sync()
+    t0 = time.perf_counter()
+    for k in range(200):
+        ser.update(float(k))
+    s.sync()
+    upd = (time.perf_counter() - t0) / 200
+    ser.close()
+    s.close()
+    o, _ = oracle_rate(w, bonds, nsteps=1)
+    t0 = time.perf_counter()
+    o.compute_stress()
+    cpu_stress = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.compute_energy()
+    cpu_energy = time.perf_counter() - t0
+    ox, ophi = o.get("x"), o.get("phi")
+    t0 = time.perf_counter()
+    mem = oregions.members(ox, box)
+    oregions.reduce(np.where(ophi[mem] > 0.3, ox[mem, 1], 0.0), "max")
+    cpu_series = time.perf_counter() - t0
+    stress_ms = sms / max(sl, 1)
+    energy_ms = ems / max(el, 1)
+    out["rows"]["stress"] = dict(kernel_ms=stress_ms, bond_evals_per_s=bonds / (stress_ms * 1e-3),
+                                 cpu_oracle_ms=cpu_stress * 1e3,
+                                 **frac_of(bonds * (4 + 1 / 8) + n * (24 + 24 + 72), stress_ms))
+    out["rows"]["energy"] = dict(kernel_ms=energy_ms, bond_evals_per_s=bonds / (energy_ms * 1e-3),
+                                 cpu_oracle_ms=cpu_energy * 1e3,
+                                 **frac_of(bonds * (4 + 1 / 8) + n * (24 + 24 + 16), energy_ms))
+    out["rows"]["series_update"] = dict(members=n, us_per_update=upd * 1e6, cpu_host_ms=cpu_series * 1e3,
+                                        **frac_of(n * (8 + 8 + 8 + 8), upd * 1e3))
+
+    # rank 4: FP64 gather forms
+    w = workload()
+    mtype = (x[:, 0] >= 0.0).astype(np.int32)
+    w["materials"] = ([(bd.K, 0.0, bd.G0), (2.0 * bd.K, 0.0, 2.0 * bd.G0)], mtype)
+    force_row("multi_material", w, precisions=("fp64",))
+    force_row("epp", workload(mechanics="epp", sigma_y=3.0 * bd.K * 1e-3), precisions=("fp64",))
+    w = workload(thermal=True, alpha=1e-5, temp0=0.0)
+    w["prescribed"] = [("temperature", np.arange(n), np.zeros(n), 1e7 * (x[:, 0] - x[:, 0].min()) / 0.1)]
+    force_row("thermal", w, precisions=("fp64",))
+    w = workload(thermal="dynamic", alpha=1e-5, temp0=0.0, kappa=50.0, cp=500.0, thermal_subcycle=10)
+    w["temperature"] = 100.0 * (x[:, 0] - x[:, 0].min()) / 0.1
+    hb = force_row("heat_transfer", w, precisions=("fp64",), extra_cls=(KCLS_OTHER, "heat_and_half_step"))
+    r = out["rows"]["heat_transfer"]["fp64"]
+    passes = steps // 10
+    heat_ms = r["heat_and_half_step_ms_total"] / max(passes, 1)
+    r["heat_pass_ms"] = heat_ms
+    r["heat_pass"] = frac_of(hb * (4 + 1 / 8) + n * (24 + 8 + 8 + 8 + 8), heat_ms)
+    return out
+
+
 def lps_leg(args, n_side=128):
     """LPS elastic-brittle cube 128^3 (2.1M particles): weighted volume,
     dilatation and force passes every step, both precision forms."""
@@ -318,9 +473,11 @@ def main():
     ap.add_argument("--max-particles", type=float, default=2e6)
     ap.add_argument("--bond-layout", choices=("auto", "compact"), default="auto",
                     help="auto: stencil slots on full lattices; compact: stored neighbor ids")
-    ap.add_argument("--workload", choices=("plate", "cube", "lps", "dem"), default="plate",
+    ap.add_argument("--workload", choices=("plate", "cube", "lps", "dem", "rows"), default="plate",
                     help="plate: the PMB fracture plate (the benchmark line); cube: PMB elastic cube 41^3; "
-                         "lps: LPS fracture cube 128^3; dem: Hertz-JKR contact box of 1e6 spheres")
+                         "lps: LPS fracture cube 128^3; dem: Hertz-JKR contact box of 1e6 spheres; "
+                         "rows: linearized models, output passes, region series, multi-material, EPP, "
+                         "thermal and heat transfer on the plate geometry")
     ap.add_argument("--copy-kernel", type=float, default=0.0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -339,6 +496,9 @@ def main():
         return
     if args.workload == "dem":
         print(json.dumps(dem_leg(args)))
+        return
+    if args.workload == "rows":
+        print(json.dumps(rows_leg(args)))
         return
     cells = tuple(int(c) for c in args.cells.split(","))
     w = configs.kw_plate(num_cells=cells, steps=args.steps)
