@@ -26,9 +26,10 @@
 // load-cycle closed forms, uniform-stretch scalings and the elastic limit;
 // thermal free expansion, held-body equivalence and break threshold;
 // prescribed values (tests/test_oracle_plastic_thermal.py).
-// Parity unpinned (no reference-held number exists): LPS fracture energy when
-// all bonds of a particle are broken, the LPS no-fail asymmetry, and the
-// polydisperse contact extension away from equal radii.
+// Behaviour no reference-held number covers is pinned by what it must be
+// (tests/test_oracle_quirks.py): the LPS no-fail asymmetry, the NaN LPS energy
+// of a particle with all bonds broken, and the polydisperse contact extension
+// against the dissimilar-sphere Hertz and JKR closed forms.
 
 #include "pd_oracle.h"
 
