@@ -973,8 +973,7 @@ kForceRows( const PDArgs* __restrict__ specs,
         if ( r + 1 < nrows )
             loadRow( r + 1 );
         float px = 0.0f, py = 0.0f, pz = 0.0f; // FP32 sum of this row's bond forces (mixed form)
-        for ( int t = 0; t < R.y; t++ )
-        {
+        auto slot = [&]( const int t ) __attribute__( ( always_inline ) ) {
             const int32_t q = R.x + t;
             // intact word (fracture) or existence word of the slot
             const uint64_t ww = scalarLoad( mwords + q );
@@ -985,7 +984,7 @@ kForceRows( const PDArgs* __restrict__ specs,
             // and masks).  The FP64 form skips through its exec mask.
             if constexpr ( PREC == kMixed && kIsLps<MODEL> )
                 if ( ww == 0 )
-                    continue;
+                    return;
             // The window always holds readable values; bonds that do not
             // exist are masked out below.
             const double* __restrict__ wr = wb + lane + t;
@@ -1084,6 +1083,22 @@ kForceRows( const PDArgs* __restrict__ specs,
                 if ( brkm && lane == 0 )
                     bits[q] = ww & ~brkm;
             }
+        };
+        // PMB mixed form: the slots of a row (at most kRowMaxSlots) unrolled,
+        // so the window reads take immediate offsets and the pass fits 70
+        // VGPRs without spills (7 % faster on the fracture plate; unrolling
+        // the FP64 form measured slower)
+        if constexpr ( PREC == kMixed && !kIsLps<MODEL> )
+        {
+#pragma unroll
+            for ( int t = 0; t < kRowMaxSlots; t++ )
+                if ( t < R.y )
+                    slot( t );
+        }
+        else
+        {
+            for ( int t = 0; t < R.y; t++ )
+                slot( t );
         }
         if constexpr ( PREC == kMixed )
         {
