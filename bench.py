@@ -35,8 +35,8 @@ DEFAULT_CELLS = (200, 400, 18)
 # plain 1 GiB device copy read through the same counters).  Summaries with
 # the commands are kept under profiles/.
 MEASURED_TRAFFIC = {
-    "kForceRows<0, true, 1, false>": 451.0e6,
-    "kForceRows<0, true, 0, false>": 712.4e6,
+    "kForceRows<0, true, 1, false>": 430.3e6,
+    "kForceRows<0, true, 0, false>": 700.7e6,
     "kForce<0, true, 1, 2, true>": 503.2e6,
     "kForce<0, true, 0, 2, true>": 744.5e6,
 }
