@@ -71,7 +71,7 @@ struct PDArgs
     // Null in the compact form, where slot q holds a bond iff q < cnt[p].
     const uint64_t* exist;
     const int32_t* soff;
-    const float4* stencil_aux; // per stencil entry (1/|xi|, bandf/|xi|, |xi|^2, 0) in FP32, mixed form
+    const float4* stencil_aux; // per stencil entry (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0) in FP32, mixed form
     double* m;
     double* theta;
     double* lpsA;           // LPS: theta_coeff * theta / m per particle

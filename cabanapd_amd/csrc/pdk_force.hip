@@ -130,15 +130,15 @@ struct Bond32
 };
 
 // Guard band of the mixed form: true when the FP32 stretch s lies within
-// bandf * (|eta|_1 / |xi| + |s| + s0) of s0, evaluated as one FMA chain with
-// bx = bandf / |xi| and bs0 = bandf * s0 per slot / per launch.  The band is
-// 2^-13 relative, far above the FP32 error of s and of this evaluation.
+// bandf * (sqrt(3) max_k |eta_k| / |xi| + |s| + s0) of s0, evaluated as an FMA
+// chain with bx = sqrt(3) bandf / |xi| and bs0 = bandf * s0 per slot / per
+// launch (sqrt(3) max_k |eta_k| >= |eta|, one max3 instead of three FMAs).  The
+// band is 2^-13 relative, far above the FP32 error of s and of this evaluation.
 __device__ __forceinline__ bool inGuardBand( const Bond32& b, float s0f, float bandf, float bx, float bs0 )
 {
     float t = fmaf( -bandf, fabsf( b.s ), fabsf( b.s - s0f ) );
-    t = fmaf( -bx, fabsf( b.ex ), t );
-    t = fmaf( -bx, fabsf( b.ey ), t );
-    t = fmaf( -bx, fabsf( b.ez ), t );
+    const float em = fmaxf( fmaxf( fabsf( b.ex ), fabsf( b.ey ) ), fabsf( b.ez ) );
+    t = fmaf( -bx, em, t );
     return t <= bs0;
 }
 
@@ -1032,7 +1032,7 @@ kForceRows( const PDArgs* __restrict__ specs,
             else
             {
                 const float4 g = scalarLoad4( sten + q );
-                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, bandf/|xi|, |xi|^2, 0)
+                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0)
                 Bond32 b;
                 bond32s<MODEL>( g, ga.x, ga.z, ui0, ui1, ui2, uj0, uj1, uj2, b );
                 if constexpr ( FRAC )
@@ -1410,7 +1410,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
             if ( live )
             {
                 const float4 g = scalarLoad4( sten + q );
-                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, bandf/|xi|, |xi|^2, 0)
+                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0)
                 const float xinv = ga.x;
                 const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
                 if ( do_m )
@@ -2073,7 +2073,7 @@ __global__ void kStencilAux( const float4* __restrict__ table, int32_t S, float 
     {
         const float w = table[q].w;
         const float xinv = __builtin_amdgcn_rcpf( w );
-        aux[q] = make_float4( xinv, bandf * xinv, w * w, 0.0f );
+        aux[q] = make_float4( xinv, 1.7320508f * bandf * xinv, w * w, 0.0f );
     }
 }
 } // namespace

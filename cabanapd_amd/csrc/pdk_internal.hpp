@@ -211,7 +211,7 @@ void launchLpsState( const PDArgs* args_dev, const PDArgs& a, bool weighted_volu
 void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, double* partial_dev,
                    int64_t nblocks, hipStream_t st );
 int64_t energyBlocks( const PDArgs& a );
-// Per stencil entry (1/|xi|, bandf/|xi|, |xi|^2, 0) for the mixed row-window passes.
+// Per stencil entry (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0) for the mixed row-window passes.
 void launchStencilAux( const float4* table, int32_t S, float bandf, float4* aux, hipStream_t st );
 // Stress tensor pass into a.stress ([9][n], device order).
 void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
