@@ -1084,11 +1084,11 @@ kForceRows( const PDArgs* __restrict__ specs,
                     bits[q] = ww & ~brkm;
             }
         };
-        // PMB mixed form: the slots of a row (at most kRowMaxSlots) unrolled,
-        // so the window reads take immediate offsets and the pass fits 70
-        // VGPRs without spills (7 % faster on the fracture plate; unrolling
-        // the FP64 form measured slower)
-        if constexpr ( PREC == kMixed && !kIsLps<MODEL> )
+        // Mixed forms: the slots of a row (at most kRowMaxSlots) unrolled, so
+        // the window reads take immediate offsets and the PMB pass fits 70
+        // VGPRs without spills (7 % faster on the fracture plate, 2 % on the
+        // LPS cube; unrolling the FP64 form measured slower)
+        if constexpr ( PREC == kMixed )
         {
 #pragma unroll
             for ( int t = 0; t < kRowMaxSlots; t++ )
