@@ -1356,11 +1356,26 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
             __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
             if ( r + 1 < nrows )
                 loadRow( r + 1 );
-            for ( int t = 0; t < R.y; t++ )
+            // mixed form: the row's slots unrolled (immediate window offsets)
+            if constexpr ( PREC == kMixed )
             {
-                const int32_t q = R.x + t;
-                const bool live = __builtin_amdgcn_inverse_ballot_w64( scalarLoad( mwords + q ) );
-                bond( live, wb + lane + t, q );
+#pragma unroll
+                for ( int t = 0; t < kRowMaxSlots; t++ )
+                    if ( t < R.y )
+                    {
+                        const int32_t q = R.x + t;
+                        const bool live = __builtin_amdgcn_inverse_ballot_w64( scalarLoad( mwords + q ) );
+                        bond( live, wb + lane + t, q );
+                    }
+            }
+            else
+            {
+                for ( int t = 0; t < R.y; t++ )
+                {
+                    const int32_t q = R.x + t;
+                    const bool live = __builtin_amdgcn_inverse_ballot_w64( scalarLoad( mwords + q ) );
+                    bond( live, wb + lane + t, q );
+                }
             }
         }
     };
