@@ -15,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <random>
 #include <string>
 #include <stdexcept>
 #include <type_traits>
@@ -119,35 +120,47 @@ class Particles
         domain( lo, hi, nc, m + 1 );
     }
 
-    // Grid creation: one particle per cell at the cell centre, vol = cell
-    // volume, rho = 1 (CabanaPD_Particles.hpp:242-307).  The user functor
-    // selects which cells get a particle.
+    // Grid creation (CabanaPD_Particles.hpp:241-400): one particle per cell,
+    // vol = cell volume, rho = 1.  Cabana::InitUniform (the default) places it
+    // at the cell centre, Cabana::InitRandom at a uniformly random point of the
+    // cell (a fixed-seed generator: the same points run to run, not the ones
+    // of Cabana's random pool).  The user functor selects which cells get a
+    // particle.
     template <class ExecSpace, class UserFunctor>
-    void create( const ExecSpace&, UserFunctor user_create, const std::size_t num_previous = 0,
+    void create( const ExecSpace& exec, Cabana::InitUniform, UserFunctor user_create, const std::size_t num_previous = 0,
                  const bool create_frozen = false )
     {
-        if ( !_has_domain )
-            throw std::runtime_error( "Cannot create() particles without calling domain() first." );
-        if ( create_frozen )
-            throw std::runtime_error( "frozen particles are not supported by this build" );
-        std::vector<double> px;
-        const double pv = dx[0] * dx[1] * dx[2];
-        for ( int k = 0; k < _nc[2]; k++ )
-            for ( int j = 0; j < _nc[1]; j++ )
-                for ( int i = 0; i < _nc[0]; i++ )
-                {
-                    double p[3] = { local_mesh_lo[0] + i * dx[0] + 0.5 * dx[0],
-                                    local_mesh_lo[1] + j * dx[1] + 0.5 * dx[1],
-                                    local_mesh_lo[2] + k * dx[2] + 0.5 * dx[2] };
-                    if ( user_create( static_cast<int>( px.size() / 3 ), p ) )
-                        px.insert( px.end(), p, p + 3 );
-                }
-        appendParticles( px, std::vector<double>( px.size() / 3, pv ), num_previous );
+        createCells( exec, false, user_create, num_previous, create_frozen );
+    }
+    template <class ExecSpace, class UserFunctor>
+    void create( const ExecSpace& exec, Cabana::InitRandom, UserFunctor user_create, const std::size_t num_previous = 0,
+                 const bool create_frozen = false )
+    {
+        createCells( exec, true, user_create, num_previous, create_frozen );
+    }
+    template <class ExecSpace>
+    void create( const ExecSpace& exec, Cabana::InitUniform, const std::size_t num_previous = 0,
+                 const bool create_frozen = false )
+    {
+        createCells( exec, false, all(), num_previous, create_frozen );
+    }
+    template <class ExecSpace>
+    void create( const ExecSpace& exec, Cabana::InitRandom, const std::size_t num_previous = 0,
+                 const bool create_frozen = false )
+    {
+        createCells( exec, true, all(), num_previous, create_frozen );
+    }
+    template <class ExecSpace, class UserFunctor,
+              std::enable_if_t<!std::is_integral_v<UserFunctor>, int> = 0>
+    void create( const ExecSpace& exec, UserFunctor user_create, const std::size_t num_previous = 0,
+                 const bool create_frozen = false )
+    {
+        createCells( exec, false, user_create, num_previous, create_frozen );
     }
     template <class ExecSpace>
     void create( const ExecSpace& exec, const std::size_t num_previous = 0, const bool create_frozen = false )
     {
-        create( exec, []( const int, const double* ) { return true; }, num_previous, create_frozen );
+        createCells( exec, false, all(), num_previous, create_frozen );
     }
     // Custom creation from positions and volumes (CabanaPD_Particles.hpp:310-368).
     template <class ExecSpace, class PositionType, class VolumeType,
@@ -312,6 +325,37 @@ class Particles
     }
 
   private:
+    static auto all()
+    {
+        return []( const int, const double* ) { return true; };
+    }
+    template <class ExecSpace, class UserFunctor>
+    void createCells( const ExecSpace&, const bool random, UserFunctor user_create, const std::size_t num_previous,
+                      const bool create_frozen )
+    {
+        if ( !_has_domain )
+            throw std::runtime_error( "Cannot create() particles without calling domain() first." );
+        if ( create_frozen )
+            throw std::runtime_error( "frozen particles are not supported by this build" );
+        std::vector<double> px;
+        const double pv = dx[0] * dx[1] * dx[2];
+        std::mt19937_64 gen( 0x5eed );
+        std::uniform_real_distribution<double> unit( 0.0, 1.0 );
+        for ( int k = 0; k < _nc[2]; k++ )
+            for ( int j = 0; j < _nc[1]; j++ )
+                for ( int i = 0; i < _nc[0]; i++ )
+                {
+                    const double o[3] = { random ? unit( gen ) : 0.5, random ? unit( gen ) : 0.5,
+                                          random ? unit( gen ) : 0.5 };
+                    double p[3] = { local_mesh_lo[0] + i * dx[0] + o[0] * dx[0],
+                                    local_mesh_lo[1] + j * dx[1] + o[1] * dx[1],
+                                    local_mesh_lo[2] + k * dx[2] + o[2] * dx[2] };
+                    if ( user_create( static_cast<int>( px.size() / 3 ), p ) )
+                        px.insert( px.end(), p, p + 3 );
+                }
+        appendParticles( px, std::vector<double>( px.size() / 3, pv ), num_previous );
+    }
+
     void appendParticles( const std::vector<double>& px, const std::vector<double>& pv, std::size_t num_previous )
     {
         ParticleStorage& s = *_s;

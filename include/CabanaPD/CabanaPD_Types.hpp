@@ -69,6 +69,18 @@ struct Sum
 };
 } // namespace Kokkos
 
+// Particle initialisation tags of Cabana::Grid::createParticles, as the
+// reference's Particles::create takes them (CabanaPD_Particles.hpp:241-400).
+namespace Cabana
+{
+struct InitUniform
+{
+};
+struct InitRandom
+{
+};
+} // namespace Cabana
+
 namespace CabanaPD
 {
 // Model tags (src/CabanaPD_Types.hpp)
