@@ -144,3 +144,42 @@ def test_gpu_series_errors():
         s.values()  # the device values went with the spec
     s.close()
     s2.close()
+
+
+def test_gpu_dogbone_full_deck_grip_force():
+    """The dogbone tensile deck (examples/mechanics/dogbone_tensile_test.cpp:
+    PMB ElasticPerfectlyPlastic, grips at 0.1 m/s, 50,000 steps) with the grip
+    force kept as a device series (sum of f_x V over the right grip, the
+    example's output_force_x, every 10,000 steps): equal to the oracle's
+    curve (tests/golden/dogbone_epp_curve.json, tests/golden/make_dogbone_curve.py)
+    to 1e-12 of its size, and elastic-plastic -- the tangent stiffness of the
+    last interval is below a quarter of the first's while the engineering
+    stress stays below the yield stress."""
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    fx = json.load(open(os.path.join(here, "golden", "dogbone_epp_curve.json")))
+    say("case: dogbone deck, 50,000 steps FP64, grip force series")
+    w = configs.dogbone()
+    assert w["steps"] == 50000
+    g = gpu_spec(w)
+    configs.setup(g, w)
+    ser = g.series(None, "f", component=0, reducer="sum", capacity=8, ids=w["right_grip"], weight_volume=True)
+    dt = w["desc"].dt
+    assert fx["steps"] == [10000, 20000, 30000, 40000, 50000]
+    for k in fx["steps"]:
+        g.step(10000)
+        ser.update(k * dt)
+    _, v = ser.values()
+    ref = np.array(fx["grip_force_x"])
+    err = np.max(np.abs(v - ref)) / np.max(np.abs(ref))
+    say(f"  grip force {v.tolist()} (oracle {ref.tolist()}), rel {err:.2e}")
+    assert v.size == 5 and err <= 1e-12
+    first = abs(ref[0]) / 10000
+    last = abs(ref[4] - ref[3]) / 10000
+    stress = abs(ref[4]) / (0.013 * 0.0032)
+    say(f"  tangent stiffness last/first {last / first:.3f}, final stress {stress / 1e6:.1f} MPa")
+    assert last < 0.25 * first and stress < 30e6
+    assert fx["yielded_bonds"] > 0 and fx["broken_bonds"] == 0
+    ser.close()
+    g.close()
