@@ -331,18 +331,18 @@ def rows_leg(args):
     out["rows"]["series_update"] = dict(members=n, us_per_update=upd * 1e6, cpu_host_ms=cpu_series * 1e3,
                                         **frac_of(n * (8 + 8 + 8 + 8), upd * 1e3))
 
-    # rank 4: FP64 gather forms
+    # rank 4: gather forms (multi-material FP64 only)
     w = workload()
     mtype = (x[:, 0] >= 0.0).astype(np.int32)
     w["materials"] = ([(bd.K, 0.0, bd.G0), (2.0 * bd.K, 0.0, 2.0 * bd.G0)], mtype)
     force_row("multi_material", w, precisions=("fp64",))
-    force_row("epp", workload(mechanics="epp", sigma_y=3.0 * bd.K * 1e-3), precisions=("fp64",))
+    force_row("epp", workload(mechanics="epp", sigma_y=3.0 * bd.K * 1e-3))
     w = workload(thermal=True, alpha=1e-5, temp0=0.0)
     w["prescribed"] = [("temperature", np.arange(n), np.zeros(n), 1e7 * (x[:, 0] - x[:, 0].min()) / 0.1)]
-    force_row("thermal", w, precisions=("fp64",))
+    force_row("thermal", w)
     w = workload(thermal="dynamic", alpha=1e-5, temp0=0.0, kappa=50.0, cp=500.0, thermal_subcycle=10)
     w["temperature"] = 100.0 * (x[:, 0] - x[:, 0].min()) / 0.1
-    hb = force_row("heat_transfer", w, precisions=("fp64",), extra_cls=(KCLS_OTHER, "heat_and_half_step"))
+    hb = force_row("heat_transfer", w, extra_cls=(KCLS_OTHER, "heat_and_half_step"))
     r = out["rows"]["heat_transfer"]["fp64"]
     passes = steps // 10
     heat_ms = r["heat_and_half_step_ms_total"] / max(passes, 1)

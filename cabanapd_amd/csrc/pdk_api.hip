@@ -665,9 +665,6 @@ int pdk_spec_create( pdk_ctx ctx, const pdk_model_desc* desc, int64_t n, pdk_spe
         require( desc->precision == PDK_FP64 || desc->precision == PDK_MIXED, "unknown precision" );
         require( !( isContact( *desc ) && desc->precision != PDK_FP64 ), "contact models run in FP64" );
         require( desc->dt > 0, "dt must be positive" );
-        require( ( desc->mechanics == PDK_MECH_ELASTIC && desc->thermal == PDK_TEMP_INDEPENDENT ) ||
-                     desc->precision == PDK_FP64,
-                 "ElasticPerfectlyPlastic and TemperatureDependent specimens are FP64 only" );
         PDK_HIP( hipSetDevice( ctx->device ) );
         auto s = std::make_unique<pdk_spec_s>();
         s->ctx = ctx;

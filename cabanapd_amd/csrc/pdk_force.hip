@@ -493,9 +493,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     // multi-material (FP64): model of the material pair; null with one material
     const uint8_t* __restrict__ mt = ( PREC == kF64 ) ? A.mtype : nullptr;
     const int ti4 = mt ? 4 * static_cast<int>( mt[pc] ) : 0;
-    // PMB family extensions (FP64): temperature, plastic stretch per bond slot
-    const double* __restrict__ tmp = ( PREC == kF64 && !kIsLps<MODEL> ) ? A.temp : nullptr;
-    double* __restrict__ spl = ( PREC == kF64 && !kIsLps<MODEL> ) ? A.splas : nullptr;
+    // PMB family extensions: temperature, plastic stretch per bond slot
+    const double* __restrict__ tmp = kIsLps<MODEL> ? nullptr : A.temp;
+    double* __restrict__ spl = kIsLps<MODEL> ? nullptr : A.splas;
     const double Ti = tmp ? tmp[pc] : 0.0;
     double Ai = 0.0, Bi = 0.0;
     if constexpr ( kIsLps<MODEL> )
@@ -719,11 +719,23 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     else
                         bond32<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
                                        uj[k][1], uj[k][2], b );
+                    // TemperatureDependent: the thermal strain alpha * temp_avg
+                    // (FP64, rounded once) leaves the stretch; the guard band
+                    // grows by its magnitude, which bounds that rounding.
+                    double temp_avg = 0.0;
+                    float ath = 0.0f;
+                    const float s_mech = b.s;
+                    if ( tmp )
+                    {
+                        temp_avg = thermalAvg( A, Ti, tj[k] );
+                        ath = static_cast<float>( A.alpha * temp_avg );
+                        b.s = b.s - ath;
+                    }
                     if constexpr ( FRAC )
                     {
                         if ( !nf_i && !nf_j )
                         {
-                            const float band = A.bandf * ( b.etan + fabsf( b.s ) + A.s0f );
+                            const float band = A.bandf * ( b.etan + fabsf( s_mech ) + A.s0f + fabsf( ath ) );
                             if ( fabsf( b.s - A.s0f ) <= band )
                             {
                                 // guard band: the reference's FP64 test on this bond
@@ -737,7 +749,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                                 Bond64 e;
                                 getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
                                                uj[k][1], uj[k][2], e );
-                                brk = criticalStretch64( e.r, e.xi, bcc );
+                                brk = criticalStretch64( e.r, e.xi, tmp ? thermalBcc( A, temp_avg ) : bcc );
                             }
                             else
                                 brk = b.s > A.s0f;
@@ -747,7 +759,14 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     {
                         float coeff;
                         if constexpr ( !kIsLps<MODEL> )
-                            coeff = ( uvol ? cv32u : static_cast<float>( c ) * static_cast<float>( vj[k] ) ) * b.s;
+                        {
+                            // ElasticPerfectlyPlastic: the plastic stretch stays FP64
+                            const float sd =
+                                spl ? static_cast<float>( plasticStretch( spl + ( base + q ) * kSlice + lane,
+                                                                          static_cast<double>( b.s ), A.sY ) )
+                                    : b.s;
+                            coeff = ( uvol ? cv32u : static_cast<float>( c ) * static_cast<float>( vj[k] ) ) * sd;
+                        }
                         else
                         {
                             const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;

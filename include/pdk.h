@@ -116,7 +116,8 @@ typedef struct pdk_model_desc {
     int32_t polydisperse;   /* 1: per-particle radii (extension; equal radii reduce to the reference) */
     int32_t ordering;       /* particle order on the device: PDK_ORDER_AUTO / _MORTON / _LATTICE_ROWS */
     int32_t bond_layout;    /* PDK_BONDS_AUTO / PDK_BONDS_COMPACT */
-    /* mechanics and thermal coupling (PMB family; zero = Elastic, TemperatureIndependent) */
+    /* mechanics and thermal coupling (PMB family; zero = Elastic, TemperatureIndependent);
+       both precision forms: mixed keeps the thermal strain and the plastic stretch in FP64 */
     int32_t mechanics;      /* pdk_mechanics */
     int32_t thermal;        /* pdk_thermal */
     double sigma_y;         /* yield stress (EPP): s_Y = sigma_y / (3 K) */
