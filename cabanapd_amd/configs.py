@@ -382,7 +382,7 @@ def lps_cube(n_side=128, steps=500, precision="fp64", seed=44758454, dt=None, si
     return w
 
 
-def serving_request(rng, dx=5e-4, rho=8000.0, nu=0.25, precision="mixed", n_range=(2e4, 2e6)):
+def serving_request(rng, dx=5e-4, rho=8000.0, nu=0.25, precision="mixed", n_range=(2e4, 2e6), steps_range=(200, 2000)):
     """One specimen of the serving stream: a rectangular PMB fracture plate
     with log-uniform particle count, aspect ratio U[1,4], thickness U[3,20]
     cells, E ~ U[50,210] GPa, G0 ~ logU[1, 5e4] J/m^2, 0-3 random straight
@@ -405,7 +405,7 @@ def serving_request(rng, dx=5e-4, rho=8000.0, nu=0.25, precision="mixed", n_rang
     high = 0.5 * np.asarray(size)
     x, vol, d = grid_cell_centred(low, high, (nx, ny, nz))
     desc = make_desc("PMB", fracture=True, precision=precision, horizon=horizon, K=K, G0=G0, dt=dt)
-    steps = int(rng.integers(200, 2001))
+    steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
     w = _base(desc, x, vol, rho, "serving_%dx%dx%d" % (nx, ny, nz), dx, steps)
     nnotch = int(rng.integers(0, 4))
     if nnotch:

@@ -931,6 +931,48 @@ int pdk_spec_set_state( pdk_spec s, const double* u, const double* v )
     } );
 }
 
+int pdk_spec_copy_state( pdk_spec dst, pdk_spec src )
+{
+    return guarded( [&] {
+        require( dst && src && dst != src, "two distinct specs required" );
+        require( dst->ctx == src->ctx, "specs belong to different contexts" );
+        require( src->inited && dst->built, "source not initialised or destination bonds not built" );
+        require( !src->in_batch && !dst->in_batch, "spec is running in the batch engine" );
+        require( !isContact( src->d ) && !isContact( dst->d ), "contact specs have no bond state to copy" );
+        require( dst->n == src->n && dst->h_perm == src->h_perm, "specs differ in particle count or device order" );
+        require( dst->d.model == src->d.model && dst->d.fracture == src->d.fracture &&
+                     dst->d.mechanics == src->d.mechanics && dst->d.thermal == src->d.thermal,
+                 "specs differ in model" );
+        require( dst->slot.slots == src->slot.slots && dst->total_rows == src->total_rows &&
+                     dst->intact.bytes() == src->intact.bytes() && dst->col.bytes() == src->col.bytes(),
+                 "specs differ in bond layout" );
+        const size_t v3 = 3 * static_cast<size_t>( src->n ) * sizeof( double );
+        const size_t v1 = static_cast<size_t>( src->n ) * sizeof( double );
+        // src's stream must be done with its state before dst's copies read it
+        PDK_HIP( hipStreamSynchronize( src->st ) );
+        auto copy = [&]( const DevBuf& a, const DevBuf& b, size_t bytes ) {
+            if ( a && b && bytes )
+                PDK_HIP( hipMemcpyAsync( a.as<void>(), b.as<void>(), bytes, hipMemcpyDeviceToDevice, dst->st ) );
+        };
+        PDK_HIP( hipMemcpyAsync( dst->Ucur(), src->Ucur(), v3, hipMemcpyDeviceToDevice, dst->st ) );
+        copy( dst->v, src->v, v3 );
+        copy( dst->f, src->f, v3 );
+        copy( dst->intact, src->intact, src->intact.bytes() );
+        copy( dst->W, src->W, v1 );
+        copy( dst->phi, src->phi, v1 );
+        copy( dst->m, src->m, v1 );
+        copy( dst->theta, src->theta, v1 );
+        copy( dst->lpsA, src->lpsA, v1 );
+        copy( dst->lpsB, src->lpsB, v1 );
+        copy( dst->temp, src->temp, v1 );
+        if ( dst->splas && src->splas && dst->splas.bytes() == src->splas.bytes() )
+            copy( dst->splas, src->splas, src->splas.bytes() );
+        dst->steps_done = src->steps_done;
+        dst->inited = true;
+        PDK_HIP( hipStreamSynchronize( dst->st ) );
+    } );
+}
+
 int pdk_spec_bytes( pdk_spec s, uint64_t* bytes )
 {
     return guarded( [&] {

@@ -34,6 +34,7 @@ _SIGS = {
     "pdk_spec_destroy": (C.c_int, [_VP]),
     "pdk_spec_upload": (C.c_int, [_VP, _D, _D, _D, _D, _D, _I32, _D]),
     "pdk_spec_set_state": (C.c_int, [_VP, _D, _D]),
+    "pdk_spec_copy_state": (C.c_int, [_VP, _VP]),
     "pdk_spec_bytes": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
     "pdk_build_bonds": (C.c_int, [_VP]),
     "pdk_bond_stats": (C.c_int, [_VP, _I64, _I32]),
@@ -200,6 +201,10 @@ class Spec:
 
     def set_state(self, u=None, v=None):
         _check(lib().pdk_spec_set_state(self.h, _dp(_f64(u, (self.n, 3))), _dp(_f64(v, (self.n, 3)))))
+
+    def copy_state_from(self, src):
+        """Restart this spec from src's state between steps (pdk_spec_copy_state)."""
+        _check(lib().pdk_spec_copy_state(self.h, src.h))
 
     def build_bonds(self):
         _check(lib().pdk_build_bonds(self.h))

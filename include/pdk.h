@@ -190,6 +190,15 @@ int pdk_spec_upload(pdk_spec spec, const double* x, const double* u, const doubl
                     const double* radius);
 /* Replace u, v (caller order) on an existing spec; f is reset to zero. */
 int pdk_spec_set_state(pdk_spec spec, const double* u, const double* v);
+/* Restart dst from src's state between steps: displacement, velocity, force,
+ * intact-bond bits, LPS weighted volume / dilatation, energy, damage,
+ * temperature, plastic stretch and the step count are copied device to
+ * device.  No counterpart in the reference (a Solver is never forked); it lets
+ * one state be advanced in both precision forms.  The specs must share the
+ * context, the particle count and device order, the model and the bond
+ * layout (same positions, same build); their precision may differ.  dst must
+ * have its bonds built; it counts as initialised afterwards. */
+int pdk_spec_copy_state(pdk_spec dst, pdk_spec src);
 int pdk_spec_bytes(pdk_spec spec, uint64_t* bytes);
 
 /* ---- bond family -------------------------------------------------------------

@@ -123,3 +123,36 @@ def test_gpu_contact_trajectory_parity(model, poly):
     say(f"  contacts {gn.size} u rel err {eu:.3e} v rel err {ev:.3e}")
     assert np.array_equal(go, oo) and np.array_equal(gn, on)
     assert eu <= 1e-9 and ev <= 1e-9
+
+
+def test_gpu_contact_rebuild_download_capacity():
+    """Regression: a contact list rebuilt with more pairs than the first
+    build.  pdk_download_bonds must size itself from the current list (the
+    count is refreshed after every rebuild) and refuse a caller buffer
+    smaller than the current pair count instead of writing past its end."""
+    import ctypes as C
+    say("case: contact list growth across rebuilds, download capacity")
+    w = configs.dem_packing(n=2000, seed=11, model="HERTZ", polydisperse=True, dt=1e-8, steps=4, rebuild_every=1,
+                            speed=0.0, gravity=False)
+    g = engine.Spec(ctx(), w["desc"], w["x"].shape[0])
+    configs.upload(g, w)
+    configs.setup(g, w)
+    centre = w["x"].mean(axis=0)
+    counts = [g.bond_stats()[0]]
+    for frac in (0.01, 0.02, 0.03):
+        # move every sphere toward the centre: gaps shrink, pairs appear at the next rebuild
+        g.set_state(u=-(w["x"] - centre) * frac, v=np.zeros_like(w["x"]))
+        g.step(1)
+        total = g.bond_stats()[0]
+        off, nbr, intact = g.bonds()
+        assert off[-1] == total == nbr.size
+        counts.append(total)
+    say(f"  pair counts after each rebuild: {counts}")
+    assert counts[-1] > counts[0]
+    small = np.zeros(max(counts[0], 1), dtype=np.int64)
+    offs = np.zeros(g.n + 1, dtype=np.int64)
+    L = engine.lib()
+    rc = L.pdk_download_bonds(g.h, small.size, offs.ctypes.data_as(C.POINTER(C.c_int64)),
+                              small.ctypes.data_as(C.POINTER(C.c_int64)), None)
+    assert rc != 0 and b"capacity" in L.pdk_last_error()
+    g.close()
