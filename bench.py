@@ -30,16 +30,44 @@ from cabanapd_amd._abi import KCLS_FORCE  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 DEFAULT_CELLS = (200, 400, 18)
 
-# HBM bytes per launch of the force pass on the default plate, measured with
-# rocprofv3 --pmc (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE; the factors come from a
-# plain 1 GiB device copy read through the same counters).  Summaries with
-# the commands are kept under profiles/.
-MEASURED_TRAFFIC = {
-    "kForceRows<0, true, 1, false>": 430.3e6,
-    "kForceRows<0, true, 0, false>": 700.7e6,
-    "kForce<0, true, 1, 2, true>": 503.2e6,
-    "kForce<0, true, 0, 2, true>": 744.5e6,
-}
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_counters(kernel, cells):
+    """HBM traffic per launch and the VALU-issue share of a kernel, as the
+    last rocprofv3 counter passes on this build measured them: the lines
+    "traffic <kernel> <cells> <bytes>" and "valu_busy <kernel> <cells> <frac>"
+    of the counter summaries under profiles/ (FETCH_SIZE x 2.0 + WRITE_SIZE x
+    1.0 per launch, the factors calibrated on a plain device copy).  None when
+    no summary covers this kernel on this workload."""
+    import re
+    tag = "%dx%dx%d" % tuple(cells)
+    out = {"traffic": None, "valu_busy": None, "source": None}
+    if not os.path.isdir(PROFILES):
+        return out
+    for name in sorted(os.listdir(PROFILES)):
+        if not name.endswith(".md"):
+            continue
+        for ln in open(os.path.join(PROFILES, name)):
+            m = re.match(r"^(traffic|valu_busy) (.+) (\d+x\d+x\d+) ([0-9.eE+-]+)\s*$", ln.strip())
+            if m and m.group(2) == kernel and m.group(3) == tag:
+                out[m.group(1)] = float(m.group(4))
+                out["source"] = "profiles/" + name
+    return out
+
+
+def host_cores():
+    """Physical cores of the host (lscpu: sockets x cores per socket) and the
+    CPUs this process may run on."""
+    import subprocess
+    phys = None
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict((a.strip(), b.strip()) for a, b in (ln.split(":", 1) for ln in txt.splitlines() if ":" in ln))
+        phys = int(kv["Socket(s)"]) * int(kv["Core(s) per socket"])
+    except Exception:
+        pass
+    return phys, len(os.sched_getaffinity(0))
 
 
 def algorithmic_bytes_per_step(n_particles, bonds, passes=1, fracture=True, lps=False):
@@ -76,15 +104,18 @@ def timed_run(spec, warmup, steps):
     return t1 - t0, force_ms, launches
 
 
-def cpu_baseline(w, bonds, budget_s=12.0):
+def cpu_baseline_child(cells, budget_s):
+    """Runs in a child process whose OpenMP environment is set before the
+    oracle loads: the oracle on the first steps of the same specimen."""
     from oracle import binding as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    threads = min(threads, 16)
+    threads = int(os.environ["OMP_NUM_THREADS"])
     orc.set_threads(threads)
+    w = configs.kw_plate(num_cells=cells, steps=1)
     d = type(w["desc"]).from_buffer_copy(w["desc"])
     d.precision = 0
     o = orc.OracleSpec(d, w["x"].shape[0], w["x"], w["vol"], u=w["u"], v=w["v"], rho=w["rho"])
     configs.setup(o, w)
+    bonds = o.bond_stats()[0]
     t0 = time.perf_counter()
     o.step(1)
     dt1 = time.perf_counter() - t0
@@ -93,11 +124,31 @@ def cpu_baseline(w, bonds, budget_s=12.0):
     if more:
         o.step(more)
     dtm = time.perf_counter() - t0
-    steps = 1 + more
-    total = dt1 + dtm
     o.close()
-    return {"value": bonds * steps / total, "unit": "bond-evals/s", "cores": threads, "kind": "port",
-            "sample": f"first {steps} steps of the same specimen (oracle, OpenMP, {threads} threads)"}
+    print(json.dumps({"bonds": bonds, "steps": 1 + more, "seconds": dt1 + dtm}))
+
+
+def cpu_baseline(cells, budget_s=12.0):
+    """The oracle (a port of the reference's CPU path: one OpenMP thread per
+    particle chunk, serial neighbor loop) timed on this host in a child
+    process with OMP_PROC_BIND=close and OMP_PLACES=cores.  Threads: the CPUs
+    this process may use, capped by OMP_NUM_THREADS when the environment sets
+    it (a GPU box allots 16 host CPUs per GPU)."""
+    import subprocess
+    phys, avail = host_cores()
+    threads = avail
+    if os.environ.get("OMP_NUM_THREADS"):
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", ",".join(map(str, cells)),
+                        "--cpu-budget", str(budget_s)], env=env, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        return {"error": p.stderr.strip()[-500:]}
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    return {"value": r["bonds"] * r["steps"] / r["seconds"], "unit": "bond-evals/s", "cores": threads,
+            "kind": "port", "threads": threads, "host_physical_cores": phys, "cpus_available": avail,
+            "binding": "OMP_PROC_BIND=close OMP_PLACES=cores",
+            "sample": "first %d steps of the same specimen (oracle, OpenMP, %d threads)" % (r["steps"], threads)}
 
 
 def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
@@ -383,6 +434,15 @@ def lps_leg(args, n_side=128):
     _, _, i32 = specs["mixed"].bonds()
     for s in specs.values():
         s.close()
+    # The specimen's own sensitivity: plain FP64 from the initial displacement
+    # rounded to FP32 once (a relative change <= 6e-8), same steps (untimed)
+    w32 = dict(w)
+    w32["u"] = w["u"].astype(np.float32).astype(np.float64)
+    f = make_spec(ctx, w32, "fp64")
+    f.step(args.warmup + args.steps)
+    uf = f.get("u")
+    _, _, i_f = f.bonds()
+    f.close()
     ctx.close()
     b_alg = algorithmic_bytes_per_step(n, bonds, passes=3, fracture=True, lps=True)
     return {"workload": "LPS elastic-brittle cube %d^3 (%d particles, %d bonds), seeded perturbation, "
@@ -391,6 +451,9 @@ def lps_leg(args, n_side=128):
             "speedup": res["mixed"]["value"] / res["fp64"]["value"],
             "u_rel_err": float(np.max(np.abs(u32 - u64)) / den) if den > 0 else 0.0,
             "broken_bonds_fp64": int((i64 == 0).sum()), "broken_bond_mismatches": int((i64 != i32).sum()),
+            "fp32_input_floor": {"u_rel_err": float(np.max(np.abs(uf - u64)) / den) if den > 0 else 0.0,
+                                 "broken_bond_mismatches": int((i64 != i_f).sum()),
+                                 "note": "plain FP64 started from u0 rounded to FP32, against plain FP64"},
             "alg_bytes_per_step": b_alg,
             "achieved_GBs_mixed": b_alg / (res["mixed"]["ms_per_step"] * 1e-3) / 1e9}
 
@@ -478,9 +541,14 @@ def main():
                          "lps: LPS fracture cube 128^3; dem: Hertz-JKR contact box of 1e6 spheres; "
                          "rows: linearized models, output passes, region series, multi-material, EPP, "
                          "thermal and heat transfer on the plate geometry")
+    ap.add_argument("--cpu-baseline-child", type=str, default="", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-kernel", type=float, default=0.0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        cpu_baseline_child(tuple(int(c) for c in args.cpu_baseline_child.split(",")), args.cpu_budget)
+        return
     if args.copy_kernel > 0:
         copy_kernel(args.copy_kernel)
         return
@@ -536,6 +604,7 @@ def main():
     avg_ms = m["force_ms"] / max(m["launches"], 1)
     b_alg = algorithmic_bytes_per_step(n, bonds, passes=1, fracture=True)
     achieved = b_alg / (avg_ms * 1e-3) / 1e9
+    ctr = measured_counters(m["kernel"], cells)
     line = {
         "metric": "bond evals/s vs plain FP64 at fixed tol; specimen solves/s + p50/p99 latency",
         "value": value,
@@ -559,16 +628,18 @@ def main():
         "mixed_vs_fp64": {"speedup": value / fp64_value, "u_rel_err": u_err,
                           "broken_bonds_fp64": broken64, "broken_bond_mismatches": broken_diff},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get(m["kernel"]) if cells == DEFAULT_CELLS else None,
-                     "kernel": m["kernel"],
-                     "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": b_alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": ctr["traffic"],
+                     "kernel": m["kernel"], "kernel_avg_ms": avg_ms, "alg_bytes_per_launch": b_alg,
+                     "valu_busy": ctr["valu_busy"], "counters_source": ctr["source"],
+                     "note": "frac is the algorithmic-bytes yardstick against HBM peak; the counters show the pass "
+                             "limited by VALU issue (valu_busy = VALU instruction cycles / launch cycles), "
+                             "not by HBM"},
         "cpu_baseline": None,
     }
     if args.requests > 0:
         line["serving"] = serving(args.requests, max_particles=args.max_particles)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w, bonds)
+        line["cpu_baseline"] = cpu_baseline(cells)
     print(json.dumps(line))
 
 

@@ -326,12 +326,17 @@ class Solver
             runImpl( first, rest... );
     }
 
+    // One step driven by the caller: afterwards the displacement, velocity
+    // and force slices (and temperature) hold the new state, as the
+    // reference's device slices do after Solver::runStep.
     void runStep( const int step )
     {
         if ( _pre )
             _pre( step * dt );
         detail::check( pdk_step( spec(), 1 ) );
         output( step );
+        neighbor->device().pull( particles.storage(), { PDK_FIELD_U, PDK_FIELD_V, PDK_FIELD_F,
+                                                        PDK_FIELD_TEMPERATURE } );
     }
     template <class BoundaryType>
     void runStep( const int step, BoundaryType boundary_condition )

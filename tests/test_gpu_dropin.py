@@ -1,25 +1,30 @@
 """The drop-in C++ API (include/CabanaPD) driven end to end on the GPU.
 
-dropin_kalthoff_winkler is the reference's Kalthoff-Winkler example
-(examples/mechanics/kalthoff_winkler.cpp) written against the drop-in headers;
-it checks the reference's regression value (7.42e-2 +- 1.1e-4,
-regression_test/tstKalthoffWinkler.hpp:53) on the input deck the reference
-ships (tests/golden/kalthoff_winkler.json).  dropin_elastic_wave is
-examples/mechanics/elastic_wave.cpp on the deck of
-regression_test/tstElasticWave.hpp; its displacement profile is checked
-against that test's polynomial fit and against the oracle.  dropin_force runs
-the tstForce flow (Force, Neighbor, computeForce, computeEnergy) against
-lattice sums.  dropin_crack_inclusion is examples/mechanics/crack_inclusion.cpp
-(two materials through createMultiForceModel, a functor force boundary
-condition) checking regression_test/tstCrackInclusion.hpp:55.  dropin_dogbone
-(ElasticPerfectlyPlastic, displacement grips), dropin_thermal_deformation
-(TemperatureDependent, imposed temperature) and dropin_thermal_heat_transfer
-(DynamicTemperature, cold planes) are the reference's examples on
-their decks; their force_update = false functors run on the host each step,
-and the results equal the engine's prescribed-value runs bit for bit.
+tests/cpp/dropin_scenarios is one program of our own that exercises the
+drop-in surface scenario by scenario on input decks in the reference's
+format (tests/golden/*.json); it prints results, the checks live here:
 
-The drivers are built on the CPU beforehand (`make -C tests/cpp`, also done by
-__graft_entry__.build()); the test builds them only when they are missing.
+* notched-impact: PMB fracture on the Kalthoff-Winkler deck, the crack-front
+  Max series against the reference's regression value 7.42e-2 +- 1.1e-4
+  (regression_test/tstKalthoffWinkler.hpp:53);
+* pulse: a LinearLPS ForceModel (which runs the LPS force) on the elastic-wave
+  deck, the displacement profile against the polynomial of
+  regression_test/tstElasticWave.hpp:54-93 and against the oracle;
+* inclusion: two LPS materials through createMultiForceModel(AverageTag), a
+  functor traction, no-fail bands, against 5.75e-3 +- 5e-5
+  (regression_test/tstCrackInclusion.hpp:55);
+* grips / heated / conduction: ElasticPerfectlyPlastic, TemperatureDependent
+  and DynamicTemperature specimens whose force_update = false functors run on
+  the host each step; the results equal the engine's prescribed-value runs
+  bit for bit;
+* rebound / pulloff: DEM-only Solvers (HertzianModel, HertzianJKRModel) on the
+  two-sphere decks, against the restitution (regression_test/tstHertz.hpp:135)
+  and pull-off (regression_test/tstHertzJKR.hpp:145) values.
+
+dropin_force runs the tstForce flow (Force, Neighbor, computeForce,
+computeEnergy) against lattice sums.  The drivers are built on the CPU
+beforehand (`make -C tests/cpp`, also done by __graft_entry__.build()); the
+test builds them only when they are missing.
 """
 import os
 import subprocess
@@ -54,13 +59,28 @@ def test_dropin_force_known_answers(tmp_path):
     assert rc == 0 and "DROPIN_FORCE_OK" in out
 
 
-def test_dropin_kalthoff_winkler_example(tmp_path):
+def scenario(name, deck, out, cwd, timeout=110):
+    rc, log = run([driver("dropin_scenarios"), name, deck, out], cwd, timeout)
+    assert rc == 0 and "SCENARIO_DONE" in log
+    vals = {}
+    for ln in log.splitlines():
+        parts = ln.split()
+        if len(parts) == 2:
+            try:
+                vals[parts[0]] = float(parts[1])
+            except ValueError:
+                pass
+    return vals
+
+
+def test_dropin_notched_impact_regression(tmp_path):
     deck = os.path.join(HERE, "golden", "kalthoff_winkler.json")
-    rc, out = run([driver("dropin_kalthoff_winkler"), deck, str(tmp_path / "crack_y.txt")], str(tmp_path), 110)
-    assert rc == 0 and "KW_REGRESSION_OK" in out
+    vals = scenario("notched-impact", deck, str(tmp_path / "crack_y.txt"), str(tmp_path))
+    say(f"  crack extent {vals['crack_extent']!r} after {vals['steps']:.0f} steps")
+    assert abs(vals["crack_extent"] - 7.42e-2) <= 1.1e-4
 
 
-def test_dropin_elastic_wave_example(tmp_path):
+def test_dropin_pulse_profile(tmp_path):
     import json
 
     import numpy as np
@@ -69,8 +89,7 @@ def test_dropin_elastic_wave_example(tmp_path):
     from tests.parity import oracle_spec
     deck = os.path.join(HERE, "golden", "elastic_wave.json")
     prof = tmp_path / "displacement_profile.txt"
-    rc, out = run([driver("dropin_elastic_wave"), deck, str(prof)], str(tmp_path), 110)
-    assert rc == 0 and "EW_DONE" in out
+    scenario("pulse", deck, str(prof), str(tmp_path))
     rows = np.array([[float(t) for t in ln.split()] for ln in prof.read_text().splitlines() if ln.strip()])
     xs, us = rows[:, 1], rows[:, 2]
     fx = json.load(open(os.path.join(HERE, "golden", "elastic_wave_polyfit.json")))
@@ -90,10 +109,11 @@ def test_dropin_elastic_wave_example(tmp_path):
     assert np.max(np.abs(us[a] - uo[b])) <= 1e-5 * np.max(np.abs(uo))
 
 
-def test_dropin_crack_inclusion_example(tmp_path):
+def test_dropin_inclusion_regression(tmp_path):
     deck = os.path.join(HERE, "golden", "crack_inclusion.json")
-    rc, out = run([driver("dropin_crack_inclusion"), deck, str(tmp_path / "incl_crack_y.txt")], str(tmp_path), 110)
-    assert rc == 0 and "CI_REGRESSION_OK" in out
+    vals = scenario("inclusion", deck, str(tmp_path / "incl_crack_y.txt"), str(tmp_path))
+    say(f"  crack extent {vals['crack_extent']!r} after {vals['steps']:.0f} steps")
+    assert abs(vals["crack_extent"] - 5.75e-3) <= 5e-5
 
 
 def _deck_variant(tmp_path, name, **changes):
@@ -106,12 +126,11 @@ def _deck_variant(tmp_path, name, **changes):
     return str(path)
 
 
-def test_dropin_dogbone_example(tmp_path):
-    """examples/mechanics/dogbone_tensile_test.cpp (EPP, displacement grips
-    through a force_update = false functor evaluated on the host each step)
-    on its deck with the grips at 20 m/s for 300 steps: the final
-    displacement equals the engine run of the same specimen with the grips as
-    prescribed a + b t values, bit for bit."""
+def test_dropin_grips_equal_engine(tmp_path):
+    """The dogbone deck with the grips pulled at 20 m/s for 300 steps through
+    a force_update = false displacement functor: the final displacement
+    equals the engine run of the same specimen with the grips as prescribed
+    a + b t values, bit for bit."""
     import numpy as np
 
     from cabanapd_amd import configs
@@ -120,8 +139,7 @@ def test_dropin_dogbone_example(tmp_path):
     deck = _deck_variant(tmp_path, "dogbone_tensile_test.json", grip_velocity=20.0, final_time=(steps + 0.5) * 3.0e-7,
                          output_frequency=100)
     out = tmp_path / "dogbone_u.bin"
-    rc, log = run([driver("dropin_dogbone"), deck, str(out)], str(tmp_path), 110)
-    assert rc == 0 and "DB_DONE" in log
+    vals = scenario("grips", deck, str(out), str(tmp_path))
     w = configs.dogbone(grip_velocity=20.0, steps=steps)
     u_drop = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
     assert u_drop.shape == w["x"].shape
@@ -130,24 +148,25 @@ def test_dropin_dogbone_example(tmp_path):
     g.step(steps)
     u = g.get("u")
     nd = int(np.sum(u_drop != u))
-    say(f"  drop-in vs engine: differing={nd} max|u|={np.max(np.abs(u)):.3e}")
+    say(f"  drop-in vs engine: differing={nd} max|u|={np.max(np.abs(u)):.3e} grip force {vals['last_grip_force']:.4e}")
     assert nd == 0
+    # the specimen resists the pull: the internal force on the moving grip points back (-x)
+    assert vals["last_grip_force"] < 0.0
     g.close()
 
 
-def test_dropin_thermal_deformation_example(tmp_path):
-    """examples/thermomechanics/thermal_deformation.cpp on its deck (13333
-    steps; the temperature BodyTerm evaluated on the host each step): the
-    final displacement and temperature equal the engine run with the
-    temperature as prescribed a + b t values, bit for bit."""
+def test_dropin_heated_equals_engine(tmp_path):
+    """The thermal-deformation deck (13333 steps; the temperature ramp set by
+    a force = false BodyTerm evaluated on the host each step): the final
+    displacement and temperature equal the engine run with the temperature as
+    prescribed a + b t values, bit for bit."""
     import numpy as np
 
     from cabanapd_amd import configs
     from tests.parity import gpu_spec
     deck = os.path.join(HERE, "golden", "thermal_deformation.json")
     out = tmp_path / "thermal_u.bin"
-    rc, log = run([driver("dropin_thermal_deformation"), deck, str(out)], str(tmp_path), 110)
-    assert rc == 0 and "TD_DONE" in log
+    scenario("heated", deck, str(out), str(tmp_path))
     w = configs.thermal_deformation()
     u_drop = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
     t_drop = np.fromfile(str(out) + ".temp", dtype=np.float64)
@@ -161,11 +180,11 @@ def test_dropin_thermal_deformation_example(tmp_path):
     g.close()
 
 
-def test_dropin_thermal_heat_transfer_example(tmp_path):
-    """examples/thermomechanics/thermal_deformation_heat_transfer.cpp on its
-    deck with heat conduction every 20 steps for 100 steps (the deck's own
-    cadence, 45000, needs 19M steps): the final displacement and temperature
-    equal the engine run, bit for bit."""
+def test_dropin_conduction_equals_engine(tmp_path):
+    """The heat-transfer deck with conduction every 20 steps for 100 steps
+    (the deck's own cadence, 45000, needs 19M steps) and both y faces held
+    cold: the final displacement and temperature equal the engine run, bit
+    for bit."""
     import numpy as np
 
     from cabanapd_amd import configs
@@ -174,8 +193,7 @@ def test_dropin_thermal_heat_transfer_example(tmp_path):
     deck = _deck_variant(tmp_path, "thermal_deformation_heat_transfer.json", final_time=(steps + 0.5) * 4.2e-7,
                          thermal_subcycle_steps=sub, output_frequency=50)
     out = tmp_path / "heat_u.bin"
-    rc, log = run([driver("dropin_thermal_heat_transfer"), deck, str(out)], str(tmp_path), 110)
-    assert rc == 0 and "HT_DONE" in log
+    scenario("conduction", deck, str(out), str(tmp_path))
     w = configs.thermal_deformation_heat_transfer(steps=steps, thermal_subcycle=sub)
     u_drop = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
     t_drop = np.fromfile(str(out) + ".temp", dtype=np.float64)
@@ -187,3 +205,17 @@ def test_dropin_thermal_heat_transfer_example(tmp_path):
     say(f"  drop-in vs engine: differing={nd} max|u|={np.max(np.abs(u)):.3e} T in [{T.min():.4f}, {T.max():.4f}]")
     assert nd == 0
     g.close()
+
+
+def test_dropin_hertz_rebound(tmp_path):
+    deck = os.path.join(HERE, "golden", "hertzian_contact.json")
+    vals = scenario("rebound", deck, str(tmp_path / "unused"), str(tmp_path))
+    say(f"  speed ratio {vals['speed_ratio']!r} after {vals['steps']:.0f} steps")
+    assert abs(vals["speed_ratio"] - 0.5) <= 1e-3
+
+
+def test_dropin_jkr_pulloff(tmp_path):
+    deck = os.path.join(HERE, "golden", "hertzian_jkr_contact.json")
+    vals = scenario("pulloff", deck, str(tmp_path / "unused"), str(tmp_path))
+    say(f"  pull-off ratio {vals['pulloff_ratio']!r} after {vals['steps']:.0f} steps")
+    assert abs(vals["pulloff_ratio"] + 1.0) <= 5e-3
