@@ -134,6 +134,25 @@ PDK_BONDS_AUTO = 0
 PDK_BONDS_COMPACT = 1
 
 
+PDK_CONTACT_NORMAL_REPULSION = 0
+
+
+class ContactDesc(C.Structure):
+    """pdk_contact_desc: a contact model on a PD specimen."""
+    _fields_ = [("model", C.c_int32), ("reserved", C.c_int32), ("radius", C.c_double),
+                ("radius_extend", C.c_double), ("horizon", C.c_double), ("K", C.c_double)]
+
+
+def make_contact(radius, radius_extend, horizon, K, model=PDK_CONTACT_NORMAL_REPULSION):
+    c = ContactDesc()
+    c.model = model
+    c.radius = radius
+    c.radius_extend = radius_extend
+    c.horizon = horizon
+    c.K = K
+    return c
+
+
 class Material(C.Structure):
     _fields_ = [("K", C.c_double), ("G", C.c_double), ("G0", C.c_double)]
 
@@ -143,6 +162,25 @@ def materials_array(mats):
     arr = (Material * len(mats))()
     for t, (K, G, G0) in enumerate(mats):
         arr[t].K, arr[t].G, arr[t].G0 = K, G, G0
+    return arr
+
+
+class PairModel(C.Structure):
+    """pdk_pair_model: the constants of the model of one ordered type pair."""
+    _fields_ = [("c", C.c_double), ("bond_break_coeff", C.c_double),
+                ("theta_coeff", C.c_double * 2), ("s_coeff", C.c_double * 2)]
+
+
+def pair_models_array(pairs):
+    """ntypes^2 dicts (row-major over (type_i, type_j)) with keys c,
+    bond_break_coeff, theta_coeff (2), s_coeff (2) -> a ctypes array."""
+    arr = (PairModel * len(pairs))()
+    for k, p in enumerate(pairs):
+        arr[k].c = p.get("c", 0.0)
+        arr[k].bond_break_coeff = p.get("bond_break_coeff", 0.0)
+        for side in range(2):
+            arr[k].theta_coeff[side] = p.get("theta_coeff", (0.0, 0.0))[side]
+            arr[k].s_coeff[side] = p.get("s_coeff", (0.0, 0.0))[side]
     return arr
 
 

@@ -16,6 +16,7 @@ import math
 
 import numpy as np
 
+from . import _abi
 from ._abi import PDK_BC_FORCE_VALUE, make_desc
 
 
@@ -495,6 +496,184 @@ def dem_fcc(n=1000000, seed=7, model="HERTZ_JKR", dt=1.5e-7, steps=2000, rebuild
     return w
 
 
+def dem_packing_radii(n, seed, r_mean=1e-3, sigma=0.1, clip=0.3):
+    """Radii exp(N(ln r_mean, sigma^2)) clipped to [1 - clip, 1 + clip] r_mean."""
+    rng = np.random.default_rng(seed)
+    r = np.exp(rng.normal(math.log(r_mean), sigma, size=n))
+    return np.clip(r, (1.0 - clip) * r_mean, (1.0 + clip) * r_mean), rng
+
+
+def contact_overlaps(spec, y, r):
+    """Pairs (i, j) of a contact spec's list and their overlaps R_i + R_j - |y_j - y_i|."""
+    off, nbr, _ = spec.bonds()
+    i = np.repeat(np.arange(off.size - 1), np.diff(off))
+    d = np.linalg.norm(y[nbr] - y[i], axis=1)
+    return i, nbr, r[i] + r[nbr] - d
+
+
+def dem_packing_walls(L, r_wall, spacing=1.8):
+    """Container of frozen spheres for a cube [0, L]^3: on each face a square
+    sheet of radius r_wall spheres, centre spacing spacing * r_wall (the gap at
+    the centre of four is 0.27 r_wall at 1.8, so no sphere of the packing can
+    pass), centres r_wall outside the face (the frozen container particles of
+    examples/dem/powder_fill.cpp:63-90)."""
+    k = int(math.ceil(L / (spacing * r_wall))) + 1
+    g = np.linspace(0.0, L, k)
+    g1, g2 = np.meshgrid(g, g, indexing="ij")
+    sheets = []
+    for d in range(3):
+        for outer in (False, True):
+            p = np.zeros((k * k, 3))
+            o = [q for q in range(3) if q != d]
+            p[:, o[0]] = g1.ravel()
+            p[:, o[1]] = g2.ravel()
+            p[:, d] = L + r_wall if outer else -r_wall
+            sheets.append(p)
+    return np.concatenate(sheets)
+
+
+def dem_random_packing(ctx, n=1000000, fraction=0.55, seed=20261203, r_mean=1e-3, sigma=0.1, clip=0.3,
+                       stage_step=0.005, relax_steps=300, gas_steps=1500, gas_speed=20.0, E=70e9, nu=0.25,
+                       rho=2600.0, dt=1.5e-7, log=None):
+    """A seeded random packing of n spheres in a cube at the given solid
+    fraction (counted over the box that holds every sphere), radii log-normal
+    about r_mean clipped to +-clip, no two spheres overlapping.
+
+    Random sequential addition jams near 0.38, so the packing is compacted on
+    this engine, inside a container of frozen sphere sheets (dem_packing_walls,
+    held by a zero-force boundary condition):
+      1. the spheres sit on randomly chosen sites of a face-centred cubic
+         lattice whose neighbour distance exceeds twice the largest radius
+         (no overlap), with a seeded jitter;
+      2. the lattice is melted: gas_steps steps of undamped Hertz contact
+         (restitution 1) from random directions at gas_speed;
+      3. stages of affine compression of every centre and wall about the box
+         centre, each raising the container's solid fraction by stage_step,
+         with the velocities cleared and then relax_steps steps of damped
+         Hertz contact (restitution 0.1), until the spheres' own bounding box
+         holds the target fraction after the residual overlaps (a few 1e-3 of
+         a radius) are removed by one common scale of the radii.
+    Returns (positions, radii, achieved fraction, stages, wall positions,
+    wall radius)."""
+    from . import engine
+    r, rng = dem_packing_radii(n, seed, r_mean, sigma, clip)
+    rmax = float(r.max())
+    m = int(math.ceil((n / 4.0) ** (1.0 / 3.0)))
+    a = 2.0 * rmax * 1.03 * math.sqrt(2.0)  # FCC neighbour distance a / sqrt(2) = 2.06 rmax
+    basis = np.array([[0.25, 0.25, 0.25], [0.75, 0.75, 0.25], [0.75, 0.25, 0.75], [0.25, 0.75, 0.75]])
+    cz, cy, cx = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    corners = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1).astype(np.float64)
+    sites = (corners[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a
+    xi = sites[np.sort(rng.choice(sites.shape[0], size=n, replace=False))]
+    xi = xi + rng.uniform(-0.01 * a, 0.01 * a, size=xi.shape)
+    xi = xi[rng.permutation(n)]
+    L = m * a
+    xw = dem_packing_walls(L, r_mean)
+    x = np.concatenate([xi, xw])
+    rr = np.concatenate([r, np.full(xw.shape[0], r_mean)])
+    N = x.shape[0]
+    vol = 4.0 / 3.0 * math.pi * rr ** 3
+    vol_in = float(vol[:n].sum())
+    walls = np.arange(n, N)
+
+    def spec(restitution, u, v):
+        desc = make_desc("HERTZ", radius=r_mean, radius_extend=0.1 * r_mean, nu=nu, E=E, restitution=restitution,
+                         dt=dt, rebuild_every=10, polydisperse=True)
+        s = engine.Spec(ctx, desc, N)
+        s.upload(x, vol, u=u, v=v, rho=np.full(N, rho), radius=rr)
+        s.build_bonds()
+        s.set_bc(PDK_BC_FORCE_VALUE, walls, 0.0)
+        s.init()
+        return s
+
+    # 2. melt the lattice
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    v0 = np.zeros_like(x)
+    v0[:n] = gas_speed * dirs
+    s = spec(1.0, None, v0)
+    s.step(gas_steps)
+    u = s.get("u")
+    s.close()
+    if log:
+        log(f"  packing: {n} spheres, {xw.shape[0]} wall spheres, lattice melted over {gas_steps} steps")
+    # 3. compaction
+    s = spec(0.1, u, None)
+    y = x + u
+    centre = np.full(3, 0.5 * L)
+    side = L
+    stages = 0
+    final = None
+    zero_v = np.zeros_like(x)
+    while stages < 400:
+        stages += 1
+        f_box = vol_in / side ** 3
+        lam = (f_box / (f_box + stage_step)) ** (1.0 / 3.0)
+        side *= lam
+        y = centre + lam * (y - centre)
+        s.set_state(u=y - x, v=zero_v)
+        s.step(relax_steps)
+        y = x + s.get("u")
+        yi = y[:n]
+        lo = (yi - r[:, None]).min(axis=0)
+        hi = (yi + r[:, None]).max(axis=0)
+        frac = vol_in / float(np.prod(hi - lo))
+        if log and (stages % 10 == 0 or frac >= fraction):
+            log(f"  packing stage {stages}: container fraction {f_box + stage_step:.4f} bounding-box fraction "
+                f"{frac:.4f}")
+        if frac < fraction:
+            continue
+        i, j, ov = contact_overlaps(s, y, rr)
+        keep = (i < n) & (j < n)
+        worst = float(np.max(ov[keep] / (rr[i[keep]] + rr[j[keep]]), initial=0.0))
+        scale = 1.0 - worst * (1.0 + 1e-6) if worst > 0 else 1.0
+        lo = (yi - scale * r[:, None]).min(axis=0)
+        hi = (yi + scale * r[:, None]).max(axis=0)
+        frac = vol_in * scale ** 3 / float(np.prod(hi - lo))
+        if log:
+            log(f"  packing stage {stages}: worst overlap {worst:.2e} of R_i + R_j, fraction after scaling {frac:.4f}")
+        if frac >= fraction:
+            final = (yi.copy(), r * scale, frac, stages, y[n:].copy(), r_mean)
+            break
+    s.close()
+    if final is None:
+        raise RuntimeError("packing did not reach the solid fraction")
+    return final
+
+
+def dem_box(ctx, n=1000000, fraction=0.55, seed=20261203, model="HERTZ_JKR", dt=1.5e-7, steps=2000,
+            rebuild_every=20, r_mean=1e-3, E=70e9, nu=0.25, rho=2600.0, restitution=0.5, gamma=0.05,
+            extend_frac=0.1, log=None):
+    """The granular box: the n spheres of dem_random_packing at the solid
+    fraction, at rest, inside its container of frozen wall spheres (the frozen
+    container particles of examples/dem/powder_fill.cpp:63-90, held by a
+    zero-force boundary condition; particles n.. of the workload), Hertz-JKR
+    (E 70 GPa, nu 0.25, rho 2600, restitution 0.5, surface energy 0.05 J/m^2,
+    no friction), gravity f_z -= 9.8 rho on the spheres
+    (examples/dem/powder_fill.cpp:138-142), contact list rebuilt every
+    rebuild_every steps."""
+    y, r, frac, stages, yw, rw = dem_random_packing(ctx, n=n, fraction=fraction, seed=seed, r_mean=r_mean, E=E,
+                                                    nu=nu, rho=rho, log=log)
+    x = np.concatenate([y, yw])
+    rad = np.concatenate([r, np.full(yw.shape[0], rw)])
+    N = x.shape[0]
+    vol = 4.0 / 3.0 * math.pi * rad ** 3
+    desc = make_desc(model, radius=r_mean, radius_extend=extend_frac * r_mean, nu=nu, E=E,
+                     restitution=restitution, gamma=gamma, mu_friction=0.0, dt=dt, rebuild_every=rebuild_every,
+                     polydisperse=True)
+    w = _base(desc, x, vol, rho, "dem_box_%s_%d" % (model.lower(), n), 2.0 * r_mean, steps)
+    w["radius"] = rad
+    b = np.zeros((N, 3))
+    b[:n, 2] = -9.8 * rho
+    w["body"] = b
+    w["bc"] = (PDK_BC_FORCE_VALUE, np.arange(n, N, dtype=np.int64), 0.0)
+    w["spheres"] = n
+    w["wall_spheres"] = N - n
+    w["solid_fraction"] = frac
+    w["packing_stages"] = stages
+    return w
+
+
 def lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.1, K=1.0, G=0.5, influence=None,
                      fracture=False, G0=1000.0):
     """Known-answer lattice of unit_test/tstForce.hpp:593-661: cell-centred
@@ -519,6 +698,39 @@ def lattice_sum_cube(model="PMB", m=3, nc=11, field="linear", coeff=0.1, K=1.0, 
     return w
 
 
+def colliding_blocks(n_cells=(8, 8, 8), gap_cells=3.5, speed=150.0, steps=200, dx=1e-3, K=130e9, rho=7800.0,
+                     critical_stretch=0.004, contact_factor=0.9, extend_factor=0.01, precision="fp64"):
+    """Two PMB fracture blocks of n_cells each, gap_cells cell spacings apart
+    along x (more than the horizon, so no bond joins them), moving toward
+    each other at +-speed: they interact only through the contact model on
+    the PD specimen (NormalRepulsion with radius contact_factor dx / 2 and
+    list skin extend_factor dx, the contact parameters of
+    examples/mechanics/inputs/fragmenting_cylinder.json), and bonds at the
+    impact faces break.  G0 = 9 K delta s_c^2 / 5 (the PMB relation for a
+    critical stretch s_c)."""
+    nx, ny, nz = n_cells
+    lx = nx * dx
+    xa, vola, _ = grid_cell_centred([-lx - 0.5 * gap_cells * dx, 0.0, 0.0],
+                                    [-0.5 * gap_cells * dx, ny * dx, nz * dx], n_cells)
+    xb, volb, _ = grid_cell_centred([0.5 * gap_cells * dx, 0.0, 0.0],
+                                    [lx + 0.5 * gap_cells * dx, ny * dx, nz * dx], n_cells)
+    x = np.concatenate([xa, xb])
+    vol = np.concatenate([vola, volb])
+    horizon = 3.01 * dx
+    G0 = 9.0 * K * horizon * (critical_stretch * critical_stretch) / 5.0
+    from .engine import critical_timestep
+    try:
+        dt = critical_timestep(dx, dx, dx, K, rho, horizon, 0.7)
+    except Exception:
+        dt = 1.0e-7
+    desc = make_desc("PMB", fracture=True, precision=precision, horizon=horizon, K=K, G0=G0, dt=dt)
+    w = _base(desc, x, vol, rho, "blocks_%dx%dx%d" % n_cells, dx, steps)
+    w["v"][: xa.shape[0], 0] = speed
+    w["v"][xa.shape[0]:, 0] = -speed
+    w["contact"] = _abi.make_contact(contact_factor * dx / 2.0, extend_factor * dx, horizon, K)
+    return w
+
+
 def upload(spec, w):
     """Send a workload to an engine spec (or an oracle spec)."""
     spec.upload(w["x"], w["vol"], u=w["u"], v=w["v"], rho=w["rho"], nofail=w["nofail"], radius=w["radius"])
@@ -529,6 +741,8 @@ def setup(spec, w, init=True):
     spec.build_bonds()
     if w.get("materials") is not None:
         spec.set_materials(*w["materials"])
+    if w.get("pair_models") is not None:
+        spec.set_pair_models(*w["pair_models"])
     if w["notches"] is not None:
         spec.prenotch(*w["notches"])
     if w["bc"] is not None:
@@ -539,5 +753,7 @@ def setup(spec, w, init=True):
         spec.set_field("temperature", w["temperature"])
     for field, idx, a, b in w.get("prescribed", []):
         spec.set_prescribed(field, idx, a, b)
+    if w.get("contact") is not None:
+        spec.set_contact(w["contact"])
     if init:
         spec.init()

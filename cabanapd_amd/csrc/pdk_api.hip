@@ -196,7 +196,8 @@ struct pdk_spec_s
     int64_t n = 0, nslices = 0;
     hipStream_t st = nullptr;
     DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
-        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stress, mtype, splas,
+        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stab_xi2d, stress, mtype,
+        splas,
         temp, cond;
     // Prescribed values (force_update == false conditions): members (device
     // ids), a and b ([count][ncomp]); one set for U and one for TEMPERATURE.
@@ -208,8 +209,22 @@ struct pdk_spec_s
         bool has_b = false;
     };
     Prescribed pre_u, pre_temp;
+    // PD specimen with a contact model (pdk_set_contact): its contact list on
+    // current positions, the displacement at the last build and a scratch
+    // pair for the largest displacement since then
+    struct PdContact
+    {
+        bool on = false;
+        pdk_contact_desc cd{};
+        double c = 0.0;
+        DevBuf slice_ptr, cnt, col, intact, u_neigh, partial;
+        BuildScratch scratch;
+        int64_t rows_cap = 0, total_rows = 0;
+        int32_t max_cnt = 0;
+    } pdc;
     // multi-material (pdk_set_materials): materials and pair tables
     int32_t nmat = 1;
+    bool pair_tables = false; // particle types and pair tables on the device
     double mat_c[16] = {}, mat_bcc[16] = {}, mat_tc[4] = {}, mat_sc[4] = {};
     double lat_h[3] = { 0, 0, 0 }; // lattice spacing when lattice_rows
     int64_t lat_dims[3] = { 0, 0, 0 };
@@ -252,7 +267,7 @@ struct pdk_spec_s
 
     // Specs whose force passes need the FP64 gather variant: several
     // materials, EPP (per-bond state) or a temperature field.
-    bool gatherOnly() const { return nmat > 1 || d.mechanics == PDK_MECH_EPP || d.thermal != PDK_TEMP_INDEPENDENT; }
+    bool gatherOnly() const { return pair_tables || d.mechanics == PDK_MECH_EPP || d.thermal != PDK_TEMP_INDEPENDENT; }
     bool hasPrescribed() const { return pre_u.count > 0 || pre_temp.count > 0; }
     bool dynamicTemp() const { return d.thermal == PDK_TEMP_DYNAMIC; }
     // Work between the initial half step and the force pass of step g:
@@ -348,6 +363,8 @@ struct pdk_spec_s
         h.u_neigh = u_neigh ? u_neigh.as<double>() : nullptr;
         h.sid = sid ? sid.as<uint8_t>() : nullptr;
         h.stencil = stab ? stab.as<float4>() : nullptr;
+        h.stencil_xi2d = stab_xi2d ? stab_xi2d.as<double>() : nullptr;
+        h.stencil_xiv = nullptr;
         h.exist = nullptr;
         h.soff = nullptr;
         h.stencil_slots = 0;
@@ -355,6 +372,8 @@ struct pdk_spec_s
         {
             h.sid = nullptr;
             h.stencil = slot.table.as<float4>();
+            h.stencil_xi2d = slot.xi2d.as<double>();
+            h.stencil_xiv = slot.xiv.as<double>();
             h.exist = slot.exist.as<uint64_t>();
             h.soff = slot.soff.as<int32_t>();
             h.stencil_slots = slot.slots;
@@ -388,7 +407,7 @@ struct pdk_spec_s
         h.thermal_horizon = d.horizon;
         h.lin_micro = d.linear_microconductivity;
         h.nmat = nmat;
-        h.mtype = nmat > 1 ? mtype.as<uint8_t>() : nullptr;
+        h.mtype = pair_tables ? mtype.as<uint8_t>() : nullptr;
         for ( int t = 0; t < 16; t++ )
         {
             h.mat_c[t] = mat_c[t];
@@ -407,9 +426,16 @@ struct pdk_spec_s
         h.dt = d.dt;
         h.half_dt = 0.5 * d.dt;
         h.s0f = static_cast<float>( k.s0 );
-        // FP32 stretch error is a few ulp of (|eta|/|xi| + |s|); 2^-13 leaves
-        // a margin of ~2^8 over that bound.
+        // The mixed stretch comes from an FP64 numerator |r|^2 - |xi|^2 and a
+        // few FP32 operations: its error is ~10 ulp of |s| (2^-20 |s|).  A
+        // bond within 2^-12 s0 (+ 2^-40 for s0 = 0) of s0 is re-tested in
+        // FP64 -- a margin of 2^8 over that error where it matters (|s| near
+        // s0; far from s0 the FP32 decision cannot be wrong).  The gather
+        // passes scale bandf = 2^-13 by the bond's own terms (thermal strain).
         h.bandf = 1.220703125e-4f;
+        h.bandK = static_cast<float>( 2.44140625e-4 * k.s0 + 9.094947017729282e-13 );
+        // uniform volume: c vol (PMB) or vol (LPS) scales each FP32 row sum
+        h.row_scale = isLpsModel( d.model ) ? vol_value : k.c * vol_value;
         h.Rs = k.Rs;
         h.Es = k.Es;
         h.coeff_h_n = k.coeff_h_n;
@@ -417,6 +443,7 @@ struct pdk_spec_s
         h.Gamma = k.Gamma;
         h.a0 = k.a0;
         h.delta_tear = k.delta_tear;
+        h.jkr_c6 = std::pow( 6.0, -1.0 / 3.0 );
         h.two_r = 2.0 * d.radius;
         h.mu_f = d.mu_friction;
         h.nu = d.nu;
@@ -435,7 +462,7 @@ struct pdk_spec_s
             if ( !slot.aux )
             {
                 slot.aux.alloc( static_cast<size_t>( slot.slots ) * sizeof( float4 ), acct() );
-                launchStencilAux( slot.table.as<float4>(), slot.slots, h.bandf, slot.aux.as<float4>(), st );
+                launchStencilAux( slot.table.as<float4>(), slot.slots, d.influence, slot.aux.as<float4>(), st );
             }
             h.stencil_aux = slot.aux.as<float4>();
         }
@@ -554,6 +581,30 @@ struct pdk_spec_s
         if ( !lps() )
             return;
         timed( kClsLps, [&] { launchLpsState( dargs(), h, wv, dil, cur, st ); } );
+    }
+
+    // Contact list of a PD specimen on the current positions x + U: cutoff
+    // 2 radius + radius_extend (+ tol at the first build), neighbors in
+    // ascending caller id; u_neigh = U.
+    void pdContactBuild( double tol )
+    {
+        NeighborQuery q{};
+        q.x = x.as<double>();
+        q.u = Ucur();
+        q.radius = nullptr;
+        q.rc = 2.0 * pdc.cd.radius + pdc.cd.radius_extend + tol;
+        q.extend = pdc.cd.radius_extend;
+        q.n = n;
+        pdc.total_rows = buildSlicedCsr( q, perm.as<int64_t>(), inv.as<int64_t>(), pdc.slice_ptr.as<int64_t>(),
+                                         pdc.cnt.as<int32_t>(), pdc.col, pdc.intact, pdc.rows_cap, pdc.scratch, acct(),
+                                         st, &pdc.max_cnt );
+        PDK_HIP( hipMemcpyAsync( pdc.u_neigh.as<double>(), Ucur(), 3 * n * sizeof( double ), hipMemcpyDeviceToDevice,
+                                 st ) );
+    }
+    PdContactArgs pdContactArgs() const
+    {
+        return PdContactArgs{ pdc.slice_ptr.as<int64_t>(), pdc.cnt.as<int32_t>(), pdc.col.as<int32_t>(),
+                              pdc.cd.radius, pdc.cd.horizon, pdc.c };
     }
 
     // Solver::updateForce (CabanaPD_Solver.hpp:462-477) without the epilogue.
@@ -938,7 +989,8 @@ int pdk_spec_copy_state( pdk_spec dst, pdk_spec src )
         require( dst->ctx == src->ctx, "specs belong to different contexts" );
         require( src->inited && dst->built, "source not initialised or destination bonds not built" );
         require( !src->in_batch && !dst->in_batch, "spec is running in the batch engine" );
-        require( !isContact( src->d ) && !isContact( dst->d ), "contact specs have no bond state to copy" );
+        require( !isContact( src->d ) && !isContact( dst->d ) && !src->pdc.on && !dst->pdc.on,
+                 "specs with contact lists cannot be copied" );
         require( dst->n == src->n && dst->h_perm == src->h_perm, "specs differ in particle count or device order" );
         require( dst->d.model == src->d.model && dst->d.fracture == src->d.fracture &&
                      dst->d.mechanics == src->d.mechanics && dst->d.thermal == src->d.thermal,
@@ -981,7 +1033,7 @@ int pdk_spec_bytes( pdk_spec s, uint64_t* bytes )
         for ( const DevBuf* b : { &s->x, &s->U0, &s->U1, &s->v, &s->f, &s->vol, &s->rho, &s->nofail, &s->bcflag,
                                   &s->body, &s->m, &s->theta, &s->lpsA, &s->lpsB, &s->W, &s->phi, &s->perm, &s->inv,
                                   &s->cnt, &s->slice_ptr, &s->col, &s->intact, &s->radius, &s->u_neigh,
-                                  &s->partial, &s->args, &s->sid, &s->stab } )
+                                  &s->partial, &s->args, &s->sid, &s->stab, &s->stab_xi2d } )
             t += b->bytes();
         *bytes = t;
     } );
@@ -1007,6 +1059,7 @@ int pdk_build_bonds( pdk_spec s )
             q.n = s->n;
             s->sid.release();
             s->stab.release();
+            s->stab_xi2d.release();
             s->nsten = 0;
             s->slot = SlotForm{};
             // Full lattice: slot q of every row is stencil entry q, so the
@@ -1044,8 +1097,8 @@ int pdk_build_bonds( pdk_spec s )
             if ( s->slot.slots == 0 && s->d.precision == PDK_MIXED && s->lattice_rows )
             {
                 s->pushArgs();
-                if ( !buildStencil( s->h, s->lat_h, q.rc, s->total_rows, s->sid, s->stab, s->scratch, s->acct(),
-                                    s->st, &s->nsten ) )
+                if ( !buildStencil( s->h, s->lat_h, q.rc, s->total_rows, s->sid, s->stab, s->stab_xi2d, s->scratch,
+                                    s->acct(), s->st, &s->nsten ) )
                     s->nsten = 0;
             }
         }
@@ -1345,21 +1398,112 @@ int pdk_compute_energy( pdk_spec s, double* total_strain_energy, double* total_d
     } );
 }
 
+int pdk_set_contact( pdk_spec s, const pdk_contact_desc* cd )
+{
+    return guarded( [&] {
+        require( s && cd, "null argument" );
+        require( !isContact( s->d ), "pdk_set_contact takes a PD specimen (DEM specimens are contact models)" );
+        require( s->built, "build the bonds before pdk_set_contact" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( cd->model == PDK_CONTACT_NORMAL_REPULSION, "unknown contact model" );
+        require( cd->radius > 0.0 && cd->radius_extend >= 0.0 && cd->horizon > 0.0, "invalid contact parameters" );
+        s->pdc.on = true;
+        s->pdc.cd = *cd;
+        // NormalRepulsionModel ctor (force_models/CabanaPD_Contact.hpp:70-80)
+        s->pdc.c = 18.0 * cd->K / ( kPi * cd->horizon * cd->horizon * cd->horizon * cd->horizon );
+        if ( !s->pdc.slice_ptr )
+        {
+            s->pdc.slice_ptr.alloc( ( s->nslices + 1 ) * sizeof( int64_t ), s->acct() );
+            s->pdc.cnt.alloc( std::max<int64_t>( s->n, 1 ) * sizeof( int32_t ), s->acct() );
+            s->pdc.u_neigh.alloc( 3 * std::max<int64_t>( s->n, 1 ) * sizeof( double ), s->acct() );
+            s->pdc.partial.alloc( ( maxDisplacementBlocks( s->n ) + 1 ) * sizeof( double ), s->acct() );
+        }
+        s->dropGraphs();
+        // Neighbor ctor with a contact model: current positions, cutoff() + tol
+        s->timed( kClsBuild, [&] { s->pdContactBuild( s->d.neighbor_tol ); } );
+        PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_contact_pairs( pdk_spec s, int64_t* pairs )
+{
+    return guarded( [&] {
+        require( s && pairs, "null argument" );
+        int64_t t = 0;
+        if ( s->pdc.on )
+        {
+            std::vector<int32_t> c( s->n );
+            PDK_HIP( hipMemcpyAsync( c.data(), s->pdc.cnt.as<int32_t>(), s->n * sizeof( int32_t ),
+                                     hipMemcpyDeviceToHost, s->st ) );
+            PDK_HIP( hipStreamSynchronize( s->st ) );
+            for ( int32_t v : c )
+                t += v;
+        }
+        *pairs = t;
+    } );
+}
+
+namespace
+{
+void requireTableSpec( pdk_spec s, int32_t ntypes, const int32_t* type )
+{
+    require( s->uploaded, "call pdk_spec_upload first" );
+    require( !s->in_batch, "spec is running in the batch engine" );
+    require( !s->inited, "set the materials before pdk_init" );
+    require( ntypes >= 1 && ntypes <= 4, "1 to 4 material types" );
+    require( !isContact( s->d ), "the contact models have no materials" );
+    require( ntypes == 1 || s->d.precision == PDK_FP64, "multi-material specimens are FP64 only" );
+    require( ntypes == 1 || type != nullptr, "type[] is required for several materials" );
+}
+
+// Install the pair tables: c and bond_break_coeff per ordered type pair, the
+// LPS coefficients per type (the i-term coefficients of pair (a, b) are those
+// of type a, the j-term ones those of type b), and the particle types.
+// `tables` false (one type, given as materials) keeps the spec's single model.
+void installPairTables( pdk_spec s, int32_t ntypes, const pdk_pair_model* pairs, const int32_t* type, bool tables )
+{
+    for ( int32_t a = 0; a < 4; a++ )
+        for ( int32_t b = 0; b < 4; b++ )
+        {
+            const bool in = a < ntypes && b < ntypes;
+            s->mat_c[4 * a + b] = in ? pairs[a * ntypes + b].c : 0.0;
+            s->mat_bcc[4 * a + b] = in ? pairs[a * ntypes + b].bond_break_coeff : 0.0;
+        }
+    for ( int32_t t = 0; t < 4; t++ )
+    {
+        s->mat_tc[t] = t < ntypes ? pairs[t * ntypes + t].theta_coeff[0] : 0.0;
+        s->mat_sc[t] = t < ntypes ? pairs[t * ntypes + t].s_coeff[0] : 0.0;
+    }
+    if ( tables )
+    {
+        std::vector<uint8_t> dev( s->n );
+        for ( int64_t cidx = 0; cidx < s->n; cidx++ )
+        {
+            const int32_t t = type ? type[cidx] : 0;
+            require( t >= 0 && t < ntypes, "material type out of range" );
+            dev[s->h_inv[cidx]] = static_cast<uint8_t>( t );
+        }
+        if ( s->mtype.bytes() < static_cast<size_t>( s->n ) )
+            s->mtype.alloc( std::max<int64_t>( s->n, 1 ), s->acct() );
+        PDK_HIP( hipMemcpyAsync( s->mtype.as<uint8_t>(), dev.data(), s->n, hipMemcpyHostToDevice, s->st ) );
+    }
+    s->nmat = ntypes;
+    s->pair_tables = tables;
+    s->pushArgs();
+    PDK_HIP( hipStreamSynchronize( s->st ) );
+}
+} // namespace
+
 int pdk_set_materials( pdk_spec s, int32_t nmat, const pdk_material* mats, const int32_t* type )
 {
     return guarded( [&] {
         require( s && mats, "null argument" );
-        require( s->uploaded, "call pdk_spec_upload first" );
-        require( !s->in_batch, "spec is running in the batch engine" );
-        require( !s->inited, "set the materials before pdk_init" );
-        require( nmat >= 1 && nmat <= 4, "1 to 4 materials" );
-        require( !isContact( s->d ), "the contact models have no materials" );
-        require( nmat == 1 || s->d.precision == PDK_FP64, "multi-material specimens are FP64 only" );
-        require( nmat == 1 || type != nullptr, "type[] is required for several materials" );
-        // createMultiForceModel( ..., AverageTag, m_0, .. ) (CabanaPD_ForceModelsMulti.hpp:338-516):
-        // pair (a, a) -> material a's model; a != b -> the averaged model
-        // (force_models/CabanaPD_PMB.hpp:51-55, CabanaPD_ForceModels.hpp:114-121).
-        double K[4], c[4], s0[4], bcc[4];
+        requireTableSpec( s, nmat, type );
+        // createMultiForceModel( ..., AverageTag, m_0, .. ) (CabanaPD_ForceModelsMulti.hpp:360-502):
+        // pair (a, a) -> material a's model; a != b -> the model averaged from a's
+        // and b's (force_models/CabanaPD_PMB.hpp:51-55, force_models/CabanaPD_LPS.hpp:55-69,
+        // CabanaPD_ForceModels.hpp:114-121).
+        double K[4], c[4], s0[4], bcc[4], tc[4], sc[4];
         for ( int32_t t = 0; t < nmat; t++ )
         {
             pdk_model_desc d = s->d;
@@ -1372,39 +1516,62 @@ int pdk_set_materials( pdk_spec s, int32_t nmat, const pdk_material* mats, const
             c[t] = k.c;
             s0[t] = k.s0;
             bcc[t] = k.bond_break_coeff;
-            s->mat_tc[t] = k.theta_coeff;
-            s->mat_sc[t] = k.s_coeff;
+            tc[t] = k.theta_coeff;
+            sc[t] = k.s_coeff;
         }
+        pdk_pair_model pairs[16];
         for ( int32_t a = 0; a < nmat; a++ )
             for ( int32_t b = 0; b < nmat; b++ )
             {
+                pdk_pair_model& p = pairs[a * nmat + b];
                 if ( a == b )
                 {
-                    s->mat_c[4 * a + b] = c[a];
-                    s->mat_bcc[4 * a + b] = bcc[a];
+                    p.c = c[a];
+                    p.bond_break_coeff = bcc[a];
                 }
                 else
                 {
-                    s->mat_c[4 * a + b] = ( c[a] + c[b] ) / 2.0;
+                    p.c = ( c[a] + c[b] ) / 2.0;
                     const double s0ab = std::sqrt( ( s0[a] * s0[a] * K[a] + s0[b] * s0[b] * K[b] ) / ( K[a] + K[b] ) );
-                    s->mat_bcc[4 * a + b] = ( 1.0 + s0ab ) * ( 1.0 + s0ab );
+                    p.bond_break_coeff = ( 1.0 + s0ab ) * ( 1.0 + s0ab );
+                }
+                p.theta_coeff[0] = tc[a];
+                p.theta_coeff[1] = tc[b];
+                p.s_coeff[0] = sc[a];
+                p.s_coeff[1] = sc[b];
+            }
+        installPairTables( s, nmat, pairs, type, nmat > 1 );
+    } );
+}
+
+int pdk_set_pair_models( pdk_spec s, int32_t ntypes, const pdk_pair_model* pairs, const int32_t* type )
+{
+    return guarded( [&] {
+        require( s && pairs, "null argument" );
+        requireTableSpec( s, ntypes, type );
+        require( s->d.precision == PDK_FP64, "multi-material specimens are FP64 only" );
+        const bool lps = s->d.model == PDK_MODEL_LPS || s->d.model == PDK_MODEL_LINEAR_LPS;
+        for ( int32_t a = 0; a < ntypes; a++ )
+            for ( int32_t b = 0; b < ntypes; b++ )
+            {
+                const pdk_pair_model& p = pairs[a * ntypes + b];
+                require( std::isfinite( p.c ) && std::isfinite( p.theta_coeff[0] ) &&
+                             std::isfinite( p.theta_coeff[1] ) && std::isfinite( p.s_coeff[0] ) &&
+                             std::isfinite( p.s_coeff[1] ),
+                         "pair model constants must be finite" );
+                require( !s->d.fracture || p.bond_break_coeff > 0.0, "bond_break_coeff must be positive" );
+                if ( lps )
+                {
+                    // per-particle folding of the LPS terms: [0] from type a, [1] from type b
+                    const pdk_pair_model& ia = pairs[a * ntypes + a];
+                    const pdk_pair_model& ib = pairs[b * ntypes + b];
+                    require( p.theta_coeff[0] == ia.theta_coeff[0] && p.s_coeff[0] == ia.s_coeff[0] &&
+                                 p.theta_coeff[1] == ib.theta_coeff[0] && p.s_coeff[1] == ib.s_coeff[0] &&
+                                 ia.theta_coeff[0] == ia.theta_coeff[1] && ia.s_coeff[0] == ia.s_coeff[1],
+                             "LPS pair models must take each particle's theta_coeff and s_coeff from its own type" );
                 }
             }
-        if ( nmat > 1 )
-        {
-            std::vector<uint8_t> dev( s->n );
-            for ( int64_t cidx = 0; cidx < s->n; cidx++ )
-            {
-                require( type[cidx] >= 0 && type[cidx] < nmat, "material type out of range" );
-                dev[s->h_inv[cidx]] = static_cast<uint8_t>( type[cidx] );
-            }
-            if ( s->mtype.bytes() < static_cast<size_t>( s->n ) )
-                s->mtype.alloc( std::max<int64_t>( s->n, 1 ), s->acct() );
-            PDK_HIP( hipMemcpyAsync( s->mtype.as<uint8_t>(), dev.data(), s->n, hipMemcpyHostToDevice, s->st ) );
-        }
-        s->nmat = nmat;
-        s->pushArgs();
-        PDK_HIP( hipStreamSynchronize( s->st ) );
+        installPairTables( s, ntypes, pairs, type, true );
     } );
 }
 
@@ -1476,6 +1643,36 @@ int pdk_step( pdk_spec s, int32_t nsteps )
         if ( isContact( s->d ) )
         {
             contactStep( s, nsteps );
+        }
+        else if ( s->pdc.on )
+        {
+            // Solver::runStep with a contact model (CabanaPD_Solver.hpp:332-404):
+            // initial half step and the largest displacement since the last
+            // contact list build, heat transfer / prescribed values, the PD
+            // force, the contact force (rebuilding the list first when that
+            // displacement exceeds radius_extend, Neighbor::update), the force
+            // terms and the final half step.
+            double* mx_dev = s->pdc.partial.as<double>() + maxDisplacementBlocks( s->n );
+            for ( int32_t k = 1; k <= nsteps; k++ )
+            {
+                s->timed( kClsOther, [&] { launchInitialHalfStep( s->dargs(), s->h, s->cur, s->st ); } );
+                s->cur ^= 1;
+                launchMaxDisplacement( s->Ucur(), s->pdc.u_neigh.as<double>(), s->n, s->pdc.partial.as<double>(),
+                                       mx_dev, s->st );
+                double mx = 0.0;
+                PDK_HIP( hipMemcpyAsync( &mx, mx_dev, sizeof( double ), hipMemcpyDeviceToHost, s->st ) );
+                PDK_HIP( hipStreamSynchronize( s->st ) );
+                if ( s->hasPreForce() )
+                    s->preForce( s->steps_done + k );
+                if ( mx > s->pdc.cd.radius_extend )
+                    s->timed( kClsBuild, [&] { s->pdContactBuild( 0.0 ); } );
+                if ( s->lps() )
+                    s->lpsState( s->d.fracture != 0, true );
+                s->timed( kClsForce, [&] { launchForce( s->dargs(), s->h, kEpiStoreOnly, s->cur, s->st ); } );
+                s->timed( kClsOther, [&] {
+                    launchPdContact( s->dargs(), s->h, s->pdContactArgs(), kEpiFinal, s->cur, s->st );
+                } );
+            }
         }
         else
         {
@@ -1955,6 +2152,7 @@ extern "C" {
 int pdk_submit( pdk_ctx ctx, pdk_spec s, int32_t nsteps, uint64_t* ticket )
 {
     return guarded( [&] {
+        require( s && !s->pdc.on, "specs with a contact model are stepped with pdk_step" );
         require( ctx && s && ticket, "null argument" );
         require( s->ctx == ctx, "spec belongs to another context" );
         require( !s->gatherOnly(),

@@ -833,7 +833,7 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
 }
 
 bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_rows, DevBuf& sid, DevBuf& table,
-                   BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out )
+                   DevBuf& xi2d, BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out )
 {
     int32_t K[3];
     for ( int d = 0; d < 3; d++ )
@@ -847,6 +847,7 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
     const int32_t D[3] = { 2 * K[0] + 1, 2 * K[1] + 1, 2 * K[2] + 1 };
     std::vector<int16_t> dense( static_cast<size_t>( D[0] ) * D[1] * D[2], -1 );
     std::vector<float4> tab;
+    std::vector<double> tab2;
     const double rc2 = rc * rc * ( 1.0 + 1e-9 );
     for ( int32_t c = -K[2]; c <= K[2]; c++ )
         for ( int32_t b = -K[1]; b <= K[1]; b++ )
@@ -861,9 +862,13 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
                 dense[( ( c + K[2] ) * D[1] + ( b + K[1] ) ) * D[0] + ( aa + K[0] )] = static_cast<int16_t>( tab.size() );
                 tab.push_back( make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
                                             static_cast<float>( std::sqrt( r2 ) ) ) );
+                tab2.push_back( r2 );
             }
     while ( tab.size() < 256 )
+    {
         tab.push_back( make_float4( 0.f, 0.f, 0.f, 1.f ) );
+        tab2.push_back( 1.0 );
+    }
     DevBuf ddense;
     ddense.alloc( dense.size() * sizeof( int16_t ), acct );
     ensure<int32_t>( s.flag, 1, acct );
@@ -872,6 +877,8 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
     sid.alloc( static_cast<size_t>( std::max<int64_t>( total_rows, 1 ) ) * kSlice, acct );
     table.alloc( 256 * sizeof( float4 ), acct );
     PDK_HIP( hipMemcpyAsync( table.as<float4>(), tab.data(), 256 * sizeof( float4 ), hipMemcpyHostToDevice, st ) );
+    xi2d.alloc( 256 * sizeof( double ), acct );
+    PDK_HIP( hipMemcpyAsync( xi2d.as<double>(), tab2.data(), 256 * sizeof( double ), hipMemcpyHostToDevice, st ) );
     const unsigned g = static_cast<unsigned>( ( a.nslices * kSlice + kThreads - 1 ) / kThreads );
     kStencilIds<<<g, kThreads, 0, st>>>( a, h[0], h[1], h[2], K[0], K[1], K[2], ddense.as<int16_t>(), sid.as<uint8_t>(),
                                          s.flag.as<int32_t>() );
@@ -883,6 +890,7 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
     {
         sid.release();
         table.release();
+        xi2d.release();
         return false;
     }
     if ( nsten_out )
@@ -973,8 +981,8 @@ std::vector<int4> slotEntries( const std::vector<SlotEntry>& es )
     return ent;
 }
 
-// Fill the slot arrays, the row table, offsets and FP32 stencil table and
-// move them into place.
+// Fill the slot arrays, the row table, offsets, the FP32 stencil table and
+// the FP64 |xi|^2 of each entry, and move them into place.
 void finishSlots( int64_t n, int64_t nslices, const LatticeBox& L, const std::vector<SlotEntry>& es,
                   const int4* dent, const double h[3], DevBuf& col, DevBuf& intact, int64_t* slice_ptr_dev,
                   int64_t& total_rows, int64_t& col_capacity_rows, SlotForm& out, MemAccount* acct,
@@ -1028,23 +1036,39 @@ void finishSlots( int64_t n, int64_t nslices, const LatticeBox& L, const std::ve
     }
     std::vector<int32_t> soff( 256, 0 );
     std::vector<float4> tab( 256, make_float4( 0.f, 0.f, 0.f, 1.f ) );
+    std::vector<double> tab2( 256, 1.0 );
+    std::vector<double> tabv( 4 * 256, 0.0 ); // (xi_x, xi_y, xi_z, |xi|^2) in FP64 per entry
     for ( int32_t e = 0; e < S; e++ )
     {
         soff[e] = static_cast<int32_t>( es[e].off );
         const double gx = es[e].a * h[0], gy = es[e].b * h[1], gz = es[e].c * h[2];
         tab[e] = make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
                               static_cast<float>( std::sqrt( es[e].r2 ) ) );
+        tab2[e] = gx * gx + gy * gy + gz * gz;
+        tabv[4 * e + 0] = gx;
+        tabv[4 * e + 1] = gy;
+        tabv[4 * e + 2] = gz;
+        tabv[4 * e + 3] = tab2[e];
     }
+    DevBuf dxiv;
+    dxiv.alloc( tabv.size() * sizeof( double ), acct );
+    PDK_HIP( hipMemcpyAsync( dxiv.as<double>(), tabv.data(), tabv.size() * sizeof( double ), hipMemcpyHostToDevice,
+                             st ) );
+    DevBuf dxi2;
     dsoff.alloc( 256 * sizeof( int32_t ), acct );
     dtab.alloc( 256 * sizeof( float4 ), acct );
+    dxi2.alloc( 256 * sizeof( double ), acct );
     PDK_HIP( hipMemcpyAsync( dsoff.as<int32_t>(), soff.data(), 256 * sizeof( int32_t ), hipMemcpyHostToDevice, st ) );
     PDK_HIP( hipMemcpyAsync( dtab.as<float4>(), tab.data(), 256 * sizeof( float4 ), hipMemcpyHostToDevice, st ) );
+    PDK_HIP( hipMemcpyAsync( dxi2.as<double>(), tab2.data(), 256 * sizeof( double ), hipMemcpyHostToDevice, st ) );
     PDK_HIP( hipStreamSynchronize( st ) );
     col = std::move( ncol );
     intact = std::move( nintact );
     out.exist = std::move( nexist );
     out.soff = std::move( dsoff );
     out.table = std::move( dtab );
+    out.xi2d = std::move( dxi2 );
+    out.xiv = std::move( dxiv );
     out.rows = std::move( drows );
     out.rowstart = std::move( drstart );
     out.nrows = nrows;

@@ -41,7 +41,8 @@ enum EpilogueMode : int32_t
 {
     kEpiStoreOnly = 0,   // computeForce: store f (after BC/body terms)
     kEpiFinal = 1,       // final half step (v += dt/2 f/rho), store f
-    kEpiFinalInitial = 2 // final half step of this step + initial half step of the next
+    kEpiFinalInitial = 2, // final half step of this step + initial half step of the next
+    kEpiApplyOnly = 3     // store f after BC/body terms; the final half step runs as its own pass
 };
 
 // Everything a PD/contact kernel needs for one specimen.  Lives in device
@@ -71,7 +72,13 @@ struct PDArgs
     // Null in the compact form, where slot q holds a bond iff q < cnt[p].
     const uint64_t* exist;
     const int32_t* soff;
-    const float4* stencil_aux; // per stencil entry (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0) in FP32, mixed form
+    // Mixed form, per stencil entry in FP32: (1/|xi|, |xi|^2, w, w2) with the
+    // LPS influence products w = infl |xi| (1 for influence 1, |xi| for
+    // influence 0; the force coefficient's factor) and w2 = infl |xi|^2 (the
+    // weighted-volume and dilatation factor)
+    const float4* stencil_aux;
+    const double* stencil_xi2d; // |xi|^2 of each stencil entry in FP64 (exact lattice offsets)
+    const double* stencil_xiv;  // stencil-slot form: (xi_x, xi_y, xi_z, |xi|^2) of each entry in FP64
     double* m;
     double* theta;
     double* lpsA;           // LPS: theta_coeff * theta / m per particle
@@ -87,10 +94,15 @@ struct PDArgs
     double dt, half_dt;
     double bc_value;
     // mixed-precision guard band: a bond whose FP32 stretch lies within
-    // [s0 - band, s0 + band] is re-tested in FP64
-    float s0f, bandf;
+    // [s0 - bandK, s0 + bandK] is re-tested in FP64 (bandf: the relative
+    // width the gather passes scale by the bond's own terms)
+    float s0f, bandf, bandK;
+    // mixed form, uniform volume: the per-bond coefficient factor applied to
+    // each FP32 row sum in FP64 (PMB: c vol, LPS: vol)
+    double row_scale;
     // contact constants
     double Rs, Es, coeff_h_n, coeff_h_d, Gamma, a0, delta_tear, two_r, mu_f, nu, E;
+    double jkr_c6; // pow(6, -1/3) (host libm): JKR tear-off of polydisperse pairs
     int32_t model;
     int32_t fracture;
     int32_t influence;

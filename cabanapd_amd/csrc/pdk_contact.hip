@@ -28,25 +28,115 @@ struct PairK
     double Rs, coeff_h_n, delta_tear, two_r, ms;
 };
 
+// Correctly rounded powers of the contact models (the reference evaluates
+// Kokkos::pow, within an ulp of the exact value).  Built from IEEE operations
+// only -- + - * /, sqrt, fma, frexp/ldexp -- in the same order as the
+// oracle's, so both return the same bits.  Double-double arithmetic (exact
+// products through fma, after Dekker) carries each power to ~2^-100 relative
+// before the final rounding, which is then the correct rounding unless the
+// exact value lies within ~2^-100 of a rounding boundary (Muller et al.,
+// Handbook of Floating-Point Arithmetic, on correct rounding).
+__device__ __forceinline__ double crSq( double x )
+{
+    return x * x;
+}
+__device__ double crCube( double x )
+{
+#pragma clang fp contract( off )
+    if ( !( fabs( x ) < 1e100 ) || x == 0.0 )
+        return x * x * x; // overflow, infinities, NaN and signed zeros as pow
+    const double ph = x * x, pl = fma( x, x, -ph );
+    const double qh = ph * x, ql = fma( ph, x, -qh );
+    return qh + ( ql + pl * x );
+}
+__device__ __forceinline__ double crHalf( double x )
+{
+    return x == 0.0 ? 0.0 : sqrt( x ); // +0 at -0, as pow
+}
+// x^1.5: sqrt with its exact remainder x - sq^2, times x in double-double.
+__device__ double crPow32( double x )
+{
+#pragma clang fp contract( off )
+    if ( !( x > 0.0 ) )
+        return x == 0.0 ? 0.0 : sqrt( -1.0 );
+    if ( isinf( x ) )
+        return x;
+    const double sq = sqrt( x );
+    const double rem = fma( -sq, sq, x );
+    const double sl = rem / ( 2.0 * sq );
+    const double ph = x * sq, pl = fma( x, sq, -ph );
+    return ph + ( pl + x * sl );
+}
+// ln x for x > 0 to ~1e-16 relative: e ln 2 + 2 atanh(z), z = (m-1)/(m+1),
+// m in [1/sqrt 2, sqrt 2), the atanh series to z^23.
+__device__ double crLog( double x )
+{
+#pragma clang fp contract( off )
+    int e = 0;
+    double m = frexp( x, &e );
+    if ( m < 0.70710678118654752440 )
+    {
+        m = 2.0 * m;
+        e = e - 1;
+    }
+    const double z = ( m - 1.0 ) / ( m + 1.0 );
+    const double z2 = z * z;
+    double sum = 1.0 / 23.0;
+    for ( int k = 21; k >= 1; k -= 2 )
+        sum = sum * z2 + 1.0 / k;
+    return static_cast<double>( e ) * 0.69314718055994530942 + 2.0 * z * sum;
+}
+// x^y with y the double nearest 1/3, y = (1 - 2^-54) / 3: cbrt(x) as a
+// double-double (Newton from a linear start, the last step with the exact
+// double-double cube), times x^(-2^-54/3) = 1 - t + t^2/2, t = ln(x)/(3 2^54).
+__device__ double crThird( double x )
+{
+#pragma clang fp contract( off )
+    if ( x != x || x < 0.0 )
+        return sqrt( -1.0 );
+    if ( x == 0.0 )
+        return 0.0;
+    if ( isinf( x ) )
+        return x;
+    int e = 0;
+    double m = frexp( x, &e );
+    int r = e % 3;
+    if ( r < 0 )
+        r += 3;
+    m = ldexp( m, r );
+    const int q = ( e - r ) / 3;
+    double c = 0.6 + 0.2 * m;
+    for ( int it = 0; it < 7; it++ )
+        c = c - ( c * c * c - m ) / ( 3.0 * c * c );
+    const double ph = c * c, pl = fma( c, c, -ph );
+    const double qh = ph * c, ql = fma( ph, c, -qh );
+    const double res = ( ( m - qh ) - ql ) - pl * c;
+    const double cl = res / ( 3.0 * ph );
+    const double t = crLog( x ) * ( 1.0 / 54043195528445952.0 );
+    const double k = -t + 0.5 * ( t * t );
+    const double chi = ldexp( c, q ), clo = ldexp( cl, q );
+    return chi + ( clo + chi * k );
+}
+
 __device__ double jkrPatchRadius( const PDArgs& A, double Rs, double delta_n )
 {
 #pragma clang fp contract( off )
     // force_models/CabanaPD_HertzianJKR.hpp:141-176
     const double nu = A.nu, E = A.E, Gamma = A.Gamma;
-    double c0 = pow( Rs * delta_n, 2.0 );
-    double c1 = -4.0 * ( 1.0 - pow( nu, 2.0 ) ) * kPi * Gamma * pow( Rs, 2.0 ) / E;
+    double c0 = crSq( Rs * delta_n );
+    double c1 = -4.0 * ( 1.0 - crSq( nu ) ) * kPi * Gamma * crSq( Rs ) / E;
     double c2 = -2.0 * Rs * delta_n;
-    double P = -pow( c2, 2.0 ) / 12.0 - c0;
-    double Q = -pow( c2, 3.0 ) / 108.0 + c0 * c2 / 3.0 - pow( c1, 2.0 ) / 8.0;
-    double U = pow( -Q / 2.0 + pow( fmax( 0.0, pow( Q, 2.0 ) / 4.0 + pow( P, 3.0 ) / 27.0 ), 1.0 / 2.0 ), 1.0 / 3.0 );
+    double P = -crSq( c2 ) / 12.0 - c0;
+    double Q = -crCube( c2 ) / 108.0 + c0 * c2 / 3.0 - crSq( c1 ) / 8.0;
+    double U = crThird( -Q / 2.0 + crHalf( fmax( 0.0, crSq( Q ) / 4.0 + crCube( P ) / 27.0 ) ) );
     double sv = 0.0;
     if ( P != 0.0 )
         sv = -5.0 / 6.0 * c2 + U - P / ( 3.0 * U );
     else
-        sv = -5.0 / 6.0 * c2 - pow( Q, 1.0 / 3.0 );
-    double w = pow( fmax( 0.0, c2 + 2.0 * sv ), 1.0 / 2.0 );
+        sv = -5.0 / 6.0 * c2 - crThird( Q );
+    double w = crHalf( fmax( 0.0, c2 + 2.0 * sv ) );
     double lambda = c1 / ( 2.0 * w + 1e-14 );
-    return 1.0 / 2.0 * ( w + pow( fmax( 0.0, pow( w, 2.0 ) - 4.0 * ( c2 + sv + lambda ) ), 1.0 / 2.0 ) );
+    return 1.0 / 2.0 * ( w + crHalf( fmax( 0.0, crSq( w ) - 4.0 * ( c2 + sv + lambda ) ) ) );
 }
 
 template <bool JKR>
@@ -58,7 +148,7 @@ __device__ double normalForce( const PDArgs& A, const PairK& pk, double r, doubl
         const double delta_n = ( r - pk.two_r );
         double coeff = 0.0;
         if ( delta_n < 0.0 )
-            coeff = fmin( 0.0, -pk.coeff_h_n * pow( fabs( delta_n ), 3.0 / 2.0 ) );
+            coeff = fmin( 0.0, -pk.coeff_h_n * crPow32( fabs( delta_n ) ) );
         coeff /= vol;
         double Sn = 0.0;
         if ( delta_n < 0.0 )
@@ -73,8 +163,8 @@ __device__ double normalForce( const PDArgs& A, const PairK& pk, double r, doubl
         if ( delta_n >= pk.delta_tear )
         {
             double a = jkrPatchRadius( A, pk.Rs, delta_n );
-            double a3 = pow( a, 3.0 );
-            coeff = -1.0 * ( pk.coeff_h_n * a3 - pow( 8.0 * A.Gamma * kPi * A.Es * a3, 1.0 / 2.0 ) );
+            double a3 = crCube( a );
+            coeff = -1.0 * ( pk.coeff_h_n * a3 - crHalf( 8.0 * A.Gamma * kPi * A.Es * a3 ) );
             double Sn = 2.0 * A.Es * a;
             coeff -= A.coeff_h_d * sqrt( Sn * pk.ms ) * vn;
             if ( delta_n <= 0.0 && vn <= 0.0 )
@@ -136,8 +226,9 @@ __global__ __launch_bounds__( kBlock ) void kContact( const PDArgs* __restrict__
             if constexpr ( JKR )
             {
                 pk.coeff_h_n = 4.0 / 3.0 * A.Es / pk.Rs;
-                const double a0 = pow( 9.0 / 2.0 * A.Gamma * kPi / A.Es * pow( pk.Rs, 2.0 ), 1.0 / 3.0 );
-                pk.delta_tear = -1.0 / 2.0 * pow( 6.0, -1.0 / 3.0 ) * pow( a0, 2.0 ) / pk.Rs;
+                const double a0 = crThird( 9.0 / 2.0 * A.Gamma * kPi / A.Es * crSq( pk.Rs ) );
+                // pow(6, -1/3) of the host's libm, as the oracle evaluates it
+                pk.delta_tear = -1.0 / 2.0 * A.jkr_c6 * crSq( a0 ) / pk.Rs;
             }
             else
                 pk.coeff_h_n = 4.0 / 3.0 * A.Es * sqrt( pk.Rs );
@@ -192,9 +283,12 @@ __global__ __launch_bounds__( kBlock ) void kContact( const PDArgs* __restrict__
     A.f[p] = fx;
     A.f[n + p] = fy;
     A.f[2 * n + p] = fz;
-    if ( epi == kEpiStoreOnly )
+    if ( epi != kEpiFinal )
         return;
-    // final half step (CabanaPD_Integrate.hpp:128-134)
+    // final half step (CabanaPD_Integrate.hpp:128-134); only when the pair
+    // force does not read velocities, since other waves of this launch read
+    // v of their neighbors
+    
     const double half_dt_m = A.half_dt / rho_i;
     A.v[p] = vi0 + half_dt_m * fx;
     A.v[n + p] = vi1 + half_dt_m * fy;
@@ -242,6 +336,21 @@ __global__ __launch_bounds__( 256 ) void kContactInitial( const PDArgs* __restri
         partial[blockIdx.x] = red[0];
 }
 
+// VelocityVerlet::finalHalfStep (CabanaPD_Integrate.hpp:128-134) as its own
+// pass, after a contact pass whose pair forces read the velocities.
+__global__ __launch_bounds__( 256 ) void kContactFinal( const PDArgs* __restrict__ specs )
+{
+#pragma clang fp contract( off )
+    const PDArgs& A = *specs;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t n = A.n;
+    if ( p >= n )
+        return;
+    const double half_dt_m = A.half_dt / A.rho[p];
+    for ( int d = 0; d < 3; d++ )
+        A.v[d * n + p] = A.v[d * n + p] + half_dt_m * A.f[d * n + p];
+}
+
 // Maximum of the per-block maxima, written to out (one block).
 __global__ __launch_bounds__( 256 ) void kMaxReduce( const double* __restrict__ partial, int64_t nb,
                                                      double* __restrict__ out )
@@ -281,13 +390,34 @@ const char* contactKernelName( const PDArgs& a )
     return jkr ? "kContact<false, true>" : "kContact<false, false>";
 }
 
+// Whether the pair force reads the velocities: the damping term (restitution
+// below 1) and, for Hertz-JKR, the approach/recede switch and the friction
+// (force_models/CabanaPD_Hertzian.hpp:74-80, CabanaPD_HertzianJKR.hpp:96-129).
+// Undamped Hertz reads them only through a zero term.
+bool contactReadsVelocity( const PDArgs& a )
+{
+    return a.model == PDK_MODEL_HERTZ_JKR || a.coeff_h_d != 0.0;
+}
+
 void launchContactForce( const PDArgs* args_dev, const PDArgs& a, int32_t epilogue_mode, int32_t parity,
                          hipStream_t st )
 {
     if ( a.nslices == 0 )
         return;
+    // a velocity-dependent pass must not update v while other waves read it
+    if ( epilogue_mode == kEpiFinal && contactReadsVelocity( a ) )
+        throw Error( "internal: velocity-dependent contact needs the split final half step" );
     const unsigned g = static_cast<unsigned>( ( a.nslices + kSlicesPerBlock - 1 ) / kSlicesPerBlock );
     hipLaunchKernelGGL( pickContact( a ), dim3( g ), dim3( kBlock ), 0, st, args_dev, epilogue_mode, parity );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchContactFinal( const PDArgs* args_dev, const PDArgs& a, hipStream_t st )
+{
+    if ( a.n == 0 )
+        return;
+    hipLaunchKernelGGL( kContactFinal, dim3( static_cast<unsigned>( ( a.n + 255 ) / 256 ) ), dim3( 256 ), 0, st,
+                        args_dev );
     PDK_HIP( hipGetLastError() );
 }
 
@@ -342,7 +472,15 @@ void contactStep( pdk_spec_s* s, int32_t nsteps )
         if ( mx > cs.extend || periodic )
             specTimed( s, 3, [&] { contactBuildImpl( s, false, cs.max_disp, cs.since_build ); } );
         v = view( s );
-        specTimed( s, 0, [&] { launchContactForce( v.dargs, *v.h, kEpiFinal, *v.cur, v.st ); } );
+        // the final half step in the force pass's epilogue, or as a pass of
+        // its own after a pass whose pair forces read the velocities
+        const bool split = contactReadsVelocity( *v.h );
+        specTimed( s, 0, [&] {
+            launchContactForce( v.dargs, *v.h, split ? static_cast<int32_t>( kEpiApplyOnly ) : kEpiFinal, *v.cur,
+                                v.st );
+        } );
+        if ( split )
+            specTimed( s, 4, [&] { launchContactFinal( v.dargs, *v.h, v.st ); } );
     }
 }
 

@@ -125,21 +125,28 @@ __device__ __forceinline__ double plasticEnergyTerm( double s, double sp, double
 struct Bond32
 {
     float s, rinv, rx, ry, rz, xi;
-    float etan;       // |eta|_1 / |xi|, for the guard band
-    float ex, ey, ez; // eta in FP32
 };
 
-// Guard band of the mixed form: true when the FP32 stretch s lies within
-// bandf * (sqrt(3) max_k |eta_k| / |xi| + |s| + s0) of s0, evaluated as an FMA
-// chain with bx = sqrt(3) bandf / |xi| and bs0 = bandf * s0 per slot / per
-// launch (sqrt(3) max_k |eta_k| >= |eta|, one max3 instead of three FMAs).  The
-// band is 2^-13 relative, far above the FP32 error of s and of this evaluation.
-__device__ __forceinline__ bool inGuardBand( const Bond32& b, float s0f, float bandf, float bx, float bs0 )
+// Mixed-form bond geometry from the FP64 bond vector r = xi + (u_j - u_i)
+// (the reference bond xi of the stencil entry, or x_j - x_i, plus the
+// displacement difference, so no large coordinate cancels): the stretch
+// numerator |r|^2 - |xi|^2 in FP64 as one FMA chain seeded with -|xi|^2 (the
+// FP64 square of the reference bond), then 1/|r| and
+//   s = (|r|^2 - |xi|^2) / (|xi| (|r| + |xi|))
+// in FP32, with |r|^2 = |xi|^2 + numerator.  The numerator is exact to FP64
+// rounding for any rotation of the bond (no xi.eta against eta.eta
+// cancellation in FP32), so the FP32 stretch is within a few ulp of s.  The
+// force passes accumulate the bond force along the FP64 r.
+__device__ __forceinline__ void mixedBond( double r0, double r1, double r2, double xi2d, float xi, Bond32& b )
 {
-    float t = fmaf( -bandf, fabsf( b.s ), fabsf( b.s - s0f ) );
-    const float em = fmaxf( fmaxf( fabsf( b.ex ), fabsf( b.ey ) ), fabsf( b.ez ) );
-    t = fmaf( -bx, em, t );
-    return t <= bs0;
+    const float num = static_cast<float>( fma( r0, r0, fma( r1, r1, fma( r2, r2, -xi2d ) ) ) );
+    const float r2f = fmaf( xi, xi, num );
+    b.rinv = __builtin_amdgcn_rsqf( r2f );
+    b.s = num * __builtin_amdgcn_rcpf( fmaf( r2f, b.rinv, xi ) * xi );
+    b.rx = static_cast<float>( r0 );
+    b.ry = static_cast<float>( r1 );
+    b.rz = static_cast<float>( r2 );
+    b.xi = xi;
 }
 
 // Wave-uniform read through the scalar data cache (p must be the same in
@@ -189,63 +196,6 @@ __device__ __forceinline__ int4 scalarLoad4( const int4* p )
     return make_int4( f[0], f[1], f[2], f[3] );
 }
 
-__device__ __forceinline__ void getDistance32( double xi0, double xi1, double xi2, double ui0,
-                                               double ui1, double ui2, double xj0, double xj1,
-                                               double xj2, double uj0, double uj1, double uj2,
-                                               Bond32& b )
-{
-    const float gx = static_cast<float>( xj0 - xi0 );
-    const float gy = static_cast<float>( xj1 - xi1 );
-    const float gz = static_cast<float>( xj2 - xi2 );
-    const float ex = static_cast<float>( uj0 - ui0 );
-    const float ey = static_cast<float>( uj1 - ui1 );
-    const float ez = static_cast<float>( uj2 - ui2 );
-    b.rx = gx + ex;
-    b.ry = gy + ey;
-    b.rz = gz + ez;
-    const float r2 = b.rx * b.rx + b.ry * b.ry + b.rz * b.rz;
-    const float g2 = gx * gx + gy * gy + gz * gz;
-    b.rinv = __builtin_amdgcn_rsqf( r2 );
-    const float r = r2 * b.rinv;
-    const float xinv = __builtin_amdgcn_rsqf( g2 );
-    b.xi = g2 * xinv;
-    const float xe = gx * ex + gy * ey + gz * ez;
-    const float ee = ex * ex + ey * ey + ez * ez;
-    const float num = 2.0f * xe + ee;
-    b.s = num * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
-    // guard-band scale: (|ex| + |ey| + |ez|) / |xi| >= |eta| / |xi|
-    b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
-    b.ex = ex;
-    b.ey = ey;
-    b.ez = ez;
-}
-
-// Same stretch with the bond vector from the lattice stencil table
-// (g.xyz = FP32 xi, g.w = |xi|, xinv = rcp(|xi|)): no x_j gather.
-__device__ __forceinline__ void getDistance32s( const float4 g, float xinv, float xi2, double ui0, double ui1,
-                                                double ui2, double uj0, double uj1, double uj2, Bond32& b )
-{
-    const float ex = static_cast<float>( uj0 - ui0 );
-    const float ey = static_cast<float>( uj1 - ui1 );
-    const float ez = static_cast<float>( uj2 - ui2 );
-    b.rx = g.x + ex;
-    b.ry = g.y + ey;
-    b.rz = g.z + ez;
-    b.xi = g.w;
-    const float xe = g.x * ex + g.y * ey + g.z * ez;
-    const float ee = ex * ex + ey * ey + ez * ez;
-    const float num = 2.0f * xe + ee; // |xi + eta|^2 - |xi|^2
-    // |xi + eta|^2 from |xi|^2 = fl(g.w * g.w) and the numerator of s
-    const float r2 = xi2 + num;
-    b.rinv = __builtin_amdgcn_rsqf( r2 );
-    const float r = r2 * b.rinv;
-    b.s = num * __builtin_amdgcn_rcpf( b.xi * ( r + b.xi ) );
-    b.etan = ( fabsf( ex ) + fabsf( ey ) + fabsf( ez ) ) * xinv;
-    b.ex = ex;
-    b.ey = ey;
-    b.ez = ez;
-}
-
 // getLinearizedDistance (CabanaPD_Force.hpp:114-138), operation for
 // operation: stretch linearized in eta, force along the reference bond.
 // Stored so that the force passes' coeff * rx / r is coeff * xi_x / xi.
@@ -283,10 +233,6 @@ __device__ __forceinline__ void linearized32( float gx, float gy, float gz, floa
     b.xi = xi;
     b.rinv = xinv;
     b.s = ( gx * ex + gy * ey + gz * ez ) * ( xinv * xinv );
-    b.etan = 0.0f;
-    b.ex = ex;
-    b.ey = ey;
-    b.ez = ez;
 }
 
 // Bond geometry of the force and energy passes for the model.
@@ -300,31 +246,18 @@ __device__ __forceinline__ void bond64( double xi0, double xi1, double xi2, doub
     else
         getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
 }
-template <int MODEL>
-__device__ __forceinline__ void bond32( double xi0, double xi1, double xi2, double ui0, double ui1, double ui2,
-                                        double xj0, double xj1, double xj2, double uj0, double uj1, double uj2,
-                                        Bond32& b )
+// Linearized stretch of the mixed form, bond vector gathered (x_j - x_i)
+// or from the stencil table (g = (xi_x, xi_y, xi_z, |xi|), xinv = 1/|xi|).
+__device__ __forceinline__ void linearized32g( double xi0, double xi1, double xi2, double ui0, double ui1,
+                                               double ui2, double xj0, double xj1, double xj2, double uj0,
+                                               double uj1, double uj2, Bond32& b )
 {
-    if constexpr ( kIsLin<MODEL> )
-    {
-        const float gx = static_cast<float>( xj0 - xi0 );
-        const float gy = static_cast<float>( xj1 - xi1 );
-        const float gz = static_cast<float>( xj2 - xi2 );
-        const float g2 = gx * gx + gy * gy + gz * gz;
-        const float xinv = __builtin_amdgcn_rsqf( g2 );
-        linearized32( gx, gy, gz, g2 * xinv, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
-    }
-    else
-        getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, uj0, uj1, uj2, b );
-}
-template <int MODEL>
-__device__ __forceinline__ void bond32s( const float4 g, float xinv, float xi2, double ui0, double ui1, double ui2,
-                                         double uj0, double uj1, double uj2, Bond32& b )
-{
-    if constexpr ( kIsLin<MODEL> )
-        linearized32( g.x, g.y, g.z, g.w, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
-    else
-        getDistance32s( g, xinv, xi2, ui0, ui1, ui2, uj0, uj1, uj2, b );
+    const float gx = static_cast<float>( xj0 - xi0 );
+    const float gy = static_cast<float>( xj1 - xi1 );
+    const float gz = static_cast<float>( xj2 - xi2 );
+    const float g2 = gx * gx + gy * gy + gz * gz;
+    const float xinv = __builtin_amdgcn_rsqf( g2 );
+    linearized32( gx, gy, gz, g2 * xinv, xinv, ui0, ui1, ui2, uj0, uj1, uj2, b );
 }
 
 // Uniform per-wave access to the spec of this wave's slice.
@@ -445,6 +378,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     // All waves of a block belong to one spec: stage its stencil table (and
     // slot offsets) in LDS before any wave leaves.
     __shared__ float4 stab[( PREC == kMixed ) ? 256 : 1];
+    __shared__ double sxi2[( PREC == kMixed ) ? 256 : 1]; // FP64 |xi|^2 of the stencil entries
     __shared__ int32_t soffs[SLOTS ? 256 + kBatch : 1];
     __shared__ uint32_t rsm[( PREC == kMixed ) ? 9 : 1]; // row-start bits of the slots (stencil-slot form)
     const bool useSten = ( PREC == kMixed ) && ( SLOTS || w.A->sid != nullptr );
@@ -462,7 +396,10 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     if constexpr ( PREC == kMixed )
     {
         if ( useSten )
+        {
             stab[threadIdx.x] = w.A->stencil[threadIdx.x];
+            sxi2[threadIdx.x] = w.A->stencil_xi2d[threadIdx.x];
+        }
     }
     if constexpr ( SLOTS || PREC == kMixed )
         __syncthreads();
@@ -522,7 +459,17 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     // stencil (at most kRowMaxSlots bonds) and the row sums in FP64; without a
     // row table every bond is its own row.
     float px = 0.0f, py = 0.0f, pz = 0.0f;
-    const float cv32u = static_cast<float>( c ) * static_cast<float>( A.vol_value );
+    // non-linear mixed form: the bond forces fr r (FP32 fr, FP64 bond vector r)
+    // summed in FP64, as in the row-window pass
+    constexpr bool XV = ( PREC == kMixed ) && !kIsLin<MODEL>;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    // Uniform volume (the specimens the row-window pass takes without its
+    // general path): the per-bond factor c vol (PMB) or vol (LPS) multiplies
+    // each row sum in FP64 instead of every bond.
+    const bool fold = uvol && !( FRAC && any_nofail );
+    const double rscale = fold ? A.row_scale : 1.0;
+    const float c32 = static_cast<float>( c );
+    const float bandK = A.bandK;
     // The neighbor ids and broken-bond words of kBatch slots are fetched one
     // batch ahead (the index stream comes from HBM; keeping a batch in
     // flight hides its latency).  Within a batch, slots are processed UNR at
@@ -630,7 +577,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     mj[k] = mt[j];
                 if ( tmp )
                     tj[k] = tmp[j];
-                if ( !useSten )
+                // the mixed form needs x_j for the bond vector r = y_j - y_i
+                // (only the linearized models take xi from the stencil)
+                if ( PREC == kF64 || !useSten || !kIsLin<MODEL> )
                 {
                     xj[k][0] = x[j];
                     xj[k][1] = x[n + j];
@@ -653,13 +602,13 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         {
             const int32_t q = q0 + k;
             bool brk = false;
-            if constexpr ( PREC == kMixed )
+            if constexpr ( PREC == kMixed && !XV )
             {
                 if ( ( rsm[q >> 5] >> ( q & 31 ) ) & 1u )
                 {
-                    fx += static_cast<double>( px );
-                    fy += static_cast<double>( py );
-                    fz += static_cast<double>( pz );
+                    fx = fma( rscale, static_cast<double>( px ), fx );
+                    fy = fma( rscale, static_cast<double>( py ), fy );
+                    fz = fma( rscale, static_cast<double>( pz ), fz );
                     px = py = pz = 0.0f;
                 }
             }
@@ -708,20 +657,60 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                 else
                 {
                     Bond32 b;
-                    if ( useSten )
+                    double r0 = 0.0, r1 = 0.0, r2 = 0.0; // XV: the FP64 bond vector
+                    float wl = 1.0f; // LPS: infl |xi|
+                    if constexpr ( kIsLin<MODEL> )
                     {
-                        const uint32_t id =
-                            SLOTS ? static_cast<uint32_t>( q ) : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
-                        const float4 g = stab[id];
-                        bond32s<MODEL>( g, __builtin_amdgcn_rcpf( g.w ), g.w * g.w, ui0, ui1, ui2, uj[k][0], uj[k][1], uj[k][2],
-                                        b );
+                        if ( useSten )
+                        {
+                            const uint32_t id = SLOTS ? static_cast<uint32_t>( q )
+                                                      : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
+                            const float4 g4 = stab[id];
+                            linearized32( g4.x, g4.y, g4.z, g4.w, __builtin_amdgcn_rcpf( g4.w ), ui0, ui1, ui2,
+                                          uj[k][0], uj[k][1], uj[k][2], b );
+                        }
+                        else
+                            linearized32g( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
+                                           uj[k][1], uj[k][2], b );
                     }
                     else
-                        bond32<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
-                                       uj[k][1], uj[k][2], b );
+                    {
+                        // r = xi + (u_j - u_i): xi the slot's FP64 stencil offset
+                        // (as in the row-window pass) or x_j - x_i
+                        double xi2d;
+                        float xil;
+                        double g0 = xj[k][0] - xi0, g1 = xj[k][1] - xi1, g2 = xj[k][2] - xi2;
+                        if ( useSten )
+                        {
+                            const uint32_t id = SLOTS ? static_cast<uint32_t>( q )
+                                                      : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
+                            xi2d = sxi2[id];
+                            xil = stab[id].w;
+                            if ( SLOTS && A.stencil_xiv )
+                            {
+                                const double* xv = A.stencil_xiv + 4 * id;
+                                g0 = xv[0];
+                                g1 = xv[1];
+                                g2 = xv[2];
+                            }
+                        }
+                        else
+                        {
+                            xi2d = fma( g0, g0, fma( g1, g1, g2 * g2 ) );
+                            const float x2f = static_cast<float>( xi2d );
+                            xil = x2f * __builtin_amdgcn_rsqf( x2f );
+                        }
+                        r0 = ( uj[k][0] - ui0 ) + g0;
+                        r1 = ( uj[k][1] - ui1 ) + g1;
+                        r2 = ( uj[k][2] - ui2 ) + g2;
+                        mixedBond( r0, r1, r2, xi2d, xil, b );
+                        wl = infl1 ? 1.0f : xil;
+                    }
                     // TemperatureDependent: the thermal strain alpha * temp_avg
                     // (FP64, rounded once) leaves the stretch; the guard band
-                    // grows by its magnitude, which bounds that rounding.
+                    // then scales with the bond's terms, which bound the
+                    // rounding of the mechanical stretch, the thermal strain
+                    // and their difference.
                     double temp_avg = 0.0;
                     float ath = 0.0f;
                     const float s_mech = b.s;
@@ -735,17 +724,11 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     {
                         if ( !nf_i && !nf_j )
                         {
-                            const float band = A.bandf * ( b.etan + fabsf( s_mech ) + A.s0f + fabsf( ath ) );
+                            const float band =
+                                tmp ? fmaf( A.bandf, fabsf( s_mech ) + A.s0f + fabsf( ath ), bandK ) : bandK;
                             if ( fabsf( b.s - A.s0f ) <= band )
                             {
                                 // guard band: the reference's FP64 test on this bond
-                                const int64_t j = jj[k];
-                                if ( useSten )
-                                {
-                                    xj[k][0] = x[j];
-                                    xj[k][1] = x[n + j];
-                                    xj[k][2] = x[2 * n + j];
-                                }
                                 Bond64 e;
                                 getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
                                                uj[k][1], uj[k][2], e );
@@ -757,7 +740,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     }
                     if ( !brk && intact )
                     {
-                        float coeff;
+                        float fr;
                         if constexpr ( !kIsLps<MODEL> )
                         {
                             // ElasticPerfectlyPlastic: the plastic stretch stays FP64
@@ -765,18 +748,27 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                                 spl ? static_cast<float>( plasticStretch( spl + ( base + q ) * kSlice + lane,
                                                                           static_cast<double>( b.s ), A.sY ) )
                                     : b.s;
-                            coeff = ( uvol ? cv32u : static_cast<float>( c ) * static_cast<float>( vj[k] ) ) * sd;
+                            fr = ( fold ? sd : ( c32 * static_cast<float>( vj[k] ) ) * sd ) * b.rinv;
                         }
                         else
                         {
-                            const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
-                            coeff = static_cast<float>( Ai + aj[k] ) + b.s * static_cast<float>( Bi + bj[k] );
-                            coeff = coeff * infl * b.xi * static_cast<float>( vj[k] );
+                            const float coeff =
+                                static_cast<float>( Ai + aj[k] ) + b.s * static_cast<float>( Bi + bj[k] );
+                            fr = ( fold ? coeff * wl : ( coeff * wl ) * static_cast<float>( vj[k] ) ) * b.rinv;
                         }
-                        const float fr = coeff * b.rinv;
-                        px = fmaf( fr, b.rx, px );
-                        py = fmaf( fr, b.ry, py );
-                        pz = fmaf( fr, b.rz, pz );
+                        if constexpr ( XV )
+                        {
+                            const double frd = static_cast<double>( fr );
+                            sx = fma( frd, r0, sx );
+                            sy = fma( frd, r1, sy );
+                            sz = fma( frd, r2, sz );
+                        }
+                        else
+                        {
+                            px = fmaf( fr, b.rx, px );
+                            py = fmaf( fr, b.ry, py );
+                            pz = fmaf( fr, b.rz, pz );
+                        }
                     }
                 }
             }
@@ -805,11 +797,17 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         elo = mlo;
         ehi = mhi;
     }
-    if constexpr ( PREC == kMixed )
+    if constexpr ( PREC == kMixed && !XV )
     {
-        fx += static_cast<double>( px );
-        fy += static_cast<double>( py );
-        fz += static_cast<double>( pz );
+        fx = fma( rscale, static_cast<double>( px ), fx );
+        fy = fma( rscale, static_cast<double>( py ), fy );
+        fz = fma( rscale, static_cast<double>( pz ), fz );
+    }
+    if constexpr ( XV )
+    {
+        fx = fma( rscale, sx, fx );
+        fy = fma( rscale, sy, fy );
+        fz = fma( rscale, sz, fz );
     }
     if ( !valid )
         return;
@@ -845,11 +843,19 @@ kForceRows( const PDArgs* __restrict__ specs,
 #pragma clang fp contract( off )
     constexpr int kW = kSlice + kRowMaxSlots - 1;  // window width
     constexpr int NX = ( PREC == kF64 ) ? 3 : 0;    // staged x components
+    // mixed form (non-linear stretch): the bond vector r = xi + (u_j - u_i)
+    // in FP64 from the slot's FP64 stencil offset xi; the windows hold u only
+    constexpr bool XV = ( PREC == kMixed ) && !kIsLin<MODEL>;
     constexpr int NAB = kIsLps<MODEL> ? 2 : 0;  // staged lpsA, lpsB
     constexpr int KV = 3 + NX + NAB;                // staged vol (GEN)
     constexpr int KNF = KV + 1;                     // staged no-fail flag (GEN)
     constexpr int NC = KV + ( GEN ? 2 : 0 );
-    __shared__ double win[kSlicesPerBlock][2][NC][kW];
+    // channel stride: PMB mixed rows pad each channel to 128 values, so the
+    // second window half is stored by every lane (no per-row store masks);
+    // values past the row's window are never read
+    constexpr bool PAD = ( PREC == kMixed ) && !kIsLps<MODEL> && !GEN;
+    constexpr int kWS = PAD ? 2 * kSlice : kW;
+    __shared__ double win[kSlicesPerBlock][2][NC][kWS];
     WaveCtx w;
     if ( !waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w ) )
         return; // waves are independent: no block-level synchronisation
@@ -867,6 +873,7 @@ kForceRows( const PDArgs* __restrict__ specs,
     const double* __restrict__ x = scalarLoad( &A.x );
     const double* __restrict__ u = scalarLoad( &A.U[w.cur] );
     const int4* __restrict__ rows = scalarLoad( &A.rows );
+    // particle i: reference position (FP64 form) and displacement
     double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0;
     if constexpr ( PREC == kF64 )
     {
@@ -884,18 +891,18 @@ kForceRows( const PDArgs* __restrict__ specs,
     const uint64_t* __restrict__ mwords = FRAC ? bits : scalarLoad( &A.exist ) + row0;
     const float4* __restrict__ sten = scalarLoad( &A.stencil );
     const float4* __restrict__ saux = scalarLoad( &A.stencil_aux );
+    const double* __restrict__ sxiv = scalarLoad( &A.stencil_xiv );
     const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
     // Loop-invariant scalars held in registers (the args block lives in
     // global memory, which the bit-word stores could alias).
     const double c = A.c;
     const double bcc = A.bcc;
     const float c32 = static_cast<float>( A.c );
-    const float vol32 = static_cast<float>( A.vol_value );
-    const float cv32u = c32 * vol32;
     const double volu = A.vol_value;
     const float s0f = A.s0f;
-    const float bandf = A.bandf;
-    const float bs0 = bandf * s0f;
+    const float bandK = A.bandK;
+    // uniform volume: c vol (PMB) or vol (LPS) scales each FP32 row sum
+    const double rscale = GEN ? 1.0 : A.row_scale;
     const int infl1 = A.influence;
     // lanes whose particle may break bonds (every lane of the wave is active here)
     const uint64_t nfi_ok = FRAC ? __ballot( !nf_i ) : 0ull;
@@ -913,6 +920,8 @@ kForceRows( const PDArgs* __restrict__ specs,
         fy = A.f[n + p];
         fz = A.f[2 * n + p];
     }
+    // XV: the bond forces fr r (FP32 fr, FP64 bond vector r) summed in FP64
+    double sx = 0.0, sy = 0.0, sz = 0.0;
 
     const double* src[KV];
     src[0] = u;
@@ -980,9 +989,9 @@ kForceRows( const PDArgs* __restrict__ specs,
 #pragma unroll
         for ( int k = 0; k < NC; k++ )
         {
-            wb[k * kW + lane] = v0[k];
-            if ( lane < R.y - 1 )
-                wb[k * kW + kSlice + lane] = v1[k];
+            wb[k * kWS + lane] = v0[k];
+            if ( PAD || lane < R.y - 1 )
+                wb[k * kWS + kSlice + lane] = v1[k];
         }
         // LDS operations of one wavefront complete in order; the fences keep
         // the compiler from moving the window reads above the stores.
@@ -992,10 +1001,46 @@ kForceRows( const PDArgs* __restrict__ specs,
         if ( r + 1 < nrows )
             loadRow( r + 1 );
         float px = 0.0f, py = 0.0f, pz = 0.0f; // FP32 sum of this row's bond forces (mixed form)
-        auto slot = [&]( const int t ) __attribute__( ( always_inline ) ) {
-            const int32_t q = R.x + t;
-            // intact word (fracture) or existence word of the slot
-            const uint64_t ww = scalarLoad( mwords + q );
+        // the row's slots q = R.x + t through row base pointers, so the
+        // unrolled slots address their words and table entries with
+        // immediate offsets (no per-slot 64-bit address arithmetic)
+        const uint64_t* __restrict__ mw_r = mwords + R.x;
+        uint64_t* __restrict__ bits_r = bits + R.x;
+        const float4* __restrict__ st_r = sten + R.x;
+        const float4* __restrict__ sa_r = saux + R.x;
+        const double* __restrict__ xv_r = sxiv + 4 * R.x;
+        // The wave-uniform data of slot t: its intact word (fracture) or
+        // existence word, its FP32 stencil entries and (XV) its FP64 offset
+        // and |xi|^2 (one 32-byte record).  Loading slot t + 1's data while
+        // slot t computes measured slower on the fracture plate (0.209 vs
+        // 0.202 ms): the register copies between slots cost more scalar
+        // issue than the hidden load latency saved.
+        struct SlotK
+        {
+            uint64_t ww;
+            float4 g, ga;
+            double x0, x1, x2, x2d;
+        };
+        auto loadK = [&]( const int t ) __attribute__( ( always_inline ) ) {
+            SlotK K{};
+            K.ww = scalarLoad( mw_r + t );
+            if constexpr ( PREC == kMixed )
+            {
+                K.g = scalarLoad4( st_r + t );
+                K.ga = scalarLoad4( sa_r + t ); // (1/|xi|, |xi|^2, infl |xi|, infl |xi|^2)
+                if constexpr ( XV )
+                {
+                    const double* xv = xv_r + 4 * t;
+                    K.x0 = scalarLoad( xv );
+                    K.x1 = scalarLoad( xv + 1 );
+                    K.x2 = scalarLoad( xv + 2 );
+                    K.x2d = scalarLoad( xv + 3 );
+                }
+            }
+            return K;
+        };
+        auto slot = [&]( const int t, const SlotK& K ) __attribute__( ( always_inline ) ) {
+            const uint64_t ww = K.ww;
             // A slot with no live bond in the wave (the slots past a face of
             // the specimen, fully broken slots) adds nothing: the LPS passes
             // skip it (measured 8 % faster on the 128^3 LPS fracture cube; the
@@ -1007,7 +1052,7 @@ kForceRows( const PDArgs* __restrict__ specs,
             // The window always holds readable values; bonds that do not
             // exist are masked out below.
             const double* __restrict__ wr = wb + lane + t;
-            const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
+            const double uj0 = wr[0], uj1 = wr[kWS], uj2 = wr[2 * kWS];
             // candidates for the break test: intact, neither end no-fail
             // (LPS tests nofail(i) twice, force/CabanaPD_LPS.hpp:417-418)
             uint64_t cand = 0;
@@ -1015,7 +1060,7 @@ kForceRows( const PDArgs* __restrict__ specs,
             {
                 cand = ww & nfi_ok;
                 if constexpr ( !kIsLps<MODEL> && GEN )
-                    cand &= __ballot( wr[KNF * kW] == 0.0 );
+                    cand &= __ballot( wr[KNF * kWS] == 0.0 );
             }
             // Masks below are wave-uniform (ballots in converged code).
             uint64_t brkm = 0; // bonds that break in this pass
@@ -1024,10 +1069,10 @@ kForceRows( const PDArgs* __restrict__ specs,
                 bool brk = false;
                 if ( __builtin_amdgcn_inverse_ballot_w64( ww ) )
                 {
-                    const double vj = GEN ? wr[KV * kW] : volu;
+                    const double vj = GEN ? wr[KV * kWS] : volu;
                     Bond64 b;
-                    bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
-                                   b );
+                    bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kWS], wr[4 * kWS], wr[5 * kWS], uj0, uj1,
+                                   uj2, b );
                     if constexpr ( FRAC )
                         brk = criticalStretch64( b.r, b.xi, bcc ) && __builtin_amdgcn_inverse_ballot_w64( cand );
                     if ( !brk )
@@ -1038,7 +1083,7 @@ kForceRows( const PDArgs* __restrict__ specs,
                         else
                         {
                             const double infl = infl1 ? 1.0 / b.xi : 1.0;
-                            coeff = ( Ai + wr[( 3 + NX ) * kW] + b.s * ( Bi + wr[( 4 + NX ) * kW] ) ) * infl * b.xi * vj;
+                            coeff = ( Ai + wr[( 3 + NX ) * kWS] + b.s * ( Bi + wr[( 4 + NX ) * kWS] ) ) * infl * b.xi * vj;
                         }
                         fx += coeff * b.rx / b.r;
                         fy += coeff * b.ry / b.r;
@@ -1050,15 +1095,24 @@ kForceRows( const PDArgs* __restrict__ specs,
             }
             else
             {
-                const float4 g = scalarLoad4( sten + q );
-                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0)
+                const float4 g = K.g;
+                const float4 ga = K.ga;
                 Bond32 b;
-                bond32s<MODEL>( g, ga.x, ga.z, ui0, ui1, ui2, uj0, uj1, uj2, b );
+                double r0 = 0.0, r1 = 0.0, r2 = 0.0; // XV: the FP64 bond vector
+                if constexpr ( XV )
+                {
+                    r0 = ( uj0 - ui0 ) + K.x0;
+                    r1 = ( uj1 - ui1 ) + K.x1;
+                    r2 = ( uj2 - ui2 ) + K.x2;
+                    mixedBond( r0, r1, r2, K.x2d, g.w, b );
+                }
+                else
+                    linearized32( g.x, g.y, g.z, g.w, ga.x, ui0, ui1, ui2, uj0, uj1, uj2, b );
                 if constexpr ( FRAC )
                 {
-                    // FP32 decision, except inside the guard band where the
+                    // FP32 decision, except within bandK of s0 where the
                     // reference's FP64 test decides
-                    const uint64_t inb = __ballot( inGuardBand( b, s0f, bandf, ga.y, bs0 ) ) & cand;
+                    const uint64_t inb = __ballot( fabsf( b.s - s0f ) <= bandK ) & cand;
                     brkm = __ballot( b.s > s0f ) & cand & ~inb;
                     if ( inb )
                     {
@@ -1066,9 +1120,14 @@ kForceRows( const PDArgs* __restrict__ specs,
                         if ( __builtin_amdgcn_inverse_ballot_w64( inb ) )
                         {
                             // neighbor j = p + R.z + t: uniform base + lane; the
-                            // coordinates of i are reloaded here (rare path)
-                            // rather than held in registers for the whole pass
-                            const double* __restrict__ xj = x + ( p0 + R.z + t );
+                            // reference positions of i and j are loaded here
+                            // (rare path; the compiler barrier keeps the loads of
+                            // x_i from being hoisted into registers held for the
+                            // whole pass), the displacements are the registers
+                            // and the window
+                            __asm__ volatile( "" ::: "memory" );
+                            const int64_t jo = p0 + R.z + t;
+                            const double* __restrict__ xj = x + jo;
                             const double* __restrict__ xp = x + p0;
                             Bond64 e;
                             getDistance64( xp[lane], ( xp + n )[lane], ( xp + 2 * n )[lane], ui0, ui1, ui2, xj[lane],
@@ -1081,26 +1140,39 @@ kForceRows( const PDArgs* __restrict__ specs,
                 // dead lanes (absent, broken or breaking bonds) add exactly
                 // zero: their window values are finite, fr is selected to 0
                 const bool live = __builtin_amdgcn_inverse_ballot_w64( ww & ~brkm );
-                const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
-                float coeff;
+                float fr;
                 if constexpr ( !kIsLps<MODEL> )
-                    coeff = ( GEN ? c32 * vj32 : cv32u ) * b.s;
+                    fr = ( GEN ? ( c32 * static_cast<float>( wr[KV * kWS] ) ) * b.s : b.s ) * b.rinv;
                 else
                 {
-                    const float infl = infl1 ? ga.x : 1.0f;
-                    coeff = static_cast<float>( Ai + wr[( 3 + NX ) * kW] ) +
-                            b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kW] );
-                    coeff = coeff * infl * b.xi * vj32;
+                    const float coeff = static_cast<float>( Ai + wr[( 3 + NX ) * kWS] ) +
+                                        b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kWS] );
+                    fr = ( GEN ? ( coeff * ga.z ) * static_cast<float>( wr[KV * kWS] ) : coeff * ga.z ) * b.rinv;
                 }
-                const float fr = live ? coeff * b.rinv : 0.0f;
-                px = fmaf( fr, b.rx, px );
-                py = fmaf( fr, b.ry, py );
-                pz = fmaf( fr, b.rz, pz );
+                fr = live ? fr : 0.0f;
+                if constexpr ( XV )
+                {
+                    const double frd = static_cast<double>( fr );
+                    sx = fma( frd, r0, sx );
+                    sy = fma( frd, r1, sy );
+                    sz = fma( frd, r2, sz );
+                }
+                else
+                {
+                    px = fmaf( fr, b.rx, px );
+                    py = fmaf( fr, b.ry, py );
+                    pz = fmaf( fr, b.rz, pz );
+                }
             }
             if constexpr ( FRAC )
             {
-                if ( brkm && lane == 0 )
-                    bits[q] = ww & ~brkm;
+                // wave-uniform test first (rare: a bond of the wave breaks),
+                // then one lane writes the word
+                if ( __builtin_expect( brkm != 0, 0 ) )
+                {
+                    if ( lane == 0 )
+                        bits_r[t] = ww & ~brkm;
+                }
             }
         };
         // Mixed forms: the slots of a row (at most kRowMaxSlots) unrolled, so
@@ -1112,19 +1184,25 @@ kForceRows( const PDArgs* __restrict__ specs,
 #pragma unroll
             for ( int t = 0; t < kRowMaxSlots; t++ )
                 if ( t < R.y )
-                    slot( t );
+                    slot( t, loadK( t ) );
         }
         else
         {
             for ( int t = 0; t < R.y; t++ )
-                slot( t );
+                slot( t, loadK( t ) );
         }
-        if constexpr ( PREC == kMixed )
+        if constexpr ( PREC == kMixed && !XV )
         {
-            fx += static_cast<double>( px );
-            fy += static_cast<double>( py );
-            fz += static_cast<double>( pz );
+            fx = fma( rscale, static_cast<double>( px ), fx );
+            fy = fma( rscale, static_cast<double>( py ), fy );
+            fz = fma( rscale, static_cast<double>( pz ), fz );
         }
+    }
+    if constexpr ( XV )
+    {
+        fx = fma( rscale, sx, fx );
+        fy = fma( rscale, sy, fy );
+        fz = fma( rscale, sz, fz );
     }
     if ( !valid )
         return;
@@ -1192,24 +1270,21 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
             }
             else
             {
-                float gx, gy, gz;
+                // infl |xi|^2: |xi| (influence 1) or |xi|^2
+                float w2;
                 if ( sten )
                 {
                     // stencil-slot form: slot q is stencil entry q
-                    const float4 g = A.stencil[q];
-                    gx = g.x;
-                    gy = g.y;
-                    gz = g.z;
+                    const float xw = A.stencil[q].w;
+                    w2 = infl1 ? xw : xw * xw;
                 }
                 else
                 {
-                    gx = static_cast<float>( x[j] - xi0 );
-                    gy = static_cast<float>( x[n + j] - xi1 );
-                    gz = static_cast<float>( x[2 * n + j] - xi2 );
+                    const double g0 = x[j] - xi0, g1 = x[n + j] - xi1, g2 = x[2 * n + j] - xi2;
+                    const float x2f = static_cast<float>( fma( g0, g0, fma( g1, g1, g2 * g2 ) ) );
+                    w2 = infl1 ? x2f * __builtin_amdgcn_rsqf( x2f ) : x2f;
                 }
-                const float xi2f = gx * gx + gy * gy + gz * gz;
-                const float wv = infl1 ? ( xi2f * __builtin_amdgcn_rsqf( xi2f ) ) : xi2f;
-                m_i += static_cast<double>( wv * static_cast<float>( vol[j] ) );
+                m_i += static_cast<double>( w2 * static_cast<float>( vol[j] ) );
             }
         }
         A.m[p] = m_i;
@@ -1242,17 +1317,28 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
             else
             {
                 Bond32 b;
+                float w2;
                 if ( sten )
                 {
-                    const float4 g = A.stencil[q];
-                    getDistance32s( g, __builtin_amdgcn_rcpf( g.w ), g.w * g.w, ui0, ui1, ui2, u[j], u[n + j],
-                                    u[2 * n + j], b );
+                    // r = xi + (u_j - u_i) with the slot's FP64 stencil offset,
+                    // as in the row-window state pass
+                    const float xw = A.stencil[q].w;
+                    const double* xv = A.stencil_xiv + 4 * q;
+                    mixedBond( ( u[j] - ui0 ) + xv[0], ( u[n + j] - ui1 ) + xv[1], ( u[2 * n + j] - ui2 ) + xv[2],
+                               A.stencil_xi2d[q], xw, b );
+                    w2 = infl1 ? xw : xw * xw;
                 }
                 else
-                    getDistance32( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j],
-                                   u[2 * n + j], b );
-                const float infl = infl1 ? __builtin_amdgcn_rcpf( b.xi ) : 1.0f;
-                th += static_cast<double>( infl * b.s * b.xi * b.xi * static_cast<float>( vol[j] ) );
+                {
+                    const double g0 = x[j] - xi0, g1 = x[n + j] - xi1, g2 = x[2 * n + j] - xi2;
+                    const double r0 = ( u[j] - ui0 ) + g0, r1 = ( u[n + j] - ui1 ) + g1, r2 = ( u[2 * n + j] - ui2 ) + g2;
+                    const double xi2d = fma( g0, g0, fma( g1, g1, g2 * g2 ) );
+                    const float x2f = static_cast<float>( xi2d );
+                    const float xil = x2f * __builtin_amdgcn_rsqf( x2f );
+                    mixedBond( r0, r1, r2, xi2d, xil, b );
+                    w2 = infl1 ? xil : x2f;
+                }
+                th += static_cast<double>( w2 * b.s * static_cast<float>( vol[j] ) );
             }
         }
         if constexpr ( PREC == kMixed )
@@ -1286,6 +1372,9 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     constexpr int kW = kSlice + kRowMaxSlots - 1;
     constexpr int NX = ( PREC == kF64 ) ? 3 : 0; // staged x components
     constexpr int KV = 3 + NX;                   // staged vol (GEN)
+    // mixed form: the bond vector r = xi + (u_j - u_i) in FP64 from the slot's
+    // FP64 stencil offset xi; the windows hold u only
+    constexpr bool XV = PREC == kMixed;
     constexpr int NC = KV + ( GEN ? 1 : 0 );
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
     WaveCtx w;
@@ -1311,6 +1400,8 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     const uint64_t* __restrict__ mwords = scalarLoad( FRAC ? &A.intact : &A.exist ) + row0;
     const float4* __restrict__ sten = scalarLoad( &A.stencil );
     const float4* __restrict__ saux = scalarLoad( &A.stencil_aux );
+    const double* __restrict__ sxi2 = scalarLoad( &A.stencil_xi2d );
+    const double* __restrict__ sxiv = scalarLoad( &A.stencil_xiv );
     const int infl1 = A.influence;
     const double volu = A.vol_value;
     const float vol32 = static_cast<float>( A.vol_value );
@@ -1443,22 +1534,18 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
         walk( [&]( bool live, const double* wr, int32_t q ) {
             if ( live )
             {
-                const float4 g = scalarLoad4( sten + q );
-                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0)
-                const float xinv = ga.x;
+                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, |xi|^2, infl |xi|, infl |xi|^2)
                 const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
                 if ( do_m )
-                {
-                    const float xi2f = g.x * g.x + g.y * g.y + g.z * g.z;
-                    const float wvb = infl1 ? ( xi2f * __builtin_amdgcn_rsqf( xi2f ) ) : xi2f;
-                    m_i += static_cast<double>( wvb * vj32 );
-                }
+                    m_i += static_cast<double>( ga.w * vj32 );
                 if ( do_theta )
                 {
                     Bond32 b;
-                    getDistance32s( g, xinv, ga.z, ui0, ui1, ui2, wr[0], wr[kW], wr[2 * kW], b );
-                    const float infl = infl1 ? xinv : 1.0f;
-                    th += static_cast<double>( infl * b.s * b.xi * b.xi * vj32 );
+                    const double* xv = sxiv + 4 * q;
+                    mixedBond( ( wr[0] - ui0 ) + scalarLoad( xv ), ( wr[kW] - ui1 ) + scalarLoad( xv + 1 ),
+                               ( wr[2 * kW] - ui2 ) + scalarLoad( xv + 2 ), scalarLoad( sxi2 + q ),
+                               scalarLoad4( sten + q ).w, b );
+                    th += static_cast<double>( ga.w * b.s * vj32 );
                 }
             }
         } );
@@ -1813,6 +1900,101 @@ __global__ __launch_bounds__( 256 ) void kInitialHalfStep( const PDArgs* __restr
 }
 
 // Force BC / body term on the stored f (init at t = 0, CabanaPD_Solver.hpp:272-274).
+// ---------------------------------------------------------------------------
+// Contact between the particles of a PD specimen: Force<NormalRepulsionModel>::
+// computeForceFull (force/CabanaPD_Contact.hpp:44-80) with forceCoeff
+// (force_models/CabanaPD_Contact.hpp:82-89), added to the stored PD force in
+// list order (computeForce with reset = false, CabanaPD_Solver.hpp:364-366),
+// then the force terms and the final half step.  FP64, the oracle's operand
+// order; a pair beyond the contact radius adds +0.0 as there.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__( kBlock ) void kPdContact( const PDArgs* __restrict__ specs, PdContactArgs ca,
+                                                        int32_t epi, int32_t parity )
+{
+#pragma clang fp contract( off )
+    const PDArgs& A = *specs;
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t slice = xcdRemap( blockIdx.x, gridDim.x ) * kSlicesPerBlock + threadIdx.x / kSlice;
+    if ( slice >= A.nslices )
+        return;
+    const int64_t n = A.n;
+    const int64_t p = slice * kSlice + lane;
+    if ( p >= n )
+        return; // no cross-lane operations in this kernel
+    const double* __restrict__ x = A.x;
+    const double* __restrict__ u = A.U[parity];
+    const double xi0 = x[p], xi1 = x[n + p], xi2 = x[2 * n + p];
+    const double ui0 = u[p], ui1 = u[n + p], ui2 = u[2 * n + p];
+    double fx = A.f[p], fy = A.f[n + p], fz = A.f[2 * n + p];
+    const int32_t mycnt = ca.cnt[p];
+    const int32_t* __restrict__ colp = ca.col + ca.slice_ptr[slice] * kSlice + lane;
+    for ( int32_t q = 0; q < mycnt; q++ )
+    {
+        const int64_t j = colp[q * kSlice];
+        Bond64 b;
+        getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j], u[j], u[n + j], u[2 * n + j], b );
+        double fcx = 0.0, fcy = 0.0, fcz = 0.0;
+        if ( b.r < ca.radius )
+        {
+            const double sc = ( b.r - ca.radius ) / ca.delta;
+            const double coeff = 15.0 * ca.c * sc * A.vol[j];
+            fcx = coeff * b.rx / b.r;
+            fcy = coeff * b.ry / b.r;
+            fcz = coeff * b.rz / b.r;
+        }
+        fx += fcx;
+        fy += fcy;
+        fz += fcz;
+    }
+    epilogue( A, p, epi, parity, fx, fy, fz );
+}
+
+// Largest |U - u_neigh| (VelocityVerlet<Contact>::initialHalfStep,
+// CabanaPD_Integrate.hpp:177-181, with the oracle's sqrt form): block maxima,
+// then one block reduces them in block order.
+__global__ __launch_bounds__( 256 ) void kMaxDisplacement( const double* __restrict__ U,
+                                                           const double* __restrict__ un, int64_t n,
+                                                           double* __restrict__ partial )
+{
+#pragma clang fp contract( off )
+    __shared__ double red[256];
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if ( p < n )
+    {
+        const double a = U[p] - un[p], b = U[n + p] - un[n + p], c = U[2 * n + p] - un[2 * n + p];
+        m = sqrt( a * a + b * b + c * c );
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for ( int w = 128; w > 0; w >>= 1 )
+    {
+        if ( threadIdx.x < w )
+            red[threadIdx.x] = fmax( red[threadIdx.x], red[threadIdx.x + w] );
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        partial[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__( 256 ) void kMaxOfPartials( const double* __restrict__ partial, int64_t nb,
+                                                         double* __restrict__ out )
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    for ( int64_t b = threadIdx.x; b < nb; b += blockDim.x )
+        m = fmax( m, partial[b] );
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for ( int w = 128; w > 0; w >>= 1 )
+    {
+        if ( threadIdx.x < w )
+            red[threadIdx.x] = fmax( red[threadIdx.x], red[threadIdx.x + w] );
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        *out = red[0];
+}
+
 __global__ __launch_bounds__( 256 ) void kApplyForceTerms( const PDArgs* __restrict__ specs )
 {
 #pragma clang fp contract( off )
@@ -2100,23 +2282,24 @@ void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipS
 
 namespace
 {
-__global__ void kStencilAux( const float4* __restrict__ table, int32_t S, float bandf, float4* __restrict__ aux )
+// Per stencil entry (FP32): 1/|xi|, |xi|^2 and the LPS influence products
+// infl |xi| and infl |xi|^2 (infl = 1/|xi| for influence 1, else 1).
+__global__ void kStencilAux( const float4* __restrict__ table, int32_t S, int32_t influence, float4* __restrict__ aux )
 {
     const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if ( q < S )
     {
         const float w = table[q].w;
         const float xinv = __builtin_amdgcn_rcpf( w );
-        aux[q] = make_float4( xinv, 1.7320508f * bandf * xinv, w * w, 0.0f );
+        aux[q] = make_float4( xinv, w * w, influence ? 1.0f : w, influence ? w : w * w );
     }
 }
 } // namespace
-
-void launchStencilAux( const float4* table, int32_t S, float bandf, float4* aux, hipStream_t st )
+void launchStencilAux( const float4* table, int32_t S, int32_t influence, float4* aux, hipStream_t st )
 {
     if ( S <= 0 )
         return;
-    hipLaunchKernelGGL( kStencilAux, dim3( ( S + 255 ) / 256 ), dim3( 256 ), 0, st, table, S, bandf, aux );
+    hipLaunchKernelGGL( kStencilAux, dim3( ( S + 255 ) / 256 ), dim3( 256 ), 0, st, table, S, influence, aux );
     PDK_HIP( hipGetLastError() );
 }
 
@@ -2136,6 +2319,33 @@ void launchInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t par
         return;
     hipLaunchKernelGGL( kInitialHalfStep, dim3( static_cast<unsigned>( ( a.n + 255 ) / 256 ) ), dim3( 256 ), 0, st,
                         args_dev, parity );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchPdContact( const PDArgs* args_dev, const PDArgs& a, const PdContactArgs& ca, int32_t epi, int32_t parity,
+                      hipStream_t st )
+{
+    if ( a.nslices == 0 )
+        return;
+    hipLaunchKernelGGL( kPdContact, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, ca, epi, parity );
+    PDK_HIP( hipGetLastError() );
+}
+
+int64_t maxDisplacementBlocks( int64_t n )
+{
+    return ( n + 255 ) / 256;
+}
+
+void launchMaxDisplacement( const double* U, const double* u_neigh, int64_t n, double* partial, double* out_dev,
+                            hipStream_t st )
+{
+    const int64_t nb = maxDisplacementBlocks( n );
+    if ( nb == 0 )
+        return;
+    hipLaunchKernelGGL( kMaxDisplacement, dim3( static_cast<unsigned>( nb ) ), dim3( 256 ), 0, st, U, u_neigh, n,
+                        partial );
+    PDK_HIP( hipGetLastError() );
+    hipLaunchKernelGGL( kMaxOfPartials, dim3( 1 ), dim3( 256 ), 0, st, partial, nb, out_dev );
     PDK_HIP( hipGetLastError() );
 }
 

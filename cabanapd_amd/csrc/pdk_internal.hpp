@@ -144,7 +144,7 @@ bool latticeOrder( const double* x_aos_host, int64_t n, std::vector<int64_t>& pe
 // Stencil ids of a lattice specimen's bonds (mixed form); false when some
 // bond is not a lattice vector or the stencil has more than 255 entries.
 bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_rows, DevBuf& sid, DevBuf& table,
-                   BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out );
+                   DevBuf& xi2d, BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out );
 // Stencil-slot form of a full lattice (dims[0]*dims[1]*dims[2] == n, lattice
 // rows): verifies that every compact row equals its in-box stencil entries in
 // one fixed order (the caller-id order of an interior particle's row), then
@@ -154,7 +154,7 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
 // specimen does not qualify.
 struct SlotForm
 {
-    DevBuf exist, soff, table, rows, rowstart, aux;
+    DevBuf exist, soff, table, xi2d, xiv, rows, rowstart, aux;
     int32_t slots = 0;
     int32_t nrows = 0; // row windows (0: the stencil does not split into <= kMaxRows rows)
 };
@@ -212,7 +212,7 @@ void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, doub
                    int64_t nblocks, hipStream_t st );
 int64_t energyBlocks( const PDArgs& a );
 // Per stencil entry (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0) for the mixed row-window passes.
-void launchStencilAux( const float4* table, int32_t S, float bandf, float4* aux, hipStream_t st );
+void launchStencilAux( const float4* table, int32_t S, int32_t influence, float4* aux, hipStream_t st );
 // Stress tensor pass into a.stress ([9][n], device order).
 void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
 const char* stressKernelName( const PDArgs& a );
@@ -220,6 +220,23 @@ const char* stressKernelName( const PDArgs& a );
 void launchScatterRows( const double* src_soa, const int64_t* perm, double* dst_aos, int64_t n, int32_t ncomp,
                         hipStream_t st );
 void launchInitialHalfStep( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
+// Contact list of a PD specimen (pdk_set_contact) and its NormalRepulsion
+// constants, passed by value to the contact pass.
+struct PdContactArgs
+{
+    const int64_t* slice_ptr;
+    const int32_t* cnt;
+    const int32_t* col;
+    double radius, delta, c;
+};
+// Adds the contact forces to the stored PD force, then the epilogue (force
+// terms and velocity-Verlet halves) of mode epi.
+void launchPdContact( const PDArgs* args_dev, const PDArgs& a, const PdContactArgs& ca, int32_t epi, int32_t parity,
+                      hipStream_t st );
+// max_p |U[p] - u_neigh[p]| into *out_dev (partial holds one double per block of 256).
+void launchMaxDisplacement( const double* U, const double* u_neigh, int64_t n, double* partial, double* out_dev,
+                            hipStream_t st );
+int64_t maxDisplacementBlocks( int64_t n );
 void launchApplyForceTerms( const PDArgs* args_dev, const PDArgs& a, hipStream_t st );
 // Heat conduction over the bonds, then forward Euler of the temperature with dt.
 void launchHeatTransfer( const PDArgs* args_dev, const PDArgs& a, double dt, hipStream_t st );

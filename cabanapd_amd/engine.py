@@ -71,8 +71,11 @@ _SIGS = {
     "pdk_compute_energy_async": (C.c_int, [_VP]),
     "pdk_compute_stress": (C.c_int, [_VP]),
     "pdk_set_materials": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.Material), _I32]),
+    "pdk_set_pair_models": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.PairModel), _I32]),
     "pdk_set_prescribed": (C.c_int, [_VP, C.c_int32, _I64, C.c_int64, _D, _D]),
     "pdk_set_field": (C.c_int, [_VP, C.c_int32, _D]),
+    "pdk_set_contact": (C.c_int, [_VP, C.POINTER(_abi.ContactDesc)]),
+    "pdk_contact_pairs": (C.c_int, [_VP, _I64]),
 }
 
 
@@ -268,6 +271,16 @@ class Spec:
         t = None if mtype is None else np.ascontiguousarray(mtype, dtype=np.int32)
         _check(lib().pdk_set_materials(self.h, len(mats), arr, None if t is None else t.ctypes.data_as(_I32)))
 
+    def set_pair_models(self, pairs, mtype):
+        """createMultiForceModel(particles, indexing, models...): pairs[a * ntypes + b]
+        holds the constants of models[indexing(a, b)] (see _abi.pair_models_array)."""
+        nt = int(round(len(pairs) ** 0.5))
+        if nt * nt != len(pairs):
+            raise ValueError("pairs must hold ntypes^2 entries")
+        arr = _abi.pair_models_array(pairs)
+        t = None if mtype is None else np.ascontiguousarray(mtype, dtype=np.int32)
+        _check(lib().pdk_set_pair_models(self.h, nt, arr, None if t is None else t.ctypes.data_as(_I32)))
+
     def compute_energy_async(self):
         _check(lib().pdk_compute_energy_async(self.h))
 
@@ -280,6 +293,15 @@ class Spec:
         b = _f64(b, (idx.size, nc)) if (b is not None and idx.size) else None
         _check(lib().pdk_set_prescribed(self.h, _abi.FIELD[field], idx.ctypes.data_as(_I64), idx.size, _dp(a),
                                         _dp(b)))
+
+    def set_contact(self, contact):
+        """A contact model on this PD specimen (an _abi.ContactDesc; pdk_set_contact)."""
+        _check(lib().pdk_set_contact(self.h, C.byref(contact)))
+
+    def contact_pairs(self):
+        t = C.c_int64()
+        _check(lib().pdk_contact_pairs(self.h, C.byref(t)))
+        return t.value
 
     def set_field(self, field, values):
         _check(lib().pdk_set_field(self.h, _abi.FIELD[field], _dp(_f64(values, (self.n,)))))

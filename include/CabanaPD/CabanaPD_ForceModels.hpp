@@ -15,13 +15,17 @@
 #ifndef CABANAPD_DROPIN_FORCEMODELS_HPP
 #define CABANAPD_DROPIN_FORCEMODELS_HPP
 
+#include <array>
 #include <cmath>
 #include <stdexcept>
+#include <algorithm>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
 #include <pdk.h>
 
+#include "CabanaPD_Indexing.hpp"
 #include "CabanaPD_Types.hpp"
 
 namespace CabanaPD
@@ -95,9 +99,26 @@ struct PDForceModel : BaseForceModel
     double s0 = 0.0;
     double bond_break_coeff = 0.0;
     double c = 0.0;
-    double theta_coeff = 0.0;
-    double s_coeff = 0.0;
+    // LPS coefficients on particle i's terms ([0]) and particle j's ([1]);
+    // both the model's own unless the model was averaged from two
+    // (force_models/CabanaPD_LPS.hpp:41-44, 55-69)
+    std::array<double, 2> theta_coeff{};
+    std::array<double, 2> s_coeff{};
     int influence_type = 0;
+
+    // This model's constants as one entry of a multi-material pair table.
+    pdk_pair_model pairModel() const
+    {
+        pdk_pair_model p{};
+        p.c = c;
+        p.bond_break_coeff = std::is_same_v<FractureType, Fracture> ? bond_break_coeff : INFINITY;
+        for ( int side = 0; side < 2; side++ )
+        {
+            p.theta_coeff[side] = theta_coeff[side];
+            p.s_coeff[side] = s_coeff[side];
+        }
+        return p;
+    }
 
     // The engine's description of this model for time step dt.
     pdk_model_desc desc( double dt ) const
@@ -135,10 +156,43 @@ struct PDForceModel : BaseForceModel
         c = k.c;
         s0 = k.s0;
         bond_break_coeff = k.bond_break_coeff;
-        theta_coeff = k.theta_coeff;
-        s_coeff = k.s_coeff;
+        theta_coeff = { k.theta_coeff, k.theta_coeff };
+        s_coeff = { k.s_coeff, k.s_coeff };
+    }
+
+    // The constructor that averages two models (BaseForceModel,
+    // CabanaPD_ForceModels.hpp:58-62: first horizon, mean K;
+    // force_models/CabanaPD_PMB.hpp:51-55: mean c;
+    // force_models/CabanaPD_LPS.hpp:55-69: mean G, [0] coefficients from the
+    // first model's K and G, [1] from the second's;
+    // BaseFractureModel, CabanaPD_ForceModels.hpp:114-121: mean G0,
+    // K-weighted s0).
+    template <class M1, class M2>
+    void average( const M1& m1, const M2& m2 )
+    {
+        force_horizon = m1.force_horizon;
+        K = ( m1.K + m2.K ) / 2.0;
+        G = ( m1.G + m2.G ) / 2.0;
+        c = ( m1.c + m2.c ) / 2.0;
+        theta_coeff = { 3.0 * m1.K - 5.0 * m1.G, 3.0 * m2.K - 5.0 * m2.G };
+        s_coeff = { 15.0 * m1.G, 15.0 * m2.G };
+        influence_type = m1.influence_type;
+        if ( m2.influence_type != m1.influence_type )
+            throw std::runtime_error( "Influence function type for each model must match for multi-material systems" );
+        if constexpr ( std::is_same_v<FractureType, Fracture> )
+        {
+            G0 = ( m1.G0 + m2.G0 ) / 2.0;
+            s0 = std::sqrt( ( m1.s0 * m1.s0 * m1.K + m2.s0 * m2.s0 * m2.K ) / ( m1.K + m2.K ) );
+            bond_break_coeff = ( 1.0 + s0 ) * ( 1.0 + s0 );
+        }
     }
 };
+
+namespace detail
+{
+template <class M>
+using IfPDModel = std::enable_if_t<std::is_base_of_v<BaseForceModel, M>, int>;
+} // namespace detail
 
 template <class Model, class Mech, class F, class Thermal, class... Fields>
 struct is_fracture<ForceModel<Model, Mech, F, Thermal, Fields...>> : std::is_same<F, Fracture>
@@ -149,6 +203,12 @@ struct is_fracture<ForceModel<Model, Mech, F, Thermal, Fields...>> : std::is_sam
 template <>
 struct ForceModel<PMB, Elastic, NoFracture> : PDForceModel<PMB, NoFracture>
 {
+    // averaged from two models (multi-material bonds between their materials)
+    template <class M1, class M2, detail::IfPDModel<M1> = 0, detail::IfPDModel<M2> = 0>
+    ForceModel( const M1& m1, const M2& m2 )
+    {
+        this->average( m1, m2 );
+    }
     ForceModel( PMB, NoFracture, const double force_horizon_, const double K_ ) { set( force_horizon_, K_ ); }
     ForceModel( PMB, Elastic, NoFracture, const double force_horizon_, const double K_ )
     {
@@ -168,6 +228,12 @@ struct ForceModel<PMB, Elastic, NoFracture> : PDForceModel<PMB, NoFracture>
 template <>
 struct ForceModel<PMB, Elastic, Fracture> : PDForceModel<PMB, Fracture>
 {
+    // averaged from two models (multi-material bonds between their materials)
+    template <class M1, class M2, detail::IfPDModel<M1> = 0, detail::IfPDModel<M2> = 0>
+    ForceModel( const M1& m1, const M2& m2 )
+    {
+        this->average( m1, m2 );
+    }
     ForceModel( PMB, const double h, const double k, const double g0, const int influence = 1 )
     {
         set( h, k, g0, influence );
@@ -445,6 +511,12 @@ ForceModel( LinearPMB, NoFracture, double, double, TT, double, double )
 template <>
 struct ForceModel<LPS, Elastic, NoFracture> : PDForceModel<LPS, NoFracture>
 {
+    // averaged from two models (multi-material bonds between their materials)
+    template <class M1, class M2, detail::IfPDModel<M1> = 0, detail::IfPDModel<M2> = 0>
+    ForceModel( const M1& m1, const M2& m2 )
+    {
+        this->average( m1, m2 );
+    }
     ForceModel( LPS, NoFracture, const double h, const double k, const double g, const int influence = 0 )
     {
         set( h, k, g, influence );
@@ -468,6 +540,12 @@ struct ForceModel<LPS, Elastic, NoFracture> : PDForceModel<LPS, NoFracture>
 template <>
 struct ForceModel<LPS, Elastic, Fracture> : PDForceModel<LPS, Fracture>
 {
+    // averaged from two models (multi-material bonds between their materials)
+    template <class M1, class M2, detail::IfPDModel<M1> = 0, detail::IfPDModel<M2> = 0>
+    ForceModel( const M1& m1, const M2& m2 )
+    {
+        this->average( m1, m2 );
+    }
     ForceModel( LPS, const double h, const double k, const double g, const double g0, const int influence = 0 )
     {
         set( h, k, g, g0, influence );
@@ -505,6 +583,12 @@ struct ForceModel<LPS, Elastic, Fracture> : PDForceModel<LPS, Fracture>
 template <>
 struct ForceModel<LinearPMB, Elastic, NoFracture> : PDForceModel<LinearPMB, NoFracture>
 {
+    // averaged from two models (multi-material bonds between their materials)
+    template <class M1, class M2, detail::IfPDModel<M1> = 0, detail::IfPDModel<M2> = 0>
+    ForceModel( const M1& m1, const M2& m2 )
+    {
+        this->average( m1, m2 );
+    }
     ForceModel( LinearPMB, NoFracture, const double h, const double k ) { set( h, k ); }
     ForceModel( LinearPMB, Elastic, NoFracture, const double h, const double k ) { set( h, k ); }
 
@@ -521,6 +605,12 @@ struct ForceModel<LinearPMB, Elastic, NoFracture> : PDForceModel<LinearPMB, NoFr
 template <>
 struct ForceModel<LinearLPS, Elastic, NoFracture> : PDForceModel<LinearLPS, NoFracture>
 {
+    // averaged from two models (multi-material bonds between their materials)
+    template <class M1, class M2, detail::IfPDModel<M1> = 0, detail::IfPDModel<M2> = 0>
+    ForceModel( const M1& m1, const M2& m2 )
+    {
+        this->average( m1, m2 );
+    }
     ForceModel( LinearLPS, NoFracture, const double h, const double k, const double g, const int influence = 0 )
     {
         set( h, k, g, influence );
@@ -658,30 +748,78 @@ struct HertzianJKRModel : ContactModel
     }
 };
 
+// NormalRepulsionModel (force_models/CabanaPD_Contact.hpp:53-90): the contact
+// model the reference pairs with a PD force model in one Solver (repulsion
+// 15 c ((r - radius) / delta) V_j for r < radius, c = 18 K / (pi delta^4)).
+struct NormalRepulsionModel : ContactModel
+{
+    using base_type = ContactModel;
+    using force_tag = NormalRepulsionModel;
+    double delta = 0.0;
+    double c = 0.0;
+    NormalRepulsionModel() = default;
+    NormalRepulsionModel( const double delta_, const double r, const double ext, const double K_ )
+        : ContactModel( r, ext ), delta( delta_ )
+    {
+        K = K_;
+        c = 18.0 * K / ( 3.141592653589793238462643383279502884 * delta * delta * delta * delta );
+    }
+    double forceCoeff( const double r, const double vol ) const
+    {
+        const double sc = ( r - radius ) / delta;
+        return 15.0 * c * sc * vol;
+    }
+    pdk_contact_desc contactDesc() const
+    {
+        pdk_contact_desc cd{};
+        cd.model = PDK_CONTACT_NORMAL_REPULSION;
+        cd.radius = radius;
+        cd.radius_extend = radius_extend;
+        cd.horizon = delta;
+        cd.K = K;
+        return cd;
+    }
+};
+
 template <class T>
 struct is_contact : std::integral_constant<bool, std::is_base_of_v<ContactModel, T>>
 {
 };
 
-// ---- multi-material models (CabanaPD_ForceModelsMulti.hpp:321-516) ------------
-// createMultiForceModel( particles, AverageTag{}, m_0, .., m_{n-1} ), 2 to 4
-// models of one model kind, fracture form and horizon: particle p uses the
-// model of material type(p) (Particles::sliceType), a bond between different
-// materials the averaged model.  The engine builds the pair tables
-// (pdk_set_materials); the explicit-indexing overloads of the reference
-// (createMultiForceModel( particles, indexing, models... )) are not provided.
+// ForceModel m12( m1, m2 ): the type of the first model.
+template <class T, class Mech, class F, class Th, class... X, class M2, detail::IfPDModel<M2> = 0>
+ForceModel( const ForceModel<T, Mech, F, Th, X...>&, const M2& ) -> ForceModel<T, Mech, F, Th, X...>;
+
+// ---- multi-material models (CabanaPD_ForceModelsMulti.hpp:185-358) ------------
+// ForceModels< Indexing, M_0, .. >: a list of models of one model kind,
+// fracture form and horizon, and an indexing; the bond from a particle of
+// material type a (Particles::sliceType) to one of type b uses the model at
+// list position indexing(a, b).  The engine receives the table of those
+// models' constants per ordered type pair (pdk_set_pair_models).
+//
+//   createMultiForceModel( particles, indexing, m_0, .., m_{k-1} )
+//   createMultiForceModel( particles, AverageTag{}, m_0, .., m_{n-1} ), 2 to 4
+//     models: the list m_0 .. m_{n-1} followed by the models averaged between
+//     each pair of materials, ordered for DiagonalIndexing<n> (PMB forms,
+//     symmetric) or FullIndexing<n> (LPS forms) as the reference orders them
+//     (CabanaPD_ForceModelsMulti.hpp:360-502).
 struct AverageTag
 {
 };
 
-template <class M0, class... Ms>
+template <class Indexing, class M0, class... Ms>
 struct ForceModels
 {
-    static_assert( sizeof...( Ms ) >= 1 && sizeof...( Ms ) <= 3, "2 to 4 materials" );
     static_assert( ( std::is_same_v<typename M0::force_tag, typename Ms::force_tag> && ... ),
                    "All forces need the same base type" );
+    static_assert( ( std::is_same_v<typename M0::model_tag, typename Ms::model_tag> && ... ),
+                   "All models need the same base model" );
+    static_assert( ( std::is_same_v<typename M0::base_model, typename Ms::base_model> && ... ),
+                   "this build needs one model form (linearized or not) for all materials" );
     static_assert( ( std::is_same_v<typename M0::fracture_type, typename Ms::fracture_type> && ... ),
                    "this build needs one fracture form for all materials" );
+    static_assert( is_symmetric<typename M0::model_tag>::value == is_symmetric<Indexing>::value,
+                   "Symmetry of model and indexing must match" );
     using model_tag = typename M0::model_tag;
     using force_tag = typename M0::force_tag;
     using base_model = typename M0::base_model;
@@ -689,19 +827,59 @@ struct ForceModels
     using mechanics_type = Elastic;
     using material_type = MultiMaterial;
     using thermal_type = TemperatureIndependent;
+    using indexing_type = Indexing;
+    static constexpr std::size_t num_models = 1 + sizeof...( Ms );
+
     double force_horizon = 0.0;
     double K = 0.0;
-    std::vector<pdk_material> materials;
+    Indexing indexing;
+    // the constants of each listed model, in list order
+    std::vector<pdk_pair_model> models;
     M0 first;
 
-    ForceModels( const M0& m0, const Ms&... ms ) : force_horizon( m0.force_horizon ), K( m0.K ), first( m0 )
+    ForceModels( const Indexing& i, const M0& m0, const Ms&... ms )
+        : force_horizon( m0.force_horizon ), K( m0.K ), indexing( i ), first( m0 )
     {
         add( m0 );
         ( add( ms ), ... );
     }
+    // A list whose constants are already known (the AverageTag form).
+    ForceModels( const Indexing& i, const M0& m0, std::vector<pdk_pair_model> list )
+        : force_horizon( m0.force_horizon ), K( m0.K ), indexing( i ), models( std::move( list ) ), first( m0 )
+    {
+    }
     auto cutoff() const { return force_horizon; }
     void updateBonds( const int, const int ) {}
     pdk_model_desc desc( double dt ) const { return first.desc( dt ); }
+
+    // Material types covered: the indexing's own count, else the particles'.
+    int32_t numTypes( const std::vector<int>& type ) const
+    {
+        unsigned n = detail::IndexingTypes<Indexing>::value;
+        if ( n == 0 )
+            for ( int t : type )
+                n = std::max( n, static_cast<unsigned>( std::max( t, 0 ) ) + 1u );
+        if ( n > 4 )
+            throw std::runtime_error( "this build supports 1 to 4 material types" );
+        return static_cast<int32_t>( std::max( n, 1u ) );
+    }
+    // The pair table for ntypes types: entry a * ntypes + b holds the model
+    // at list position indexing(a, b).
+    std::vector<pdk_pair_model> pairTable( int32_t ntypes ) const
+    {
+        std::vector<pdk_pair_model> t( static_cast<size_t>( ntypes ) * ntypes );
+        for ( int32_t a = 0; a < ntypes; a++ )
+            for ( int32_t b = 0; b < ntypes; b++ )
+            {
+                const unsigned k = indexing( static_cast<unsigned>( a ), static_cast<unsigned>( b ) );
+                if ( k > Indexing::MaxValidIndex )
+                    throw std::out_of_range( "CabanaPD: Indexing computed an index that is out of its valid range" );
+                if ( k >= models.size() )
+                    throw std::out_of_range( "Functor outside of range requested" );
+                t[static_cast<size_t>( a ) * ntypes + b] = models[k];
+            }
+        return t;
+    }
 
   private:
     template <class M>
@@ -710,33 +888,77 @@ struct ForceModels
         // CabanaPD_ForceModelsMulti.hpp:295-311: equal horizons required
         if ( std::abs( m.force_horizon - force_horizon ) > 1e-10 )
             throw std::runtime_error( "Horizon for each model must match for multi-material systems." );
-        materials.push_back( pdk_material{ m.K, m.G, m.G0 } );
+        models.push_back( m.pairModel() );
     }
 };
 
-template <class M0, class... Ms>
-struct is_fracture<ForceModels<M0, Ms...>> : std::is_same<typename M0::fracture_type, Fracture>
+template <class Indexing, class M0, class... Ms>
+struct is_fracture<ForceModels<Indexing, M0, Ms...>> : std::is_same<typename M0::fracture_type, Fracture>
 {
 };
 
-template <class ParticleType, class M0, class... Ms>
-auto createMultiForceModel( ParticleType, AverageTag, M0 m0, Ms... ms )
+template <class ParticleType, class Indexing, class M0, class... Ms>
+auto createMultiForceModel( ParticleType, Indexing indexing, M0 m0, Ms... ms )
 {
-    return ForceModels<M0, Ms...>( m0, ms... );
+    return ForceModels<Indexing, M0, Ms...>( indexing, m0, ms... );
 }
 
 namespace detail
 {
-// Materials of a model for pdk_set_materials (empty: one material).
-template <class Model>
-std::vector<pdk_material> materialsOf( const Model& )
+template <unsigned N, class Sym>
+using AveragedIndexing = std::conditional_t<Sym::value, DiagonalIndexing<N>, FullIndexing<N>>;
+} // namespace detail
+
+template <class ParticleType, class M0, class... Ms>
+auto createMultiForceModel( ParticleType, AverageTag, M0 m0, Ms... ms )
 {
+    constexpr unsigned n = 1 + sizeof...( Ms );
+    static_assert( n >= 2 && n <= 4, "2 to 4 materials" );
+    static_assert( ( std::is_same_v<M0, Ms> && ... ), "this build averages models of one type" );
+    static_assert( std::is_constructible_v<M0, const M0&, const M0&>,
+                   "this build averages the Elastic PMB, LPS and linearized forms" );
+    using Sym = is_symmetric<typename M0::model_tag>;
+    using Indexing = detail::AveragedIndexing<n, Sym>;
+    const std::vector<M0> mats{ m0, ms... };
+    for ( const M0& m : mats )
+        // CabanaPD_ForceModelsMulti.hpp:295-311: equal horizons required
+        if ( std::abs( m.force_horizon - m0.force_horizon ) > 1e-10 )
+            throw std::runtime_error( "Horizon for each model must match for multi-material systems." );
+    auto pairOf = [&]( unsigned a, unsigned b ) {
+        return a == b ? mats[a].pairModel() : M0( mats[a], mats[b] ).pairModel();
+    };
+    std::vector<pdk_pair_model> models;
+    if constexpr ( Sym::value )
+    {
+        // diagonal d = 0 (the materials), then d = 1 .. n-1, each by min(a, b)
+        for ( unsigned d = 0; d < n; d++ )
+            for ( unsigned a = 0; a + d < n; a++ )
+                models.push_back( pairOf( a, a + d ) );
+    }
+    else
+    {
+        for ( unsigned a = 0; a < n; a++ )
+            for ( unsigned b = 0; b < n; b++ )
+                models.push_back( pairOf( a, b ) );
+    }
+    return ForceModels<Indexing, M0, Ms...>( Indexing{}, m0, std::move( models ) );
+}
+
+namespace detail
+{
+// The pair table of a model for pdk_set_pair_models (empty: one material).
+template <class Model>
+std::vector<pdk_pair_model> pairTableOf( const Model&, const std::vector<int>&, int32_t& ntypes )
+{
+    ntypes = 0;
     return {};
 }
-template <class M0, class... Ms>
-std::vector<pdk_material> materialsOf( const ForceModels<M0, Ms...>& m )
+template <class Indexing, class M0, class... Ms>
+std::vector<pdk_pair_model> pairTableOf( const ForceModels<Indexing, M0, Ms...>& m, const std::vector<int>& type,
+                                         int32_t& ntypes )
 {
-    return m.materials;
+    ntypes = m.numTypes( type );
+    return m.pairTable( ntypes );
 }
 } // namespace detail
 

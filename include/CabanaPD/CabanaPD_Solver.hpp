@@ -63,9 +63,11 @@ struct DeviceSpec
             (void)pdk_spec_destroy( spec );
     }
 
-    // Particles::create + update -> device; materials of a multi-material
-    // model (Particles::sliceType); then the Neighbor build.
-    void attach( const pdk_model_desc& d, ParticleStorage& s, const std::vector<pdk_material>& mats = {} )
+    // Particles::create + update -> device; the pair table of a
+    // multi-material model over ntypes material types (Particles::sliceType);
+    // then the Neighbor build.
+    void attach( const pdk_model_desc& d, ParticleStorage& s, const std::vector<pdk_pair_model>& pairs = {},
+                 int32_t ntypes = 0 )
     {
         lps = d.model == PDK_MODEL_LPS || d.model == PDK_MODEL_LINEAR_LPS;
         fracture = d.fracture != 0;
@@ -75,10 +77,10 @@ struct DeviceSpec
                                 s.nofail.data(), nullptr ) );
         if ( thermal )
             check( pdk_set_field( spec, PDK_FIELD_TEMPERATURE, s.temp.data() ) );
-        if ( !mats.empty() )
+        if ( !pairs.empty() )
         {
             std::vector<int32_t> type( s.type.begin(), s.type.end() );
-            check( pdk_set_materials( spec, static_cast<int32_t>( mats.size() ), mats.data(), type.data() ) );
+            check( pdk_set_pair_models( spec, ntypes, pairs.data(), type.data() ) );
         }
         check( pdk_build_bonds( spec ) );
     }
@@ -135,7 +137,9 @@ class Neighbor
     {
         pdk_model_desc d = model.desc( dt );
         d.neighbor_tol = tol;
-        _dev->attach( d, particles.storage(), detail::materialsOf( model ) );
+        int32_t ntypes = 0;
+        auto pairs = detail::pairTableOf( model, particles.storage().type, ntypes );
+        _dev->attach( d, particles.storage(), pairs, ntypes );
     }
     unsigned getMaxLocal() const
     {
@@ -249,12 +253,25 @@ class Solver
     Solver( Inputs _inputs, ParticleType _particles, ForceModelType _force_model )
         : particles( _particles ), inputs( _inputs ), force_model( _force_model )
     {
-        static_assert( std::is_same_v<ContactModelType, NoContact>,
-                       "PD + contact in one Solver is not supported by this build" );
         static_assert( !is_temperature_dependent<typename ForceModelType::thermal_type>::value ||
                            is_temperature_dependent<typename ParticleType::thermal_type>::value,
                        "a TemperatureDependent force model needs TemperatureDependent particles" );
         setup();
+    }
+
+    // A PD force model and a contact model (CabanaPD_Solver.hpp:134-151): the
+    // contact list is built on the current positions now; each step adds the
+    // contact forces after the PD force (pdk_set_contact).
+    Solver( Inputs _inputs, ParticleType _particles, ForceModelType _force_model, ContactModelType _contact_model )
+        : particles( _particles ), inputs( _inputs ), force_model( _force_model )
+    {
+        static_assert( std::is_same_v<ContactModelType, NormalRepulsionModel>,
+                       "the contact model of a PD Solver is NormalRepulsionModel" );
+        static_assert( !is_contact<ForceModelType>::value, "the force model of a PD + contact Solver is a PD model" );
+        setup();
+        contact_model = _contact_model;
+        const pdk_contact_desc cd = contact_model.contactDesc();
+        detail::check( pdk_set_contact( spec(), &cd ) );
     }
 
     void setup()
@@ -493,6 +510,7 @@ class Solver
 
     Inputs inputs;
     ForceModelType force_model;
+    ContactModelType contact_model{};
     std::shared_ptr<neighbor_type> neighbor;
     std::shared_ptr<force_type> force;
     double _last_energy = 0.0;

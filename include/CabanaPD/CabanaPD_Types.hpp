@@ -104,6 +104,21 @@ struct LinearLPS
     using base_type = LPS;
     using model_tag = LPS;
 };
+// Models whose pair force does not depend on the order of the two particles'
+// materials, and indexings that give (a, b) and (b, a) one model
+// (src/CabanaPD_Types.hpp:163-191).
+template <class T>
+struct is_symmetric : std::false_type
+{
+};
+template <>
+struct is_symmetric<PMB> : std::true_type
+{
+};
+template <>
+struct is_symmetric<LinearPMB> : std::true_type
+{
+};
 struct Elastic
 {
 };

@@ -265,6 +265,60 @@ typedef struct pdk_material {
 } pdk_material;
 int pdk_set_materials(pdk_spec spec, int32_t nmat, const pdk_material* materials, const int32_t* type);
 
+/* ---- pair models (explicit indexing) --------------------------------------------
+ * createMultiForceModel( particles, indexing, m_0, .., m_{k-1} )
+ * (CabanaPD_ForceModelsMulti.hpp:351-358; dispatch :228-250): the bond from a
+ * particle of material type a to one of type b uses model m_{indexing(a, b)}.
+ * pairs[a * ntypes + b] (1 <= ntypes <= 4) holds that model's constants as the
+ * reference's model objects carry them: c (PMB forms), bond_break_coeff =
+ * (1 + s0)^2 (fracture; +inf for a pair that never breaks, as
+ * BaseNoFractureModel), and the LPS coefficients theta_coeff[2], s_coeff[2]
+ * (force_models/CabanaPD_LPS.hpp:41-44, 117-132: [0] on particle i's terms, [1]
+ * on particle j's).  type[n] (caller order) holds each particle's material.
+ * This engine folds the LPS coefficients into per-particle terms, so an LPS
+ * spec needs theta_coeff[0] and s_coeff[0] of every pair (a, b) to depend on a
+ * only and [1] on b only, with the same per-type values (any table built from
+ * per-material models and their averages is of this form); other tables are
+ * rejected.  pdk_set_materials is the special case of the averaged pair table.
+ * FP64 only; call after pdk_spec_upload and before pdk_init. */
+typedef struct pdk_pair_model {
+    double c;
+    double bond_break_coeff;
+    double theta_coeff[2];
+    double s_coeff[2];
+} pdk_pair_model;
+int pdk_set_pair_models(pdk_spec spec, int32_t ntypes, const pdk_pair_model* pairs, const int32_t* type);
+
+/* ---- contact between the particles of a PD specimen ---------------------------
+ * Solver( inputs, particles, force_model, contact_model )
+ * (CabanaPD_Solver.hpp:134-151, 332-404): a second neighbor list on the current
+ * positions x + u with cutoff 2 radius + radius_extend (+ the neighbor tolerance
+ * at the first build, CabanaPD_Neighbor.hpp:60-64), rebuilt when the largest
+ * displacement since the last build exceeds radius_extend
+ * (CabanaPD_Neighbor.hpp:106-132, VelocityVerlet<Contact>,
+ * CabanaPD_Integrate.hpp:147-193).  Each step the contact forces are added
+ * after the PD force pass and before the force boundary conditions and the
+ * final half step; Solver::init computes no contact force.  NormalRepulsion
+ * (force/CabanaPD_Contact.hpp:44-80, force_models/CabanaPD_Contact.hpp:53-90):
+ * for every listed pair with r < radius, f_i += 15 c ((r - radius) / delta)
+ * V_j (x_j + u_j - x_i - u_i) / r with c = 18 K / (pi delta^4); FP64 in every
+ * precision.  Call after pdk_build_bonds and before pdk_init; specs with
+ * contact are stepped with pdk_step (not the batch engine). */
+enum pdk_contact_kind {
+    PDK_CONTACT_NORMAL_REPULSION = 0
+};
+typedef struct pdk_contact_desc {
+    int32_t model;          /* pdk_contact_kind */
+    int32_t reserved;
+    double radius;          /* ContactModel::radius */
+    double radius_extend;   /* ContactModel::radius_extend */
+    double horizon;         /* NormalRepulsionModel delta */
+    double K;               /* NormalRepulsionModel K */
+} pdk_contact_desc;
+int pdk_set_contact(pdk_spec spec, const pdk_contact_desc* contact);
+/* Pairs in the contact list of a PD specimen with contact (0 without). */
+int pdk_contact_pairs(pdk_spec spec, int64_t* pairs);
+
 /* ---- force / energy (free functions, CabanaPD_Force.hpp:176-219) ---------- */
 int pdk_compute_weighted_volume(pdk_spec spec);   /* Force<LPS>::computeWeightedVolume; no-op for PMB */
 int pdk_compute_dilatation(pdk_spec spec);        /* Force<LPS>::computeDilatation; no-op for PMB */

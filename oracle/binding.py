@@ -42,6 +42,7 @@ _SIGS = {
     "orc_compute_energy": (C.c_int, [_VP, _D, _D]),
     "orc_compute_stress": (C.c_int, [_VP]),
     "orc_set_materials": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.Material), _I32]),
+    "orc_set_pair_models": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.PairModel), _I32]),
     "orc_set_prescribed": (C.c_int, [_VP, C.c_int32, _I64, C.c_int64, _D, _D]),
     "orc_set_field": (C.c_int, [_VP, C.c_int32, _D]),
     "orc_get_plastic": (C.c_int, [_VP, _D]),
@@ -49,6 +50,9 @@ _SIGS = {
     "orc_step": (C.c_int, [_VP, C.c_int32]),
     "orc_get": (C.c_int, [_VP, C.c_int32, _D]),
     "orc_set_state": (C.c_int, [_VP, _D, _D]),
+    "orc_set_contact": (C.c_int, [_VP, C.POINTER(_abi.ContactDesc)]),
+    "orc_contact_pairs": (C.c_int, [_VP, _I64]),
+    "orc_cr_pow": (C.c_int, [C.c_int32, C.c_int64, _D, _D]),
     "orc_bond_pmb": (C.c_int, [C.POINTER(_abi.ModelDesc), _D, _D, C.c_double, _D, _D, _I32]),
     "orc_bond_prenotch": (C.c_int, [_D, _D, _D, _D, _D, _I32]),
     "orc_jkr_normal_force": (C.c_int, [C.POINTER(_abi.ModelDesc), C.c_double, C.c_double, C.c_double,
@@ -113,6 +117,14 @@ def critical_timestep(dx, dy, dz, K, rho, horizon, safety=0.85):
     return out.value
 
 
+def cr_pow(which, x):
+    """The oracle's correctly rounded contact powers: which in {"1.5", "1/3", "3", "0.5"}."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    _check(lib().orc_cr_pow({"1.5": 0, "1/3": 1, "3": 2, "0.5": 3}[which], x.size, _dp(x), _dp(out)))
+    return out
+
+
 def bond_pmb(desc, xi, eta, vol_j):
     f = np.zeros(3)
     w = C.c_double()
@@ -172,6 +184,15 @@ class OracleSpec:
     def build_bonds(self):
         _check(lib().orc_build_neighbors(self.h))
 
+    def set_contact(self, contact):
+        """A contact model on this PD specimen (an _abi.ContactDesc)."""
+        _check(lib().orc_set_contact(self.h, C.byref(contact)), "orc_set_contact")
+
+    def contact_pairs(self):
+        t = C.c_int64()
+        _check(lib().orc_contact_pairs(self.h, C.byref(t)))
+        return t.value
+
     def bond_stats(self):
         t = C.c_int64()
         m = C.c_int32()
@@ -222,6 +243,13 @@ class OracleSpec:
         arr = _abi.materials_array(mats)
         t = np.ascontiguousarray(mtype, dtype=np.int32)
         _check(lib().orc_set_materials(self.h, len(mats), arr, t.ctypes.data_as(_I32)))
+
+    def set_pair_models(self, pairs, mtype):
+        """Explicit-indexing pair table: pairs[a * ntypes + b] (see _abi.pair_models_array)."""
+        nt = int(round(len(pairs) ** 0.5))
+        arr = _abi.pair_models_array(pairs)
+        t = np.ascontiguousarray(mtype, dtype=np.int32)
+        _check(lib().orc_set_pair_models(self.h, nt, arr, t.ctypes.data_as(_I32)))
 
     def set_prescribed(self, field, idx, a, b=None):
         """Values a + b t applied before each force pass (and at t = 0 in init)."""

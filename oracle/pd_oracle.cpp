@@ -72,11 +72,14 @@ struct Sys
     double max_disp = 0.0;
     int64_t steps_since_build = 0;
     double total_W = 0.0, total_D = 0.0;
-    // multi-material (createMultiForceModel with AverageTag): material per
-    // particle, constants per material pair and per material
+    // multi-material (createMultiForceModel): material type per particle and
+    // the constants of the model of each ordered type pair (type(i), type(j));
+    // LPS pair models carry theta_coeff / s_coeff [0] for particle i's terms
+    // and [1] for particle j's (force_models/CabanaPD_LPS.hpp:41-44)
     int32_t nmat = 1;
+    bool pair_tables = false;
     std::vector<int32_t> mtype;
-    double pair_c[16] = {}, pair_bcc[16] = {}, mat_tc[4] = {}, mat_sc[4] = {};
+    double pair_c[16] = {}, pair_bcc[16] = {}, pair_tc[16][2] = {}, pair_sc[16][2] = {};
     // EPP: plastic stretch per bond sp[i * maxn + slot] (BasePlasticity::_s_p,
     // CabanaPD_ForceModels.hpp:149-163, zeroed when the bonds are built)
     std::vector<double> sp;
@@ -93,6 +96,13 @@ struct Sys
     };
     Prescribed pre_u, pre_temp;
     int64_t steps_done = 0;
+    // PD specimen with a contact model (Solver with four arguments,
+    // CabanaPD_Solver.hpp:134-151): its own neighbor list on current positions
+    bool pdc = false;
+    pdk_contact_desc cd{};
+    double crep = 0.0; // NormalRepulsionModel c (force_models/CabanaPD_Contact.hpp:77-80)
+    int32_t cmaxn = 0;
+    std::vector<int32_t> ccnt, cnbr;
 };
 
 bool isContact( const pdk_model_desc& d )
@@ -292,7 +302,8 @@ inline void bondGeometry( const Sys& s, int64_t i, int64_t j, double& xi, double
 // full-neighbor, cell-binned search): j != i with |p_j - p_i|^2 <= rc^2.
 // For polydisperse contact the pair cutoff is R_i + R_j + extend.
 // ---------------------------------------------------------------------------
-void buildNeighbors( Sys& s, const std::vector<double>& p, double rc, bool poly, double extend )
+void buildNeighbors( Sys& s, const std::vector<double>& p, double rc, bool poly, double extend, int32_t& out_maxn,
+                     std::vector<int32_t>& out_cnt, std::vector<int32_t>& out_nbr )
 {
     const int64_t n = s.n;
     double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
@@ -364,27 +375,31 @@ void buildNeighbors( Sys& s, const std::vector<double>& p, double rc, bool poly,
                 }
     };
 
-    s.cnt.assign( n, 0 );
+    out_cnt.assign( n, 0 );
 #pragma omp parallel for schedule( dynamic, 256 )
     for ( int64_t i = 0; i < n; i++ )
     {
         int32_t c = 0;
         visit( i, [&]( int64_t ) { c++; } );
-        s.cnt[i] = c;
+        out_cnt[i] = c;
     }
     int32_t maxn = 0;
     for ( int64_t i = 0; i < n; i++ )
-        maxn = std::max( maxn, s.cnt[i] );
-    s.maxn = maxn;
-    s.nbr.assign( static_cast<size_t>( n ) * std::max( maxn, 1 ), 0 );
+        maxn = std::max( maxn, out_cnt[i] );
+    out_maxn = maxn;
+    out_nbr.assign( static_cast<size_t>( n ) * std::max( maxn, 1 ), 0 );
 #pragma omp parallel for schedule( dynamic, 256 )
     for ( int64_t i = 0; i < n; i++ )
     {
-        int32_t* row = s.nbr.data() + i * s.maxn;
+        int32_t* row = out_nbr.data() + i * maxn;
         int32_t c = 0;
         visit( i, [&]( int64_t j ) { row[c++] = static_cast<int32_t>( j ); } );
         std::sort( row, row + c );
     }
+}
+void buildNeighbors( Sys& s, const std::vector<double>& p, double rc, bool poly, double extend )
+{
+    buildNeighbors( s, p, rc, poly, extend, s.maxn, s.cnt, s.nbr );
     s.built = true;
 }
 
@@ -552,36 +567,39 @@ void computeDilatation( Sys& s )
 // (CabanaPD_ForceModelsMulti.hpp:257-273, 338-372).
 inline int32_t matOf( const Sys& s, int64_t i )
 {
-    return s.nmat > 1 ? s.mtype[i] : 0;
+    return s.pair_tables ? s.mtype[i] : 0;
+}
+inline int32_t pairOf( const Sys& s, int64_t i, int64_t j )
+{
+    return 4 * matOf( s, i ) + matOf( s, j );
 }
 inline double pairC( const Sys& s, int64_t i, int64_t j )
 {
-    return s.nmat > 1 ? s.pair_c[4 * matOf( s, i ) + matOf( s, j )] : s.k.c;
+    return s.pair_tables ? s.pair_c[pairOf( s, i, j )] : s.k.c;
 }
 inline double pairBcc( const Sys& s, int64_t i, int64_t j )
 {
-    return s.nmat > 1 ? s.pair_bcc[4 * matOf( s, i ) + matOf( s, j )] : s.k.bond_break_coeff;
+    return s.pair_tables ? s.pair_bcc[pairOf( s, i, j )] : s.k.bond_break_coeff;
 }
-// LPS coefficients of particle i's terms: i's material, also in an averaged
-// pair model (force_models/CabanaPD_LPS.hpp:55-69: theta_coeff[0]/s_coeff[0]
-// from the first model, [1] from the second).
-inline double thetaCoeff( const Sys& s, int64_t i )
+// LPS coefficients of the bond's model: `side` 0 for particle i's terms, 1 for
+// particle j's (force_models/CabanaPD_LPS.hpp:125-128, 143-144).
+inline double thetaCoeff( const Sys& s, int64_t i, int64_t j, int side )
 {
-    return s.nmat > 1 ? s.mat_tc[matOf( s, i )] : s.k.theta_coeff;
+    return s.pair_tables ? s.pair_tc[pairOf( s, i, j )][side] : s.k.theta_coeff;
 }
-inline double sCoeff( const Sys& s, int64_t i )
+inline double sCoeff( const Sys& s, int64_t i, int64_t j, int side )
 {
-    return s.nmat > 1 ? s.mat_sc[matOf( s, i )] : s.k.s_coeff;
+    return s.pair_tables ? s.pair_sc[pairOf( s, i, j )][side] : s.k.s_coeff;
 }
 
 inline double lpsForceCoeff( const Sys& s, int64_t i, int64_t j, double st, double xi, double vol, double m_i,
                              double m_j, double theta_i, double theta_j )
 {
     const double infl = influence( s, xi );
-    const double theta_coeff_i = thetaCoeff( s, i );
-    const double theta_coeff_j = thetaCoeff( s, j );
-    const double s_coeff_i = sCoeff( s, i );
-    const double s_coeff_j = sCoeff( s, j );
+    const double theta_coeff_i = thetaCoeff( s, i, j, 0 );
+    const double theta_coeff_j = thetaCoeff( s, i, j, 1 );
+    const double s_coeff_i = sCoeff( s, i, j, 0 );
+    const double s_coeff_j = sCoeff( s, i, j, 1 );
     return ( theta_coeff_i * theta_i / m_i + theta_coeff_j * theta_j / m_j +
              st * ( s_coeff_i / m_i + s_coeff_j / m_j ) ) *
            infl * xi * vol;
@@ -745,8 +763,8 @@ void computeEnergyFull( Sys& s )
             {
                 // model( EnergyTag ) (force_models/CabanaPD_LPS.hpp:136-148)
                 const double infl = influence( s, xi );
-                const double theta_coeff_i = thetaCoeff( s, i );
-                const double s_coeff_i = sCoeff( s, i );
+                const double theta_coeff_i = thetaCoeff( s, i, j, 0 );
+                const double s_coeff_i = sCoeff( s, i, j, 0 );
                 w = 1.0 / num_bonds * 0.5 * theta_coeff_i / 3.0 * ( s.theta[i] * s.theta[i] ) +
                     0.5 * ( s_coeff_i / s.m[i] ) * infl * st * st * xi * xi * s.vol[j];
             }
@@ -842,6 +860,102 @@ void computeStressFull( Sys& s )
 }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded powers of the contact models.  The reference evaluates
+// Kokkos::pow (the platform's libm, within an ulp of the exact value); the
+// oracle and the engine both use these routines, built from IEEE operations
+// only (+ - * /, sqrt, fma, frexp/ldexp), so they return the same bits on the
+// CPU and the GPU.  Method: double-double arithmetic (Dekker's exact
+// products, here through fma) carries each power to ~2^-100 relative before
+// the one final rounding (Muller et al., Handbook of Floating-Point
+// Arithmetic, ch. 11 on correct rounding): the result is the correctly
+// rounded value unless the exact value lies within ~2^-100 of a rounding
+// boundary.
+//   crSq(x)    = x^2        (one product: exact rounding of x^2)
+//   crCube(x)  = x^3        (x^2 as a double-double, times x)
+//   crHalf(x)  = x^0.5      (sqrt; +0 at -0, as pow)
+//   crPow32(x) = x^1.5      (sqrt with its exact remainder, times x)
+//   crThird(x) = x^y, y = the double nearest 1/3 = (1 - 2^-54) / 3:
+//                cbrt(x) in double-double (Newton steps, the last with the
+//                exact double-double cube) times x^(-2^-54 / 3) =
+//                1 - t + t^2/2, t = ln(x) / (3 2^54), ln from an atanh series.
+// ---------------------------------------------------------------------------
+double crSq( double x )
+{
+    return x * x;
+}
+double crCube( double x )
+{
+    if ( !( std::fabs( x ) < 1e100 ) || x == 0.0 )
+        return x * x * x; // overflow, infinities, NaN and signed zeros as pow
+    const double ph = x * x, pl = std::fma( x, x, -ph );
+    const double qh = ph * x, ql = std::fma( ph, x, -qh );
+    return qh + ( ql + pl * x );
+}
+double crHalf( double x )
+{
+    return x == 0.0 ? 0.0 : std::sqrt( x );
+}
+double crPow32( double x )
+{
+    if ( !( x > 0.0 ) )
+        return x == 0.0 ? 0.0 : std::sqrt( -1.0 ); // pow(0, 1.5) = 0; NaN for x < 0 and NaN
+    if ( x == HUGE_VAL )
+        return x;
+    const double sq = std::sqrt( x );
+    const double rem = std::fma( -sq, sq, x ); // x - sq^2, exact
+    const double sl = rem / ( 2.0 * sq );      // sqrt(x) = sq + sl to ~2^-106
+    const double ph = x * sq, pl = std::fma( x, sq, -ph );
+    return ph + ( pl + x * sl );
+}
+double crLog( double x ) // x > 0 normal; ~1e-16 relative, the same operations everywhere
+{
+    int e = 0;
+    double m = std::frexp( x, &e ); // x = m 2^e, m in [0.5, 1)
+    if ( m < 0.70710678118654752440 )
+    {
+        m = 2.0 * m;
+        e = e - 1;
+    }
+    const double z = ( m - 1.0 ) / ( m + 1.0 ); // |z| <= 0.1716
+    const double z2 = z * z;
+    double sum = 1.0 / 23.0;
+    for ( int k = 21; k >= 1; k -= 2 )
+        sum = sum * z2 + 1.0 / k;
+    return static_cast<double>( e ) * 0.69314718055994530942 + 2.0 * z * sum;
+}
+double crThird( double x )
+{
+    if ( x != x || x < 0.0 )
+        return std::sqrt( -1.0 );
+    if ( x == 0.0 )
+        return 0.0; // +0 for -0, as pow
+    if ( x == HUGE_VAL )
+        return x;
+    int e = 0;
+    double m = std::frexp( x, &e ); // x = m 2^e
+    // m 2^r with e - r divisible by 3, m in [0.5, 4)
+    int r = e % 3;
+    if ( r < 0 )
+        r += 3;
+    m = std::ldexp( m, r );
+    const int q = ( e - r ) / 3;
+    // Newton for c^3 = m from a linear start on [0.5, 4): converges to ~1 ulp
+    double c = 0.6 + 0.2 * m;
+    for ( int it = 0; it < 7; it++ )
+        c = c - ( c * c * c - m ) / ( 3.0 * c * c );
+    // one Newton step on the double-double cube: m - c^3 exactly to ~2^-106
+    const double ph = c * c, pl = std::fma( c, c, -ph );
+    const double qh = ph * c, ql = std::fma( ph, c, -qh );
+    const double res = ( ( m - qh ) - ql ) - pl * c;
+    const double cl = res / ( 3.0 * ph );
+    // x^(-2^-54/3) = exp(-t), t = ln(x) / (3 2^54)
+    const double t = crLog( x ) * ( 1.0 / 54043195528445952.0 );
+    const double k = -t + 0.5 * ( t * t );
+    const double chi = std::ldexp( c, q ), clo = std::ldexp( cl, q );
+    return chi + ( clo + chi * k );
+}
+
+// ---------------------------------------------------------------------------
 // Contact (force/CabanaPD_Contact.hpp:109-172).
 // ---------------------------------------------------------------------------
 struct PairK
@@ -854,21 +968,20 @@ double jkrPatchRadius( const Sys& s, const PairK& pk, double delta_n )
     // force_models/CabanaPD_HertzianJKR.hpp:141-176
     const double Rs = pk.Rs;
     const double nu = s.d.nu, E = s.d.E, Gamma = s.k.Gamma;
-    double c0 = std::pow( Rs * delta_n, 2.0 );
-    double c1 = -4.0 * ( 1.0 - std::pow( nu, 2.0 ) ) * kPi * Gamma * std::pow( Rs, 2.0 ) / E;
+    double c0 = crSq( Rs * delta_n );
+    double c1 = -4.0 * ( 1.0 - crSq( nu ) ) * kPi * Gamma * crSq( Rs ) / E;
     double c2 = -2.0 * Rs * delta_n;
-    double P = -std::pow( c2, 2.0 ) / 12.0 - c0;
-    double Q = -std::pow( c2, 3.0 ) / 108.0 + c0 * c2 / 3.0 - std::pow( c1, 2.0 ) / 8.0;
-    double U = std::pow( -Q / 2.0 + std::pow( std::max( 0.0, std::pow( Q, 2.0 ) / 4.0 + std::pow( P, 3.0 ) / 27.0 ), 1.0 / 2.0 ),
-                         1.0 / 3.0 );
+    double P = -crSq( c2 ) / 12.0 - c0;
+    double Q = -crCube( c2 ) / 108.0 + c0 * c2 / 3.0 - crSq( c1 ) / 8.0;
+    double U = crThird( -Q / 2.0 + crHalf( std::max( 0.0, crSq( Q ) / 4.0 + crCube( P ) / 27.0 ) ) );
     double sv = 0.0;
     if ( P != 0.0 )
         sv = -5.0 / 6.0 * c2 + U - P / ( 3.0 * U );
     else
-        sv = -5.0 / 6.0 * c2 - std::pow( Q, 1.0 / 3.0 );
-    double w = std::pow( std::max( 0.0, c2 + 2.0 * sv ), 1.0 / 2.0 );
+        sv = -5.0 / 6.0 * c2 - crThird( Q );
+    double w = crHalf( std::max( 0.0, c2 + 2.0 * sv ) );
     double lambda = c1 / ( 2.0 * w + 1e-14 );
-    return 1.0 / 2.0 * ( w + std::pow( std::max( 0.0, std::pow( w, 2.0 ) - 4.0 * ( c2 + sv + lambda ) ), 1.0 / 2.0 ) );
+    return 1.0 / 2.0 * ( w + crHalf( std::max( 0.0, crSq( w ) - 4.0 * ( c2 + sv + lambda ) ) ) );
 }
 
 inline PairK pairConstants( const Sys& s, int64_t i, int64_t j )
@@ -897,8 +1010,8 @@ inline PairK pairConstants( const Sys& s, int64_t i, int64_t j )
     else
     {
         pk.coeff_h_n = 4.0 / 3.0 * s.k.Es / pk.Rs;
-        pk.a0 = std::pow( 9.0 / 2.0 * s.k.Gamma * kPi / s.k.Es * std::pow( pk.Rs, 2.0 ), 1.0 / 3.0 );
-        pk.delta_tear = -1.0 / 2.0 * std::pow( 6.0, -1.0 / 3.0 ) * std::pow( pk.a0, 2.0 ) / pk.Rs;
+        pk.a0 = crThird( 9.0 / 2.0 * s.k.Gamma * kPi / s.k.Es * crSq( pk.Rs ) );
+        pk.delta_tear = -1.0 / 2.0 * std::pow( 6.0, -1.0 / 3.0 ) * crSq( pk.a0 ) / pk.Rs;
     }
     return pk;
 }
@@ -921,7 +1034,7 @@ double normalForce( const Sys& s, const PairK& pk, double ms, double r, double v
         const double delta_n = ( r - pk.two_r );
         double coeff = 0.0;
         if ( delta_n < 0.0 )
-            coeff = std::min( 0.0, -pk.coeff_h_n * std::pow( std::abs( delta_n ), 3.0 / 2.0 ) );
+            coeff = std::min( 0.0, -pk.coeff_h_n * crPow32( std::abs( delta_n ) ) );
         coeff /= vol;
         double Sn = 0.0;
         if ( delta_n < 0.0 )
@@ -935,8 +1048,8 @@ double normalForce( const Sys& s, const PairK& pk, double ms, double r, double v
     if ( delta_n >= pk.delta_tear )
     {
         auto a = jkrPatchRadius( s, pk, delta_n );
-        auto a3 = std::pow( a, 3.0 );
-        coeff = -1.0 * ( pk.coeff_h_n * a3 - std::pow( 8.0 * s.k.Gamma * kPi * s.k.Es * a3, 1.0 / 2.0 ) );
+        auto a3 = crCube( a );
+        coeff = -1.0 * ( pk.coeff_h_n * a3 - crHalf( 8.0 * s.k.Gamma * kPi * s.k.Es * a3 ) );
         double Sn = 2.0 * s.k.Es * a;
         coeff -= s.k.coeff_h_d * std::sqrt( Sn * ms ) * vn;
         if ( delta_n <= 0.0 && vn <= 0.0 )
@@ -1008,6 +1121,52 @@ void computeContactForceFull( Sys& s )
     }
 }
 
+// Contact list of a PD specimen on current positions (the Neighbor ctor with
+// a contact model, CabanaPD_Neighbor.hpp:60-64, and Neighbor::update,
+// :106-132): cutoff 2 radius + radius_extend (+ tol at the first build).
+void pdContactBuild( Sys& s, double tol )
+{
+    std::vector<double> y( 3 * s.n );
+    for ( int64_t i = 0; i < 3 * s.n; i++ )
+        y[i] = s.x[i] + s.u[i];
+    buildNeighbors( s, y, 2.0 * s.cd.radius + s.cd.radius_extend + tol, false, 0.0, s.cmaxn, s.ccnt, s.cnbr );
+    s.u_neigh = s.u;
+    s.max_disp = 0.0;
+}
+
+// Force<NormalRepulsionModel>::computeForceFull (force/CabanaPD_Contact.hpp:44-80)
+// with NormalRepulsionModel::forceCoeff (force_models/CabanaPD_Contact.hpp:82-89),
+// added to f (computeForce with reset = false, CabanaPD_Solver.hpp:364-366).
+void computeNormalRepulsion( Sys& s )
+{
+    if ( s.max_disp > s.cd.radius_extend )
+        pdContactBuild( s, 0.0 );
+    const double radius = s.cd.radius, delta = s.cd.horizon, c = s.crep;
+#pragma omp parallel for schedule( static )
+    for ( int64_t i = 0; i < s.n; i++ )
+    {
+        const int32_t* row = s.cnbr.data() + i * s.cmaxn;
+        for ( int32_t q = 0; q < s.ccnt[i]; q++ )
+        {
+            const int64_t j = row[q];
+            double fcx_i = 0.0, fcy_i = 0.0, fcz_i = 0.0;
+            double xi, r, st, rx, ry, rz;
+            getDistance( s, i, j, xi, r, st, rx, ry, rz );
+            if ( r < radius )
+            {
+                const double sc = ( r - radius ) / delta;
+                const double coeff = 15.0 * c * sc * s.vol[j];
+                fcx_i = coeff * rx / r;
+                fcy_i = coeff * ry / r;
+                fcz_i = coeff * rz / r;
+            }
+            s.f[3 * i + 0] += fcx_i;
+            s.f[3 * i + 1] += fcy_i;
+            s.f[3 * i + 2] += fcz_i;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Integrator (CabanaPD_Integrate.hpp:88-193).
 // ---------------------------------------------------------------------------
@@ -1015,7 +1174,9 @@ void initialHalfStep( Sys& s )
 {
     const double dt = s.d.dt;
     const double half_dt = 0.5 * dt;
-    const bool contact = isContact( s.d );
+    // VelocityVerlet<Contact> tracks the largest displacement since the last
+    // contact list build (DEM specimens and PD specimens with contact)
+    const bool contact = isContact( s.d ) || s.pdc;
     double max_u = 0.0;
 #pragma omp parallel for schedule( static ) reduction( max : max_u )
     for ( int64_t i = 0; i < s.n; i++ )
@@ -1432,24 +1593,88 @@ int orc_step( orc_sys s, int32_t nsteps )
             computeHeatTransfer( *s, s->d.thermal_subcycle * s->d.dt );
         applyPrescribed( *s, s->steps_done * s->d.dt );
         updateForce( *s );
+        if ( s->pdc )
+            computeNormalRepulsion( *s );
         applyForceTerms( *s );
         finalHalfStep( *s );
     }
     return 0;
 }
 
+int orc_set_contact( orc_sys s, const pdk_contact_desc* cd )
+{
+    if ( !s || !cd || isContact( s->d ) || !s->built || cd->model != PDK_CONTACT_NORMAL_REPULSION )
+        return 1;
+    s->pdc = true;
+    s->cd = *cd;
+    // NormalRepulsionModel ctor (force_models/CabanaPD_Contact.hpp:70-80)
+    s->crep = 18.0 * cd->K / ( kPi * cd->horizon * cd->horizon * cd->horizon * cd->horizon );
+    pdContactBuild( *s, s->d.neighbor_tol );
+    return 0;
+}
+
+int orc_cr_pow( int32_t which, int64_t n, const double* x, double* out )
+{
+    for ( int64_t i = 0; i < n; i++ )
+        out[i] = which == 0 ? crPow32( x[i] ) : which == 1 ? crThird( x[i] ) : which == 2 ? crCube( x[i] )
+                                                                                           : crHalf( x[i] );
+    return 0;
+}
+
+int orc_contact_pairs( orc_sys s, int64_t* pairs )
+{
+    int64_t t = 0;
+    if ( s->pdc )
+        for ( int32_t c : s->ccnt )
+            t += c;
+    *pairs = t;
+    return 0;
+}
+
+namespace
+{
+// Install a pair table of ntypes^2 models and the particle types.
+int installPairs( orc_sys s, int32_t ntypes, const pdk_pair_model* pairs, const int32_t* type, bool tables )
+{
+    for ( int32_t a = 0; a < ntypes; a++ )
+        for ( int32_t b = 0; b < ntypes; b++ )
+        {
+            const pdk_pair_model& p = pairs[a * ntypes + b];
+            s->pair_c[4 * a + b] = p.c;
+            s->pair_bcc[4 * a + b] = p.bond_break_coeff;
+            for ( int side = 0; side < 2; side++ )
+            {
+                s->pair_tc[4 * a + b][side] = p.theta_coeff[side];
+                s->pair_sc[4 * a + b][side] = p.s_coeff[side];
+            }
+        }
+    s->mtype.assign( s->n, 0 );
+    if ( tables )
+        for ( int64_t i = 0; i < s->n; i++ )
+        {
+            const int32_t t = type ? type[i] : 0;
+            if ( t < 0 || t >= ntypes )
+                return 1;
+            s->mtype[i] = t;
+        }
+    s->nmat = ntypes;
+    s->pair_tables = tables;
+    return 0;
+}
+} // namespace
+
 int orc_set_materials( orc_sys s, int32_t nmat, const pdk_material* mats, const int32_t* type )
 {
     // createMultiForceModel( particles, AverageTag, m_0, .., m_{nmat-1} )
-    // (CabanaPD_ForceModelsMulti.hpp:338-516): the pair (a, a) uses material
-    // a's model, a pair a != b the averaged model: PMB c = (c_a + c_b) / 2
-    // (force_models/CabanaPD_PMB.hpp:51-55), s0 = sqrt((s0_a^2 K_a + s0_b^2
-    // K_b) / (K_a + K_b)), bond_break_coeff = (1 + s0)^2
-    // (CabanaPD_ForceModels.hpp:114-121); LPS keeps each material's
-    // coefficients on its own particle's terms.
+    // (CabanaPD_ForceModelsMulti.hpp:360-502): the pair (a, a) uses material
+    // a's model, a pair a != b the model averaged from a's and b's: PMB c =
+    // (c_a + c_b) / 2 (force_models/CabanaPD_PMB.hpp:51-55), s0 = sqrt((s0_a^2
+    // K_a + s0_b^2 K_b) / (K_a + K_b)), bond_break_coeff = (1 + s0)^2
+    // (CabanaPD_ForceModels.hpp:114-121), LPS theta_coeff / s_coeff [0] from
+    // a's model and [1] from b's (force_models/CabanaPD_LPS.hpp:55-69).
     if ( !s || nmat < 1 || nmat > 4 || !mats || ( nmat > 1 && !type ) || isContact( s->d ) )
         return 1;
-    double K[4], c[4], s0[4], bcc[4];
+    double K[4], c[4], s0[4], bcc[4], tc[4], sc[4];
     for ( int32_t t = 0; t < nmat; t++ )
     {
         pdk_model_desc d = s->d;
@@ -1463,34 +1688,42 @@ int orc_set_materials( orc_sys s, int32_t nmat, const pdk_material* mats, const 
         c[t] = k.c;
         s0[t] = k.s0;
         bcc[t] = k.bond_break_coeff;
-        s->mat_tc[t] = k.theta_coeff;
-        s->mat_sc[t] = k.s_coeff;
+        tc[t] = k.theta_coeff;
+        sc[t] = k.s_coeff;
     }
+    pdk_pair_model pairs[16];
     for ( int32_t a = 0; a < nmat; a++ )
         for ( int32_t b = 0; b < nmat; b++ )
         {
+            pdk_pair_model& p = pairs[a * nmat + b];
             if ( a == b )
             {
-                s->pair_c[4 * a + b] = c[a];
-                s->pair_bcc[4 * a + b] = bcc[a];
+                p.c = c[a];
+                p.bond_break_coeff = bcc[a];
             }
             else
             {
-                s->pair_c[4 * a + b] = ( c[a] + c[b] ) / 2.0;
+                p.c = ( c[a] + c[b] ) / 2.0;
                 const double s0ab = std::sqrt( ( s0[a] * s0[a] * K[a] + s0[b] * s0[b] * K[b] ) / ( K[a] + K[b] ) );
-                s->pair_bcc[4 * a + b] = ( 1.0 + s0ab ) * ( 1.0 + s0ab );
+                p.bond_break_coeff = ( 1.0 + s0ab ) * ( 1.0 + s0ab );
             }
+            p.theta_coeff[0] = tc[a];
+            p.theta_coeff[1] = tc[b];
+            p.s_coeff[0] = sc[a];
+            p.s_coeff[1] = sc[b];
         }
-    s->nmat = nmat;
-    s->mtype.assign( s->n, 0 );
-    if ( nmat > 1 )
-        for ( int64_t i = 0; i < s->n; i++ )
-        {
-            if ( type[i] < 0 || type[i] >= nmat )
-                return 1;
-            s->mtype[i] = type[i];
-        }
-    return 0;
+    return installPairs( s, nmat, pairs, type, nmat > 1 );
+}
+
+int orc_set_pair_models( orc_sys s, int32_t ntypes, const pdk_pair_model* pairs, const int32_t* type )
+{
+    // createMultiForceModel( particles, indexing, m_0, .. )
+    // (CabanaPD_ForceModelsMulti.hpp:351-358): the bond (i, j) calls the model
+    // m_{indexing(type(i), type(j))} (:228-250), whose constants the caller
+    // has placed at pairs[type(i) * ntypes + type(j)].
+    if ( !s || ntypes < 1 || ntypes > 4 || !pairs || ( ntypes > 1 && !type ) || isContact( s->d ) )
+        return 1;
+    return installPairs( s, ntypes, pairs, type, true );
 }
 
 int orc_set_prescribed( orc_sys s, int32_t field, const int64_t* idx, int64_t count, const double* a,

@@ -46,6 +46,7 @@ int orc_compute_force(orc_sys s, int32_t reset);
 int orc_compute_energy(orc_sys s, double* W, double* D);
 int orc_compute_stress(orc_sys s);
 int orc_set_materials(orc_sys s, int32_t nmat, const pdk_material* mats, const int32_t* type);
+int orc_set_pair_models(orc_sys s, int32_t ntypes, const pdk_pair_model* pairs, const int32_t* type);
 int orc_set_prescribed(orc_sys s, int32_t field, const int64_t* idx, int64_t count, const double* a,
                        const double* b);
 int orc_set_field(orc_sys s, int32_t field, const double* values);
@@ -54,6 +55,11 @@ int orc_init(orc_sys s);
 int orc_step(orc_sys s, int32_t nsteps);
 int orc_get(orc_sys s, int32_t field, double* out);
 int orc_set_state(orc_sys s, const double* u, const double* v);
+int orc_set_contact(orc_sys s, const pdk_contact_desc* contact);
+int orc_contact_pairs(orc_sys s, int64_t* pairs);
+/* The correctly rounded powers of the contact models, elementwise:
+ * which = 0: x^1.5, 1: x^(double nearest 1/3), 2: x^3, 3: x^0.5. */
+int orc_cr_pow(int32_t which, int64_t n, const double* x, double* out);
 
 /* Single-bond kernels exposed for closed-form pins. */
 int orc_bond_pmb(const pdk_model_desc* desc, const double* xi3, const double* eta3, double vol_j,

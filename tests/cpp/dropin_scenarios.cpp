@@ -16,12 +16,16 @@
 //   inclusion       LPS fracture with a stiffer disc (two materials, averaged
 //                   bonds), no-fail bands and a traction pair; prints the last
 //                   crack-front value.
+//   inclusion-indexed  the same with the averaged models built by hand and
+//                   passed with FullIndexing<2> (explicit indexing).
 //   grips           ElasticPerfectlyPlastic PMB dogbone held by displacement
 //                   grips (force_update = false); writes u to <output>.
 //   heated          TemperatureDependent PMB, a temperature ramp imposed by a
 //                   BodyTerm (force = false); writes u and T.
 //   conduction      DynamicTemperature PMB with two planes held cold; writes u
 //                   and T.
+//   collide         two PMB fracture blocks that meet only through a contact
+//                   model (PD + NormalRepulsion Solver); writes u to <output>.
 //   rebound         Hertz contact of two spheres (DEM-only Solver); prints the
 //                   ratio of final to initial speed.
 //   pulloff         Hertz-JKR contact of two spheres stepped with runStep;
@@ -191,7 +195,7 @@ int pulse( const Args& a )
 }
 
 // ---------------------------------------------------------------------------
-int inclusion( const Args& a )
+int inclusionRun( const Args& a, bool indexed )
 {
     CabanaPD::Inputs deck( a.deck );
     const Vec3 lo = deck["low_corner"];
@@ -236,33 +240,42 @@ int inclusion( const Args& a )
         if ( x0( p, 1 ) <= band_lo || x0( p, 1 ) >= band_hi )
             keep( p ) = 1;
     } );
-    auto models = CabanaPD::createMultiForceModel( body, CabanaPD::AverageTag{}, plateModel, discModel );
-    CabanaPD::Solver solver( deck, body, models );
+    // the same model table either through AverageTag or spelled out for
+    // FullIndexing<2> with the averaged models built by hand
+    auto run = [&]( auto models ) {
+        CabanaPD::Solver solver( deck, body, models );
 
-    Box bottom( lo[0], hi[0], lo[1] - dy, lo[1] + dy, lo[2], hi[2] );
-    Box top( lo[0], hi[0], hi[1] - dy, hi[1] + dy, lo[2], hi[2] );
-    const double sigma = deck["traction"];
-    auto f = solver.particles.sliceForce();
-    auto xs = solver.particles.sliceReferencePosition();
-    // +-sigma / dy on the face layers, pulling the faces apart
-    auto pull = [=]( const int p, const double ) { f( p, 1 ) += std::copysign( sigma / dy, xs( p, 1 ) ); };
-    auto traction = CabanaPD::createBoundaryCondition( pull, ExecSpace{}, solver.particles, true, bottom, top );
+        Box bottom( lo[0], hi[0], lo[1] - dy, lo[1] + dy, lo[2], hi[2] );
+        Box top( lo[0], hi[0], hi[1] - dy, hi[1] + dy, lo[2], hi[2] );
+        const double sigma = deck["traction"];
+        auto f = solver.particles.sliceForce();
+        auto xs = solver.particles.sliceReferencePosition();
+        // +-sigma / dy on the face layers, pulling the faces apart
+        auto pull = [=]( const int p, const double ) { f( p, 1 ) += std::copysign( sigma / dy, xs( p, 1 ) ); };
+        auto traction = CabanaPD::createBoundaryCondition( pull, ExecSpace{}, solver.particles, true, bottom, top );
 
-    // one notch at mid height from the low-x face, half the length
-    const double length = deck["system_size"][0];
-    const double thick = deck["system_size"][2];
-    const double ymid = 0.5 * ( lo[1] + hi[1] );
-    CabanaPD::Prenotch<1> notch( Vec3{ length / 2.0, 0.0, 0.0 }, Vec3{ 0.0, 0.0, thick },
-                                 { Vec3{ lo[0], ymid, lo[2] } } );
-    auto dmg = solver.particles.sliceDamage();
-    auto front = [=]( const int p, const double ) { return dmg( p ) > 0.3 ? xs( p, 1 ) : 0.0; };
-    auto series = CabanaPD::createOutputTimeSeries<Kokkos::Max<double>>( a.out, deck, ExecSpace{},
-                                                                         solver.particles, front, Box( lo, hi ) );
-    solver.init( traction, notch );
-    solver.run( traction, series );
-    print( "steps", solver.num_steps );
-    print( "crack_extent", lastValueIn( a.out ) );
-    return 0;
+        // one notch at mid height from the low-x face, half the length
+        const double length = deck["system_size"][0];
+        const double thick = deck["system_size"][2];
+        const double ymid = 0.5 * ( lo[1] + hi[1] );
+        CabanaPD::Prenotch<1> notch( Vec3{ length / 2.0, 0.0, 0.0 }, Vec3{ 0.0, 0.0, thick },
+                                     { Vec3{ lo[0], ymid, lo[2] } } );
+        auto dmg = solver.particles.sliceDamage();
+        auto front = [=]( const int p, const double ) { return dmg( p ) > 0.3 ? xs( p, 1 ) : 0.0; };
+        auto series = CabanaPD::createOutputTimeSeries<Kokkos::Max<double>>( a.out, deck, ExecSpace{},
+                                                                             solver.particles, front, Box( lo, hi ) );
+        solver.init( traction, notch );
+        solver.run( traction, series );
+        print( "steps", solver.num_steps );
+        print( "crack_extent", lastValueIn( a.out ) );
+        return 0;
+    };
+    if ( !indexed )
+        return run( CabanaPD::createMultiForceModel( body, CabanaPD::AverageTag{}, plateModel, discModel ) );
+    CabanaPD::ForceModel plateDisc( plateModel, discModel );
+    CabanaPD::ForceModel discPlate( discModel, plateModel );
+    return run( CabanaPD::createMultiForceModel( body, CabanaPD::FullIndexing<2>{}, plateModel, plateDisc, discPlate,
+                                                 discModel ) );
 }
 
 // ---------------------------------------------------------------------------
@@ -427,6 +440,75 @@ int conduction( const Args& a )
 }
 
 // ---------------------------------------------------------------------------
+// Positions and volumes of a custom particle set (Particles::create(x, vol)).
+struct PointSet
+{
+    std::vector<double> xyz, vol;
+    double operator()( const int p, const int d ) const { return xyz[3 * p + d]; }
+    double operator()( const int p ) const { return vol[p]; }
+    std::size_t size() const { return vol.size(); }
+};
+
+// One particle at the centre of each cell of the box [lo, hi] split into n cells.
+void addBlock( PointSet& ps, const Vec3& lo, const Vec3& hi, const std::array<int, 3>& n )
+{
+    const Vec3 d = { ( hi[0] - lo[0] ) / n[0], ( hi[1] - lo[1] ) / n[1], ( hi[2] - lo[2] ) / n[2] };
+    for ( int k = 0; k < n[2]; k++ )
+        for ( int j = 0; j < n[1]; j++ )
+            for ( int i = 0; i < n[0]; i++ )
+            {
+                ps.xyz.push_back( lo[0] + i * d[0] + 0.5 * d[0] );
+                ps.xyz.push_back( lo[1] + j * d[1] + 0.5 * d[1] );
+                ps.xyz.push_back( lo[2] + k * d[2] + 0.5 * d[2] );
+                ps.vol.push_back( d[0] * d[1] * d[2] );
+            }
+}
+
+int collide( const Args& a )
+{
+    CabanaPD::Inputs deck( a.deck );
+    const Vec3 dxv = deck["dx"];
+    const double dx = dxv[0];
+    const std::array<int, 3> cells = deck["block_cells"];
+    const double gap = deck["gap_cells"];
+    const double speed = deck["impact_speed"];
+    const double K = deck["bulk_modulus"];
+    const double rho0 = deck["density"];
+    const double sc = deck["critical_stretch"];
+    const double horizon = deck["horizon"];
+    const double cfac = deck["contact_horizon_factor"];
+    const double efac = deck["contact_horizon_extend_factor"];
+    // G0 of the PMB relation s_c = sqrt(5 G0 / (9 K delta))
+    const double G0 = 9.0 * K * horizon * ( sc * sc ) / 5.0;
+    CabanaPD::ForceModel model( CabanaPD::PMB{}, horizon, K, G0 );
+    CabanaPD::NormalRepulsionModel contact( horizon, cfac * dx / 2.0, efac * dx, K );
+
+    // block A on the left, block B on the right, gap cell spacings apart
+    const double lx = cells[0] * dx;
+    PointSet ps;
+    addBlock( ps, { -lx - 0.5 * gap * dx, 0.0, 0.0 }, { -0.5 * gap * dx, cells[1] * dx, cells[2] * dx }, cells );
+    const std::size_t na = ps.size();
+    addBlock( ps, { 0.5 * gap * dx, 0.0, 0.0 }, { lx + 0.5 * gap * dx, cells[1] * dx, cells[2] * dx }, cells );
+    CabanaPD::Particles blocks( MemSpace{}, CabanaPD::NormalRepulsionModel{} );
+    blocks.domain( deck["low_corner"], deck["high_corner"], deck["num_cells"], ghostLayers( deck, horizon ) );
+    blocks.create( ExecSpace{}, ps, ps );
+    auto rho = blocks.sliceDensity();
+    auto vel = blocks.sliceVelocity();
+    blocks.update( ExecSpace{}, [=]( const int p ) {
+        rho( p ) = rho0;
+        vel( p, 0 ) = static_cast<std::size_t>( p ) < na ? speed : -speed;
+    } );
+    CabanaPD::Solver solver( deck, blocks, model, contact );
+    solver.init();
+    solver.run();
+    auto u = solver.particles.sliceDisplacement();
+    writeDoubles( a.out, &u( 0, 0 ), 3 * u.size() );
+    print( "particles", static_cast<double>( u.size() ) );
+    print( "steps", solver.num_steps );
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Two equal spheres on the x axis at +-gap, moving toward each other at 1 m/s.
 struct TwoSpheres
 {
@@ -507,12 +589,22 @@ int pulloff( const Args& a )
 }
 } // namespace
 
+int inclusion( const Args& a )
+{
+    return inclusionRun( a, false );
+}
+int inclusionIndexed( const Args& a )
+{
+    return inclusionRun( a, true );
+}
+
 int main( int argc, char* argv[] )
 {
     const std::map<std::string, std::function<int( const Args& )>> scenarios = {
         { "notched-impact", notchedImpact }, { "pulse", pulse },   { "inclusion", inclusion },
+        { "inclusion-indexed", inclusionIndexed },
         { "grips", grips },                  { "heated", heated }, { "conduction", conduction },
-        { "rebound", rebound },              { "pulloff", pulloff } };
+        { "rebound", rebound },              { "pulloff", pulloff },    { "collide", collide } };
     if ( argc < 4 || !scenarios.count( argv[1] ) )
     {
         std::fprintf( stderr, "usage: %s <scenario> <deck.json> <output>\n", argv[0] );

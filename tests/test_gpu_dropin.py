@@ -116,6 +116,20 @@ def test_dropin_inclusion_regression(tmp_path):
     assert abs(vals["crack_extent"] - 5.75e-3) <= 5e-5
 
 
+def test_dropin_inclusion_explicit_indexing_equals_average_tag(tmp_path):
+    """createMultiForceModel(particles, FullIndexing<2>{}, m0, m01, m10, m1)
+    with the averaged models built by hand (ForceModel(m_a, m_b)) runs the
+    inclusion deck exactly as createMultiForceModel(particles, AverageTag{},
+    m0, m1): the crack-front time series files are identical."""
+    deck = os.path.join(HERE, "golden", "crack_inclusion.json")
+    out_a, out_b = tmp_path / "avg.txt", tmp_path / "idx.txt"
+    a = scenario("inclusion", deck, str(out_a), str(tmp_path))
+    b = scenario("inclusion-indexed", deck, str(out_b), str(tmp_path))
+    say(f"  crack extent {a['crack_extent']!r} / {b['crack_extent']!r}")
+    assert out_a.read_bytes() == out_b.read_bytes()
+    assert a["crack_extent"] == b["crack_extent"]
+
+
 def _deck_variant(tmp_path, name, **changes):
     import json
     deck = json.load(open(os.path.join(HERE, "golden", name)))
@@ -219,3 +233,33 @@ def test_dropin_jkr_pulloff(tmp_path):
     vals = scenario("pulloff", deck, str(tmp_path / "unused"), str(tmp_path))
     say(f"  pull-off ratio {vals['pulloff_ratio']!r} after {vals['steps']:.0f} steps")
     assert abs(vals["pulloff_ratio"] + 1.0) <= 5e-3
+
+
+def test_dropin_collide_equals_engine(tmp_path):
+    """A PD force model and a NormalRepulsion contact model in one Solver
+    (the four-argument constructor) on our colliding-blocks deck
+    (tests/golden/colliding_blocks.json): the final displacement equals the
+    engine run of the same specimen with pdk_set_contact, bit for bit."""
+    import json
+
+    import numpy as np
+
+    from cabanapd_amd import configs
+    from tests.parity import gpu_spec
+    deck = os.path.join(HERE, "golden", "colliding_blocks.json")
+    out = tmp_path / "blocks_u.bin"
+    vals = scenario("collide", deck, str(out), str(tmp_path))
+    w = configs.colliding_blocks(steps=int(vals["steps"]))
+    d = json.load(open(deck))
+    assert d["timestep"]["value"] == w["desc"].dt and d["horizon"]["value"] == w["desc"].horizon
+    u_drop = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
+    assert u_drop.shape == w["x"].shape
+    g = gpu_spec(w)
+    configs.setup(g, w)
+    g.step(w["steps"])
+    u = g.get("u")
+    nd = int(np.sum(u_drop != u))
+    say(f"  drop-in vs engine after {w['steps']} steps: differing={nd} max|u|={np.max(np.abs(u)):.3e} "
+        f"contact pairs {g.contact_pairs()}")
+    assert nd == 0
+    g.close()

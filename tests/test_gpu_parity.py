@@ -402,6 +402,82 @@ def test_gpu_multi_material_rejects_mixed():
     s.close()
 
 
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_gpu_pair_models_bitwise(model):
+    """createMultiForceModel(particles, indexing, models...): three material
+    types scattered over a fracturing cube.  PMB: a BinaryIndexing-style
+    table (one model for like pairs, an independent stiffer, tougher one for
+    unlike pairs); LPS: the FullIndexing table of three materials and their
+    averages.  Bonds, u, v, f, W, phi and stress equal the oracle's bit for bit
+    after 20 steps."""
+    from tests.test_oracle_pair_models import pair as pm
+    say(f"case: {model} explicit pair table, three types, cube 11^3, 20 steps")
+    w = configs.lattice_sum_cube(model=model, m=3, nc=11, field="linear", coeff=0.0, K=1e9, G=0.5e9,
+                                 fracture=True, G0=1.0)
+    x = w["x"]
+    rng = np.random.default_rng(6)
+    w["u"] = rng.uniform(-2e-3, 2e-3, size=x.shape) * (2.0 / 11)
+    w["rho"] = np.full(x.shape[0], 1000.0)
+    w["desc"].dt = 1e-6
+    mtype = rng.integers(0, 3, x.shape[0]).astype(np.int32)
+    mats = [(1e9, 0.5e9, 1.0), (3e9, 1.2e9, 4.0), (2e9, 0.8e9, 2.0)]
+    if model == "PMB":
+        from oracle import binding as orc
+        k = orc.model_constants(w["desc"])
+        like = pm(c=k["c"], bcc=k["bond_break_coeff"])
+        unlike = pm(c=1.5 * k["c"], bcc=(1.0 + 2.0 * k["s0"]) ** 2)
+        table = [like if a == b else unlike for a in range(3) for b in range(3)]
+    else:
+        table = averaged_pairs_for(w["desc"], mats)
+    w["pair_models"] = (table, mtype)
+    g, o = pair(w)
+    g.step(20)
+    o.step(20)
+    _full_compare(g, o)
+    say(f"  broken bonds {len(broken_set(g))}")
+    assert len(broken_set(g)) > 0
+
+
+def averaged_pairs_for(desc, mats):
+    """The averaged FullIndexing table of mats for this spec's horizon and influence."""
+    import math
+    from oracle import binding as orc
+    from tests.test_oracle_pair_models import pair as pm
+    ks = []
+    for (K, G, G0) in mats:
+        d = type(desc).from_buffer_copy(desc)
+        d.K, d.G, d.G0 = K, G, G0
+        ks.append((K, G, orc.model_constants(d)))
+    out = []
+    for a in range(len(mats)):
+        for b in range(len(mats)):
+            (Ka, Ga, ka), (Kb, Gb, kb) = ks[a], ks[b]
+            if a == b:
+                c, bcc = ka["c"], ka["bond_break_coeff"]
+            else:
+                c = (ka["c"] + kb["c"]) / 2.0
+                s0 = math.sqrt((ka["s0"] * ka["s0"] * Ka + kb["s0"] * kb["s0"] * Kb) / (Ka + Kb))
+                bcc = (1.0 + s0) * (1.0 + s0)
+            out.append(pm(c=c, bcc=bcc, tc=(3.0 * Ka - 5.0 * Ga, 3.0 * Kb - 5.0 * Gb), sc=(15.0 * Ga, 15.0 * Gb)))
+    return out
+
+
+def test_gpu_pair_models_rejects_unfoldable_lps():
+    """An LPS pair table whose i-term coefficients depend on the partner's
+    type cannot be folded into per-particle terms: refused with an error."""
+    from cabanapd_amd import engine
+    from tests.test_oracle_pair_models import pair as pm
+    w = configs.lattice_sum_cube(model="LPS", m=3, nc=7, field="linear", coeff=0.0, K=1e9, G=0.5e9,
+                                 fracture=True, G0=1.0)
+    s = engine.Spec(ctx(), w["desc"], w["x"].shape[0])
+    configs.upload(s, w)
+    same = pm(c=0.0, bcc=1.1, tc=(1.0, 1.0), sc=(2.0, 2.0))
+    odd = pm(c=0.0, bcc=1.1, tc=(3.0, 1.0), sc=(2.0, 2.0))
+    with pytest.raises(engine.PdkError, match="own type"):
+        s.set_pair_models([same, odd, odd, same], np.zeros(w["x"].shape[0], dtype=np.int32))
+    s.close()
+
+
 def _full_compare(g, o, fields=("u", "v", "f")):
     assert broken_set(g) == broken_set(o)
     for fld in fields:

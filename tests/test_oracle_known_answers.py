@@ -418,7 +418,7 @@ def test_stress_two_particle_closed_form():
     assert np.count_nonzero(sig[:, 1:, 1:]) == 0
 
 
-def multi_material_run(model, mats, mtype, steps=20):
+def multi_material_run(model, mats, mtype, steps=20, pairs=None):
     w = configs.lattice_sum_cube(model=model, m=3, nc=9, field="linear", coeff=0.0, K=1e9, G=0.5e9,
                                  fracture=True, G0=1.0)
     x = w["x"]
@@ -428,6 +428,8 @@ def multi_material_run(model, mats, mtype, steps=20):
     s.build_bonds()
     if mats is not None:
         s.set_materials(mats, mtype)
+    if pairs is not None:
+        s.set_pair_models(pairs, mtype)
     w["desc"].dt = 1e-6
     s.init()
     s.step(steps)
