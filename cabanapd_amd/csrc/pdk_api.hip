@@ -669,6 +669,55 @@ int pdk_critical_timestep( double dx, double dy, double dz, double K, double rho
     } );
 }
 
+} // extern "C"
+
+// ---- fixed device scratch of pdk_compute_energy_noalloc ------------------------
+// Per-block partial totals of the energy pass (2 doubles per block of 4
+// slices) followed by the two final totals.  Every GPU the library runs on
+// holds one copy until the process ends; all specs on that GPU share it.
+namespace
+{
+constexpr int64_t kNoAllocMaxBlocks = 65536; // up to 65536 * 256 = 16,777,216 particles
+constexpr int64_t kNoAllocScratchDoubles = 2 * kNoAllocMaxBlocks + 2;
+static __device__ __align__( 256 ) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::once_flag g_scratch_once[kMaxDevices];
+
+// Device address of this GPU's copy of scratch, looked up once per device
+// with that device current (the host-side name is not a device address).
+void initScratch( int device )
+{
+    require( device >= 0 && device < kMaxDevices, "device index beyond the scratch table" );
+    std::call_once( g_scratch_once[device], [device] {
+        void* p = nullptr;
+        PDK_HIP( hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) ) );
+        g_scratch_ptr[device] = static_cast<double*>( p );
+    } );
+    require( g_scratch_ptr[device] != nullptr, "scratch address lookup failed" );
+}
+
+// Sums the per-block partials in block order -- the order of the host loop of
+// pdk_compute_energy -- so both entry points return the same bits.
+__global__ void kSumPartialsInOrder( double* __restrict__ part, int64_t nblocks )
+{
+#pragma clang fp contract( off )
+    if ( threadIdx.x != 0 || blockIdx.x != 0 )
+        return;
+    double Phi = 0.0, D = 0.0;
+    for ( int64_t b = 0; b < nblocks; b++ )
+    {
+        Phi += part[2 * b];
+        D += part[2 * b + 1];
+    }
+    part[2 * nblocks] = Phi;
+    part[2 * nblocks + 1] = D;
+}
+} // namespace
+
+extern "C" {
+
 int pdk_ctx_create( int device, uint64_t hbm_budget, pdk_ctx* out )
 {
     return guarded( [&] {
@@ -677,6 +726,7 @@ int pdk_ctx_create( int device, uint64_t hbm_budget, pdk_ctx* out )
         PDK_HIP( hipGetDeviceCount( &count ) );
         require( device >= 0 && device < count, "no such HIP device" );
         PDK_HIP( hipSetDevice( device ) );
+        initScratch( device );
         auto c = std::make_unique<pdk_ctx_s>();
         c->device = device;
         size_t freeb = 0, total = 0;
@@ -1588,6 +1638,39 @@ int pdk_compute_stress( pdk_spec s )
         }
         s->timed( kClsOther, [&] { launchStress( s->dargs(), s->h, s->cur, s->st ); } );
         PDK_HIP( hipStreamSynchronize( s->st ) );
+    } );
+}
+
+int pdk_compute_energy_noalloc( pdk_spec s, double* total_strain_energy, double* total_damage )
+{
+    return guarded( [&] {
+        require( s != nullptr, "null spec" );
+        // blocks of the energy pass: kSlicesPerBlock slices of kSlice particles
+        const int64_t nb = ( s->nslices + kSlicesPerBlock - 1 ) / kSlicesPerBlock;
+        if ( nb > kNoAllocMaxBlocks )
+            throw Error( "pdk_compute_energy_noalloc: " + std::to_string( s->n ) +
+                         " particles exceed the fixed scratch (at most " +
+                         std::to_string( kNoAllocMaxBlocks * kBlock ) + ")" );
+        require( s->built, "bonds not built" );
+        require( !s->in_batch, "spec is running in the batch engine" );
+        require( nb == energyBlocks( s->h ), "energy pass block count mismatch" );
+        double tot[2] = { 0.0, 0.0 };
+        if ( !isContact( s->d ) )
+        {
+            double* part = g_scratch_ptr[s->ctx->device];
+            require( part != nullptr, "scratch not initialised for this device" );
+            // every block writes its own pair of partials, then the in-order
+            // sum writes the totals: nothing has to start at zero
+            launchEnergy( s->dargs(), s->h, s->cur, part, nb, s->st );
+            hipLaunchKernelGGL( kSumPartialsInOrder, dim3( 1 ), dim3( 64 ), 0, s->st, part, nb );
+            PDK_HIP( hipGetLastError() );
+            PDK_HIP( hipMemcpyAsync( tot, part + 2 * nb, 2 * sizeof( double ), hipMemcpyDeviceToHost, s->st ) );
+            PDK_HIP( hipStreamSynchronize( s->st ) );
+        }
+        if ( total_strain_energy )
+            *total_strain_energy = tot[0];
+        if ( total_damage )
+            *total_damage = s->d.fracture ? tot[1] : 0.0;
     } );
 }
 

@@ -69,6 +69,7 @@ _SIGS = {
     "pdk_series_values": (C.c_int, [_VP, C.c_int64, _D, _D, _I64]),
     "pdk_series_gather": (C.c_int, [_VP, C.c_int64, _I64, _D, _I64]),
     "pdk_compute_energy_async": (C.c_int, [_VP]),
+    "pdk_compute_energy_noalloc": (C.c_int, [_VP, _D, _D]),
     "pdk_compute_stress": (C.c_int, [_VP]),
     "pdk_set_materials": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.Material), _I32]),
     "pdk_set_pair_models": (C.c_int, [_VP, C.c_int32, C.POINTER(_abi.PairModel), _I32]),
@@ -280,6 +281,13 @@ class Spec:
         arr = _abi.pair_models_array(pairs)
         t = None if mtype is None else np.ascontiguousarray(mtype, dtype=np.int32)
         _check(lib().pdk_set_pair_models(self.h, nt, arr, None if t is None else t.ctypes.data_as(_I32)))
+
+    def compute_energy_noalloc(self):
+        """pdk_compute_energy through the library's fixed scratch (no allocation)."""
+        W = C.c_double()
+        D = C.c_double()
+        _check(lib().pdk_compute_energy_noalloc(self.h, C.byref(W), C.byref(D)))
+        return W.value, D.value
 
     def compute_energy_async(self):
         _check(lib().pdk_compute_energy_async(self.h))

@@ -405,6 +405,16 @@ int pdk_series_gather(pdk_series series, int64_t capacity, int64_t* ids, double*
 /* computeEnergy (CabanaPD_Force.hpp:200-219) without reading the totals
  * back: refreshes W and damage on the device for the series that follow. */
 int pdk_compute_energy_async(pdk_spec spec);
+/* No counterpart in the reference.  pdk_compute_energy for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same W, damage and totals, bit
+ * for bit, with the per-block partial sums in a fixed device array of the
+ * library instead of the spec's own buffer.  The call allocates nothing.
+ * Limit: 16,777,216 particles per spec (a larger spec fails with an error).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two host threads, two streams) race and must be serialised by the
+ * caller. */
+int pdk_compute_energy_noalloc(pdk_spec spec, double* total_strain_energy, double* total_damage);
 
 /* ---- tuning --------------------------------------------------------------------
  * Force-pass variant: unroll = 1, 2, 4 or 8 gathers the neighbor data of that
