@@ -595,9 +595,11 @@ struct pdk_spec_s
         q.rc = 2.0 * pdc.cd.radius + pdc.cd.radius_extend + tol;
         q.extend = pdc.cd.radius_extend;
         q.n = n;
+        // after the first build: one capped pass (see contactBuildImpl)
+        const int32_t cap = tol > 0.0 || pdc.max_cnt <= 0 ? 0 : ( ( pdc.max_cnt + 4 + 7 ) / 8 ) * 8;
         pdc.total_rows = buildSlicedCsr( q, perm.as<int64_t>(), inv.as<int64_t>(), pdc.slice_ptr.as<int64_t>(),
                                          pdc.cnt.as<int32_t>(), pdc.col, pdc.intact, pdc.rows_cap, pdc.scratch, acct(),
-                                         st, &pdc.max_cnt );
+                                         st, &pdc.max_cnt, cap );
         PDK_HIP( hipMemcpyAsync( pdc.u_neigh.as<double>(), Ucur(), 3 * n * sizeof( double ), hipMemcpyDeviceToDevice,
                                  st ) );
     }
@@ -2194,9 +2196,12 @@ void contactBuildImpl( pdk_spec s, bool initial, double* max_disp, int64_t* sinc
     q.extend = s->d.radius_extend + tol;
     q.n = s->n;
     q.rmax = s->radius_max;
+    // after the first build: one capped pass sized from the last list's
+    // largest count (the counting path if a particle outgrows it)
+    const int32_t cap = initial || s->max_cnt <= 0 ? 0 : ( ( s->max_cnt + 4 + 7 ) / 8 ) * 8;
     s->total_rows = buildSlicedCsr( q, s->perm.as<int64_t>(), s->inv.as<int64_t>(), s->slice_ptr.as<int64_t>(),
                                     s->cnt.as<int32_t>(), s->col, s->intact, s->col_rows_cap, s->scratch, s->acct(),
-                                    s->st, &s->max_cnt );
+                                    s->st, &s->max_cnt, cap );
     PDK_HIP( hipMemcpyAsync( s->u_neigh.as<double>(), s->Ucur(), 3 * s->n * sizeof( double ), hipMemcpyDeviceToDevice,
                              s->st ) );
     s->stats_dirty = true;

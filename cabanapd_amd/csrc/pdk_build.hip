@@ -261,16 +261,22 @@ __global__ __launch_bounds__( kThreads ) void kCountNeighbors( NeighborQuery q, 
 // in the output column itself.
 constexpr int kFillLds = 128;
 
-// One wavefront (one workgroup) per slice.
+// One wavefront (one workgroup) per slice.  CAPPED: the single-pass form of
+// the contact lists -- every slice has the same number of rows (the capacity),
+// the pass counts the neighbors itself (cnt, the slice maxima in slice_len)
+// and raises *overflow when a particle has more than fit.
+template <bool CAPPED>
 __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, CellGrid g,
                                                             const int32_t* __restrict__ cell_start,
                                                             const int32_t* __restrict__ list,
                                                             const int64_t* __restrict__ perm,
                                                             const int64_t* __restrict__ inv,
                                                             const int64_t* __restrict__ slice_ptr,
-                                                            const int32_t* __restrict__ cnt,
+                                                            int32_t* __restrict__ cnt,
                                                             int32_t* __restrict__ col,
-                                                            uint64_t* __restrict__ intact )
+                                                            uint64_t* __restrict__ intact,
+                                                            int64_t* __restrict__ slice_len,
+                                                            int32_t* __restrict__ overflow )
 {
     __shared__ int32_t keys[kFillLds * kSlice];
     const int lane = threadIdx.x;
@@ -281,12 +287,16 @@ __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, Cel
     int32_t* row = col + base * kSlice + lane;
     const bool valid = p < q.n;
     int32_t c = 0;
+    bool over = false;
     if ( valid && len <= kFillLds )
     {
         int32_t* mine = keys + lane;
         visitNeighbors( q, g, cell_start, list, p, [&]( int64_t j ) {
             if ( c >= len )
-                return; // cannot happen: len is the slice's maximum count
+            {
+                over = true; // CAPPED only: len is otherwise the slice's maximum count
+                return;
+            }
             mine[c * kSlice] = static_cast<int32_t>( perm[j] );
             c++;
         } );
@@ -304,7 +314,10 @@ __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, Cel
         // Collect caller ids, insertion-sort them, map back to sorted ids.
         visitNeighbors( q, g, cell_start, list, p, [&]( int64_t j ) {
             if ( c >= len )
-                return; // cannot happen: len is the slice's maximum count
+            {
+                over = true;
+                return;
+            }
             const int32_t key = static_cast<int32_t>( perm[j] );
             int32_t k = c;
             while ( k > 0 && row[( k - 1 ) * kSlice] > key )
@@ -321,7 +334,22 @@ __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, Cel
     const int32_t self = static_cast<int32_t>( valid ? p : slice * kSlice );
     for ( int32_t k = c; k < len; k++ )
         row[k * kSlice] = self;
-    const int32_t mycnt = valid ? cnt[p] : 0;
+    int32_t mycnt = 0;
+    if constexpr ( CAPPED )
+    {
+        mycnt = c;
+        if ( valid )
+            cnt[p] = c;
+        int32_t m = c;
+        for ( int off = 32; off > 0; off >>= 1 )
+            m = max( m, __shfl_xor( m, off ) );
+        if ( lane == 0 )
+            slice_len[slice] = m;
+        if ( __ballot( over ) && lane == 0 )
+            *overflow = 1;
+    }
+    else
+        mycnt = valid ? cnt[p] : 0;
     for ( int32_t k = 0; k < len; k++ )
     {
         const uint64_t bits = __ballot( k < mycnt );
@@ -725,10 +753,18 @@ void launchGatherFlags( const int32_t* src, const int64_t* perm, uint8_t* dst, i
     PDK_HIP( hipGetLastError() );
 }
 
+// slice_ptr[s] = s * cap for s = 0 .. nslices (the capped single-pass lists)
+__global__ void kSliceRamp( int64_t* __restrict__ slice_ptr, int64_t nslices, int32_t cap )
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( s <= nslices )
+        slice_ptr[s] = s * cap;
+}
+
 int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64_t* inv,
                         int64_t* slice_ptr, int32_t* cnt, DevBuf& col, DevBuf& intact,
                         int64_t& col_capacity_rows, BuildScratch& s, MemAccount* acct,
-                        hipStream_t st, int32_t* max_cnt_out )
+                        hipStream_t st, int32_t* max_cnt_out, int32_t cap_hint )
 {
     const int64_t n = q.n;
     const int64_t nslices = ( n + kSlice - 1 ) / kSlice;
@@ -791,8 +827,48 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
     kCellFill<<<gridFor( n ), kThreads, 0, st>>>( s.cell.as<int32_t>(), n, s.cell_fill.as<int32_t>(),
                                                   s.cell_list.as<int32_t>() );
     PDK_HIP( hipGetLastError() );
-    // Counts and slot rows per slice.
+    // Single pass into cap_hint rows per slice (contact lists, whose counts
+    // change little between rebuilds); a particle with more neighbors than
+    // fit sends the build to the counting path below.
     ensure<int64_t>( s.slice_len, nslices + 1, acct );
+    if ( cap_hint > 0 && cap_hint <= kFillLds )
+    {
+        const int64_t rows = nslices * cap_hint;
+        if ( rows > col_capacity_rows || !col )
+        {
+            col.alloc( static_cast<size_t>( rows ) * kSlice * sizeof( int32_t ), acct );
+            intact.alloc( static_cast<size_t>( rows ) * sizeof( uint64_t ), acct );
+            col_capacity_rows = rows;
+        }
+        ensure<int32_t>( s.flag, 1, acct );
+        PDK_HIP( hipMemsetAsync( s.flag.as<int32_t>(), 0, sizeof( int32_t ), st ) );
+        kSliceRamp<<<static_cast<unsigned>( ( nslices + 1 + kThreads - 1 ) / kThreads ), kThreads, 0, st>>>(
+            slice_ptr, nslices, cap_hint );
+        PDK_HIP( hipGetLastError() );
+        kFillNeighbors<true><<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>(
+            q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), perm, inv, slice_ptr, cnt, col.as<int32_t>(),
+            intact.as<uint64_t>(), s.slice_len.as<int64_t>(), s.flag.as<int32_t>() );
+        PDK_HIP( hipGetLastError() );
+        int32_t over = 0;
+        PDK_HIP( hipMemcpyAsync( &over, s.flag.as<int32_t>(), sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
+        PDK_HIP( hipStreamSynchronize( st ) );
+        if ( !over )
+        {
+            if ( max_cnt_out )
+            {
+                std::vector<int64_t> sl( nslices );
+                PDK_HIP( hipMemcpyAsync( sl.data(), s.slice_len.as<int64_t>(), nslices * sizeof( int64_t ),
+                                         hipMemcpyDeviceToHost, st ) );
+                PDK_HIP( hipStreamSynchronize( st ) );
+                int64_t m = 0;
+                for ( int64_t v : sl )
+                    m = std::max( m, v );
+                *max_cnt_out = static_cast<int32_t>( m );
+            }
+            return rows;
+        }
+    }
+    // Counts and slot rows per slice.
     PDK_HIP( hipMemsetAsync( s.slice_len.as<int64_t>(), 0, ( nslices + 1 ) * sizeof( int64_t ), st ) );
     const unsigned gslices = static_cast<unsigned>( ( nslices * kSlice + kThreads - 1 ) / kThreads );
     kCountNeighbors<<<gslices, kThreads, 0, st>>>( q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), cnt,
@@ -814,9 +890,9 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
         intact.alloc( static_cast<size_t>( cap ) * sizeof( uint64_t ), acct );
         col_capacity_rows = cap;
     }
-    kFillNeighbors<<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>( q, g, s.cell_start.as<int32_t>(),
-                                                                          s.cell_list.as<int32_t>(), perm, inv, slice_ptr,
-                                                                          cnt, col.as<int32_t>(), intact.as<uint64_t>() );
+    kFillNeighbors<false><<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>(
+        q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), perm, inv, slice_ptr, cnt, col.as<int32_t>(),
+        intact.as<uint64_t>(), nullptr, nullptr );
     PDK_HIP( hipGetLastError() );
     if ( max_cnt_out )
     {

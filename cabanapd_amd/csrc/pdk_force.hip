@@ -406,6 +406,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     if ( !active )
         return;
     const PDArgs& A = *w.A;
+    const double* __restrict__ sxiv = scalarLoad( &A.stencil_xiv );
     const int lane = threadIdx.x & ( kSlice - 1 );
     const int64_t n = A.n;
     const int64_t p = w.slice * kSlice + lane;
@@ -577,9 +578,10 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     mj[k] = mt[j];
                 if ( tmp )
                     tj[k] = tmp[j];
-                // the mixed form needs x_j for the bond vector r = y_j - y_i
-                // (only the linearized models take xi from the stencil)
-                if ( PREC == kF64 || !useSten || !kIsLin<MODEL> )
+                // x_j: the FP64 form and the layouts without a stencil; the
+                // stencil-slot mixed forms take xi from the stencil (the
+                // guard band's FP64 retest loads x_j itself)
+                if ( PREC == kF64 || !useSten || ( !kIsLin<MODEL> && !SLOTS ) )
                 {
                     xj[k][0] = x[j];
                     xj[k][1] = x[n + j];
@@ -679,23 +681,33 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                         // (as in the row-window pass) or x_j - x_i
                         double xi2d;
                         float xil;
-                        double g0 = xj[k][0] - xi0, g1 = xj[k][1] - xi1, g2 = xj[k][2] - xi2;
+                        double g0, g1, g2;
                         if ( useSten )
                         {
                             const uint32_t id = SLOTS ? static_cast<uint32_t>( q )
                                                       : ( sb[( g + k ) / 4] >> ( 8 * ( ( g + k ) % 4 ) ) ) & 0xffu;
                             xi2d = sxi2[id];
                             xil = stab[id].w;
-                            if ( SLOTS && A.stencil_xiv )
+                            if constexpr ( SLOTS )
                             {
-                                const double* xv = A.stencil_xiv + 4 * id;
-                                g0 = xv[0];
-                                g1 = xv[1];
-                                g2 = xv[2];
+                                // slot q is wave-uniform: scalar loads of its FP64 record
+                                const double* xv = sxiv + 4 * q;
+                                g0 = scalarLoad( xv );
+                                g1 = scalarLoad( xv + 1 );
+                                g2 = scalarLoad( xv + 2 );
+                            }
+                            else
+                            {
+                                g0 = xj[k][0] - xi0;
+                                g1 = xj[k][1] - xi1;
+                                g2 = xj[k][2] - xi2;
                             }
                         }
                         else
                         {
+                            g0 = xj[k][0] - xi0;
+                            g1 = xj[k][1] - xi1;
+                            g2 = xj[k][2] - xi2;
                             xi2d = fma( g0, g0, fma( g1, g1, g2 * g2 ) );
                             const float x2f = static_cast<float>( xi2d );
                             xil = x2f * __builtin_amdgcn_rsqf( x2f );
@@ -730,7 +742,10 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                             {
                                 // guard band: the reference's FP64 test on this bond
                                 Bond64 e;
-                                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xj[k][0], xj[k][1], xj[k][2], uj[k][0],
+                                const int64_t j = jj[k];
+                                const bool xl = PREC == kF64 || !useSten || ( !kIsLin<MODEL> && !SLOTS );
+                                getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xl ? xj[k][0] : x[j],
+                                               xl ? xj[k][1] : x[n + j], xl ? xj[k][2] : x[2 * n + j], uj[k][0],
                                                uj[k][1], uj[k][2], e );
                                 brk = criticalStretch64( e.r, e.xi, tmp ? thermalBcc( A, temp_avg ) : bcc );
                             }

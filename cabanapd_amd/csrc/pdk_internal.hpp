@@ -195,10 +195,11 @@ struct NeighborQuery
 
 // Builds the sliced CSR; (re)allocates col/intact when capacity is short.
 // Returns total slot rows.
+// cap_hint > 0: try one pass into cap_hint rows per slice first (contact lists).
 int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64_t* inv,
                         int64_t* slice_ptr, int32_t* cnt, DevBuf& col, DevBuf& intact,
                         int64_t& col_capacity_rows, BuildScratch& s, MemAccount* acct,
-                        hipStream_t st, int32_t* max_cnt_out );
+                        hipStream_t st, int32_t* max_cnt_out, int32_t cap_hint = 0 );
 
 // Prenotch<N>::create over the sliced CSR.
 void launchPrenotch( const PDArgs& a, int32_t nnotch, const double* notch_dev, hipStream_t st );
