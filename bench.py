@@ -403,7 +403,7 @@ def rows_leg(args):
     w = workload()
     mtype = (x[:, 0] >= 0.0).astype(np.int32)
     w["materials"] = ([(bd.K, 0.0, bd.G0), (2.0 * bd.K, 0.0, 2.0 * bd.G0)], mtype)
-    force_row("multi_material", w, precisions=("fp64",))
+    force_row("multi_material", w)
     force_row("epp", workload(mechanics="epp", sigma_y=3.0 * bd.K * 1e-3))
     w = workload(thermal=True, alpha=1e-5, temp0=0.0)
     w["prescribed"] = [("temperature", np.arange(n), np.zeros(n), 1e7 * (x[:, 0] - x[:, 0].min()) / 0.1)]

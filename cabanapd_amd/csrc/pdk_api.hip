@@ -412,6 +412,12 @@ struct pdk_spec_s
         {
             h.mat_c[t] = mat_c[t];
             h.mat_bcc[t] = mat_bcc[t];
+            // s0 of the pair from bond_break_coeff = (1 + s0)^2; a pair that
+            // never breaks (+inf) gets no band, so its FP32 test is final
+            const double s0p = std::sqrt( mat_bcc[t] ) - 1.0;
+            h.mat_s0f[t] = static_cast<float>( s0p );
+            h.mat_bandK[t] = std::isfinite( s0p ) ? static_cast<float>( 2.44140625e-4 * s0p + 9.094947017729282e-13 )
+                                                  : 0.0f;
         }
         for ( int t = 0; t < 4; t++ )
         {
@@ -1504,8 +1510,10 @@ void requireTableSpec( pdk_spec s, int32_t ntypes, const int32_t* type )
     require( !s->inited, "set the materials before pdk_init" );
     require( ntypes >= 1 && ntypes <= 4, "1 to 4 material types" );
     require( !isContact( s->d ), "the contact models have no materials" );
-    require( ntypes == 1 || s->d.precision == PDK_FP64, "multi-material specimens are FP64 only" );
     require( ntypes == 1 || type != nullptr, "type[] is required for several materials" );
+    require( s->d.precision == PDK_FP64 ||
+                 ( s->d.model != PDK_MODEL_LINEAR_PMB && s->d.model != PDK_MODEL_LINEAR_LPS ),
+             "the mixed form of several materials covers the PMB and LPS models" );
 }
 
 // Install the pair tables: c and bond_break_coeff per ordered type pair, the
@@ -1601,7 +1609,6 @@ int pdk_set_pair_models( pdk_spec s, int32_t ntypes, const pdk_pair_model* pairs
     return guarded( [&] {
         require( s && pairs, "null argument" );
         requireTableSpec( s, ntypes, type );
-        require( s->d.precision == PDK_FP64, "multi-material specimens are FP64 only" );
         const bool lps = s->d.model == PDK_MODEL_LPS || s->d.model == PDK_MODEL_LINEAR_LPS;
         for ( int32_t a = 0; a < ntypes; a++ )
             for ( int32_t b = 0; b < ntypes; b++ )

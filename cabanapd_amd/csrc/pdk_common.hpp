@@ -132,6 +132,9 @@ struct PDArgs
     const uint8_t* mtype;
     int32_t nmat;
     double mat_c[16], mat_bcc[16], mat_tc[4], mat_sc[4];
+    // mixed form, several materials: the pair's critical stretch in FP32 and
+    // its guard band (as s0f and bandK for one material)
+    float mat_s0f[16], mat_bandK[16];
     // PMB family extensions (FP64 gather passes).  EPP: plastic stretch of
     // bond slot q of lane L of a slice at splas[(slice_ptr[slice] + q) *
     // kSlice + L] (null: Elastic) and the yield stretch sY.

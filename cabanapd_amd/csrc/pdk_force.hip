@@ -362,10 +362,13 @@ __device__ __forceinline__ void epilogue( const PDArgs& A, int64_t p, int32_t ep
 // ---------------------------------------------------------------------------
 // SLOTS selects the stencil-slot form (neighbor ids from the slot offsets,
 // existence bits per slot row) over the compact form (neighbor id stream).
+// MM: the mixed form of a multi-material specimen (pair tables), an instance
+// of its own so that the one-material mixed instances carry none of its
+// per-bond table lookups (the FP64 form reads the tables when present).
 // ---------------------------------------------------------------------------
 constexpr int kBatch = 16; // bond slots whose ids / bit words are fetched ahead
 
-template <int MODEL, bool FRAC, int PREC, int UNR, bool SLOTS>
+template <int MODEL, bool FRAC, int PREC, int UNR, bool SLOTS, bool MM = false>
 __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ specs,
                                                     const int32_t* __restrict__ tile_spec,
                                                     const int64_t* __restrict__ tile_slice,
@@ -428,8 +431,8 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     const bool nf_i = FRAC ? ( A.nofail[pc] != 0 ) : false;
     const double c = A.c;
     const double bcc = A.bcc;
-    // multi-material (FP64): model of the material pair; null with one material
-    const uint8_t* __restrict__ mt = ( PREC == kF64 ) ? A.mtype : nullptr;
+    // multi-material: model of the material pair; null with one material
+    const uint8_t* __restrict__ mt = ( PREC == kF64 || MM ) ? A.mtype : nullptr;
     const int ti4 = mt ? 4 * static_cast<int>( mt[pc] ) : 0;
     // PMB family extensions: temperature, plastic stretch per bond slot
     const double* __restrict__ tmp = kIsLps<MODEL> ? nullptr : A.temp;
@@ -467,7 +470,8 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     // Uniform volume (the specimens the row-window pass takes without its
     // general path): the per-bond factor c vol (PMB) or vol (LPS) multiplies
     // each row sum in FP64 instead of every bond.
-    const bool fold = uvol && !( FRAC && any_nofail );
+    // (PMB with several materials: c varies per pair, so it stays in each bond)
+    const bool fold = uvol && !( FRAC && any_nofail ) && !( mt && !kIsLps<MODEL> );
     const double rscale = fold ? A.row_scale : 1.0;
     const float c32 = static_cast<float>( c );
     const float bandK = A.bandK;
@@ -736,9 +740,12 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                     {
                         if ( !nf_i && !nf_j )
                         {
+                            // the pair's critical stretch and band (several materials)
+                            const float s0p = mt ? A.mat_s0f[ti4 + mj[k]] : A.s0f;
+                            const float bandp = mt ? A.mat_bandK[ti4 + mj[k]] : bandK;
                             const float band =
-                                tmp ? fmaf( A.bandf, fabsf( s_mech ) + A.s0f + fabsf( ath ), bandK ) : bandK;
-                            if ( fabsf( b.s - A.s0f ) <= band )
+                                tmp ? fmaf( A.bandf, fabsf( s_mech ) + s0p + fabsf( ath ), bandp ) : bandp;
+                            if ( fabsf( b.s - s0p ) <= band )
                             {
                                 // guard band: the reference's FP64 test on this bond
                                 Bond64 e;
@@ -747,10 +754,12 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                                 getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xl ? xj[k][0] : x[j],
                                                xl ? xj[k][1] : x[n + j], xl ? xj[k][2] : x[2 * n + j], uj[k][0],
                                                uj[k][1], uj[k][2], e );
-                                brk = criticalStretch64( e.r, e.xi, tmp ? thermalBcc( A, temp_avg ) : bcc );
+                                brk = criticalStretch64( e.r, e.xi,
+                                                         tmp ? thermalBcc( A, temp_avg )
+                                                             : ( mt ? A.mat_bcc[ti4 + mj[k]] : bcc ) );
                             }
                             else
-                                brk = b.s > A.s0f;
+                                brk = b.s > s0p;
                         }
                     }
                     if ( !brk && intact )
@@ -763,7 +772,8 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                                 spl ? static_cast<float>( plasticStretch( spl + ( base + q ) * kSlice + lane,
                                                                           static_cast<double>( b.s ), A.sY ) )
                                     : b.s;
-                            fr = ( fold ? sd : ( c32 * static_cast<float>( vj[k] ) ) * sd ) * b.rinv;
+                            const float cp = mt ? static_cast<float>( A.mat_c[ti4 + mj[k]] ) : c32;
+                            fr = ( fold ? sd : ( cp * static_cast<float>( vj[k] ) ) * sd ) * b.rinv;
                         }
                         else
                         {
@@ -2050,15 +2060,15 @@ __global__ __launch_bounds__( 256 ) void kPrescribe( const int64_t* __restrict__
 
 using ForceFn = void ( * )( const PDArgs*, const int32_t*, const int64_t*, int32_t, int32_t, int64_t, int32_t );
 
-template <int MODEL, bool FRAC, int PREC, bool SLOTS>
+template <int MODEL, bool FRAC, int PREC, bool SLOTS, bool MM = false>
 ForceFn pickUnroll( int unroll )
 {
     switch ( unroll )
     {
-    case 1: return kForce<MODEL, FRAC, PREC, 1, SLOTS>;
-    case 2: return kForce<MODEL, FRAC, PREC, 2, SLOTS>;
-    case 8: return kForce<MODEL, FRAC, PREC, 8, SLOTS>;
-    default: return kForce<MODEL, FRAC, PREC, 4, SLOTS>;
+    case 1: return kForce<MODEL, FRAC, PREC, 1, SLOTS, MM>;
+    case 2: return kForce<MODEL, FRAC, PREC, 2, SLOTS, MM>;
+    case 8: return kForce<MODEL, FRAC, PREC, 8, SLOTS, MM>;
+    default: return kForce<MODEL, FRAC, PREC, 4, SLOTS, MM>;
     }
 }
 
@@ -2069,6 +2079,9 @@ ForceFn pickForceModel( const PDArgs& a )
     if constexpr ( kIsLin<MODEL> ) // no fracture form
         return mixed ? pickUnroll<MODEL, false, kMixed, SLOTS>( a.unroll )
                      : pickUnroll<MODEL, false, kF64, SLOTS>( a.unroll );
+    else if ( mixed && a.mtype )
+        return a.fracture ? pickUnroll<MODEL, true, kMixed, SLOTS, true>( a.unroll )
+                          : pickUnroll<MODEL, false, kMixed, SLOTS, true>( a.unroll );
     else if ( a.fracture )
         return mixed ? pickUnroll<MODEL, true, kMixed, SLOTS>( a.unroll )
                      : pickUnroll<MODEL, true, kF64, SLOTS>( a.unroll );
@@ -2152,9 +2165,10 @@ const char* forceKernelName( const PDArgs& a )
                        rowsGeneral( a ) ? "true" : "false" );
         return g_force_name;
     }
-    std::snprintf( g_force_name, sizeof( g_force_name ), "kForce<%d, %s, %d, %d, %s>", a.model,
+    const bool mm = a.precision == PDK_MIXED && a.mtype != nullptr && ( a.model == kPMB || a.model == kLPS );
+    std::snprintf( g_force_name, sizeof( g_force_name ), "kForce<%d, %s, %d, %d, %s%s>", a.model,
                    a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0, normalizeUnroll( a.unroll ),
-                   a.exist ? "true" : "false" );
+                   a.exist ? "true" : "false", mm ? ", true" : "" );
     return g_force_name;
 }
 const char* lpsKernelName( const PDArgs& a )

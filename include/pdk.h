@@ -258,8 +258,10 @@ int pdk_set_field(pdk_spec spec, int32_t field, const double* values);
  * (force_models/CabanaPD_PMB.hpp:51-55); s0 = sqrt((s0_a^2 K_a + s0_b^2 K_b) /
  * (K_a + K_b)), bond_break_coeff = (1 + s0)^2 (CabanaPD_ForceModels.hpp:114-121);
  * LPS keeps each material's theta/s coefficients on its own particle's terms
- * (force_models/CabanaPD_LPS.hpp:55-69).  FP64 only; the force pass then uses
- * the gather variant.  Call after pdk_spec_upload and before pdk_init. */
+ * (force_models/CabanaPD_LPS.hpp:55-69).  Both precision forms; the force
+ * pass then uses the gather variant (the mixed form with each pair's c,
+ * critical stretch and guard band).  Call after pdk_spec_upload and before
+ * pdk_init. */
 typedef struct pdk_material {
     double K, G, G0;
 } pdk_material;
@@ -280,7 +282,7 @@ int pdk_set_materials(pdk_spec spec, int32_t nmat, const pdk_material* materials
  * only and [1] on b only, with the same per-type values (any table built from
  * per-material models and their averages is of this form); other tables are
  * rejected.  pdk_set_materials is the special case of the averaged pair table.
- * FP64 only; call after pdk_spec_upload and before pdk_init. */
+ * Both precision forms; call after pdk_spec_upload and before pdk_init. */
 typedef struct pdk_pair_model {
     double c;
     double bond_break_coeff;

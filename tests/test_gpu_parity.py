@@ -389,17 +389,81 @@ def test_gpu_crack_inclusion_regression():
     crack.close()
 
 
-def test_gpu_multi_material_rejects_mixed():
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_gpu_multi_material_mixed_matches_fp64(model):
+    """The mixed form of a two-material specimen (per-pair c, critical
+    stretch and guard band in the gather pass): against the FP64 form on the
+    same fracturing cube after 20 steps, the same broken bonds and u within
+    1e-5 relative."""
     from cabanapd_amd import engine
-    from tests.parity import ctx
-    w = configs.lattice_sum_cube(model="PMB", m=3, nc=7, field="linear", coeff=0.0, K=1e9, fracture=True, G0=1.0)
-    d = type(w["desc"]).from_buffer_copy(w["desc"])
-    d.precision = 1
-    s = engine.Spec(ctx(), d, w["x"].shape[0])
-    configs.upload(s, w)
-    with pytest.raises(engine.PdkError, match="FP64 only"):
-        s.set_materials([(1e9, 0.5e9, 1.0), (2e9, 0.5e9, 1.0)], np.zeros(w["x"].shape[0], dtype=np.int32))
-    s.close()
+    say(f"case: {model} two materials, mixed vs FP64, cube 11^3, 20 steps")
+    out = {}
+    for prec in (0, 1):
+        w = configs.lattice_sum_cube(model=model, m=3, nc=11, field="linear", coeff=0.0, K=1e9, G=0.5e9,
+                                     fracture=True, G0=1.0)
+        x = w["x"]
+        rng = np.random.default_rng(4)
+        w["u"] = rng.uniform(-2e-3, 2e-3, size=x.shape) * (2.0 / 11)
+        w["rho"] = np.full(x.shape[0], 1000.0)
+        w["desc"].dt = 1e-6
+        w["desc"].precision = prec
+        w["materials"] = ([(1e9, 0.5e9, 1.0), (3e9, 1.2e9, 4.0)], rng.integers(0, 2, x.shape[0]).astype(np.int32))
+        s = engine.Spec(ctx(), w["desc"], x.shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        s.step(20)
+        out[prec] = (s.get("u"), broken_set(s), s.kernel_name(0) if hasattr(s, "kernel_name") else "")
+        s.close()
+    err = rel_err(out[1][0], out[0][0])
+    say(f"  kernel {out[1][2]}: u rel err {err:.3e}, broken {len(out[0][1])}, mismatches "
+        f"{len(out[0][1] ^ out[1][1])}")
+    assert out[0][1] == out[1][1] and len(out[0][1]) > 0
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("model", ["PMB", "LPS"])
+def test_gpu_multi_material_guard_band(model):
+    """Two materials with different critical stretches (three pair values:
+    each material's and the averaged one): a uniform stretch swept through
+    each pair's own critical stretch in steps far below FP32 resolution puts
+    that pair's bonds inside its guard band; the mixed form decides every
+    bond exactly as the FP64 form."""
+    from cabanapd_amd import engine
+    from cabanapd_amd._abi import make_desc
+    from oracle import binding as orc
+    say(f"case: guard band, {model}, two materials")
+    nc = 12
+    x, vol, d = configs.grid_cell_centred([-1, -1, -1], [1, 1, 1], (nc, nc, nc))
+    horizon = 3.01 * d[0]
+    mats = [(1e9, 0.5e9, 1e-3), (1e9, 0.5e9, 3e-3)]
+    mtype = np.random.default_rng(7).integers(0, 2, x.shape[0]).astype(np.int32)
+    kw = dict(fracture=True, horizon=horizon, K=1e9, G0=1e-3, dt=1e-9)
+    if model == "LPS":
+        kw["G"] = 0.5e9
+    import math
+    s0s = []
+    for (K, G, G0) in mats:
+        dd = make_desc(model, **dict(kw, K=K, G=G, G0=G0))
+        s0s.append(orc.model_constants(dd)["s0"])
+    s0ab = math.sqrt((s0s[0] ** 2 * mats[0][0] + s0s[1] ** 2 * mats[1][0]) / (mats[0][0] + mats[1][0]))
+    for crit in (s0s[0], s0ab, s0s[1]):
+        for rel in (-1e-7, -3e-9, 0.0, 3e-9, 1e-7):
+            res = {}
+            for prec in ("fp64", "mixed"):
+                desc = make_desc(model, precision=prec, **kw)
+                s = engine.Spec(ctx(), desc, x.shape[0])
+                s.upload(x, vol, u=(crit * (1 + rel)) * x)
+                s.build_bonds()
+                s.set_materials(mats, mtype)
+                s.compute_weighted_volume()
+                s.compute_dilatation()
+                s.compute_force(True)
+                res[prec] = s.bonds()[2]
+                s.close()
+            nb = int((res["fp64"] == 0).sum())
+            say(f"  crit {crit:.6e} rel {rel:+.0e}: fp64 broken {nb}/{res['fp64'].size} "
+                f"mismatches {(res['fp64'] != res['mixed']).sum()}")
+            assert np.array_equal(res["fp64"], res["mixed"])
 
 
 @pytest.mark.parametrize("model", ["PMB", "LPS"])
