@@ -242,13 +242,41 @@ def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
         ctx.close()
         return summarize(lat, wall, bond_evals)
 
+    def small_batch(graph_steps, n_range, count=16, steps=400):
+        """count small specimens of n_range particles stepped together in the
+        batch engine; packed-step time with the step replayed as a captured
+        graph for up to graph_steps steps (0: eager launches)."""
+        brng = np.random.default_rng(seed + 1)
+        ws = [configs.serving_request(brng, n_range=n_range, steps_range=(steps, steps)) for _ in range(count)]
+        ctx = engine.Context(0)
+        ctx.set_graph(graph_steps)
+        specs = [make(w, "mixed", ctx) for w in ws]
+        # warm up: a few packed steps, then the timed run of `steps` steps
+        def drain(tickets):
+            ctx.progress(0)
+            while not all(ctx.poll(t) for t in tickets):
+                time.sleep(1e-4)
+
+        drain([ctx.submit(s, 8) for s in specs])
+        t0 = time.perf_counter()
+        drain([ctx.submit(s, steps) for s in specs])
+        dt = time.perf_counter() - t0
+        npart = sum(w["x"].shape[0] for w in ws)
+        for s in specs:
+            s.close()
+        ctx.close()
+        return {"specimens": count, "particles": int(npart), "steps": steps, "graph_steps": graph_steps,
+                "ms_per_step": 1e3 * dt / steps}
+
     mixed = stream("mixed", True)
     fp64 = stream("fp64", False)
     sizes = [r["x"].shape[0] for r in reqs]
     return {"requests": num_requests, "arrival_rate_per_s": rate, "seed": seed,
             "particles_min": int(min(sizes)), "particles_max": int(max(sizes)),
             "steps_total": int(sum(r["steps"] for r in reqs)),
-            "mixed_batched": mixed, "fp64_per_specimen": fp64}
+            "mixed_batched": mixed, "fp64_per_specimen": fp64,
+            "small_batch_graph": small_batch(64, (2e4, 4e4)), "small_batch_eager": small_batch(0, (2e4, 4e4)),
+            "tiny_batch_graph": small_batch(64, (1e3, 2e3)), "tiny_batch_eager": small_batch(0, (1e3, 2e3))}
 
 
 def copy_kernel(mib, repeats=5):

@@ -2276,4 +2276,12 @@ int pdk_batch_progress( pdk_ctx ctx, int32_t max_steps, int32_t* launched )
     } );
 }
 
+int pdk_batch_set_graph( pdk_ctx ctx, int32_t steps )
+{
+    return guarded( [&] {
+        require( ctx != nullptr, "null ctx" );
+        ctx->batch->setGraphSteps( steps );
+    } );
+}
+
 } // extern "C"

@@ -112,6 +112,7 @@ struct PDArgs
     int32_t parity_offset;  // batch: current U index = launch parity ^ parity_offset
     int32_t unroll;         // force pass: bond slots loaded per group (1, 2, 4, 8)
     int64_t end_launch;     // batch: launch index of this spec's last step
+    const int64_t* launch_ctr; // batch: the launch index on the device (graph replays pass launch < 0)
     // Exact shortcuts: every vol[j] equals vol_value / no particle is no-fail.
     int32_t vol_uniform;
     int32_t any_nofail;

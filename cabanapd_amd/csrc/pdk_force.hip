@@ -290,8 +290,12 @@ __device__ __forceinline__ bool waveSetup( const PDArgs* __restrict__ specs,
         // the tile tables are written by the host before the launch
         w.A = specs + scalarLoad( tile_spec + t );
         w.slice = scalarLoad( tile_slice + t ) + wave;
-        w.cur = parity ^ w.A->parity_offset;
-        w.epi = ( launch + 1 < w.A->end_launch ) ? kEpiFinalInitial : kEpiFinal;
+        // launch < 0: step -1 - launch of a replayed graph, whose first step's
+        // index is the device counter (written by an earlier kernel of the
+        // stream; the scalar cache starts every dispatch invalidated)
+        const int64_t L = launch >= 0 ? launch : scalarLoad( scalarLoad( &w.A->launch_ctr ) ) + ( -1 - launch );
+        w.cur = static_cast<int32_t>( L & 1 ) ^ w.A->parity_offset;
+        w.epi = ( L + 1 < w.A->end_launch ) ? kEpiFinalInitial : kEpiFinal;
     }
     return w.slice < w.A->nslices;
 }
@@ -2239,6 +2243,32 @@ void launchLpsState( const PDArgs* args_dev, const PDArgs& a, bool weighted_volu
     LpsFn fn = pickLps( a );
     hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, nullptr, nullptr, parity,
                         int64_t( 0 ), int32_t( weighted_volume ), int32_t( dilatation ) );
+    PDK_HIP( hipGetLastError() );
+}
+
+// The packed steps of a replayed graph done: the device launch counter
+// advances by their number (the graph's last node).
+__global__ void kAdvanceLaunch( int64_t* __restrict__ ctr, int64_t steps )
+{
+    if ( blockIdx.x == 0 && threadIdx.x == 0 )
+        *ctr = *ctr + steps;
+}
+// The device launch counter set to L before replays.
+__global__ void kSetLaunch( int64_t* __restrict__ ctr, int64_t L )
+{
+    if ( blockIdx.x == 0 && threadIdx.x == 0 )
+        *ctr = L;
+}
+
+void launchAdvanceLaunch( int64_t* ctr, int64_t steps, hipStream_t st )
+{
+    hipLaunchKernelGGL( kAdvanceLaunch, dim3( 1 ), dim3( 64 ), 0, st, ctr, steps );
+    PDK_HIP( hipGetLastError() );
+}
+
+void launchSetLaunch( int64_t* ctr, int64_t L, hipStream_t st )
+{
+    hipLaunchKernelGGL( kSetLaunch, dim3( 1 ), dim3( 64 ), 0, st, ctr, L );
     PDK_HIP( hipGetLastError() );
 }
 

@@ -248,6 +248,9 @@ void launchPrescribed( const int64_t* idx_dev, int64_t count, const double* a_de
 // taken from `variant`): tile t covers slices [tile_slice[t], +4) of
 // specs[tile_spec[t]]; each spec's epilogue and displacement buffer follow
 // from launch_index, its end_launch and parity_offset.
+// The device launch counter of the batch engine's graph replays.
+void launchAdvanceLaunch( int64_t* ctr, int64_t steps, hipStream_t st );
+void launchSetLaunch( int64_t* ctr, int64_t L, hipStream_t st );
 void launchForceBatched( const PDArgs* specs_dev, const PDArgs& variant, const int32_t* tile_spec,
                          const int64_t* tile_slice, int64_t ntiles, int64_t launch_index,
                          hipStream_t st );

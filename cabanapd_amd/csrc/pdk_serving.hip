@@ -9,6 +9,7 @@
 // retirement only rewrite the small tile tables, never the specimen state.
 #include <algorithm>
 #include <deque>
+#include <limits>
 #include <map>
 #include <set>
 #include <tuple>
@@ -64,6 +65,19 @@ struct BatchEngine::Impl
     std::deque<hipEvent_t> inflight;
     std::vector<Group> groups;
     bool dirty = true;
+    // Graph replay of the packed step: kGraphSteps consecutive steps' launches
+    // (gexec_n) and one step's (gexec_1) captured once per group
+    // configuration, with the launch indexes counted from the device counter
+    // ctr (each graph's last node advances it); replayed for up to
+    // graph_steps steps at a time, never past the next retirement.  0 (the
+    // default): eager launches only, which measure as fast here (the host
+    // queues up to max_inflight steps ahead, so launch cost is hidden and a
+    // packed step lasts as long as its kernels).
+    static constexpr int64_t kGraphSteps = 8;
+    int32_t graph_steps = 0;
+    DevBuf ctr;
+    hipGraphExec_t gexec_n = nullptr;
+    hipGraphExec_t gexec_1 = nullptr;
 
     explicit Impl( pdk_ctx_s* c ) : ctx( c )
     {
@@ -76,6 +90,10 @@ struct BatchEngine::Impl
     ~Impl()
     {
         (void)hipStreamSynchronize( st );
+        if ( gexec_n )
+            (void)hipGraphExecDestroy( gexec_n );
+        if ( gexec_1 )
+            (void)hipGraphExecDestroy( gexec_1 );
         for ( auto& e : done_events )
             (void)hipEventDestroy( e.second );
         for ( auto e : inflight )
@@ -115,6 +133,8 @@ struct BatchEngine::Impl
 
     void rebuildGroups()
     {
+        if ( !ctr )
+            ctr.alloc( sizeof( int64_t ), &ctx_acct() ); // the device launch counter, on first use
         std::map<std::tuple<int, int, int, int>, size_t> key;
         std::vector<Group> ng;
         for ( const Run& r : running )
@@ -141,6 +161,7 @@ struct BatchEngine::Impl
             PDArgs a = *v.h;
             a.parity_offset = r.parity_offset;
             a.end_launch = r.end_launch;
+            a.launch_ctr = ctr.as<int64_t>();
             const int32_t idx = static_cast<int32_t>( g.h_specs.size() );
             g.h_specs.push_back( a );
             for ( int64_t s0 = 0; s0 < v.nslices; s0 += kSlicesPerBlock )
@@ -166,6 +187,60 @@ struct BatchEngine::Impl
         }
         groups.swap( ng );
         dirty = false;
+        dropGraph(); // the old graph's launches name the old tables
+    }
+
+    void dropGraph()
+    {
+        if ( gexec_n || gexec_1 )
+            PDK_HIP( hipStreamSynchronize( st ) );
+        for ( hipGraphExec_t* g : { &gexec_n, &gexec_1 } )
+            if ( *g )
+            {
+                PDK_HIP( hipGraphExecDestroy( *g ) );
+                *g = nullptr;
+            }
+    }
+
+    // One packed step's launches in group order, at launch index L (eager) or
+    // at the device counter's index + (-1 - L) (L < 0, a captured step).
+    void launchStep( int64_t L )
+    {
+        for ( Group& g : groups )
+        {
+            if ( g.lps )
+                launchLpsStateBatched( g.specs.as<PDArgs>(), g.variant, g.tspec.as<int32_t>(), g.tslice.as<int64_t>(),
+                                       g.ntiles, L, st );
+            launchForceBatched( g.specs.as<PDArgs>(), g.variant, g.tspec.as<int32_t>(), g.tslice.as<int64_t>(),
+                                g.ntiles, L, st );
+        }
+    }
+
+    // Capture `steps` consecutive packed steps and the counter's advance.
+    hipGraphExec_t captureSteps( int64_t steps )
+    {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        PDK_HIP( hipStreamBeginCapture( st, hipStreamCaptureModeThreadLocal ) );
+        try
+        {
+            for ( int64_t j = 0; j < steps; j++ )
+                launchStep( -1 - j );
+            launchAdvanceLaunch( ctr.as<int64_t>(), steps, st );
+        }
+        catch ( ... )
+        {
+            (void)hipStreamEndCapture( st, &graph );
+            if ( graph )
+                (void)hipGraphDestroy( graph );
+            throw;
+        }
+        PDK_HIP( hipStreamEndCapture( st, &graph ) );
+        const hipError_t e = hipGraphInstantiate( &exec, graph, nullptr, nullptr, 0 );
+        (void)hipGraphDestroy( graph );
+        if ( e != hipSuccess )
+            throw Error( std::string( "batch engine: graph instantiation failed: " ) + hipGetErrorString( e ) );
+        return exec;
     }
 
     MemAccount& ctx_acct();
@@ -214,17 +289,34 @@ struct BatchEngine::Impl
                 break;
             if ( dirty )
                 rebuildGroups();
-            for ( Group& g : groups )
+            // steps until the next retirement (arrivals wait for one: admit()
+            // has taken every pending spec that fits)
+            int64_t k = std::numeric_limits<int64_t>::max();
+            for ( const Run& r : running )
+                k = std::min( k, r.end_launch - L );
+            if ( max_steps > 0 )
+                k = std::min<int64_t>( k, max_steps - launched );
+            k = std::min<int64_t>( k, graph_steps );
+            if ( k >= 2 )
             {
-                if ( g.lps )
-                    launchLpsStateBatched( g.specs.as<PDArgs>(), g.variant, g.tspec.as<int32_t>(),
-                                           g.tslice.as<int64_t>(), g.ntiles, L, st );
-                launchForceBatched( g.specs.as<PDArgs>(), g.variant, g.tspec.as<int32_t>(), g.tslice.as<int64_t>(),
-                                    g.ntiles, L, st );
+                if ( k >= kGraphSteps && !gexec_n )
+                    gexec_n = captureSteps( kGraphSteps );
+                if ( k % kGraphSteps != 0 && !gexec_1 )
+                    gexec_1 = captureSteps( 1 );
+                launchSetLaunch( ctr.as<int64_t>(), L, st );
+                for ( int64_t i = 0; i < k / kGraphSteps; i++ )
+                    PDK_HIP( hipGraphLaunch( gexec_n, st ) );
+                for ( int64_t i = 0; i < k % kGraphSteps; i++ )
+                    PDK_HIP( hipGraphLaunch( gexec_1, st ) );
             }
-            L++;
+            else
+            {
+                k = 1;
+                launchStep( L );
+            }
+            L += k;
             retire();
-            launched++;
+            launched += static_cast<int32_t>( k );
             hipEvent_t e;
             PDK_HIP( hipEventCreateWithFlags( &e, hipEventDisableTiming ) );
             PDK_HIP( hipEventRecord( e, st ) );
@@ -319,6 +411,14 @@ int32_t BatchEngine::progress( int32_t max_steps )
 void BatchEngine::forget( pdk_spec_s* s )
 {
     impl_->forget( s );
+}
+
+void BatchEngine::setGraphSteps( int32_t steps )
+{
+    if ( steps < 0 )
+        throw Error( "graph steps must be 0 (eager) or positive" );
+    impl_->dropGraph();
+    impl_->graph_steps = steps;
 }
 
 } // namespace pdk

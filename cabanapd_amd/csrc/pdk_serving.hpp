@@ -56,6 +56,8 @@ class BatchEngine
     bool done( uint64_t ticket ) const;
     int32_t progress( int32_t max_steps );
     void forget( pdk_spec_s* s );
+    // Replay the captured packed step for up to steps steps at a time (0: eager).
+    void setGraphSteps( int32_t steps );
 
   private:
     struct Impl;

@@ -61,6 +61,7 @@ _SIGS = {
     "pdk_submit": (C.c_int, [_VP, _VP, C.c_int32, C.POINTER(C.c_uint64)]),
     "pdk_poll": (C.c_int, [_VP, C.c_uint64, _I32]),
     "pdk_batch_progress": (C.c_int, [_VP, C.c_int32, _I32]),
+    "pdk_batch_set_graph": (C.c_int, [_VP, C.c_int32]),
     "pdk_series_create": (C.c_int, [_VP, C.POINTER(_abi.RegionDesc), C.POINTER(_abi.Quantity), C.c_int32,
                                     C.c_int64, C.POINTER(_VP)]),
     "pdk_series_destroy": (C.c_int, [_VP]),
@@ -162,6 +163,10 @@ class Context:
         n = C.c_int32()
         _check(lib().pdk_batch_progress(self.h, max_steps, C.byref(n)))
         return n.value
+
+    def set_graph(self, steps):
+        """Replay the captured packed step for up to steps steps (0: eager launches)."""
+        _check(lib().pdk_batch_set_graph(self.h, steps))
 
     def __enter__(self):
         return self

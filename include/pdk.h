@@ -465,6 +465,14 @@ int pdk_poll(pdk_ctx ctx, uint64_t ticket, int32_t* done);
 /* Advance the batch engine by up to max_steps packed steps (0 = until idle).
  * Returns the number of packed steps launched in *launched. */
 int pdk_batch_progress(pdk_ctx ctx, int32_t max_steps, int32_t* launched);
+/* No counterpart in the reference (a CabanaPD run is one specimen per
+ * process).  The packed step's launches are captured as one HIP graph per
+ * configuration of running specs and replayed for up to steps packed steps
+ * at a time, never past the next completion (the launch index is then read
+ * from a device counter that the graph advances); results equal the eager
+ * launches bit for bit.  steps = 0: eager launches only (the default: with the
+ * host queueing steps ahead, eager launches keep the GPU as busy). */
+int pdk_batch_set_graph(pdk_ctx ctx, int32_t steps);
 
 #ifdef __cplusplus
 }

@@ -33,8 +33,12 @@ def workloads():
     ]
 
 
-def test_alone_equals_mix():
-    say("case: six specimens alone vs packed batch with staggered admission")
+@pytest.mark.parametrize("graph_steps", [64, 5, 0])
+def test_alone_equals_mix(graph_steps):
+    """graph_steps: the packed step replayed as a captured graph for up to that
+    many steps at a time (0: eager launches)."""
+    say(f"case: six specimens alone vs packed batch with staggered admission, graph steps {graph_steps}")
+    ctx().set_graph(graph_steps)
     wl = workloads()
     alone = []
     for w, prec, steps in wl:
@@ -67,6 +71,57 @@ def test_alone_equals_mix():
         assert np.array_equal(v, alone[k][1])
         assert np.array_equal(intact, alone[k][2])
         s.close()
+    ctx().set_graph(0)
+
+
+def test_graph_replays_across_eager_work():
+    """Graph replays of the packed step, interleaved with the program's own
+    eager work and synchronisations: two specimens run in the batch engine in
+    chunks of at most 11 steps (replays of the 8-step and the 1-step graph),
+    each finished run checked against the same number of solo steps, then one
+    specimen continued alone through two more progress calls and checked
+    again."""
+    say("case: graph replays across submissions and progress chunks")
+    c = ctx()
+    c.set_graph(11)
+    wa = configs.kw_plate(num_cells=(40, 80, 9), steps=41, name="ga")
+    wb = configs.lps_cube(n_side=14, steps=13, fracture=True)
+
+    def solo(w, prec, steps):
+        s = make(w, prec)
+        s.step(steps)
+        out = (s.get("u"), s.get("v"), s.bonds()[2])
+        s.close()
+        return out
+
+    def wait(*tickets):
+        deadline = time.time() + 60.0
+        while not all(c.poll(t) for t in tickets) and time.time() < deadline:
+            time.sleep(1e-3)
+        assert all(c.poll(t) for t in tickets)
+
+    ref_a20, ref_a41, ref_b13 = solo(wa, "mixed", 20), solo(wa, "mixed", 41), solo(wb, "mixed", 13)
+    a, b = make(wa, "mixed"), make(wb, "mixed")
+    ta, tb = c.submit(a, 20), c.submit(b, 13)
+    c.progress(9)
+    c.progress(0)
+    wait(ta, tb)
+    for name, s, ref in (("a@20", a, ref_a20), ("b@13", b, ref_b13)):
+        ok = [np.array_equal(s.get("u"), ref[0]), np.array_equal(s.get("v"), ref[1]),
+              np.array_equal(s.bonds()[2], ref[2])]
+        say(f"  {name}: u/v/bonds equal {ok}")
+        assert all(ok)
+    ta = c.submit(a, 21)
+    c.progress(2)
+    c.progress(0)
+    wait(ta)
+    ok = [np.array_equal(a.get("u"), ref_a41[0]), np.array_equal(a.get("v"), ref_a41[1]),
+          np.array_equal(a.bonds()[2], ref_a41[2])]
+    say(f"  a@41: u/v/bonds equal {ok}")
+    assert all(ok)
+    a.close()
+    b.close()
+    c.set_graph(0)
 
 
 def test_budget_rejects_oversized_specimen():
