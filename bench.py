@@ -179,7 +179,8 @@ def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
     from concurrent.futures import ThreadPoolExecutor
 
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
+    # unit-rate arrival times, scaled by each stream's rate
+    arrivals_unit = np.cumsum(rng.exponential(1.0, size=num_requests))
     reqs = [configs.serving_request(rng, n_range=(2e4, max_particles)) for _ in range(num_requests)]
 
     def make(w, precision, ctx):
@@ -195,7 +196,8 @@ def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
         return {"solves_per_s": len(lat) / wall, "p50_s": float(np.percentile(lat, 50)),
                 "p99_s": float(np.percentile(lat, 99)), "wall_s": wall, "bond_evals_per_s": bond_evals / wall}
 
-    def stream(precision, batched):
+    def stream(precision, batched, rate):
+        arrivals = arrivals_unit / rate
         ctx = engine.Context(0)
         pool = ThreadPoolExecutor(max_workers=2)
         t0 = time.perf_counter()
@@ -268,13 +270,20 @@ def serving(num_requests=24, rate=50.0, seed=20261202, max_particles=2e6):
         return {"specimens": count, "particles": int(npart), "steps": steps, "graph_steps": graph_steps,
                 "ms_per_step": 1e3 * dt / steps}
 
-    mixed = stream("mixed", True)
-    fp64 = stream("fp64", False)
+    mixed = stream("mixed", True, rate)
+    fp64 = stream("fp64", False, rate)
+    # the same requests offered below the batched capacity: latencies of a
+    # sustained stream rather than of a queue draining
+    sustained_rate = 5.0
+    sustained = stream("mixed", True, sustained_rate)
+    sustained["arrival_rate_per_s"] = sustained_rate
     sizes = [r["x"].shape[0] for r in reqs]
+    bond_evals = mixed["bond_evals_per_s"] * mixed["wall_s"] / num_requests
     return {"requests": num_requests, "arrival_rate_per_s": rate, "seed": seed,
             "particles_min": int(min(sizes)), "particles_max": int(max(sizes)),
             "steps_total": int(sum(r["steps"] for r in reqs)),
-            "mixed_batched": mixed, "fp64_per_specimen": fp64,
+            "bond_evals_per_request": bond_evals, "offered_bond_evals_per_s": bond_evals * rate,
+            "mixed_batched": mixed, "fp64_per_specimen": fp64, "mixed_batched_sustained": sustained,
             "small_batch_graph": small_batch(64, (2e4, 4e4)), "small_batch_eager": small_batch(0, (2e4, 4e4)),
             "tiny_batch_graph": small_batch(64, (1e3, 2e3)), "tiny_batch_eager": small_batch(0, (1e3, 2e3))}
 

@@ -1844,6 +1844,340 @@ __global__ __launch_bounds__( kBlock ) void kStress( const PDArgs* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Output passes on lattice-row windows: the energy and stress passes above
+// for the stencil-slot layout with a row table, with no per-particle material
+// type, temperature or plastic stretch (those keep the gathers).  Each
+// wavefront stages the neighbour windows of its slice's lattice rows in LDS
+// (u, x, then NE further per-particle arrays, then vol when volumes differ)
+// with raw buffer loads, as the row-window force pass does, and visits the
+// slots in slot order, so every per-particle sum takes the gathers' terms in
+// the gathers' order: results are bit-identical to kEnergy / kStress.
+// ---------------------------------------------------------------------------
+template <int NC>
+struct RowWalk64
+{
+    static constexpr int kW = kSlice + kRowMaxSlots - 1; // window width
+    __amdgpu_buffer_rsrc_t rs[NC];
+    const int4* __restrict__ rows;
+    int nrows;
+    int32_t p032; // first particle of the slice
+    int lane;
+
+    // bond(q, wr) for every slot q of the slice in slot order; wr is the
+    // lane's window position (channel k at wr[k * kW]).  win: this wave's
+    // [2][NC][kW] LDS buffers.
+    template <class Bond>
+    __device__ __forceinline__ void walk( double* __restrict__ win, Bond&& bond ) const
+    {
+        double v0[NC], v1[NC];
+#pragma unroll
+        for ( int k = 0; k < NC; k++ )
+            v1[k] = 0.0;
+        const uint32_t lane8 = static_cast<uint32_t>( lane ) * 8u;
+        auto loadHalf = [&]( double* dst, int32_t g ) {
+            // g < 0: per-lane offsets, the negative ones wrap out of range and read 0
+            const uint32_t vo = g >= 0 ? lane8 : static_cast<uint32_t>( g + lane ) * 8u;
+            const uint32_t so = g >= 0 ? static_cast<uint32_t>( g ) * 8u : 0u;
+#pragma unroll
+            for ( int k = 0; k < NC; k++ )
+                dst[k] = bufferLoad64( rs[k], vo, so );
+        };
+        auto loadRow = [&]( int r ) {
+            const int4 R = scalarLoad4( rows + r );
+            const int32_t g = p032 + R.z; // uniform
+            loadHalf( v0, g );
+            if ( lane < R.y - 1 )
+                loadHalf( v1, g + kSlice );
+        };
+        loadRow( 0 );
+        for ( int r = 0; r < nrows; r++ )
+        {
+            const int4 R = scalarLoad4( rows + r );
+            double* __restrict__ wb = win + ( r & 1 ) * NC * kW;
+#pragma unroll
+            for ( int k = 0; k < NC; k++ )
+            {
+                wb[k * kW + lane] = v0[k];
+                if ( lane < R.y - 1 )
+                    wb[k * kW + kSlice + lane] = v1[k];
+            }
+            // LDS operations of one wavefront complete in order; the fences
+            // keep the compiler from moving the window reads above the stores
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+            if ( r + 1 < nrows )
+                loadRow( r + 1 );
+            for ( int t = 0; t < R.y; t++ )
+                bond( R.x + t, wb + lane + t );
+        }
+    }
+};
+
+// Set up the walker of slice `slice` over channels src[0 .. NC) (n values each).
+template <int NC>
+__device__ __forceinline__ RowWalk64<NC> rowWalk( const PDArgs& A, int64_t slice, int lane, const double* const* src )
+{
+    RowWalk64<NC> w;
+    const int64_t n = scalarLoad( &A.n );
+    // row tables exist only for n < 2^28 (byte sizes and offsets fit 32 bits)
+    const uint32_t nbytes = static_cast<uint32_t>( n ) * 8u;
+#pragma unroll
+    for ( int k = 0; k < NC; k++ )
+        w.rs[k] = bufferOf( src[k], nbytes );
+    w.rows = scalarLoad( &A.rows );
+    w.nrows = scalarLoad( &A.nrows );
+    w.p032 = static_cast<int32_t>( slice * kSlice );
+    w.lane = lane;
+    return w;
+}
+
+template <int MODEL, bool FRAC, bool GEN>
+__global__ __launch_bounds__( kBlock ) void kEnergyRows( const PDArgs* __restrict__ specs, int32_t parity,
+                                                         const double* __restrict__ mvals,
+                                                         const double* __restrict__ thetavals,
+                                                         double* __restrict__ partial )
+{
+#pragma clang fp contract( off )
+    constexpr int NC = 6 + ( GEN ? 1 : 0 );
+    constexpr int kW = RowWalk64<NC>::kW;
+    __shared__ double win[kSlicesPerBlock][2][NC][kW];
+    __shared__ double red[2][kSlicesPerBlock];
+    const PDArgs& A = *specs;
+    const int wv = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    // the partial sums of logical block t, as in kEnergy
+    const int64_t t = xcdRemap( blockIdx.x, gridDim.x );
+    const int64_t slice = t * kSlicesPerBlock + wv;
+    double Phi = 0.0, D = 0.0;
+    if ( slice < A.nslices )
+    {
+        const int64_t n = scalarLoad( &A.n );
+        const int64_t p0 = slice * kSlice;
+        const int64_t p = p0 + lane;
+        const bool valid = p < n;
+        const int64_t pc = valid ? p : p0;
+        const double* __restrict__ x = scalarLoad( &A.x );
+        const double* __restrict__ u = scalarLoad( &A.U[parity] );
+        const double* __restrict__ volp = scalarLoad( &A.vol );
+        const double* src[NC] = { u, u + n, u + 2 * n, x, x + n, x + 2 * n };
+        if constexpr ( GEN )
+            src[6] = volp;
+        const RowWalk64<NC> rw = rowWalk<NC>( A, slice, lane, src );
+        const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
+        const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
+        // per-slot lane masks: existence (every bond, broken or not) and intact
+        const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + slice );
+        const int32_t nslot = static_cast<int32_t>( scalarLoad( scalarLoad( &A.slice_ptr ) + slice + 1 ) - row0 );
+        const uint64_t* __restrict__ ew = scalarLoad( &A.exist ) + row0;
+        const uint64_t* __restrict__ iw = scalarLoad( &A.intact ) + row0;
+        const double c = A.c;
+        const double tc_i = A.theta_coeff;
+        const double sc_i = A.s_coeff;
+        const double volu = A.vol_value;
+        const int infl1 = A.influence;
+        double num_bonds = 0.0;
+        double m_i = 0.0, theta_i = 0.0;
+        if constexpr ( kIsLps<MODEL> )
+        {
+            m_i = mvals[pc];
+            theta_i = thetavals[pc];
+            if constexpr ( FRAC )
+            {
+                for ( int32_t q = 0; q < nslot; q++ )
+                    num_bonds += static_cast<double>( ( scalarLoad( iw + q ) >> lane ) & 1ull ); // 0 on empty slots
+            }
+            else
+                num_bonds = static_cast<double>( A.cnt[pc] );
+        }
+        const double vol_i = volp[pc];
+        double W = 0.0, phi_i = 0.0, vol_H_i = 0.0;
+        rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr ) {
+            if ( !__builtin_amdgcn_inverse_ballot_w64( scalarLoad( ew + q ) ) )
+                return;
+            Bond64 b;
+            bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], wr[0], wr[kW],
+                           wr[2 * kW], b );
+            const double vj = GEN ? wr[6 * kW] : volu;
+            double wb;
+            if constexpr ( kIsLps<MODEL> )
+            {
+                const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                wb = 1.0 / num_bonds * 0.5 * tc_i / 3.0 * ( theta_i * theta_i ) +
+                     0.5 * ( sc_i / m_i ) * infl * b.s * b.s * b.xi * b.xi * vj;
+            }
+            else
+                wb = 0.25 * c * b.s * b.s * b.xi * vj;
+            if constexpr ( FRAC )
+            {
+                const double mu = static_cast<double>( ( scalarLoad( iw + q ) >> lane ) & 1ull );
+                wb = mu * wb;
+                W += wb;
+                phi_i += mu * vj;
+                vol_H_i += vj;
+            }
+            else
+            {
+                W += wb;
+                Phi += wb * vol_i;
+            }
+        } );
+        if ( valid )
+        {
+            A.W[p] = W;
+            if constexpr ( FRAC )
+            {
+                Phi = W * vol_i;
+                const double ph = 1 - phi_i / vol_H_i;
+                A.phi[p] = ph;
+                D = ph;
+            }
+        }
+        else
+            Phi = 0.0;
+    }
+    for ( int off = 32; off > 0; off >>= 1 )
+    {
+        Phi += __shfl_xor( Phi, off );
+        D += __shfl_xor( D, off );
+    }
+    if ( lane == 0 )
+    {
+        red[0][wv] = Phi;
+        red[1][wv] = D;
+    }
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        double a = 0.0, d = 0.0;
+        for ( int k = 0; k < kSlicesPerBlock; k++ )
+        {
+            a += red[0][k];
+            d += red[1][k];
+        }
+        partial[2 * t] = a;
+        partial[2 * t + 1] = d;
+    }
+}
+
+template <int MODEL, bool FRAC, bool GEN>
+__global__ __launch_bounds__( kBlock ) void kStressRows( const PDArgs* __restrict__ specs, int32_t parity )
+{
+#pragma clang fp contract( off )
+    constexpr int NE = kIsLps<MODEL> ? 2 : 0; // staged lpsA, lpsB
+    constexpr int KV = 6 + NE;                // staged vol (GEN)
+    constexpr int NC = KV + ( GEN ? 1 : 0 );
+    constexpr int kW = RowWalk64<NC>::kW;
+    __shared__ double win[kSlicesPerBlock][2][NC][kW];
+    const PDArgs& A = *specs;
+    const int wv = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t slice = xcdRemap( blockIdx.x, gridDim.x ) * kSlicesPerBlock + wv;
+    if ( slice >= A.nslices )
+        return; // waves are independent: no block-level synchronisation
+    const int64_t n = scalarLoad( &A.n );
+    const int64_t p0 = slice * kSlice;
+    const int64_t p = p0 + lane;
+    const bool valid = p < n;
+    const int64_t pc = valid ? p : p0;
+    const double* __restrict__ x = scalarLoad( &A.x );
+    const double* __restrict__ u = scalarLoad( &A.U[parity] );
+    const double* src[NC];
+    src[0] = u;
+    src[1] = u + n;
+    src[2] = u + 2 * n;
+    src[3] = x;
+    src[4] = x + n;
+    src[5] = x + 2 * n;
+    if constexpr ( NE )
+    {
+        src[6] = scalarLoad( &A.lpsA );
+        src[7] = scalarLoad( &A.lpsB );
+    }
+    if constexpr ( GEN )
+        src[KV] = scalarLoad( &A.vol );
+    const RowWalk64<NC> rw = rowWalk<NC>( A, slice, lane, src );
+    const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
+    const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
+    const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + slice );
+    const uint64_t* __restrict__ ew = scalarLoad( &A.exist ) + row0;
+    const uint64_t* __restrict__ iw = scalarLoad( &A.intact ) + row0;
+    const double c = A.c;
+    const double volu = A.vol_value;
+    const int infl1 = A.influence;
+    double Ai = 0.0, Bi = 0.0;
+    if constexpr ( kIsLps<MODEL> )
+    {
+        Ai = A.lpsA[pc];
+        Bi = A.lpsB[pc];
+    }
+    double s00 = 0.0, s01 = 0.0, s02 = 0.0, s10 = 0.0, s11 = 0.0, s12 = 0.0, s20 = 0.0, s21 = 0.0, s22 = 0.0;
+    rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr ) {
+        uint64_t live = scalarLoad( ew + q );
+        uint64_t intact = ~0ull;
+        if constexpr ( FRAC )
+        {
+            intact = scalarLoad( iw + q );
+            if constexpr ( kIsLps<MODEL> )
+                live &= intact; // LPS fracture visits unbroken bonds only
+        }
+        if ( !__builtin_amdgcn_inverse_ballot_w64( live ) )
+            return;
+        const double xj0 = wr[3 * kW], xj1 = wr[4 * kW], xj2 = wr[5 * kW];
+        Bond64 b;
+        bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, xj0, xj1, xj2, wr[0], wr[kW], wr[2 * kW], b );
+        const double xi_x = xj0 - xi0;
+        const double xi_y = xj1 - xi1;
+        const double xi_z = xj2 - xi2;
+        const double vj = GEN ? wr[KV * kW] : volu;
+        double coeff;
+        if constexpr ( kIsLps<MODEL> )
+        {
+            const double infl = infl1 ? 1.0 / b.xi : 1.0;
+            coeff = ( Ai + wr[6 * kW] + b.s * ( Bi + wr[7 * kW] ) ) * infl * b.xi * vj;
+            coeff *= 0.5;
+        }
+        else
+            coeff = 0.5 * ( c * b.s * vj );
+        double fx, fy, fz;
+        if constexpr ( FRAC )
+        {
+            const double mu = static_cast<double>( ( intact >> lane ) & 1ull );
+            fx = mu * coeff * b.rx / b.r;
+            fy = mu * coeff * b.ry / b.r;
+            fz = mu * coeff * b.rz / b.r;
+        }
+        else
+        {
+            fx = coeff * b.rx / b.r;
+            fy = coeff * b.ry / b.r;
+            fz = coeff * b.rz / b.r;
+        }
+        s00 += fx * xi_x;
+        s11 += fy * xi_y;
+        s22 += fz * xi_z;
+        s01 += fx * xi_y;
+        s10 += fy * xi_x;
+        s02 += fx * xi_z;
+        s20 += fz * xi_x;
+        s12 += fy * xi_z;
+        s21 += fz * xi_y;
+    } );
+    if ( !valid )
+        return;
+    double* __restrict__ st = A.stress;
+    st[p] = s00;
+    st[n + p] = s01;
+    st[2 * n + p] = s02;
+    st[3 * n + p] = s10;
+    st[4 * n + p] = s11;
+    st[5 * n + p] = s12;
+    st[6 * n + p] = s20;
+    st[7 * n + p] = s21;
+    st[8 * n + p] = s22;
+}
+
+// ---------------------------------------------------------------------------
 // Heat transfer (DynamicTemperature): HeatTransfer::computeHeatTransferFull
 // (CabanaPD_HeatTransfer.hpp:41-62, 101-140) -- conduction of particle i over
 // its bonds (intact bonds only with fracture), in the bond order of the other
@@ -2152,6 +2486,8 @@ ForceFn pickForce( const PDArgs& a )
 
 thread_local char g_force_name[64];
 thread_local char g_lps_name[64];
+thread_local char g_energy_name[64];
+thread_local char g_stress_name[64];
 
 } // namespace
 
@@ -2187,8 +2523,24 @@ const char* lpsKernelName( const PDArgs& a )
     return a.fracture ? ( a.precision == PDK_MIXED ? "kLpsState<true, 1>" : "kLpsState<true, 0>" )
                       : ( a.precision == PDK_MIXED ? "kLpsState<false, 1>" : "kLpsState<false, 0>" );
 }
+// The output passes take the row windows wherever the layout has a row table
+// and the specimen has no per-particle material type, temperature or plastic
+// stretch (the same results either way).
+static bool outputRows( const PDArgs& a )
+{
+    return a.rows != nullptr && a.nrows > 0 && a.exist != nullptr && a.mtype == nullptr && a.temp == nullptr &&
+           a.splas == nullptr;
+}
+
 const char* energyKernelName( const PDArgs& a )
 {
+    if ( outputRows( a ) )
+    {
+        std::snprintf( g_energy_name, sizeof( g_energy_name ), "kEnergyRows<%d, %s, %s>", a.model,
+                       a.fracture && !( a.model == kLinPMB || a.model == kLinLPS ) ? "true" : "false",
+                       a.vol_uniform ? "false" : "true" );
+        return g_energy_name;
+    }
     if ( a.model == kLinPMB )
         return "kEnergy<4, false>";
     if ( a.model == kLinLPS )
@@ -2296,6 +2648,31 @@ void launchLpsStateBatched( const PDArgs* specs_dev, const PDArgs& variant, cons
     PDK_HIP( hipGetLastError() );
 }
 
+using EnergyFn = void ( * )( const PDArgs*, int32_t, const double*, const double*, double* );
+template <bool GEN>
+EnergyFn pickEnergyRows( const PDArgs& a )
+{
+    switch ( a.model )
+    {
+    case kLinPMB: return kEnergyRows<kLinPMB, false, GEN>;
+    case kLinLPS: return kEnergyRows<kLinLPS, false, GEN>;
+    case kLPS: return a.fracture ? kEnergyRows<kLPS, true, GEN> : kEnergyRows<kLPS, false, GEN>;
+    default: return a.fracture ? kEnergyRows<kPMB, true, GEN> : kEnergyRows<kPMB, false, GEN>;
+    }
+}
+using StressFn = void ( * )( const PDArgs*, int32_t );
+template <bool GEN>
+StressFn pickStressRows( const PDArgs& a )
+{
+    switch ( a.model )
+    {
+    case kLinPMB: return kStressRows<kLinPMB, false, GEN>;
+    case kLinLPS: return kStressRows<kLinLPS, false, GEN>;
+    case kLPS: return a.fracture ? kStressRows<kLPS, true, GEN> : kStressRows<kLPS, false, GEN>;
+    default: return a.fracture ? kStressRows<kPMB, true, GEN> : kStressRows<kPMB, false, GEN>;
+    }
+}
+
 int64_t energyBlocks( const PDArgs& a )
 {
     return sliceBlocks( a );
@@ -2310,6 +2687,8 @@ void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, doub
               : a.model == kLinLPS ? kEnergy<kLinLPS, false>
               : a.model == kLPS    ? ( a.fracture ? kEnergy<kLPS, true> : kEnergy<kLPS, false> )
                                    : ( a.fracture ? kEnergy<kPMB, true> : kEnergy<kPMB, false> );
+    if ( outputRows( a ) )
+        fn = a.vol_uniform ? pickEnergyRows<false>( a ) : pickEnergyRows<true>( a );
     hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( nblocks ) ), dim3( kBlock ), 0, st, args_dev, parity,
                         a.m, a.theta, partial_dev );
     PDK_HIP( hipGetLastError() );
@@ -2335,6 +2714,8 @@ void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipS
               : a.model == kLinLPS ? kStress<kLinLPS, false>
               : a.model == kLPS    ? ( a.fracture ? kStress<kLPS, true> : kStress<kLPS, false> )
                                    : ( a.fracture ? kStress<kPMB, true> : kStress<kPMB, false> );
+    if ( outputRows( a ) )
+        fn = a.vol_uniform ? pickStressRows<false>( a ) : pickStressRows<true>( a );
     hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, parity );
     PDK_HIP( hipGetLastError() );
 }
@@ -2364,6 +2745,13 @@ void launchStencilAux( const float4* table, int32_t S, int32_t influence, float4
 
 const char* stressKernelName( const PDArgs& a )
 {
+    if ( outputRows( a ) )
+    {
+        std::snprintf( g_stress_name, sizeof( g_stress_name ), "kStressRows<%d, %s, %s>", a.model,
+                       a.fracture && !( a.model == kLinPMB || a.model == kLinLPS ) ? "true" : "false",
+                       a.vol_uniform ? "false" : "true" );
+        return g_stress_name;
+    }
     if ( a.model == kLinPMB )
         return "kStress<4, false>";
     if ( a.model == kLinLPS )
