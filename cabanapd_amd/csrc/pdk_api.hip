@@ -268,6 +268,12 @@ struct pdk_spec_s
     // Specs whose force passes need the FP64 gather variant: several
     // materials, EPP (per-bond state) or a temperature field.
     bool gatherOnly() const { return pair_tables || d.mechanics == PDK_MECH_EPP || d.thermal != PDK_TEMP_INDEPENDENT; }
+    // the row-window force pass covers the specimen (with its features, if any)
+    bool rowsCover() const
+    {
+        return !gatherOnly() || rowFeaturesSupported( d.model, pair_tables, d.thermal != PDK_TEMP_INDEPENDENT,
+                                                      d.mechanics == PDK_MECH_EPP );
+    }
     bool hasPrescribed() const { return pre_u.count > 0 || pre_temp.count > 0; }
     bool dynamicTemp() const { return d.thermal == PDK_TEMP_DYNAMIC; }
     // Work between the initial half step and the force pass of step g:
@@ -390,11 +396,12 @@ struct pdk_spec_s
             h.nrows = 0;
         }
         // tuning knob: -1 = default (row windows where available, else groups
-        // of 4); multi-material specimens use the gather variant
+        // of 4); feature combinations the row windows do not cover use the
+        // gather variant
         h.unroll = unroll;
         if ( h.unroll < 0 )
-            h.unroll = ( h.nrows > 0 && !gatherOnly() ) ? 0 : 4;
-        if ( h.unroll == 0 && ( h.nrows == 0 || gatherOnly() ) )
+            h.unroll = ( h.nrows > 0 && rowsCover() ) ? 0 : 4;
+        if ( h.unroll == 0 && ( h.nrows == 0 || !rowsCover() ) )
             h.unroll = 4;
         h.splas = splas ? splas.as<double>() : nullptr;
         h.sY = k.s_Y;
@@ -2051,8 +2058,8 @@ int pdk_set_tuning( pdk_spec s, int32_t unroll )
         require( s != nullptr, "null spec" );
         require( unroll == 0 || unroll == 1 || unroll == 2 || unroll == 4 || unroll == 8,
                  "unroll must be 0 (row windows), 1, 2, 4 or 8" );
-        require( unroll != 0 || !s->gatherOnly(),
-                 "multi-material, EPP and temperature-dependent specimens use the gather variant" );
+        require( unroll != 0 || s->rowsCover(),
+                 "this combination of materials, EPP and temperature dependence uses the gather variant" );
         require( unroll != 0 || !s->built || s->slot.nrows > 0,
                  "row windows need the stencil-slot layout of a full lattice" );
         s->unroll = unroll;
@@ -2082,7 +2089,7 @@ int pdk_autotune( pdk_spec s, int32_t launches, int32_t* unroll_out )
         float best = 1e30f;
         int32_t best_u = s->unroll;
         std::vector<int32_t> cand;
-        if ( s->slot.nrows > 0 && !s->gatherOnly() )
+        if ( s->slot.nrows > 0 && s->rowsCover() )
             cand.push_back( 0 ); // row windows staged in LDS
         for ( int32_t u : { 1, 2, 4, 8 } )
             cand.push_back( u );

@@ -1239,6 +1239,354 @@ kForceRows( const PDArgs* __restrict__ specs,
 }
 
 // ---------------------------------------------------------------------------
+// The row-window force pass for specimens with per-particle features: several
+// materials (pair tables), TemperatureDependent (per-particle temperature) and
+// ElasticPerfectlyPlastic (per-bond plastic stretch).  FEAT is a set of
+// kFeat* bits.  The window channels are u, x (FP64 form), lpsA/lpsB (LPS),
+// the volume, a tag (2 x material + no-fail flag) and the temperature; the
+// plastic stretch of slot q sits at splas[(row0 + q) * 64 + lane].  Per bond
+// the arithmetic and the order of the sums are those of the gather pass
+// (kForce with the slot layout), so both forms give identical results.
+// ---------------------------------------------------------------------------
+enum : int
+{
+    kFeatMT = 1, // pair tables (multi-material)
+    kFeatTH = 2, // temperature-dependent stretch
+    kFeatEP = 4  // elastic-perfectly-plastic bonds
+};
+
+template <int MODEL, bool FRAC, int PREC, int FEAT>
+__global__ __launch_bounds__( kBlock ) void kForceRowsF( const PDArgs* __restrict__ specs,
+                                                         const int32_t* __restrict__ tile_spec,
+                                                         const int64_t* __restrict__ tile_slice,
+                                                         int32_t epi, int32_t parity, int64_t launch,
+                                                         int32_t accumulate )
+{
+#pragma clang fp contract( off )
+    constexpr bool MT = ( FEAT & kFeatMT ) != 0;
+    constexpr bool TH = ( FEAT & kFeatTH ) != 0 && !kIsLps<MODEL>;
+    constexpr bool EP = ( FEAT & kFeatEP ) != 0 && !kIsLps<MODEL>;
+    constexpr int kW = kSlice + kRowMaxSlots - 1; // window width
+    constexpr int NX = ( PREC == kF64 ) ? 3 : 0;  // staged x components
+    constexpr int NAB = kIsLps<MODEL> ? 2 : 0;    // staged lpsA, lpsB
+    constexpr int KV = 3 + NX + NAB;              // volume
+    constexpr int KT = KV + 1;                    // tag: 2 x material + no-fail flag
+    constexpr bool TAG = MT || ( FRAC && !kIsLps<MODEL> );
+    constexpr int KH = KT + ( TAG ? 1 : 0 );      // temperature
+    constexpr int NC = KH + ( TH ? 1 : 0 );
+    // one buffer per wave: the next row's window waits in registers until
+    // the current row's reads are done
+    __shared__ double win[kSlicesPerBlock][NC][kW];
+    WaveCtx w;
+    if ( !waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w ) )
+        return; // waves are independent: no block-level synchronisation
+    const PDArgs& A = *w.A;
+    const int wv = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t n = scalarLoad( &A.n );
+    const int64_t p0 = w.slice * kSlice;
+    const int64_t p = p0 + lane;
+    const bool valid = p < n;
+    const int64_t pc = valid ? p : p0;
+    const double* __restrict__ x = scalarLoad( &A.x );
+    const double* __restrict__ u = scalarLoad( &A.U[w.cur] );
+    const int4* __restrict__ rows = scalarLoad( &A.rows );
+    double xi0 = 0.0, xi1 = 0.0, xi2 = 0.0;
+    if constexpr ( PREC == kF64 )
+    {
+        xi0 = x[pc];
+        xi1 = x[n + pc];
+        xi2 = x[2 * n + pc];
+    }
+    const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
+    const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + w.slice );
+    uint64_t* __restrict__ bits = scalarLoad( &A.intact ) + row0;
+    const uint64_t* __restrict__ mwords = FRAC ? bits : scalarLoad( &A.exist ) + row0;
+    const float4* __restrict__ sten = scalarLoad( &A.stencil );
+    const double* __restrict__ sxiv = scalarLoad( &A.stencil_xiv );
+    const uint8_t* __restrict__ nfp = scalarLoad( &A.nofail );
+    const uint8_t* __restrict__ mtp = MT ? scalarLoad( &A.mtype ) : nullptr;
+    const double* __restrict__ tmpp = TH ? scalarLoad( &A.temp ) : nullptr;
+    double* __restrict__ spl = EP ? scalarLoad( &A.splas ) + row0 * kSlice + lane : nullptr;
+    const bool nf_i = FRAC ? ( nfp[pc] != 0 ) : false;
+    const int ti4 = MT ? 4 * static_cast<int>( mtp[pc] ) : 0;
+    const double Ti = TH ? tmpp[pc] : 0.0;
+    const double c = A.c;
+    const double bcc = A.bcc;
+    const float c32 = static_cast<float>( A.c );
+    const float s0f = A.s0f;
+    const float bandK = A.bandK;
+    const int infl1 = A.influence;
+    // uniform volume: c vol (PMB, one material) or vol (LPS) scales the sum
+    // (as in the gather pass)
+    const bool fold = A.vol_uniform != 0 && !( FRAC && A.any_nofail != 0 ) && !( MT && !kIsLps<MODEL> );
+    const double rscale = ( PREC == kMixed && fold ) ? A.row_scale : 1.0;
+    double Ai = 0.0, Bi = 0.0;
+    if constexpr ( kIsLps<MODEL> )
+    {
+        Ai = A.lpsA[pc]; // theta_coeff * theta_i / m_i
+        Bi = A.lpsB[pc]; // s_coeff / m_i
+    }
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if ( accumulate && valid )
+    {
+        fx = A.f[p];
+        fy = A.f[n + p];
+        fz = A.f[2 * n + p];
+    }
+    double sx = 0.0, sy = 0.0, sz = 0.0; // mixed form: the bond forces fr r summed in FP64
+
+    const double* src[KV + 1];
+    src[0] = u;
+    src[1] = u + n;
+    src[2] = u + 2 * n;
+    if constexpr ( NX )
+    {
+        src[3] = x;
+        src[4] = x + n;
+        src[5] = x + 2 * n;
+    }
+    if constexpr ( NAB )
+    {
+        src[3 + NX] = scalarLoad( &A.lpsA );
+        src[4 + NX] = scalarLoad( &A.lpsB );
+    }
+    src[KV] = scalarLoad( &A.vol );
+    // row tables exist only for n < 2^28 (byte sizes and offsets fit 32 bits)
+    const int32_t p032 = static_cast<int32_t>( p0 );
+    const uint32_t nbytes = static_cast<uint32_t>( n ) * 8u;
+    __amdgpu_buffer_rsrc_t rs[KV + 1];
+#pragma unroll
+    for ( int k = 0; k <= KV; k++ )
+        rs[k] = bufferOf( src[k], nbytes );
+    const __amdgpu_buffer_rsrc_t rnf = bufferOf( nfp, static_cast<uint32_t>( n ) );
+    const __amdgpu_buffer_rsrc_t rmt = bufferOf( MT ? mtp : nfp, static_cast<uint32_t>( n ) );
+    const __amdgpu_buffer_rsrc_t rth = bufferOf( TH ? tmpp : src[0], nbytes );
+    const uint32_t lane8 = static_cast<uint32_t>( lane ) * 8u;
+    double v0[NC], v1[NC];
+#pragma unroll
+    for ( int k = 0; k < NC; k++ )
+        v1[k] = 0.0;
+    auto loadHalf = [&]( double* dst, int32_t g ) {
+        // g >= 0: lane offset + uniform start; g < 0: per-lane offsets, the
+        // negative ones wrap out of range and read 0
+        const uint32_t vo = g >= 0 ? lane8 : static_cast<uint32_t>( g + lane ) * 8u;
+        const uint32_t so = g >= 0 ? static_cast<uint32_t>( g ) * 8u : 0u;
+#pragma unroll
+        for ( int k = 0; k <= KV; k++ )
+            dst[k] = bufferLoad64( rs[k], vo, so );
+        if constexpr ( TAG )
+        {
+            int tag = 0;
+            if constexpr ( FRAC && !kIsLps<MODEL> )
+                tag = bufferLoad8( rnf, vo >> 3, so >> 3 ) != 0 ? 1 : 0;
+            if constexpr ( MT )
+                tag += 2 * static_cast<int>( bufferLoad8( rmt, vo >> 3, so >> 3 ) );
+            dst[KT] = static_cast<double>( tag );
+        }
+        if constexpr ( TH )
+            dst[KH] = bufferLoad64( rth, vo, so );
+    };
+    auto loadRow = [&]( int r ) {
+        const int4 R = scalarLoad4( rows + r );
+        const int32_t g = p032 + R.z; // uniform
+        loadHalf( v0, g );
+        if ( lane < R.y - 1 )
+            loadHalf( v1, g + kSlice );
+    };
+
+    double* __restrict__ wb = &win[wv][0][0];
+    const int nrows = scalarLoad( &A.nrows );
+    loadRow( 0 );
+    for ( int r = 0; r < nrows; r++ )
+    {
+        const int4 R = scalarLoad4( rows + r );
+        // the previous row's window reads stay above this row's stores (a
+        // lane's read of a position another lane overwrites)
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+#pragma unroll
+        for ( int k = 0; k < NC; k++ )
+        {
+            wb[k * kW + lane] = v0[k];
+            if ( lane < R.y - 1 )
+                wb[k * kW + kSlice + lane] = v1[k];
+        }
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+        if ( r + 1 < nrows )
+            loadRow( r + 1 );
+        auto slot = [&]( const int t ) __attribute__( ( always_inline ) ) {
+            const int32_t q = R.x + t;
+            const uint64_t ww = scalarLoad( mwords + q );
+            const double* __restrict__ wr = wb + lane + t;
+            const bool live = __builtin_amdgcn_inverse_ballot_w64( ww );
+            bool brk = false;
+            if ( live )
+            {
+                const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
+                const double vj = wr[KV * kW];
+                int mj = 0;
+                bool nf_j = false;
+                if constexpr ( TAG )
+                {
+                    const int tag = static_cast<int>( wr[KT * kW] );
+                    mj = tag >> 1;
+                    nf_j = ( tag & 1 ) != 0;
+                }
+                if constexpr ( kIsLps<MODEL> )
+                    nf_j = nf_i; // LPS tests nofail(i) twice (force/CabanaPD_LPS.hpp:417-418)
+                const double tj = TH ? wr[KH * kW] : 0.0;
+                if constexpr ( PREC == kF64 )
+                {
+                    Bond64 b;
+                    bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
+                                   b );
+                    double temp_avg = 0.0;
+                    if constexpr ( TH )
+                    {
+                        temp_avg = thermalAvg( A, Ti, tj );
+                        b.s = b.s - ( A.alpha * temp_avg );
+                    }
+                    if constexpr ( FRAC )
+                    {
+                        const double bcc_ij = TH ? thermalBcc( A, temp_avg ) : ( MT ? A.mat_bcc[ti4 + mj] : bcc );
+                        if ( criticalStretch64( b.r, b.xi, bcc_ij ) && !nf_i && !nf_j )
+                            brk = true;
+                    }
+                    if ( !brk )
+                    {
+                        double coeff;
+                        if constexpr ( !kIsLps<MODEL> )
+                        {
+                            const double sd = EP ? plasticStretch( spl + q * kSlice, b.s, A.sY ) : b.s;
+                            coeff = ( MT ? A.mat_c[ti4 + mj] : c ) * sd * vj;
+                        }
+                        else
+                        {
+                            const double infl = infl1 ? 1.0 / b.xi : 1.0;
+                            coeff = ( Ai + wr[( 3 + NX ) * kW] + b.s * ( Bi + wr[( 4 + NX ) * kW] ) ) * infl * b.xi *
+                                    vj;
+                        }
+                        fx += coeff * b.rx / b.r;
+                        fy += coeff * b.ry / b.r;
+                        fz += coeff * b.rz / b.r;
+                    }
+                }
+                else
+                {
+                    // r = xi + (u_j - u_i) from the slot's FP64 stencil record
+                    const double* xv = sxiv + 4 * q;
+                    const double r0 = ( uj0 - ui0 ) + scalarLoad( xv );
+                    const double r1 = ( uj1 - ui1 ) + scalarLoad( xv + 1 );
+                    const double r2 = ( uj2 - ui2 ) + scalarLoad( xv + 2 );
+                    const float xil = scalarLoad4( sten + q ).w;
+                    Bond32 b;
+                    mixedBond( r0, r1, r2, scalarLoad( xv + 3 ), xil, b );
+                    const float wl = infl1 ? 1.0f : xil; // LPS: infl |xi|
+                    double temp_avg = 0.0;
+                    float ath = 0.0f;
+                    const float s_mech = b.s;
+                    if constexpr ( TH )
+                    {
+                        temp_avg = thermalAvg( A, Ti, tj );
+                        ath = static_cast<float>( A.alpha * temp_avg );
+                        b.s = b.s - ath;
+                    }
+                    if constexpr ( FRAC )
+                    {
+                        if ( !nf_i && !nf_j )
+                        {
+                            // the pair's critical stretch and band; within it
+                            // the reference's FP64 test decides
+                            const float s0p = MT ? A.mat_s0f[ti4 + mj] : s0f;
+                            const float bandp = MT ? A.mat_bandK[ti4 + mj] : bandK;
+                            const float band =
+                                TH ? fmaf( A.bandf, fabsf( s_mech ) + s0p + fabsf( ath ), bandp ) : bandp;
+                            if ( fabsf( b.s - s0p ) <= band )
+                            {
+                                // rare path: the reference positions are loaded
+                                // here (the compiler barrier keeps the loads of
+                                // x_i from being hoisted into registers held for
+                                // the whole pass)
+                                __asm__ volatile( "" ::: "memory" );
+                                const int64_t j = p + R.z + t;
+                                Bond64 e;
+                                getDistance64( x[p], x[n + p], x[2 * n + p], ui0, ui1, ui2, x[j], x[n + j],
+                                               x[2 * n + j], uj0, uj1, uj2, e );
+                                brk = criticalStretch64( e.r, e.xi,
+                                                         TH ? thermalBcc( A, temp_avg )
+                                                            : ( MT ? A.mat_bcc[ti4 + mj] : bcc ) );
+                            }
+                            else
+                                brk = b.s > s0p;
+                        }
+                    }
+                    if ( !brk )
+                    {
+                        float fr;
+                        if constexpr ( !kIsLps<MODEL> )
+                        {
+                            // ElasticPerfectlyPlastic: the plastic stretch stays FP64
+                            const float sd =
+                                EP ? static_cast<float>( plasticStretch( spl + q * kSlice, static_cast<double>( b.s ),
+                                                                         A.sY ) )
+                                   : b.s;
+                            const float cp = MT ? static_cast<float>( A.mat_c[ti4 + mj] ) : c32;
+                            fr = ( fold ? sd : ( cp * static_cast<float>( vj ) ) * sd ) * b.rinv;
+                        }
+                        else
+                        {
+                            const float coeff = static_cast<float>( Ai + wr[( 3 + NX ) * kW] ) +
+                                                b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kW] );
+                            fr = ( fold ? coeff * wl : ( coeff * wl ) * static_cast<float>( vj ) ) * b.rinv;
+                        }
+                        const double frd = static_cast<double>( fr );
+                        sx = fma( frd, r0, sx );
+                        sy = fma( frd, r1, sy );
+                        sz = fma( frd, r2, sz );
+                    }
+                }
+            }
+            if constexpr ( FRAC )
+            {
+                // wave-uniform test first (rare: a bond of the wave breaks),
+                // then one lane writes the word
+                const uint64_t m = __ballot( brk ) & ww;
+                if ( __builtin_expect( m != 0, 0 ) )
+                {
+                    if ( lane == 0 )
+                        bits[q] = ww & ~m;
+                }
+            }
+        };
+        if constexpr ( PREC == kMixed )
+        {
+#pragma unroll
+            for ( int t = 0; t < kRowMaxSlots; t++ )
+                if ( t < R.y )
+                    slot( t );
+        }
+        else
+        {
+            for ( int t = 0; t < R.y; t++ )
+                slot( t );
+        }
+    }
+    if constexpr ( PREC == kMixed )
+    {
+        fx = fma( rscale, sx, fx );
+        fy = fma( rscale, sy, fy );
+        fz = fma( rscale, sz, fz );
+    }
+    if ( !valid )
+        return;
+    epilogue( A, p, w.epi, w.cur, fx, fy, fz );
+}
+
+// ---------------------------------------------------------------------------
 // LPS state pass: Force<LPS,*>::computeWeightedVolume + computeDilatation
 // (force/CabanaPD_LPS.hpp:89-141, 287-378), fused: m_i and theta_i need only
 // particle i's own bonds.  Also stores the per-particle force-coefficient
@@ -2477,10 +2825,44 @@ ForceFn pickForceRows( const PDArgs& a )
     return rowsGeneral( a ) ? pickForceRowsGen<true>( a ) : pickForceRowsGen<false>( a );
 }
 
+// Per-particle features of the force pass (kFeat* bits): several materials,
+// temperature, plastic stretch (the LPS model has the first only).
+int rowFeatures( const PDArgs& a )
+{
+    const bool lps = a.model == kLPS || a.model == kLinLPS;
+    return ( a.mtype ? kFeatMT : 0 ) | ( !lps && a.temp ? kFeatTH : 0 ) | ( !lps && a.splas ? kFeatEP : 0 );
+}
+
+template <int MODEL, int FEAT>
+ForceFn pickRowsFeat( const PDArgs& a )
+{
+    const bool mixed = a.precision == PDK_MIXED;
+    if constexpr ( ( FEAT & kFeatEP ) != 0 ) // ElasticPerfectlyPlastic exists with fracture only
+        return mixed ? kForceRowsF<MODEL, true, kMixed, FEAT> : kForceRowsF<MODEL, true, kF64, FEAT>;
+    else if ( a.fracture )
+        return mixed ? kForceRowsF<MODEL, true, kMixed, FEAT> : kForceRowsF<MODEL, true, kF64, FEAT>;
+    else
+        return mixed ? kForceRowsF<MODEL, false, kMixed, FEAT> : kForceRowsF<MODEL, false, kF64, FEAT>;
+}
+
+// The combinations the feature instances cover: several materials (PMB or
+// LPS), or a temperature and / or a plastic stretch (PMB); the others take
+// the gathers.
+ForceFn pickForceRowsFeat( const PDArgs& a )
+{
+    switch ( rowFeatures( a ) )
+    {
+    case kFeatMT: return a.model == kLPS ? pickRowsFeat<kLPS, kFeatMT>( a ) : pickRowsFeat<kPMB, kFeatMT>( a );
+    case kFeatTH: return pickRowsFeat<kPMB, kFeatTH>( a );
+    case kFeatEP: return pickRowsFeat<kPMB, kFeatEP>( a );
+    default: return pickRowsFeat<kPMB, kFeatTH | kFeatEP>( a );
+    }
+}
+
 ForceFn pickForce( const PDArgs& a )
 {
     if ( useRowWindows( a ) )
-        return pickForceRows( a );
+        return rowFeatures( a ) ? pickForceRowsFeat( a ) : pickForceRows( a );
     return a.exist ? pickForceForm<true>( a ) : pickForceForm<false>( a );
 }
 
@@ -2496,8 +2878,21 @@ int normalizeUnroll( int u )
     return ( u == 1 || u == 2 || u == 8 ) ? u : 4;
 }
 
+bool rowFeaturesSupported( int32_t model, bool pair_tables, bool thermal, bool plastic )
+{
+    if ( pair_tables )
+        return ( model == kPMB || model == kLPS ) && !thermal && !plastic;
+    return model == kPMB || !( thermal || plastic );
+}
+
 const char* forceKernelName( const PDArgs& a )
 {
+    if ( useRowWindows( a ) && rowFeatures( a ) )
+    {
+        std::snprintf( g_force_name, sizeof( g_force_name ), "kForceRowsF<%d, %s, %d, %d>", a.model,
+                       a.fracture ? "true" : "false", a.precision == PDK_MIXED ? 1 : 0, rowFeatures( a ) );
+        return g_force_name;
+    }
     if ( useRowWindows( a ) )
     {
         std::snprintf( g_force_name, sizeof( g_force_name ), "kForceRows<%d, %s, %d, %s>", a.model,

@@ -257,6 +257,9 @@ void launchForceBatched( const PDArgs* specs_dev, const PDArgs& variant, const i
 void launchLpsStateBatched( const PDArgs* specs_dev, const PDArgs& variant, const int32_t* tile_spec,
                             const int64_t* tile_slice, int64_t ntiles, int64_t launch_index,
                             hipStream_t st );
+// Whether the row-window force pass covers a specimen with these features
+// (several materials as pair tables, a temperature, a plastic stretch).
+bool rowFeaturesSupported( int32_t model, bool pair_tables, bool thermal, bool plastic );
 const char* forceKernelName( const PDArgs& a );
 const char* lpsKernelName( const PDArgs& a );
 const char* energyKernelName( const PDArgs& a );

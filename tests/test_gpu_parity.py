@@ -728,8 +728,8 @@ def test_gpu_heat_transfer_bitwise():
 
 
 def test_gpu_plastic_thermal_rejections():
-    """The gather variant only; prescribed specs stay out of the batch
-    engine; prescribed ids are checked."""
+    """LinearPMB with a temperature has the gather variant only; prescribed
+    specs stay out of the batch engine; prescribed ids are checked."""
     from cabanapd_amd import engine
     from cabanapd_amd._abi import make_desc
     say("case: EPP / thermal argument checks")
@@ -737,11 +737,16 @@ def test_gpu_plastic_thermal_rejections():
     base = dict(horizon=w["desc"].horizon, K=1e9, dt=1e-6)
     with pytest.raises(engine.PdkError, match="fracture"):
         gpu_spec(w, desc=make_desc("PMB", mechanics="epp", sigma_y=1e6, **base))
-    w["desc"] = make_desc("PMB", fracture=True, G0=1.0, mechanics="epp", sigma_y=1e6, **base)
+    # LinearPMB with a temperature: the gathers only
+    w["desc"] = make_desc("LINEAR_PMB", thermal=True, alpha=1e-5, **base)
     g = gpu_spec(w)
     configs.setup(g, w)
     with pytest.raises(engine.PdkError, match="gather"):
         g.set_tuning(0)
+    g.close()
+    w["desc"] = make_desc("PMB", fracture=True, G0=1.0, mechanics="epp", sigma_y=1e6, **base)
+    g = gpu_spec(w)
+    configs.setup(g, w)
     with pytest.raises(engine.PdkError, match="repeated"):
         g.set_prescribed("u", [1, 1], np.zeros((2, 3)))
     with pytest.raises(engine.PdkError, match="temperature"):
@@ -757,3 +762,75 @@ def test_gpu_plastic_thermal_rejections():
     with pytest.raises(engine.PdkError, match="prescribed"):
         ctx().submit(e, 1)
     e.close()
+
+
+def _feature_plate(case):
+    """The 24x48x6 impact plate (a full lattice: row windows apply) with
+    random displacements and the features of `case`: mt_pmb / mt_lps (two
+    materials scattered at random), th (TemperatureDependent, random
+    temperatures), ep (ElasticPerfectlyPlastic), th_ep (both), each with
+    fracture; th_nf without fracture."""
+    from cabanapd_amd._abi import make_desc
+    w = configs.kw_plate(num_cells=(24, 48, 6), size=(0.06, 0.12, 0.015), steps=25)
+    bd = w["desc"]
+    x = w["x"]
+    n = x.shape[0]
+    rng = np.random.default_rng(21)
+    w["u"] = rng.uniform(-1e-6, 1e-6, size=x.shape)
+    kw = dict(horizon=bd.horizon, K=bd.K, dt=bd.dt)
+    if case.startswith("mt"):
+        model = "LPS" if case == "mt_lps" else "PMB"
+        w["desc"] = make_desc(model, fracture=True, G=0.6 * bd.K, G0=bd.G0, **kw)
+        w["materials"] = ([(bd.K, 0.6 * bd.K, bd.G0), (2.0 * bd.K, 1.2 * bd.K, 3.0 * bd.G0)],
+                          rng.integers(0, 2, n).astype(np.int32))
+        return w
+    if case != "th_nf":
+        kw.update(fracture=True, G0=bd.G0)
+    else:
+        w["notches"] = None
+    if "ep" in case:
+        kw.update(mechanics="epp", sigma_y=3.0 * bd.K * 1e-3)
+    if "th" in case:
+        kw.update(thermal=True, alpha=1e-5, temp0=10.0)
+        w["temperature"] = rng.uniform(0.0, 400.0, size=n)
+    w["desc"] = make_desc("PMB", **kw)
+    return w
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["mt_pmb", "mt_lps", "th", "th_nf", "ep", "th_ep"])
+def test_gpu_feature_row_windows_match_gathers(case, precision):
+    """The row-window force pass of specimens with several materials, a
+    temperature or a plastic stretch evaluates the gather pass's expressions
+    in the gather pass's order: u, v, f and the broken-bond set
+    bit-identical after 25 steps, in both precision forms, and the FP64 form
+    equal to the oracle (the plastic
+    stretch of an EPP bond feeds its force, so u and f cover it)."""
+    from cabanapd_amd import engine
+    say(f"case: feature row windows vs gathers, {case} {precision}, plate 24x48x6, 25 steps")
+    w = _feature_plate(case)
+    out = {}
+    for variant in (0, 4):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.precision = {"fp64": 0, "mixed": 1}[precision]
+        s = engine.Spec(ctx(), d, w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        s.set_tuning(variant)
+        name = s.kernel_name(0)
+        say(f"  variant {variant}: {name}")
+        assert name.startswith("kForceRowsF<") == (variant == 0)
+        s.step(25)
+        out[variant] = [s.get("u"), s.get("v"), s.get("f"), s.bonds()[2]]
+        s.close()
+    for a, b, what in zip(out[0], out[4], ("u", "v", "f", "intact")):
+        assert_same(a, b, f"{what} (row windows vs gathers)")
+    if precision == "fp64":
+        o = oracle_spec(w)
+        configs.setup(o, w)
+        o.step(25)
+        for a, fld in zip(out[0], ("u", "v", "f")):
+            assert_same(a, o.get(fld), f"{fld} (row windows vs oracle)")
+    if case != "th_nf":
+        nb = int(np.sum(out[0][3] == 0))
+        say(f"  broken bonds {nb}")
