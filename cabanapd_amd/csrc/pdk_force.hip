@@ -111,6 +111,22 @@ __device__ __forceinline__ double plasticStretch( double* slot, double s, double
     }
     return s - sp;
 }
+// The same with the slot's stored plastic stretch already loaded (sp).
+__device__ __forceinline__ double plasticStretchAt( double* slot, double sp, double s, double sY )
+{
+#pragma clang fp contract( off )
+    if ( s >= sp + sY )
+    {
+        sp = s - sY;
+        *slot = sp;
+    }
+    else if ( s <= sp - sY )
+    {
+        sp = s + sY;
+        *slot = sp;
+    }
+    return s - sp;
+}
 // EPP EnergyTag stretch term (force_models/CabanaPD_PMB.hpp:123-146).
 __device__ __forceinline__ double plasticEnergyTerm( double s, double sp, double sY )
 {
@@ -1255,8 +1271,12 @@ enum : int
     kFeatEP = 4  // elastic-perfectly-plastic bonds
 };
 
+// Waves per SIMD the register allocator must leave room for: 6 for the mixed
+// PMB instances with several materials or a plastic stretch (measured 8 % and
+// 10 % faster on the fracture plate than at 4-5 waves); the temperature
+// instances spill at 6 and stay unconstrained.
 template <int MODEL, bool FRAC, int PREC, int FEAT>
-__global__ __launch_bounds__( kBlock ) void kForceRowsF( const PDArgs* __restrict__ specs,
+__global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( PREC == kMixed && !kIsLps<MODEL> && ( FEAT == kFeatMT || FEAT == kFeatEP ) ) ? 6 : 1, 8 ) ) ) void kForceRowsF( const PDArgs* __restrict__ specs,
                                                          const int32_t* __restrict__ tile_spec,
                                                          const int64_t* __restrict__ tile_slice,
                                                          int32_t epi, int32_t parity, int64_t launch,
@@ -1277,9 +1297,26 @@ __global__ __launch_bounds__( kBlock ) void kForceRowsF( const PDArgs* __restric
     // one buffer per wave: the next row's window waits in registers until
     // the current row's reads are done
     __shared__ double win[kSlicesPerBlock][NC][kW];
+    // several materials: the pair tables (c, bond_break_coeff, FP32 s0 and
+    // band) of the block's spec, indexed 4 a + b
+    __shared__ double tab_c[MT ? 16 : 1], tab_bcc[MT ? 16 : 1];
+    __shared__ float tab_s0[MT ? 16 : 1], tab_band[MT ? 16 : 1];
     WaveCtx w;
-    if ( !waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w ) )
-        return; // waves are independent: no block-level synchronisation
+    const bool active = waveSetup( specs, tile_spec, tile_slice, epi, parity, launch, w );
+    if constexpr ( MT )
+    {
+        // every wave of a block belongs to one spec
+        if ( threadIdx.x < 16 )
+        {
+            tab_c[threadIdx.x] = w.A->mat_c[threadIdx.x];
+            tab_bcc[threadIdx.x] = w.A->mat_bcc[threadIdx.x];
+            tab_s0[threadIdx.x] = w.A->mat_s0f[threadIdx.x];
+            tab_band[threadIdx.x] = w.A->mat_bandK[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if ( !active )
+        return; // waves are otherwise independent: no further block-level synchronisation
     const PDArgs& A = *w.A;
     const int wv = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
     const int lane = threadIdx.x & ( kSlice - 1 );
@@ -1423,24 +1460,30 @@ __global__ __launch_bounds__( kBlock ) void kForceRowsF( const PDArgs* __restric
             const uint64_t ww = scalarLoad( mwords + q );
             const double* __restrict__ wr = wb + lane + t;
             const bool live = __builtin_amdgcn_inverse_ballot_w64( ww );
-            bool brk = false;
-            if ( live )
+            int mj = 0;
+            bool nf_j = false;
+            if constexpr ( TAG )
             {
-                const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
-                const double vj = wr[KV * kW];
-                int mj = 0;
-                bool nf_j = false;
-                if constexpr ( TAG )
+                const int tag = static_cast<int>( wr[KT * kW] );
+                mj = tag >> 1;
+                nf_j = ( tag & 1 ) != 0;
+            }
+            if constexpr ( kIsLps<MODEL> )
+                nf_j = nf_i; // LPS tests nofail(i) twice (force/CabanaPD_LPS.hpp:417-418)
+            // ElasticPerfectlyPlastic: the slot's plastic stretch is loaded
+            // up front (every lane has an entry), so its latency overlaps the
+            // bond arithmetic; it is written back only where it changes
+            double spv = 0.0;
+            if constexpr ( EP )
+                spv = spl[q * kSlice];
+            if constexpr ( PREC == kF64 )
+            {
+                bool brk = false;
+                if ( live )
                 {
-                    const int tag = static_cast<int>( wr[KT * kW] );
-                    mj = tag >> 1;
-                    nf_j = ( tag & 1 ) != 0;
-                }
-                if constexpr ( kIsLps<MODEL> )
-                    nf_j = nf_i; // LPS tests nofail(i) twice (force/CabanaPD_LPS.hpp:417-418)
-                const double tj = TH ? wr[KH * kW] : 0.0;
-                if constexpr ( PREC == kF64 )
-                {
+                    const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
+                    const double vj = wr[KV * kW];
+                    const double tj = TH ? wr[KH * kW] : 0.0;
                     Bond64 b;
                     bond64<MODEL>( xi0, xi1, xi2, ui0, ui1, ui2, wr[3 * kW], wr[4 * kW], wr[5 * kW], uj0, uj1, uj2,
                                    b );
@@ -1452,7 +1495,7 @@ __global__ __launch_bounds__( kBlock ) void kForceRowsF( const PDArgs* __restric
                     }
                     if constexpr ( FRAC )
                     {
-                        const double bcc_ij = TH ? thermalBcc( A, temp_avg ) : ( MT ? A.mat_bcc[ti4 + mj] : bcc );
+                        const double bcc_ij = TH ? thermalBcc( A, temp_avg ) : ( MT ? tab_bcc[ti4 + mj] : bcc );
                         if ( criticalStretch64( b.r, b.xi, bcc_ij ) && !nf_i && !nf_j )
                             brk = true;
                     }
@@ -1461,8 +1504,8 @@ __global__ __launch_bounds__( kBlock ) void kForceRowsF( const PDArgs* __restric
                         double coeff;
                         if constexpr ( !kIsLps<MODEL> )
                         {
-                            const double sd = EP ? plasticStretch( spl + q * kSlice, b.s, A.sY ) : b.s;
-                            coeff = ( MT ? A.mat_c[ti4 + mj] : c ) * sd * vj;
+                            const double sd = EP ? plasticStretchAt( spl + q * kSlice, spv, b.s, A.sY ) : b.s;
+                            coeff = ( MT ? tab_c[ti4 + mj] : c ) * sd * vj;
                         }
                         else
                         {
@@ -1475,90 +1518,108 @@ __global__ __launch_bounds__( kBlock ) void kForceRowsF( const PDArgs* __restric
                         fz += coeff * b.rz / b.r;
                     }
                 }
-                else
+                if constexpr ( FRAC )
                 {
-                    // r = xi + (u_j - u_i) from the slot's FP64 stencil record
-                    const double* xv = sxiv + 4 * q;
-                    const double r0 = ( uj0 - ui0 ) + scalarLoad( xv );
-                    const double r1 = ( uj1 - ui1 ) + scalarLoad( xv + 1 );
-                    const double r2 = ( uj2 - ui2 ) + scalarLoad( xv + 2 );
-                    const float xil = scalarLoad4( sten + q ).w;
-                    Bond32 b;
-                    mixedBond( r0, r1, r2, scalarLoad( xv + 3 ), xil, b );
-                    const float wl = infl1 ? 1.0f : xil; // LPS: infl |xi|
-                    double temp_avg = 0.0;
-                    float ath = 0.0f;
-                    const float s_mech = b.s;
-                    if constexpr ( TH )
+                    // wave-uniform test first (rare: a bond of the wave
+                    // breaks), then one lane writes the word
+                    const uint64_t m = __ballot( brk ) & ww;
+                    if ( __builtin_expect( m != 0, 0 ) )
                     {
-                        temp_avg = thermalAvg( A, Ti, tj );
-                        ath = static_cast<float>( A.alpha * temp_avg );
-                        b.s = b.s - ath;
-                    }
-                    if constexpr ( FRAC )
-                    {
-                        if ( !nf_i && !nf_j )
-                        {
-                            // the pair's critical stretch and band; within it
-                            // the reference's FP64 test decides
-                            const float s0p = MT ? A.mat_s0f[ti4 + mj] : s0f;
-                            const float bandp = MT ? A.mat_bandK[ti4 + mj] : bandK;
-                            const float band =
-                                TH ? fmaf( A.bandf, fabsf( s_mech ) + s0p + fabsf( ath ), bandp ) : bandp;
-                            if ( fabsf( b.s - s0p ) <= band )
-                            {
-                                // rare path: the reference positions are loaded
-                                // here (the compiler barrier keeps the loads of
-                                // x_i from being hoisted into registers held for
-                                // the whole pass)
-                                __asm__ volatile( "" ::: "memory" );
-                                const int64_t j = p + R.z + t;
-                                Bond64 e;
-                                getDistance64( x[p], x[n + p], x[2 * n + p], ui0, ui1, ui2, x[j], x[n + j],
-                                               x[2 * n + j], uj0, uj1, uj2, e );
-                                brk = criticalStretch64( e.r, e.xi,
-                                                         TH ? thermalBcc( A, temp_avg )
-                                                            : ( MT ? A.mat_bcc[ti4 + mj] : bcc ) );
-                            }
-                            else
-                                brk = b.s > s0p;
-                        }
-                    }
-                    if ( !brk )
-                    {
-                        float fr;
-                        if constexpr ( !kIsLps<MODEL> )
-                        {
-                            // ElasticPerfectlyPlastic: the plastic stretch stays FP64
-                            const float sd =
-                                EP ? static_cast<float>( plasticStretch( spl + q * kSlice, static_cast<double>( b.s ),
-                                                                         A.sY ) )
-                                   : b.s;
-                            const float cp = MT ? static_cast<float>( A.mat_c[ti4 + mj] ) : c32;
-                            fr = ( fold ? sd : ( cp * static_cast<float>( vj ) ) * sd ) * b.rinv;
-                        }
-                        else
-                        {
-                            const float coeff = static_cast<float>( Ai + wr[( 3 + NX ) * kW] ) +
-                                                b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kW] );
-                            fr = ( fold ? coeff * wl : ( coeff * wl ) * static_cast<float>( vj ) ) * b.rinv;
-                        }
-                        const double frd = static_cast<double>( fr );
-                        sx = fma( frd, r0, sx );
-                        sy = fma( frd, r1, sy );
-                        sz = fma( frd, r2, sz );
+                        if ( lane == 0 )
+                            bits[q] = ww & ~m;
                     }
                 }
             }
-            if constexpr ( FRAC )
+            else
             {
-                // wave-uniform test first (rare: a bond of the wave breaks),
-                // then one lane writes the word
-                const uint64_t m = __ballot( brk ) & ww;
-                if ( __builtin_expect( m != 0, 0 ) )
+                // every lane evaluates (window values are finite for every
+                // lane); bonds that do not exist, are broken or break add
+                // exactly zero.  r = xi + (u_j - u_i) from the slot's FP64
+                // stencil record.
+                const double* xv = sxiv + 4 * q;
+                const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
+                const double r0 = ( uj0 - ui0 ) + scalarLoad( xv );
+                const double r1 = ( uj1 - ui1 ) + scalarLoad( xv + 1 );
+                const double r2 = ( uj2 - ui2 ) + scalarLoad( xv + 2 );
+                const float xil = scalarLoad4( sten + q ).w;
+                Bond32 b;
+                mixedBond( r0, r1, r2, scalarLoad( xv + 3 ), xil, b );
+                double temp_avg = 0.0;
+                float ath = 0.0f;
+                const float s_mech = b.s;
+                if constexpr ( TH )
                 {
-                    if ( lane == 0 )
-                        bits[q] = ww & ~m;
+                    temp_avg = thermalAvg( A, Ti, wr[KH * kW] );
+                    ath = static_cast<float>( A.alpha * temp_avg );
+                    b.s = b.s - ath;
+                }
+                uint64_t brkm = 0; // bonds that break in this pass
+                if constexpr ( FRAC )
+                {
+                    // candidates: intact, neither end no-fail; FP32 decision
+                    // except within the pair's band of s0, where the
+                    // reference's FP64 test decides
+                    const uint64_t cand = ww & __ballot( !nf_i && !nf_j );
+                    const float s0p = MT ? tab_s0[ti4 + mj] : s0f;
+                    const float bandp = MT ? tab_band[ti4 + mj] : bandK;
+                    const float band = TH ? fmaf( A.bandf, fabsf( s_mech ) + s0p + fabsf( ath ), bandp ) : bandp;
+                    const uint64_t inb = __ballot( fabsf( b.s - s0p ) <= band ) & cand;
+                    brkm = __ballot( b.s > s0p ) & cand & ~inb;
+                    if ( inb )
+                    {
+                        bool crit = false;
+                        if ( __builtin_amdgcn_inverse_ballot_w64( inb ) )
+                        {
+                            // rare path: the reference positions are loaded
+                            // here (the compiler barrier keeps the loads of
+                            // x_i from being hoisted into registers held for
+                            // the whole pass)
+                            __asm__ volatile( "" ::: "memory" );
+                            const int64_t j = p + R.z + t;
+                            Bond64 e;
+                            getDistance64( x[p], x[n + p], x[2 * n + p], ui0, ui1, ui2, x[j], x[n + j], x[2 * n + j],
+                                           uj0, uj1, uj2, e );
+                            crit = criticalStretch64( e.r, e.xi,
+                                                      TH ? thermalBcc( A, temp_avg ) : ( MT ? tab_bcc[ti4 + mj] : bcc ) );
+                        }
+                        brkm |= __ballot( crit ) & inb;
+                    }
+                }
+                const bool ok = __builtin_amdgcn_inverse_ballot_w64( ww & ~brkm );
+                float fr;
+                if constexpr ( !kIsLps<MODEL> )
+                {
+                    // ElasticPerfectlyPlastic: the plastic stretch stays FP64
+                    // and is updated on the bonds that carry force
+                    float sd = b.s;
+                    if constexpr ( EP )
+                    {
+                        if ( ok )
+                            sd = static_cast<float>(
+                                plasticStretchAt( spl + q * kSlice, spv, static_cast<double>( b.s ), A.sY ) );
+                    }
+                    const float cp = MT ? static_cast<float>( tab_c[ti4 + mj] ) : c32;
+                    fr = ( fold ? sd : ( cp * static_cast<float>( wr[KV * kW] ) ) * sd ) * b.rinv;
+                }
+                else
+                {
+                    const float wl = infl1 ? 1.0f : xil; // infl |xi|
+                    const float coeff = static_cast<float>( Ai + wr[( 3 + NX ) * kW] ) +
+                                        b.s * static_cast<float>( Bi + wr[( 4 + NX ) * kW] );
+                    fr = ( fold ? coeff * wl : ( coeff * wl ) * static_cast<float>( wr[KV * kW] ) ) * b.rinv;
+                }
+                fr = ok ? fr : 0.0f;
+                const double frd = static_cast<double>( fr );
+                sx = fma( frd, r0, sx );
+                sy = fma( frd, r1, sy );
+                sz = fma( frd, r2, sz );
+                if constexpr ( FRAC )
+                {
+                    if ( __builtin_expect( brkm != 0, 0 ) )
+                    {
+                        if ( lane == 0 )
+                            bits[q] = ww & ~brkm;
+                    }
                 }
             }
         };
