@@ -1113,7 +1113,7 @@ void finishSlots( int64_t n, int64_t nslices, const LatticeBox& L, const std::ve
     std::vector<int32_t> soff( 256, 0 );
     std::vector<float4> tab( 256, make_float4( 0.f, 0.f, 0.f, 1.f ) );
     std::vector<double> tab2( 256, 1.0 );
-    std::vector<double> tabv( 4 * 256, 0.0 ); // (xi_x, xi_y, xi_z, |xi|^2) in FP64 per entry
+    std::vector<double> tabv( 4 * 256, 0.0 ); // xi_x, |xi|^2, xi_y, xi_z in FP64 per entry
     for ( int32_t e = 0; e < S; e++ )
     {
         soff[e] = static_cast<int32_t>( es[e].off );
@@ -1121,10 +1121,10 @@ void finishSlots( int64_t n, int64_t nslices, const LatticeBox& L, const std::ve
         tab[e] = make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
                               static_cast<float>( std::sqrt( es[e].r2 ) ) );
         tab2[e] = gx * gx + gy * gy + gz * gz;
-        tabv[4 * e + 0] = gx;
-        tabv[4 * e + 1] = gy;
-        tabv[4 * e + 2] = gz;
-        tabv[4 * e + 3] = tab2[e];
+        tabv[4 * e + kXiX] = gx;
+        tabv[4 * e + kXiY] = gy;
+        tabv[4 * e + kXiZ] = gz;
+        tabv[4 * e + kXi2] = tab2[e];
     }
     DevBuf dxiv;
     dxiv.alloc( tabv.size() * sizeof( double ), acct );

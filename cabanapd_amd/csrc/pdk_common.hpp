@@ -45,6 +45,11 @@ enum EpilogueMode : int32_t
     kEpiApplyOnly = 3     // store f after BC/body terms; the final half step runs as its own pass
 };
 
+// Layout of an entry of the FP64 stencil record (PDArgs::stencil_xiv): xi_x
+// and |xi|^2 first, so the row-window pass loads the two values a slot needs
+// with one scalar load (xi_y and xi_z are per lattice row).
+constexpr int kXiX = 0, kXi2 = 1, kXiY = 2, kXiZ = 3;
+
 // Everything a PD/contact kernel needs for one specimen.  Lives in device
 // memory (one per spec, or an array of them for a packed batch).
 struct PDArgs
@@ -78,7 +83,7 @@ struct PDArgs
     // weighted-volume and dilatation factor)
     const float4* stencil_aux;
     const double* stencil_xi2d; // |xi|^2 of each stencil entry in FP64 (exact lattice offsets)
-    const double* stencil_xiv;  // stencil-slot form: (xi_x, xi_y, xi_z, |xi|^2) of each entry in FP64
+    const double* stencil_xiv;  // stencil-slot form: 4 FP64 values per entry at kXiX, kXi2, kXiY, kXiZ
     double* m;
     double* theta;
     double* lpsA;           // LPS: theta_coeff * theta / m per particle

@@ -716,9 +716,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                             {
                                 // slot q is wave-uniform: scalar loads of its FP64 record
                                 const double* xv = sxiv + 4 * q;
-                                g0 = scalarLoad( xv );
-                                g1 = scalarLoad( xv + 1 );
-                                g2 = scalarLoad( xv + 2 );
+                                g0 = scalarLoad( xv + kXiX );
+                                g1 = scalarLoad( xv + kXiY );
+                                g2 = scalarLoad( xv + kXiZ );
                             }
                             else
                             {
@@ -736,9 +736,12 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                             const float x2f = static_cast<float>( xi2d );
                             xil = x2f * __builtin_amdgcn_rsqf( x2f );
                         }
+                        // (the y and z terms add the displacement of j last,
+                        // so the row-window pass can form xi - u_i once per
+                        // lattice row)
                         r0 = ( uj[k][0] - ui0 ) + g0;
-                        r1 = ( uj[k][1] - ui1 ) + g1;
-                        r2 = ( uj[k][2] - ui2 ) + g2;
+                        r1 = uj[k][1] + ( g1 - ui1 );
+                        r2 = uj[k][2] + ( g2 - ui2 );
                         mixedBond( r0, r1, r2, xi2d, xil, b );
                         wl = infl1 ? 1.0f : xil;
                     }
@@ -946,6 +949,9 @@ kForceRows( const PDArgs* __restrict__ specs,
     const double volu = A.vol_value;
     const float s0f = A.s0f;
     const float bandK = A.bandK;
+    // lower edge of the decision path: every stretch within bandK of s0 or
+    // above it lies at or above s0 - 2 bandK
+    const float s0lo = s0f - 2.0f * bandK;
     // uniform volume: c vol (PMB) or vol (LPS) scales each FP32 row sum
     const double rscale = GEN ? 1.0 : A.row_scale;
     const int infl1 = A.influence;
@@ -1054,6 +1060,14 @@ kForceRows( const PDArgs* __restrict__ specs,
         const float4* __restrict__ st_r = sten + R.x;
         const float4* __restrict__ sa_r = saux + R.x;
         const double* __restrict__ xv_r = sxiv + 4 * R.x;
+        // XV: xi_y - u_i,y and xi_z - u_i,z, one value each for the row (the
+        // slots of a lattice row differ in xi_x only)
+        double ay = 0.0, az = 0.0;
+        if constexpr ( XV )
+        {
+            ay = scalarLoad( xv_r + kXiY ) - ui1;
+            az = scalarLoad( xv_r + kXiZ ) - ui2;
+        }
         // The wave-uniform data of slot t: its intact word (fracture) or
         // existence word, its FP32 stencil entries and (XV) its FP64 offset
         // and |xi|^2 (one 32-byte record).  Loading slot t + 1's data while
@@ -1064,7 +1078,7 @@ kForceRows( const PDArgs* __restrict__ specs,
         {
             uint64_t ww;
             float4 g, ga;
-            double x0, x1, x2, x2d;
+            double x0, x2d;
         };
         auto loadK = [&]( const int t ) __attribute__( ( always_inline ) ) {
             SlotK K{};
@@ -1076,10 +1090,8 @@ kForceRows( const PDArgs* __restrict__ specs,
                 if constexpr ( XV )
                 {
                     const double* xv = xv_r + 4 * t;
-                    K.x0 = scalarLoad( xv );
-                    K.x1 = scalarLoad( xv + 1 );
-                    K.x2 = scalarLoad( xv + 2 );
-                    K.x2d = scalarLoad( xv + 3 );
+                    K.x0 = scalarLoad( xv + kXiX );
+                    K.x2d = scalarLoad( xv + kXi2 );
                 }
             }
             return K;
@@ -1147,8 +1159,8 @@ kForceRows( const PDArgs* __restrict__ specs,
                 if constexpr ( XV )
                 {
                     r0 = ( uj0 - ui0 ) + K.x0;
-                    r1 = ( uj1 - ui1 ) + K.x1;
-                    r2 = ( uj2 - ui2 ) + K.x2;
+                    r1 = uj1 + ay;
+                    r2 = uj2 + az;
                     mixedBond( r0, r1, r2, K.x2d, g.w, b );
                 }
                 else
@@ -1156,7 +1168,11 @@ kForceRows( const PDArgs* __restrict__ specs,
                 if constexpr ( FRAC )
                 {
                     // FP32 decision, except within bandK of s0 where the
-                    // reference's FP64 test decides
+                    // reference's FP64 test decides.  One compare first:
+                    // only a wave with a candidate at or above s0 - 2 bandK
+                    // (rare: bonds near breaking) takes the decision path.
+                    if ( __ballot( b.s >= s0lo ) & cand )
+                    {
                     const uint64_t inb = __ballot( fabsf( b.s - s0f ) <= bandK ) & cand;
                     brkm = __ballot( b.s > s0f ) & cand & ~inb;
                     if ( inb )
@@ -1180,6 +1196,7 @@ kForceRows( const PDArgs* __restrict__ specs,
                             crit = criticalStretch64( e.r, e.xi, bcc );
                         }
                         brkm |= __ballot( crit ) & inb;
+                    }
                     }
                 }
                 // dead lanes (absent, broken or breaking bonds) add exactly
@@ -1538,12 +1555,12 @@ __global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( P
                 // stencil record.
                 const double* xv = sxiv + 4 * q;
                 const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
-                const double r0 = ( uj0 - ui0 ) + scalarLoad( xv );
-                const double r1 = ( uj1 - ui1 ) + scalarLoad( xv + 1 );
-                const double r2 = ( uj2 - ui2 ) + scalarLoad( xv + 2 );
+                const double r0 = ( uj0 - ui0 ) + scalarLoad( xv + kXiX );
+                const double r1 = uj1 + ( scalarLoad( xv + kXiY ) - ui1 );
+                const double r2 = uj2 + ( scalarLoad( xv + kXiZ ) - ui2 );
                 const float xil = scalarLoad4( sten + q ).w;
                 Bond32 b;
-                mixedBond( r0, r1, r2, scalarLoad( xv + 3 ), xil, b );
+                mixedBond( r0, r1, r2, scalarLoad( xv + kXi2 ), xil, b );
                 double temp_avg = 0.0;
                 float ath = 0.0f;
                 const float s_mech = b.s;
@@ -1762,7 +1779,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
                     // as in the row-window state pass
                     const float xw = A.stencil[q].w;
                     const double* xv = A.stencil_xiv + 4 * q;
-                    mixedBond( ( u[j] - ui0 ) + xv[0], ( u[n + j] - ui1 ) + xv[1], ( u[2 * n + j] - ui2 ) + xv[2],
+                    mixedBond( ( u[j] - ui0 ) + xv[kXiX], ( u[n + j] - ui1 ) + xv[kXiY], ( u[2 * n + j] - ui2 ) + xv[kXiZ],
                                A.stencil_xi2d[q], xw, b );
                     w2 = infl1 ? xw : xw * xw;
                 }
@@ -1980,8 +1997,8 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
                 {
                     Bond32 b;
                     const double* xv = sxiv + 4 * q;
-                    mixedBond( ( wr[0] - ui0 ) + scalarLoad( xv ), ( wr[kW] - ui1 ) + scalarLoad( xv + 1 ),
-                               ( wr[2 * kW] - ui2 ) + scalarLoad( xv + 2 ), scalarLoad( sxi2 + q ),
+                    mixedBond( ( wr[0] - ui0 ) + scalarLoad( xv + kXiX ), ( wr[kW] - ui1 ) + scalarLoad( xv + kXiY ),
+                               ( wr[2 * kW] - ui2 ) + scalarLoad( xv + kXiZ ), scalarLoad( sxi2 + q ),
                                scalarLoad4( sten + q ).w, b );
                     th += static_cast<double>( ga.w * b.s * vj32 );
                 }
