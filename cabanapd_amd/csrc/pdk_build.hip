@@ -191,11 +191,26 @@ __global__ void kCellFill( const int32_t* __restrict__ cell, int64_t n, int32_t*
     list[pos] = static_cast<int32_t>( i );
 }
 
+// The query position (x + u) and radius of every particle in cell-list
+// order, so that a cell's candidates are one contiguous run of 32-byte
+// records (one load each) instead of an id followed by dependent gathers.
+__global__ void kCellPack( NeighborQuery q, const int32_t* __restrict__ list, double4* __restrict__ rec )
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( k >= q.n )
+        return;
+    const int64_t j = list[k];
+    double a, b, c;
+    posOf( q, j, a, b, c );
+    rec[k] = make_double4( a, b, c, q.radius ? q.radius[j] : 0.0 );
+}
+
 // Visit all j != i within the cutoff; fn(j) is called in cell order.
 template <class Fn>
 __device__ __forceinline__ void visitNeighbors( const NeighborQuery& q, const CellGrid& g,
                                                 const int32_t* __restrict__ cell_start,
-                                                const int32_t* __restrict__ list, int64_t i, Fn&& fn )
+                                                const int32_t* __restrict__ list,
+                                                const double4* __restrict__ rec, int64_t i, Fn&& fn )
 {
 #pragma clang fp contract( off )
     double pi0, pi1, pi2;
@@ -213,16 +228,15 @@ __device__ __forceinline__ void visitNeighbors( const NeighborQuery& q, const Ce
                     const int64_t j = list[k];
                     if ( j == i )
                         continue;
-                    double pj0, pj1, pj2;
-                    posOf( q, j, pj0, pj1, pj2 );
-                    const double dx = pj0 - pi0;
-                    const double dy = pj1 - pi1;
-                    const double dz = pj2 - pi2;
+                    const double4 r = rec[k];
+                    const double dx = r.x - pi0;
+                    const double dy = r.y - pi1;
+                    const double dz = r.z - pi2;
                     const double dist2 = dx * dx + dy * dy + dz * dz;
                     double cut2 = rc2;
                     if ( q.radius )
                     {
-                        const double cut = ri + q.radius[j] + q.extend;
+                        const double cut = ri + r.w + q.extend;
                         cut2 = cut * cut;
                     }
                     if ( dist2 <= cut2 )
@@ -235,13 +249,14 @@ __device__ __forceinline__ void visitNeighbors( const NeighborQuery& q, const Ce
 __global__ __launch_bounds__( kThreads ) void kCountNeighbors( NeighborQuery q, CellGrid g,
                                                                const int32_t* __restrict__ cell_start,
                                                                const int32_t* __restrict__ list,
+                                                               const double4* __restrict__ rec,
                                                                int32_t* __restrict__ cnt,
                                                                int64_t* __restrict__ slice_len )
 {
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int32_t c = 0;
     if ( p < q.n )
-        visitNeighbors( q, g, cell_start, list, p, [&]( int64_t ) { c++; } );
+        visitNeighbors( q, g, cell_start, list, rec, p, [&]( int64_t ) { c++; } );
     if ( p < q.n )
         cnt[p] = c;
     // wavefront max
@@ -254,21 +269,25 @@ __global__ __launch_bounds__( kThreads ) void kCountNeighbors( NeighborQuery q, 
         slice_len[slice] = m;
 }
 
-// Rows of up to kFillLds neighbors are sorted in LDS: every lane appends the
+// Rows of up to LDSN neighbors are sorted in LDS: every lane appends the
 // caller ids of its neighbors to its own LDS column, then places each id at
 // its rank (number of smaller ids of the row).  Each lane touches only its own
 // column, so no barrier is needed.  Longer rows fall back to an insertion sort
-// in the output column itself.
+// in the output column itself.  The capped contact lists (at most 32 rows)
+// use 8 KB columns, so LDS admits 20 of these one-wave workgroups per CU
+// instead of 5.
 constexpr int kFillLds = 128;
+constexpr int kFillLdsCapped = 32;
 
 // One wavefront (one workgroup) per slice.  CAPPED: the single-pass form of
 // the contact lists -- every slice has the same number of rows (the capacity),
 // the pass counts the neighbors itself (cnt, the slice maxima in slice_len)
 // and raises *overflow when a particle has more than fit.
-template <bool CAPPED>
+template <bool CAPPED, int LDSN>
 __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, CellGrid g,
                                                             const int32_t* __restrict__ cell_start,
                                                             const int32_t* __restrict__ list,
+                                                            const double4* __restrict__ rec,
                                                             const int64_t* __restrict__ perm,
                                                             const int64_t* __restrict__ inv,
                                                             const int64_t* __restrict__ slice_ptr,
@@ -278,7 +297,7 @@ __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, Cel
                                                             int64_t* __restrict__ slice_len,
                                                             int32_t* __restrict__ overflow )
 {
-    __shared__ int32_t keys[kFillLds * kSlice];
+    __shared__ int32_t keys[LDSN * kSlice];
     const int lane = threadIdx.x;
     const int64_t slice = blockIdx.x;
     const int64_t p = slice * kSlice + lane;
@@ -288,10 +307,10 @@ __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, Cel
     const bool valid = p < q.n;
     int32_t c = 0;
     bool over = false;
-    if ( valid && len <= kFillLds )
+    if ( valid && len <= LDSN )
     {
         int32_t* mine = keys + lane;
-        visitNeighbors( q, g, cell_start, list, p, [&]( int64_t j ) {
+        visitNeighbors( q, g, cell_start, list, rec, p, [&]( int64_t j ) {
             if ( c >= len )
             {
                 over = true; // CAPPED only: len is otherwise the slice's maximum count
@@ -312,7 +331,7 @@ __global__ __launch_bounds__( kSlice ) void kFillNeighbors( NeighborQuery q, Cel
     else if ( valid )
     {
         // Collect caller ids, insertion-sort them, map back to sorted ids.
-        visitNeighbors( q, g, cell_start, list, p, [&]( int64_t j ) {
+        visitNeighbors( q, g, cell_start, list, rec, p, [&]( int64_t j ) {
             if ( c >= len )
             {
                 over = true;
@@ -827,11 +846,14 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
     kCellFill<<<gridFor( n ), kThreads, 0, st>>>( s.cell.as<int32_t>(), n, s.cell_fill.as<int32_t>(),
                                                   s.cell_list.as<int32_t>() );
     PDK_HIP( hipGetLastError() );
+    ensure<double4>( s.cell_rec, n, acct );
+    kCellPack<<<gridFor( n ), kThreads, 0, st>>>( q, s.cell_list.as<int32_t>(), s.cell_rec.as<double4>() );
+    PDK_HIP( hipGetLastError() );
     // Single pass into cap_hint rows per slice (contact lists, whose counts
     // change little between rebuilds); a particle with more neighbors than
     // fit sends the build to the counting path below.
     ensure<int64_t>( s.slice_len, nslices + 1, acct );
-    if ( cap_hint > 0 && cap_hint <= kFillLds )
+    if ( cap_hint > 0 && cap_hint <= kFillLdsCapped )
     {
         const int64_t rows = nslices * cap_hint;
         if ( rows > col_capacity_rows || !col )
@@ -845,9 +867,10 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
         kSliceRamp<<<static_cast<unsigned>( ( nslices + 1 + kThreads - 1 ) / kThreads ), kThreads, 0, st>>>(
             slice_ptr, nslices, cap_hint );
         PDK_HIP( hipGetLastError() );
-        kFillNeighbors<true><<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>(
-            q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), perm, inv, slice_ptr, cnt, col.as<int32_t>(),
-            intact.as<uint64_t>(), s.slice_len.as<int64_t>(), s.flag.as<int32_t>() );
+        kFillNeighbors<true, kFillLdsCapped><<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>(
+            q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), s.cell_rec.as<double4>(), perm, inv,
+            slice_ptr, cnt, col.as<int32_t>(), intact.as<uint64_t>(), s.slice_len.as<int64_t>(),
+            s.flag.as<int32_t>() );
         PDK_HIP( hipGetLastError() );
         int32_t over = 0;
         PDK_HIP( hipMemcpyAsync( &over, s.flag.as<int32_t>(), sizeof( int32_t ), hipMemcpyDeviceToHost, st ) );
@@ -871,8 +894,8 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
     // Counts and slot rows per slice.
     PDK_HIP( hipMemsetAsync( s.slice_len.as<int64_t>(), 0, ( nslices + 1 ) * sizeof( int64_t ), st ) );
     const unsigned gslices = static_cast<unsigned>( ( nslices * kSlice + kThreads - 1 ) / kThreads );
-    kCountNeighbors<<<gslices, kThreads, 0, st>>>( q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), cnt,
-                                                    s.slice_len.as<int64_t>() );
+    kCountNeighbors<<<gslices, kThreads, 0, st>>>( q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(),
+                                                    s.cell_rec.as<double4>(), cnt, s.slice_len.as<int64_t>() );
     PDK_HIP( hipGetLastError() );
     tmp = 0;
     PDK_HIP( hipcub::DeviceScan::ExclusiveSum( nullptr, tmp, s.slice_len.as<int64_t>(), slice_ptr,
@@ -890,9 +913,9 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
         intact.alloc( static_cast<size_t>( cap ) * sizeof( uint64_t ), acct );
         col_capacity_rows = cap;
     }
-    kFillNeighbors<false><<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>(
-        q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), perm, inv, slice_ptr, cnt, col.as<int32_t>(),
-        intact.as<uint64_t>(), nullptr, nullptr );
+    kFillNeighbors<false, kFillLds><<<static_cast<unsigned>( nslices ), kSlice, 0, st>>>(
+        q, g, s.cell_start.as<int32_t>(), s.cell_list.as<int32_t>(), s.cell_rec.as<double4>(), perm, inv, slice_ptr,
+        cnt, col.as<int32_t>(), intact.as<uint64_t>(), nullptr, nullptr );
     PDK_HIP( hipGetLastError() );
     if ( max_cnt_out )
     {

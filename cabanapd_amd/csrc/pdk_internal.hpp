@@ -128,7 +128,7 @@ class DevBuf
 // ---- bond build (pdk_build.hip) ---------------------------------------------
 struct BuildScratch
 {
-    DevBuf keys, keys2, vals, vals2, tmp, cell, cell_start, cell_fill, cell_list, slice_len, bbox,
+    DevBuf keys, keys2, vals, vals2, tmp, cell, cell_start, cell_fill, cell_list, cell_rec, slice_len, bbox,
         flag;
 };
 
