@@ -932,16 +932,18 @@ double crThird( double x )
     if ( x == HUGE_VAL )
         return x;
     int e = 0;
-    double m = std::frexp( x, &e ); // x = m 2^e
-    // m 2^r with e - r divisible by 3, m in [0.5, 4)
+    const double m0 = std::frexp( x, &e ); // x = m0 2^e, m0 in [0.5, 1)
+    // m = m0 2^r with e - r divisible by 3, m in [0.5, 4)
     int r = e % 3;
     if ( r < 0 )
         r += 3;
-    m = std::ldexp( m, r );
+    const double m = std::ldexp( m0, r );
     const int q = ( e - r ) / 3;
-    // Newton for c^3 = m from a linear start on [0.5, 4): converges to ~1 ulp
-    double c = 0.6 + 0.2 * m;
-    for ( int it = 0; it < 7; it++ )
+    // Newton for c^3 = m from a cubic start (cbrt on [0.5, 1), relative
+    // error below 1.4e-4, times 2^(r/3)): three steps reach ~1 ulp
+    double c = 0.44019728644131806 + m0 * ( 0.9266652504745495 + m0 * ( -0.5110337078357359 + m0 * 0.1442771968559388 ) );
+    c = c * ( r == 0 ? 1.0 : ( r == 1 ? 1.2599210498948732 : 1.5874010519681994 ) );
+    for ( int it = 0; it < 3; it++ )
         c = c - ( c * c * c - m ) / ( 3.0 * c * c );
     // one Newton step on the double-double cube: m - c^3 exactly to ~2^-106
     const double ph = c * c, pl = std::fma( c, c, -ph );
