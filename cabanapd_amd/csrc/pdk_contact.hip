@@ -121,6 +121,23 @@ __device__ double crThird( double x )
     return chi + ( clo + chi * k );
 }
 
+// cbrt(x) to 1.4e-4 relative for finite x > 0: the cubic start of crThird
+// (the exponent (1 - 2^-54) / 3 of the exact power changes the value by less
+// than 1.3e-14 relative, which the bound absorbs).
+__device__ __forceinline__ double cbrtEstimate( double x )
+{
+#pragma clang fp contract( off )
+    int e = 0;
+    const double m0 = frexp( x, &e );
+    int r = e % 3;
+    if ( r < 0 )
+        r += 3;
+    const int q = ( e - r ) / 3;
+    double c = 0.44019728644131806 + m0 * ( 0.9266652504745495 + m0 * ( -0.5110337078357359 + m0 * 0.1442771968559388 ) );
+    c = c * ( r == 0 ? 1.0 : ( r == 1 ? 1.2599210498948732 : 1.5874010519681994 ) );
+    return ldexp( c, q );
+}
+
 __device__ double jkrPatchRadius( const PDArgs& A, double Rs, double delta_n )
 {
 #pragma clang fp contract( off )
@@ -229,9 +246,24 @@ __global__ __launch_bounds__( kBlock ) void kContact( const PDArgs* __restrict__
             if constexpr ( JKR )
             {
                 pk.coeff_h_n = 4.0 / 3.0 * A.Es / pk.Rs;
-                const double a0 = crThird( 9.0 / 2.0 * A.Gamma * kPi / A.Es * crSq( pk.Rs ) );
-                // pow(6, -1/3) of the host's libm, as the oracle evaluates it
-                pk.delta_tear = -1.0 / 2.0 * A.jkr_c6 * crSq( a0 ) / pk.Rs;
+                const double xa = 9.0 / 2.0 * A.Gamma * kPi / A.Es * crSq( pk.Rs );
+                // The tear distance only places delta_n = 2R - r on one side
+                // of it (normalForce): an estimate within 3e-4 relative
+                // decides unless delta_n lies within 1e-3 of it, and only
+                // then is the correctly rounded value formed.  The decision,
+                // and with it every force, is that of the exact value.
+                // (pow(6, -1/3) of the host's libm, as the oracle evaluates it)
+                const double a0e = cbrtEstimate( xa );
+                const double dte = -1.0 / 2.0 * A.jkr_c6 * crSq( a0e ) / pk.Rs;
+                const double dn = pk.two_r - r;
+                if ( xa > 0.0 && xa < 1e300 && fabs( dte ) > 1e-290 && fabs( dte ) < 1e290 &&
+                     fabs( dn - dte ) > 1e-3 * fabs( dte ) )
+                    pk.delta_tear = dn >= dte ? -INFINITY : INFINITY;
+                else
+                {
+                    const double a0 = crThird( xa );
+                    pk.delta_tear = -1.0 / 2.0 * A.jkr_c6 * crSq( a0 ) / pk.Rs;
+                }
             }
             else
                 pk.coeff_h_n = 4.0 / 3.0 * A.Es * sqrt( pk.Rs );
