@@ -2655,6 +2655,63 @@ __global__ __launch_bounds__( kBlock ) void kHeatConduction( const PDArgs* __res
     A.cond[p] = conduction;
 }
 
+// The conduction pass on lattice-row windows (the stencil-slot layout with a
+// row table): x and T of the neighbours (and the volumes when they differ)
+// staged per row as in the output passes, the bonds in slot order, so the
+// sums equal kHeatConduction's bit for bit.
+template <bool FRAC, bool GEN>
+__global__ __launch_bounds__( kBlock ) void kHeatConductionRows( const PDArgs* __restrict__ specs )
+{
+#pragma clang fp contract( off )
+    constexpr int NC = 4 + ( GEN ? 1 : 0 ); // x, T (, vol)
+    constexpr int kW = RowWalk64<NC>::kW;
+    __shared__ double win[kSlicesPerBlock][2][NC][kW];
+    const PDArgs& A = *specs;
+    const int wv = __builtin_amdgcn_readfirstlane( static_cast<int>( threadIdx.x / kSlice ) );
+    const int lane = threadIdx.x & ( kSlice - 1 );
+    const int64_t slice = xcdRemap( blockIdx.x, gridDim.x ) * kSlicesPerBlock + wv;
+    if ( slice >= A.nslices )
+        return; // waves are independent: no block-level synchronisation
+    const int64_t n = scalarLoad( &A.n );
+    const int64_t p0 = slice * kSlice;
+    const int64_t p = p0 + lane;
+    const bool valid = p < n;
+    const int64_t pc = valid ? p : p0;
+    const double* __restrict__ x = scalarLoad( &A.x );
+    const double* __restrict__ T = scalarLoad( &A.temp );
+    const double* src[NC];
+    src[0] = x;
+    src[1] = x + n;
+    src[2] = x + 2 * n;
+    src[3] = T;
+    if constexpr ( GEN )
+        src[4] = scalarLoad( &A.vol );
+    const RowWalk64<NC> rw = rowWalk<NC>( A, slice, lane, src );
+    const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
+    const double Ti = T[pc];
+    const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + slice );
+    const uint64_t* __restrict__ mw = scalarLoad( FRAC ? &A.intact : &A.exist ) + row0;
+    const double tc = A.thermal_coeff;
+    const double th = A.thermal_horizon;
+    const int lin = A.lin_micro;
+    const double volu = A.vol_value;
+    double conduction = 0.0;
+    rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr ) {
+        // intact bonds under fracture (a subset of the existing ones)
+        if ( !__builtin_amdgcn_inverse_ballot_w64( scalarLoad( mw + q ) ) )
+            return;
+        const double xi_x = wr[0] - xi0;
+        const double xi_y = wr[kW] - xi1;
+        const double xi_z = wr[2 * kW] - xi2;
+        const double xi = sqrt( xi_x * xi_x + xi_y * xi_y + xi_z * xi_z );
+        // microconductivity_function (CabanaPD_ForceModels.hpp:302-308)
+        const double coeff = lin ? 4.0 * tc * ( 1.0 - xi / th ) : tc;
+        conduction += coeff * ( wr[3 * kW] - Ti ) / xi / xi * ( GEN ? wr[4 * kW] : volu );
+    } );
+    if ( valid )
+        A.cond[p] = conduction;
+}
+
 __global__ __launch_bounds__( 256 ) void kHeatEuler( const PDArgs* __restrict__ specs, double dt )
 {
 #pragma clang fp contract( off )
@@ -3171,8 +3228,11 @@ void launchHeatTransfer( const PDArgs* args_dev, const PDArgs& a, double dt, hip
 {
     if ( a.nslices == 0 )
         return;
-    hipLaunchKernelGGL( a.fracture ? kHeatConduction<true> : kHeatConduction<false>, dim3( sliceBlocks( a ) ),
-                        dim3( kBlock ), 0, st, args_dev );
+    auto fn = a.fracture ? kHeatConduction<true> : kHeatConduction<false>;
+    if ( a.rows != nullptr && a.nrows > 0 && a.exist != nullptr )
+        fn = a.vol_uniform ? ( a.fracture ? kHeatConductionRows<true, false> : kHeatConductionRows<false, false> )
+                           : ( a.fracture ? kHeatConductionRows<true, true> : kHeatConductionRows<false, true> );
+    hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev );
     PDK_HIP( hipGetLastError() );
     hipLaunchKernelGGL( kHeatEuler, dim3( static_cast<unsigned>( ( a.n + 255 ) / 256 ) ), dim3( 256 ), 0, st, args_dev,
                         dt );

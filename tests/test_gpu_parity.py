@@ -768,7 +768,8 @@ def _feature_plate(case):
     """The 24x48x6 impact plate (a full lattice: row windows apply) with
     random displacements and the features of `case`: mt_pmb / mt_lps (two
     materials scattered at random), th (TemperatureDependent, random
-    temperatures), ep (ElasticPerfectlyPlastic), th_ep (both), each with
+    temperatures), ep (ElasticPerfectlyPlastic), th_ep (both), th_dyn
+    (DynamicTemperature: heat conduction every other step), each with
     fracture; th_nf without fracture."""
     from cabanapd_amd._abi import make_desc
     w = configs.kw_plate(num_cells=(24, 48, 6), size=(0.06, 0.12, 0.015), steps=25)
@@ -791,14 +792,16 @@ def _feature_plate(case):
     if "ep" in case:
         kw.update(mechanics="epp", sigma_y=3.0 * bd.K * 1e-3)
     if "th" in case:
-        kw.update(thermal=True, alpha=1e-5, temp0=10.0)
+        kw.update(thermal="dynamic" if case == "th_dyn" else True, alpha=1e-5, temp0=10.0)
+        if case == "th_dyn":
+            kw.update(kappa=5e4, cp=10.0, thermal_subcycle=2)
         w["temperature"] = rng.uniform(0.0, 400.0, size=n)
     w["desc"] = make_desc("PMB", **kw)
     return w
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("case", ["mt_pmb", "mt_lps", "th", "th_nf", "ep", "th_ep"])
+@pytest.mark.parametrize("case", ["mt_pmb", "mt_lps", "th", "th_nf", "ep", "th_ep", "th_dyn"])
 def test_gpu_feature_row_windows_match_gathers(case, precision):
     """The row-window force pass of specimens with several materials, a
     temperature or a plastic stretch evaluates the gather pass's expressions
@@ -822,14 +825,17 @@ def test_gpu_feature_row_windows_match_gathers(case, precision):
         assert name.startswith("kForceRowsF<") == (variant == 0)
         s.step(25)
         out[variant] = [s.get("u"), s.get("v"), s.get("f"), s.bonds()[2]]
+        if "th" in case:
+            out[variant].append(s.get("temperature"))
         s.close()
-    for a, b, what in zip(out[0], out[4], ("u", "v", "f", "intact")):
+    for a, b, what in zip(out[0], out[4], ("u", "v", "f", "intact", "temperature")):
         assert_same(a, b, f"{what} (row windows vs gathers)")
     if precision == "fp64":
         o = oracle_spec(w)
         configs.setup(o, w)
         o.step(25)
-        for a, fld in zip(out[0], ("u", "v", "f")):
+        flds = ("u", "v", "f") + (("temperature",) if "th" in case else ())
+        for a, fld in zip(out[0][:3] + out[0][4:], flds):
             assert_same(a, o.get(fld), f"{fld} (row windows vs oracle)")
     if case != "th_nf":
         nb = int(np.sum(out[0][3] == 0))
