@@ -258,10 +258,10 @@ int pdk_set_field(pdk_spec spec, int32_t field, const double* values);
  * (force_models/CabanaPD_PMB.hpp:51-55); s0 = sqrt((s0_a^2 K_a + s0_b^2 K_b) /
  * (K_a + K_b)), bond_break_coeff = (1 + s0)^2 (CabanaPD_ForceModels.hpp:114-121);
  * LPS keeps each material's theta/s coefficients on its own particle's terms
- * (force_models/CabanaPD_LPS.hpp:55-69).  Both precision forms; the force
- * pass then uses the gather variant (the mixed form with each pair's c,
- * critical stretch and guard band).  Call after pdk_spec_upload and before
- * pdk_init. */
+ * (force_models/CabanaPD_LPS.hpp:55-69).  Both precision forms (the mixed
+ * form with each pair's c, critical stretch and guard band); the force pass
+ * runs on lattice-row windows where the layout has them.  Call after
+ * pdk_spec_upload and before pdk_init. */
 typedef struct pdk_material {
     double K, G, G0;
 } pdk_material;
@@ -421,8 +421,11 @@ int pdk_compute_energy_noalloc(pdk_spec spec, double* total_strain_energy, doubl
 /* ---- tuning --------------------------------------------------------------------
  * Force-pass variant: unroll = 1, 2, 4 or 8 gathers the neighbor data of that
  * many bond slots per group; unroll = 0 (stencil-slot layout only) stages
- * the neighbor data of each lattice row window in LDS instead.  Results do
- * not depend on the choice.  The default is 0 where available, else 4.
+ * the neighbor data of each lattice row window in LDS instead (also for
+ * several materials, a temperature-dependent PMB and ElasticPerfectlyPlastic
+ * PMB, one at a time; other combinations and the linearized models with a
+ * temperature take the gathers).  Results do not depend on the choice.  The
+ * default is 0 where available, else 4.
  * pdk_autotune times the candidates on the current state (after pdk_init;
  * the probes repeat the init force pass, which is idempotent) and keeps the
  * fastest. */
