@@ -422,10 +422,10 @@ int pdk_compute_energy_noalloc(pdk_spec spec, double* total_strain_energy, doubl
  * Force-pass variant: unroll = 1, 2, 4 or 8 gathers the neighbor data of that
  * many bond slots per group; unroll = 0 (stencil-slot layout only) stages
  * the neighbor data of each lattice row window in LDS instead (also for
- * several materials, a temperature-dependent PMB and ElasticPerfectlyPlastic
- * PMB, one at a time; other combinations and the linearized models with a
- * temperature take the gathers).  Results do not depend on the choice.  The
- * default is 0 where available, else 4.
+ * several materials, or a temperature-dependent and/or ElasticPerfectlyPlastic
+ * PMB; several materials together with either, and the linearized models
+ * with a temperature, take the gathers).  Results do not depend on the
+ * choice.  The default is 0 where available, else 4.
  * pdk_autotune times the candidates on the current state (after pdk_init;
  * the probes repeat the init force pass, which is idempotent) and keeps the
  * fastest. */
