@@ -2289,9 +2289,9 @@ struct RowWalk64
     int32_t p032; // first particle of the slice
     int lane;
 
-    // bond(q, wr) for every slot q of the slice in slot order; wr is the
-    // lane's window position (channel k at wr[k * kW]).  win: this wave's
-    // [2][NC][kW] LDS buffers.
+    // bond(q, wr, g) for every slot q of the slice in slot order; wr is the
+    // lane's window position (channel k at wr[k * kW]) and g + lane its
+    // neighbour's id.  win: this wave's [2][NC][kW] LDS buffers.
     template <class Bond>
     __device__ __forceinline__ void walk( double* __restrict__ win, Bond&& bond ) const
     {
@@ -2335,7 +2335,7 @@ struct RowWalk64
             if ( r + 1 < nrows )
                 loadRow( r + 1 );
             for ( int t = 0; t < R.y; t++ )
-                bond( R.x + t, wb + lane + t );
+                bond( R.x + t, wb + lane + t, p032 + R.z + t );
         }
     }
 };
@@ -2358,14 +2358,20 @@ __device__ __forceinline__ RowWalk64<NC> rowWalk( const PDArgs& A, int64_t slice
     return w;
 }
 
-template <int MODEL, bool FRAC, bool GEN>
+// FEAT (kFeat* bits): several materials (the neighbour's type read at its
+// id), a temperature (a staged channel), a plastic stretch (per slot).
+template <int MODEL, bool FRAC, bool GEN, int FEAT = 0>
 __global__ __launch_bounds__( kBlock ) void kEnergyRows( const PDArgs* __restrict__ specs, int32_t parity,
                                                          const double* __restrict__ mvals,
                                                          const double* __restrict__ thetavals,
                                                          double* __restrict__ partial )
 {
 #pragma clang fp contract( off )
-    constexpr int NC = 6 + ( GEN ? 1 : 0 );
+    constexpr bool MT = ( FEAT & kFeatMT ) != 0;
+    constexpr bool TH = ( FEAT & kFeatTH ) != 0 && !kIsLps<MODEL>;
+    constexpr bool EP = ( FEAT & kFeatEP ) != 0 && !kIsLps<MODEL>;
+    constexpr int KT = 6 + ( GEN ? 1 : 0 ); // temperature channel
+    constexpr int NC = KT + ( TH ? 1 : 0 );
     constexpr int kW = RowWalk64<NC>::kW;
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
     __shared__ double red[2][kSlicesPerBlock];
@@ -2386,9 +2392,17 @@ __global__ __launch_bounds__( kBlock ) void kEnergyRows( const PDArgs* __restric
         const double* __restrict__ x = scalarLoad( &A.x );
         const double* __restrict__ u = scalarLoad( &A.U[parity] );
         const double* __restrict__ volp = scalarLoad( &A.vol );
-        const double* src[NC] = { u, u + n, u + 2 * n, x, x + n, x + 2 * n };
+        const double* src[NC];
+        src[0] = u;
+        src[1] = u + n;
+        src[2] = u + 2 * n;
+        src[3] = x;
+        src[4] = x + n;
+        src[5] = x + 2 * n;
         if constexpr ( GEN )
             src[6] = volp;
+        if constexpr ( TH )
+            src[KT] = scalarLoad( &A.temp );
         const RowWalk64<NC> rw = rowWalk<NC>( A, slice, lane, src );
         const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
         const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
@@ -2397,9 +2411,13 @@ __global__ __launch_bounds__( kBlock ) void kEnergyRows( const PDArgs* __restric
         const int32_t nslot = static_cast<int32_t>( scalarLoad( scalarLoad( &A.slice_ptr ) + slice + 1 ) - row0 );
         const uint64_t* __restrict__ ew = scalarLoad( &A.exist ) + row0;
         const uint64_t* __restrict__ iw = scalarLoad( &A.intact ) + row0;
+        const uint8_t* __restrict__ mtp = MT ? scalarLoad( &A.mtype ) : nullptr;
+        const double* __restrict__ spl = EP ? scalarLoad( &A.splas ) + row0 * kSlice + lane : nullptr;
+        const int ti4 = MT ? 4 * static_cast<int>( mtp[pc] ) : 0;
+        const double Ti = TH ? A.temp[pc] : 0.0;
         const double c = A.c;
-        const double tc_i = A.theta_coeff;
-        const double sc_i = A.s_coeff;
+        const double tc_i = MT ? A.mat_tc[mtp[pc]] : A.theta_coeff;
+        const double sc_i = MT ? A.mat_sc[mtp[pc]] : A.s_coeff;
         const double volu = A.vol_value;
         const int infl1 = A.influence;
         double num_bonds = 0.0;
@@ -2418,7 +2436,7 @@ __global__ __launch_bounds__( kBlock ) void kEnergyRows( const PDArgs* __restric
         }
         const double vol_i = volp[pc];
         double W = 0.0, phi_i = 0.0, vol_H_i = 0.0;
-        rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr ) {
+        rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr, int32_t g ) {
             if ( !__builtin_amdgcn_inverse_ballot_w64( scalarLoad( ew + q ) ) )
                 return;
             Bond64 b;
@@ -2433,7 +2451,15 @@ __global__ __launch_bounds__( kBlock ) void kEnergyRows( const PDArgs* __restric
                      0.5 * ( sc_i / m_i ) * infl * b.s * b.s * b.xi * b.xi * vj;
             }
             else
-                wb = 0.25 * c * b.s * b.s * b.xi * vj;
+            {
+                if constexpr ( TH )
+                    b.s = b.s - ( A.alpha * thermalAvg( A, Ti, wr[KT * kW] ) );
+                const double cc = MT ? A.mat_c[ti4 + mtp[g + lane]] : c;
+                if constexpr ( EP )
+                    wb = 0.25 * cc * plasticEnergyTerm( b.s, spl[q * kSlice], A.sY ) * b.xi * vj;
+                else
+                    wb = 0.25 * cc * b.s * b.s * b.xi * vj;
+            }
             if constexpr ( FRAC )
             {
                 const double mu = static_cast<double>( ( scalarLoad( iw + q ) >> lane ) & 1ull );
@@ -2486,13 +2512,17 @@ __global__ __launch_bounds__( kBlock ) void kEnergyRows( const PDArgs* __restric
     }
 }
 
-template <int MODEL, bool FRAC, bool GEN>
+template <int MODEL, bool FRAC, bool GEN, int FEAT = 0>
 __global__ __launch_bounds__( kBlock ) void kStressRows( const PDArgs* __restrict__ specs, int32_t parity )
 {
 #pragma clang fp contract( off )
+    constexpr bool MT = ( FEAT & kFeatMT ) != 0 && !kIsLps<MODEL>; // LPS: the types are in lpsA / lpsB
+    constexpr bool TH = ( FEAT & kFeatTH ) != 0 && !kIsLps<MODEL>;
+    constexpr bool EP = ( FEAT & kFeatEP ) != 0 && !kIsLps<MODEL>;
     constexpr int NE = kIsLps<MODEL> ? 2 : 0; // staged lpsA, lpsB
     constexpr int KV = 6 + NE;                // staged vol (GEN)
-    constexpr int NC = KV + ( GEN ? 1 : 0 );
+    constexpr int KT = KV + ( GEN ? 1 : 0 );  // staged temperature (TH)
+    constexpr int NC = KT + ( TH ? 1 : 0 );
     constexpr int kW = RowWalk64<NC>::kW;
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
     const PDArgs& A = *specs;
@@ -2522,12 +2552,20 @@ __global__ __launch_bounds__( kBlock ) void kStressRows( const PDArgs* __restric
     }
     if constexpr ( GEN )
         src[KV] = scalarLoad( &A.vol );
+    if constexpr ( TH )
+        src[KT] = scalarLoad( &A.temp );
     const RowWalk64<NC> rw = rowWalk<NC>( A, slice, lane, src );
     const double xi0 = x[pc], xi1 = x[n + pc], xi2 = x[2 * n + pc];
     const double ui0 = u[pc], ui1 = u[n + pc], ui2 = u[2 * n + pc];
     const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + slice );
     const uint64_t* __restrict__ ew = scalarLoad( &A.exist ) + row0;
     const uint64_t* __restrict__ iw = scalarLoad( &A.intact ) + row0;
+    const uint8_t* __restrict__ mtp = MT ? scalarLoad( &A.mtype ) : nullptr;
+    // EPP: the stress pass updates the plastic stretch of every bond it visits
+    // (force_models/CabanaPD_PMB.hpp:373-376)
+    double* __restrict__ spl = EP ? scalarLoad( &A.splas ) + row0 * kSlice + lane : nullptr;
+    const int ti4 = MT ? 4 * static_cast<int>( mtp[pc] ) : 0;
+    const double Ti = TH ? A.temp[pc] : 0.0;
     const double c = A.c;
     const double volu = A.vol_value;
     const int infl1 = A.influence;
@@ -2538,7 +2576,7 @@ __global__ __launch_bounds__( kBlock ) void kStressRows( const PDArgs* __restric
         Bi = A.lpsB[pc];
     }
     double s00 = 0.0, s01 = 0.0, s02 = 0.0, s10 = 0.0, s11 = 0.0, s12 = 0.0, s20 = 0.0, s21 = 0.0, s22 = 0.0;
-    rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr ) {
+    rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr, int32_t g ) {
         uint64_t live = scalarLoad( ew + q );
         uint64_t intact = ~0ull;
         if constexpr ( FRAC )
@@ -2564,7 +2602,12 @@ __global__ __launch_bounds__( kBlock ) void kStressRows( const PDArgs* __restric
             coeff *= 0.5;
         }
         else
-            coeff = 0.5 * ( c * b.s * vj );
+        {
+            if constexpr ( TH )
+                b.s = b.s - ( A.alpha * thermalAvg( A, Ti, wr[KT * kW] ) );
+            const double sd = EP ? plasticStretch( spl + q * kSlice, b.s, A.sY ) : b.s;
+            coeff = 0.5 * ( ( MT ? A.mat_c[ti4 + mtp[g + lane]] : c ) * sd * vj );
+        }
         double fx, fy, fz;
         if constexpr ( FRAC )
         {
@@ -2696,7 +2739,7 @@ __global__ __launch_bounds__( kBlock ) void kHeatConductionRows( const PDArgs* _
     const int lin = A.lin_micro;
     const double volu = A.vol_value;
     double conduction = 0.0;
-    rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr ) {
+    rw.walk( &win[wv][0][0][0], [&]( int32_t q, const double* __restrict__ wr, int32_t g ) {
         // intact bonds under fracture (a subset of the existing ones)
         if ( !__builtin_amdgcn_inverse_ballot_w64( scalarLoad( mw + q ) ) )
             return;
@@ -3058,17 +3101,35 @@ const char* lpsKernelName( const PDArgs& a )
 // stretch (the same results either way).
 static bool outputRows( const PDArgs& a )
 {
-    return a.rows != nullptr && a.nrows > 0 && a.exist != nullptr && a.mtype == nullptr && a.temp == nullptr &&
-           a.splas == nullptr;
+    if ( a.rows == nullptr || a.nrows <= 0 || a.exist == nullptr )
+        return false;
+    const bool lps = a.model == kLPS || a.model == kLinLPS;
+    const int feat = ( a.mtype ? kFeatMT : 0 ) | ( !lps && a.temp ? kFeatTH : 0 ) | ( !lps && a.splas ? kFeatEP : 0 );
+    switch ( feat )
+    {
+    case 0: return true;
+    case kFeatMT: return a.model == kPMB || a.model == kLPS;
+    default: return ( feat & kFeatMT ) == 0 && a.model == kPMB; // temperature and / or plastic stretch
+    }
+}
+static int outputFeatures( const PDArgs& a )
+{
+    const bool lps = a.model == kLPS || a.model == kLinLPS;
+    return ( a.mtype ? kFeatMT : 0 ) | ( !lps && a.temp ? kFeatTH : 0 ) | ( !lps && a.splas ? kFeatEP : 0 );
 }
 
 const char* energyKernelName( const PDArgs& a )
 {
     if ( outputRows( a ) )
     {
-        std::snprintf( g_energy_name, sizeof( g_energy_name ), "kEnergyRows<%d, %s, %s>", a.model,
-                       a.fracture && !( a.model == kLinPMB || a.model == kLinLPS ) ? "true" : "false",
-                       a.vol_uniform ? "false" : "true" );
+        const int feat = outputFeatures( a );
+        if ( feat )
+            std::snprintf( g_energy_name, sizeof( g_energy_name ), "kEnergyRows<%d, %s, true, %d>", a.model,
+                           a.fracture ? "true" : "false", feat );
+        else
+            std::snprintf( g_energy_name, sizeof( g_energy_name ), "kEnergyRows<%d, %s, %s>", a.model,
+                           a.fracture && !( a.model == kLinPMB || a.model == kLinLPS ) ? "true" : "false",
+                           a.vol_uniform ? "false" : "true" );
         return g_energy_name;
     }
     if ( a.model == kLinPMB )
@@ -3203,6 +3264,37 @@ StressFn pickStressRows( const PDArgs& a )
     }
 }
 
+// Feature specimens (the combinations outputRows admits) stage the volumes
+// always (GEN), which gives the same values when they are uniform.
+EnergyFn pickEnergyRowsFeat( const PDArgs& a )
+{
+    switch ( outputFeatures( a ) )
+    {
+    case kFeatMT:
+        if ( a.model == kLPS )
+            return a.fracture ? kEnergyRows<kLPS, true, true, kFeatMT> : kEnergyRows<kLPS, false, true, kFeatMT>;
+        return a.fracture ? kEnergyRows<kPMB, true, true, kFeatMT> : kEnergyRows<kPMB, false, true, kFeatMT>;
+    case kFeatTH:
+        return a.fracture ? kEnergyRows<kPMB, true, true, kFeatTH> : kEnergyRows<kPMB, false, true, kFeatTH>;
+    case kFeatEP: return kEnergyRows<kPMB, true, true, kFeatEP>;
+    default: return kEnergyRows<kPMB, true, true, kFeatTH | kFeatEP>;
+    }
+}
+StressFn pickStressRowsFeat( const PDArgs& a )
+{
+    switch ( outputFeatures( a ) )
+    {
+    case kFeatMT:
+        if ( a.model == kLPS )
+            return a.fracture ? kStressRows<kLPS, true, true, kFeatMT> : kStressRows<kLPS, false, true, kFeatMT>;
+        return a.fracture ? kStressRows<kPMB, true, true, kFeatMT> : kStressRows<kPMB, false, true, kFeatMT>;
+    case kFeatTH:
+        return a.fracture ? kStressRows<kPMB, true, true, kFeatTH> : kStressRows<kPMB, false, true, kFeatTH>;
+    case kFeatEP: return kStressRows<kPMB, true, true, kFeatEP>;
+    default: return kStressRows<kPMB, true, true, kFeatTH | kFeatEP>;
+    }
+}
+
 int64_t energyBlocks( const PDArgs& a )
 {
     return sliceBlocks( a );
@@ -3218,7 +3310,8 @@ void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, doub
               : a.model == kLPS    ? ( a.fracture ? kEnergy<kLPS, true> : kEnergy<kLPS, false> )
                                    : ( a.fracture ? kEnergy<kPMB, true> : kEnergy<kPMB, false> );
     if ( outputRows( a ) )
-        fn = a.vol_uniform ? pickEnergyRows<false>( a ) : pickEnergyRows<true>( a );
+        fn = outputFeatures( a ) ? pickEnergyRowsFeat( a )
+                                 : ( a.vol_uniform ? pickEnergyRows<false>( a ) : pickEnergyRows<true>( a ) );
     hipLaunchKernelGGL( fn, dim3( static_cast<unsigned>( nblocks ) ), dim3( kBlock ), 0, st, args_dev, parity,
                         a.m, a.theta, partial_dev );
     PDK_HIP( hipGetLastError() );
@@ -3248,7 +3341,8 @@ void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipS
               : a.model == kLPS    ? ( a.fracture ? kStress<kLPS, true> : kStress<kLPS, false> )
                                    : ( a.fracture ? kStress<kPMB, true> : kStress<kPMB, false> );
     if ( outputRows( a ) )
-        fn = a.vol_uniform ? pickStressRows<false>( a ) : pickStressRows<true>( a );
+        fn = outputFeatures( a ) ? pickStressRowsFeat( a )
+                                 : ( a.vol_uniform ? pickStressRows<false>( a ) : pickStressRows<true>( a ) );
     hipLaunchKernelGGL( fn, dim3( sliceBlocks( a ) ), dim3( kBlock ), 0, st, args_dev, parity );
     PDK_HIP( hipGetLastError() );
 }
@@ -3280,9 +3374,14 @@ const char* stressKernelName( const PDArgs& a )
 {
     if ( outputRows( a ) )
     {
-        std::snprintf( g_stress_name, sizeof( g_stress_name ), "kStressRows<%d, %s, %s>", a.model,
-                       a.fracture && !( a.model == kLinPMB || a.model == kLinLPS ) ? "true" : "false",
-                       a.vol_uniform ? "false" : "true" );
+        const int feat = outputFeatures( a );
+        if ( feat )
+            std::snprintf( g_stress_name, sizeof( g_stress_name ), "kStressRows<%d, %s, true, %d>", a.model,
+                           a.fracture ? "true" : "false", feat );
+        else
+            std::snprintf( g_stress_name, sizeof( g_stress_name ), "kStressRows<%d, %s, %s>", a.model,
+                           a.fracture && !( a.model == kLinPMB || a.model == kLinLPS ) ? "true" : "false",
+                           a.vol_uniform ? "false" : "true" );
         return g_stress_name;
     }
     if ( a.model == kLinPMB )

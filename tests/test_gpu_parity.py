@@ -813,6 +813,7 @@ def test_gpu_feature_row_windows_match_gathers(case, precision):
     say(f"case: feature row windows vs gathers, {case} {precision}, plate 24x48x6, 25 steps")
     w = _feature_plate(case)
     out = {}
+    rows_spec = None
     for variant in (0, 4):
         d = type(w["desc"]).from_buffer_copy(w["desc"])
         d.precision = {"fp64": 0, "mixed": 1}[precision]
@@ -827,7 +828,10 @@ def test_gpu_feature_row_windows_match_gathers(case, precision):
         out[variant] = [s.get("u"), s.get("v"), s.get("f"), s.bonds()[2]]
         if "th" in case:
             out[variant].append(s.get("temperature"))
-        s.close()
+        if variant == 0:
+            rows_spec = s
+        else:
+            s.close()
     for a, b, what in zip(out[0], out[4], ("u", "v", "f", "intact", "temperature")):
         assert_same(a, b, f"{what} (row windows vs gathers)")
     if precision == "fp64":
@@ -837,6 +841,25 @@ def test_gpu_feature_row_windows_match_gathers(case, precision):
         flds = ("u", "v", "f") + (("temperature",) if "th" in case else ())
         for a, fld in zip(out[0][:3] + out[0][4:], flds):
             assert_same(a, o.get(fld), f"{fld} (row windows vs oracle)")
+        # the output passes on row windows with the same features
+        g = rows_spec
+        ename = g.kernel_name(2)
+        say(f"  energy kernel {ename}")
+        assert ename.startswith("kEnergyRows<")
+        Wg, Dg = g.compute_energy()
+        Wo, Do = o.compute_energy()
+        assert_same(g.get("W"), o.get("W"), "W")
+        assert_total(Wg, Wo, "total strain energy")
+        if case != "th_nf":
+            assert_same(g.get("phi"), o.get("phi"), "phi")
+        g.compute_stress()
+        o.compute_stress()
+        assert_same(g.get("stress"), o.get("stress"), "stress")
+        # EPP: the stress pass updated the plastic stretch of every bond
+        g.step(3)
+        o.step(3)
+        assert_same(g.get("u"), o.get("u"), "u three steps after the stress pass")
+    rows_spec.close()
     if case != "th_nf":
         nb = int(np.sum(out[0][3] == 0))
         say(f"  broken bonds {nb}")
