@@ -598,7 +598,9 @@ def dem_leg(args, n=1000000):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's own count -- plate 1000, cube 500, lps 500, "
+                         "dem 2000, rows 200)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cells", type=str, default="200,400,18")
@@ -618,6 +620,8 @@ def main():
     ap.add_argument("--copy-kernel", type=float, default=0.0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"plate": 1000, "cube": 500, "lps": 500, "dem": 2000, "rows": 200}[args.workload]
     if args.cpu_baseline_child:
         cpu_baseline_child(tuple(int(c) for c in args.cpu_baseline_child.split(",")), args.cpu_budget)
         return
