@@ -248,6 +248,11 @@ struct pdk_spec_s
     // contact bookkeeping
     int64_t steps_since_build = 0;
     double max_disp = 0.0;
+    // contact step: pinned host copy of the largest displacement since the
+    // last build, and the event after its copy (the host waits on that, not
+    // on the stream)
+    double* disp_pinned = nullptr;
+    hipEvent_t disp_ready = nullptr;
     double radius_max = 0.0; // largest per-particle radius (polydisperse contact)
     // timing
     bool timing = false;
@@ -916,6 +921,10 @@ int pdk_spec_destroy( pdk_spec s )
             (void)hipStreamSynchronize( s->st );
         s->dropGraphs();
         detachSeries( s );
+        if ( s->disp_ready )
+            (void)hipEventDestroy( s->disp_ready );
+        if ( s->disp_pinned )
+            (void)hipHostFree( s->disp_pinned );
         if ( s->st )
             (void)hipStreamDestroy( s->st );
         if ( s->ctx )
@@ -2235,6 +2244,13 @@ ContactState contactState( pdk_spec s )
     c.rebuild_every = s->d.rebuild_every;
     c.partial = &s->partial;
     c.acct = s->acct();
+    if ( !s->disp_pinned )
+    {
+        PDK_HIP( hipHostMalloc( reinterpret_cast<void**>( &s->disp_pinned ), sizeof( double ), hipHostMallocDefault ) );
+        PDK_HIP( hipEventCreateWithFlags( &s->disp_ready, hipEventDisableTiming ) );
+    }
+    c.disp_pinned = s->disp_pinned;
+    c.disp_ready = s->disp_ready;
     return c;
 }
 

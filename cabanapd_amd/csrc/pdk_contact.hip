@@ -497,25 +497,33 @@ void contactStep( pdk_spec_s* s, int32_t nsteps )
             PDK_HIP( hipGetLastError() );
         } );
         *v.cur ^= 1;
-        // the host decides on a rebuild (CabanaPD_Neighbor.hpp:116-119)
-        double mx = 0.0;
-        PDK_HIP( hipMemcpyAsync( &mx, part + nb, sizeof( double ), hipMemcpyDeviceToHost, v.st ) );
-        PDK_HIP( hipStreamSynchronize( v.st ) );
-        *cs.max_disp = mx;
+        // The host decides on a rebuild (CabanaPD_Neighbor.hpp:116-119) from
+        // the largest displacement, copied to pinned memory.  Unless the
+        // periodic rebuild is due, the force pass (which writes f only; the
+        // final half step is its own pass) is launched with the current list
+        // before the host waits for that copy, so the GPU is not left idle
+        // meanwhile; if the displacement calls for a rebuild, the list is
+        // rebuilt and the force pass runs again over the new list, which
+        // overwrites f.  The results are those of deciding first.
+        PDK_HIP( hipMemcpyAsync( cs.disp_pinned, part + nb, sizeof( double ), hipMemcpyDeviceToHost, v.st ) );
+        PDK_HIP( hipEventRecord( cs.disp_ready, v.st ) );
         *cs.since_build += 1;
         const bool periodic = cs.rebuild_every > 0 && *cs.since_build >= cs.rebuild_every;
+        auto force = [&] {
+            v = view( s );
+            specTimed( s, 0, [&] { launchContactForce( v.dargs, *v.h, kEpiApplyOnly, *v.cur, v.st ); } );
+        };
+        if ( !periodic )
+            force();
+        PDK_HIP( hipEventSynchronize( cs.disp_ready ) );
+        const double mx = *cs.disp_pinned;
+        *cs.max_disp = mx;
         if ( mx > cs.extend || periodic )
+        {
             specTimed( s, 3, [&] { contactBuildImpl( s, false, cs.max_disp, cs.since_build ); } );
-        v = view( s );
-        // the final half step in the force pass's epilogue, or as a pass of
-        // its own after a pass whose pair forces read the velocities
-        const bool split = contactReadsVelocity( *v.h );
-        specTimed( s, 0, [&] {
-            launchContactForce( v.dargs, *v.h, split ? static_cast<int32_t>( kEpiApplyOnly ) : kEpiFinal, *v.cur,
-                                v.st );
-        } );
-        if ( split )
-            specTimed( s, 4, [&] { launchContactFinal( v.dargs, *v.h, v.st ); } );
+            force();
+        }
+        specTimed( s, 4, [&] { launchContactFinal( v.dargs, *v.h, v.st ); } );
     }
 }
 

@@ -39,6 +39,8 @@ struct ContactState
     int32_t rebuild_every;
     DevBuf* partial;
     MemAccount* acct;
+    double* disp_pinned;   // pinned host copy of the largest displacement
+    hipEvent_t disp_ready; // recorded after that copy
 };
 ContactState contactState( pdk_spec_s* s );
 void contactBuildImpl( pdk_spec_s* s, bool initial, double* max_disp, int64_t* since_build );
