@@ -1779,7 +1779,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
                     // as in the row-window state pass
                     const float xw = A.stencil[q].w;
                     const double* xv = A.stencil_xiv + 4 * q;
-                    mixedBond( ( u[j] - ui0 ) + xv[kXiX], ( u[n + j] - ui1 ) + xv[kXiY], ( u[2 * n + j] - ui2 ) + xv[kXiZ],
+                    mixedBond( ( u[j] - ui0 ) + xv[kXiX], u[n + j] + ( xv[kXiY] - ui1 ), u[2 * n + j] + ( xv[kXiZ] - ui2 ),
                                A.stencil_xi2d[q], xw, b );
                     w2 = infl1 ? xw : xw * xw;
                 }
@@ -1829,7 +1829,6 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     constexpr int KV = 3 + NX;                   // staged vol (GEN)
     // mixed form: the bond vector r = xi + (u_j - u_i) in FP64 from the slot's
     // FP64 stencil offset xi; the windows hold u only
-    constexpr bool XV = PREC == kMixed;
     constexpr int NC = KV + ( GEN ? 1 : 0 );
     __shared__ double win[kSlicesPerBlock][2][NC][kW];
     WaveCtx w;
@@ -1855,7 +1854,6 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     const uint64_t* __restrict__ mwords = scalarLoad( FRAC ? &A.intact : &A.exist ) + row0;
     const float4* __restrict__ sten = scalarLoad( &A.stencil );
     const float4* __restrict__ saux = scalarLoad( &A.stencil_aux );
-    const double* __restrict__ sxi2 = scalarLoad( &A.stencil_xi2d );
     const double* __restrict__ sxiv = scalarLoad( &A.stencil_xiv );
     const int infl1 = A.influence;
     const double volu = A.vol_value;
@@ -1921,16 +1919,21 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
             __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
             if ( r + 1 < nrows )
                 loadRow( r + 1 );
-            // mixed form: the row's slots unrolled (immediate window offsets)
+            // mixed form: the row's slots unrolled (immediate window offsets);
+            // xi_y - u_i,y and xi_z - u_i,z once per row (the slots of a
+            // lattice row differ in xi_x only)
             if constexpr ( PREC == kMixed )
             {
+                const double* xv_r = sxiv + 4 * R.x;
+                const double ay = scalarLoad( xv_r + kXiY ) - ui1;
+                const double az = scalarLoad( xv_r + kXiZ ) - ui2;
 #pragma unroll
                 for ( int t = 0; t < kRowMaxSlots; t++ )
                     if ( t < R.y )
                     {
                         const int32_t q = R.x + t;
                         const bool live = __builtin_amdgcn_inverse_ballot_w64( scalarLoad( mwords + q ) );
-                        bond( live, wb + lane + t, q );
+                        bond( live, wb + lane + t, q, ay, az );
                     }
             }
             else
@@ -1939,7 +1942,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
                 {
                     const int32_t q = R.x + t;
                     const bool live = __builtin_amdgcn_inverse_ballot_w64( scalarLoad( mwords + q ) );
-                    bond( live, wb + lane + t, q );
+                    bond( live, wb + lane + t, q, 0.0, 0.0 );
                 }
             }
         }
@@ -1951,7 +1954,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     {
         if ( do_m )
         {
-            walk( [&]( bool live, const double* wr, int32_t ) {
+            walk( [&]( bool live, const double* wr, int32_t, double, double ) {
                 if ( live )
                 {
                     Bond64 b;
@@ -1967,7 +1970,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
         const bool mpos = FRAC ? ( m_i > 0 ) : true;
         if ( do_theta )
         {
-            walk( [&]( bool live, const double* wr, int32_t ) {
+            walk( [&]( bool live, const double* wr, int32_t, double, double ) {
                 if ( live && mpos )
                 {
                     Bond64 b;
@@ -1986,7 +1989,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
             m_i = A.m[p];
         // m_i for the theta condition is only known after the walk: sum both,
         // apply the condition at the end (the sums are the same per-bond terms)
-        walk( [&]( bool live, const double* wr, int32_t q ) {
+        walk( [&]( bool live, const double* wr, int32_t q, double ay, double az ) {
             if ( live )
             {
                 const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, |xi|^2, infl |xi|, infl |xi|^2)
@@ -1997,9 +2000,8 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
                 {
                     Bond32 b;
                     const double* xv = sxiv + 4 * q;
-                    mixedBond( ( wr[0] - ui0 ) + scalarLoad( xv + kXiX ), ( wr[kW] - ui1 ) + scalarLoad( xv + kXiY ),
-                               ( wr[2 * kW] - ui2 ) + scalarLoad( xv + kXiZ ), scalarLoad( sxi2 + q ),
-                               scalarLoad4( sten + q ).w, b );
+                    mixedBond( ( wr[0] - ui0 ) + scalarLoad( xv + kXiX ), wr[kW] + ay, wr[2 * kW] + az,
+                               scalarLoad( xv + kXi2 ), scalarLoad4( sten + q ).w, b );
                     th += static_cast<double>( ga.w * b.s * vj32 );
                 }
             }
