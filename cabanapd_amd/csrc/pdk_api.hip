@@ -475,6 +475,7 @@ struct pdk_spec_s
         h.vol_value = vol_value;
         h.any_nofail = any_nofail ? 1 : 0;
         h.stencil_aux = nullptr;
+        h.stencil_rec = nullptr;
         if ( slot.slots > 0 )
         {
             if ( !slot.aux )
@@ -482,7 +483,14 @@ struct pdk_spec_s
                 slot.aux.alloc( static_cast<size_t>( slot.slots ) * sizeof( float4 ), acct() );
                 launchStencilAux( slot.table.as<float4>(), slot.slots, d.influence, slot.aux.as<float4>(), st );
             }
+            if ( !slot.rec )
+            {
+                slot.rec.alloc( static_cast<size_t>( slot.slots ) * kRecStride * sizeof( double ), acct() );
+                launchStencilRec( slot.table.as<float4>(), slot.aux.as<float4>(), slot.xiv.as<double>(), slot.slots,
+                                  slot.rec.as<double>(), st );
+            }
             h.stencil_aux = slot.aux.as<float4>();
+            h.stencil_rec = slot.rec.as<double>();
         }
         PDK_HIP( hipMemcpyAsync( args.as<PDArgs>(), &h, sizeof( PDArgs ), hipMemcpyHostToDevice, st ) );
     }

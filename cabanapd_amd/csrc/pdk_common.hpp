@@ -49,6 +49,11 @@ enum EpilogueMode : int32_t
 // and |xi|^2 first, so the row-window pass loads the two values a slot needs
 // with one scalar load (xi_y and xi_z are per lattice row).
 constexpr int kXiX = 0, kXi2 = 1, kXiY = 2, kXiZ = 3;
+// One 48-byte record per stencil entry for the row-window passes, so that a
+// slot's values come in one scalar load: xi_x and |xi|^2 (FP64), then four
+// FP32 values -- |xi|, infl |xi|, infl |xi|^2, 1/|xi| (the stencil table's w
+// and the aux table's z, w, x) -- then xi_y and xi_z (FP64, read per row).
+constexpr int kRecStride = 6, kRecXiX = 0, kRecXi2 = 1, kRecF = 2, kRecXiY = 4, kRecXiZ = 5;
 
 // Everything a PD/contact kernel needs for one specimen.  Lives in device
 // memory (one per spec, or an array of them for a packed batch).
@@ -84,6 +89,7 @@ struct PDArgs
     const float4* stencil_aux;
     const double* stencil_xi2d; // |xi|^2 of each stencil entry in FP64 (exact lattice offsets)
     const double* stencil_xiv;  // stencil-slot form: 4 FP64 values per entry at kXiX, kXi2, kXiY, kXiZ
+    const double* stencil_rec;  // stencil-slot form: kRecStride values per entry (kRec* layout)
     double* m;
     double* theta;
     double* lpsA;           // LPS: theta_coeff * theta / m per particle

@@ -1074,6 +1074,10 @@ kForceRows( const PDArgs* __restrict__ specs,
         // slot t computes measured slower on the fracture plate (0.209 vs
         // 0.202 ms): the register copies between slots cost more scalar
         // issue than the hidden load latency saved.
+        // (the 48-byte records of the other row passes, one scalar load for
+        // all of a slot's values, measured 2.8 % slower here: the 32-byte FP64
+        // record and the FP32 table entry are two loads but stay within one
+        // scalar-cache line each)
         struct SlotK
         {
             uint64_t ww;
@@ -1356,8 +1360,7 @@ __global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( P
     const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + w.slice );
     uint64_t* __restrict__ bits = scalarLoad( &A.intact ) + row0;
     const uint64_t* __restrict__ mwords = FRAC ? bits : scalarLoad( &A.exist ) + row0;
-    const float4* __restrict__ sten = scalarLoad( &A.stencil );
-    const double* __restrict__ sxiv = scalarLoad( &A.stencil_xiv );
+    const double* __restrict__ srec = scalarLoad( &A.stencil_rec );
     const uint8_t* __restrict__ nfp = scalarLoad( &A.nofail );
     const uint8_t* __restrict__ mtp = MT ? scalarLoad( &A.mtype ) : nullptr;
     const double* __restrict__ tmpp = TH ? scalarLoad( &A.temp ) : nullptr;
@@ -1553,14 +1556,14 @@ __global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( P
                 // lane); bonds that do not exist, are broken or break add
                 // exactly zero.  r = xi + (u_j - u_i) from the slot's FP64
                 // stencil record.
-                const double* xv = sxiv + 4 * q;
+                const double* rr = srec + kRecStride * q;
                 const double uj0 = wr[0], uj1 = wr[kW], uj2 = wr[2 * kW];
-                const double r0 = ( uj0 - ui0 ) + scalarLoad( xv + kXiX );
-                const double r1 = uj1 + ( scalarLoad( xv + kXiY ) - ui1 );
-                const double r2 = uj2 + ( scalarLoad( xv + kXiZ ) - ui2 );
-                const float xil = scalarLoad4( sten + q ).w;
+                const double r0 = ( uj0 - ui0 ) + scalarLoad( rr + kRecXiX );
+                const double r1 = uj1 + ( scalarLoad( rr + kRecXiY ) - ui1 );
+                const double r2 = uj2 + ( scalarLoad( rr + kRecXiZ ) - ui2 );
+                const float xil = scalarLoad( reinterpret_cast<const float*>( rr + kRecF ) ); // |xi|
                 Bond32 b;
-                mixedBond( r0, r1, r2, scalarLoad( xv + kXi2 ), xil, b );
+                mixedBond( r0, r1, r2, scalarLoad( rr + kRecXi2 ), xil, b );
                 double temp_avg = 0.0;
                 float ath = 0.0f;
                 const float s_mech = b.s;
@@ -1852,9 +1855,7 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
     // wave-uniform: scalar loads
     const int64_t row0 = scalarLoad( scalarLoad( &A.slice_ptr ) + w.slice );
     const uint64_t* __restrict__ mwords = scalarLoad( FRAC ? &A.intact : &A.exist ) + row0;
-    const float4* __restrict__ sten = scalarLoad( &A.stencil );
-    const float4* __restrict__ saux = scalarLoad( &A.stencil_aux );
-    const double* __restrict__ sxiv = scalarLoad( &A.stencil_xiv );
+    const double* __restrict__ srec = scalarLoad( &A.stencil_rec );
     const int infl1 = A.influence;
     const double volu = A.vol_value;
     const float vol32 = static_cast<float>( A.vol_value );
@@ -1924,9 +1925,9 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
             // lattice row differ in xi_x only)
             if constexpr ( PREC == kMixed )
             {
-                const double* xv_r = sxiv + 4 * R.x;
-                const double ay = scalarLoad( xv_r + kXiY ) - ui1;
-                const double az = scalarLoad( xv_r + kXiZ ) - ui2;
+                const double* rec_r = srec + kRecStride * R.x;
+                const double ay = scalarLoad( rec_r + kRecXiY ) - ui1;
+                const double az = scalarLoad( rec_r + kRecXiZ ) - ui2;
 #pragma unroll
                 for ( int t = 0; t < kRowMaxSlots; t++ )
                     if ( t < R.y )
@@ -1992,17 +1993,19 @@ __global__ __launch_bounds__( kBlock ) void kLpsStateRows( const PDArgs* __restr
         walk( [&]( bool live, const double* wr, int32_t q, double ay, double az ) {
             if ( live )
             {
-                const float4 ga = scalarLoad4( saux + q ); // (1/|xi|, |xi|^2, infl |xi|, infl |xi|^2)
+                // the slot's record: xi_x, |xi|^2, |xi|, infl |xi|^2
+                const double* rr = srec + kRecStride * q;
+                const float* rf = reinterpret_cast<const float*>( rr + kRecF );
+                const float w2 = scalarLoad( rf + 2 ); // infl |xi|^2
                 const float vj32 = GEN ? static_cast<float>( wr[KV * kW] ) : vol32;
                 if ( do_m )
-                    m_i += static_cast<double>( ga.w * vj32 );
+                    m_i += static_cast<double>( w2 * vj32 );
                 if ( do_theta )
                 {
                     Bond32 b;
-                    const double* xv = sxiv + 4 * q;
-                    mixedBond( ( wr[0] - ui0 ) + scalarLoad( xv + kXiX ), wr[kW] + ay, wr[2 * kW] + az,
-                               scalarLoad( xv + kXi2 ), scalarLoad4( sten + q ).w, b );
-                    th += static_cast<double>( ga.w * b.s * vj32 );
+                    mixedBond( ( wr[0] - ui0 ) + scalarLoad( rr + kRecXiX ), wr[kW] + ay, wr[2 * kW] + az,
+                               scalarLoad( rr + kRecXi2 ), scalarLoad( rf ), b );
+                    th += static_cast<double>( w2 * b.s * vj32 );
                 }
             }
         } );
@@ -3363,7 +3366,34 @@ __global__ void kStencilAux( const float4* __restrict__ table, int32_t S, int32_
         aux[q] = make_float4( xinv, w * w, influence ? 1.0f : w, influence ? w : w * w );
     }
 }
+__global__ void kStencilRec( const float4* __restrict__ table, const float4* __restrict__ aux,
+                             const double* __restrict__ xiv, int32_t S, double* __restrict__ rec )
+{
+    const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if ( q < S )
+    {
+        double* r = rec + kRecStride * q;
+        r[kRecXiX] = xiv[4 * q + kXiX];
+        r[kRecXi2] = xiv[4 * q + kXi2];
+        r[kRecXiY] = xiv[4 * q + kXiY];
+        r[kRecXiZ] = xiv[4 * q + kXiZ];
+        float* f = reinterpret_cast<float*>( r + kRecF );
+        const float4 a = aux[q];
+        f[0] = table[q].w;
+        f[1] = a.z;
+        f[2] = a.w;
+        f[3] = a.x;
+    }
+}
 } // namespace
+void launchStencilRec( const float4* table, const float4* aux, const double* xiv, int32_t S, double* rec,
+                       hipStream_t st )
+{
+    if ( S <= 0 )
+        return;
+    hipLaunchKernelGGL( kStencilRec, dim3( ( S + 255 ) / 256 ), dim3( 256 ), 0, st, table, aux, xiv, S, rec );
+    PDK_HIP( hipGetLastError() );
+}
 void launchStencilAux( const float4* table, int32_t S, int32_t influence, float4* aux, hipStream_t st )
 {
     if ( S <= 0 )

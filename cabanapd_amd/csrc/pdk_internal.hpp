@@ -154,7 +154,7 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
 // specimen does not qualify.
 struct SlotForm
 {
-    DevBuf exist, soff, table, xi2d, xiv, rows, rowstart, aux;
+    DevBuf exist, soff, table, xi2d, xiv, rows, rowstart, aux, rec;
     int32_t slots = 0;
     int32_t nrows = 0; // row windows (0: the stencil does not split into <= kMaxRows rows)
 };
@@ -214,6 +214,10 @@ void launchEnergy( const PDArgs* args_dev, const PDArgs& a, int32_t parity, doub
 int64_t energyBlocks( const PDArgs& a );
 // Per stencil entry (1/|xi|, sqrt(3) bandf/|xi|, |xi|^2, 0) for the mixed row-window passes.
 void launchStencilAux( const float4* table, int32_t S, int32_t influence, float4* aux, hipStream_t st );
+// The row-window record of each stencil entry (kRec* layout) from the FP32
+// table, its aux table and the FP64 offsets.
+void launchStencilRec( const float4* table, const float4* aux, const double* xiv, int32_t S, double* rec,
+                       hipStream_t st );
 // Stress tensor pass into a.stress ([9][n], device order).
 void launchStress( const PDArgs* args_dev, const PDArgs& a, int32_t parity, hipStream_t st );
 const char* stressKernelName( const PDArgs& a );
