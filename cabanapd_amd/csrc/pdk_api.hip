@@ -196,7 +196,7 @@ struct pdk_spec_s
     int64_t n = 0, nslices = 0;
     hipStream_t st = nullptr;
     DevBuf x, U0, U1, v, f, vol, rho, nofail, bcflag, body, m, theta, lpsA, lpsB, W, phi, perm, inv, cnt,
-        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stab_xi2d, stress, mtype,
+        slice_ptr, col, intact, radius, u_neigh, partial, args, notch, staging, sid, stab, stab_xi2d, stab_xiv, stress, mtype,
         splas,
         temp, cond;
     // Prescribed values (force_update == false conditions): members (device
@@ -228,6 +228,7 @@ struct pdk_spec_s
     double mat_c[16] = {}, mat_bcc[16] = {}, mat_tc[4] = {}, mat_sc[4] = {};
     double lat_h[3] = { 0, 0, 0 }; // lattice spacing when lattice_rows
     int64_t lat_dims[3] = { 0, 0, 0 };
+    double lat_dev = 0.0;          // largest distance of a position from its lattice site / spacing
     int32_t nsten = 0;             // stencil entries (0 = no stencil)
     SlotForm slot;                 // stencil-slot form (slot.slots == 0: compact form)
     std::vector<int64_t> h_perm, h_inv;
@@ -375,7 +376,7 @@ struct pdk_spec_s
         h.sid = sid ? sid.as<uint8_t>() : nullptr;
         h.stencil = stab ? stab.as<float4>() : nullptr;
         h.stencil_xi2d = stab_xi2d ? stab_xi2d.as<double>() : nullptr;
-        h.stencil_xiv = nullptr;
+        h.stencil_xiv = stab_xiv ? stab_xiv.as<double>() : nullptr;
         h.exist = nullptr;
         h.soff = nullptr;
         h.stencil_slots = 0;
@@ -420,6 +421,18 @@ struct pdk_spec_s
         h.lin_micro = d.linear_microconductivity;
         h.nmat = nmat;
         h.mtype = pair_tables ? mtype.as<uint8_t>() : nullptr;
+        // Mixed form on a lattice: bond vectors come from the stencil (k h),
+        // which differs from x_j - x_i by up to |e| = 2 lat_dev |h| (|h| the
+        // spacing vector's length); the stretch then moves by at most about
+        // (2 + s) |e| / |xi| with |xi| >= min h, and the band grows by twice
+        // that bound.
+        double band_lat = 0.0;
+        if ( slot.slots > 0 || sid )
+        {
+            const double hmin = std::min( lat_h[0], std::min( lat_h[1], lat_h[2] ) );
+            const double hn = std::sqrt( lat_h[0] * lat_h[0] + lat_h[1] * lat_h[1] + lat_h[2] * lat_h[2] );
+            band_lat = hmin > 0.0 ? 4.0 * lat_dev * hn / hmin : 0.0;
+        }
         for ( int t = 0; t < 16; t++ )
         {
             h.mat_c[t] = mat_c[t];
@@ -428,7 +441,8 @@ struct pdk_spec_s
             // never breaks (+inf) gets no band, so its FP32 test is final
             const double s0p = std::sqrt( mat_bcc[t] ) - 1.0;
             h.mat_s0f[t] = static_cast<float>( s0p );
-            h.mat_bandK[t] = std::isfinite( s0p ) ? static_cast<float>( 2.44140625e-4 * s0p + 9.094947017729282e-13 )
+            h.mat_bandK[t] = std::isfinite( s0p ) ? static_cast<float>( 2.44140625e-4 * s0p + 9.094947017729282e-13 +
+                                                                        ( 2.0 + s0p ) * band_lat )
                                                   : 0.0f;
         }
         for ( int t = 0; t < 4; t++ )
@@ -451,7 +465,7 @@ struct pdk_spec_s
         // s0; far from s0 the FP32 decision cannot be wrong).  The gather
         // passes scale bandf = 2^-13 by the bond's own terms (thermal strain).
         h.bandf = 1.220703125e-4f;
-        h.bandK = static_cast<float>( 2.44140625e-4 * k.s0 + 9.094947017729282e-13 );
+        h.bandK = static_cast<float>( 2.44140625e-4 * k.s0 + 9.094947017729282e-13 + ( 2.0 + k.s0 ) * band_lat );
         // uniform volume: c vol (PMB) or vol (LPS) scales each FP32 row sum
         h.row_scale = isLpsModel( d.model ) ? vol_value : k.c * vol_value;
         h.Rs = k.Rs;
@@ -967,7 +981,7 @@ int pdk_spec_upload( pdk_spec s, const double* x, const double* u, const double*
         std::vector<int64_t> lperm;
         bool lattice = false;
         if ( s->d.ordering != PDK_ORDER_MORTON )
-            lattice = latticeOrder( x, n, lperm, s->lat_h, s->lat_dims );
+            lattice = latticeOrder( x, n, lperm, s->lat_h, s->lat_dims, &s->lat_dev );
         require( lattice || s->d.ordering != PDK_ORDER_LATTICE_ROWS, "positions do not form a lattice" );
         s->lattice_rows = lattice;
         s->vol_uniform = true;
@@ -1115,7 +1129,7 @@ int pdk_spec_bytes( pdk_spec s, uint64_t* bytes )
         for ( const DevBuf* b : { &s->x, &s->U0, &s->U1, &s->v, &s->f, &s->vol, &s->rho, &s->nofail, &s->bcflag,
                                   &s->body, &s->m, &s->theta, &s->lpsA, &s->lpsB, &s->W, &s->phi, &s->perm, &s->inv,
                                   &s->cnt, &s->slice_ptr, &s->col, &s->intact, &s->radius, &s->u_neigh,
-                                  &s->partial, &s->args, &s->sid, &s->stab, &s->stab_xi2d } )
+                                  &s->partial, &s->args, &s->sid, &s->stab, &s->stab_xi2d, &s->stab_xiv } )
             t += b->bytes();
         *bytes = t;
     } );
@@ -1142,6 +1156,7 @@ int pdk_build_bonds( pdk_spec s )
             s->sid.release();
             s->stab.release();
             s->stab_xi2d.release();
+            s->stab_xiv.release();
             s->nsten = 0;
             s->slot = SlotForm{};
             // Full lattice: slot q of every row is stencil entry q, so the
@@ -1179,7 +1194,7 @@ int pdk_build_bonds( pdk_spec s )
             if ( s->slot.slots == 0 && s->d.precision == PDK_MIXED && s->lattice_rows )
             {
                 s->pushArgs();
-                if ( !buildStencil( s->h, s->lat_h, q.rc, s->total_rows, s->sid, s->stab, s->stab_xi2d, s->scratch,
+                if ( !buildStencil( s->h, s->lat_h, q.rc, s->total_rows, s->sid, s->stab, s->stab_xi2d, s->stab_xiv, s->scratch,
                                     s->acct(), s->st, &s->nsten ) )
                     s->nsten = 0;
             }

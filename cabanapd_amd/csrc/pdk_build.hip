@@ -674,7 +674,8 @@ void launchInvertPerm( const int64_t* perm_dev, int64_t n, int64_t* inv_dev, hip
     PDK_HIP( hipGetLastError() );
 }
 
-bool latticeOrder( const double* x, int64_t n, std::vector<int64_t>& perm, double h_out[3], int64_t dims_out[3] )
+bool latticeOrder( const double* x, int64_t n, std::vector<int64_t>& perm, double h_out[3], int64_t dims_out[3],
+                   double* dev_out )
 {
     if ( n < 2 )
         return false;
@@ -715,6 +716,21 @@ bool latticeOrder( const double* x, int64_t n, std::vector<int64_t>& perm, doubl
         h_out[d] = nk[d] > 1 ? ( hi - lo[d] ) / static_cast<double>( nk[d] - 1 ) : h[d];
         dims_out[d] = nk[d];
     }
+    double dev = 0.0;
+    for ( int d = 0; d < 3; d++ )
+    {
+        double e = 0.0, mag = 0.0;
+        for ( int64_t p = 0; p < n; p++ )
+        {
+            const double site = lo[d] + static_cast<double>( k[3 * p + d] ) * h_out[d];
+            e = std::max( e, std::fabs( x[3 * p + d] - site ) );
+            mag = std::max( mag, std::fabs( x[3 * p + d] ) );
+        }
+        // + 4 ulp of the largest coordinate: the rounding of site and difference
+        dev = std::max( dev, ( e + 4.0 * std::ldexp( mag, -52 ) ) / h_out[d] );
+    }
+    if ( dev_out )
+        *dev_out = dev;
     std::vector<uint64_t> key( n );
     bool increasing = true;
     for ( int64_t p = 0; p < n; p++ )
@@ -932,7 +948,7 @@ int64_t buildSlicedCsr( const NeighborQuery& q, const int64_t* perm, const int64
 }
 
 bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_rows, DevBuf& sid, DevBuf& table,
-                   DevBuf& xi2d, BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out )
+                   DevBuf& xi2d, DevBuf& xiv, BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out )
 {
     int32_t K[3];
     for ( int d = 0; d < 3; d++ )
@@ -947,6 +963,7 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
     std::vector<int16_t> dense( static_cast<size_t>( D[0] ) * D[1] * D[2], -1 );
     std::vector<float4> tab;
     std::vector<double> tab2;
+    std::vector<double> tabv( 4 * 256, 0.0 ); // xi_x, |xi|^2, xi_y, xi_z in FP64 per entry
     const double rc2 = rc * rc * ( 1.0 + 1e-9 );
     for ( int32_t c = -K[2]; c <= K[2]; c++ )
         for ( int32_t b = -K[1]; b <= K[1]; b++ )
@@ -959,12 +976,18 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
                 if ( tab.size() >= 255 )
                     return false;
                 dense[( ( c + K[2] ) * D[1] + ( b + K[1] ) ) * D[0] + ( aa + K[0] )] = static_cast<int16_t>( tab.size() );
+                double* v = &tabv[4 * tab.size()];
+                v[kXiX] = gx;
+                v[kXi2] = r2;
+                v[kXiY] = gy;
+                v[kXiZ] = gz;
                 tab.push_back( make_float4( static_cast<float>( gx ), static_cast<float>( gy ), static_cast<float>( gz ),
                                             static_cast<float>( std::sqrt( r2 ) ) ) );
                 tab2.push_back( r2 );
             }
     while ( tab.size() < 256 )
     {
+        tabv[4 * tab.size() + kXi2] = 1.0;
         tab.push_back( make_float4( 0.f, 0.f, 0.f, 1.f ) );
         tab2.push_back( 1.0 );
     }
@@ -978,6 +1001,8 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
     PDK_HIP( hipMemcpyAsync( table.as<float4>(), tab.data(), 256 * sizeof( float4 ), hipMemcpyHostToDevice, st ) );
     xi2d.alloc( 256 * sizeof( double ), acct );
     PDK_HIP( hipMemcpyAsync( xi2d.as<double>(), tab2.data(), 256 * sizeof( double ), hipMemcpyHostToDevice, st ) );
+    xiv.alloc( tabv.size() * sizeof( double ), acct );
+    PDK_HIP( hipMemcpyAsync( xiv.as<double>(), tabv.data(), tabv.size() * sizeof( double ), hipMemcpyHostToDevice, st ) );
     const unsigned g = static_cast<unsigned>( ( a.nslices * kSlice + kThreads - 1 ) / kThreads );
     kStencilIds<<<g, kThreads, 0, st>>>( a, h[0], h[1], h[2], K[0], K[1], K[2], ddense.as<int16_t>(), sid.as<uint8_t>(),
                                          s.flag.as<int32_t>() );
@@ -990,6 +1015,7 @@ bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_
         sid.release();
         table.release();
         xi2d.release();
+        xiv.release();
         return false;
     }
     if ( nsten_out )

@@ -88,7 +88,7 @@ struct PDArgs
     // weighted-volume and dilatation factor)
     const float4* stencil_aux;
     const double* stencil_xi2d; // |xi|^2 of each stencil entry in FP64 (exact lattice offsets)
-    const double* stencil_xiv;  // stencil-slot form: 4 FP64 values per entry at kXiX, kXi2, kXiY, kXiZ
+    const double* stencil_xiv;  // lattice stencil: 4 FP64 values per entry at kXiX, kXi2, kXiY, kXiZ
     const double* stencil_rec;  // stencil-slot form: kRecStride values per entry (kRec* layout)
     double* m;
     double* theta;

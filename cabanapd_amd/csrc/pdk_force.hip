@@ -404,6 +404,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     __shared__ double sxi2[( PREC == kMixed ) ? 256 : 1]; // FP64 |xi|^2 of the stencil entries
     __shared__ int32_t soffs[SLOTS ? 256 + kBatch : 1];
     __shared__ uint32_t rsm[( PREC == kMixed ) ? 9 : 1]; // row-start bits of the slots (stencil-slot form)
+    // compact form with stencil ids: the FP64 record of every stencil entry
+    // (kXiX, kXi2, kXiY, kXiZ), so that the mixed form reads xi instead of x_j
+    __shared__ __align__( 16 ) double sxv[( PREC == kMixed && !SLOTS ) ? 4 * 256 : 1];
     const bool useSten = ( PREC == kMixed ) && ( SLOTS || w.A->sid != nullptr );
     if constexpr ( PREC == kMixed )
     {
@@ -422,6 +425,12 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         {
             stab[threadIdx.x] = w.A->stencil[threadIdx.x];
             sxi2[threadIdx.x] = w.A->stencil_xi2d[threadIdx.x];
+            if constexpr ( !SLOTS )
+            {
+#pragma unroll
+                for ( int k = 0; k < 4; k++ )
+                    sxv[4 * threadIdx.x + k] = w.A->stencil_xiv[4 * threadIdx.x + k];
+            }
         }
     }
     if constexpr ( SLOTS || PREC == kMixed )
@@ -603,9 +612,9 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                 if ( tmp )
                     tj[k] = tmp[j];
                 // x_j: the FP64 form and the layouts without a stencil; the
-                // stencil-slot mixed forms take xi from the stencil (the
+                // mixed forms on a lattice take xi from the stencil (the
                 // guard band's FP64 retest loads x_j itself)
-                if ( PREC == kF64 || !useSten || ( !kIsLin<MODEL> && !SLOTS ) )
+                if ( PREC == kF64 || !useSten )
                 {
                     xj[k][0] = x[j];
                     xj[k][1] = x[n + j];
@@ -722,9 +731,11 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                             }
                             else
                             {
-                                g0 = xj[k][0] - xi0;
-                                g1 = xj[k][1] - xi1;
-                                g2 = xj[k][2] - xi2;
+                                const double* xv = sxv + 4 * id;
+                                g0 = xv[kXiX];
+                                g1 = xv[kXiY];
+                                g2 = xv[kXiZ];
+                                xi2d = xv[kXi2];
                             }
                         }
                         else
@@ -773,7 +784,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
                                 // guard band: the reference's FP64 test on this bond
                                 Bond64 e;
                                 const int64_t j = jj[k];
-                                const bool xl = PREC == kF64 || !useSten || ( !kIsLin<MODEL> && !SLOTS );
+                                const bool xl = PREC == kF64 || !useSten;
                                 getDistance64( xi0, xi1, xi2, ui0, ui1, ui2, xl ? xj[k][0] : x[j],
                                                xl ? xj[k][1] : x[n + j], xl ? xj[k][2] : x[2 * n + j], uj[k][0],
                                                uj[k][1], uj[k][2], e );
@@ -1702,9 +1713,11 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
     const uint64_t* __restrict__ bits = A.intact + base;
     const double* __restrict__ vol = A.vol;
     const int infl1 = A.influence;
-    // mixed form in the stencil-slot layout: bond vectors from the stencil
-    // table, as in the row-window state pass
-    const bool sten = ( PREC == kMixed ) && A.exist != nullptr;
+    // mixed form on a lattice: bond vectors from the stencil table (entry q
+    // in the stencil-slot layout, the bond's stencil id in the compact one),
+    // as in the row-window state pass
+    const bool sten = ( PREC == kMixed ) && ( A.exist != nullptr || A.sid != nullptr );
+    const uint8_t* __restrict__ sidp = A.sid ? A.sid + base * kSlice + lane : nullptr;
 
     double m_i;
     if ( do_m )
@@ -1732,8 +1745,8 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
                 float w2;
                 if ( sten )
                 {
-                    // stencil-slot form: slot q is stencil entry q
-                    const float xw = A.stencil[q].w;
+                    // stencil entry: slot q (stencil-slot form) or the bond's id
+                    const float xw = A.stencil[A.exist ? q : static_cast<int32_t>( sidp[q * kSlice] )].w;
                     w2 = infl1 ? xw : xw * xw;
                 }
                 else
@@ -1780,10 +1793,11 @@ __global__ __launch_bounds__( kBlock ) void kLpsState( const PDArgs* __restrict_
                 {
                     // r = xi + (u_j - u_i) with the slot's FP64 stencil offset,
                     // as in the row-window state pass
-                    const float xw = A.stencil[q].w;
-                    const double* xv = A.stencil_xiv + 4 * q;
+                    const int32_t id = A.exist ? q : static_cast<int32_t>( sidp[q * kSlice] );
+                    const float xw = A.stencil[id].w;
+                    const double* xv = A.stencil_xiv + 4 * id;
                     mixedBond( ( u[j] - ui0 ) + xv[kXiX], u[n + j] + ( xv[kXiY] - ui1 ), u[2 * n + j] + ( xv[kXiZ] - ui2 ),
-                               A.stencil_xi2d[q], xw, b );
+                               xv[kXi2], xw, b );
                     w2 = infl1 ? xw : xw * xw;
                 }
                 else

@@ -138,13 +138,16 @@ void launchMortonOrder( const double* x_aos_dev, int64_t n, const double lo[3], 
                         hipStream_t st );
 // Lattice detection: true when every position is lo + k h (k integer per
 // axis, one particle per site); perm then lists particles in lattice index
-// order with x fastest.
+// order with x fastest.  dev_out: the largest distance of a position from
+// its site lo + k h along an axis, as a fraction of that axis' spacing, plus
+// the rounding of the site (stencil bond vectors differ from x_j - x_i by at
+// most twice that per axis).
 bool latticeOrder( const double* x_aos_host, int64_t n, std::vector<int64_t>& perm, double h_out[3],
-                   int64_t dims_out[3] );
+                   int64_t dims_out[3], double* dev_out );
 // Stencil ids of a lattice specimen's bonds (mixed form); false when some
 // bond is not a lattice vector or the stencil has more than 255 entries.
 bool buildStencil( const PDArgs& a, const double h[3], double rc, int64_t total_rows, DevBuf& sid, DevBuf& table,
-                   DevBuf& xi2d, BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out );
+                   DevBuf& xi2d, DevBuf& xiv, BuildScratch& s, MemAccount* acct, hipStream_t st, int32_t* nsten_out );
 // Stencil-slot form of a full lattice (dims[0]*dims[1]*dims[2] == n, lattice
 // rows): verifies that every compact row equals its in-box stencil entries in
 // one fixed order (the caller-id order of an interior particle's row), then

@@ -223,6 +223,43 @@ def test_mixed_stencil_matches_gathered_bond_vectors():
         assert e <= 1e-6
 
 
+@pytest.mark.parametrize("layout", ["slots", "compact"])
+def test_mixed_jittered_lattice_keeps_break_decisions(layout):
+    """A lattice whose x coordinates sit up to 1e-8 spacings off their sites
+    (the largest jitter the lattice test accepts across 40 sites) still takes
+    the stencil, whose bond vectors then differ from x_j - x_i; the guard
+    band grows with the measured deviation, and the mixed form's broken bonds
+    equal the FP64 form's."""
+    say(f"case: PMB fracture plate 40x80x9 x jittered 1e-8 h, mixed {layout} vs FP64, 150 steps")
+    w = configs.kw_plate(num_cells=(40, 80, 9), size=(0.1, 0.2, 0.0225), steps=150)
+    dx = 0.1 / 40
+    rng = np.random.default_rng(7)
+    w["x"] = w["x"].copy()
+    w["x"][:, 0] += rng.uniform(-1e-8 * dx, 1e-8 * dx, size=w["x"].shape[0])
+    res = {}
+    for prec in (0, 1):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.precision = prec
+        d.ordering = 2
+        d.bond_layout = {"slots": 0, "compact": 1}[layout]
+        s = engine.Spec(ctx(), d, w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        lay = s.layout()
+        say(f"  precision {prec}: {lay}")
+        if prec == 1:
+            assert lay["stencil_entries"] == 122
+            assert lay["stencil_slots"] == (layout == "slots")
+        s.step(150)
+        res[prec] = (s.get("u"), s.bonds()[2])
+        s.close()
+    mism = int((res[0][1] != res[1][1]).sum())
+    e = rel_err(res[1][0], res[0][0])
+    say(f"  broken {int((res[0][1] == 0).sum())}, mismatched {mism}, u rel err {e:.3e}")
+    assert mism == 0
+    assert e <= 1e-5
+
+
 @pytest.mark.parametrize("model", ["PMB", "LPS"])
 def test_mixed_row_windows_match_gathers(model):
     """The mixed force pass with neighbor data staged in LDS row windows and
