@@ -9,7 +9,7 @@ import pytest
 
 from cabanapd_amd import configs, engine
 from cabanapd_amd._abi import make_desc
-from tests.parity import ctx, rel_err, say
+from tests.parity import ctx, pair, rel_err, say
 
 pytestmark = pytest.mark.gpu
 
@@ -258,6 +258,63 @@ def test_mixed_jittered_lattice_keeps_break_decisions(layout):
     say(f"  broken {int((res[0][1] == 0).sum())}, mismatched {mism}, u rel err {e:.3e}")
     assert mism == 0
     assert e <= 1e-5
+
+
+def _holed_plate(nc=(30, 60, 7), steps=80):
+    """The pre-notched plate (cubic cells of 2.5 mm) with a cylindrical hole
+    of radius 6 cells through the thickness: a lattice with missing sites,
+    which takes the compact form (with stencil ids in the mixed form)."""
+    dx = 0.0025
+    w = configs.kw_plate(num_cells=nc, size=tuple(c * dx for c in nc), steps=steps)
+    keep = (w["x"][:, 0] - 0.01) ** 2 + w["x"][:, 1] ** 2 > (6 * dx) ** 2
+    remap = np.cumsum(keep) - 1
+    for k in ("x", "vol", "rho", "u", "v"):
+        w[k] = np.ascontiguousarray(w[k][keep])
+    kind, idx, val = w["bc"]
+    w["bc"] = (kind, remap[idx[keep[idx]]].astype(np.int64), val)
+    return w
+
+
+def test_holed_lattice_compact_stencil():
+    """A lattice with missing sites: FP64 bit for bit against the oracle; the
+    mixed form (compact bonds, bond vectors from the stencil by stencil id)
+    with FP64's broken bonds and u within 1e-5, and within 1e-6 of the mixed
+    form gathering x_j (Morton order)."""
+    say("case: PMB plate 30x60x7 with a hole of radius 6 cells, FP64 vs oracle, mixed vs FP64, 80 steps")
+    w = _holed_plate()
+    g, o = pair(w)
+    say(f"  particles {w['x'].shape[0]}, FP64 layout {g.layout()}")
+    assert g.layout()["lattice_rows"] and not g.layout()["stencil_slots"]
+    g.step(80)
+    o.step(80)
+    for fld in ("u", "v", "f"):
+        assert np.array_equal(g.get(fld), o.get(fld)), fld
+    ref_u, ref_b = g.get("u"), g.bonds()[2]
+    g.close()
+    res = {}
+    for name, order in (("lattice", 2), ("morton", 1)):
+        d = type(w["desc"]).from_buffer_copy(w["desc"])
+        d.precision = 1
+        d.ordering = order
+        s = engine.Spec(ctx(), d, w["x"].shape[0])
+        configs.upload(s, w)
+        configs.setup(s, w)
+        lay = s.layout()
+        say(f"  mixed {name}: {lay} {s.kernel_name(0)}")
+        if order == 2:
+            assert lay["stencil_entries"] == 122 and not lay["stencil_slots"]
+        s.step(80)
+        res[name] = (s.get("u"), s.bonds()[2])
+        s.close()
+    e = rel_err(res["lattice"][0], ref_u)
+    e2 = rel_err(res["lattice"][0], res["morton"][0])
+    mism = int((res["lattice"][1] != ref_b).sum())
+    say(f"  broken {int((ref_b == 0).sum())}, mixed vs FP64: u rel err {e:.3e} mismatched {mism}; "
+        f"stencil vs x_j gathers: u rel err {e2:.3e}")
+    assert mism == 0
+    assert np.array_equal(res["morton"][1], ref_b)
+    assert e <= 1e-5
+    assert e2 <= 1e-6
 
 
 @pytest.mark.parametrize("model", ["PMB", "LPS"])
