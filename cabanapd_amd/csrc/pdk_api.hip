@@ -402,13 +402,15 @@ struct pdk_spec_s
             h.nrows = 0;
         }
         // tuning knob: -1 = default (row windows where available, else groups
-        // of 4); feature combinations the row windows do not cover use the
-        // gather variant
+        // of 4, or of 2 for the mixed form in the compact layout, measured
+        // faster there); feature combinations the row windows do not cover
+        // use the gather variant
+        const int32_t gather_unroll = ( d.precision == PDK_MIXED && slot.slots == 0 ) ? 2 : 4;
         h.unroll = unroll;
         if ( h.unroll < 0 )
-            h.unroll = ( h.nrows > 0 && rowsCover() ) ? 0 : 4;
+            h.unroll = ( h.nrows > 0 && rowsCover() ) ? 0 : gather_unroll;
         if ( h.unroll == 0 && ( h.nrows == 0 || !rowsCover() ) )
-            h.unroll = 4;
+            h.unroll = gather_unroll;
         h.splas = splas ? splas.as<double>() : nullptr;
         h.sY = k.s_Y;
         h.temp = temp ? temp.as<double>() : nullptr;

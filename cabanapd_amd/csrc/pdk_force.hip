@@ -389,7 +389,7 @@ __device__ __forceinline__ void epilogue( const PDArgs& A, int64_t p, int32_t ep
 constexpr int kBatch = 16; // bond slots whose ids / bit words are fetched ahead
 
 template <int MODEL, bool FRAC, int PREC, int UNR, bool SLOTS, bool MM = false>
-__global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ specs,
+__global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( PREC == kMixed && !SLOTS && UNR <= 2 ) ? 4 : 1, 8 ) ) ) void kForce( const PDArgs* __restrict__ specs,
                                                     const int32_t* __restrict__ tile_spec,
                                                     const int64_t* __restrict__ tile_slice,
                                                     int32_t epi, int32_t parity, int64_t launch,
@@ -504,31 +504,35 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
     const double rscale = fold ? A.row_scale : 1.0;
     const float c32 = static_cast<float>( c );
     const float bandK = A.bandK;
-    // The neighbor ids and broken-bond words of kBatch slots are fetched one
+    // The neighbor ids and broken-bond words of KB slots are fetched one
     // batch ahead (the index stream comes from HBM; keeping a batch in
     // flight hides its latency).  Within a batch, slots are processed UNR at
     // a time: the gathers of a group are issued together, then its bonds are
     // evaluated in slot order (the summation order is unchanged).
-    static_assert( kBatch % UNR == 0, "batch must hold whole groups" );
-    int32_t jb[SLOTS ? 1 : kBatch], jn[SLOTS ? 1 : kBatch];
-    uint32_t sb[kBatch / 4], sn[kBatch / 4]; // stencil ids, four per word
+    // (compact mixed form: batches of 4 slots (or one group), which leave
+    // the registers for 4 waves per SIMD: 0.997 -> 0.788 ms on a 1.44M
+    // particle plate; the FP64 form keeps 16, it runs slower with 4)
+    constexpr int KB = ( PREC == kMixed && !SLOTS ) ? ( UNR > 4 ? UNR : 4 ) : kBatch;
+    static_assert( KB % UNR == 0, "batch must hold whole groups" );
+    int32_t jb[SLOTS ? 1 : KB], jn[SLOTS ? 1 : KB];
+    uint32_t sb[KB / 4], sn[KB / 4]; // stencil ids, four per word
     uint32_t wlo = 0, whi = 0, nlo = 0, nhi = 0;
     uint32_t elo = 0, ehi = 0, mlo = 0, mhi = 0; // existence words (stencil-slot form)
     const int32_t self = static_cast<int32_t>( pc );
     const uint64_t* __restrict__ exist = SLOTS ? A.exist + base : nullptr;
-    auto fetch = [&]( int32_t qb, int32_t( &dst )[SLOTS ? 1 : kBatch], uint32_t( &sdst )[kBatch / 4], uint32_t& lo,
+    auto fetch = [&]( int32_t qb, int32_t( &dst )[SLOTS ? 1 : KB], uint32_t( &sdst )[KB / 4], uint32_t& lo,
                       uint32_t& hi, uint32_t& xlo, uint32_t& xhi ) {
         if constexpr ( SLOTS )
         {
             // lane k holds the existence word of slot qb + k
-            const uint64_t xv = ( lane < kBatch && qb + lane < len ) ? exist[qb + lane] : 0ull;
+            const uint64_t xv = ( lane < KB && qb + lane < len ) ? exist[qb + lane] : 0ull;
             xlo = static_cast<uint32_t>( xv );
             xhi = static_cast<uint32_t>( xv >> 32 );
         }
         else
         {
 #pragma unroll
-            for ( int k = 0; k < kBatch; k++ )
+            for ( int k = 0; k < KB; k++ )
                 dst[k] = ( qb + k < len ) ? colp[( qb + k ) * kSlice] : self;
         }
         if constexpr ( PREC == kMixed && !SLOTS )
@@ -536,7 +540,7 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
             if ( useSten )
             {
 #pragma unroll
-                for ( int k4 = 0; k4 < kBatch / 4; k4++ )
+                for ( int k4 = 0; k4 < KB / 4; k4++ )
                 {
                     uint32_t pk = 0;
 #pragma unroll
@@ -553,18 +557,18 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         if constexpr ( FRAC )
         {
             // lane k holds the word of slot qb + k
-            const uint64_t wv = ( lane < kBatch && qb + lane < len ) ? bits[qb + lane] : 0ull;
+            const uint64_t wv = ( lane < KB && qb + lane < len ) ? bits[qb + lane] : 0ull;
             lo = static_cast<uint32_t>( wv );
             hi = static_cast<uint32_t>( wv >> 32 );
         }
     };
     fetch( 0, jb, sb, wlo, whi, elo, ehi );
-    for ( int32_t qb = 0; qb < len; qb += kBatch )
+    for ( int32_t qb = 0; qb < len; qb += KB )
     {
-        if ( qb + kBatch < len )
-            fetch( qb + kBatch, jn, sn, nlo, nhi, mlo, mhi );
+        if ( qb + KB < len )
+            fetch( qb + KB, jn, sn, nlo, nhi, mlo, mhi );
 #pragma unroll
-        for ( int g = 0; g < kBatch; g += UNR )
+        for ( int g = 0; g < KB; g += UNR )
         {
         const int32_t q0 = qb + g;
         if ( q0 >= len )
@@ -845,11 +849,11 @@ __global__ __launch_bounds__( kBlock ) void kForce( const PDArgs* __restrict__ s
         if constexpr ( !SLOTS )
         {
 #pragma unroll
-            for ( int k = 0; k < kBatch; k++ )
+            for ( int k = 0; k < KB; k++ )
                 jb[k] = jn[k];
         }
 #pragma unroll
-        for ( int k4 = 0; k4 < kBatch / 4; k4++ )
+        for ( int k4 = 0; k4 < KB / 4; k4++ )
             sb[k4] = sn[k4];
         wlo = nlo;
         whi = nhi;

@@ -425,7 +425,8 @@ int pdk_compute_energy_noalloc(pdk_spec spec, double* total_strain_energy, doubl
  * several materials, or a temperature-dependent and/or ElasticPerfectlyPlastic
  * PMB; several materials together with either, and the linearized models
  * with a temperature, take the gathers).  Results do not depend on the
- * choice.  The default is 0 where available, else 4.
+ * choice.  The default is 0 where available, else 4 (2 for the mixed form
+ * in the compact layout).
  * pdk_autotune times the candidates on the current state (after pdk_init;
  * the probes repeat the init force pass, which is idempotent) and keeps the
  * fastest. */
