@@ -894,7 +894,8 @@ __global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( P
 // below): 6 for the PMB passes (80 VGPRs, one spill outside the loops;
 // measured 6 % faster on the fracture plate than the unconstrained
 // allocation at 5 waves); LPS (models 1 and 5) stays unconstrained, since
-// forcing it spills inside the slot loop.
+// forcing it spills inside the slot loop (mixed LPS at 6 waves: 8 VGPRs
+// spilled, 0.349 -> 0.354 ms on a 128^3 cube).
 template <int MODEL, bool FRAC, int PREC, bool GEN>
 __global__ __launch_bounds__( kBlock ) __attribute__( ( amdgpu_waves_per_eu( ( MODEL == 1 || MODEL == 5 ) ? 1 : 6, 8 ) ) ) void
 kForceRows( const PDArgs* __restrict__ specs,
